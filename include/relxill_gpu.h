@@ -1,0 +1,202 @@
+/*
+ * relxill_gpu.h -- C ABI of librelxill_gpu.so, the MI355X (gfx950) implementation
+ * of relxill's relativistic smearing path (relline, relline_lp, relconv,
+ * relconv_lp, plus a batched API and a serving loop).
+ *
+ * Drop-in boundary.  The four XSPEC local-model entry points below have the
+ * exact signature and parameter order of the functions the reference's wrapper
+ * generator emits (src/create_wrapper_xspec.py:153-162, parameter order from
+ * src/modelfiles/lmodel_relxill_public.dat:1-44).  A host that loads the
+ * reference library's symbols `lmodrelline`, `lmodrellinelp`, `lmodrelconv`,
+ * `lmodrelconvlp` binds these instead.  `spectrum`, `fluxError` and `init` are
+ * ignored as in the reference.  On failure the flux is zeroed and the status is
+ * available from rx_last_status()/rx_last_error() (the reference throws
+ * ModelEvalFailed through its extern "C" frame, src/LocalModel.cpp:143-160).
+ *
+ * No torch types, no C++ types: plain pointers and sizes.  Thread safety: every
+ * entry point may be called from several host threads; calls are serialised
+ * per device except the server API, which batches concurrent requests.
+ */
+#ifndef RELXILL_GPU_H_
+#define RELXILL_GPU_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define RX_OK 0
+#define RX_ERR_PARAM 1        /* parameters rejected (check_parameter_bounds) */
+#define RX_ERR_UNSUPPORTED 2  /* feature out of scope (returning radiation, extended jet) */
+#define RX_ERR_TABLE 10       /* table interpolation failed (grid invariants) */
+#define RX_ERR_INPUT 50       /* relconv input flux with sum <= 0 (src/LocalModel.cpp:84-86) */
+#define RX_ERR_DEVICE 100     /* HIP error / no device */
+#define RX_ERR_ARG 101        /* invalid argument (sizes, null pointers) */
+#define RX_ERR_NOMEM 102
+
+/* ---- model ids: the reference's MOD_TYPE_* values (src/common.h:85-95) ---- */
+#define RX_MODEL_RELLINE 1
+#define RX_MODEL_RELLINELP 2
+#define RX_MODEL_RELCONV 11
+#define RX_MODEL_RELCONVLP 12
+#define RX_MODEL_RELXILL_CONV (-1) /* relxill-style: angle-resolved reflection, one zone */
+
+/* ---- precision of the batched path ---- */
+#define RX_FP64 0 /* plain form: FP64 throughout */
+#define RX_FP32 1 /* FP32 integrand + Kahan sums; FP64 decisions, edges, norms */
+
+/* ===================================================================== */
+/* XSPEC local-model entry points (replace the generated lmod<name>,       */
+/* src/create_wrapper_xspec.py:153-162 -> xspec_C_wrapper_eval_model,      */
+/* src/LocalModel.cpp:143-160).                                            */
+/* ===================================================================== */
+
+/* relline: lineE Index1 Index2 Rbr a Incl Rin Rout z limb
+ * (src/modelfiles/lmodel_relxill_public.dat:1-11) */
+void lmodrelline(const double *energy, int Nflux, const double *parameter, int spectrum,
+                 double *flux, double *fluxError, const char *init);
+
+/* relline_lp: lineE h a Incl Rin Rout z limb gamma switch_returnrad
+ * (lmodel_relxill_public.dat:23-33); switch_returnrad must be 0 */
+void lmodrellinelp(const double *energy, int Nflux, const double *parameter, int spectrum,
+                   double *flux, double *fluxError, const char *init);
+
+/* relconv (con model, flux is input and output): Index1 Index2 Rbr a Incl Rin Rout limb
+ * (lmodel_relxill_public.dat:13-21) */
+void lmodrelconv(const double *energy, int Nflux, const double *parameter, int spectrum,
+                 double *flux, double *fluxError, const char *init);
+
+/* relconv_lp: h beta a Incl Rin Rout limb gamma switch_returnrad
+ * (lmodel_relxill_public.dat:35-44); switch_returnrad must be 0 */
+void lmodrelconvlp(const double *energy, int Nflux, const double *parameter, int spectrum,
+                   double *flux, double *fluxError, const char *init);
+
+/* Status-returning evaluation of one model (the body of xspec_C_wrapper_eval_model,
+ * src/LocalModel.cpp:143-160, and LocalModel::eval_model, src/LocalModel.h:108-128).
+ * precision: RX_FP64 or RX_FP32. */
+int rx_eval_model(int model, const double *parameter, const double *energy, double *flux,
+                  int n_flux, int precision);
+
+int rx_last_status(void);
+const char *rx_last_error(void);
+
+/* Processed parameters of one XSPEC parameter array (get_rel_params with
+ * check_parameter_bounds, src/ModelDefinition.cpp:168-334): out12 receives
+ * a, incl[rad], Index1, Index2, Rbr, Rin, Rout, lineE, z, h, gamma, beta. */
+int rx_model_params(int model, const double *parameter, double *out12);
+
+/* ===================================================================== */
+/* library lifecycle and tables                                          */
+/* ===================================================================== */
+
+/* Selects the device and uploads the tables (lazily done by the first call
+ * otherwise).  Tables: RELXILL_TABLE_PATH/rel_table.rxtb and rel_lp_table.rxtb
+ * when present (own binary format), else the synthetic tables of the given
+ * seeds at the reference dims used by the benchmarks (25x22x100x20 relline,
+ * 20x100x100 lamp post). */
+int rx_init(int device);
+void rx_shutdown(void);
+
+/* Replace the resident tables with seeded synthetic ones of the given dims
+ * (host-only work plus an upload when a device is initialised). */
+int rx_tables_synthetic(uint64_t seed, int n_a, int n_mu0, int n_r, int n_g);
+int rx_lptable_synthetic(uint64_t seed, int n_a, int n_h, int n_r);
+/* Load/save tables in the library's binary format (RXTB). */
+int rx_tables_load(const char *path);
+int rx_lptable_load(const char *path);
+int rx_tables_save(const char *path);
+int rx_lptable_save(const char *path);
+
+/* Read-only host view of the resident tables (for checking). */
+typedef struct {
+  int n_a, n_mu0, n_r, n_g;
+  const float *a, *mu0, *r, *gmin, *gmax, *trff1, *trff2, *cosne1, *cosne2;
+} rx_reltable_view;
+typedef struct {
+  int n_a, n_h, n_r;
+  const float *a, *h, *rad, *intens, *del, *del_inc;
+} rx_lptable_view;
+int rx_tables_view(rx_reltable_view *rel, rx_lptable_view *lp);
+
+/* Convolution grid of relconv (reference: 4096 log bins 0.00035-2000 keV,
+ * src/Xillspec.h:36-38, src/Xillspec.cpp:44-54).  n must be a power of two
+ * (<= 8192).  Fine radial grid size (reference N_FRAD = 1000). */
+int rx_set_conv_grid(int n, double emin, double emax);
+int rx_set_nr_fine(int nr_fine);
+
+/* ===================================================================== */
+/* batched API: inputs resident in HBM, many parameter sets and uneven    */
+/* energy grids per launch (project-defined; no reference counterpart)    */
+/* ===================================================================== */
+
+typedef struct rx_batch rx_batch;
+
+typedef struct {
+  int model;               /* RX_MODEL_* */
+  int n_flux;              /* bins of the caller grid */
+  const double *energy;    /* n_flux+1 edges (host memory, copied at create) */
+  const double *parameter; /* XSPEC parameter array of the model (copied) */
+  const double *flux_in;   /* conv models: input spectrum n_flux (copied) */
+  int refl_set;            /* RX_MODEL_RELXILL_CONV: reflection set id */
+} rx_request;
+
+/* Builds a device-resident batch: copies grids, parameters and inputs to HBM,
+ * validates parameters (per-request status), plans tiles.  NULL on failure. */
+rx_batch *rx_batch_create(const rx_request *req, int n_req);
+/* Runs the whole path on the device (stream-ordered, asynchronous). */
+int rx_batch_run(rx_batch *b, int precision);
+int rx_batch_sync(rx_batch *b);
+/* Copies request i's output spectrum (n_flux values) to host memory and
+ * returns its status. */
+int rx_batch_output(rx_batch *b, int i, double *flux);
+int rx_batch_status(rx_batch *b, int i);
+/* emission-angle distribution of a relxill-style request (n_incl values) */
+int rx_batch_dist(rx_batch *b, int i, double *dist, int n_incl);
+/* lamp-post reflection fractions of request i: refl_frac, f_bh, f_ad, f_inf, f_inf_rest */
+int rx_batch_reflfrac(rx_batch *b, int i, double *out5);
+/* tile shape: radius chunks per request (0: automatic / autotuned) */
+int rx_batch_set_rchunks(rx_batch *b, int n_rch);
+void rx_batch_destroy(rx_batch *b);
+
+/* Resident reflection set for relxill-style requests: n_incl spectra of
+ * n_refl bins (photons/bin) on refl_energy (n_refl+1 edges).  Returns the set
+ * id (>= 0) or a negative status. */
+int rx_reflset_create(const double *refl_energy, int n_refl, const double *refl, int n_incl);
+
+/* Kernel timing of the last rx_batch_run with profiling enabled: per kernel
+ * (0 setup, 1 integrate, 2 finalize, 3 conv) the duration in ms measured with
+ * HIP events on the batch's stream. */
+int rx_profile_enable(int on);
+int rx_batch_kernel_ms(rx_batch *b, double *ms4);
+
+/* Autotuner: times candidate tile shapes for this batch on the device and
+ * keeps the fastest for batches of the same shape class.  Returns the chosen
+ * radius-chunk count. */
+int rx_batch_autotune(rx_batch *b, int precision);
+
+/* ===================================================================== */
+/* serving loop: continuous batching with HBM admission                  */
+/* ===================================================================== */
+
+typedef struct rx_server rx_server;
+/* max_inflight: requests on the device at once; hbm_budget_bytes: admission
+ * cap for per-request device memory; n_streams: concurrent batches. */
+rx_server *rx_server_start(int max_inflight, int64_t hbm_budget_bytes, int n_streams,
+                           int precision);
+/* Submits a request (copied); returns a ticket >= 0 or a negative status. */
+int64_t rx_server_submit(rx_server *s, const rx_request *req);
+/* Non-blocking poll: 1 done (flux copied, *status set, *latency_ms set),
+ * 0 pending, <0 unknown ticket. */
+int rx_server_poll(rx_server *s, int64_t ticket, double *flux, int *status, double *latency_ms);
+/* Blocks until every submitted request completed. */
+int rx_server_drain(rx_server *s);
+/* counters: completed, batches launched, max concurrent batches */
+int rx_server_stats(rx_server *s, int64_t *out3);
+void rx_server_stop(rx_server *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,302 @@
+"""Python binding of the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker or the timed CPU baseline; never
+by the product library or its Python mirror.
+
+The oracle restates relxill 2.4's relline / relline_lp / relconv algorithm in
+plain C (see relxill_oracle.c for the per-function reference citations).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_DIR, "liboracle.so")
+
+MOD_RELLINE = 1
+MOD_RELLINELP = 2
+MOD_RELCONV = 11
+MOD_RELCONVLP = 12
+MOD_RELXILL = -1
+
+_D = ctypes.POINTER(ctypes.c_double)
+_F = ctypes.POINTER(ctypes.c_float)
+_I = ctypes.c_int
+
+
+class RelTable(ctypes.Structure):
+    _fields_ = [("n_a", _I), ("n_mu0", _I), ("n_r", _I), ("n_g", _I), ("a", _F), ("mu0", _F),
+                ("r", _F), ("gmin", _F), ("gmax", _F), ("trff1", _F), ("trff2", _F),
+                ("cosne1", _F), ("cosne2", _F)]
+
+
+class LpTable(ctypes.Structure):
+    _fields_ = [("n_a", _I), ("n_h", _I), ("n_r", _I), ("a", _F), ("h", _F), ("rad", _F),
+                ("intens", _F), ("del_", _F), ("del_inc", _F)]
+
+
+class RelParam(ctypes.Structure):
+    _fields_ = [("model_type", _I), ("emis_type", _I), ("a", ctypes.c_double),
+                ("incl", ctypes.c_double), ("emis1", ctypes.c_double), ("emis2", ctypes.c_double),
+                ("rbr", ctypes.c_double), ("rin", ctypes.c_double), ("rout", ctypes.c_double),
+                ("lineE", ctypes.c_double), ("z", ctypes.c_double), ("height", ctypes.c_double),
+                ("htop", ctypes.c_double), ("gamma", ctypes.c_double), ("beta", ctypes.c_double),
+                ("limb", _I), ("num_zones", _I), ("return_rad", _I), ("do_renorm", _I)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("relb_func_calls", ctypes.c_int64), ("romberg_calls", ctypes.c_int64),
+                ("romberg_levels", ctypes.c_int64), ("edge_calls", ctypes.c_int64),
+                ("midpoint_calls", ctypes.c_int64), ("bin_integrals", ctypes.c_int64)]
+
+
+class SysPar(ctypes.Structure):
+    _fields_ = [("nr", _I), ("ng", _I), ("re", _D), ("gmin", _D), ("gmax", _D), ("gstar", _D),
+                ("d_gstar", _D), ("trff", _D), ("cosne", _D), ("emis", _D), ("del_emit", _D),
+                ("del_inc", _D), ("limb_law", _I),
+                ("refl", ctypes.c_double * 5)]
+
+
+class Job(ctypes.Structure):
+    _fields_ = [("model_type", _I), ("parameter", _D), ("energy", _D), ("flux", _D),
+                ("n_flux", _I), ("status", _I)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} missing: run `make -C oracle`")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "orc_kerr_rms": (ctypes.c_double, [ctypes.c_double]),
+        "orc_kerr_rplus": (ctypes.c_double, [ctypes.c_double]),
+        "orc_ut_disk": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
+        "orc_gi_potential_lp": (ctypes.c_double, [ctypes.c_double] * 5),
+        "orc_lp_emissivity_newton": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
+        "orc_get_log_grid": (None, [_D, _I, ctypes.c_double, ctypes.c_double]),
+        "orc_get_fine_radial_grid": (None, [ctypes.c_double, ctypes.c_double, _D, _I]),
+        "orc_binary_search": (_I, [_D, _I, ctypes.c_double]),
+        "orc_inv_binary_search": (_I, [_D, _I, ctypes.c_double]),
+        "orc_interp_lin_1d": (ctypes.c_double, [ctypes.c_double] * 3),
+        "orc_interp_lin_2d": (ctypes.c_double, [ctypes.c_double] * 6),
+        "orc_interp_lin_2d_float": (ctypes.c_double, [ctypes.c_double] * 2 + [ctypes.c_float] * 4),
+        "orc_rebin_spectrum": (None, [_D, _D, _I, _D, _D, _I]),
+        "orc_rebin_mean_flux": (None, [_D, _D, _I, _D, _D, _I]),
+        "orc_get_rzone_grid": (_I, [ctypes.c_double, ctypes.c_double, _I, ctypes.c_double, _D]),
+        "orc_get_rel_params": (_I, [_I, _D, _I, ctypes.POINTER(RelParam)]),
+        "orc_syspar_new": (ctypes.POINTER(SysPar), [_I, _I]),
+        "orc_syspar_free": (None, [ctypes.POINTER(SysPar)]),
+        "orc_get_system_parameters": (ctypes.POINTER(SysPar),
+                                      [ctypes.POINTER(RelTable), ctypes.POINTER(LpTable),
+                                       ctypes.POINTER(RelParam), _I, ctypes.POINTER(_I)]),
+        "orc_relbase": (_I, [ctypes.POINTER(RelTable), ctypes.POINTER(LpTable),
+                             ctypes.POINTER(RelParam), _D, _I, _I, _D, _D, _I,
+                             ctypes.POINTER(Stats)]),
+        "orc_fft": (None, [_D, _D, _I, _I]),
+        "orc_fft_conv_spectrum": (None, [_D, _D, _D, _D, _I]),
+        "orc_fft_norm_factor": (ctypes.c_double, [_D, _D, _D, _D, _I]),
+        "orc_conv_normalized": (None, [_D, _D, _D, _D, _I]),
+        "orc_relconv_kernel": (_I, [ctypes.POINTER(RelTable), ctypes.POINTER(LpTable),
+                                    ctypes.POINTER(RelParam), _D, _I, _D, _D, _I, _I,
+                                    ctypes.POINTER(Stats)]),
+        "orc_relxill_conv": (_I, [ctypes.POINTER(RelTable), ctypes.POINTER(RelParam), _D, _I,
+                                  _D, _D, _I, _I, _D, _D, _I, _I, _D, ctypes.POINTER(Stats)]),
+        "orc_eval_model": (_I, [_I, ctypes.POINTER(RelTable), ctypes.POINTER(LpTable), _D, _D,
+                                _D, _I, _D, _I, _I, ctypes.POINTER(Stats)]),
+        "orc_eval_batch_mt": (_I, [ctypes.POINTER(RelTable), ctypes.POINTER(LpTable),
+                                   ctypes.POINTER(Job), _I, _D, _I, _I]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
+lib = _load()
+
+
+def _arr(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _fp(a):
+    return a.ctypes.data_as(_F)
+
+
+class Tables:
+    """Oracle view of a relline table (and optional lamp-post table) given as
+    dicts of numpy arrays in the layout of relxill_amd.tables_view()."""
+
+    def __init__(self, rel, lp=None):
+        self._keep = []
+
+        def f32(k, d):
+            a = np.ascontiguousarray(d[k], dtype=np.float32)
+            self._keep.append(a)
+            return a
+        self.rel = RelTable(rel["n_a"], rel["n_mu0"], rel["n_r"], rel["n_g"],
+                            *[_fp(f32(k, rel)) for k in ("a", "mu0", "r", "gmin", "gmax",
+                                                         "trff1", "trff2", "cosne1", "cosne2")])
+        self.lp = None
+        if lp is not None:
+            self.lp = LpTable(lp["n_a"], lp["n_h"], lp["n_r"],
+                              *[_fp(f32(k, lp)) for k in ("a", "h", "rad", "intens", "del_",
+                                                          "del_inc")])
+        self.rel_arrays = rel
+        self.lp_arrays = lp
+
+    @property
+    def lp_ptr(self):
+        return ctypes.byref(self.lp) if self.lp is not None else None
+
+
+def rel_params(model, params, phys_norm=False):
+    p = RelParam()
+    st = lib.orc_get_rel_params(model, _dp(_arr(params)), int(phys_norm), ctypes.byref(p))
+    return p, st
+
+
+def eval_model(model, tables, params, energy, flux_in=None, conv_grid=None, phys_norm=False,
+               stats=False):
+    """Model-level evaluation (what lmod<name> computes); returns (flux, status[, stats])."""
+    e = _arr(energy)
+    n = e.size - 1
+    f = _arr(flux_in).copy() if flux_in is not None else np.zeros(n)
+    st_obj = Stats()
+    cg = _arr(conv_grid) if conv_grid is not None else None
+    st = lib.orc_eval_model(model, ctypes.byref(tables.rel), tables.lp_ptr, _dp(_arr(params)),
+                            _dp(e), _dp(f), n, _dp(cg) if cg is not None else None,
+                            (cg.size - 1) if cg is not None else 0, int(phys_norm),
+                            ctypes.byref(st_obj))
+    if stats:
+        return f, st, {k: getattr(st_obj, k) for k, _ in Stats._fields_}
+    return f, st
+
+
+def relbase(tables, relparam, ener, nr_fine=1000, n_incl=0, stats=False):
+    e = _arr(ener)
+    n = e.size - 1
+    flux = np.zeros(n * max(1, relparam.num_zones))
+    dist = np.zeros(max(1, n_incl) * max(1, relparam.num_zones))
+    so = Stats()
+    st = lib.orc_relbase(ctypes.byref(tables.rel), tables.lp_ptr, ctypes.byref(relparam), _dp(e),
+                         n, nr_fine, _dp(flux), _dp(dist) if n_incl > 0 else None, n_incl,
+                         ctypes.byref(so))
+    out = (flux, dist[:n_incl] if n_incl > 0 else None, st)
+    if stats:
+        return out + ({k: getattr(so, k) for k, _ in Stats._fields_},)
+    return out
+
+
+def system_parameters(tables, relparam, nr_fine=1000):
+    """(re, gmin, gmax, trff[nr,ng,2], cosne, emis, del_emit, del_inc, refl5), status"""
+    st = ctypes.c_int(0)
+    sp = lib.orc_get_system_parameters(ctypes.byref(tables.rel), tables.lp_ptr,
+                                       ctypes.byref(relparam), nr_fine, ctypes.byref(st))
+    if not sp:
+        return None, st.value
+    s = sp.contents
+    nr, ng = s.nr, s.ng
+    g = lambda p, n: np.ctypeslib.as_array(p, shape=(n,)).copy()
+    out = dict(re=g(s.re, nr), gmin=g(s.gmin, nr), gmax=g(s.gmax, nr),
+               trff=g(s.trff, nr * ng * 2).reshape(nr, ng, 2),
+               cosne=g(s.cosne, nr * ng * 2).reshape(nr, ng, 2), emis=g(s.emis, nr),
+               del_emit=g(s.del_emit, nr), del_inc=g(s.del_inc, nr), refl=np.array(s.refl[:]),
+               gstar=g(s.gstar, ng), d_gstar=g(s.d_gstar, ng))
+    lib.orc_syspar_free(sp)
+    return out, st.value
+
+
+def rebin(ener, ener0, flu0):
+    e = _arr(ener)
+    e0 = _arr(ener0)
+    f0 = _arr(flu0)
+    out = np.zeros(e.size - 1)
+    lib.orc_rebin_spectrum(_dp(e), _dp(out), e.size - 1, _dp(e0), _dp(f0), e0.size - 1)
+    return out
+
+
+def rebin_mean_flux(xn, x0, y0):
+    xn = _arr(xn)
+    x0 = _arr(x0)
+    y0 = _arr(y0)
+    out = np.zeros(xn.size - 1)
+    lib.orc_rebin_mean_flux(_dp(xn), _dp(out), xn.size - 1, _dp(x0), _dp(y0), x0.size - 1)
+    return out
+
+
+def log_grid(n_edges, emin, emax):
+    out = np.zeros(n_edges)
+    lib.orc_get_log_grid(_dp(out), n_edges, emin, emax)
+    return out
+
+
+def fine_radial_grid(rin, rout, nr):
+    out = np.zeros(nr)
+    lib.orc_get_fine_radial_grid(rin, rout, _dp(out), nr)
+    return out
+
+
+def fft(x, inverse=False):
+    x = np.asarray(x, dtype=np.complex128)
+    re = np.ascontiguousarray(x.real.copy())
+    im = np.ascontiguousarray(x.imag.copy())
+    lib.orc_fft(_dp(re), _dp(im), x.size, int(inverse))
+    return re + 1j * im
+
+
+def fft_conv(ener, fxill, frel, normalized=True):
+    e = _arr(ener)
+    n = e.size - 1
+    out = np.zeros(n)
+    fn = lib.orc_conv_normalized if normalized else lib.orc_fft_conv_spectrum
+    fn(_dp(e), _dp(_arr(fxill)), _dp(_arr(frel)), _dp(out), n)
+    return out
+
+
+def rzone_grid(rmin, rmax, nzones, h):
+    out = np.zeros(nzones + 1)
+    lib.orc_get_rzone_grid(rmin, rmax, nzones, h, _dp(out))
+    return out
+
+
+def relxill_conv(tables, params, conv_grid, refl_ener, refl, out_ener, nr_fine=1000,
+                 phys_norm=False):
+    p, st = rel_params(MOD_RELXILL, params, phys_norm)
+    if st:
+        return None, None, st
+    cg = _arr(conv_grid)
+    r = _arr(refl)
+    n_incl, n_refl = r.shape
+    oe = _arr(out_ener)
+    out = np.zeros(oe.size - 1)
+    dist = np.zeros(n_incl)
+    st = lib.orc_relxill_conv(ctypes.byref(tables.rel), ctypes.byref(p), _dp(cg), cg.size - 1,
+                              _dp(_arr(refl_ener)), _dp(r.reshape(-1)), n_refl, n_incl, _dp(oe),
+                              _dp(out), oe.size - 1, nr_fine, _dp(dist), None)
+    return out, dist, st
+
+
+def eval_batch_mt(tables, jobs, n_threads, conv_grid=None):
+    """jobs: list of (model, params, energy, flux_in or None); returns (fluxes, statuses)."""
+    arr = (Job * len(jobs))()
+    keep = []
+    outs = []
+    for i, (m, p, e, f) in enumerate(jobs):
+        pa = _arr(p)
+        ea = _arr(e)
+        fa = _arr(f).copy() if f is not None else np.zeros(ea.size - 1)
+        keep += [pa, ea]
+        outs.append(fa)
+        arr[i] = Job(m, _dp(pa), _dp(ea), _dp(fa), ea.size - 1, 0)
+    cg = _arr(conv_grid) if conv_grid is not None else None
+    lib.orc_eval_batch_mt(ctypes.byref(tables.rel), tables.lp_ptr, arr, len(jobs),
+                          _dp(cg) if cg is not None else None,
+                          (cg.size - 1) if cg is not None else 0, n_threads)
+    return outs, [arr[i].status for i in range(len(jobs))]

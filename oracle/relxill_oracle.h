@@ -1,0 +1,195 @@
+/*
+ * relxill_oracle.h -- FP64 CPU oracle for the relativistic smearing path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of relxill 2.4's
+ * relline / relline_lp / relconv algorithm (the reference is C/C++; every
+ * function in relxill_oracle.c cites the reference file:line it follows).
+ * It is linked only by tests/, by __graft_entry__.smoke() and by bench.py's
+ * cpu_baseline leg, always as the checker or the timed CPU baseline, never as
+ * part of the product path.
+ *
+ * Conventions follow the reference:
+ *   - energy grids have n+1 ascending edges, fluxes n values (photons/bin);
+ *   - relline tables are float32, node-major [ia][imu0][ir](.. [ig]);
+ *   - radii on the fine grid are descending (rout -> rin).
+ */
+#ifndef RELXILL_ORACLE_H_
+#define RELXILL_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* model types: same numeric ids as the reference (src/common.h:85-95) */
+#define ORC_MOD_RELLINE 1
+#define ORC_MOD_RELLINELP 2
+#define ORC_MOD_RELCONV 11
+#define ORC_MOD_RELCONVLP 12
+#define ORC_MOD_RELXILL (-1)
+
+/* emissivity types (src/common.h:50-55) */
+#define ORC_EMIS_BKN 1
+#define ORC_EMIS_LP 2
+
+/* relline transfer-function table (src/reltable.h:24-48), dims at run time */
+typedef struct {
+  int n_a, n_mu0, n_r, n_g;
+  const float *a;      /* [n_a] ascending spin */
+  const float *mu0;    /* [n_mu0] ascending cos(incl) */
+  const float *r;      /* [n_a][n_mu0][n_r] descending */
+  const float *gmin;   /* [n_a][n_mu0][n_r] */
+  const float *gmax;   /* [n_a][n_mu0][n_r] */
+  const float *trff1;  /* [n_a][n_mu0][n_r][n_g] */
+  const float *trff2;
+  const float *cosne1;
+  const float *cosne2;
+} orc_reltable;
+
+/* lamp-post table (src/reltable.h:51-69): the h grid is stored per spin */
+typedef struct {
+  int n_a, n_h, n_r;
+  const float *a;        /* [n_a] ascending */
+  const float *h;        /* [n_a][n_h] ascending */
+  const float *rad;      /* [n_a][n_r] ascending */
+  const float *intens;   /* [n_a][n_h][n_r] */
+  const float *del;      /* [n_a][n_h][n_r], already |.| as at load time */
+  const float *del_inc;  /* [n_a][n_h][n_r] */
+} orc_lptable;
+
+/* relativistic parameters (src/common.h:191-214, subset on this path) */
+typedef struct {
+  int model_type;
+  int emis_type;
+  double a, incl; /* incl in rad */
+  double emis1, emis2, rbr;
+  double rin, rout;
+  double lineE, z;
+  double height, htop, gamma, beta;
+  int limb;
+  int num_zones;
+  int return_rad;
+  int do_renorm;
+} orc_relparam;
+
+/* reflection fractions (src/common.h:279-285) */
+typedef struct {
+  double refl_frac, f_bh, f_ad, f_inf, f_inf_rest;
+} orc_reflfrac;
+
+/* per-call work counters (optional) */
+typedef struct {
+  int64_t relb_func_calls;
+  int64_t romberg_calls;
+  int64_t romberg_levels;
+  int64_t edge_calls;
+  int64_t midpoint_calls;
+  int64_t bin_integrals;
+} orc_stats;
+
+/* ---- physics helpers (src/Relphysics.cpp) ---- */
+double orc_kerr_rms(double a);
+double orc_kerr_rplus(double a);
+double orc_ut_disk(double r, double a);
+double orc_gi_potential_lp(double r, double a, double h, double bet, double del);
+double orc_lp_emissivity_newton(double h, double r);
+
+/* ---- grids and utilities (src/relutility.c) ---- */
+void orc_get_log_grid(double *ener, int n_edges, double emin, double emax);
+void orc_get_fine_radial_grid(double rin, double rout, double *re, int nr);
+int orc_binary_search(const double *arr, int n, double val);
+int orc_binary_search_float(const float *arr, int n, float val);
+int orc_inv_binary_search(const double *arr, int n, double val);
+double orc_interp_lin_1d(double ifac, double rlo, double rhi);
+double orc_interp_lin_2d(double i1, double i2, double r11, double r12, double r21, double r22);
+double orc_interp_lin_2d_float(double i1, double i2, float r11, float r12, float r21, float r22);
+void orc_rebin_spectrum(const double *ener, double *flu, int nbins,
+                        const double *ener0, const double *flu0, int nbins0);
+void orc_rebin_mean_flux(const double *xn, double *yn, int nn,
+                         const double *x0, const double *y0, int n0);
+int orc_get_rzone_grid(double rmin, double rmax, int nzones, double h, double *rgrid);
+
+/* ---- parameters (src/ModelDefinition.cpp:153-334, src/LocalModel.cpp) ---- */
+/* Fills *p from the XSPEC parameter array of model `model_type`; returns 0 on
+ * success, nonzero when the reference would reject the parameters. */
+int orc_get_rel_params(int model_type, const double *parameter, int phys_norm_env,
+                       orc_relparam *p);
+int orc_n_params(int model_type);
+
+/* ---- system parameters (src/Relprofile.cpp:141-303) ---- */
+typedef struct {
+  int nr, ng;
+  double *re, *gmin, *gmax;   /* [nr] */
+  double *gstar, *d_gstar;    /* [ng] */
+  double *trff, *cosne;       /* [nr][ng][2] */
+  double *emis, *del_emit, *del_inc; /* [nr] */
+  int limb_law;
+  orc_reflfrac refl;
+} orc_syspar;
+
+orc_syspar *orc_syspar_new(int nr, int ng);
+void orc_syspar_free(orc_syspar *s);
+int orc_interpol_reltable(const orc_reltable *tab, double a, double incl, double rin,
+                          double rout, orc_syspar *out);
+int orc_calc_emis_profile(const orc_lptable *lp, const orc_relparam *p, orc_syspar *s);
+/* everything get_system_parameters does (src/Relprofile.cpp:310-358) */
+orc_syspar *orc_get_system_parameters(const orc_reltable *tab, const orc_lptable *lp,
+                                      const orc_relparam *p, int nr_fine, int *status);
+
+/* ---- relline profile (src/Relprofile.cpp:835-958, :749-796) ---- */
+/* flux: [nzones][n_ener]; dist (optional, may be NULL): [nzones][n_incl] */
+int orc_calc_relline_profile(const orc_syspar *s, const double *ener, int n_ener,
+                             const double *rgrid, int nzones, double *flux,
+                             double *dist, int n_incl, orc_stats *st);
+void orc_renorm_relline_profile(const double *ener, int n_ener, int nzones, double *flux,
+                                double *dist, int n_incl, const orc_relparam *p);
+/* relbase (src/Relbase.cpp:524-538): syspar + zones + profile + renorm */
+int orc_relbase(const orc_reltable *tab, const orc_lptable *lp, const orc_relparam *p,
+                const double *ener, int n_ener, int nr_fine, double *flux,
+                double *dist, int n_incl, orc_stats *st);
+
+/* ---- FFT convolution (src/Relbase.cpp:93-246) ---- */
+void orc_fft(double *re, double *im, int n, int inverse);
+void orc_fft_conv_spectrum(const double *ener, const double *fxill, const double *frel,
+                           double *fout, int n);
+double orc_fft_norm_factor(const double *ener, const double *fxill, const double *frel,
+                           const double *fout, int n);
+void orc_conv_normalized(const double *ener, const double *fxill, const double *frel,
+                         double *fout, int n);
+void orc_zero_outside(const double *ener, double *spec, int n, double emin, double emax);
+/* relconv_kernel (src/Relbase.cpp:257-289) on a given convolution grid */
+int orc_relconv_kernel(const orc_reltable *tab, const orc_lptable *lp, const orc_relparam *p,
+                       const double *conv_ener, int n_conv, const double *ener_inp,
+                       double *spec_inp, int n_inp, int nr_fine, orc_stats *st);
+
+/* relxill-style angle-resolved convolution (src/Relxill.cpp:432-482 with
+ * src/Xillspec.cpp:557-573), one zone: refl[n_incl][n_refl] on refl_ener. */
+int orc_relxill_conv(const orc_reltable *tab, const orc_relparam *p, const double *conv_ener,
+                     int n_conv, const double *refl_ener, const double *refl, int n_refl,
+                     int n_incl, const double *out_ener, double *out, int n_out, int nr_fine,
+                     double *dist_out, orc_stats *st);
+
+/* ---- model level: what lmod<name> computes (src/LocalModel.cpp:32-98) ---- */
+/* energy: n_flux+1 edges; flux: n_flux (input spectrum for conv models).
+ * conv_ener/n_conv: the convolution grid for conv models (NULL -> reference grid). */
+int orc_eval_model(int model_type, const orc_reltable *tab, const orc_lptable *lp,
+                   const double *parameter, const double *energy, double *flux, int n_flux,
+                   const double *conv_ener, int n_conv, int phys_norm_env, orc_stats *st);
+
+/* multi-threaded batch of independent model evaluations (CPU baseline) */
+typedef struct {
+  int model_type;
+  const double *parameter;
+  const double *energy;
+  double *flux;
+  int n_flux;
+  int status;
+} orc_job;
+int orc_eval_batch_mt(const orc_reltable *tab, const orc_lptable *lp, orc_job *jobs, int n_jobs,
+                      const double *conv_ener, int n_conv, int n_threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

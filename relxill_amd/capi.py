@@ -1,0 +1,317 @@
+"""ctypes mirror of include/relxill_gpu.h.
+
+Function names, argument meaning and error behaviour follow the XSPEC local
+models of the reference (src/create_wrapper_xspec.py:153-162,
+src/LocalModel.cpp:32-160): energy grids are n+1 ascending edges in keV,
+fluxes n values in photons/bin; convolution models take the input spectrum in
+``flux`` and overwrite it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "librelxill_gpu.so")
+
+RX_OK = 0
+RX_ERR_PARAM = 1
+RX_ERR_UNSUPPORTED = 2
+RX_ERR_TABLE = 10
+RX_ERR_INPUT = 50
+RX_ERR_DEVICE = 100
+RX_ERR_ARG = 101
+
+MODEL_RELLINE = 1
+MODEL_RELLINELP = 2
+MODEL_RELCONV = 11
+MODEL_RELCONVLP = 12
+MODEL_RELXILL_CONV = -1
+
+FP64 = 0
+FP32 = 1
+
+N_PARAMS = {MODEL_RELLINE: 10, MODEL_RELCONV: 8, MODEL_RELLINELP: 10, MODEL_RELCONVLP: 9,
+            MODEL_RELXILL_CONV: 8}
+
+# default parameter values of the reference's lmodel.dat
+# (src/modelfiles/lmodel_relxill_public.dat:1-44)
+DEFAULTS = {
+    MODEL_RELLINE: [6.4, 3.0, 3.0, 15.0, 0.998, 30.0, -1.0, 400.0, 0.0, 0.0],
+    MODEL_RELCONV: [3.0, 3.0, 15.0, 0.998, 30.0, -1.0, 400.0, 0.0],
+    MODEL_RELLINELP: [6.4, 6.0, 0.998, 30.0, -1.0, 400.0, 0.0, 0.0, 2.0, 1.0],
+    MODEL_RELCONVLP: [6.0, 0.0, 0.998, 30.0, -1.0, 400.0, 0.0, 2.0, 1.0],
+    MODEL_RELXILL_CONV: [3.0, 3.0, 15.0, 0.998, 30.0, -1.0, 400.0, 0.0],
+}
+
+
+class RxError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"relxill_gpu status {status}: {msg}")
+        self.status = status
+
+
+_D = ctypes.POINTER(ctypes.c_double)
+_F = ctypes.POINTER(ctypes.c_float)
+
+
+class _Request(ctypes.Structure):
+    _fields_ = [("model", ctypes.c_int), ("n_flux", ctypes.c_int), ("energy", _D),
+                ("parameter", _D), ("flux_in", _D), ("refl_set", ctypes.c_int)]
+
+
+class _RelView(ctypes.Structure):
+    _fields_ = [("n_a", ctypes.c_int), ("n_mu0", ctypes.c_int), ("n_r", ctypes.c_int),
+                ("n_g", ctypes.c_int), ("a", _F), ("mu0", _F), ("r", _F), ("gmin", _F),
+                ("gmax", _F), ("trff1", _F), ("trff2", _F), ("cosne1", _F), ("cosne2", _F)]
+
+
+class _LpView(ctypes.Structure):
+    _fields_ = [("n_a", ctypes.c_int), ("n_h", ctypes.c_int), ("n_r", ctypes.c_int),
+                ("a", _F), ("h", _F), ("rad", _F), ("intens", _F), ("del_", _F),
+                ("del_inc", _F)]
+
+
+_EXPORTS = {
+    # name: (restype, argtypes)
+    "lmodrelline": (None, [_D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_char_p]),
+    "lmodrellinelp": (None, [_D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_char_p]),
+    "lmodrelconv": (None, [_D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_char_p]),
+    "lmodrelconvlp": (None, [_D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_char_p]),
+    "rx_eval_model": (ctypes.c_int, [ctypes.c_int, _D, _D, _D, ctypes.c_int, ctypes.c_int]),
+    "rx_last_status": (ctypes.c_int, []),
+    "rx_last_error": (ctypes.c_char_p, []),
+    "rx_model_params": (ctypes.c_int, [ctypes.c_int, _D, _D]),
+    "rx_init": (ctypes.c_int, [ctypes.c_int]),
+    "rx_shutdown": (None, []),
+    "rx_tables_synthetic": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int]),
+    "rx_lptable_synthetic": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int]),
+    "rx_tables_load": (ctypes.c_int, [ctypes.c_char_p]),
+    "rx_lptable_load": (ctypes.c_int, [ctypes.c_char_p]),
+    "rx_tables_save": (ctypes.c_int, [ctypes.c_char_p]),
+    "rx_lptable_save": (ctypes.c_int, [ctypes.c_char_p]),
+    "rx_tables_view": (ctypes.c_int, [ctypes.POINTER(_RelView), ctypes.POINTER(_LpView)]),
+    "rx_set_conv_grid": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double]),
+    "rx_set_nr_fine": (ctypes.c_int, [ctypes.c_int]),
+    "rx_batch_create": (ctypes.c_void_p, [ctypes.POINTER(_Request), ctypes.c_int]),
+    "rx_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "rx_batch_sync": (ctypes.c_int, [ctypes.c_void_p]),
+    "rx_batch_output": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D]),
+    "rx_batch_status": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "rx_batch_dist": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_int]),
+    "rx_batch_reflfrac": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D]),
+    "rx_batch_set_rchunks": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "rx_batch_destroy": (None, [ctypes.c_void_p]),
+    "rx_reflset_create": (ctypes.c_int, [_D, ctypes.c_int, _D, ctypes.c_int]),
+    "rx_profile_enable": (ctypes.c_int, [ctypes.c_int]),
+    "rx_batch_kernel_ms": (ctypes.c_int, [ctypes.c_void_p, _D]),
+    "rx_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "rx_server_start": (ctypes.c_void_p, [ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int]),
+    "rx_server_submit": (ctypes.c_int64, [ctypes.c_void_p, ctypes.POINTER(_Request)]),
+    "rx_server_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, _D,
+                                      ctypes.POINTER(ctypes.c_int), _D]),
+    "rx_server_drain": (ctypes.c_int, [ctypes.c_void_p]),
+    "rx_server_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    "rx_server_stop": (None, [ctypes.c_void_p]),
+}
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m relxill_amd.build` "
+            "(there is no CPU fallback)")
+    lib_ = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _EXPORTS.items():
+        fn = getattr(lib_, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib_
+
+
+lib = _load()
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _arr(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+def _check(status):
+    if status != RX_OK:
+        raise RxError(status, lib.rx_last_error().decode())
+
+
+def model_params(model, params):
+    """Processed parameters: a, incl[rad], Index1, Index2, Rbr, Rin, Rout, lineE, z, h, gamma, beta."""
+    p = _arr(params)
+    out = np.zeros(12)
+    st = lib.rx_model_params(model, _dp(p), _dp(out))
+    return out, st
+
+
+def eval_model(model, params, energy, flux_in=None, precision=FP64, raise_on_error=True):
+    """One evaluation through rx_eval_model; returns (flux, status)."""
+    e = _arr(energy)
+    p = _arr(params)
+    n = e.size - 1
+    flux = _arr(flux_in).copy() if flux_in is not None else np.zeros(n)
+    st = lib.rx_eval_model(model, _dp(p), _dp(e), _dp(flux), n, precision)
+    if raise_on_error and st not in (RX_OK,):
+        raise RxError(st, lib.rx_last_error().decode())
+    return flux, st
+
+
+def _lmod(fn, energy, params, flux=None):
+    e = _arr(energy)
+    p = _arr(params)
+    n = e.size - 1
+    f = _arr(flux).copy() if flux is not None else np.zeros(n)
+    fn(_dp(e), n, _dp(p), 0, _dp(f), None, None)
+    return f
+
+
+def lmodrelline(energy, params):
+    return _lmod(lib.lmodrelline, energy, params)
+
+
+def lmodrellinelp(energy, params):
+    return _lmod(lib.lmodrellinelp, energy, params)
+
+
+def lmodrelconv(energy, params, flux):
+    return _lmod(lib.lmodrelconv, energy, params, flux)
+
+
+def lmodrelconvlp(energy, params, flux):
+    return _lmod(lib.lmodrelconvlp, energy, params, flux)
+
+
+def tables_synthetic(seed=1, n_a=25, n_mu0=22, n_r=100, n_g=20):
+    _check(lib.rx_tables_synthetic(seed, n_a, n_mu0, n_r, n_g))
+
+
+def lptable_synthetic(seed=3, n_a=20, n_h=100, n_r=100):
+    _check(lib.rx_lptable_synthetic(seed, n_a, n_h, n_r))
+
+
+def set_conv_grid(n=4096, emin=0.00035, emax=2000.0):
+    _check(lib.rx_set_conv_grid(n, emin, emax))
+
+
+def tables_view():
+    """Copies of the resident tables as numpy arrays (rel, lp dicts)."""
+    rv, lv = _RelView(), _LpView()
+    _check(lib.rx_tables_view(ctypes.byref(rv), ctypes.byref(lv)))
+    nn = rv.n_a * rv.n_mu0
+
+    def f(ptr, n):
+        return np.ctypeslib.as_array(ptr, shape=(n,)).astype(np.float32).copy()
+
+    rel = dict(n_a=rv.n_a, n_mu0=rv.n_mu0, n_r=rv.n_r, n_g=rv.n_g,
+               a=f(rv.a, rv.n_a), mu0=f(rv.mu0, rv.n_mu0),
+               r=f(rv.r, nn * rv.n_r), gmin=f(rv.gmin, nn * rv.n_r), gmax=f(rv.gmax, nn * rv.n_r),
+               trff1=f(rv.trff1, nn * rv.n_r * rv.n_g), trff2=f(rv.trff2, nn * rv.n_r * rv.n_g),
+               cosne1=f(rv.cosne1, nn * rv.n_r * rv.n_g), cosne2=f(rv.cosne2, nn * rv.n_r * rv.n_g))
+    n3 = lv.n_a * lv.n_h * lv.n_r
+    lp = dict(n_a=lv.n_a, n_h=lv.n_h, n_r=lv.n_r, a=f(lv.a, lv.n_a), h=f(lv.h, lv.n_a * lv.n_h),
+              rad=f(lv.rad, lv.n_a * lv.n_r), intens=f(lv.intens, n3), del_=f(lv.del_, n3),
+              del_inc=f(lv.del_inc, n3))
+    return rel, lp
+
+
+def reflset_create(refl_energy, refl):
+    """Resident reflection set; refl has shape (n_incl, n_refl)."""
+    e = _arr(refl_energy)
+    r = _arr(refl)
+    n_incl, n_refl = r.shape
+    sid = lib.rx_reflset_create(_dp(e), n_refl, _dp(r.reshape(-1)), n_incl)
+    if sid < 0:
+        raise RxError(-sid, lib.rx_last_error().decode())
+    return sid
+
+
+class Batch:
+    """Device-resident batch of requests (rx_batch_*)."""
+
+    def __init__(self, requests):
+        # requests: list of dicts with model, energy, params, [flux_in], [refl_set]
+        self._keep = []
+        arr = (_Request * len(requests))()
+        self.n_flux = []
+        for i, r in enumerate(requests):
+            e = _arr(r["energy"])
+            p = _arr(r["params"])
+            self._keep += [e, p]
+            arr[i].model = r["model"]
+            arr[i].n_flux = e.size - 1
+            arr[i].energy = _dp(e)
+            arr[i].parameter = _dp(p)
+            if r.get("flux_in") is not None:
+                f = _arr(r["flux_in"])
+                self._keep.append(f)
+                arr[i].flux_in = _dp(f)
+            else:
+                arr[i].flux_in = None
+            arr[i].refl_set = r.get("refl_set", -1)
+            self.n_flux.append(e.size - 1)
+        h = lib.rx_batch_create(arr, len(requests))
+        if not h:
+            raise RxError(lib.rx_last_status(), lib.rx_last_error().decode())
+        self.h = h
+        self.n = len(requests)
+        self._keep = []
+
+    def run(self, precision=FP64):
+        _check(lib.rx_batch_run(self.h, precision))
+
+    def sync(self):
+        _check(lib.rx_batch_sync(self.h))
+
+    def output(self, i):
+        out = np.zeros(self.n_flux[i])
+        st = lib.rx_batch_output(self.h, i, _dp(out))
+        return out, st
+
+    def outputs(self):
+        return [self.output(i) for i in range(self.n)]
+
+    def dist(self, i, n_incl):
+        d = np.zeros(n_incl)
+        _check(lib.rx_batch_dist(self.h, i, _dp(d), n_incl))
+        return d
+
+    def reflfrac(self, i):
+        o = np.zeros(5)
+        _check(lib.rx_batch_reflfrac(self.h, i, _dp(o)))
+        return o
+
+    def set_rchunks(self, n):
+        _check(lib.rx_batch_set_rchunks(self.h, n))
+
+    def autotune(self, precision):
+        r = lib.rx_batch_autotune(self.h, precision)
+        if r <= 0:
+            raise RxError(r, lib.rx_last_error().decode())
+        return r
+
+    def kernel_ms(self):
+        o = np.zeros(4)
+        _check(lib.rx_batch_kernel_ms(self.h, _dp(o)))
+        return o
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.rx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,481 @@
+// capi.cpp -- extern "C" entry points of librelxill_gpu.so.
+//
+// Nothing throws across this boundary: every error becomes a status (the
+// reference lets ModelEvalFailed escape its extern "C" wrapper,
+// src/LocalModel.cpp:143-160).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+
+#include "engine.h"
+#include "rx_math.h"
+
+using namespace rx;
+
+struct rx_batch {
+  Batch b;
+};
+
+namespace {
+
+// one reusable single-request workspace for the XSPEC entry points
+Batch &plain_batch() {
+  static Batch b;
+  return b;
+}
+
+int eval_one(int model, const double *parameter, const double *energy, double *flux, int n_flux,
+             int precision) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  std::string err;
+  if (!energy || !parameter || !flux || n_flux <= 0) {
+    set_last(RX_ERR_ARG, "null pointer or empty grid");
+    return RX_ERR_ARG;
+  }
+  int st = ensure_device(err);
+  if (st != RX_OK) {
+    for (int i = 0; i < n_flux; i++) flux[i] = 0.0;
+    set_last(st, err);
+    return st;
+  }
+  rx_request r;
+  r.model = model;
+  r.n_flux = n_flux;
+  r.energy = energy;
+  r.parameter = parameter;
+  r.flux_in = flux;
+  r.refl_set = -1;
+  Batch &b = plain_batch();
+  st = batch_build(b, &r, 1, err);
+  if (st == RX_OK) st = batch_run(b, precision, err);
+  if (st == RX_OK) st = batch_sync(b);
+  if (st == RX_OK) {
+    if (hipMemcpy(flux, b.d_out.p, sizeof(double) * n_flux, hipMemcpyDeviceToHost) != hipSuccess) {
+      st = RX_ERR_DEVICE;
+      err = "device to host copy failed";
+    }
+  }
+  if (st == RX_OK) {
+    int dst = 0;
+    if (hipMemcpy(&dst, b.d_status.p, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) dst = RX_ERR_DEVICE;
+    if (dst != 0) {
+      st = (dst >= 10 && dst < 50) ? RX_ERR_TABLE : dst;
+      err = "model evaluation failed (status " + std::to_string(dst) + ")";
+    }
+  }
+  if (st != RX_OK) {
+    for (int i = 0; i < n_flux; i++) flux[i] = 0.0;
+  }
+  set_last(st, err);
+  return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rx_eval_model(int model, const double *parameter, const double *energy, double *flux, int n_flux,
+                  int precision) {
+  try {
+    return eval_one(model, parameter, energy, flux, n_flux, precision);
+  } catch (const std::bad_alloc &) {
+    set_last(RX_ERR_NOMEM, "out of host memory");
+    return RX_ERR_NOMEM;
+  } catch (...) {
+    set_last(RX_ERR_DEVICE, "unexpected exception");
+    return RX_ERR_DEVICE;
+  }
+}
+
+void lmodrelline(const double *energy, int Nflux, const double *parameter, int, double *flux, double *,
+                 const char *) {
+  rx_eval_model(RX_MODEL_RELLINE, parameter, energy, flux, Nflux, RX_FP64);
+}
+
+void lmodrellinelp(const double *energy, int Nflux, const double *parameter, int, double *flux, double *,
+                   const char *) {
+  rx_eval_model(RX_MODEL_RELLINELP, parameter, energy, flux, Nflux, RX_FP64);
+}
+
+void lmodrelconv(const double *energy, int Nflux, const double *parameter, int, double *flux, double *,
+                 const char *) {
+  rx_eval_model(RX_MODEL_RELCONV, parameter, energy, flux, Nflux, RX_FP64);
+}
+
+void lmodrelconvlp(const double *energy, int Nflux, const double *parameter, int, double *flux, double *,
+                   const char *) {
+  rx_eval_model(RX_MODEL_RELCONVLP, parameter, energy, flux, Nflux, RX_FP64);
+}
+
+int rx_last_status(void) { return last_status(); }
+const char *rx_last_error(void) { return last_error(); }
+
+int rx_init(int device) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (g.device_ready && g.device != device) {
+    set_last(RX_ERR_ARG, "library already initialised on another device");
+    return RX_ERR_ARG;
+  }
+  g.device = device;
+  std::string err;
+  int st = ensure_device(err);
+  set_last(st, err);
+  return st;
+}
+
+void rx_shutdown(void) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (g.device_ready) (void)hipDeviceSynchronize();
+}
+
+int rx_tables_synthetic(uint64_t seed, int n_a, int n_mu0, int n_r, int n_g) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (n_a < 2 || n_mu0 < 2 || n_r < 3 || n_g < 2 || n_r > MAX_NR_TAB || n_g > MAX_NG) {
+    set_last(RX_ERR_ARG, "table dims out of range");
+    return RX_ERR_ARG;
+  }
+  try {
+    g.rel = make_synthetic_reltable(seed, n_a, n_mu0, n_r, n_g);
+  } catch (...) {
+    set_last(RX_ERR_NOMEM, "table allocation failed");
+    return RX_ERR_NOMEM;
+  }
+  g.rel_loaded = true;
+  g.rel_dirty = true;
+  std::string e = validate_reltable(g.rel);
+  set_last(e.empty() ? RX_OK : RX_ERR_TABLE, e);
+  return e.empty() ? RX_OK : RX_ERR_TABLE;
+}
+
+int rx_lptable_synthetic(uint64_t seed, int n_a, int n_h, int n_r) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (n_a < 2 || n_h < 2 || n_r < 2 || n_r > MAX_LP_R) {
+    set_last(RX_ERR_ARG, "lamp-post table dims out of range");
+    return RX_ERR_ARG;
+  }
+  g.lp = make_synthetic_lptable(seed, n_a, n_h, n_r);
+  g.lp_loaded = true;
+  g.lp_dirty = true;
+  std::string e = validate_lptable(g.lp);
+  set_last(e.empty() ? RX_OK : RX_ERR_TABLE, e);
+  return e.empty() ? RX_OK : RX_ERR_TABLE;
+}
+
+int rx_tables_load(const char *path) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!path) return RX_ERR_ARG;
+  RelTableHost t;
+  std::string e = load_reltable(t, path);
+  if (!e.empty()) {
+    set_last(RX_ERR_TABLE, e);
+    return RX_ERR_TABLE;
+  }
+  g.rel = std::move(t);
+  g.rel_loaded = true;
+  g.rel_dirty = true;
+  return RX_OK;
+}
+
+int rx_lptable_load(const char *path) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!path) return RX_ERR_ARG;
+  LpTableHost t;
+  std::string e = load_lptable(t, path);
+  if (!e.empty()) {
+    set_last(RX_ERR_TABLE, e);
+    return RX_ERR_TABLE;
+  }
+  g.lp = std::move(t);
+  g.lp_loaded = true;
+  g.lp_dirty = true;
+  return RX_OK;
+}
+
+int rx_tables_save(const char *path) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  ensure_tables_host();
+  std::string e = save_reltable(g.rel, path ? path : "");
+  set_last(e.empty() ? RX_OK : RX_ERR_ARG, e);
+  return e.empty() ? RX_OK : RX_ERR_ARG;
+}
+
+int rx_lptable_save(const char *path) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  ensure_tables_host();
+  std::string e = save_lptable(g.lp, path ? path : "");
+  set_last(e.empty() ? RX_OK : RX_ERR_ARG, e);
+  return e.empty() ? RX_OK : RX_ERR_ARG;
+}
+
+int rx_tables_view(rx_reltable_view *rel, rx_lptable_view *lp) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  ensure_tables_host();
+  if (rel) {
+    const RelTableHost &t = g.rel;
+    *rel = {t.n_a, t.n_mu0, t.n_r, t.n_g, t.a.data(), t.mu0.data(), t.r.data(), t.gmin.data(),
+            t.gmax.data(), t.trff1.data(), t.trff2.data(), t.cosne1.data(), t.cosne2.data()};
+  }
+  if (lp) {
+    const LpTableHost &t = g.lp;
+    *lp = {t.n_a, t.n_h, t.n_r, t.a.data(), t.h.data(), t.rad.data(), t.intens.data(), t.del.data(),
+           t.del_inc.data()};
+  }
+  return RX_OK;
+}
+
+int rx_set_conv_grid(int n, double emin, double emax) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (n < 64 || n > MAX_FFT || (n & (n - 1)) != 0 || !(emin > 0) || !(emax > emin)) {
+    set_last(RX_ERR_ARG, "convolution grid must have a power-of-two size in [64, 8192]");
+    return RX_ERR_ARG;
+  }
+  if (!(emin < 1.0 && emax > 1.0)) {
+    set_last(RX_ERR_ARG, "convolution grid must contain 1 keV");
+    return RX_ERR_ARG;
+  }
+  if (!g.refl_sets.empty() && n != g.conv_n) {
+    set_last(RX_ERR_ARG, "cannot change the convolution grid size with resident reflection sets");
+    return RX_ERR_ARG;
+  }
+  g.conv_n = n;
+  g.conv_emin = emin;
+  g.conv_emax = emax;
+  g.conv_dirty = true;
+  ensure_tables_host();
+  return RX_OK;
+}
+
+int rx_set_nr_fine(int nr_fine) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (nr_fine < 3 || nr_fine > 100000) return RX_ERR_ARG;
+  g.nr_fine = nr_fine;
+  return RX_OK;
+}
+
+/* ---- batches ---- */
+
+rx_batch *rx_batch_create(const rx_request *req, int n_req) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  std::string err;
+  try {
+    int st = ensure_device(err);
+    if (st != RX_OK) {
+      set_last(st, err);
+      return nullptr;
+    }
+    std::unique_ptr<rx_batch> b(new rx_batch);
+    st = batch_build(b->b, req, n_req, err);
+    if (st != RX_OK) {
+      set_last(st, err);
+      return nullptr;
+    }
+    set_last(RX_OK, "");
+    return b.release();
+  } catch (...) {
+    set_last(RX_ERR_NOMEM, "batch allocation failed");
+    return nullptr;
+  }
+}
+
+int rx_batch_run(rx_batch *b, int precision) {
+  if (!b) return RX_ERR_ARG;
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  std::string err;
+  int st = ensure_device(err);
+  if (st == RX_OK) st = batch_run(b->b, precision, err);
+  set_last(st, err);
+  return st;
+}
+
+int rx_batch_sync(rx_batch *b) {
+  if (!b) return RX_ERR_ARG;
+  return batch_sync(b->b);
+}
+
+int rx_batch_status(rx_batch *b, int i) {
+  if (!b || i < 0 || i >= b->b.n_req) return RX_ERR_ARG;
+  int st = 0;
+  if (hipMemcpy(&st, b->b.d_status.as<int>() + i, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+    return RX_ERR_DEVICE;
+  if (st >= 10 && st < 50) st = RX_ERR_TABLE;
+  return st;
+}
+
+int rx_batch_output(rx_batch *b, int i, double *flux) {
+  if (!b || !flux || i < 0 || i >= b->b.n_req) return RX_ERR_ARG;
+  const ReqParams &p = b->b.h_req[i];
+  if (hipMemcpy(flux, b->b.d_out.as<double>() + p.flux_off, sizeof(double) * p.n_in,
+                hipMemcpyDeviceToHost) != hipSuccess)
+    return RX_ERR_DEVICE;
+  return rx_batch_status(b, i);
+}
+
+int rx_batch_dist(rx_batch *b, int i, double *dist, int n_incl) {
+  if (!b || !dist || i < 0 || i >= b->b.n_req || n_incl > MAX_INCL) return RX_ERR_ARG;
+  if (hipMemcpy(dist, b->b.d_dist.as<double>() + (size_t)i * MAX_INCL, sizeof(double) * n_incl,
+                hipMemcpyDeviceToHost) != hipSuccess)
+    return RX_ERR_DEVICE;
+  return RX_OK;
+}
+
+int rx_batch_reflfrac(rx_batch *b, int i, double *out5) {
+  if (!b || !out5 || i < 0 || i >= b->b.n_req) return RX_ERR_ARG;
+  LpRefl r;
+  if (hipMemcpy(&r, b->b.d_refl.as<LpRefl>() + i, sizeof(LpRefl), hipMemcpyDeviceToHost) != hipSuccess)
+    return RX_ERR_DEVICE;
+  out5[0] = r.refl_frac;
+  out5[1] = r.f_bh;
+  out5[2] = r.f_ad;
+  out5[3] = r.f_inf;
+  out5[4] = r.f_inf_rest;
+  return RX_OK;
+}
+
+int rx_batch_set_rchunks(rx_batch *b, int n_rch) {
+  if (!b || n_rch < 0 || n_rch > 1000) return RX_ERR_ARG;
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  b->b.n_rch_forced = n_rch;
+  batch_plan(b->b, n_rch);
+  return batch_upload(b->b);
+}
+
+void rx_batch_destroy(rx_batch *b) {
+  if (!b) return;
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (b->b.stream) (void)hipStreamSynchronize(b->b.stream);
+  delete b;
+}
+
+int rx_profile_enable(int on) {
+  G().profile = (on != 0);
+  return RX_OK;
+}
+
+int rx_batch_kernel_ms(rx_batch *b, double *ms4) {
+  if (!b || !ms4) return RX_ERR_ARG;
+  for (int k = 0; k < 4; k++) ms4[k] = b->b.kernel_ms[k];
+  return b->b.profiled ? RX_OK : RX_ERR_ARG;
+}
+
+int rx_batch_autotune(rx_batch *b, int precision) {
+  if (!b) return RX_ERR_ARG;
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  std::string err;
+  const int cands[] = {1, 2, 4, 8, 16, 32};
+  double best = 1e30;
+  int best_r = 1;
+  for (int r : cands) {
+    batch_plan(b->b, r);
+    if (batch_upload(b->b) != RX_OK) return RX_ERR_DEVICE;
+    if (batch_run(b->b, precision, err) != RX_OK || batch_sync(b->b) != RX_OK) return RX_ERR_DEVICE;
+    auto t0 = std::chrono::steady_clock::now();
+    const int reps = 2;
+    for (int k = 0; k < reps; k++) {
+      if (batch_run(b->b, precision, err) != RX_OK) return RX_ERR_DEVICE;
+    }
+    if (batch_sync(b->b) != RX_OK) return RX_ERR_DEVICE;
+    double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / reps;
+    if (dt < best * 0.97) {
+      best = dt;
+      best_r = r;
+    }
+  }
+  bool found = false;
+  for (auto &t : g.tune)
+    if (t.first == b->b.shape_class) {
+      t.second = best_r;
+      found = true;
+    }
+  if (!found) g.tune.push_back({b->b.shape_class, best_r});
+  b->b.n_rch_forced = best_r;
+  batch_plan(b->b, best_r);
+  if (batch_upload(b->b) != RX_OK) return RX_ERR_DEVICE;
+  return best_r;
+}
+
+/* ---- reflection sets ---- */
+
+int rx_reflset_create(const double *refl_energy, int n_refl, const double *refl, int n_incl) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!refl_energy || !refl || n_refl <= 0 || n_incl <= 0 || n_incl > MAX_INCL) {
+    set_last(RX_ERR_ARG, "invalid reflection set");
+    return -RX_ERR_ARG;
+  }
+  std::string err;
+  int st = ensure_device(err);
+  if (st != RX_OK) {
+    set_last(st, err);
+    return -st;
+  }
+  const int n = g.conv_n;
+  // buffers: energy, spectra, rebinned spectra, band sums, FFTs
+  DevBuf *bufs = new DevBuf[5];
+  int e = bufs[0].ensure(sizeof(double) * (n_refl + 1)) | bufs[1].ensure(sizeof(double) * (size_t)n_refl * n_incl) |
+          bufs[2].ensure(sizeof(double) * (size_t)n * n_incl) | bufs[3].ensure(sizeof(double) * n_incl) |
+          bufs[4].ensure(sizeof(double) * (size_t)(n / 2 + 1) * 2 * n_incl);
+  if (e) {
+    delete[] bufs;
+    set_last(RX_ERR_NOMEM, "device allocation failed");
+    return -RX_ERR_NOMEM;
+  }
+  e |= (int)hipMemcpy(bufs[0].p, refl_energy, sizeof(double) * (n_refl + 1), hipMemcpyHostToDevice);
+  e |= (int)hipMemcpy(bufs[1].p, refl, sizeof(double) * (size_t)n_refl * n_incl, hipMemcpyHostToDevice);
+  BatchDev bd;
+  memset(&bd, 0, sizeof(bd));
+  fill_dev_views(bd);
+  e |= launch_refl_prepare(bd.cg, bufs[0].as<double>(), bufs[1].as<double>(), n_refl, n_incl,
+                           bufs[2].as<double>(), bufs[3].as<double>(), bufs[4].as<double>(), nullptr);
+  e |= (int)hipDeviceSynchronize();
+  if (e) {
+    delete[] bufs;
+    set_last(RX_ERR_DEVICE, "reflection set preparation failed");
+    return -RX_ERR_DEVICE;
+  }
+  DevReflSet rs;
+  rs.n_incl = n_incl;
+  rs.spec = bufs[2].as<double>();
+  rs.bandsum = bufs[3].as<double>();
+  rs.fft = bufs[4].as<double>();
+  g.refl_sets.push_back(rs);
+  g.refl_bufs.push_back(bufs);
+  g.refl_dirty = true;
+  st = ensure_device(err);
+  if (st != RX_OK) return -st;
+  return (int)g.refl_sets.size() - 1;
+}
+
+/* ---- parameter processing (host mirror of get_rel_params), for checking ---- */
+int rx_model_params(int model, const double *parameter, double *out12) {
+  if (!parameter || !out12) return RX_ERR_ARG;
+  ReqParams p;
+  std::string err;
+  int st = parse_request(model, parameter, phys_norm_env(), p, err);
+  const double v[12] = {p.a, p.incl, p.emis1, p.emis2, p.rbr, p.rin, p.rout, p.lineE, p.z, p.height,
+                        p.gamma, p.beta};
+  memcpy(out12, v, sizeof v);
+  set_last(st, err);
+  return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,558 @@
+// engine.cpp -- host runtime: parameters, tables on the device, batch planning
+// and kernel launches.
+#include "engine.h"
+#include "rx_math.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+namespace rx {
+
+namespace {
+constexpr double kPi = 3.14159265358979323846;
+thread_local int t_status = 0;
+thread_local std::string t_error;
+}  // namespace
+
+void set_last(int status, const std::string &msg) {
+  t_status = status;
+  t_error = msg;
+}
+int last_status() { return t_status; }
+const char *last_error() { return t_error.c_str(); }
+
+Global &G() {
+  static Global g;
+  return g;
+}
+
+DevBuf::~DevBuf() {
+  if (p) (void)hipFree(p);
+}
+
+int DevBuf::ensure(size_t n) {
+  if (n == 0) n = 8;
+  if (p && bytes >= n) return 0;
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+  hipError_t e = hipMalloc(&p, n);
+  if (e != hipSuccess) {
+    p = nullptr;
+    return (int)e;
+  }
+  bytes = n;
+  return 0;
+}
+
+bool phys_norm_env() {
+  // do_not_normalize_relline, src/relutility.c:386-396
+  const char *env = getenv("RELLINE_PHYSICAL_NORM");
+  return env && (int)strtod(env, nullptr) == 1;
+}
+
+int n_model_params(int model) {
+  switch (model) {
+    case MOD_RELLINE: return 10;
+    case MOD_RELCONV: return 8;
+    case MOD_RELLINELP: return 10;
+    case MOD_RELCONVLP: return 9;
+    case MOD_RELXILL_CONV: return 8;
+    default: return -1;
+  }
+}
+
+// get_rel_params (src/ModelDefinition.cpp:278-334) with check_parameter_bounds (:168-273)
+int parse_request(int model, const double *par, bool phys_norm, ReqParams &p, std::string &err) {
+  memset(&p, 0, sizeof(p));
+  p.model = model;
+  p.pgrid_off = -1;
+  p.prof_off = -1;
+  double incl_deg = 0, htop = 0;
+  int return_rad = 0;
+  switch (model) {
+    case MOD_RELLINE:  // lineE Index1 Index2 Rbr a Incl Rin Rout z limb
+      p.emis_type = EMIS_BKN;
+      p.lineE = par[0]; p.emis1 = par[1]; p.emis2 = par[2]; p.rbr = par[3]; p.a = par[4];
+      incl_deg = par[5]; p.rin = par[6]; p.rout = par[7]; p.z = par[8];
+      p.limb = (int)lround(par[9]);
+      break;
+    case MOD_RELCONV:  // Index1 Index2 Rbr a Incl Rin Rout limb
+    case MOD_RELXILL_CONV:
+      p.emis_type = EMIS_BKN;
+      p.emis1 = par[0]; p.emis2 = par[1]; p.rbr = par[2]; p.a = par[3]; incl_deg = par[4];
+      p.rin = par[5]; p.rout = par[6]; p.limb = (int)lround(par[7]);
+      break;
+    case MOD_RELLINELP:  // lineE h a Incl Rin Rout z limb gamma switch_returnrad
+      p.emis_type = EMIS_LP;
+      p.lineE = par[0]; p.height = par[1]; p.a = par[2]; incl_deg = par[3]; p.rin = par[4];
+      p.rout = par[5]; p.z = par[6]; p.limb = (int)lround(par[7]); p.gamma = par[8];
+      return_rad = (int)lround(par[9]);
+      break;
+    case MOD_RELCONVLP:  // h beta a Incl Rin Rout limb gamma switch_returnrad
+      p.emis_type = EMIS_LP;
+      p.height = par[0]; p.beta = par[1]; p.a = par[2]; incl_deg = par[3]; p.rin = par[4];
+      p.rout = par[5]; p.limb = (int)lround(par[6]); p.gamma = par[7];
+      return_rad = (int)lround(par[8]);
+      break;
+    default:
+      err = "unknown model id";
+      return RX_ERR_ARG;
+  }
+  p.incl = incl_deg * kPi / 180;
+  // do_renorm_model (src/relutility.c:603-623)
+  if (model < 0) {
+    p.do_renorm = (p.emis_type == EMIS_LP || phys_norm) ? 0 : 1;
+  } else {
+    p.do_renorm = phys_norm ? 0 : 1;
+  }
+  // check_parameter_bounds
+  int status = RX_OK;
+  if (p.rin < 0) p.rin = -1.0 * p.rin * kerr_rms(p.a);
+  if (p.rout < 0) p.rout = -1.0 * p.rout * kerr_rms(p.a);
+  if (p.rbr < 0) p.rbr = -1.0 * p.rbr * kerr_rms(p.a);
+  if (p.rout <= p.rin) status = RX_ERR_PARAM;
+  const double rms = kerr_rms(p.a);
+  if (p.rin < rms) p.rin = rms;
+  if (p.a > 0.9982) { err = "spin a > 0.9982"; return RX_ERR_PARAM; }
+  if (p.a < -1) { err = "spin a < -1"; return RX_ERR_PARAM; }
+  if (p.incl < 3 * kPi / 180 || p.incl > 87 * kPi / 180) {
+    err = "inclination outside 3-87 deg";
+    return RX_ERR_PARAM;
+  }
+  if (p.rout <= p.rin) { err = "Rout <= Rin"; return RX_ERR_PARAM; }
+  if (p.rout > 1000.0) p.rout = 1000.0;
+  if (p.emis_type == EMIS_BKN) {
+    if (p.rbr < p.rin) p.rbr = p.rin;
+    if (p.rbr > p.rout) p.rbr = p.rout;
+  }
+  if (p.emis_type == EMIS_LP) {
+    if (p.beta < 0) p.beta = 0.0;
+    if (p.beta > 0.99) p.beta = 0.99;
+    if (p.height < 0) p.height = -1.0 * p.height * kerr_rplus(p.a);
+    if (htop < 0) htop = -1.0 * htop * kerr_rplus(p.a);
+    const double r_event = kerr_rplus(p.a);
+    if ((1.1 * r_event - p.height) > 1e-4) p.height = r_event * 1.1;
+  }
+  if (status != RX_OK) {
+    err = "Rin >= Rout";
+    return status;
+  }
+  if (return_rad != 0) {
+    err = "returning radiation (switch_returnrad != 0) is not supported";
+    return RX_ERR_UNSUPPORTED;
+  }
+  p.norm = 1.0;
+  if (model < 0 && p.emis_type == EMIS_BKN) {
+    // norm_factor_semi_infinite_slab, src/Xillspec.cpp:528-530
+    p.norm = 0.5 * cos((p.incl * 180.0 / kPi) * kPi / 180);
+  }
+  return RX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// tables and grids
+// ---------------------------------------------------------------------------
+
+void ensure_tables_host() {
+  Global &g = G();
+  if (!g.rel_loaded) {
+    bool ok = false;
+    if (const char *path = getenv("RELXILL_TABLE_PATH")) {
+      std::string e = load_reltable(g.rel, std::string(path) + "/rel_table.rxtb");
+      ok = e.empty();
+    }
+    if (!ok) g.rel = make_synthetic_reltable(1, 25, 22, 100, 20);
+    g.rel_loaded = true;
+    g.rel_dirty = true;
+  }
+  if (!g.lp_loaded) {
+    bool ok = false;
+    if (const char *path = getenv("RELXILL_TABLE_PATH")) {
+      std::string e = load_lptable(g.lp, std::string(path) + "/rel_lp_table.rxtb");
+      ok = e.empty();
+    }
+    if (!ok) g.lp = make_synthetic_lptable(3, 20, 100, 100);
+    g.lp_loaded = true;
+    g.lp_dirty = true;
+  }
+  if (g.conv_ener.empty()) g.conv_dirty = true;
+  if (g.conv_dirty) {
+    const int n = g.conv_n;
+    g.conv_ener.resize(n + 1);
+    // get_log_grid, src/relutility.c:399-405
+    for (int ii = 0; ii < n + 1; ii++) {
+      double e = 1.0 * ii / (n + 1 - 1) * (log(g.conv_emax) - log(g.conv_emin)) + log(g.conv_emin);
+      g.conv_ener[ii] = exp(e);
+    }
+    g.conv_cfac.resize(n);
+    for (int ii = 0; ii < n; ii++) {
+      // calculate_energyflux_conversion, src/Relbase.cpp:93-103
+      g.conv_cfac[ii] = 0.5 * (g.conv_ener[ii] + g.conv_ener[ii + 1]) / (g.conv_ener[ii + 1] - g.conv_ener[ii]);
+    }
+    g.conv_i1kev = bsearch_asc(g.conv_ener.data(), n + 1, 1.0);
+    g.conv_tw.resize(n);  // n/2 complex
+    for (int t = 0; t < n / 2; t++) {
+      long double ang = -2.0L * 3.141592653589793238462643383279502884L * t / n;
+      g.conv_tw[2 * t] = (double)cosl(ang);
+      g.conv_tw[2 * t + 1] = (double)sinl(ang);
+    }
+  }
+}
+
+static int upload(DevBuf &d, const void *src, size_t bytes) {
+  int e = d.ensure(bytes);
+  if (e) return e;
+  if (bytes) e = (int)hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice);
+  return e;
+}
+
+template <class T>
+static int upload_vec(DevBuf &d, const std::vector<T> &v) {
+  return upload(d, v.data(), v.size() * sizeof(T));
+}
+
+int ensure_device(std::string &err) {
+  Global &g = G();
+  ensure_tables_host();
+  if (!g.device_ready) {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= g.device) {
+      err = "no HIP device available";
+      return RX_ERR_DEVICE;
+    }
+    e = hipSetDevice(g.device);
+    if (e != hipSuccess) {
+      err = "hipSetDevice failed";
+      return RX_ERR_DEVICE;
+    }
+    g.device_ready = true;
+  } else {
+    (void)hipSetDevice(g.device);
+  }
+  int e = 0;
+  if (g.rel_dirty) {
+    const RelTableHost &t = g.rel;
+    if (t.n_r > MAX_NR_TAB || t.n_g > MAX_NG) {
+      err = "relline table dims exceed kernel limits";
+      return RX_ERR_ARG;
+    }
+    e |= upload_vec(g.t_a, t.a) | upload_vec(g.t_mu0, t.mu0) | upload_vec(g.t_r, t.r) |
+         upload_vec(g.t_gmin, t.gmin) | upload_vec(g.t_gmax, t.gmax) | upload_vec(g.t_trff1, t.trff1) |
+         upload_vec(g.t_trff2, t.trff2) | upload_vec(g.t_cosne1, t.cosne1) | upload_vec(g.t_cosne2, t.cosne2);
+    g.rel_dirty = false;
+  }
+  if (g.lp_dirty) {
+    const LpTableHost &t = g.lp;
+    if (t.n_r > MAX_LP_R) {
+      err = "lamp-post table dims exceed kernel limits";
+      return RX_ERR_ARG;
+    }
+    e |= upload_vec(g.l_a, t.a) | upload_vec(g.l_h, t.h) | upload_vec(g.l_rad, t.rad) |
+         upload_vec(g.l_int, t.intens) | upload_vec(g.l_del, t.del) | upload_vec(g.l_inc, t.del_inc);
+    g.lp_dirty = false;
+  }
+  if (g.conv_dirty) {
+    e |= upload_vec(g.c_ener, g.conv_ener) | upload_vec(g.c_cfac, g.conv_cfac) | upload_vec(g.c_tw, g.conv_tw);
+    g.conv_dirty = false;
+  }
+  if (g.refl_dirty) {
+    e |= upload_vec(g.d_refl_sets, g.refl_sets);
+    g.refl_dirty = false;
+  }
+  if (e) {
+    err = "device upload failed";
+    return RX_ERR_DEVICE;
+  }
+  return RX_OK;
+}
+
+void fill_dev_views(BatchDev &bd) {
+  Global &g = G();
+  bd.rt.n_a = g.rel.n_a;
+  bd.rt.n_mu0 = g.rel.n_mu0;
+  bd.rt.n_r = g.rel.n_r;
+  bd.rt.n_g = g.rel.n_g;
+  bd.rt.a = g.t_a.as<float>();
+  bd.rt.mu0 = g.t_mu0.as<float>();
+  bd.rt.r = g.t_r.as<float>();
+  bd.rt.gmin = g.t_gmin.as<float>();
+  bd.rt.gmax = g.t_gmax.as<float>();
+  bd.rt.trff1 = g.t_trff1.as<float>();
+  bd.rt.trff2 = g.t_trff2.as<float>();
+  bd.rt.cosne1 = g.t_cosne1.as<float>();
+  bd.rt.cosne2 = g.t_cosne2.as<float>();
+  bd.lp.n_a = g.lp.n_a;
+  bd.lp.n_h = g.lp.n_h;
+  bd.lp.n_r = g.lp.n_r;
+  bd.lp.a = g.l_a.as<float>();
+  bd.lp.h = g.l_h.as<float>();
+  bd.lp.rad = g.l_rad.as<float>();
+  bd.lp.intens = g.l_int.as<float>();
+  bd.lp.del = g.l_del.as<float>();
+  bd.lp.del_inc = g.l_inc.as<float>();
+  bd.cg.n = g.conv_n;
+  bd.cg.i1kev = g.conv_i1kev;
+  bd.cg.ener = g.c_ener.as<double>();
+  bd.cg.cfac = g.c_cfac.as<double>();
+  bd.cg.tw = g.c_tw.as<double>();
+  bd.refl_sets = g.d_refl_sets.as<DevReflSet>();
+}
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+
+Batch::~Batch() {
+  for (auto &e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+static bool is_conv(int model) {
+  return model == MOD_RELCONV || model == MOD_RELCONVLP || model == MOD_RELXILL_CONV;
+}
+
+int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
+  Global &g = G();
+  if (n_req <= 0 || !req) {
+    err = "empty batch";
+    return RX_ERR_ARG;
+  }
+  const bool phys = phys_norm_env();
+  b.n_req = n_req;
+  b.h_req.assign(n_req, ReqParams());
+  b.h_status.assign(n_req, 0);
+  b.h_conv.clear();
+  int64_t o_ener = 0, o_flux = 0, o_pgrid = 0, o_prof = 0;
+  for (int i = 0; i < n_req; i++) {
+    const rx_request &r = req[i];
+    if (r.n_flux <= 0 || !r.energy || !r.parameter) {
+      err = "request with empty grid or missing parameters";
+      return RX_ERR_ARG;
+    }
+    ReqParams &p = b.h_req[i];
+    std::string perr;
+    int st = parse_request(r.model, r.parameter, phys, p, perr);
+    if (st == RX_ERR_ARG) {
+      err = perr;
+      return st;
+    }
+    p.n_in = r.n_flux;
+    p.ener_off = o_ener;
+    o_ener += r.n_flux + 1;
+    p.flux_off = o_flux;
+    o_flux += r.n_flux;
+    p.refl_set = r.refl_set;
+    if (is_conv(r.model)) {
+      p.n_e = g.conv_n;
+      p.pgrid_off = -1;
+      p.prof_off = o_prof;
+      o_prof += g.conv_n;
+      b.h_conv.push_back(i);
+      if (st == RX_OK && r.model != MOD_RELXILL_CONV) {
+        if (!r.flux_in) {
+          err = "convolution request without input spectrum";
+          return RX_ERR_ARG;
+        }
+        double sum = 0.0;  // src/LocalModel.cpp:84-86
+        for (int k = 0; k < r.n_flux; k++) sum += r.flux_in[k];
+        if (!(sum > 0.0)) st = RX_ERR_INPUT;
+      }
+      if (st == RX_OK && r.model == MOD_RELXILL_CONV) {
+        if (r.refl_set < 0 || r.refl_set >= (int)g.refl_sets.size()) {
+          err = "unknown reflection set";
+          return RX_ERR_ARG;
+        }
+        p.n_incl = g.refl_sets[r.refl_set].n_incl;
+      }
+    } else {
+      p.n_e = r.n_flux;
+      p.pgrid_off = o_pgrid;
+      o_pgrid += r.n_flux + 1;
+    }
+    p.need_cosne = (p.limb != 0 || p.n_incl > 0) ? 1 : 0;
+    p.pre_status = st;
+    b.h_status[i] = st;
+  }
+  b.tot_ener = o_ener;
+  b.tot_flux = o_flux;
+  b.tot_pgrid = o_pgrid;
+  b.tot_prof = o_prof;
+  b.h_ener.resize(o_ener);
+  b.h_flux_in.assign(o_flux, 0.0);
+  for (int i = 0; i < n_req; i++) {
+    const rx_request &r = req[i];
+    const ReqParams &p = b.h_req[i];
+    memcpy(&b.h_ener[p.ener_off], r.energy, sizeof(double) * (r.n_flux + 1));
+    if (r.flux_in) memcpy(&b.h_flux_in[p.flux_off], r.flux_in, sizeof(double) * r.n_flux);
+  }
+  b.n_tiles_base.assign(n_req, 0);
+  for (int i = 0; i < n_req; i++) b.n_tiles_base[i] = (b.h_req[i].n_e + 63) / 64;
+  // shape class for the autotuner: log2 of the total bin chunks, conv flag
+  int64_t chunks = 0;
+  for (int i = 0; i < n_req; i++) chunks += b.n_tiles_base[i];
+  int lg = 0;
+  while ((1LL << lg) < chunks) lg++;
+  b.shape_class = lg * 2 + (b.h_conv.empty() ? 0 : 1);
+  batch_plan(b, b.n_rch_forced);
+  return batch_upload(b);
+}
+
+void batch_plan(Batch &b, int n_rch) {
+  Global &g = G();
+  int64_t chunks = 0;
+  for (int i = 0; i < b.n_req; i++)
+    if (b.h_status[i] == RX_OK) chunks += b.n_tiles_base[i];
+  if (n_rch <= 0) {
+    for (auto &t : g.tune)
+      if (t.first == b.shape_class) n_rch = t.second;
+  }
+  if (n_rch <= 0) {
+    // enough one-wave tiles to keep 256 CUs busy: ~16k waves
+    const int64_t target = 16384;
+    n_rch = (int)std::min<int64_t>(32, std::max<int64_t>(1, (target + chunks - 1) / std::max<int64_t>(1, chunks)));
+  }
+  b.n_rch = n_rch;
+  // tiles, ordered by a cost proxy so that heavy tiles start first: bins near
+  // and above the line energy (Romberg region, many contributing radii)
+  struct TC {
+    Tile t;
+    float cost;
+  };
+  std::vector<TC> tc;
+  tc.reserve((size_t)chunks * n_rch);
+  for (int i = 0; i < b.n_req; i++) {
+    if (b.h_status[i] != RX_OK) continue;
+    const ReqParams &p = b.h_req[i];
+    const double *e = nullptr;
+    std::vector<double> tmp;
+    if (p.pgrid_off >= 0) {
+      tmp.resize(p.n_in + 1);
+      for (int k = 0; k <= p.n_in; k++) {
+        double x = b.h_ener[p.ener_off + k];
+        if (p.z > 0) x *= (1 + p.z);
+        tmp[k] = x / p.lineE;
+      }
+      e = tmp.data();
+    } else {
+      e = g.conv_ener.data();
+    }
+    for (int c = 0; c < b.n_tiles_base[i]; c++) {
+      const int j0 = c * 64, j1 = std::min(p.n_e, j0 + 64);
+      const double lo = std::max(e[j0], 0.05), hi = std::min(e[j1], 1.6);
+      float cost = 0.01f;
+      if (hi > lo) cost += (float)(log(hi / lo) * (e[j1] > 0.95 ? 4.0 : 1.0) / log(e[j1] / e[j0]));
+      for (int r = 0; r < n_rch; r++) tc.push_back({{i, c, r, 0}, cost});
+    }
+  }
+  std::stable_sort(tc.begin(), tc.end(), [](const TC &x, const TC &y) { return x.cost > y.cost; });
+  b.h_tiles.resize(tc.size());
+  for (size_t k = 0; k < tc.size(); k++) b.h_tiles[k] = tc[k].t;
+  int64_t o_part = 0;
+  for (int i = 0; i < b.n_req; i++) {
+    b.h_req[i].part_off = o_part;
+    o_part += (int64_t)n_rch * b.h_req[i].n_e;
+  }
+  b.tot_part = o_part;
+}
+
+int batch_upload(Batch &b) {
+  Global &g = G();
+  int e = 0;
+  if (!b.stream) {
+    e |= (int)hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking);
+    for (auto &x : b.ev) e |= (int)hipEventCreate(&x);
+  }
+  const size_t nreq = b.n_req;
+  e |= b.d_req.ensure(sizeof(ReqParams) * nreq);
+  e |= b.d_ener.ensure(sizeof(double) * b.tot_ener);
+  e |= b.d_flux_in.ensure(sizeof(double) * b.tot_flux);
+  e |= b.d_out.ensure(sizeof(double) * b.tot_flux);
+  e |= b.d_pgrid.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_pgrid));
+  e |= b.d_node.ensure(sizeof(double) * nreq * 2 * (size_t)g.rel.n_r * g.rel.n_g * 2);
+  e |= b.d_rad.ensure(sizeof(RadRec) * nreq * g.nr_fine);
+  e |= b.d_part.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_part));
+  e |= b.d_prof.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_prof));
+  e |= b.d_dist.ensure(sizeof(double) * nreq * MAX_INCL);
+  e |= b.d_refl.ensure(sizeof(LpRefl) * nreq);
+  e |= b.d_status.ensure(sizeof(int) * nreq);
+  e |= b.d_tiles.ensure(sizeof(Tile) * std::max<size_t>(1, b.h_tiles.size()));
+  e |= b.d_conv.ensure(sizeof(int) * std::max<size_t>(1, b.h_conv.size()));
+  if (e) return RX_ERR_NOMEM;
+  e |= (int)hipMemcpyAsync(b.d_req.p, b.h_req.data(), sizeof(ReqParams) * nreq, hipMemcpyHostToDevice, b.stream);
+  e |= (int)hipMemcpyAsync(b.d_ener.p, b.h_ener.data(), sizeof(double) * b.tot_ener, hipMemcpyHostToDevice, b.stream);
+  e |= (int)hipMemcpyAsync(b.d_flux_in.p, b.h_flux_in.data(), sizeof(double) * b.tot_flux, hipMemcpyHostToDevice, b.stream);
+  e |= (int)hipMemcpyAsync(b.d_status.p, b.h_status.data(), sizeof(int) * nreq, hipMemcpyHostToDevice, b.stream);
+  if (!b.h_tiles.empty())
+    e |= (int)hipMemcpyAsync(b.d_tiles.p, b.h_tiles.data(), sizeof(Tile) * b.h_tiles.size(), hipMemcpyHostToDevice, b.stream);
+  if (!b.h_conv.empty())
+    e |= (int)hipMemcpyAsync(b.d_conv.p, b.h_conv.data(), sizeof(int) * b.h_conv.size(), hipMemcpyHostToDevice, b.stream);
+  e |= (int)hipStreamSynchronize(b.stream);
+  return e ? RX_ERR_DEVICE : RX_OK;
+}
+
+static BatchDev dev_view(Batch &b) {
+  Global &g = G();
+  BatchDev bd;
+  memset(&bd, 0, sizeof(bd));
+  bd.n_req = b.n_req;
+  bd.nr_fine = g.nr_fine;
+  bd.n_rch = b.n_rch;
+  bd.node_stride = 2 * g.rel.n_r * g.rel.n_g * 2;
+  bd.req = b.d_req.as<ReqParams>();
+  bd.ener_in = b.d_ener.as<double>();
+  bd.flux_in = b.d_flux_in.as<double>();
+  bd.out = b.d_out.as<double>();
+  bd.pgrid = b.d_pgrid.as<double>();
+  bd.node = b.d_node.as<double>();
+  bd.rad = b.d_rad.as<RadRec>();
+  bd.part = b.d_part.as<double>();
+  bd.prof = b.d_prof.as<double>();
+  bd.dist = b.d_dist.as<double>();
+  bd.refl = b.d_refl.as<LpRefl>();
+  bd.status = b.d_status.as<int>();
+  fill_dev_views(bd);
+  return bd;
+}
+
+int batch_run(Batch &b, int precision, std::string &err) {
+  Global &g = G();
+  BatchDev bd = dev_view(b);
+  hipStream_t s = b.stream;
+  const bool prof = g.profile;
+  int e = 0;
+  if (prof) e |= (int)hipEventRecord(b.ev[0], s);
+  e |= launch_setup(bd, s);
+  if (prof) e |= (int)hipEventRecord(b.ev[1], s);
+  e |= launch_integrate(bd, b.d_tiles.as<Tile>(), (int)b.h_tiles.size(), precision == RX_FP32 ? 1 : 0, s);
+  if (prof) e |= (int)hipEventRecord(b.ev[2], s);
+  e |= launch_finalize(bd, s);
+  if (prof) e |= (int)hipEventRecord(b.ev[3], s);
+  e |= launch_conv(bd, b.d_conv.as<int>(), (int)b.h_conv.size(), s);
+  if (prof) e |= (int)hipEventRecord(b.ev[4], s);
+  b.profiled = prof;
+  if (e) {
+    err = "kernel launch failed";
+    return RX_ERR_DEVICE;
+  }
+  return RX_OK;
+}
+
+int batch_sync(Batch &b) {
+  if (!b.stream) return RX_OK;
+  if (hipStreamSynchronize(b.stream) != hipSuccess) return RX_ERR_DEVICE;
+  if (b.profiled) {
+    for (int k = 0; k < 4; k++) {
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, b.ev[k], b.ev[k + 1]);
+      b.kernel_ms[k] = ms;
+    }
+  }
+  return RX_OK;
+}
+
+}  // namespace rx

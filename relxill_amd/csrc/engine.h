@@ -1,0 +1,96 @@
+// engine.h -- host runtime of librelxill_gpu (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/relxill_gpu.h"
+#include "rx_engine.h"
+#include "rx_tables.h"
+
+namespace rx {
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf();
+  int ensure(size_t n);  // grows (never shrinks); returns hipError_t
+  template <class T>
+  T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+// Host mirror of get_rel_params + check_parameter_bounds
+// (src/ModelDefinition.cpp:153-334) for one request.  Returns RX_OK or a status.
+int parse_request(int model, const double *par, bool phys_norm, ReqParams &p, std::string &err);
+int n_model_params(int model);
+
+struct Batch {
+  int n_req = 0;
+  int n_rch = 1;
+  int n_rch_forced = 0;
+  std::vector<ReqParams> h_req;
+  std::vector<int> h_status;  // host-side parse status
+  std::vector<double> h_ener, h_flux_in;
+  std::vector<Tile> h_tiles;
+  std::vector<int> h_conv;
+  std::vector<int> n_tiles_base;  // per request bin chunks
+  int64_t tot_ener = 0, tot_flux = 0, tot_pgrid = 0, tot_part = 0, tot_prof = 0;
+  DevBuf d_req, d_ener, d_flux_in, d_out, d_pgrid, d_node, d_rad, d_part, d_prof, d_dist,
+      d_refl, d_status, d_tiles, d_conv;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  bool profiled = false;
+  double kernel_ms[4] = {0, 0, 0, 0};
+  int shape_class = 0;
+  ~Batch();
+};
+
+// Fills b (re-using device buffers when large enough) from the requests.
+int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err);
+// Plans tiles for n_rch radius chunks (0: automatic).
+void batch_plan(Batch &b, int n_rch);
+int batch_upload(Batch &b);
+int batch_run(Batch &b, int precision, std::string &err);
+int batch_sync(Batch &b);
+
+// global state
+struct Global {
+  std::recursive_mutex mu;
+  bool device_ready = false;
+  int device = 0;
+  RelTableHost rel;
+  LpTableHost lp;
+  bool rel_loaded = false, lp_loaded = false;
+  bool rel_dirty = true, lp_dirty = true;
+  DevBuf t_a, t_mu0, t_r, t_gmin, t_gmax, t_trff1, t_trff2, t_cosne1, t_cosne2;
+  DevBuf l_a, l_h, l_rad, l_int, l_del, l_inc;
+  // convolution grid
+  int conv_n = 4096;
+  double conv_emin = 0.00035, conv_emax = 2000.0;
+  std::vector<double> conv_ener, conv_cfac, conv_tw;
+  int conv_i1kev = 0;
+  bool conv_dirty = true;
+  DevBuf c_ener, c_cfac, c_tw;
+  int nr_fine = 1000;
+  // reflection sets
+  std::vector<DevReflSet> refl_sets;
+  std::vector<DevBuf *> refl_bufs;
+  DevBuf d_refl_sets;
+  bool refl_dirty = false;
+  bool profile = false;
+  // autotune cache: shape class -> n_rch
+  std::vector<std::pair<int, int>> tune;
+};
+
+Global &G();
+void ensure_tables_host();
+int ensure_device(std::string &err);  // device + uploads (tables, conv grid, refl sets)
+void fill_dev_views(BatchDev &bd);
+void set_last(int status, const std::string &msg);
+int last_status();
+const char *last_error();
+bool phys_norm_env();
+
+}  // namespace rx

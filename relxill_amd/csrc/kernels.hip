@@ -1,0 +1,799 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of the relativistic smearing path.
+//
+//   k_setup      one workgroup per request: (a, mu0) bilinear table
+//                interpolation (interpol_relTable, src/Relprofile.cpp:141-303),
+//                fine radial grid, emissivity (src/Rellp.cpp:518-564), energy
+//                windows, angle distribution (src/Relprofile.cpp:908-938)
+//   k_integrate  one 64-lane wavefront per tile (request, 64-bin chunk,
+//                radius chunk): every lane owns one energy bin and sums the
+//                bin integral over the tile's radii (calc_relline_profile,
+//                src/Relprofile.cpp:857-900); the per-radius transfer
+//                function is lerped from the staged node into LDS
+//   k_finalize   one workgroup per request: radius-chunk reduction, photons
+//                per bin and renormalisation (src/Relprofile.cpp:749-796)
+//   k_conv       one workgroup per convolution request: rebin onto the
+//                convolution grid, real FFT convolution held in LDS with the
+//                1 keV rotation and E/dE weighting, band normalisation, rebin
+//                back and zeroing outside 0.01-1000 keV
+//                (src/Relbase.cpp:93-289)
+#include <hip/hip_runtime.h>
+
+#include "rx_engine.h"
+#include "rx_math.h"
+
+namespace rx {
+
+namespace {
+
+constexpr int SETUP_THREADS = 256;
+constexpr int CONV_THREADS = 256;
+
+__device__ inline double lp_ifac(float value, const float *arr, int n, int &ind) {
+  // get_ipol_factor: float search and float quotient (src/relutility.c:419-423)
+  ind = bsearch_asc(arr, n, value);
+  return (double)((value - arr[ind]) / (arr[ind + 1] - arr[ind]));
+}
+
+struct LpFine {
+  double emis, del_emit, del_inc;
+  int err;
+};
+
+// rebin_emisprofile_on_radial_grid for one fine radius (src/Rellp.cpp:120-175):
+// the table walk reduces to kk = clamp(#{jet_rad <= r} - 1, 0, n-2).
+__device__ inline LpFine lp_on_fine(double re, const double *jr, const double *je,
+                                    const double *jd, const double *ji, int ntr) {
+  LpFine o;
+  o.err = 0;
+  int lo = 0, hi = ntr;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (jr[mid] <= re) lo = mid + 1; else hi = mid;
+  }
+  int kk = lo - 1;
+  if (kk < 0) kk = 0;
+  if (kk >= ntr - 1) {
+    if (!(re - 1000.0 <= 1e-6)) o.err = 22;
+    kk = ntr - 2;
+  }
+  double inter_r;
+  if (jd[kk] / PI * 180.0 <= 75.0) {  // get_ipol_factor_radius, src/relutility.c:407-417
+    inter_r = (re - jr[kk]) / (jr[kk + 1] - jr[kk]);
+  } else {
+    inter_r = (log(re) - log(jr[kk])) / (log(jr[kk + 1]) - log(jr[kk]));
+  }
+  o.emis = exp(inter_r * log(je[kk + 1]) + (1.0 - inter_r) * log(je[kk]));
+  o.del_emit = lerp1(inter_r, jd[kk], jd[kk + 1]);
+  o.del_inc = lerp1(inter_r, ji[kk], ji[kk + 1]);
+  return o;
+}
+
+// g = E/E_i for the lamp post (src/Relphysics.cpp:163-207)
+__device__ inline double gi_potential_lp(double r, double a, double h, double bet, double del) {
+  double ut_d = ((r * sqrt(r) + a) / (sqrt(r) * sqrt(r * r - 3 * r + 2 * a * sqrt(r))));
+  double ut_h = sqrt((h * h + a * a) / (h * h - 2 * h + a * a));
+  double gi = ut_d / ut_h;
+  if (fabs(bet) < 1e-6) return gi;
+  double gam = 1.0 / sqrt(1.0 - bet * bet);
+  double sgn = (del > PI / 2) ? -1.0 : 1.0;
+  double delta_eq = h * h - 2 * h + a * a;
+  double sd = sin(del);
+  double hh = h * h + a * a;
+  double q2 = (sd * sd) * (hh * hh / delta_eq) - a * a;
+  double beta_fac = sqrt(hh * hh - delta_eq * (q2 + a * a));
+  beta_fac = gam * (1.0 + sgn * beta_fac / hh * bet);
+  return gi / beta_fac;
+}
+
+__device__ inline double relat_abberation(double del, double beta) {
+  return acos((cos(del) - beta) / (1 - beta * cos(del)));
+}
+
+// block-wide deterministic sum (fixed tree order)
+__device__ inline double block_sum(double v, double *red) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (tid < s) red[tid] += red[tid + s];
+    __syncthreads();
+  }
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// setup
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b) {
+  const int req = blockIdx.x;
+  const int tid = threadIdx.x;
+  const ReqParams rq = b.req[req];
+  const DevRelTable &T = b.rt;
+  const int nr = b.nr_fine;
+  const int n_r = T.n_r, ng = T.n_g;
+  if (rq.pre_status != 0) return;  // status already set by the host
+
+  __shared__ double s_tre[MAX_NR_TAB], s_tgmin[MAX_NR_TAB], s_tgmax[MAX_NR_TAB];
+  __shared__ double s_jr[MAX_LP_R], s_je[MAX_LP_R], s_jd[MAX_LP_R], s_ji[MAX_LP_R];
+  __shared__ double s_red[SETUP_THREADS];
+  __shared__ double s_dist[MAX_INCL];
+  __shared__ int s_st, s_ia, s_im, s_rmin, s_lia, s_lih0, s_lih1;
+  __shared__ double s_fa, s_fm, s_rms, s_lfa, s_lfh0, s_lfh1;
+
+  // profile grid: caller grid shifted to line-energy units (XspecSpectrum.h:61-76)
+  const double *E;
+  if (rq.pgrid_off >= 0) {
+    double *pg = b.pgrid + rq.pgrid_off;
+    const double *ein = b.ener_in + rq.ener_off;
+    for (int k = tid; k <= rq.n_in; k += SETUP_THREADS) {
+      double e = ein[k];
+      if (rq.z > 0) e *= (1 + rq.z);
+      pg[k] = e / rq.lineE;
+    }
+    E = pg;
+  } else {
+    E = b.cg.ener;
+  }
+
+  if (tid == 0) {
+    int st = 0;
+    const double rms = kerr_rms_hd(rq.a);
+    if (!(rq.rout > rq.rin) || !(rq.rin >= rms)) st = 10;
+    const double mu0 = cos(rq.incl);
+    const int ia = bsearch_asc(T.a, T.n_a, (float)rq.a);
+    const int im = bsearch_asc(T.mu0, T.n_mu0, (float)mu0);
+    const float fa = ((float)rq.a - T.a[ia]) / (T.a[ia + 1] - T.a[ia]);
+    const float fm = ((float)mu0 - T.mu0[im]) / (T.mu0[im + 1] - T.mu0[im]);
+    if (!(fm >= 0 && fm <= 1 && fa >= 0 && fa <= 1)) st = 11;
+    s_st = st;
+    s_ia = ia;
+    s_im = im;
+    s_fa = fa;
+    s_fm = fm;
+    s_rms = rms;
+  }
+  __syncthreads();
+  if (s_st) {
+    if (tid == 0) b.status[req] = s_st;
+    return;
+  }
+  const int ia = s_ia, im = s_im;
+  const double fa = s_fa, fm = s_fm;
+  const size_t n00 = (size_t)ia * T.n_mu0 + im, n10 = (size_t)(ia + 1) * T.n_mu0 + im;
+  const size_t n01 = (size_t)ia * T.n_mu0 + im + 1, n11 = (size_t)(ia + 1) * T.n_mu0 + im + 1;
+
+  // node radii: lerp in a only (src/Relprofile.cpp:190-193)
+  for (int ir = tid; ir < n_r; ir += SETUP_THREADS) {
+    s_tre[ir] = lerp1(fa, (double)T.r[n00 * n_r + ir], (double)T.r[n10 * n_r + ir]);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int st = 0;
+    const double rms = s_rms;
+    // ISCO snap (:195-200)
+    if ((s_tre[n_r - 1] > rms) && ((s_tre[n_r - 1] - rms) / s_tre[n_r - 1] < 1e-3)) {
+      s_tre[n_r - 1] = rms;
+    }
+    const int ind_rmin = bsearch_desc(s_tre, n_r, rq.rin);
+    const int ind_rmax = bsearch_desc(s_tre, n_r, rq.rout);
+    // 1000 rg snap (:235-238)
+    if (s_tre[ind_rmax] < 1000.0 && s_tre[ind_rmax] * 1.01 > 1000.0) s_tre[ind_rmax] = 1000.0;
+    if (!(ind_rmin > 0) || !(s_tre[ind_rmin + 1] <= rq.rin) || !(s_tre[ind_rmin] >= rq.rin) ||
+        !(s_tre[ind_rmax + 1] <= rq.rout) || !(s_tre[ind_rmax] >= rq.rout) ||
+        !(ind_rmax <= ind_rmin) || !(rq.rout <= 1000.0)) {
+      st = 12;
+    }
+    s_rmin = ind_rmin;
+    s_st = st;
+  }
+  // node values: bilinear in (a, mu0) (interpol_a_mu0, :39-80)
+  for (int ir = tid; ir < n_r; ir += SETUP_THREADS) {
+    s_tgmin[ir] = lerp2f(fa, fm, T.gmin[n00 * n_r + ir], T.gmin[n10 * n_r + ir],
+                         T.gmin[n01 * n_r + ir], T.gmin[n11 * n_r + ir]);
+    s_tgmax[ir] = lerp2f(fa, fm, T.gmax[n00 * n_r + ir], T.gmax[n10 * n_r + ir],
+                         T.gmax[n01 * n_r + ir], T.gmax[n11 * n_r + ir]);
+  }
+  {
+    double *node = b.node + (size_t)req * b.node_stride;
+    const size_t nq = (size_t)n_r * ng;
+    const size_t s00 = n00 * nq, s10 = n10 * nq, s01 = n01 * nq, s11 = n11 * nq;
+    for (size_t q = tid; q < nq; q += SETUP_THREADS) {
+      node[q * 2 + 0] = lerp2f(fa, fm, T.trff1[s00 + q], T.trff1[s10 + q], T.trff1[s01 + q], T.trff1[s11 + q]);
+      node[q * 2 + 1] = lerp2f(fa, fm, T.trff2[s00 + q], T.trff2[s10 + q], T.trff2[s01 + q], T.trff2[s11 + q]);
+      if (rq.need_cosne) {
+        node[nq * 2 + q * 2 + 0] = lerp2f(fa, fm, T.cosne1[s00 + q], T.cosne1[s10 + q], T.cosne1[s01 + q], T.cosne1[s11 + q]);
+        node[nq * 2 + q * 2 + 1] = lerp2f(fa, fm, T.cosne2[s00 + q], T.cosne2[s10 + q], T.cosne2[s01 + q], T.cosne2[s11 + q]);
+      }
+    }
+  }
+  // lamp-post table node (interpol_lptable, src/Rellp.cpp:219-247)
+  const bool lp = (rq.emis_type == EMIS_LP);
+  const DevLpTable &L = b.lp;
+  if (lp && tid == 0) {
+    int lia;
+    const double lfa = lp_ifac((float)rq.a, L.a, L.n_a, lia);
+    int h0, h1;
+    const double fh0 = lp_ifac((float)rq.height, L.h + (size_t)lia * L.n_h, L.n_h, h0);
+    const double fh1 = lp_ifac((float)rq.height, L.h + (size_t)(lia + 1) * L.n_h, L.n_h, h1);
+    s_lia = lia;
+    s_lfa = lfa;
+    s_lih0 = h0;
+    s_lih1 = h1;
+    s_lfh0 = fh0;
+    s_lfh1 = fh1;
+  }
+  __syncthreads();
+  if (s_st) {
+    if (tid == 0) b.status[req] = s_st;
+    return;
+  }
+  const int ntr = L.n_r;
+  if (lp) {
+    const int lia = s_lia, h0 = s_lih0, h1 = s_lih1;
+    const double lfa = s_lfa, fh0 = s_lfh0, fh1 = s_lfh1;
+    for (int ii = tid; ii < ntr; ii += SETUP_THREADS) {
+      s_jr[ii] = lerp1(lfa, (double)L.rad[(size_t)lia * ntr + ii], (double)L.rad[(size_t)(lia + 1) * ntr + ii]);
+      const size_t l0 = ((size_t)lia * L.n_h + h0) * ntr + ii;
+      const size_t u0 = ((size_t)lia * L.n_h + h0 + 1) * ntr + ii;
+      const size_t l1 = ((size_t)(lia + 1) * L.n_h + h1) * ntr + ii;
+      const size_t u1 = ((size_t)(lia + 1) * L.n_h + h1 + 1) * ntr + ii;
+      s_je[ii] = (1.0 - lfa) * lerp1(fh0, L.intens[l0], L.intens[u0]) + (lfa)*lerp1(fh1, L.intens[l1], L.intens[u1]);
+      s_jd[ii] = (1.0 - lfa) * lerp1(fh0, L.del[l0], L.del[u0]) + (lfa)*lerp1(fh1, L.del[l1], L.del[u1]);
+      s_ji[ii] = (1.0 - lfa) * lerp1(fh0, L.del_inc[l0], L.del_inc[u0]) + (lfa)*lerp1(fh1, L.del_inc[l1], L.del_inc[u1]);
+    }
+    __syncthreads();
+  }
+
+  // fine radii: node -> fine map (:249-300), emissivity, energy window (:859-878)
+  RadRec *rad = b.rad + (size_t)req * nr;
+  double area = 0.0;
+  int err = 0;
+  for (int ii = tid; ii < nr; ii += SETUP_THREADS) {
+    const double re = fine_radius(rq.rin, rq.rout, ii, nr);
+    const double rprev = (ii > 0) ? fine_radius(rq.rin, rq.rout, ii - 1, nr) : 0.0;
+    const double rnext = (ii < nr - 1) ? fine_radius(rq.rin, rq.rout, ii + 1, nr) : 0.0;
+    // K = largest k with tre[k] > re (the reference's downward walk)
+    int lo = 0, hi = n_r;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (s_tre[mid] > re) lo = mid + 1; else hi = mid;
+    }
+    int K = lo - 1;
+    if (K < 0) {
+      if (re - 1000.0 <= 1e-6) K = 0; else err = 13;
+    }
+    if (K >= n_r - 1) {  // cannot happen for rin >= tre[n_r-1]; keep in bounds
+      K = n_r - 2;
+      err = 13;
+    }
+    const double ifac_r = (re - s_tre[K + 1]) / (s_tre[K] - s_tre[K + 1]);
+    if (ifac_r > 1.0 && K > 0) err = 14;
+    const double gmin = lerp1(ifac_r, s_tgmin[K + 1], s_tgmin[K]);
+    const double gmax = lerp1(ifac_r, s_tgmax[K + 1], s_tgmax[K]);
+    const double tz = trapez_half(rprev, re, rnext, ii, nr);
+    const double weight = tz / 2;
+    double emis;
+    if (!lp) {  // broken power law (get_emis_bkn, src/Rellp.cpp:421-437)
+      const double alpha = (re > rq.rbr) ? rq.emis2 : rq.emis1;
+      emis = pow(re / rq.rbr, -alpha);
+      area += emis * (tz * 2);
+    } else {  // lamp post + flux boost (src/Rellp.cpp:249-253, src/Relphysics.cpp:301-311)
+      LpFine f = lp_on_fine(re, s_jr, s_je, s_jd, s_ji, ntr);
+      if (f.err) err = f.err;
+      double boost = pow(gi_potential_lp(re, rq.a, rq.height, rq.beta, f.del_emit), rq.gamma);
+      if (rq.beta > 1e-6) {
+        const double d = sqrt(1.0 - rq.beta * rq.beta) / (1.0 + rq.beta * cos(f.del_emit));
+        boost *= d * d;
+      }
+      emis = f.emis * boost;
+    }
+    // energy window
+    double egmin = gmin, egmax = gmax;
+    int ielo = 1, iehi = 0;
+    if ((egmax > E[0]) && (egmin < E[rq.n_e])) {
+      if (egmin < E[0]) egmin = E[0];
+      if (egmax > E[rq.n_e]) egmax = E[rq.n_e];
+      ielo = bsearch_asc(E, rq.n_e + 1, egmin);
+      iehi = bsearch_asc(E, rq.n_e + 1, egmax);
+    }
+    RadRec r;
+    r.gmin = gmin;
+    r.gmax = gmax;
+    r.del_g = 1. / (gmax - gmin);
+    r.coef = weight;   // scaled below once the emissivity norm is known
+    r.ifac_r = ifac_r;
+    r.emisw = emis;
+    r.K = K;
+    r.ielo = ielo;
+    r.iehi = iehi;
+    r.pad = 0;
+    rad[ii] = r;
+  }
+  if (err) atomicMax(&s_st, err);
+  // BKN normalisation: sum emis * 2 pi r dr (norm_emis_profile, src/Rellp.cpp:91-109)
+  const double total = block_sum(area, s_red);
+  if (s_st) {
+    if (tid == 0) b.status[req] = s_st;
+    return;
+  }
+  const double inv_area_den = lp ? 1.0 : total;
+  for (int ii = tid; ii < nr; ii += SETUP_THREADS) {
+    RadRec r = rad[ii];
+    const double emis = r.emisw / inv_area_den;
+    const double weight = r.coef;
+    r.emisw = emis * weight;
+    r.coef = emis / (r.gmax - r.gmin) * weight;
+    rad[ii] = r;
+  }
+
+  // reflection fractions (calc_refl_frac, src/Rellp.cpp:36-89)
+  if (lp && tid == 0) {
+    auto fr = [&](int ii) { return fine_radius(rq.rin, rq.rout, ii, nr); };
+    int klo = 0, khi = nr - 1;
+    while (khi - klo > 1) {
+      int k = (khi + klo) / 2;
+      if (fr(k) < rq.rin) khi = k; else klo = k;
+    }
+    const int ibh = klo;
+    klo = 0;
+    khi = nr - 1;
+    while (khi - klo > 1) {
+      int k = (khi + klo) / 2;
+      if (fr(k) < rq.rout) khi = k; else klo = k;
+    }
+    const int iad = klo;
+    double del_bh = lp_on_fine(fr(ibh), s_jr, s_je, s_jd, s_ji, ntr).del_emit;
+    double del_ad = lp_on_fine(fr(iad), s_jr, s_je, s_jd, s_ji, ntr).del_emit;
+    double dmax = s_jd[ntr - 1];
+    if (dmax < PI / 2.0) dmax = PI / 2.0;
+    if (rq.beta > 1e-6) {
+      del_bh = relat_abberation(del_bh, -1. * rq.beta);
+      del_ad = relat_abberation(del_ad, -1. * rq.beta);
+    }
+    LpRefl o;
+    o.f_bh = 0.5 * (1.0 - cos(del_bh));
+    o.f_ad = 0.5 * (cos(del_bh) - cos(del_ad));
+    o.f_inf_rest = 0.5 * (1.0 + cos(dmax));
+    o.f_inf = (rq.beta > 1e-6) ? 0.5 * (1.0 + cos(relat_abberation(dmax, -1. * rq.beta))) : o.f_inf_rest;
+    o.refl_frac = o.f_ad / o.f_inf;
+    if (!(o.f_inf + o.f_ad < 1.0)) s_st = 20;
+    b.refl[req] = o;
+  }
+
+  // emission-angle distribution (:908-938)
+  if (rq.n_incl > 0) {
+    for (int k = tid; k < rq.n_incl; k += SETUP_THREADS) s_dist[k] = 0.0;
+    __syncthreads();
+    const double *node = b.node + (size_t)req * b.node_stride;
+    const size_t nq2 = (size_t)n_r * ng * 2;
+    const double step = (1.0 - 2 * GFAC_H) / (ng - 1);
+    const double gs0 = gstar_node(step, 0), gs1 = gstar_node(step, 1);
+    const int items = nr * ng * 2;
+    int derr = 0;
+    for (int it = tid; it < items; it += SETUP_THREADS) {
+      const int ii = it / (ng * 2);
+      const int q = it - ii * ng * 2;
+      const int g = q >> 1;
+      const RadRec r = rad[ii];
+      if (r.ielo > r.iehi) continue;
+      const double re = fine_radius(rq.rin, rq.rout, ii, nr);
+      const double trff = lerp1(r.ifac_r, node[(size_t)(r.K + 1) * ng * 2 + q], node[(size_t)r.K * ng * 2 + q]);
+      const double cosne = lerp1(r.ifac_r, node[nq2 + (size_t)(r.K + 1) * ng * 2 + q], node[nq2 + (size_t)r.K * ng * 2 + q]);
+      const double gs = gstar_node(step, g);
+      const double gv = gs * (r.gmax - r.gmin) + r.gmin;
+      const int imu = ((int)(rq.n_incl * (1 - cosne) + 1)) - 1;
+      const double dgs = (g == 0 || g == ng - 1) ? 0.5 * (gs1 - gs0) + GFAC_H : gs1 - gs0;
+      const double pr = 2 * PI * gv * re;
+      const double tmp = re * (pr * pr) / sqrt(gs - gs * gs) * trff * r.emisw * dgs;
+      if (imu < 0 || imu >= rq.n_incl) {
+        derr = 40;
+        continue;
+      }
+      if (tmp != tmp) derr = 41;
+      atomicAdd(&s_dist[imu], tmp);
+    }
+    if (derr) atomicMax(&s_st, derr);
+    __syncthreads();
+    for (int k = tid; k < rq.n_incl; k += SETUP_THREADS) b.dist[(size_t)req * MAX_INCL + k] = s_dist[k];
+  }
+  __syncthreads();
+  if (tid == 0) b.status[req] = s_st;
+}
+
+// ---------------------------------------------------------------------------
+// integrate: the hot loop
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(64) void k_integrate(BatchDev b, const Tile *__restrict__ tiles) {
+  __shared__ double sT64[2][2 * MAX_NG];
+  __shared__ double sC64[2][2 * MAX_NG];
+  __shared__ float sT32[2][2 * MAX_NG];
+  __shared__ float sC32[2][2 * MAX_NG];
+
+  const Tile tl = tiles[blockIdx.x];
+  const int req = tl.req;
+  const int lane = threadIdx.x;
+  const ReqParams &rq = b.req[req];
+  const int n_e = rq.n_e;
+  const int j0 = tl.bchunk * 64;
+  const int j = j0 + lane;
+  double *part = b.part + rq.part_off + (size_t)tl.rch * n_e;
+  if (b.status[req] != 0) {
+    if (j < n_e) part[j] = 0.0;
+    return;
+  }
+  const int nr = b.nr_fine;
+  const int ng = b.rt.n_g;
+  const int nq = 2 * ng;
+  const int limb = rq.limb;
+  const double *E = (rq.pgrid_off >= 0) ? b.pgrid + rq.pgrid_off : b.cg.ener;
+  const bool valid = j < n_e;
+  const double elo = valid ? E[j] : 0.0;
+  const double ehi = valid ? E[j + 1] : 0.0;
+  const RadRec *rad = b.rad + (size_t)req * nr;
+  const double *node = b.node + (size_t)req * b.node_stride;
+  const double *cnode = node + (size_t)b.rt.n_r * ng * 2;
+  const int rlen = (nr + b.n_rch - 1) / b.n_rch;
+  const int i0 = tl.rch * rlen;
+  const int i1 = min(nr, i0 + rlen);
+  const int jlast = j0 + 63;
+
+  const double step = (1.0 - 2 * GFAC_H) / (ng - 1);
+  RadiusView v;
+  v.ng = ng;
+  v.limb = limb;
+  v.gs_step = step;
+  v.inv_step = 1.0 / step;
+
+  double acc = 0.0;     // FP64 mode
+  float accf = 0.0f;    // FP32 mode: Kahan sum
+  float comp = 0.0f;
+  int buf = 0;
+  for (int i = i0; i < i1; i++) {
+    const RadRec r = rad[i];
+    if (r.iehi < j0 || r.ielo > jlast || r.ielo > r.iehi) continue;
+    for (int q = lane; q < nq; q += 64) {
+      const double t = lerp1(r.ifac_r, node[(size_t)(r.K + 1) * nq + q], node[(size_t)r.K * nq + q]);
+      sT64[buf][q] = t;
+      sT32[buf][q] = (float)t;
+      if (limb) {
+        const double c = lerp1(r.ifac_r, cnode[(size_t)(r.K + 1) * nq + q], cnode[(size_t)r.K * nq + q]);
+        sC64[buf][q] = c;
+        sC32[buf][q] = (float)c;
+      }
+    }
+    __syncthreads();
+    if (valid && j >= r.ielo && j <= r.iehi) {
+      v.gmin = r.gmin;
+      v.gmax = r.gmax;
+      v.del_g = r.del_g;
+      v.T64 = sT64[buf];
+      v.C64 = sC64[buf];
+      v.T32 = sT32[buf];
+      v.C32 = sC32[buf];
+      if (MODE == 0) {
+        acc += integ_bin<0>(elo, ehi, v) * r.coef;
+      } else {
+        const double val = integ_bin<1>(elo, ehi, v) * r.coef;
+        // Kahan-compensated FP32 accumulation over radii
+        const float y = (float)val - comp;
+        const float t = accf + y;
+        comp = (t - accf) - y;
+        accf = t;
+      }
+    }
+    buf ^= 1;
+  }
+  if (valid) part[j] = (MODE == 0) ? acc : ((double)accf - (double)comp);
+}
+
+// ---------------------------------------------------------------------------
+// finalize: renormalisation (renorm_relline_profile, src/Relprofile.cpp:749-796)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(SETUP_THREADS) void k_finalize(BatchDev b) {
+  __shared__ double s_red[SETUP_THREADS];
+  const int req = blockIdx.x;
+  const int tid = threadIdx.x;
+  const ReqParams rq = b.req[req];
+  const int n_e = rq.n_e;
+  double *dst = (rq.prof_off >= 0) ? b.prof + rq.prof_off : b.out + rq.flux_off;
+  const int st = b.status[req];
+  if (st != 0) {
+    for (int j = tid; j < n_e; j += SETUP_THREADS) dst[j] = 0.0;
+    return;
+  }
+  const double *E = (rq.pgrid_off >= 0) ? b.pgrid + rq.pgrid_off : b.cg.ener;
+  const double *part = b.part + rq.part_off;
+  double loc = 0.0;
+  for (int j = tid; j < n_e; j += SETUP_THREADS) {
+    double v = 0.0;
+    for (int c = 0; c < b.n_rch; c++) v += part[(size_t)c * n_e + j];
+    v /= 0.5 * (E[j] + E[j + 1]);
+    dst[j] = v;
+    loc += v;
+  }
+  const double sum = block_sum(loc, s_red);
+  if (rq.do_renorm) {
+    const double f = rq.norm / sum;
+    for (int j = tid; j < n_e; j += SETUP_THREADS) dst[j] *= f;
+  }
+  if (rq.n_incl > 0) {
+    double *d = b.dist + (size_t)req * MAX_INCL;
+    double ds = 0.0;
+    for (int k = tid; k < rq.n_incl; k += SETUP_THREADS) ds += d[k];
+    const double dsum = block_sum(ds, s_red);
+    for (int k = tid; k < rq.n_incl; k += SETUP_THREADS) d[k] /= dsum;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// convolution
+// ---------------------------------------------------------------------------
+namespace {
+
+// value of output bin [elo, ehi] of _rebin_spectrum (src/relutility.c:549-601)
+// from the input grid e0 (n0+1 edges) / spectrum f0 (n0 bins).  The
+// sequential walk of the reference reduces to upper-bound counts.
+__device__ inline double rebin_one(double elo_, double ehi_, const double *e0, const double *f0, int n0) {
+  if (!((e0[0] <= ehi_) && (e0[n0] >= elo_))) return 0.0;
+  int lo = 0, hi = n0 + 1;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (e0[mid] <= elo_) lo = mid + 1; else hi = mid;
+  }
+  int imin = lo > 0 ? lo - 1 : 0;
+  if (imin >= n0) return 0.0;  // zero-width overlap at the top edge
+  lo = 0;
+  hi = n0 + 1;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (e0[mid] <= ehi_) lo = mid + 1; else hi = mid;
+  }
+  int imax = lo < n0 ? lo : n0;
+  if (imax > 0) imax--;
+  double elo = elo_, ehi = ehi_;
+  if (elo < e0[imin]) elo = e0[imin];
+  if (ehi > e0[imax + 1]) ehi = e0[imax + 1];
+  if (imax == imin) {
+    return (ehi - elo) / (e0[imin + 1] - e0[imin]) * f0[imin];
+  }
+  const double dmin = (e0[imin + 1] - elo) / (e0[imin + 1] - e0[imin]);
+  const double dmax = (ehi - e0[imax]) / (e0[imax + 1] - e0[imax]);
+  double flu = 0.0;
+  flu += f0[imin] * dmin + f0[imax] * dmax;
+  for (int jj = imin + 1; jj <= imax - 1; jj++) flu += f0[jj];
+  return flu;
+}
+
+__device__ inline double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ inline int bitrev(int x, int logn) { return (int)(__brev((unsigned)x) >> (32 - logn)); }
+
+// in-place radix-2 DIT FFT of length n (power of two) held in LDS, input
+// already in bit-reversed order.  sign -1: forward, +1: inverse (unnormalised).
+// tw[t] = exp(-2 pi i t / N) for t < N/2, N = stride_n * n.
+__device__ inline void lds_fft(double2 *x, int n, int logn, const double2 *tw, int tw_stride, int sign) {
+  for (int s = 1; s <= logn; s++) {
+    const int half = 1 << (s - 1);
+    const int tstride = tw_stride * (n >> s);
+    for (int bf = threadIdx.x; bf < (n >> 1); bf += blockDim.x) {
+      const int grp = bf >> (s - 1);
+      const int k = bf & (half - 1);
+      const int i = grp * (half << 1) + k;
+      const int jx = i + half;
+      double2 w = tw[k * tstride];
+      if (sign > 0) w.y = -w.y;
+      const double2 u = x[i];
+      const double2 t = cmul(w, x[jx]);
+      x[i] = make_double2(u.x + t.x, u.y + t.y);
+      x[jx] = make_double2(u.x - t.x, u.y - t.y);
+    }
+    __syncthreads();
+  }
+}
+
+__device__ inline bool in_band(const double *e, int i) {
+  return e[i] >= 0.01 && e[i + 1] < 1000.0;  // calcFFTNormFactor band (src/Relbase.cpp:205)
+}
+
+}  // namespace
+
+// Convolution of one request (relconv_kernel minus relbase, src/Relbase.cpp:257-289).
+__global__ __launch_bounds__(CONV_THREADS) void k_conv(BatchDev b, const int *__restrict__ reqs) {
+  extern __shared__ double2 s_buf[];  // n complex
+  __shared__ double s_red[CONV_THREADS];
+  const int req = reqs[blockIdx.x];
+  const int tid = threadIdx.x;
+  const ReqParams rq = b.req[req];
+  const DevConvGrid cg = b.cg;
+  const int n = cg.n;
+  const int m = n >> 1;
+  int logn = 0;
+  while ((1 << logn) < n) logn++;
+  double *out = b.out + rq.flux_off;
+  if (b.status[req] != 0) {
+    for (int j = tid; j < rq.n_in; j += CONV_THREADS) out[j] = 0.0;
+    return;
+  }
+  const double *Ec = cg.ener;
+  const double *cf = cg.cfac;
+  const double *prof = b.prof + rq.prof_off;
+  const double *ein = b.ener_in + rq.ener_off;
+  const double2 *tw = reinterpret_cast<const double2 *>(cg.tw);
+  const bool angular = (rq.model == MOD_RELXILL_CONV);
+
+  // 1. x = rebinned input * E/dE (real part), y = rotated profile * E/dE (imag part)
+  double sx = 0.0, sr = 0.0;
+  const double *dist = b.dist + (size_t)req * MAX_INCL;
+  const DevReflSet *rs = angular ? &b.refl_sets[rq.refl_set] : nullptr;
+  for (int i = tid; i < n; i += CONV_THREADS) {
+    double xr = 0.0;
+    if (!angular) {
+      xr = rebin_one(Ec[i], Ec[i + 1], ein, b.flux_in + rq.flux_off, rq.n_in);
+      if (in_band(Ec, i)) sx += xr;
+    }
+    const double pv = prof[i];
+    if (in_band(Ec, i)) sr += pv;
+    const int irot = (i - cg.i1kev + n) % n;
+    s_buf[bitrev(i, logn)].x = xr * cf[i];
+    s_buf[bitrev(irot, logn)].y = pv * cf[i];
+  }
+  const double sum_xill_local = block_sum(sx, s_red);
+  const double sum_rel = block_sum(sr, s_red);
+  double sum_xill = sum_xill_local;
+  if (angular) {
+    sum_xill = 0.0;
+    for (int k = 0; k < rs->n_incl; k++) sum_xill += dist[k] * rs->bandsum[k];
+  }
+  __syncthreads();
+  // 2. forward complex FFT of x + i y
+  lds_fft(s_buf, n, logn, tw, 1, -1);
+  // 3. X = (W_k + conj W_{n-k})/2, Y = (W_k - conj W_{n-k})/(2i), P = X Y (k = 0..m);
+  //    then pack for the half-length inverse real FFT:
+  //    Z_k = (P_k + conj P_{m-k}) + i (P_k - conj P_{m-k}) e^{+2 pi i k/n}
+  constexpr int MAXC = (MAX_FFT / 2) / CONV_THREADS;
+  double2 zk[MAXC];
+#pragma unroll
+  for (int c = 0; c < MAXC; c++) {
+    const int k = tid + c * CONV_THREADS;
+    if (k >= m) continue;
+    double2 P[2];
+    const int ks[2] = {k, m - k};
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int kk = ks[u];
+      const double2 wk = s_buf[kk];
+      const double2 wn = s_buf[(n - kk) & (n - 1)];
+      double2 Y = make_double2(0.5 * (wk.y + wn.y), -0.5 * (wk.x - wn.x));
+      double2 X;
+      if (angular) {
+        X = make_double2(0.0, 0.0);
+        for (int a = 0; a < rs->n_incl; a++) {
+          const double *F = rs->fft + ((size_t)a * (m + 1) + kk) * 2;
+          X.x += dist[a] * F[0];
+          X.y += dist[a] * F[1];
+        }
+      } else {
+        X = make_double2(0.5 * (wk.x + wn.x), 0.5 * (wk.y - wn.y));
+      }
+      P[u] = cmul(X, Y);
+    }
+    if (k == 0) {  // c2r ignores the imaginary parts of P_0 and P_m
+      P[0].y = 0.0;
+      P[1].y = 0.0;
+    }
+    const double2 A = make_double2(P[0].x + P[1].x, P[0].y - P[1].y);
+    const double2 Bd = make_double2(P[0].x - P[1].x, P[0].y + P[1].y);
+    double2 w = tw[k];
+    w.y = -w.y;  // e^{+2 pi i k/n}
+    const double2 Bw = cmul(Bd, w);
+    zk[c] = make_double2(A.x - Bw.y, A.y + Bw.x);
+  }
+  __syncthreads();
+  const int logm = logn - 1;
+#pragma unroll
+  for (int c = 0; c < MAXC; c++) {
+    const int k = tid + c * CONV_THREADS;
+    if (k < m) s_buf[bitrev(k, logm)] = zk[c];
+  }
+  __syncthreads();
+  // 4. half-length inverse FFT: z_j = out_{2j} + i out_{2j+1}
+  lds_fft(s_buf, m, logm, tw, 2, +1);
+  double *o = reinterpret_cast<double *>(s_buf);
+  double sc = 0.0;
+  for (int i = tid; i < n; i += CONV_THREADS) {
+    const double v = o[i] / cf[i];
+    o[i] = v;
+    if (in_band(Ec, i)) sc += v;
+  }
+  const double sum_conv = block_sum(sc, s_red);
+  const double fnorm = sum_rel * sum_xill / sum_conv;
+  for (int i = tid; i < n; i += CONV_THREADS) o[i] *= fnorm;
+  __syncthreads();
+  // 5. rebin onto the caller grid; relconv zeroes outside 0.01-1000 keV
+  for (int jb = tid; jb < rq.n_in; jb += CONV_THREADS) {
+    double v = rebin_one(ein[jb], ein[jb + 1], Ec, o, n);
+    if (!angular && (ein[jb + 1] < 0.01 || ein[jb] > 1000.0)) v = 0.0;
+    out[jb] = v;
+  }
+}
+
+// Resident reflection set: per angle, rebin onto the convolution grid, band
+// sum, and the real FFT of (spectrum * E/dE), stored for k = 0..n/2.
+__global__ __launch_bounds__(CONV_THREADS) void k_refl_prepare(DevConvGrid cg, const double *refl_ener,
+                                                              const double *refl, int n_refl,
+                                                              double *spec, double *bandsum, double *fft) {
+  extern __shared__ double2 s_buf[];
+  __shared__ double s_red[CONV_THREADS];
+  const int a = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int n = cg.n, m = n >> 1;
+  int logn = 0;
+  while ((1 << logn) < n) logn++;
+  const double *f0 = refl + (size_t)a * n_refl;
+  double sx = 0.0;
+  for (int i = tid; i < n; i += CONV_THREADS) {
+    const double xr = rebin_one(cg.ener[i], cg.ener[i + 1], refl_ener, f0, n_refl);
+    spec[(size_t)a * n + i] = xr;
+    if (in_band(cg.ener, i)) sx += xr;
+    s_buf[bitrev(i, logn)] = make_double2(xr * cg.cfac[i], 0.0);
+  }
+  const double s = block_sum(sx, s_red);
+  if (tid == 0) bandsum[a] = s;
+  __syncthreads();
+  lds_fft(s_buf, n, logn, reinterpret_cast<const double2 *>(cg.tw), 1, -1);
+  for (int k = tid; k <= m; k += CONV_THREADS) {
+    fft[((size_t)a * (m + 1) + k) * 2 + 0] = s_buf[k].x;
+    fft[((size_t)a * (m + 1) + k) * 2 + 1] = s_buf[k].y;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+int launch_setup(const BatchDev &b, void *stream) {
+  if (b.n_req <= 0) return 0;
+  hipLaunchKernelGGL(k_setup, dim3(b.n_req), dim3(SETUP_THREADS), 0, (hipStream_t)stream, b);
+  return (int)hipGetLastError();
+}
+
+int launch_integrate(const BatchDev &b, const Tile *tiles, int n_tiles, int mode, void *stream) {
+  if (n_tiles <= 0) return 0;
+  if (mode == 0) {
+    hipLaunchKernelGGL(k_integrate<0>, dim3(n_tiles), dim3(64), 0, (hipStream_t)stream, b, tiles);
+  } else {
+    hipLaunchKernelGGL(k_integrate<1>, dim3(n_tiles), dim3(64), 0, (hipStream_t)stream, b, tiles);
+  }
+  return (int)hipGetLastError();
+}
+
+int launch_finalize(const BatchDev &b, void *stream) {
+  if (b.n_req <= 0) return 0;
+  hipLaunchKernelGGL(k_finalize, dim3(b.n_req), dim3(SETUP_THREADS), 0, (hipStream_t)stream, b);
+  return (int)hipGetLastError();
+}
+
+int launch_conv(const BatchDev &b, const int *conv_reqs, int n_conv_reqs, void *stream) {
+  if (n_conv_reqs <= 0) return 0;
+  const size_t lds = (size_t)b.cg.n * sizeof(double2);
+  (void)hipFuncSetAttribute((const void *)k_conv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(k_conv, dim3(n_conv_reqs), dim3(CONV_THREADS), lds, (hipStream_t)stream, b, conv_reqs);
+  return (int)hipGetLastError();
+}
+
+int launch_refl_prepare(const DevConvGrid &cg, const double *refl_ener, const double *refl, int n_refl,
+                        int n_incl, double *spec, double *bandsum, double *fft, void *stream) {
+  const size_t lds = (size_t)cg.n * sizeof(double2);
+  (void)hipFuncSetAttribute((const void *)k_refl_prepare, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(k_refl_prepare, dim3(n_incl), dim3(CONV_THREADS), lds, (hipStream_t)stream, cg, refl_ener,
+                     refl, n_refl, spec, bandsum, fft);
+  return (int)hipGetLastError();
+}
+
+}  // namespace rx

@@ -1,0 +1,138 @@
+// rx_engine.h -- device-side data layout of a batch of spectra and the kernel
+// launch interface (internal to librelxill_gpu).
+//
+// HBM layout of one batch (all arrays flat, offsets per request in ReqParams):
+//   req[n]              processed model parameters (host mirror of
+//                       get_rel_params, src/ModelDefinition.cpp:278-334)
+//   ener_in             caller energy grids, n_in+1 edges each
+//   flux_in             caller input spectra (convolution models)
+//   out                 caller-grid outputs, n_in each
+//   pgrid               profile grids (line models: the caller grid shifted
+//                       to line-energy units; conv models use the shared
+//                       convolution grid instead)
+//   node[n]             (a,mu0)-interpolated table node, FP64:
+//                       trff[n_r][n_g][2] followed by cosne[n_r][n_g][2]
+//   rad[n][nr_fine]     per fine radius record (RadRec)
+//   part                per request n_rch partial profiles of n_e bins
+//   prof                per request final profile on the conv grid (conv models)
+//   dist[n][n_incl]     emission-angle distribution (relxill-style requests)
+#pragma once
+#include <stdint.h>
+
+namespace rx {
+
+constexpr int MAX_NR_TAB = 256;   // table radii held in LDS by the setup kernel
+constexpr int MAX_NG = 64;        // g* points (trff values per radius: 2*ng)
+constexpr int MAX_LP_R = 256;     // lamp-post table radii
+constexpr int MAX_INCL = 64;      // emission-angle bins
+constexpr int MAX_FFT = 8192;     // convolution grid size limit
+
+enum ModelKind : int {
+  MOD_RELLINE = 1,
+  MOD_RELLINELP = 2,
+  MOD_RELCONV = 11,
+  MOD_RELCONVLP = 12,
+  MOD_RELXILL_CONV = -1,  // relxill-style: angle-resolved reflection, one zone
+};
+
+enum EmisKind : int { EMIS_BKN = 1, EMIS_LP = 2 };
+
+struct ReqParams {
+  int model;
+  int emis_type;
+  int limb;
+  int do_renorm;
+  int n_e;        // bins of the profile grid
+  int n_in;       // bins of the caller grid
+  int need_cosne; // limb != 0 or angle distribution requested
+  int n_incl;     // >0: compute the angle distribution with n_incl bins
+  double a, incl, emis1, emis2, rbr, rin, rout, lineE, z, height, gamma, beta;
+  double norm;    // renormalisation target (1, or 0.5 cos i for relxill BKN)
+  int64_t ener_off;   // caller grid edges in ener_in
+  int64_t flux_off;   // caller input / output spectrum offset (n_in)
+  int64_t pgrid_off;  // profile grid edges in pgrid (-1: shared conv grid)
+  int64_t part_off;   // partial profiles (n_rch * n_e)
+  int64_t prof_off;   // final profile (n_e) for conv models, -1 otherwise
+  int refl_set;       // relxill-style: which resident reflection set
+  int pre_status;     // host-side parameter status; nonzero requests are skipped
+};
+
+// One fine radius (64 bytes).  The bin integral at this radius is
+//   flux[j] += integ_bin(E_j, E_j+1) * coef,   coef = emis*weight/(gmax-gmin)
+struct RadRec {
+  double gmin, gmax, del_g, coef, ifac_r, emisw;
+  int K;       // table-node bracket: values lerp between node K+1 (lo) and K (hi)
+  int ielo;    // first bin of the window (ielo > iehi: radius does not contribute)
+  int iehi;    // last bin of the window (inclusive)
+  int pad;
+};
+
+struct LpRefl {
+  double refl_frac, f_bh, f_ad, f_inf, f_inf_rest;
+};
+
+struct DevRelTable {
+  int n_a, n_mu0, n_r, n_g;
+  const float *a, *mu0, *r, *gmin, *gmax, *trff1, *trff2, *cosne1, *cosne2;
+};
+
+struct DevLpTable {
+  int n_a, n_h, n_r;
+  const float *a, *h, *rad, *intens, *del, *del_inc;
+};
+
+// Reflection set resident on the device for relxill-style requests: the
+// per-angle spectra rebinned to the convolution grid, their band sums and
+// the real FFT of (spectrum * E/dE) on the convolution grid.
+struct DevReflSet {
+  int n_incl;
+  const double *spec;    // [n_incl][n_conv]   rebinned, photons/bin
+  const double *bandsum; // [n_incl]
+  const double *fft;     // [n_incl][n_conv/2+1][2]
+};
+
+struct DevConvGrid {
+  int n;               // bins (power of two)
+  int i1kev;           // binary_search(ener, n+1, 1.0)
+  const double *ener;  // n+1 edges
+  const double *cfac;  // E/dE conversion, n
+  const double *tw;    // twiddles exp(-2 pi i k/n), k < n/2, interleaved re/im
+};
+
+struct BatchDev {
+  int n_req;
+  int nr_fine;
+  int n_rch;      // radius chunks per request
+  int node_stride;  // doubles per request in node (2 * n_r * n_g * 2)
+  ReqParams *req;
+  const double *ener_in;
+  const double *flux_in;
+  double *out;
+  double *pgrid;
+  double *node;
+  RadRec *rad;
+  double *part;
+  double *prof;
+  double *dist;
+  LpRefl *refl;
+  int *status;
+  DevRelTable rt;
+  DevLpTable lp;
+  DevConvGrid cg;
+  const DevReflSet *refl_sets;
+};
+
+struct Tile {
+  int req, bchunk, rch, pad;
+};
+
+// ---- launchers (kernels.hip) ----
+int launch_setup(const BatchDev &b, void *stream);
+int launch_integrate(const BatchDev &b, const Tile *tiles, int n_tiles, int mode, void *stream);
+int launch_finalize(const BatchDev &b, void *stream);
+int launch_conv(const BatchDev &b, const int *conv_reqs, int n_conv_reqs, void *stream);
+int launch_refl_prepare(const DevConvGrid &cg, const double *refl_ener, const double *refl,
+                        int n_refl, int n_incl, double *spec, double *bandsum, double *fft,
+                        void *stream);
+
+}  // namespace rx

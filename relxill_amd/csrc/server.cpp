@@ -1,0 +1,303 @@
+// server.cpp -- continuous-batching serving loop.
+//
+// Requests are copied at submit time into a pending queue.  A scheduler thread
+// keeps up to n_streams batches on the device at once, each on its own HIP
+// stream: whenever a slot is free and requests are pending, it admits pending
+// requests in arrival order while their device working set fits the HBM
+// budget (minus what the running batches hold) and the in-flight cap, builds
+// a batch and launches it.  Finished batches (polled with hipStreamQuery) are
+// copied back, their memory returned to the budget, and the requests marked
+// done with their latency (submit -> completion seen).  Tables and the
+// convolution grid stay resident for the server's lifetime.
+//
+// Each request is computed with a fixed tile shape (radius chunks), so its
+// result does not depend on which other requests share its batch.
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "engine.h"
+
+using namespace rx;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+constexpr int SERVER_RCHUNKS = 4;
+
+struct Pending {
+  int64_t ticket;
+  int model;
+  int refl_set;
+  std::vector<double> energy, params, flux_in;
+  Clock::time_point t_submit;
+  int64_t bytes;
+};
+
+struct Result {
+  bool done = false;
+  int status = 0;
+  double latency_ms = 0;
+  std::vector<double> flux;
+};
+
+struct Slot {
+  Batch batch;
+  bool busy = false;
+  std::vector<Pending> reqs;
+  int64_t bytes = 0;
+};
+
+int64_t request_bytes(const Pending &p) {
+  Global &g = G();
+  const bool conv = (p.model == MOD_RELCONV || p.model == MOD_RELCONVLP || p.model == MOD_RELXILL_CONV);
+  const int64_t n_in = (int64_t)p.energy.size() - 1;
+  const int64_t n_e = conv ? g.conv_n : n_in;
+  int64_t b = sizeof(ReqParams) + 8 * (2 * (n_in + 1) + 2 * n_in);
+  b += 8 * (n_e + 1) + 8 * n_e * (SERVER_RCHUNKS + 1);
+  b += (int64_t)sizeof(RadRec) * g.nr_fine;
+  b += 8 * 2 * (int64_t)g.rel.n_r * g.rel.n_g * 2;
+  b += 8 * MAX_INCL + sizeof(LpRefl) + 64;
+  return b;
+}
+
+}  // namespace
+
+struct rx_server {
+  int max_inflight, n_streams, precision;
+  int64_t budget;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Pending> pending;
+  std::unordered_map<int64_t, Result> results;
+  std::vector<std::unique_ptr<Slot>> slots;
+  std::atomic<bool> stop{false};
+  std::thread th;
+  int64_t next_ticket = 0;
+  int64_t submitted = 0, completed = 0, batches = 0, max_concurrent = 0;
+  int64_t inflight_bytes = 0;
+  int inflight_reqs = 0;
+  int device = 0;
+
+  void loop();
+  void launch_ready();
+  bool reap();
+};
+
+void rx_server::launch_ready() {
+  // called with mu held
+  for (auto &sp : slots) {
+    Slot &s = *sp;
+    if (s.busy || pending.empty()) continue;
+    std::vector<Pending> take;
+    int64_t bytes = 0;
+    while (!pending.empty()) {
+      Pending &p = pending.front();
+      if (inflight_reqs + (int)take.size() >= max_inflight) break;
+      if (inflight_bytes + bytes + p.bytes > budget && !(take.empty() && inflight_reqs == 0)) break;
+      bytes += p.bytes;
+      take.push_back(std::move(p));
+      pending.pop_front();
+    }
+    if (take.empty()) break;
+    std::vector<rx_request> rq(take.size());
+    for (size_t i = 0; i < take.size(); i++) {
+      rq[i].model = take[i].model;
+      rq[i].n_flux = (int)take[i].energy.size() - 1;
+      rq[i].energy = take[i].energy.data();
+      rq[i].parameter = take[i].params.data();
+      rq[i].flux_in = take[i].flux_in.empty() ? nullptr : take[i].flux_in.data();
+      rq[i].refl_set = take[i].refl_set;
+    }
+    std::string err;
+    int st;
+    {
+      std::lock_guard<std::recursive_mutex> lk(G().mu);
+      st = ensure_device(err);
+      if (st == RX_OK) {
+        s.batch.n_rch_forced = SERVER_RCHUNKS;
+        st = batch_build(s.batch, rq.data(), (int)rq.size(), err);
+      }
+      if (st == RX_OK) st = batch_run(s.batch, precision, err);
+    }
+    if (st != RX_OK) {
+      const auto now = Clock::now();
+      for (auto &p : take) {
+        Result &r = results[p.ticket];
+        r.done = true;
+        r.status = st;
+        r.flux.assign(p.energy.size() - 1, 0.0);
+        r.latency_ms = std::chrono::duration<double, std::milli>(now - p.t_submit).count();
+        completed++;
+      }
+      cv.notify_all();
+      continue;
+    }
+    s.busy = true;
+    s.reqs = std::move(take);
+    s.bytes = bytes;
+    inflight_bytes += bytes;
+    inflight_reqs += (int)s.reqs.size();
+    batches++;
+    int64_t conc = 0;
+    for (auto &o : slots) conc += o->busy ? 1 : 0;
+    if (conc > max_concurrent) max_concurrent = conc;
+  }
+}
+
+bool rx_server::reap() {
+  // called with mu held
+  bool any = false;
+  for (auto &sp : slots) {
+    Slot &s = *sp;
+    if (!s.busy) continue;
+    hipError_t q = hipStreamQuery(s.batch.stream);
+    if (q == hipErrorNotReady) continue;
+    const auto now = Clock::now();
+    std::vector<int> st(s.batch.n_req, RX_ERR_DEVICE);
+    std::vector<double> out(s.batch.tot_flux, 0.0);
+    if (q == hipSuccess) {
+      bool ok = hipMemcpy(out.data(), s.batch.d_out.p, sizeof(double) * s.batch.tot_flux, hipMemcpyDeviceToHost) == hipSuccess;
+      ok = ok && hipMemcpy(st.data(), s.batch.d_status.p, sizeof(int) * s.batch.n_req, hipMemcpyDeviceToHost) == hipSuccess;
+      if (!ok) st.assign(s.batch.n_req, RX_ERR_DEVICE);
+    }
+    for (size_t i = 0; i < s.reqs.size(); i++) {
+      const ReqParams &p = s.batch.h_req[i];
+      Result &r = results[s.reqs[i].ticket];
+      r.done = true;
+      int code = st[i];
+      if (code >= 10 && code < 50) code = RX_ERR_TABLE;
+      r.status = code;
+      r.flux.assign(out.begin() + p.flux_off, out.begin() + p.flux_off + p.n_in);
+      if (code != RX_OK) std::fill(r.flux.begin(), r.flux.end(), 0.0);
+      r.latency_ms = std::chrono::duration<double, std::milli>(now - s.reqs[i].t_submit).count();
+      completed++;
+    }
+    inflight_bytes -= s.bytes;
+    inflight_reqs -= (int)s.reqs.size();
+    s.reqs.clear();
+    s.busy = false;
+    any = true;
+  }
+  if (any) cv.notify_all();
+  return any;
+}
+
+void rx_server::loop() {
+  (void)hipSetDevice(device);
+  std::unique_lock<std::mutex> lk(mu);
+  while (!stop.load()) {
+    launch_ready();
+    bool progressed = reap();
+    bool busy = false;
+    for (auto &s : slots) busy |= s->busy;
+    if (!progressed) {
+      if (busy) {
+        lk.unlock();
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
+        lk.lock();
+      } else {
+        cv.wait_for(lk, std::chrono::milliseconds(5), [&] { return stop.load() || !pending.empty(); });
+      }
+    }
+  }
+  // drain whatever is still on the device
+  for (auto &s : slots)
+    if (s->busy) (void)hipStreamSynchronize(s->batch.stream);
+  reap();
+}
+
+extern "C" {
+
+rx_server *rx_server_start(int max_inflight, int64_t hbm_budget_bytes, int n_streams, int precision) {
+  if (max_inflight <= 0 || n_streams <= 0 || n_streams > 16 || hbm_budget_bytes <= 0) {
+    set_last(RX_ERR_ARG, "invalid server configuration");
+    return nullptr;
+  }
+  {
+    std::lock_guard<std::recursive_mutex> lk(G().mu);
+    std::string err;
+    int st = ensure_device(err);
+    if (st != RX_OK) {
+      set_last(st, err);
+      return nullptr;
+    }
+  }
+  auto *s = new rx_server;
+  s->max_inflight = max_inflight;
+  s->budget = hbm_budget_bytes;
+  s->n_streams = n_streams;
+  s->precision = precision;
+  s->device = G().device;
+  for (int i = 0; i < n_streams; i++) s->slots.emplace_back(new Slot);
+  s->th = std::thread([s] { s->loop(); });
+  return s;
+}
+
+int64_t rx_server_submit(rx_server *s, const rx_request *req) {
+  if (!s || !req || !req->energy || !req->parameter || req->n_flux <= 0) return -RX_ERR_ARG;
+  const int np = n_model_params(req->model);
+  if (np < 0) return -RX_ERR_ARG;
+  Pending p;
+  p.model = req->model;
+  p.refl_set = req->refl_set;
+  p.energy.assign(req->energy, req->energy + req->n_flux + 1);
+  p.params.assign(req->parameter, req->parameter + np);
+  if (req->flux_in) p.flux_in.assign(req->flux_in, req->flux_in + req->n_flux);
+  p.t_submit = Clock::now();
+  std::lock_guard<std::mutex> lk(s->mu);
+  p.bytes = request_bytes(p);
+  p.ticket = s->next_ticket++;
+  s->results[p.ticket] = Result();
+  s->submitted++;
+  const int64_t t = p.ticket;
+  s->pending.push_back(std::move(p));
+  s->cv.notify_all();
+  return t;
+}
+
+int rx_server_poll(rx_server *s, int64_t ticket, double *flux, int *status, double *latency_ms) {
+  if (!s) return -RX_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->results.find(ticket);
+  if (it == s->results.end()) return -1;
+  if (!it->second.done) return 0;
+  if (flux) std::memcpy(flux, it->second.flux.data(), sizeof(double) * it->second.flux.size());
+  if (status) *status = it->second.status;
+  if (latency_ms) *latency_ms = it->second.latency_ms;
+  s->results.erase(it);
+  return 1;
+}
+
+int rx_server_drain(rx_server *s) {
+  if (!s) return RX_ERR_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->cv.wait(lk, [&] { return s->completed >= s->submitted; });
+  return RX_OK;
+}
+
+int rx_server_stats(rx_server *s, int64_t *out3) {
+  if (!s || !out3) return RX_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  out3[0] = s->completed;
+  out3[1] = s->batches;
+  out3[2] = s->max_concurrent;
+  return RX_OK;
+}
+
+void rx_server_stop(rx_server *s) {
+  if (!s) return;
+  s->stop.store(true);
+  s->cv.notify_all();
+  if (s->th.joinable()) s->th.join();
+  delete s;
+}
+
+}  // extern "C"

@@ -1,0 +1,95 @@
+"""Shared inputs and metrics for the parity tests.
+
+Workloads follow the benchmark configurations: a 25 x 22 x 100 x 20 synthetic
+relline table (seed 1), a 20 x 100 x 100 lamp-post table (seed 3), 3000 log
+bins over 0.1-10 keV for relline, the reference convolution grid (4096 log bins
+over 0.00035-2000 keV) for relconv.
+"""
+import numpy as np
+
+import oracle
+import relxill_amd as rx
+
+RELLINE_CFG0 = [6.4, 3.0, 3.0, 15.0, 0.998, 30.0, -1.0, 400.0, 0.0, 0.0]
+
+
+def log_grid(n_bins, emin, emax):
+    """get_log_grid (src/relutility.c:399-405) with n_bins+1 edges."""
+    return oracle.log_grid(n_bins + 1, emin, emax)
+
+
+_TABLES = {}
+
+
+def tables():
+    """(rx tables view, oracle.Tables) of the resident default tables."""
+    if "t" not in _TABLES:
+        rx.tables_synthetic(1, 25, 22, 100, 20)
+        rx.lptable_synthetic(3, 20, 100, 100)
+        rel, lp = rx.tables_view()
+        _TABLES["t"] = (rel, lp, oracle.Tables(rel, lp))
+    return _TABLES["t"]
+
+
+def rel_err(y, yref, floor=1e-10):
+    """max |y - y*|/|y*| over bins with |y*| > floor * max|y*| (the reference's
+    comparison filter, test/refdata/test_refdata_relxill.sl:162-163)."""
+    y = np.asarray(y, dtype=np.float64)
+    yref = np.asarray(yref, dtype=np.float64)
+    m = np.abs(yref).max()
+    if m == 0:
+        return float(np.abs(y).max())
+    sel = np.abs(yref) > floor * m
+    return float(np.max(np.abs(y[sel] - yref[sel]) / np.abs(yref[sel])))
+
+
+def abs_err(y, yref):
+    """unfiltered max |y - y*| / max |y*|"""
+    m = np.abs(yref).max()
+    return float(np.abs(np.asarray(y) - np.asarray(yref)).max() / (m if m else 1.0))
+
+
+def reflection_spectrum(n_bins=2999, emin=0.07, emax=1000.0, n_incl=10, seed=2):
+    """Seeded synthetic reflection spectra (photons/bin) on a log grid: a cutoff
+    power law (Gamma=2, Ecut=300 keV) plus Gaussian lines at 6.4 and 6.97 keV,
+    with a smooth per-angle modulation.  Returns (edges, spectra[n_incl, n_bins])."""
+    rng = np.random.default_rng(seed)
+    e = log_grid(n_bins, emin, emax)
+    em = 0.5 * (e[1:] + e[:-1])
+    de = np.diff(e)
+    base = em ** -2.0 * np.exp(-em / 300.0)
+    lines = 0.02 * np.exp(-0.5 * ((em - 6.4) / 0.05) ** 2) + 0.008 * np.exp(-0.5 * ((em - 6.97) / 0.05) ** 2)
+    out = np.empty((n_incl, n_bins))
+    for k in range(n_incl):
+        mu = 1.0 - (k + 0.5) / n_incl
+        amp = 1.0 + 0.3 * (mu - 0.5) + 0.05 * rng.standard_normal()
+        tilt = 0.05 * (mu - 0.5)
+        out[k] = (base * em ** tilt + lines * (1.0 + 0.5 * mu)) * amp * de
+    return e, out
+
+
+def draw_lp_params(n, seed=4, lineE=6.4):
+    """cfg2 priors: a U[-0.998,0.998], h U[3,100], incl U[5,85] deg, Rin U[ISCO,50]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        a = rng.uniform(-0.998, 0.998)
+        h = rng.uniform(3.0, 100.0)
+        incl = rng.uniform(5.0, 85.0)
+        isco = oracle.lib.orc_kerr_rms(a)
+        rin = rng.uniform(isco, 50.0)
+        out.append([lineE, h, a, incl, rin, 400.0, 0.0, 0.0, 2.0, 0.0])
+    return out
+
+
+def draw_relconv_params(n, seed=5):
+    """cfg3 priors with a broken power law q=3: relconv parameter arrays."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        a = rng.uniform(-0.998, 0.998)
+        incl = rng.uniform(5.0, 85.0)
+        isco = oracle.lib.orc_kerr_rms(a)
+        rin = rng.uniform(isco, 50.0)
+        out.append([3.0, 3.0, 15.0, a, incl, rin, 400.0, 0.0])
+    return out

@@ -1,0 +1,132 @@
+"""GPU parity: HIP path (through the C ABI) against the FP64 CPU oracle.
+
+Bars: FP64 path within 1e-8 relative per bin, FP32 path within 1e-4, over
+bins above 1e-10 of the maximum (the reference's comparison filter,
+test/refdata/test_refdata_relxill.sl:162-163).
+"""
+import numpy as np
+import pytest
+
+import oracle
+import relxill_amd as rx
+from tests.helpers import (RELLINE_CFG0, abs_err, draw_lp_params, draw_relconv_params, log_grid,
+                           reflection_spectrum, rel_err, tables)
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-8
+TOL32 = 1e-4
+
+
+def _report(name, y, yref):
+    print(f"{name}: rel {rel_err(y, yref):.3e} abs {abs_err(y, yref):.3e} sum {np.sum(y):.12f} ref {np.sum(yref):.12f}",
+          flush=True)
+
+
+@pytest.fixture(scope="module")
+def tabs():
+    rel, lp, T = tables()
+    assert rx.lib.rx_init(0) == 0, rx.lib.rx_last_error()
+    return T
+
+
+def test_relline_cfg0_fp64(tabs):
+    e = log_grid(3000, 0.1, 10.0)
+    ref, st = oracle.eval_model(oracle.MOD_RELLINE, tabs, RELLINE_CFG0, e)
+    assert st == 0
+    y, st = rx.eval_model(rx.MODEL_RELLINE, RELLINE_CFG0, e, precision=rx.FP64)
+    _report("relline cfg0 fp64", y, ref)
+    assert rel_err(y, ref) < TOL64
+
+
+def test_relline_cfg0_fp32(tabs):
+    e = log_grid(3000, 0.1, 10.0)
+    ref, _ = oracle.eval_model(oracle.MOD_RELLINE, tabs, RELLINE_CFG0, e)
+    y, st = rx.eval_model(rx.MODEL_RELLINE, RELLINE_CFG0, e, precision=rx.FP32)
+    _report("relline cfg0 fp32", y, ref)
+    assert rel_err(y, ref) < TOL32
+
+
+def test_lmodrelline_symbol_matches(tabs):
+    e = log_grid(3000, 0.1, 10.0)
+    y1 = rx.lmodrelline(e, RELLINE_CFG0)
+    y2, _ = rx.eval_model(rx.MODEL_RELLINE, RELLINE_CFG0, e)
+    assert np.array_equal(y1, y2)
+
+
+@pytest.mark.parametrize("limb", [1, 2])
+def test_relline_limb(tabs, limb):
+    e = log_grid(1500, 0.1, 10.0)
+    p = list(RELLINE_CFG0)
+    p[9] = limb
+    p[4] = 0.5
+    p[5] = 60.0
+    ref, st = oracle.eval_model(oracle.MOD_RELLINE, tabs, p, e)
+    assert st == 0
+    y, _ = rx.eval_model(rx.MODEL_RELLINE, p, e, precision=rx.FP64)
+    _report(f"relline limb{limb} fp64", y, ref)
+    assert rel_err(y, ref) < TOL64
+    y32, _ = rx.eval_model(rx.MODEL_RELLINE, p, e, precision=rx.FP32)
+    assert rel_err(y32, ref) < TOL32
+
+
+def test_relline_lp_fp64_fp32(tabs):
+    e = log_grid(3000, 0.1, 10.0)
+    for p in draw_lp_params(6, seed=11):
+        ref, st = oracle.eval_model(oracle.MOD_RELLINELP, tabs, p, e)
+        assert st == 0, p
+        y, _ = rx.eval_model(rx.MODEL_RELLINELP, p, e, precision=rx.FP64)
+        _report(f"relline_lp {np.round(p[1:5], 3)} fp64", y, ref)
+        assert rel_err(y, ref) < TOL64
+        y32, _ = rx.eval_model(rx.MODEL_RELLINELP, p, e, precision=rx.FP32)
+        assert rel_err(y32, ref) < TOL32
+
+
+def test_relconv_fp64(tabs):
+    e_ref, refl = reflection_spectrum(n_incl=1)
+    e = log_grid(3000, 0.1, 100.0)
+    fin = oracle.rebin(e, e_ref, refl[0])
+    p = [3.0, 3.0, 15.0, 0.998, 30.0, -1.0, 400.0, 0.0]
+    ref, st = oracle.eval_model(oracle.MOD_RELCONV, tabs, p, e, flux_in=fin)
+    assert st == 0
+    y, _ = rx.eval_model(rx.MODEL_RELCONV, p, e, flux_in=fin, precision=rx.FP64)
+    _report("relconv fp64", y, ref)
+    assert rel_err(y, ref) < TOL64
+    y32, _ = rx.eval_model(rx.MODEL_RELCONV, p, e, flux_in=fin, precision=rx.FP32)
+    _report("relconv fp32", y32, ref)
+    assert rel_err(y32, ref) < TOL32
+
+
+def test_relconv_lp_fp64(tabs):
+    e_ref, refl = reflection_spectrum(n_incl=1)
+    e = log_grid(2000, 0.1, 100.0)
+    fin = oracle.rebin(e, e_ref, refl[0])
+    p = [10.0, 0.3, 0.7, 45.0, -2.0, 300.0, 0.0, 2.2, 0.0]
+    ref, st = oracle.eval_model(oracle.MOD_RELCONVLP, tabs, p, e, flux_in=fin)
+    assert st == 0
+    y, _ = rx.eval_model(rx.MODEL_RELCONVLP, p, e, flux_in=fin, precision=rx.FP64)
+    _report("relconv_lp fp64", y, ref)
+    assert rel_err(y, ref) < TOL64
+
+
+def test_batch_uneven_relconv_fp32(tabs):
+    rng = np.random.default_rng(7)
+    e_ref, refl = reflection_spectrum(n_incl=1)
+    reqs, refs = [], []
+    for p in draw_relconv_params(12, seed=9):
+        n = int(np.exp(rng.uniform(np.log(300), np.log(8000))))
+        e = log_grid(n, 0.1, 100.0)
+        fin = oracle.rebin(e, e_ref, refl[0])
+        reqs.append(dict(model=rx.MODEL_RELCONV, energy=e, params=p, flux_in=fin))
+        ref, st = oracle.eval_model(oracle.MOD_RELCONV, tabs, p, e, flux_in=fin)
+        assert st == 0
+        refs.append(ref)
+    b = rx.Batch(reqs)
+    for prec, tol in ((rx.FP64, TOL64), (rx.FP32, TOL32)):
+        b.run(prec)
+        b.sync()
+        for i, ref in enumerate(refs):
+            y, st = b.output(i)
+            assert st == 0
+            assert rel_err(y, ref) < tol, (prec, i, rel_err(y, ref))
+    b.close()
