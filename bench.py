@@ -1,0 +1,339 @@
+#!/usr/bin/env python3
+"""Benchmark of the relativistic smearing path on one MI355X.
+
+Headline workload (the value): an uneven batch of 4096 relconv requests, each
+with its own log energy grid of 300-8000 bins (log-uniform) over 0.1-100 keV,
+parameters from the lamp-post priors (a U[-0.998,0.998], incl U[5,85] deg,
+Rin U[ISCO,50]) with a broken power law q=3, Rout=400, on the 4096-bin
+convolution grid; 25x22x100x20 relline table.  One step = one pass of the whole
+device path (setup, relline integration, renormalisation, FFT convolution,
+rebin) over the batch, FP32 with Kahan sums, inputs resident in HBM.
+
+Also reported: the plain FP64 path (one request per launch), the batched
+lamp-post relline (1024 requests, 3000 bins), the serving loop's p50/p99
+latency under Poisson arrivals (relline/relconv/lamp-post mix 40/40/20), the
+maximum relative error of the FP32 path against the FP64 oracle, the dominant
+kernel's roofline figures and the FP64 CPU oracle timed on the host's cores.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kerr_rms(a):
+    s = -1.0 if a < 0 else 1.0
+    z1 = 1.0 + (1.0 - a * a) ** (1 / 3) * ((1.0 + a) ** (1 / 3) + (1.0 - a) ** (1 / 3))
+    z2 = np.sqrt(3.0 * a * a + z1 * z1)
+    return 3.0 + z2 - s * np.sqrt((3.0 - z1) * (3.0 + z1 + 2 * z2))
+
+
+def log_grid(n_bins, emin, emax):
+    i = np.arange(n_bins + 1, dtype=np.float64)
+    return np.exp(i / n_bins * (np.log(emax) - np.log(emin)) + np.log(emin))
+
+
+def refl_input(e):
+    """cutoff power law (Gamma=2, Ecut=300 keV) + Gaussians at 6.4 / 6.97 keV, photons/bin"""
+    em = 0.5 * (e[1:] + e[:-1])
+    de = np.diff(e)
+    lines = 0.02 * np.exp(-0.5 * ((em - 6.4) / 0.05) ** 2) + 0.008 * np.exp(-0.5 * ((em - 6.97) / 0.05) ** 2)
+    return (em ** -2.0 * np.exp(-em / 300.0) + lines) * de
+
+
+def make_cfg3(n_req, seed=5):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(n_req):
+        n = int(np.exp(rng.uniform(np.log(300), np.log(8000))))
+        a = rng.uniform(-0.998, 0.998)
+        incl = rng.uniform(5.0, 85.0)
+        rin = rng.uniform(kerr_rms(a), 50.0)
+        e = log_grid(n, 0.1, 100.0)
+        reqs.append(dict(model=11, energy=e, params=[3.0, 3.0, 15.0, a, incl, rin, 400.0, 0.0],
+                         flux_in=refl_input(e)))
+    return reqs
+
+
+def make_cfg2(n_req, seed=4):
+    rng = np.random.default_rng(seed)
+    e = log_grid(3000, 0.1, 10.0)
+    reqs = []
+    for _ in range(n_req):
+        a = rng.uniform(-0.998, 0.998)
+        h = rng.uniform(3.0, 100.0)
+        incl = rng.uniform(5.0, 85.0)
+        rin = rng.uniform(kerr_rms(a), 50.0)
+        reqs.append(dict(model=2, energy=e, params=[6.4, h, a, incl, rin, 400.0, 0.0, 0.0, 2.0, 0.0]))
+    return reqs
+
+
+def rel_err(y, yref, floor=1e-10):
+    m = np.abs(yref).max()
+    if m == 0:
+        return float(np.abs(y).max())
+    sel = np.abs(yref) > floor * m
+    return float(np.max(np.abs(y[sel] - yref[sel]) / np.abs(yref[sel])))
+
+
+def timed_steps(rx, b, prec, steps, warmup):
+    for _ in range(warmup):
+        b.run(prec)
+    b.sync()
+    rx.lib.rx_device_synchronize()
+    kms = np.zeros(4)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run(prec)
+        b.sync()
+        kms += b.kernel_ms()
+    rx.lib.rx_device_synchronize()
+    t1 = time.perf_counter()
+    return (t1 - t0) / steps, kms / steps
+
+
+def serve(rx, seconds, qps, seed=6):
+    """Poisson arrivals, mix relline/relconv/lamp-post 40/40/20, grids 300-8000 bins."""
+    rng = np.random.default_rng(seed)
+    srv = rx.Server(max_inflight=4096, hbm_budget=16 << 30, n_streams=4, precision=rx.FP32)
+    tickets = []
+    t_start = time.perf_counter()
+    t_next = t_start
+    n_sub = 0
+    while True:
+        now = time.perf_counter()
+        if now - t_start > seconds:
+            break
+        if now < t_next:
+            time.sleep(min(t_next - now, 0.0005))
+            continue
+        t_next += rng.exponential(1.0 / qps)
+        n = int(np.exp(rng.uniform(np.log(300), np.log(8000))))
+        a = rng.uniform(-0.998, 0.998)
+        incl = rng.uniform(5.0, 85.0)
+        rin = rng.uniform(kerr_rms(a), 50.0)
+        u = rng.uniform()
+        if u < 0.4:
+            e = log_grid(n, 0.1, 10.0)
+            t = srv.submit(1, e, [6.4, 3.0, 3.0, 15.0, a, incl, rin, 400.0, 0.0, 0.0])
+        elif u < 0.8:
+            e = log_grid(n, 0.1, 100.0)
+            t = srv.submit(11, e, [3.0, 3.0, 15.0, a, incl, rin, 400.0, 0.0], flux_in=refl_input(e))
+        else:
+            e = log_grid(n, 0.1, 10.0)
+            h = rng.uniform(3.0, 100.0)
+            t = srv.submit(2, e, [6.4, h, a, incl, rin, 400.0, 0.0, 0.0, 2.0, 0.0])
+        tickets.append(t)
+        n_sub += 1
+    srv.drain()
+    t_end = time.perf_counter()
+    lats, bad = [], 0
+    for t in tickets:
+        r = srv.poll(t)
+        lats.append(r[2])
+        bad += r[1] != 0
+    st = srv.stats()
+    srv.stop()
+    lats = np.array(lats)
+    return dict(qps_offered=qps, submitted=n_sub, failed=int(bad),
+                p50_ms=float(np.percentile(lats, 50)), p99_ms=float(np.percentile(lats, 99)),
+                spectra_per_s=n_sub / (t_end - t_start), batches=st["batches"],
+                max_concurrent_batches=st["max_concurrent_batches"])
+
+
+def cpu_baseline(reqs, n_threads, gpu_out):
+    import oracle
+    import relxill_amd as rx
+    rel, lp = rx.tables_view()
+    T = oracle.Tables(rel, lp)
+    jobs = [(r["model"], r["params"], r["energy"], r["flux_in"]) for r in reqs]
+    # single core
+    t0 = time.perf_counter()
+    outs1, st1 = oracle.eval_batch_mt(T, jobs[:16], 1)
+    t1 = time.perf_counter()
+    single = 16 / (t1 - t0)
+    t0 = time.perf_counter()
+    outs, st = oracle.eval_batch_mt(T, jobs, n_threads)
+    t1 = time.perf_counter()
+    allc = len(jobs) / (t1 - t0)
+    err = max(rel_err(gpu_out[i], outs[i]) for i in range(len(jobs)) if st[i] == 0)
+    return dict(single_core=single, all_cores=allc, cpu_seconds=(t1 - t0) * n_threads,
+                max_rel_err_fp32_vs_oracle=err, n=len(jobs))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n-requests", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--cpu-sample", type=int, default=768)
+    ap.add_argument("--shape", type=str, default="",
+                    help="fixed tile shape 'radius_chunks,groups' instead of autotuning")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only the headline batch (FP32), for profiler runs")
+    ap.add_argument("--copy-calibration", type=float, default=0.0,
+                    help="run a plain copy kernel of this many MiB (counter calibration) and exit")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(requests shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import relxill_amd as rx
+    rx.tables_synthetic(1, 25, 22, 100, 20)
+    rx.lptable_synthetic(3, 20, 100, 100)
+    rx.set_conv_grid(4096, 0.00035, 2000.0)
+    if rx.lib.rx_init(0) != 0:
+        raise SystemExit("no GPU: " + rx.lib.rx_last_error().decode())
+    rx.lib.rx_profile_enable(1)
+    if args.copy_calibration > 0:
+        nbytes = int(args.copy_calibration * (1 << 20))
+        ms = np.zeros(1)
+        assert rx.lib.rx_copy_test(nbytes, 5, ms.ctypes.data_as(rx.capi._D)) == 0
+        print(json.dumps({"copy_bytes": nbytes, "copy_ms": float(ms[0]),
+                          "copy_GBps": 2 * nbytes / ms[0] / 1e6}))
+        return
+
+    log("building cfg3 batch ...")
+    reqs = make_cfg3(args.n_requests)
+    nbins = np.array([r["energy"].size - 1 for r in reqs])
+    b = rx.Batch(reqs)
+    if args.shape:
+        rch, grp = (int(x) for x in args.shape.split(","))
+        b.set_rchunks(rch)
+        b.set_groups(grp)
+    else:
+        rch, grp = b.autotune(rx.FP32)
+    log(f"autotuned radius chunks: {rch}, radius groups: {grp}")
+    dt32, kms32 = timed_steps(rx, b, rx.FP32, args.steps, args.warmup)
+    value = len(reqs) / dt32
+    log(f"cfg3 fp32: {dt32 * 1e3:.2f} ms/step, {value:.1f} spectra/s, kernels ms {np.round(kms32, 3)}")
+    if args.profile_only:
+        print(json.dumps({"ms_per_step": dt32 * 1e3, "kernel_ms": kms32.tolist(), "value": value}))
+        return
+    out32 = [b.output(i)[0] for i in range(len(reqs))]
+
+    dt64b, kms64b = timed_steps(rx, b, rx.FP64, max(2, args.steps // 3), 1)
+    out64 = [b.output(i)[0] for i in range(len(reqs))]
+    err32_64 = max(rel_err(out32[i], out64[i]) for i in range(len(reqs)))
+    log(f"cfg3 fp64 batched: {dt64b * 1e3:.2f} ms/step; max rel err fp32 vs fp64 {err32_64:.3e}")
+    b.close()
+
+    # plain FP64: one request per launch (device-resident single-request batches)
+    lat = []
+    for r in reqs[:16]:
+        pb = rx.Batch([r])
+        pb.set_rchunks(0)
+        pb.set_groups(1)
+        d, _ = timed_steps(rx, pb, rx.FP64, 3, 1)
+        lat.append(d)
+        pb.close()
+    plain64 = 1.0 / float(np.mean(lat))
+    log(f"plain fp64: {np.mean(lat) * 1e3:.3f} ms/spectrum -> {plain64:.1f} spectra/s")
+    # XSPEC entry point (host arrays in/out, includes PCIe and launch overhead)
+    t0 = time.perf_counter()
+    for r in reqs[:16]:
+        rx.lmodrelconv(r["energy"], r["params"], r["flux_in"])
+    lmod_ms = (time.perf_counter() - t0) / 16 * 1e3
+
+    # cfg2: lamp-post relline batch
+    b2 = rx.Batch(make_cfg2(1024))
+    b2.autotune(rx.FP32)
+    dt2, kms2 = timed_steps(rx, b2, rx.FP32, max(3, args.steps // 2), 1)
+    o2_32 = [b2.output(i)[0] for i in range(1024)]
+    timed_steps(rx, b2, rx.FP64, 1, 0)
+    o2_64 = [b2.output(i)[0] for i in range(1024)]
+    err2 = max(rel_err(o2_32[i], o2_64[i]) for i in range(1024))
+    cfg2 = dict(spectra_per_s=1024 / dt2, ms_per_step=dt2 * 1e3, max_rel_err_fp32_vs_fp64=err2)
+    log(f"cfg2 lp relline fp32: {cfg2}")
+    b2.close()
+
+    serving = None
+    if not args.no_serve:
+        qps = float(min(4000.0, 0.4 * value))
+        serving = serve(rx, args.serve_seconds, qps)
+        log(f"serving: {serving}")
+
+    # roofline of the dominant kernel (relline integration): algorithmic bytes
+    # per spectrum = table corners 68,800 B + grid/in/out 16 N + 8 B
+    alg_bytes = float(np.sum(68800 + 16 * nbins + 8))
+    t_int = kms32[1] * 1e-3
+    achieved = alg_bytes / t_int / 1e9
+    roofline = dict(bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=achieved / HBM_PEAK_GBS, traffic=None,
+                    kernel="k_integrate<1>", kernel_ms=round(kms32[1], 4),
+                    kernel_ms_all=dict(setup=round(kms32[0], 4), integrate=round(kms32[1], 4),
+                                       finalize=round(kms32[2], 4), conv=round(kms32[3], 4)))
+
+    cpu = None
+    if not args.no_cpu:
+        ncpu = min(16, os.cpu_count() or 1)
+        sample = list(range(min(args.cpu_sample, len(reqs))))
+        cb = cpu_baseline([reqs[i] for i in sample], ncpu, [out32[i] for i in sample])
+        log(f"cpu baseline: {cb}")
+        cpu = dict(value=round(cb["all_cores"], 2), unit="spectra/s", cores=ncpu, kind="port",
+                   sample=f"first {cb['n']} requests of the same batch, FP64 oracle, "
+                          f"{cb['cpu_seconds']:.1f} s of CPU work; single core "
+                          f"{cb['single_core']:.2f} spectra/s",
+                   single_core=round(cb["single_core"], 3))
+        max_err_oracle = cb["max_rel_err_fp32_vs_oracle"]
+    else:
+        max_err_oracle = None
+
+    res = {
+        "metric": "spectra/sec batched FP32 vs plain FP64 on 1 GPU @<=1e-4 rel err; p99 latency",
+        "value": round(value, 2),
+        "unit": "spectra/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt32 * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "uneven relconv batch: 4096 requests, own log grids of 300-8000 bins "
+                        "(log-uniform) over 0.1-100 keV, 4096-bin convolution grid 0.00035-2000 keV, "
+                        "25x22x100x20 relline table, FP32 + Kahan",
+            "n_requests": len(reqs), "mean_bins": float(nbins.mean()), "radius_chunks": rch,
+            "radius_groups": grp,
+        },
+        "plain_fp64_spectra_per_s": round(plain64, 2),
+        "ratio_vs_plain_fp64": round(value / plain64, 2),
+        "batched_fp64_spectra_per_s": round(len(reqs) / dt64b, 2),
+        "lmodrelconv_ms_per_call": round(lmod_ms, 3),
+        "max_rel_err_fp32_vs_fp64_gpu": err32_64,
+        "max_rel_err_fp32_vs_oracle": max_err_oracle,
+        "lp_relline_batch": cfg2,
+        "serving": serving,
+        "p99_latency_ms": serving["p99_ms"] if serving else None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        res["ratio_vs_cpu_all_cores"] = round(value / cpu["value"], 1)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -125,6 +125,10 @@ int rx_tables_view(rx_reltable_view *rel, rx_lptable_view *lp);
  * (<= 8192).  Fine radial grid size (reference N_FRAD = 1000). */
 int rx_set_conv_grid(int n, double emin, double emax);
 int rx_set_nr_fine(int nr_fine);
+/* FP32 path: relative guard band around the Romberg stop threshold (default
+ * 2e-3); stop tests inside it are redone in FP64.  Widening it exercises the
+ * FP64 re-check path (diagnostics). */
+int rx_set_fp32_guard(double rel);
 
 /* ===================================================================== */
 /* batched API: inputs resident in HBM, many parameter sets and uneven    */
@@ -156,8 +160,10 @@ int rx_batch_status(rx_batch *b, int i);
 int rx_batch_dist(rx_batch *b, int i, double *dist, int n_incl);
 /* lamp-post reflection fractions of request i: refl_frac, f_bh, f_ad, f_inf, f_inf_rest */
 int rx_batch_reflfrac(rx_batch *b, int i, double *out5);
-/* tile shape: radius chunks per request (0: automatic / autotuned) */
+/* tile shape: radius chunks per request (0: automatic / autotuned) and
+ * radius groups per wavefront (1, 2 or 4; 0: automatic) */
 int rx_batch_set_rchunks(rx_batch *b, int n_rch);
+int rx_batch_set_groups(rx_batch *b, int groups);
 void rx_batch_destroy(rx_batch *b);
 
 /* Resident reflection set for relxill-style requests: n_incl spectra of
@@ -169,11 +175,17 @@ int rx_reflset_create(const double *refl_energy, int n_refl, const double *refl,
  * (0 setup, 1 integrate, 2 finalize, 3 conv) the duration in ms measured with
  * HIP events on the batch's stream. */
 int rx_profile_enable(int on);
+/* Streams `bytes` through a plain device copy kernel `reps` times (after one
+ * warm-up) and returns the average kernel time; used to calibrate HBM byte
+ * counters and as a bandwidth reference. */
+int rx_copy_test(int64_t bytes, int reps, double *ms);
+/* Waits for all work of the library on its device. */
+int rx_device_synchronize(void);
 int rx_batch_kernel_ms(rx_batch *b, double *ms4);
 
-/* Autotuner: times candidate tile shapes for this batch on the device and
- * keeps the fastest for batches of the same shape class.  Returns the chosen
- * radius-chunk count. */
+/* Autotuner: times candidate tile shapes (radius chunks x radius groups) for
+ * this batch on the device and keeps the fastest for batches of the same
+ * shape class.  Returns 16 * radius_chunks + groups (>0) or -status. */
 int rx_batch_autotune(rx_batch *b, int precision);
 
 /* ===================================================================== */

@@ -95,6 +95,7 @@ _EXPORTS = {
     "rx_tables_view": (ctypes.c_int, [ctypes.POINTER(_RelView), ctypes.POINTER(_LpView)]),
     "rx_set_conv_grid": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double]),
     "rx_set_nr_fine": (ctypes.c_int, [ctypes.c_int]),
+    "rx_set_fp32_guard": (ctypes.c_int, [ctypes.c_double]),
     "rx_batch_create": (ctypes.c_void_p, [ctypes.POINTER(_Request), ctypes.c_int]),
     "rx_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "rx_batch_sync": (ctypes.c_int, [ctypes.c_void_p]),
@@ -103,9 +104,12 @@ _EXPORTS = {
     "rx_batch_dist": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_int]),
     "rx_batch_reflfrac": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D]),
     "rx_batch_set_rchunks": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "rx_batch_set_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "rx_batch_destroy": (None, [ctypes.c_void_p]),
     "rx_reflset_create": (ctypes.c_int, [_D, ctypes.c_int, _D, ctypes.c_int]),
     "rx_profile_enable": (ctypes.c_int, [ctypes.c_int]),
+    "rx_device_synchronize": (ctypes.c_int, []),
+    "rx_copy_test": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _D]),
     "rx_batch_kernel_ms": (ctypes.c_int, [ctypes.c_void_p, _D]),
     "rx_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "rx_server_start": (ctypes.c_void_p, [ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int]),
@@ -294,11 +298,15 @@ class Batch:
     def set_rchunks(self, n):
         _check(lib.rx_batch_set_rchunks(self.h, n))
 
+    def set_groups(self, g):
+        _check(lib.rx_batch_set_groups(self.h, g))
+
     def autotune(self, precision):
+        """returns (radius_chunks, groups)"""
         r = lib.rx_batch_autotune(self.h, precision)
         if r <= 0:
-            raise RxError(r, lib.rx_last_error().decode())
-        return r
+            raise RxError(-r, lib.rx_last_error().decode())
+        return r // 16, r % 16
 
     def kernel_ms(self):
         o = np.zeros(4)
@@ -313,5 +321,69 @@ class Batch:
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching serving loop (rx_server_*)."""
+
+    def __init__(self, max_inflight=4096, hbm_budget=8 << 30, n_streams=4, precision=FP32):
+        h = lib.rx_server_start(max_inflight, hbm_budget, n_streams, precision)
+        if not h:
+            raise RxError(lib.rx_last_status(), lib.rx_last_error().decode())
+        self.h = h
+        self._nflux = {}
+
+    def submit(self, model, energy, params, flux_in=None, refl_set=-1):
+        e = _arr(energy)
+        p = _arr(params)
+        r = _Request()
+        r.model = model
+        r.n_flux = e.size - 1
+        r.energy = _dp(e)
+        r.parameter = _dp(p)
+        f = None
+        if flux_in is not None:
+            f = _arr(flux_in)
+            r.flux_in = _dp(f)
+        else:
+            r.flux_in = None
+        r.refl_set = refl_set
+        t = lib.rx_server_submit(self.h, ctypes.byref(r))
+        if t < 0:
+            raise RxError(-t, "submit failed")
+        self._nflux[t] = e.size - 1
+        return t
+
+    def poll(self, ticket):
+        """None while pending, else (flux, status, latency_ms)."""
+        out = np.zeros(self._nflux[ticket])
+        st = ctypes.c_int(0)
+        lat = np.zeros(1)
+        r = lib.rx_server_poll(self.h, ticket, _dp(out), ctypes.byref(st), _dp(lat))
+        if r == 0:
+            return None
+        if r < 0:
+            raise RxError(r, "unknown ticket")
+        del self._nflux[ticket]
+        return out, st.value, float(lat[0])
+
+    def drain(self):
+        _check(lib.rx_server_drain(self.h))
+
+    def stats(self):
+        o = (ctypes.c_int64 * 3)()
+        _check(lib.rx_server_stats(self.h, o))
+        return dict(completed=o[0], batches=o[1], max_concurrent_batches=o[2])
+
+    def stop(self):
+        if getattr(self, "h", None):
+            lib.rx_server_stop(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.stop()
         except Exception:
             pass

@@ -353,7 +353,7 @@ int rx_batch_set_rchunks(rx_batch *b, int n_rch) {
   Global &g = G();
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   b->b.n_rch_forced = n_rch;
-  batch_plan(b->b, n_rch);
+  batch_plan(b->b, n_rch, b->b.groups_forced);
   return batch_upload(b->b);
 }
 
@@ -363,6 +363,43 @@ void rx_batch_destroy(rx_batch *b) {
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   if (b->b.stream) (void)hipStreamSynchronize(b->b.stream);
   delete b;
+}
+
+int rx_copy_test(int64_t bytes, int reps, double *ms) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  std::string err;
+  int st = ensure_device(err);
+  if (st != RX_OK || bytes <= 0 || reps <= 0) return st != RX_OK ? st : RX_ERR_ARG;
+  DevBuf a, b;
+  if (a.ensure((size_t)bytes) || b.ensure((size_t)bytes)) return RX_ERR_NOMEM;
+  (void)hipMemset(a.p, 1, (size_t)bytes);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  int e = launch_copy(a.p, b.p, (size_t)bytes, nullptr);
+  (void)hipEventRecord(e0, nullptr);
+  for (int r = 0; r < reps; r++) e |= launch_copy(a.p, b.p, (size_t)bytes, nullptr);
+  (void)hipEventRecord(e1, nullptr);
+  (void)hipEventSynchronize(e1);
+  float t = 0;
+  (void)hipEventElapsedTime(&t, e0, e1);
+  if (ms) *ms = t / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return e ? RX_ERR_DEVICE : RX_OK;
+}
+
+int rx_device_synchronize(void) {
+  Global &g = G();
+  if (!g.device_ready) return RX_OK;
+  return hipDeviceSynchronize() == hipSuccess ? RX_OK : RX_ERR_DEVICE;
+}
+
+int rx_set_fp32_guard(double rel) {
+  if (!(rel >= 0.0 && rel < 1.0)) return RX_ERR_ARG;
+  G().guard = (float)rel;
+  return RX_OK;
 }
 
 int rx_profile_enable(int on) {
@@ -376,41 +413,59 @@ int rx_batch_kernel_ms(rx_batch *b, double *ms4) {
   return b->b.profiled ? RX_OK : RX_ERR_ARG;
 }
 
+int rx_batch_set_groups(rx_batch *b, int groups) {
+  if (!b || (groups != 0 && groups != 1 && groups != 2 && groups != 4)) return RX_ERR_ARG;
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  b->b.groups_forced = groups;
+  batch_plan(b->b, b->b.n_rch_forced, groups);
+  return batch_upload(b->b);
+}
+
+// Times every (radius chunks, radius groups) candidate on this batch and keeps
+// the fastest for batches of the same shape class (log2 of the bin chunks and
+// whether convolutions are present).
 int rx_batch_autotune(rx_batch *b, int precision) {
   if (!b) return RX_ERR_ARG;
   Global &g = G();
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   std::string err;
-  const int cands[] = {1, 2, 4, 8, 16, 32};
+  const int rchs[] = {1, 2, 4, 8, 16, 32};
+  const int grps[] = {1, 2, 4};
   double best = 1e30;
-  int best_r = 1;
-  for (int r : cands) {
-    batch_plan(b->b, r);
-    if (batch_upload(b->b) != RX_OK) return RX_ERR_DEVICE;
-    if (batch_run(b->b, precision, err) != RX_OK || batch_sync(b->b) != RX_OK) return RX_ERR_DEVICE;
-    auto t0 = std::chrono::steady_clock::now();
-    const int reps = 2;
-    for (int k = 0; k < reps; k++) {
-      if (batch_run(b->b, precision, err) != RX_OK) return RX_ERR_DEVICE;
-    }
-    if (batch_sync(b->b) != RX_OK) return RX_ERR_DEVICE;
-    double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / reps;
-    if (dt < best * 0.97) {
-      best = dt;
-      best_r = r;
+  int best_r = 1, best_g = 1;
+  for (int gr : grps) {
+    for (int r : rchs) {
+      batch_plan(b->b, r, gr);
+      if (batch_upload(b->b) != RX_OK) return -RX_ERR_DEVICE;
+      if (batch_run(b->b, precision, err) != RX_OK || batch_sync(b->b) != RX_OK) return -RX_ERR_DEVICE;
+      auto t0 = std::chrono::steady_clock::now();
+      const int reps = 2;
+      for (int k = 0; k < reps; k++) {
+        if (batch_run(b->b, precision, err) != RX_OK) return -RX_ERR_DEVICE;
+      }
+      if (batch_sync(b->b) != RX_OK) return -RX_ERR_DEVICE;
+      double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / reps;
+      if (dt < best * 0.97) {
+        best = dt;
+        best_r = r;
+        best_g = gr;
+      }
     }
   }
   bool found = false;
   for (auto &t : g.tune)
-    if (t.first == b->b.shape_class) {
-      t.second = best_r;
+    if (t.shape == b->b.shape_class) {
+      t.n_rch = best_r;
+      t.groups = best_g;
       found = true;
     }
-  if (!found) g.tune.push_back({b->b.shape_class, best_r});
+  if (!found) g.tune.push_back({b->b.shape_class, best_r, best_g});
   b->b.n_rch_forced = best_r;
-  batch_plan(b->b, best_r);
-  if (batch_upload(b->b) != RX_OK) return RX_ERR_DEVICE;
-  return best_r;
+  b->b.groups_forced = best_g;
+  batch_plan(b->b, best_r, best_g);
+  if (batch_upload(b->b) != RX_OK) return -RX_ERR_DEVICE;
+  return best_r * 16 + best_g;
 }
 
 /* ---- reflection sets ---- */

@@ -30,6 +30,8 @@ struct Batch {
   int n_req = 0;
   int n_rch = 1;
   int n_rch_forced = 0;
+  int groups = 1;         // radius groups per wavefront
+  int groups_forced = 0;
   std::vector<ReqParams> h_req;
   std::vector<int> h_status;  // host-side parse status
   std::vector<double> h_ener, h_flux_in;
@@ -39,6 +41,12 @@ struct Batch {
   int64_t tot_ener = 0, tot_flux = 0, tot_pgrid = 0, tot_part = 0, tot_prof = 0;
   DevBuf d_req, d_ener, d_flux_in, d_out, d_pgrid, d_node, d_rad, d_part, d_prof, d_dist,
       d_refl, d_status, d_tiles, d_conv;
+  DevBuf d_irec, d_f32, d_f32c, d_f64, d_f64c, d_chunk, d_fix, d_fixc;
+  int64_t tot_chunks = 0;
+  bool any_cosne = false, any_limb = false;
+  int fix_cap = 0;
+  int last_mode = 1;
+  bool fix_overflow = false;
   hipStream_t stream = nullptr;
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   bool profiled = false;
@@ -49,8 +57,9 @@ struct Batch {
 
 // Fills b (re-using device buffers when large enough) from the requests.
 int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err);
-// Plans tiles for n_rch radius chunks (0: automatic).
-void batch_plan(Batch &b, int n_rch);
+// Plans tiles for n_rch radius chunks and `groups` radius groups per
+// wavefront (0: automatic / autotuned).
+void batch_plan(Batch &b, int n_rch, int groups);
 int batch_upload(Batch &b);
 int batch_run(Batch &b, int precision, std::string &err);
 int batch_sync(Batch &b);
@@ -64,6 +73,7 @@ struct Global {
   LpTableHost lp;
   bool rel_loaded = false, lp_loaded = false;
   bool rel_dirty = true, lp_dirty = true;
+  double rel_gmin_all = 0.0, rel_gmax_all = 1e30;  // g range reachable by any window
   DevBuf t_a, t_mu0, t_r, t_gmin, t_gmax, t_trff1, t_trff2, t_cosne1, t_cosne2;
   DevBuf l_a, l_h, l_rad, l_int, l_del, l_inc;
   // convolution grid
@@ -80,8 +90,12 @@ struct Global {
   DevBuf d_refl_sets;
   bool refl_dirty = false;
   bool profile = false;
-  // autotune cache: shape class -> n_rch
-  std::vector<std::pair<int, int>> tune;
+  float guard = 2e-3f;
+  // autotune cache: shape class -> (n_rch, groups)
+  struct Tuned {
+    int shape, n_rch, groups;
+  };
+  std::vector<Tuned> tune;
 };
 
 Global &G();

@@ -18,6 +18,8 @@
 //                (src/Relbase.cpp:93-289)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "rx_engine.h"
 #include "rx_math.h"
 
@@ -115,6 +117,10 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b) {
   const DevRelTable &T = b.rt;
   const int nr = b.nr_fine;
   const int n_r = T.n_r, ng = T.n_g;
+  if (req == 0 && tid == 0) {  // reset the FP64 re-check queue of this run
+    b.fix_count[0] = 0;
+    b.fix_count[1] = 0;
+  }
   if (rq.pre_status != 0) return;  // status already set by the host
 
   __shared__ double s_tre[MAX_NR_TAB], s_tgmin[MAX_NR_TAB], s_tgmax[MAX_NR_TAB];
@@ -330,6 +336,16 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b) {
     rad[ii] = r;
   }
 
+  // chunk radius ranges start empty; k_fine fills them
+  {
+    int *cr = b.chunk_rng + 2 * rq.chunk_off;
+    const int n_chunks = (rq.n_e + 63) / 64;
+    for (int c = tid; c < n_chunks; c += SETUP_THREADS) {
+      cr[2 * c] = 0x7fffffff;
+      cr[2 * c + 1] = -1;
+    }
+  }
+
   // reflection fractions (calc_refl_frac, src/Rellp.cpp:36-89)
   if (lp && tid == 0) {
     auto fr = [&](int ii) { return fine_radius(rq.rin, rq.rout, ii, nr); };
@@ -407,40 +423,162 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b) {
 // ---------------------------------------------------------------------------
 // integrate: the hot loop
 // ---------------------------------------------------------------------------
-template <int MODE>
-__global__ __launch_bounds__(64) void k_integrate(BatchDev b, const Tile *__restrict__ tiles) {
-  __shared__ double sT64[2][2 * MAX_NG];
-  __shared__ double sC64[2][2 * MAX_NG];
-  __shared__ float sT32[2][2 * MAX_NG];
-  __shared__ float sC32[2][2 * MAX_NG];
+// ---------------------------------------------------------------------------
+// fine: per fine radius, the transfer function (and cosines) lerped from the
+// table node onto the fine grid (src/Relprofile.cpp:280-293), the FP64 values
+// of the two edge brackets, the compact integration record, and the chunk
+// radius ranges.  One wavefront per radius (lanes over the g*-branch values).
+// ---------------------------------------------------------------------------
+constexpr int FINE_RADII_PER_BLOCK = 64;
 
-  const Tile tl = tiles[blockIdx.x];
-  const int req = tl.req;
-  const int lane = threadIdx.x;
+__global__ __launch_bounds__(256) void k_fine(BatchDev b, int blocks_per_req) {
+  __shared__ double sT[4][2 * MAX_NG];
+  __shared__ double sC[4][2 * MAX_NG];
+  const int req = blockIdx.x / blocks_per_req;
+  const int rb = blockIdx.x - req * blocks_per_req;
+  if (b.status[req] != 0) return;
   const ReqParams &rq = b.req[req];
-  const int n_e = rq.n_e;
-  const int j0 = tl.bchunk * 64;
-  const int j = j0 + lane;
-  double *part = b.part + rq.part_off + (size_t)tl.rch * n_e;
-  if (b.status[req] != 0) {
-    if (j < n_e) part[j] = 0.0;
-    return;
-  }
   const int nr = b.nr_fine;
   const int ng = b.rt.n_g;
   const int nq = 2 * ng;
+  const size_t nq2 = (size_t)b.rt.n_r * nq;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const double *node = b.node + (size_t)req * b.node_stride;
+  const size_t fbase = (size_t)req * nr * nq;
+  const bool cos_needed = rq.need_cosne != 0;
   const int limb = rq.limb;
-  const double *E = (rq.pgrid_off >= 0) ? b.pgrid + rq.pgrid_off : b.cg.ener;
+  int *cr = b.chunk_rng + 2 * rq.chunk_off;
+  const int r_beg = rb * FINE_RADII_PER_BLOCK;
+  const int r_end = min(nr, r_beg + FINE_RADII_PER_BLOCK);
+  const double step = (1.0 - 2 * GFAC_H) / (ng - 1);
+  for (int base = r_beg; base < r_end; base += 4) {  // same trip count in every wave
+    const int ii = base + wave;
+    const bool act = ii < r_end;
+    RadRec r;
+    if (act) r = b.rad[(size_t)req * nr + ii];
+    if (act) {
+      for (int q = lane; q < nq; q += 64) {
+        const double t = lerp1(r.ifac_r, node[(size_t)(r.K + 1) * nq + q], node[(size_t)r.K * nq + q]);
+        sT[wave][q] = t;
+        b.fine32[fbase + (size_t)ii * nq + q] = (float)t;
+        if (b.mode == 0) b.fine64[fbase + (size_t)ii * nq + q] = t;
+        if (cos_needed) {
+          const double c = lerp1(r.ifac_r, node[nq2 + (size_t)(r.K + 1) * nq + q], node[nq2 + (size_t)r.K * nq + q]);
+          sC[wave][q] = c;
+          b.fine32c[fbase + (size_t)ii * nq + q] = (float)c;
+          if (b.mode == 0 && limb) b.fine64c[fbase + (size_t)ii * nq + q] = c;
+        }
+      }
+    }
+    __syncthreads();
+    if (act && lane == 0) {
+      RadiusView v;
+      v.gmin = r.gmin;
+      v.gmax = r.gmax;
+      v.del_g = r.del_g;
+      v.T64 = sT[wave];
+      v.C64 = sC[wave];
+      v.T32 = nullptr;
+      v.C32 = nullptr;
+      v.ng = ng;
+      v.limb = limb;
+      v.gs_step = step;
+      v.inv_step = 1.0 / step;
+      v.guard = 0.0f;
+      IntRec w;
+      w.gmin = r.gmin;
+      w.gmax = r.gmax;
+      w.del_g = r.del_g;
+      w.coef = r.coef;
+      w.normL = edge_norm64(false, v);
+      w.normU = edge_norm64(true, v);
+      w.ielo = r.ielo;
+      w.iehi = r.iehi;
+      b.irec[(size_t)req * nr + ii] = w;
+      if (r.ielo <= r.iehi) {
+        for (int c = r.ielo >> 6; c <= (r.iehi >> 6); c++) {
+          atomicMin(&cr[2 * c], ii);
+          atomicMax(&cr[2 * c + 1], ii);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <class T>
+__device__ inline T ld_const(const T *p) {
+  // uniform read-only data: scalar loads through the constant address space
+#if defined(__HIP_DEVICE_COMPILE__)
+  return *(const __attribute__((address_space(4))) T *)(p);
+#else
+  return *p;
+#endif
+}
+
+// Tile = (request, W-bin chunk, radius chunk), W = 64/G.  Lane l works on bin
+// chunk*W + l%W for the radius group l/W: per iteration the wavefront stages G
+// consecutive fine radii (their transfer-function values are contiguous in
+// memory) and each group integrates its radius for its W bins.  The next
+// iteration's values and records are prefetched while the current one is
+// integrated.  Sums: FP64 (MODE 0) or Kahan-compensated FP32 (MODE 1) per lane,
+// combined over the groups in a fixed order at the end.
+template <int MODE, int G>
+__global__ __launch_bounds__(64) void k_integrate(BatchDev b, const Tile *__restrict__ tiles) {
+  constexpr int W = 64 / G;
+  constexpr int UQ = (G * 2 * MAX_NG + 63) / 64;  // value slots per lane
+  extern __shared__ double s_dyn[];
+
+  const Tile tl = ld_const(tiles + blockIdx.x);
+  const int req = tl.req;
+  const int lane = threadIdx.x;
+  const int grp = lane / W;
+  const int bl = lane - grp * W;
+  const ReqParams *rqp = b.req + req;
+  const int n_e = ld_const(&rqp->n_e);
+  const int limb = ld_const(&rqp->limb);
+  const int64_t part_off = ld_const(&rqp->part_off);
+  const int64_t pgrid_off = ld_const(&rqp->pgrid_off);
+  const int64_t chunk_off = ld_const(&rqp->chunk_off);
+  const int j0 = tl.bchunk * W;
+  const int j = j0 + bl;
   const bool valid = j < n_e;
+  double *part = b.part + part_off + (size_t)tl.rch * n_e;
+  const int nr = b.nr_fine;
+  const int rlen = (nr + b.n_rch - 1) / b.n_rch;
+  int i0 = tl.rch * rlen;
+  int i1 = min(nr, i0 + rlen);
+  if (ld_const(b.status + req) != 0) {
+    i1 = i0;
+  } else {
+    const int c64 = j0 >> 6;
+    const int first = ld_const(b.chunk_rng + 2 * (chunk_off + c64));
+    const int last = ld_const(b.chunk_rng + 2 * (chunk_off + c64) + 1);
+    i0 = max(i0, first);
+    i1 = min(i1, last + 1);
+  }
+  if (i0 >= i1) {
+    if (valid && grp == 0) part[j] = 0.0;
+    return;
+  }
+  const int ng = b.rt.n_g;
+  const int nq = 2 * ng;
+  const int gq = G * nq;
+  const double *E = (pgrid_off >= 0) ? b.pgrid + pgrid_off : b.cg.ener;
   const double elo = valid ? E[j] : 0.0;
   const double ehi = valid ? E[j + 1] : 0.0;
-  const RadRec *rad = b.rad + (size_t)req * nr;
-  const double *node = b.node + (size_t)req * b.node_stride;
-  const double *cnode = node + (size_t)b.rt.n_r * ng * 2;
-  const int rlen = (nr + b.n_rch - 1) / b.n_rch;
-  const int i0 = tl.rch * rlen;
-  const int i1 = min(nr, i0 + rlen);
-  const int jlast = j0 + 63;
+  const IntRec *irec = b.irec + (size_t)req * nr;
+  const size_t fbase = (size_t)req * nr * nq;
+  const float *f32 = b.fine32 + fbase;
+  const float *f32c = limb ? b.fine32c + fbase : nullptr;
+  const double *f64 = (MODE == 0) ? b.fine64 + fbase : nullptr;
+  const double *f64c = (MODE == 0 && limb) ? b.fine64c + fbase : nullptr;
+
+  // LDS per buffer: doubles first, then floats
+  const int nd = (MODE == 0) ? gq * (limb ? 2 : 1) : 0;
+  const int nf = (MODE == 1) ? gq * (limb ? 2 : 1) : 0;
+  const int buf_doubles = nd + (nf + 1) / 2;
 
   const double step = (1.0 - 2 * GFAC_H) / (ng - 1);
   RadiusView v;
@@ -448,47 +586,155 @@ __global__ __launch_bounds__(64) void k_integrate(BatchDev b, const Tile *__rest
   v.limb = limb;
   v.gs_step = step;
   v.inv_step = 1.0 / step;
+  v.guard = b.guard;
 
-  double acc = 0.0;     // FP64 mode
-  float accf = 0.0f;    // FP32 mode: Kahan sum
-  float comp = 0.0f;
-  int buf = 0;
-  for (int i = i0; i < i1; i++) {
-    const RadRec r = rad[i];
-    if (r.iehi < j0 || r.ielo > jlast || r.ielo > r.iehi) continue;
-    for (int q = lane; q < nq; q += 64) {
-      const double t = lerp1(r.ifac_r, node[(size_t)(r.K + 1) * nq + q], node[(size_t)r.K * nq + q]);
-      sT64[buf][q] = t;
-      sT32[buf][q] = (float)t;
-      if (limb) {
-        const double c = lerp1(r.ifac_r, cnode[(size_t)(r.K + 1) * nq + q], cnode[(size_t)r.K * nq + q]);
-        sC64[buf][q] = c;
-        sC32[buf][q] = (float)c;
+  float pt[UQ], pc[UQ];
+  double pd[UQ], pdc[UQ];
+  IntRec nrec;
+  auto fetch = [&](int base) {
+    const size_t o = (size_t)base * nq;
+    const int lim = (i1 - base) * nq;  // values of radii < i1
+#pragma unroll
+    for (int u = 0; u < UQ; u++) {
+      const int k = lane + 64 * u;
+      if (k < gq && k < lim) {
+        if (MODE == 1) {
+          pt[u] = f32[o + k];
+          if (limb) pc[u] = f32c[o + k];
+        } else {
+          pd[u] = f64[o + k];
+          if (limb) pdc[u] = f64c[o + k];
+        }
       }
     }
+    const int mine = base + grp;
+    if (mine < i1) nrec = irec[mine];
+  };
+  fetch(i0);
+
+  double acc = 0.0;
+  float accf = 0.0f, comp = 0.0f;
+  int buf = 0;
+  for (int base = i0; base < i1; base += G) {
+    double *bd = s_dyn + buf * buf_doubles;
+    float *bf = reinterpret_cast<float *>(bd + nd);
+#pragma unroll
+    for (int u = 0; u < UQ; u++) {
+      const int k = lane + 64 * u;
+      if (k < gq) {
+        if (MODE == 1) {
+          bf[k] = pt[u];
+          if (limb) bf[gq + k] = pc[u];
+        } else {
+          bd[k] = pd[u];
+          if (limb) bd[gq + k] = pdc[u];
+        }
+      }
+    }
+    const IntRec r = nrec;
+    const int i = base + grp;
     __syncthreads();
-    if (valid && j >= r.ielo && j <= r.iehi) {
+    if (base + G < i1) fetch(base + G);
+    if (i < i1 && valid && j >= r.ielo && j <= r.iehi) {
       v.gmin = r.gmin;
       v.gmax = r.gmax;
       v.del_g = r.del_g;
-      v.T64 = sT64[buf];
-      v.C64 = sC64[buf];
-      v.T32 = sT32[buf];
-      v.C32 = sC32[buf];
+      v.normL = r.normL;
+      v.normU = r.normU;
       if (MODE == 0) {
+        v.T64 = bd + grp * nq;
+        v.C64 = bd + gq + grp * nq;
+        v.T32 = nullptr;
+        v.C32 = nullptr;
         acc += integ_bin<0>(elo, ehi, v) * r.coef;
       } else {
-        const double val = integ_bin<1>(elo, ehi, v) * r.coef;
-        // Kahan-compensated FP32 accumulation over radii
-        const float y = (float)val - comp;
+        v.T64 = nullptr;
+        v.C64 = nullptr;
+        v.T32 = bf + grp * nq;
+        v.C32 = bf + gq + grp * nq;
+        bool need64 = false;
+        const float c32 = (float)(integ_bin<1>(elo, ehi, v, &need64) * r.coef);
+        const float y = c32 - comp;  // Kahan-compensated FP32 sum over radii
         const float t = accf + y;
         comp = (t - accf) - y;
         accf = t;
+        if (need64) {
+          const int slot = atomicAdd(&b.fix_count[0], 1);
+          if (slot < b.fix_cap) {
+            FixEntry fe;
+            fe.req = req;
+            fe.rad = i;
+            fe.bin = j;
+            fe.rch = tl.rch;
+            fe.v32 = (double)c32;
+            b.fix[slot] = fe;
+          } else {
+            atomicOr(&b.fix_count[1], 1);
+          }
+        }
       }
     }
     buf ^= 1;
   }
-  if (valid) part[j] = (MODE == 0) ? acc : ((double)accf - (double)comp);
+  // combine the radius groups of each bin in group order
+  double mine = (MODE == 0) ? acc : ((double)accf - (double)comp);
+  if (G > 1) {
+    __syncthreads();
+    s_dyn[lane] = mine;
+    __syncthreads();
+    if (grp == 0) {
+      mine = 0.0;
+#pragma unroll
+      for (int gg = 0; gg < G; gg++) mine += s_dyn[gg * W + bl];
+    }
+  }
+  if (valid && grp == 0) part[j] = mine;
+}
+
+// FP64 re-check of guarded FP32 bins: one wavefront per entry, grid-stride over
+// the queue filled by k_integrate<1>.
+__global__ __launch_bounds__(64) void k_fixup(BatchDev b) {
+  __shared__ double sT64[2 * MAX_NG];
+  __shared__ double sC64[2 * MAX_NG];
+  const int n = min(b.fix_count[0], b.fix_cap);
+  const int lane = threadIdx.x;
+  const int ng = b.rt.n_g;
+  const int nq = 2 * ng;
+  const size_t nq2 = (size_t)b.rt.n_r * nq;
+  const double step = (1.0 - 2 * GFAC_H) / (ng - 1);
+  for (int e = blockIdx.x; e < n; e += gridDim.x) {
+    const FixEntry fe = b.fix[e];
+    const ReqParams &rq = b.req[fe.req];
+    const RadRec r = b.rad[(size_t)fe.req * b.nr_fine + fe.rad];
+    const IntRec ir = b.irec[(size_t)fe.req * b.nr_fine + fe.rad];
+    const double *node = b.node + (size_t)fe.req * b.node_stride;
+    for (int q = lane; q < nq; q += 64) {
+      sT64[q] = lerp1(r.ifac_r, node[(size_t)(r.K + 1) * nq + q], node[(size_t)r.K * nq + q]);
+      if (rq.limb) sC64[q] = lerp1(r.ifac_r, node[nq2 + (size_t)(r.K + 1) * nq + q], node[nq2 + (size_t)r.K * nq + q]);
+    }
+    __syncthreads();
+    if (lane == 0) {
+      const double *E = (rq.pgrid_off >= 0) ? b.pgrid + rq.pgrid_off : b.cg.ener;
+      RadiusView v;
+      v.ng = ng;
+      v.limb = rq.limb;
+      v.gs_step = step;
+      v.inv_step = 1.0 / step;
+      v.guard = b.guard;
+      v.gmin = r.gmin;
+      v.gmax = r.gmax;
+      v.del_g = r.del_g;
+      v.normL = ir.normL;
+      v.normU = ir.normU;
+      v.T64 = sT64;
+      v.C64 = sC64;
+      v.T32 = nullptr;
+      v.C32 = nullptr;
+      const double val = integ_bin<0>(E[fe.bin], E[fe.bin + 1], v) * r.coef;
+      atomicAdd(&b.part[rq.part_off + (size_t)fe.rch * rq.n_e + fe.bin], val - fe.v32);
+    }
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -511,7 +757,9 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_finalize(BatchDev b) {
   double loc = 0.0;
   for (int j = tid; j < n_e; j += SETUP_THREADS) {
     double v = 0.0;
-    for (int c = 0; c < b.n_rch; c++) v += part[(size_t)c * n_e + j];
+    if (j >= rq.tile_j0 && j < rq.tile_j1) {
+      for (int c = 0; c < b.n_rch; c++) v += part[(size_t)c * n_e + j];
+    }
     v /= 0.5 * (E[j] + E[j + 1]);
     dst[j] = v;
     loc += v;
@@ -754,6 +1002,13 @@ __global__ __launch_bounds__(CONV_THREADS) void k_refl_prepare(DevConvGrid cg, c
   }
 }
 
+// plain streaming copy, used to calibrate the HBM byte counters
+__global__ __launch_bounds__(256) void k_copy(const double4 *__restrict__ src, double4 *__restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    dst[i] = src[i];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -763,13 +1018,43 @@ int launch_setup(const BatchDev &b, void *stream) {
   return (int)hipGetLastError();
 }
 
-int launch_integrate(const BatchDev &b, const Tile *tiles, int n_tiles, int mode, void *stream) {
+template <int MODE, int G>
+static void launch_int(const BatchDev &b, const Tile *tiles, int n_tiles, hipStream_t st) {
+  const int nq = 2 * b.rt.n_g;
+  const int gq = G * nq;
+  const bool limb = true;  // size for the largest layout (cosines present)
+  const int nd = (MODE == 0) ? gq * (limb ? 2 : 1) : 0;
+  const int nf = (MODE == 1) ? gq * (limb ? 2 : 1) : 0;
+  const size_t per_buf = (size_t)(nd + (nf + 1) / 2);
+  const size_t lds = std::max<size_t>(2 * per_buf, 64) * sizeof(double);
+  hipLaunchKernelGGL((k_integrate<MODE, G>), dim3(n_tiles), dim3(64), lds, st, b, tiles);
+}
+
+int launch_integrate(const BatchDev &b, const Tile *tiles, int n_tiles, int mode, int groups, void *stream) {
   if (n_tiles <= 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
   if (mode == 0) {
-    hipLaunchKernelGGL(k_integrate<0>, dim3(n_tiles), dim3(64), 0, (hipStream_t)stream, b, tiles);
+    if (groups == 4) launch_int<0, 4>(b, tiles, n_tiles, st);
+    else if (groups == 2) launch_int<0, 2>(b, tiles, n_tiles, st);
+    else launch_int<0, 1>(b, tiles, n_tiles, st);
   } else {
-    hipLaunchKernelGGL(k_integrate<1>, dim3(n_tiles), dim3(64), 0, (hipStream_t)stream, b, tiles);
+    if (groups == 4) launch_int<1, 4>(b, tiles, n_tiles, st);
+    else if (groups == 2) launch_int<1, 2>(b, tiles, n_tiles, st);
+    else launch_int<1, 1>(b, tiles, n_tiles, st);
   }
+  return (int)hipGetLastError();
+}
+
+int launch_fine(const BatchDev &b, void *stream) {
+  if (b.n_req <= 0) return 0;
+  const int bpr = (b.nr_fine + FINE_RADII_PER_BLOCK - 1) / FINE_RADII_PER_BLOCK;
+  hipLaunchKernelGGL(k_fine, dim3(b.n_req * bpr), dim3(256), 0, (hipStream_t)stream, b, bpr);
+  return (int)hipGetLastError();
+}
+
+int launch_fixup(const BatchDev &b, void *stream) {
+  if (b.mode != 1 || b.n_req <= 0) return 0;
+  hipLaunchKernelGGL(k_fixup, dim3(1024), dim3(64), 0, (hipStream_t)stream, b);
   return (int)hipGetLastError();
 }
 
@@ -784,6 +1069,12 @@ int launch_conv(const BatchDev &b, const int *conv_reqs, int n_conv_reqs, void *
   const size_t lds = (size_t)b.cg.n * sizeof(double2);
   (void)hipFuncSetAttribute((const void *)k_conv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL(k_conv, dim3(n_conv_reqs), dim3(CONV_THREADS), lds, (hipStream_t)stream, b, conv_reqs);
+  return (int)hipGetLastError();
+}
+
+int launch_copy(const void *src, void *dst, size_t bytes, void *stream) {
+  const size_t n = bytes / sizeof(double4);
+  hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, (hipStream_t)stream, (const double4 *)src, (double4 *)dst, n);
   return (int)hipGetLastError();
 }
 

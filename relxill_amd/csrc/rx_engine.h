@@ -53,6 +53,8 @@ struct ReqParams {
   int64_t pgrid_off;  // profile grid edges in pgrid (-1: shared conv grid)
   int64_t part_off;   // partial profiles (n_rch * n_e)
   int64_t prof_off;   // final profile (n_e) for conv models, -1 otherwise
+  int64_t chunk_off;  // per 64-bin chunk radius ranges (int2) in chunk_rng
+  int tile_j0, tile_j1;  // bins covered by integration tiles (others are 0)
   int refl_set;       // relxill-style: which resident reflection set
   int pre_status;     // host-side parameter status; nonzero requests are skipped
 };
@@ -65,6 +67,13 @@ struct RadRec {
   int ielo;    // first bin of the window (ielo > iehi: radius does not contribute)
   int iehi;    // last bin of the window (inclusive)
   int pad;
+};
+
+// What the integration reads per fine radius (56 bytes): the window, the
+// factor emis*weight/(gmax-gmin) and the two int_edge norms.
+struct IntRec {
+  double gmin, gmax, del_g, coef, normL, normU;
+  int ielo, iehi;
 };
 
 struct LpRefl {
@@ -99,10 +108,22 @@ struct DevConvGrid {
   const double *tw;    // twiddles exp(-2 pi i k/n), k < n/2, interleaved re/im
 };
 
+struct Tile {
+  int req, bchunk, rch, pad;
+};
+
+// FP32 bins whose Romberg stop test fell in the guard band: recomputed in FP64
+// by k_fixup, which adds (FP64 value - FP32 value) to the partial profile.
+struct FixEntry {
+  int req, rad, bin, rch;
+  double v32;  // contribution already accumulated (float value, widened)
+};
+
 struct BatchDev {
   int n_req;
   int nr_fine;
   int n_rch;      // radius chunks per request
+  int groups;     // radius groups per wavefront (chunk width 64/groups bins)
   int node_stride;  // doubles per request in node (2 * n_r * n_g * 2)
   ReqParams *req;
   const double *ener_in;
@@ -111,26 +132,37 @@ struct BatchDev {
   double *pgrid;
   double *node;
   RadRec *rad;
+  IntRec *irec;
   double *part;
   double *prof;
   double *dist;
   LpRefl *refl;
   int *status;
+  // fine-radius transfer function [req][nr_fine][2*ng] (FP32 always, FP64 in
+  // FP64 mode); the *c arrays hold the emission-angle cosines (limb
+  // darkening / angle distribution), else null
+  float *fine32, *fine32c;
+  double *fine64, *fine64c;
+  int *chunk_rng;     // [sum of chunks][2]: first/last radius touching the chunk
+  FixEntry *fix;
+  int *fix_count;     // [0] entries, [1] overflow flag
+  int fix_cap;
+  int mode;           // 0 FP64, 1 FP32
+  float guard;        // FP32 Romberg guard band (relative)
   DevRelTable rt;
   DevLpTable lp;
   DevConvGrid cg;
   const DevReflSet *refl_sets;
 };
 
-struct Tile {
-  int req, bchunk, rch, pad;
-};
-
 // ---- launchers (kernels.hip) ----
 int launch_setup(const BatchDev &b, void *stream);
-int launch_integrate(const BatchDev &b, const Tile *tiles, int n_tiles, int mode, void *stream);
+int launch_fine(const BatchDev &b, void *stream);
+int launch_integrate(const BatchDev &b, const Tile *tiles, int n_tiles, int mode, int groups, void *stream);
+int launch_fixup(const BatchDev &b, void *stream);
 int launch_finalize(const BatchDev &b, void *stream);
 int launch_conv(const BatchDev &b, const int *conv_reqs, int n_conv_reqs, void *stream);
+int launch_copy(const void *src, void *dst, size_t bytes, void *stream);
 int launch_refl_prepare(const DevConvGrid &cg, const double *refl_ener, const double *refl,
                         int n_refl, int n_incl, double *spec, double *bandsum, double *fft,
                         void *stream);

@@ -1,6 +1,5 @@
 // rx_math.h -- per-element math of the relline path, shared by the HIP kernels
-// (compiled for gfx950) and by the host-compiled kernel-emulation used in CPU
-// unit tests of these building blocks.
+// (compiled for gfx950) and by host builds of the same functions.
 //
 // Reference correspondence (relxill 2.4):
 //   g* grid ................. src/Relprofile.cpp:104-116
@@ -17,14 +16,18 @@
 //     (a + pas*i with pas halved exactly), only the summation order differs;
 //   * the per-radius factor emis/(gmax-gmin) of relb_func is applied once per
 //     bin integral instead of once per evaluation;
+//   * the value of the integrand at g* = H and 1-H that int_edge needs is a
+//     per-radius constant, computed once per radius (edge norms);
 //   * the g* bracket is found by a direct index computation corrected to the
 //     exact binary_search result (src/relutility.c:155-171).
-// FP32 mode keeps every decision in FP64 (bin -> g* mapping, edge tests, g*
-// bracket, red-wing switch) and evaluates the integrand and the Romberg
-// tables in FP32; a Romberg stop test whose FP32 precision estimate falls
-// within a guard band of the 2% threshold is recomputed in FP64, so both modes
-// stop at the level the FP64 algorithm stops.  The 1/sqrt endpoint integrals
-// (int_edge) are FP64 in both modes.
+// FP32 mode keeps every decision in FP64 (bin -> g* mapping, edge tests,
+// red-wing switch) and evaluates the integrand in bin-local FP32 coordinates:
+// g*(s) = g*_lo + s dg for s in [0,1].  A point whose FP32 g* lies within a few
+// rounding errors of a g*-grid node is re-done with FP64 coordinates, so the
+// g* bracket always equals the FP64 one.  The Romberg tables run in FP32; a stop
+// test whose FP32 estimate falls within a guard band of the 2% threshold is
+// flagged and re-done in FP64 by the caller, so both modes stop at the level
+// the FP64 algorithm stops.  Edge integrals are FP64 in both modes.
 #pragma once
 
 #if defined(__HIPCC__)
@@ -44,22 +47,22 @@ constexpr double GFAC_H = 5e-3;          // src/Relprofile.cpp:36
 constexpr double ROMB_PREC = 0.02;       // src/Relprofile.cpp:525
 constexpr int ROMB_ITERMAX = 5;          // src/Relprofile.cpp:528 (niter <= 6)
 constexpr double RED_WING = 0.95;        // src/Relprofile.cpp:636
-constexpr float ROMB_GUARD = 2e-3f;      // relative guard band around ROMB_PREC (FP32 mode)
+constexpr float ROMB_GUARD = 2e-3f;      // default relative guard band (FP32 mode)
 constexpr double PI = 3.14159265358979323846;
 
-// Per-radius view used by the bin integral.  T64/C64 hold the fine-radius
-// transfer function / emission-angle cosines [ng][2] in FP64, T32/C32 the same
-// rounded to FP32 (FP32 mode only).
+// Per-radius view used by the bin integral.
 struct RadiusView {
   double gmin, gmax, del_g;
-  const double *T64;
-  const double *C64;
-  const float *T32;
+  double normL, normU;  // sqrt(H) * sum_k f_k at g* = H and g* = 1-H (reduced units)
+  const double *T64;    // FP64 mode: transfer function [ng][2]
+  const double *C64;    // FP64 mode: emission-angle cosines [ng][2] (limb != 0)
+  const float *T32;     // FP32 mode
   const float *C32;
   int ng;
   int limb;  // 0 none, 1 Laor, 2 Haardt
   double gs_step;   // (1-2H)/(ng-1)
   double inv_step;  // 1/gs_step
+  float guard;      // FP32 mode: relative guard band around the Romberg threshold
 };
 
 RX_HD inline double gstar_node(double step, int i) {
@@ -97,61 +100,74 @@ RX_HD inline float limb_factor32(int limb, float fmu0) {
   return 1.0f;
 }
 
-// Reduced integrand eg^3 ftrf_k / sqrt(g*(1-g*)) [* limb] for k = 0 and 1;
-// relb_func = reduced * emis / (gmax - gmin).
-RX_HD inline void eval64_pair(double eg, const RadiusView &v, double &f0, double &f1) {
+// ---------------------------------------------------------------------------
+// FP64 integrand
+// ---------------------------------------------------------------------------
+
+// Reduced integrand eg^3 ftrf_k / sqrt(g*(1-g*)) [* limb] for k = 0 and 1 from
+// t4 = {T[ind][0], T[ind][1], T[ind+1][0], T[ind+1][1]} (relb_func's swapped
+// lerp weights, src/Relprofile.cpp:502-504); relb_func = reduced * emis/(gmax-gmin).
+RX_HD inline void eval64_at(double eg, double egs, int ind, const RadiusView &v, const double *t4,
+                            const double *c4, double &f0, double &f1) {
 #pragma clang fp contract(off)
-  const double egs = (eg - v.gmin) * v.del_g;
-  const int ind = g_bracket(egs, v.ng, v.gs_step, v.inv_step);
   const double g0 = gstar_node(v.gs_step, ind);
   const double g1 = gstar_node(v.gs_step, ind + 1);
   const double inte = (egs - g0) / (g1 - g0);
   const double inte1 = 1.0 - inte;
   const double base = eg * eg * eg / sqrt(egs - egs * egs);
-  const double t0 = inte * v.T64[ind * 2 + 0] + inte1 * v.T64[(ind + 1) * 2 + 0];
-  const double t1 = inte * v.T64[ind * 2 + 1] + inte1 * v.T64[(ind + 1) * 2 + 1];
+  const double t0 = inte * t4[0] + inte1 * t4[2];
+  const double t1 = inte * t4[1] + inte1 * t4[3];
   f0 = base * t0;
   f1 = base * t1;
   if (v.limb) {
-    const double m0 = inte * v.C64[ind * 2 + 0] + inte1 * v.C64[(ind + 1) * 2 + 0];
-    const double m1 = inte * v.C64[ind * 2 + 1] + inte1 * v.C64[(ind + 1) * 2 + 1];
+    const double m0 = inte * c4[0] + inte1 * c4[2];
+    const double m1 = inte * c4[1] + inte1 * c4[3];
     f0 *= limb_factor64(v.limb, m0);
     f1 *= limb_factor64(v.limb, m1);
   }
 }
 
-RX_HD inline float rx_rsqrtf(float x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return __builtin_amdgcn_rsqf(x);
-#else
-  return 1.0f / sqrtf(x);
-#endif
-}
-
-// FP32 integrand: coordinates and bracket in FP64, values in FP32.
-RX_HD inline void eval32_pair(double eg, const RadiusView &v, float &f0, float &f1) {
+RX_HD inline void eval64_pair(double eg, const RadiusView &v, double &f0, double &f1) {
+#pragma clang fp contract(off)
   const double egs = (eg - v.gmin) * v.del_g;
   const int ind = g_bracket(egs, v.ng, v.gs_step, v.inv_step);
-  const double g0 = gstar_node(v.gs_step, ind);
-  const float inte = (float)((egs - g0) * v.inv_step);
-  const float t = (float)egs;
-  const float u = (float)(1.0 - egs);
-  const float e = (float)eg;
-  const float base = e * e * e * rx_rsqrtf(t * u);
-  const float a0 = v.T32[ind * 2 + 0], b0 = v.T32[(ind + 1) * 2 + 0];
-  const float a1 = v.T32[ind * 2 + 1], b1 = v.T32[(ind + 1) * 2 + 1];
-  f0 = base * fmaf(inte, a0 - b0, b0);
-  f1 = base * fmaf(inte, a1 - b1, b1);
-  if (v.limb) {
-    const float c0 = v.C32[ind * 2 + 0], d0 = v.C32[(ind + 1) * 2 + 0];
-    const float c1 = v.C32[ind * 2 + 1], d1 = v.C32[(ind + 1) * 2 + 1];
-    f0 *= limb_factor32(v.limb, fmaf(inte, c0 - d0, d0));
-    f1 *= limb_factor32(v.limb, fmaf(inte, c1 - d1, d1));
+  eval64_at(eg, egs, ind, v, v.T64 + ind * 2, v.limb ? v.C64 + ind * 2 : nullptr, f0, f1);
+}
+
+// Edge norm of int_edge (src/Relprofile.cpp:609-618): sqrt(H) * sum_k relb_func
+// at g* = hex, in reduced units.  T64/C64 must hold the radius' FP64 values.
+RX_HD inline double edge_norm64(bool upper, const RadiusView &v) {
+#pragma clang fp contract(off)
+  const double hex = upper ? 1.0 - GFAC_H : GFAC_H;
+  const double e = (hex * (v.gmax - v.gmin) + v.gmin) * 1.0;
+  double f0, f1;
+  eval64_pair(e, v, f0, f1);
+  double norm = 0.0;
+  norm = norm + f0;
+  norm = norm + f1;
+  return norm * sqrt(GFAC_H);
+}
+
+// int_edge (src/Relprofile.cpp:585-621) from the precomputed edge norms.
+RX_HD inline double int_edge_pre(double blo, double bhi, const RadiusView &v) {
+#pragma clang fp contract(off)
+  double lo, hi, norm;
+  if (blo <= 0.5) {
+    lo = blo;
+    hi = bhi;
+    norm = v.normL;
+  } else {
+    lo = 1.0 - bhi;
+    hi = 1.0 - blo;
+    norm = v.normU;
   }
+  return 2 * norm * (sqrt(hi) - sqrt(lo)) * 1.0 * (v.gmax - v.gmin);
 }
 
 // Romberg for both branches on [a,b], FP64.  Returns t_0[niter0][0] + t_1[niter1][0].
-RX_HD inline double romberg64_pair(double a, double b, const RadiusView &v, int *levels = nullptr) {
+// R[ii] holds the current anti-diagonal t[ii][n-ii] of the reference's table
+// (src/Relprofile.cpp:559-576), updated in place level by level.
+RX_HD inline double romberg64_pair(double a, double b, const RadiusView &v) {
 #pragma clang fp contract(off)
   double fa0, fa1, fb0, fb1;
   eval64_pair(a, v, fa0, fa1);
@@ -159,17 +175,14 @@ RX_HD inline double romberg64_pair(double a, double b, const RadiusView &v, int 
   const double ta0 = (fa0 + fb0) / 2.0, ta1 = (fa1 + fb1) / 2.0;
   double pas = b - a;
   double s0 = ta0, s1 = ta1;  // ta + sum of interior points so far
-  double p0[ROMB_ITERMAX + 2], p1[ROMB_ITERMAX + 2];  // previous anti-diagonal
-  double c0[ROMB_ITERMAX + 2], c1[ROMB_ITERMAX + 2];
-  p0[0] = ta0 * pas;
-  p1[0] = ta1 * pas;
-  double res0 = p0[0], res1 = p1[0];
+  double R0[ROMB_ITERMAX + 2], R1[ROMB_ITERMAX + 2];
+  R0[0] = ta0 * pas;
+  R1[0] = ta1 * pas;
+  double res0 = R0[0], res1 = R1[0];
   bool run0 = true, run1 = true;
-  int lev = 0;
 #pragma unroll
   for (int n = 1; n <= ROMB_ITERMAX + 1; n++) {
     if (!(run0 || run1)) break;
-    lev = n;
     pas = pas / 2.0;
     const int nnew = 1 << (n - 1);
     double add0 = 0.0, add1 = 0.0;
@@ -181,132 +194,190 @@ RX_HD inline double romberg64_pair(double a, double b, const RadiusView &v, int 
     }
     s0 += add0;
     s1 += add1;
-    c0[0] = s0 * pas;
-    c1[0] = s1 * pas;
+    double prv0 = R0[0], prv1 = R1[0], top0 = 0.0, top1 = 0.0;
+    R0[0] = s0 * pas;
+    R1[0] = s1 * pas;
     double r = 1.0;
 #pragma unroll
     for (int ii = 1; ii <= ROMB_ITERMAX + 1; ii++) {
       if (ii > n) break;
       r *= 4.0;
-      c0[ii] = (r * c0[ii - 1] - p0[ii - 1]) / (r - 1.0);
-      c1[ii] = (r * c1[ii - 1] - p1[ii - 1]) / (r - 1.0);
+      const double o0 = R0[ii], o1 = R1[ii];
+      if (ii == n) {
+        top0 = prv0;  // t[n-1][0] of the previous level
+        top1 = prv1;
+      }
+      R0[ii] = (r * R0[ii - 1] - prv0) / (r - 1.0);
+      R1[ii] = (r * R1[ii - 1] - prv1) / (r - 1.0);
+      prv0 = o0;
+      prv1 = o1;
     }
     if (run0) {
-      res0 = c0[n];
-      const double ob = fabs(c0[n] - p0[n - 1]) / c0[n];
+      res0 = R0[n];
+      const double ob = fabs(R0[n] - top0) / R0[n];
       run0 = (ob > ROMB_PREC) && (n <= ROMB_ITERMAX);
     }
     if (run1) {
-      res1 = c1[n];
-      const double ob = fabs(c1[n] - p1[n - 1]) / c1[n];
+      res1 = R1[n];
+      const double ob = fabs(R1[n] - top1) / R1[n];
       run1 = (ob > ROMB_PREC) && (n <= ROMB_ITERMAX);
     }
-#pragma unroll
-    for (int ii = 0; ii <= ROMB_ITERMAX + 1; ii++) {
-      if (ii > n) break;
-      p0[ii] = c0[ii];
-      p1[ii] = c1[ii];
-    }
   }
-  if (levels) *levels = lev;
   return res0 + res1;
+}
+
+// ---------------------------------------------------------------------------
+// FP32 integrand in bin-local coordinates
+// ---------------------------------------------------------------------------
+
+RX_HD inline float rx_rsqrtf(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rsqf(x);
+#else
+  return 1.0f / sqrtf(x);
+#endif
+}
+
+// Interval [rlo, rhi] of one bin integral: g*(s) = ga + s dg from the low end,
+// gb - (1-s) dg from the high end (whichever is closer keeps the relative error
+// of g* and 1-g* at FP32 rounding), E(s) = rlo + s dE.
+struct BinF {
+  double rlo, rhi;  // FP64 ends (fallback path)
+  float ga, gb, ua, ub, dg, e0, de;
+};
+
+RX_HD inline BinF make_binf(double rlo, double rhi, const RadiusView &v) {
+  BinF b;
+  b.rlo = rlo;
+  b.rhi = rhi;
+  const double ga = (rlo - v.gmin) * v.del_g;
+  const double gb = (rhi - v.gmin) * v.del_g;
+  b.ga = (float)ga;
+  b.gb = (float)gb;
+  b.ua = (float)(1.0 - ga);
+  b.ub = (float)(1.0 - gb);
+  b.dg = (float)(gb - ga);
+  b.e0 = (float)rlo;
+  b.de = (float)(rhi - rlo);
+  return b;
+}
+
+// Both branches at s (exact dyadic in [0,1]).
+RX_HD inline void eval32s(float s, const BinF &bf, const RadiusView &v, float &f0, float &f1) {
+  float t, u;
+  if (s <= 0.5f) {
+    t = fmaf(s, bf.dg, bf.ga);
+    u = fmaf(-s, bf.dg, bf.ua);
+  } else {
+    const float sr = 1.0f - s;
+    t = fmaf(-sr, bf.dg, bf.gb);
+    u = fmaf(sr, bf.dg, bf.ub);
+  }
+  const float inv_step = (float)v.inv_step;
+  const float q = (t - (float)GFAC_H) * inv_step;
+  const float qi = rintf(q);
+  const float thr = 1e-6f * (inv_step + 1.0f);
+  int ind;
+  float inte;
+  float e = fmaf(s, bf.de, bf.e0);
+  if (fabsf(q - qi) < thr && qi >= 1.0f && qi <= (float)(v.ng - 2)) {
+    // close to a g* node: decide the bracket with the FP64 coordinates
+    const double eg = bf.rlo + (bf.rhi - bf.rlo) * (double)s;
+    const double egs = (eg - v.gmin) * v.del_g;
+    ind = g_bracket(egs, v.ng, v.gs_step, v.inv_step);
+    inte = (float)((egs - gstar_node(v.gs_step, ind)) * v.inv_step);
+    t = (float)egs;
+    u = (float)(1.0 - egs);
+    e = (float)eg;
+  } else {
+    ind = (int)floorf(q);
+    ind = ind < 0 ? 0 : (ind > v.ng - 2 ? v.ng - 2 : ind);
+    inte = q - (float)ind;
+  }
+  const float base = e * e * e * rx_rsqrtf(t * u);
+  const float a0 = v.T32[ind * 2 + 0], a1 = v.T32[ind * 2 + 1];
+  const float b0 = v.T32[ind * 2 + 2], b1 = v.T32[ind * 2 + 3];
+  f0 = base * fmaf(inte, a0 - b0, b0);
+  f1 = base * fmaf(inte, a1 - b1, b1);
+  if (v.limb) {
+    const float c0 = v.C32[ind * 2 + 0], c1 = v.C32[ind * 2 + 1];
+    const float d0 = v.C32[ind * 2 + 2], d1 = v.C32[ind * 2 + 3];
+    f0 *= limb_factor32(v.limb, fmaf(inte, c0 - d0, d0));
+    f1 *= limb_factor32(v.limb, fmaf(inte, c1 - d1, d1));
+  }
 }
 
 // Romberg for both branches, FP32 values.  Sets *need64 when a stop decision
 // fell inside the guard band; the caller then recomputes in FP64.
-RX_HD inline float romberg32_pair(double a, double b, const RadiusView &v, bool *need64) {
+RX_HD inline float romberg32_pair(const BinF &bf, const RadiusView &v, bool *need64) {
   float fa0, fa1, fb0, fb1;
-  eval32_pair(a, v, fa0, fa1);
-  eval32_pair(b, v, fb0, fb1);
+  eval32s(0.0f, bf, v, fa0, fa1);
+  eval32s(1.0f, bf, v, fb0, fb1);
   const float ta0 = (fa0 + fb0) * 0.5f, ta1 = (fa1 + fb1) * 0.5f;
-  double pas = b - a;
+  float pas = bf.de;
   float s0 = ta0, s1 = ta1;
-  float p0[ROMB_ITERMAX + 2], p1[ROMB_ITERMAX + 2];
-  float c0[ROMB_ITERMAX + 2], c1[ROMB_ITERMAX + 2];
-  const float pf = (float)pas;
-  p0[0] = ta0 * pf;
-  p1[0] = ta1 * pf;
-  float res0 = p0[0], res1 = p1[0];
+  float R0[ROMB_ITERMAX + 2], R1[ROMB_ITERMAX + 2];
+  R0[0] = ta0 * pas;
+  R1[0] = ta1 * pas;
+  float res0 = R0[0], res1 = R1[0];
   bool run0 = true, run1 = true;
-  const float lo = (float)ROMB_PREC * (1.0f - ROMB_GUARD);
-  const float hi = (float)ROMB_PREC * (1.0f + ROMB_GUARD);
+  const float lo = (float)ROMB_PREC * (1.0f - v.guard);
+  const float hi = (float)ROMB_PREC * (1.0f + v.guard);
 #pragma unroll
   for (int n = 1; n <= ROMB_ITERMAX + 1; n++) {
     if (!(run0 || run1)) break;
-    pas = pas * 0.5;
+    pas = pas * 0.5f;
     const int nnew = 1 << (n - 1);
+    const float ds = 1.0f / (float)(2 * nnew);
     float add0 = 0.0f, add1 = 0.0f;
     for (int m = 0; m < nnew; m++) {
       float g0v, g1v;
-      eval32_pair(a + pas * (double)(2 * m + 1), v, g0v, g1v);
+      eval32s((float)(2 * m + 1) * ds, bf, v, g0v, g1v);
       add0 += g0v;
       add1 += g1v;
     }
     s0 += add0;
     s1 += add1;
-    const float pfn = (float)pas;
-    c0[0] = s0 * pfn;
-    c1[0] = s1 * pfn;
+    float prv0 = R0[0], prv1 = R1[0], top0 = 0.0f, top1 = 0.0f;
+    R0[0] = s0 * pas;
+    R1[0] = s1 * pas;
     float r = 1.0f;
 #pragma unroll
     for (int ii = 1; ii <= ROMB_ITERMAX + 1; ii++) {
       if (ii > n) break;
       r *= 4.0f;
       const float inv = 1.0f / (r - 1.0f);
-      c0[ii] = (r * c0[ii - 1] - p0[ii - 1]) * inv;
-      c1[ii] = (r * c1[ii - 1] - p1[ii - 1]) * inv;
+      const float o0 = R0[ii], o1 = R1[ii];
+      if (ii == n) {
+        top0 = prv0;
+        top1 = prv1;
+      }
+      R0[ii] = (r * R0[ii - 1] - prv0) * inv;
+      R1[ii] = (r * R1[ii - 1] - prv1) * inv;
+      prv0 = o0;
+      prv1 = o1;
     }
     if (run0) {
-      res0 = c0[n];
-      const float ob = fabsf(c0[n] - p0[n - 1]) / c0[n];
+      res0 = R0[n];
+      const float ob = fabsf(R0[n] - top0) / R0[n];
       if (ob >= lo && ob <= hi && n <= ROMB_ITERMAX) *need64 = true;
       run0 = (ob > (float)ROMB_PREC) && (n <= ROMB_ITERMAX);
     }
     if (run1) {
-      res1 = c1[n];
-      const float ob = fabsf(c1[n] - p1[n - 1]) / c1[n];
+      res1 = R1[n];
+      const float ob = fabsf(R1[n] - top1) / R1[n];
       if (ob >= lo && ob <= hi && n <= ROMB_ITERMAX) *need64 = true;
       run1 = (ob > (float)ROMB_PREC) && (n <= ROMB_ITERMAX);
     }
-#pragma unroll
-    for (int ii = 0; ii <= ROMB_ITERMAX + 1; ii++) {
-      if (ii > n) break;
-      p0[ii] = c0[ii];
-      p1[ii] = c1[ii];
-    }
   }
   return res0 + res1;
-}
-
-// int_edge in reduced units (the trailing (gmax-gmin) cancels the Jacobian
-// inside relb_func), FP64 in both modes.
-RX_HD inline double int_edge64(double blo, double bhi, const RadiusView &v) {
-#pragma clang fp contract(off)
-  double hex, lo, hi;
-  if (blo <= 0.5) {
-    hex = GFAC_H;
-    lo = blo;
-    hi = bhi;
-  } else {
-    hex = 1.0 - GFAC_H;
-    lo = 1.0 - bhi;
-    hi = 1.0 - blo;
-  }
-  const double e = (hex * (v.gmax - v.gmin) + v.gmin) * 1.0;
-  double f0, f1;
-  eval64_pair(e, v, f0, f1);
-  double norm = 0.0;
-  norm = norm + f0;
-  norm = norm + f1;
-  norm = norm * sqrt(GFAC_H);
-  return 2 * norm * (sqrt(hi) - sqrt(lo)) * 1.0 * (v.gmax - v.gmin);
 }
 
 // Integral of one energy bin [rlo0, rhi0] (line-energy units) at one radius,
 // in reduced units (multiply by emis/(gmax-gmin) for relb_func units).
 // MODE 0: FP64; MODE 1: FP32 interior with FP64 decisions and edges.
 template <int MODE>
-RX_HD inline double integ_bin(double rlo0, double rhi0, const RadiusView &v) {
+RX_HD inline double integ_bin(double rlo0, double rhi0, const RadiusView &v, bool *need64 = nullptr) {
 #pragma clang fp contract(off)
   double gblo = (rlo0 / 1.0 - v.gmin) * v.del_g;
   if (gblo < 0.0) gblo = 0.0; else if (gblo > 1.0) gblo = 1.0;
@@ -322,7 +393,7 @@ RX_HD inline double integ_bin(double rlo0, double rhi0, const RadiusView &v) {
       hhi = gbhi;
       rlo = -1.0;
     }
-    flu = flu + int_edge64(hlo, hhi, v);
+    flu = flu + int_edge_pre(hlo, hhi, v);
   }
   if (gbhi >= (1.0 - GFAC_H)) {
     double hhi = gbhi, hlo = 1.0 - GFAC_H;
@@ -331,27 +402,28 @@ RX_HD inline double integ_bin(double rlo0, double rhi0, const RadiusView &v) {
       hlo = gblo;
       rhi = -1.0;
     }
-    flu = flu + int_edge64(hlo, hhi, v);
+    flu = flu + int_edge_pre(hlo, hhi, v);
   }
   if ((rhi >= 0) && (rlo >= 0)) {
     if (rlo >= 1.0 * RED_WING) {
       if (MODE == 0) {
         flu = flu + romberg64_pair(rlo, rhi, v);
       } else {
-        bool need64 = false;
-        const float r32 = romberg32_pair(rlo, rhi, v, &need64);
-        flu = flu + (need64 ? romberg64_pair(rlo, rhi, v) : (double)r32);
+        // a stop decision inside the guard band is re-done in FP64 by the caller
+        const BinF bf = make_binf(rlo, rhi, v);
+        flu = flu + (double)romberg32_pair(bf, v, need64);
       }
     } else {
-      const double mid = (rhi + rlo) / 2.0;
       if (MODE == 0) {
+        const double mid = (rhi + rlo) / 2.0;
         double f0, f1;
         eval64_pair(mid, v, f0, f1);
         flu = flu + (f0 * (rhi - rlo) + f1 * (rhi - rlo));
       } else {
+        const BinF bf = make_binf(rlo, rhi, v);
         float f0, f1;
-        eval32_pair(mid, v, f0, f1);
-        flu = flu + (double)((f0 + f1) * (float)(rhi - rlo));
+        eval32s(0.5f, bf, v, f0, f1);
+        flu = flu + (double)((f0 + f1) * bf.de);
       }
     }
   }

@@ -31,6 +31,7 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 constexpr int SERVER_RCHUNKS = 4;
+constexpr int SERVER_GROUPS = 2;
 
 struct Pending {
   int64_t ticket;
@@ -123,6 +124,7 @@ void rx_server::launch_ready() {
       st = ensure_device(err);
       if (st == RX_OK) {
         s.batch.n_rch_forced = SERVER_RCHUNKS;
+        s.batch.groups_forced = SERVER_GROUPS;
         st = batch_build(s.batch, rq.data(), (int)rq.size(), err);
       }
       if (st == RX_OK) st = batch_run(s.batch, precision, err);

@@ -93,3 +93,17 @@ def draw_relconv_params(n, seed=5):
         rin = rng.uniform(isco, 50.0)
         out.append([3.0, 3.0, 15.0, a, incl, rin, 400.0, 0.0])
     return out
+
+
+def flat_table(gmin, gmax, trff_sum=1.0, cosne=0.5):
+    """The default table's a / mu0 / radius grids with a constant window
+    [gmin, gmax] and a flat transfer function trff1 = trff2 = trff_sum/2."""
+    rel, lp, T = tables()
+    out = dict(rel)
+    out["gmin"] = np.full_like(rel["gmin"], gmin)
+    out["gmax"] = np.full_like(rel["gmax"], gmax)
+    out["trff1"] = np.full_like(rel["trff1"], trff_sum / 2)
+    out["trff2"] = np.full_like(rel["trff2"], trff_sum / 2)
+    out["cosne1"] = np.full_like(rel["cosne1"], cosne)
+    out["cosne2"] = np.full_like(rel["cosne2"], cosne)
+    return out

@@ -130,3 +130,20 @@ def test_batch_uneven_relconv_fp32(tabs):
             assert st == 0
             assert rel_err(y, ref) < tol, (prec, i, rel_err(y, ref))
     b.close()
+
+
+def test_fp32_guard_fixup_path(tabs):
+    """Widen the FP32 guard band so that nearly every Romberg stop test is
+    re-done in FP64 by the fix-up pass: the FP32 path then agrees with the
+    oracle at close to FP64 accuracy."""
+    e = log_grid(3000, 0.1, 10.0)
+    ref, _ = oracle.eval_model(oracle.MOD_RELLINE, tabs, RELLINE_CFG0, e)
+    y_default, _ = rx.eval_model(rx.MODEL_RELLINE, RELLINE_CFG0, e, precision=rx.FP32)
+    assert rx.lib.rx_set_fp32_guard(0.95) == 0
+    try:
+        y, _ = rx.eval_model(rx.MODEL_RELLINE, RELLINE_CFG0, e, precision=rx.FP32)
+    finally:
+        rx.lib.rx_set_fp32_guard(2e-3)
+    _report("relline fp32 wide guard", y, ref)
+    assert rel_err(y, ref) < TOL32
+    assert rel_err(y, ref) <= rel_err(y_default, ref) * 1.5

@@ -1,0 +1,290 @@
+"""Pins the FP64 oracle: the reference's own known-answer tests (golden
+fixture tests/golden/reference_kats.json), closed forms, brute force on small
+inputs, and the invariants the reference's tests assert.  CPU only."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.helpers import RELLINE_CFG0, log_grid, reflection_spectrum, tables
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
+
+
+def test_rebin_kat():
+    k = GOLD["rebin_spectrum"]
+    out = oracle.rebin(k["ener"], k["ener0"], k["val0"])
+    assert np.max(np.abs(out - np.array(k["expected"]))) < k["tol"]
+
+
+def test_rebin_mean_flux_kat():
+    k = GOLD["rebin_mean_flux"]
+    out = oracle.rebin_mean_flux(k["ener"], k["ener0"], k["val0"])
+    assert np.max(np.abs(out - np.array(k["expected"]))) < k["tol"]
+
+
+def test_interp_kats():
+    k = GOLD["interp_lin_1d"]
+    assert abs(oracle.lib.orc_interp_lin_1d(k["ifac"], k["rlo"], k["rhi"]) - k["expected"]) < k["tol"]
+    k = GOLD["interp_lin_2d"]
+    args = (k["ifac1"], k["ifac2"], k["r11"], k["r12"], k["r21"], k["r22"])
+    assert abs(oracle.lib.orc_interp_lin_2d(*args) - k["expected"]) < k["tol"]
+    assert abs(oracle.lib.orc_interp_lin_2d_float(*args) - k["expected"]) < k["tol"]
+
+
+def test_kerr_closed_forms():
+    k = GOLD["kerr_isco"]
+    for a, r, tol in zip(k["a"], k["rms"], k["tol"]):
+        assert abs(oracle.lib.orc_kerr_rms(a) - r) < tol, (a, oracle.lib.orc_kerr_rms(a))
+    assert oracle.lib.orc_kerr_rplus(0.0) == 2.0
+    assert abs(oracle.lib.orc_kerr_rplus(0.998) - (1 + np.sqrt(1 - 0.998 ** 2))) < 1e-15
+    # ISCO is monotonically decreasing and concave in a: table chords stay below it
+    a = np.linspace(-0.998, 0.998, 401)
+    r = np.array([oracle.lib.orc_kerr_rms(x) for x in a])
+    assert np.all(np.diff(r) < 0)
+    assert np.all(np.diff(r, 2) < 1e-12)
+
+
+def test_log_and_fine_grids_closed_form():
+    e = oracle.log_grid(4097, 0.00035, 2000.0)
+    assert abs(e[0] - 0.00035) < 1e-15 and abs(e[-1] - 2000.0) / 2000.0 < 1e-12
+    assert np.allclose(np.diff(np.log(e)), np.log(2000.0 / 0.00035) / 4096, rtol=1e-9)
+    re = oracle.fine_radial_grid(1.2369, 400.0, 1000)
+    assert abs(re[0] - 400.0) / 400 < 1e-13 and abs(re[-1] - 1.2369) / 1.2369 < 1e-13
+    assert np.allclose(np.diff(re ** -0.5), (1.2369 ** -0.5 - 400 ** -0.5) / 999, rtol=1e-9)
+
+
+def test_energy_grid_shifts():
+    """lineE and redshift act as E/lineE and E(1+z) on the grid
+    (test/unit/test-cppspectrum.cpp:57-77)."""
+    rel, lp, T = tables()
+    e = log_grid(600, 0.1, 10.0)
+    p = list(RELLINE_CFG0)
+    f1, st = oracle.eval_model(oracle.MOD_RELLINE, T, p, e)
+    assert st == 0
+    p2 = list(p)
+    p2[0] = 1.0
+    f2, _ = oracle.eval_model(oracle.MOD_RELLINE, T, p2, e / 6.4)
+    assert np.array_equal(f1, f2)
+    p3 = list(p)
+    p3[8] = 0.5
+    f3, _ = oracle.eval_model(oracle.MOD_RELLINE, T, p3, e)
+    f4, _ = oracle.eval_model(oracle.MOD_RELLINE, T, p, e * 1.5)
+    assert np.array_equal(f3, f4)
+
+
+def test_fft_matches_dft():
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(256) + 1j * rng.standard_normal(256)
+    assert np.max(np.abs(oracle.fft(x) - np.fft.fft(x))) < 1e-11
+    assert np.max(np.abs(oracle.fft(x, inverse=True) - np.fft.ifft(x) * 256)) < 1e-11
+
+
+def test_fft_convolution_brute_force():
+    """fftw_conv_spectrum (src/Relbase.cpp:119-188) equals the direct circular
+    convolution of the E/dE-weighted spectra with the 1 keV rotation."""
+    rng = np.random.default_rng(1)
+    n = 256
+    e = log_grid(n, 0.05, 20.0)
+    fx = rng.uniform(0.1, 1.0, n)
+    fr = rng.uniform(0.0, 1.0, n)
+    c = 0.5 * (e[1:] + e[:-1]) / np.diff(e)
+    i1 = oracle.lib.orc_binary_search(e.ctypes.data_as(oracle._D), n + 1, 1.0)
+    x = fx * c
+    y = np.zeros(n)
+    for i in range(n):
+        y[(i - i1 + n) % n] = fr[i] * c[i]
+    direct = np.array([sum(x[m] * y[(k - m) % n] for m in range(n)) for k in range(n)]) * n
+    out = oracle.fft_conv(e, fx, fr, normalized=False)
+    assert np.max(np.abs(out * c - direct)) / np.max(np.abs(direct)) < 1e-12
+    # delta input reproduces the rotated, re-weighted kernel
+    fd = np.zeros(n)
+    fd[i1] = 1.0
+    out = oracle.fft_conv(e, fd, fr, normalized=False)
+    expect = np.array([n * c[i1] * y[(k - i1) % n] / c[k] for k in range(n)])
+    assert np.max(np.abs(out - expect)) / np.max(np.abs(expect)) < 1e-12
+
+
+def test_conv_normalisation_invariants():
+    """relline on the convolution grid sums to 1; the normalised convolution keeps
+    the reflection flux in 0.01-1000 keV; a corrupted profile does not
+    (test/unit/test-stdfunctions.cpp:253-298)."""
+    k = GOLD["fft_normalisation"]
+    rel, lp, T = tables()
+    cg = oracle.log_grid(4097, 0.00035, 2000.0)
+    p, st = oracle.rel_params(oracle.MOD_RELLINE, [1.0] + RELLINE_CFG0[1:])
+    prof, _, st = oracle.relbase(T, p, cg)
+    assert st == 0
+    assert abs(prof.sum() - k["relline_sum"]) < k["relline_sum_tol"]
+    e_ref, refl = reflection_spectrum(n_incl=1)
+    xill = oracle.rebin(cg, e_ref, refl[0])
+    band = (cg[:-1] >= k["band_emin"]) & (cg[1:] <= k["band_emax"])
+    out = oracle.fft_conv(cg, xill, prof)
+    assert abs(out[band].sum() - xill[band].sum()) < k["band_sum_tol"] * max(1.0, xill[band].sum())
+    bad = prof.copy()
+    bad[k["corrupt_bin"]] = k["corrupt_value"]
+    out = oracle.fft_conv(cg, xill, bad)
+    assert abs(out[band].sum() - xill[band].sum()) > k["band_sum_tol"]
+
+
+def test_relconv_outside_band_is_zero():
+    """tests-execmodel.cpp:211-235: relconv outside 0.01-1000 keV returns zeros,
+    inside it keeps the normalisation."""
+    k = GOLD["relconv_outside_band"]
+    rel, lp, T = tables()
+    p = [3.0, 3.0, 15.0, 0.998, 30.0, -1.0, 400.0, 0.0]
+    lo, hi, n = k["outside_grid"]
+    e = log_grid(n, lo, hi)
+    f = np.zeros(n)
+    f[n // 2] = k["test_value"]
+    out, st = oracle.eval_model(oracle.MOD_RELCONV, T, p, e, flux_in=f)
+    assert st == 0 and abs(out.sum()) < k["outside_sum_tol"]
+    lo, hi, n = k["inside_grid"]
+    e = log_grid(n, lo, hi)
+    f = np.zeros(n)
+    f[n // 2] = k["test_value"]
+    out, st = oracle.eval_model(oracle.MOD_RELCONV, T, p, e, flux_in=f)
+    assert st == 0 and abs(out.sum() - k["test_value"]) < k["inside_sum_tol"] * k["test_value"]
+
+
+def test_relconv_of_narrow_line_is_relline():
+    """relconv applied to a narrow Gaussian line at 6.4 keV reproduces relline
+    (test/e2e/tests/test_convolution_models.sl:7-33,72): both on 600 log bins
+    over 0.2-10 keV, relconv rescaled to relline's sum, goodness over
+    2 < E_lo < 6 keV below 8e-4; the convolution keeps the input's photon
+    number within 1e-3 (test_convolution_models.sl:77)."""
+    from math import erf, sqrt
+    k = GOLD["relconv_vs_relline"]
+    rel, lp, T = tables()
+    e = log_grid(600, 0.2, 10.0)
+    sig = 0.01
+    cdf = np.array([0.5 * (1.0 + erf((x - 6.4) / (sqrt(2.0) * sig))) for x in e])
+    f = np.diff(cdf)
+    pc = [3.0, 3.0, 15.0, 0.998, 30.0, -1.0, 400.0, 0.0]
+    conv, st = oracle.eval_model(oracle.MOD_RELCONV, T, pc, e, flux_in=f)
+    assert st == 0
+    line, st = oracle.eval_model(oracle.MOD_RELLINE, T, [6.4, 3.0, 3.0, 15.0, 0.998, 30.0, -1.0, 400.0, 0, 0], e)
+    assert st == 0
+    assert abs(conv.sum() / f.sum() - 1.0) < k["norm_tol"]
+    conv = conv * line.sum() / conv.sum()
+    sel = (e[:-1] > 2.0) & (e[:-1] < 6.0) & (conv != 0)
+    goodness = np.sqrt(np.sum((line[sel] / conv[sel] - 1) ** 2)) / sel.sum()
+    print("relconv vs relline: norm", conv.sum() / line.sum(), "goodness", goodness)
+    assert goodness < k["goodness_limit"]
+
+
+def test_relline_sums_to_one():
+    rel, lp, T = tables()
+    e = log_grid(3000, 0.1, 10.0)
+    f, st = oracle.eval_model(oracle.MOD_RELLINE, T, RELLINE_CFG0, e)
+    assert st == 0 and abs(f.sum() - 1.0) < 1e-12
+    f, st = oracle.eval_model(oracle.MOD_RELLINELP, T, [6.4, 6.0, 0.998, 30.0, -1.0, 400.0, 0.0, 0.0, 2.0, 0.0], e)
+    assert st == 0 and abs(f.sum() - 1.0) < 1e-12
+
+
+def test_flat_transfer_function_gives_pi():
+    """Closed form of the r/g* integration: with trff_1 + trff_2 = 1, a window
+    of width 2e-3 around E = 1 and a normalised emissivity, the photon flux is
+    sum_r weight emis * int_0^1 dg*/sqrt(g*(1-g*)) = pi/4 (edges, Romberg and
+    the g* bracket logic all take part)."""
+    from tests.helpers import flat_table
+    rel = flat_table(0.999, 1.001)
+    T = oracle.Tables(rel)
+    e = np.linspace(0.998, 1.002, 4001)
+    p = [1.0, 3.0, 3.0, 15.0, 0.5, 40.0, -1.0, 400.0, 0.0, 0.0]
+    f, st = oracle.eval_model(oracle.MOD_RELLINE, T, p, e, phys_norm=True)
+    assert st == 0
+    assert abs(f.sum() / (np.pi / 4) - 1.0) < 2e-3
+
+
+def test_angle_weighting_linearity():
+    """Sum_mu w_mu (X_mu * R) == (sum_mu w_mu X_mu) * R for the unnormalised
+    FFT convolution (the identity that allows per-angle convolution summed in
+    frequency space)."""
+    rel, lp, T = tables()
+    cg = oracle.log_grid(1025, 0.01, 1000.0)
+    e_ref, refl = reflection_spectrum(n_incl=5)
+    rb = np.array([oracle.rebin(cg, e_ref, r) for r in refl])
+    p, _ = oracle.rel_params(oracle.MOD_RELLINE, [1.0] + RELLINE_CFG0[1:])
+    prof, _, _ = oracle.relbase(T, p, cg)
+    w = np.array([0.1, 0.3, 0.25, 0.2, 0.15])
+    lhs = sum(w[k] * oracle.fft_conv(cg, rb[k], prof, normalized=False) for k in range(5))
+    rhs = oracle.fft_conv(cg, (w[:, None] * rb).sum(0), prof, normalized=False)
+    assert np.max(np.abs(lhs - rhs)) / np.max(np.abs(rhs)) < 1e-12
+
+
+def test_rzone_grid():
+    g = oracle.rzone_grid(2.0, 400.0, 1, 6.0)
+    assert list(g) == [2.0, 400.0]
+    g = oracle.rzone_grid(2.0, 400.0, 10, 6.0)
+    assert abs(g[0] - 2.0) < 1e-12 and abs(g[-1] - 400.0) < 1e-9 and np.all(np.diff(g) > 0)
+
+
+def test_rebin_conserves_total():
+    rng = np.random.default_rng(3)
+    e0 = np.sort(rng.uniform(0.5, 50.0, 300))
+    f0 = rng.uniform(0, 1, 299)
+    e = log_grid(777, 0.1, 100.0)
+    out = oracle.rebin(e, e0, f0)
+    assert abs(out.sum() - f0.sum()) < 1e-10 * f0.sum()
+
+
+def test_rebin_shared_top_edge_is_defined():
+    """An output edge equal to the last input edge: the reference's walk reads
+    past the input grid there (src/relutility.c:564); the overlap has zero
+    width and the oracle defines the value as 0."""
+    e0 = np.array([1.0, 2.0, 3.0])
+    f0 = np.array([1.0, 1.0])
+    out = oracle.rebin(np.array([3.0, 4.0, 5.0]), e0, f0)
+    assert np.array_equal(out, [0.0, 0.0])
+    out = oracle.rebin(np.array([2.5, 3.0]), e0, f0)
+    assert abs(out[0] - 0.5) < 1e-15
+
+
+@pytest.mark.parametrize("bad, status", [
+    ([6.4, 3, 3, 15, 0.999, 30, -1, 400, 0, 0], 1),    # a > 0.9982
+    ([6.4, 3, 3, 15, 0.5, 2.0, -1, 400, 0, 0], 1),      # incl < 3 deg
+    ([6.4, 3, 3, 15, 0.5, 88.0, -1, 400, 0, 0], 1),     # incl > 87 deg
+    ([6.4, 3, 3, 15, 0.5, 30, 50, 40, 0, 0], 1),        # Rin >= Rout
+])
+def test_parameter_rejections(bad, status):
+    p, st = oracle.rel_params(oracle.MOD_RELLINE, bad)
+    assert st == status
+
+
+def test_parameter_clamps():
+    p, st = oracle.rel_params(oracle.MOD_RELLINE, [6.4, 3, 3, 1000, 0.5, 30, -2, 2000, 0, 0])
+    assert st == 0
+    isco = oracle.lib.orc_kerr_rms(0.5)
+    assert abs(p.rin - 2 * isco) < 1e-12 and p.rout == 1000.0 and p.rbr == 1000.0
+    p, st = oracle.rel_params(oracle.MOD_RELLINE, [6.4, 3, 3, 1.0, 0.5, 30, 0.5, 400, 0, 0])
+    assert st == 0 and p.rin == isco and p.rbr == isco
+    p, st = oracle.rel_params(oracle.MOD_RELLINELP, [6.4, 1.0, 0.9, 30, -1, 400, 0, 0, 2, 0])
+    assert st == 0 and abs(p.height - 1.1 * oracle.lib.orc_kerr_rplus(0.9)) < 1e-12
+    p, st = oracle.rel_params(oracle.MOD_RELCONVLP, [-3.0, 1.5, 0.9, 30, -1, 400, 0, 2, 0])
+    assert st == 0 and p.beta == 0.99 and abs(p.height - 3 * oracle.lib.orc_kerr_rplus(0.9)) < 1e-12
+    p, st = oracle.rel_params(oracle.MOD_RELLINELP, [6.4, 6.0, 0.9, 30, -1, 400, 0, 0, 2, 1])
+    assert st == 2  # returning radiation is out of scope
+
+
+def test_lp_closed_forms():
+    for r, a, h in [(10.0, 0.5, 6.0), (3.0, 0.998, 3.0), (50.0, -0.5, 20.0)]:
+        ut_d = (r * np.sqrt(r) + a) / (np.sqrt(r) * np.sqrt(r * r - 3 * r + 2 * a * np.sqrt(r)))
+        ut_h = np.sqrt((h * h + a * a) / (h * h - 2 * h + a * a))
+        assert abs(oracle.lib.orc_gi_potential_lp(r, a, h, 0.0, 1.0) - ut_d / ut_h) < 1e-13
+    # Newtonian lamp post: (1/(2 pi h^2)) (1 + (r/h)^2)^(-3/2)
+    h, r = 10.0, 30.0
+    assert abs(oracle.lib.orc_lp_emissivity_newton(h, r) - (1 + (r / h) ** 2) ** -1.5 / (2 * np.pi * h * h)) < 1e-16
+
+
+def test_work_counters_cfg0():
+    """Work per cfg0 spectrum (used for the roofline's flop count)."""
+    rel, lp, T = tables()
+    e = log_grid(3000, 0.1, 10.0)
+    f, st, s = oracle.eval_model(oracle.MOD_RELLINE, T, RELLINE_CFG0, e, stats=True)
+    assert st == 0
+    assert 1e5 < s["relb_func_calls"] < 1e7
+    # every bin integral is a midpoint, a Romberg pair (two branches) or edge-only
+    assert s["bin_integrals"] >= s["midpoint_calls"] + s["romberg_calls"] // 2
