@@ -29,6 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+KERNELS = ("k_setup", "k_integrate", "k_fixup", "k_finalize", "k_conv")
 
 
 def log(*a):
@@ -95,7 +96,7 @@ def timed_steps(rx, b, prec, steps, warmup):
         b.run(prec)
     b.sync()
     rx.lib.rx_device_synchronize()
-    kms = np.zeros(4)
+    kms = np.zeros(len(KERNELS))
     t0 = time.perf_counter()
     for _ in range(steps):
         b.run(prec)
@@ -184,9 +185,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=4.0)
-    ap.add_argument("--cpu-sample", type=int, default=768)
+    ap.add_argument("--cpu-sample", type=int, default=2048)
     ap.add_argument("--shape", type=str, default="",
-                    help="fixed tile shape 'radius_chunks,groups' instead of autotuning")
+                    help="fixed work shape 'seg_bins,waves_per_cu' instead of autotuning")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the headline batch (FP32), for profiler runs")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
@@ -217,12 +218,11 @@ def main():
     nbins = np.array([r["energy"].size - 1 for r in reqs])
     b = rx.Batch(reqs)
     if args.shape:
-        rch, grp = (int(x) for x in args.shape.split(","))
-        b.set_rchunks(rch)
-        b.set_groups(grp)
+        seg, wpc = (int(x) for x in args.shape.split(","))
+        b.set_shape(seg, wpc)
     else:
-        rch, grp = b.autotune(rx.FP32)
-    log(f"autotuned radius chunks: {rch}, radius groups: {grp}")
+        seg, wpc = b.autotune(rx.FP32)
+    log(f"work shape: {seg} bins per item, {wpc} waves per CU")
     dt32, kms32 = timed_steps(rx, b, rx.FP32, args.steps, args.warmup)
     value = len(reqs) / dt32
     log(f"cfg3 fp32: {dt32 * 1e3:.2f} ms/step, {value:.1f} spectra/s, kernels ms {np.round(kms32, 3)}")
@@ -241,8 +241,6 @@ def main():
     lat = []
     for r in reqs[:16]:
         pb = rx.Batch([r])
-        pb.set_rchunks(0)
-        pb.set_groups(1)
         d, _ = timed_steps(rx, pb, rx.FP64, 3, 1)
         lat.append(d)
         pb.close()
@@ -280,8 +278,7 @@ def main():
     roofline = dict(bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=achieved / HBM_PEAK_GBS, traffic=None,
                     kernel="k_integrate<1>", kernel_ms=round(kms32[1], 4),
-                    kernel_ms_all=dict(setup=round(kms32[0], 4), integrate=round(kms32[1], 4),
-                                       finalize=round(kms32[2], 4), conv=round(kms32[3], 4)))
+                    kernel_ms_all={k: round(v, 4) for k, v in zip(KERNELS, kms32)})
 
     cpu = None
     if not args.no_cpu:
@@ -315,8 +312,8 @@ def main():
             "workload": "uneven relconv batch: 4096 requests, own log grids of 300-8000 bins "
                         "(log-uniform) over 0.1-100 keV, 4096-bin convolution grid 0.00035-2000 keV, "
                         "25x22x100x20 relline table, FP32 + Kahan",
-            "n_requests": len(reqs), "mean_bins": float(nbins.mean()), "radius_chunks": rch,
-            "radius_groups": grp,
+            "n_requests": len(reqs), "mean_bins": float(nbins.mean()), "seg_bins": seg,
+            "waves_per_cu": wpc,
         },
         "plain_fp64_spectra_per_s": round(plain64, 2),
         "ratio_vs_plain_fp64": round(value / plain64, 2),

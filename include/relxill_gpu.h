@@ -45,7 +45,7 @@ extern "C" {
 
 /* ---- precision of the batched path ---- */
 #define RX_FP64 0 /* plain form: FP64 throughout */
-#define RX_FP32 1 /* FP32 integrand + Kahan sums; FP64 decisions, edges, norms */
+#define RX_FP32 1 /* FP32 integrand and Romberg; FP64 decisions, edges, sums over radii, norms */
 
 /* ===================================================================== */
 /* XSPEC local-model entry points (replace the generated lmod<name>,       */
@@ -147,7 +147,7 @@ typedef struct {
 } rx_request;
 
 /* Builds a device-resident batch: copies grids, parameters and inputs to HBM,
- * validates parameters (per-request status), plans tiles.  NULL on failure. */
+ * validates parameters (per-request status), sizes the work list.  NULL on failure. */
 rx_batch *rx_batch_create(const rx_request *req, int n_req);
 /* Runs the whole path on the device (stream-ordered, asynchronous). */
 int rx_batch_run(rx_batch *b, int precision);
@@ -160,10 +160,10 @@ int rx_batch_status(rx_batch *b, int i);
 int rx_batch_dist(rx_batch *b, int i, double *dist, int n_incl);
 /* lamp-post reflection fractions of request i: refl_frac, f_bh, f_ad, f_inf, f_inf_rest */
 int rx_batch_reflfrac(rx_batch *b, int i, double *out5);
-/* tile shape: radius chunks per request (0: automatic / autotuned) and
- * radius groups per wavefront (1, 2 or 4; 0: automatic) */
-int rx_batch_set_rchunks(rx_batch *b, int n_rch);
-int rx_batch_set_groups(rx_batch *b, int groups);
+/* Integration work shape: bins per work item (segment of one chunk of 64
+ * fine radii; 0: automatic / autotuned) and persistent wavefronts per CU
+ * (0: automatic / autotuned). */
+int rx_batch_set_shape(rx_batch *b, int seg_bins, int waves_per_cu);
 void rx_batch_destroy(rx_batch *b);
 
 /* Resident reflection set for relxill-style requests: n_incl spectra of
@@ -172,8 +172,9 @@ void rx_batch_destroy(rx_batch *b);
 int rx_reflset_create(const double *refl_energy, int n_refl, const double *refl, int n_incl);
 
 /* Kernel timing of the last rx_batch_run with profiling enabled: per kernel
- * (0 setup, 1 integrate, 2 finalize, 3 conv) the duration in ms measured with
- * HIP events on the batch's stream. */
+ * (0 k_setup (with k_reset), 1 k_integrate, 2 k_fixup, 3 k_finalize, 4 k_conv)
+ * the duration in ms measured with HIP events on the batch's stream. */
+#define RX_N_TIMED 5
 int rx_profile_enable(int on);
 /* Streams `bytes` through a plain device copy kernel `reps` times (after one
  * warm-up) and returns the average kernel time; used to calibrate HBM byte
@@ -181,11 +182,11 @@ int rx_profile_enable(int on);
 int rx_copy_test(int64_t bytes, int reps, double *ms);
 /* Waits for all work of the library on its device. */
 int rx_device_synchronize(void);
-int rx_batch_kernel_ms(rx_batch *b, double *ms4);
+int rx_batch_kernel_ms(rx_batch *b, double *ms);
 
-/* Autotuner: times candidate tile shapes (radius chunks x radius groups) for
+/* Autotuner: times candidate work shapes (segment bins x waves per CU) for
  * this batch on the device and keeps the fastest for batches of the same
- * shape class.  Returns 16 * radius_chunks + groups (>0) or -status. */
+ * shape class.  Returns 256 * seg_bins + waves_per_cu (>0) or -status. */
 int rx_batch_autotune(rx_batch *b, int precision);
 
 /* ===================================================================== */

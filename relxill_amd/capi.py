@@ -103,8 +103,7 @@ _EXPORTS = {
     "rx_batch_status": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "rx_batch_dist": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_int]),
     "rx_batch_reflfrac": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D]),
-    "rx_batch_set_rchunks": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
-    "rx_batch_set_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "rx_batch_set_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "rx_batch_destroy": (None, [ctypes.c_void_p]),
     "rx_reflset_create": (ctypes.c_int, [_D, ctypes.c_int, _D, ctypes.c_int]),
     "rx_profile_enable": (ctypes.c_int, [ctypes.c_int]),
@@ -295,21 +294,20 @@ class Batch:
         _check(lib.rx_batch_reflfrac(self.h, i, _dp(o)))
         return o
 
-    def set_rchunks(self, n):
-        _check(lib.rx_batch_set_rchunks(self.h, n))
-
-    def set_groups(self, g):
-        _check(lib.rx_batch_set_groups(self.h, g))
+    def set_shape(self, seg_bins, waves_per_cu):
+        """bins per work item and persistent wavefronts per CU (0: automatic)"""
+        _check(lib.rx_batch_set_shape(self.h, seg_bins, waves_per_cu))
 
     def autotune(self, precision):
-        """returns (radius_chunks, groups)"""
+        """returns (seg_bins, waves_per_cu)"""
         r = lib.rx_batch_autotune(self.h, precision)
         if r <= 0:
             raise RxError(-r, lib.rx_last_error().decode())
-        return r // 16, r % 16
+        return r // 256, r % 256
 
     def kernel_ms(self):
-        o = np.zeros(4)
+        """per kernel ms of the last profiled run: setup, integrate, fixup, finalize, conv"""
+        o = np.zeros(5)
         _check(lib.rx_batch_kernel_ms(self.h, _dp(o)))
         return o
 

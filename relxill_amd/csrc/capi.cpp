@@ -63,7 +63,7 @@ int eval_one(int model, const double *parameter, const double *energy, double *f
     int dst = 0;
     if (hipMemcpy(&dst, b.d_status.p, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) dst = RX_ERR_DEVICE;
     if (dst != 0) {
-      st = (dst >= 10 && dst < 50) ? RX_ERR_TABLE : dst;
+      st = public_status(dst);
       err = "model evaluation failed (status " + std::to_string(dst) + ")";
     }
   }
@@ -262,7 +262,7 @@ int rx_set_conv_grid(int n, double emin, double emax) {
 int rx_set_nr_fine(int nr_fine) {
   Global &g = G();
   std::lock_guard<std::recursive_mutex> lk(g.mu);
-  if (nr_fine < 3 || nr_fine > 100000) return RX_ERR_ARG;
+  if (nr_fine < 3 || nr_fine > MAX_NR_FINE) return RX_ERR_ARG;
   g.nr_fine = nr_fine;
   return RX_OK;
 }
@@ -314,8 +314,7 @@ int rx_batch_status(rx_batch *b, int i) {
   int st = 0;
   if (hipMemcpy(&st, b->b.d_status.as<int>() + i, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
     return RX_ERR_DEVICE;
-  if (st >= 10 && st < 50) st = RX_ERR_TABLE;
-  return st;
+  return public_status(st);
 }
 
 int rx_batch_output(rx_batch *b, int i, double *flux) {
@@ -348,12 +347,13 @@ int rx_batch_reflfrac(rx_batch *b, int i, double *out5) {
   return RX_OK;
 }
 
-int rx_batch_set_rchunks(rx_batch *b, int n_rch) {
-  if (!b || n_rch < 0 || n_rch > 1000) return RX_ERR_ARG;
+int rx_batch_set_shape(rx_batch *b, int seg_bins, int waves_per_cu) {
+  if (!b || seg_bins < 0 || seg_bins > 4096 || waves_per_cu < 0 || waves_per_cu > 32) return RX_ERR_ARG;
   Global &g = G();
   std::lock_guard<std::recursive_mutex> lk(g.mu);
-  b->b.n_rch_forced = n_rch;
-  batch_plan(b->b, n_rch, b->b.groups_forced);
+  b->b.seg_forced = seg_bins;
+  b->b.waves_forced = waves_per_cu;
+  batch_plan(b->b, seg_bins, waves_per_cu);
   return batch_upload(b->b);
 }
 
@@ -407,36 +407,27 @@ int rx_profile_enable(int on) {
   return RX_OK;
 }
 
-int rx_batch_kernel_ms(rx_batch *b, double *ms4) {
-  if (!b || !ms4) return RX_ERR_ARG;
-  for (int k = 0; k < 4; k++) ms4[k] = b->b.kernel_ms[k];
+int rx_batch_kernel_ms(rx_batch *b, double *ms) {
+  if (!b || !ms) return RX_ERR_ARG;
+  for (int k = 0; k < RX_N_TIMED; k++) ms[k] = b->b.kernel_ms[k];
   return b->b.profiled ? RX_OK : RX_ERR_ARG;
 }
 
-int rx_batch_set_groups(rx_batch *b, int groups) {
-  if (!b || (groups != 0 && groups != 1 && groups != 2 && groups != 4)) return RX_ERR_ARG;
-  Global &g = G();
-  std::lock_guard<std::recursive_mutex> lk(g.mu);
-  b->b.groups_forced = groups;
-  batch_plan(b->b, b->b.n_rch_forced, groups);
-  return batch_upload(b->b);
-}
-
-// Times every (radius chunks, radius groups) candidate on this batch and keeps
-// the fastest for batches of the same shape class (log2 of the bin chunks and
-// whether convolutions are present).
+// Times every (segment bins, waves per CU) candidate on this batch and keeps
+// the fastest for batches of the same shape class (log2 of the profile bins
+// and whether convolutions are present).
 int rx_batch_autotune(rx_batch *b, int precision) {
   if (!b) return RX_ERR_ARG;
   Global &g = G();
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   std::string err;
-  const int rchs[] = {1, 2, 4, 8, 16, 32};
-  const int grps[] = {1, 2, 4};
+  const int segs[] = {32, 64, 128, 256};
+  const int wpcs[] = {4, 8, 16};
   double best = 1e30;
-  int best_r = 1, best_g = 1;
-  for (int gr : grps) {
-    for (int r : rchs) {
-      batch_plan(b->b, r, gr);
+  int best_s = 64, best_w = 8;
+  for (int w : wpcs) {
+    for (int sg : segs) {
+      batch_plan(b->b, sg, w);
       if (batch_upload(b->b) != RX_OK) return -RX_ERR_DEVICE;
       if (batch_run(b->b, precision, err) != RX_OK || batch_sync(b->b) != RX_OK) return -RX_ERR_DEVICE;
       auto t0 = std::chrono::steady_clock::now();
@@ -448,24 +439,24 @@ int rx_batch_autotune(rx_batch *b, int precision) {
       double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / reps;
       if (dt < best * 0.97) {
         best = dt;
-        best_r = r;
-        best_g = gr;
+        best_s = sg;
+        best_w = w;
       }
     }
   }
   bool found = false;
   for (auto &t : g.tune)
     if (t.shape == b->b.shape_class) {
-      t.n_rch = best_r;
-      t.groups = best_g;
+      t.seg_bins = best_s;
+      t.waves_per_cu = best_w;
       found = true;
     }
-  if (!found) g.tune.push_back({b->b.shape_class, best_r, best_g});
-  b->b.n_rch_forced = best_r;
-  b->b.groups_forced = best_g;
-  batch_plan(b->b, best_r, best_g);
+  if (!found) g.tune.push_back({b->b.shape_class, best_s, best_w});
+  b->b.seg_forced = best_s;
+  b->b.waves_forced = best_w;
+  batch_plan(b->b, best_s, best_w);
   if (batch_upload(b->b) != RX_OK) return -RX_ERR_DEVICE;
-  return best_r * 16 + best_g;
+  return best_s * 256 + best_w;
 }
 
 /* ---- reflection sets ---- */

@@ -233,6 +233,9 @@ int ensure_device(std::string &err) {
       err = "hipSetDevice failed";
       return RX_ERR_DEVICE;
     }
+    int n_cu = 0;
+    if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, g.device) == hipSuccess && n_cu > 0)
+      g.n_cu = n_cu;
     g.device_ready = true;
   } else {
     (void)hipSetDevice(g.device);
@@ -394,119 +397,42 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
     memcpy(&b.h_ener[p.ener_off], r.energy, sizeof(double) * (r.n_flux + 1));
     if (r.flux_in) memcpy(&b.h_flux_in[p.flux_off], r.flux_in, sizeof(double) * r.n_flux);
   }
-  b.n_tiles_base.assign(n_req, 0);
-  for (int i = 0; i < n_req; i++) b.n_tiles_base[i] = (b.h_req[i].n_e + 63) / 64;
-  int64_t o_chunk = 0;
   b.any_cosne = false;
   b.any_limb = false;
+  int64_t bins = 0;
   for (int i = 0; i < n_req; i++) {
-    b.h_req[i].chunk_off = o_chunk;
-    o_chunk += b.n_tiles_base[i];
     b.any_cosne |= b.h_req[i].need_cosne != 0;
     b.any_limb |= b.h_req[i].limb != 0;
+    bins += b.h_req[i].n_e;
   }
-  b.tot_chunks = o_chunk;
-  // shape class for the autotuner: log2 of the total bin chunks, conv flag
-  int64_t chunks = 0;
-  for (int i = 0; i < n_req; i++) chunks += b.n_tiles_base[i];
+  // shape class for the autotuner: log2 of the total profile bins, conv flag
   int lg = 0;
-  while ((1LL << lg) < chunks) lg++;
+  while ((1LL << lg) < bins) lg++;
   b.shape_class = lg * 2 + (b.h_conv.empty() ? 0 : 1);
-  batch_plan(b, b.n_rch_forced, b.groups_forced);
+  batch_plan(b, b.seg_forced, b.waves_forced);
   return batch_upload(b);
 }
 
-void batch_plan(Batch &b, int n_rch, int groups) {
+void batch_plan(Batch &b, int seg_bins, int waves_per_cu) {
   Global &g = G();
-  if (n_rch <= 0 || groups <= 0) {
+  if (seg_bins <= 0 || waves_per_cu <= 0) {
     for (auto &t : g.tune) {
       if (t.shape == b.shape_class) {
-        if (n_rch <= 0) n_rch = t.n_rch;
-        if (groups <= 0) groups = t.groups;
+        if (seg_bins <= 0) seg_bins = t.seg_bins;
+        if (waves_per_cu <= 0) waves_per_cu = t.waves_per_cu;
       }
     }
   }
-  if (groups != 1 && groups != 2 && groups != 4) groups = 1;
-  const int W = 64 / groups;
-  int64_t chunks = 0;
+  if (seg_bins <= 0) seg_bins = 64;
+  if (waves_per_cu <= 0) waves_per_cu = 8;
+  b.seg_bins = seg_bins;
+  b.waves_per_cu = waves_per_cu;
+  // work list bound: every radius chunk of every request covering all its bins
+  const int n_rc = (g.nr_fine + RAD_CHUNK - 1) / RAD_CHUNK;
+  int64_t cap = 0;
   for (int i = 0; i < b.n_req; i++)
-    if (b.h_status[i] == RX_OK) chunks += (b.h_req[i].n_e + W - 1) / W;
-  if (n_rch <= 0) {
-    // enough one-wave tiles to keep 256 CUs busy: ~16k waves
-    const int64_t target = 16384;
-    n_rch = (int)std::min<int64_t>(32, std::max<int64_t>(1, (target + chunks - 1) / std::max<int64_t>(1, chunks)));
-  }
-  b.n_rch = n_rch;
-  b.groups = groups;
-  // tiles, ordered by a cost proxy so that heavy tiles start first: bins near
-  // and above the line energy (Romberg region, many contributing radii)
-  struct TC {
-    Tile t;
-    float cost;
-  };
-  std::vector<TC> tc;
-  tc.reserve((size_t)chunks * n_rch);
-  for (int i = 0; i < b.n_req; i++) {
-    if (b.h_status[i] != RX_OK) continue;
-    const ReqParams &p = b.h_req[i];
-    const double *e = nullptr;
-    std::vector<double> tmp;
-    if (p.pgrid_off >= 0) {
-      tmp.resize(p.n_in + 1);
-      for (int k = 0; k <= p.n_in; k++) {
-        double x = b.h_ener[p.ener_off + k];
-        if (p.z > 0) x *= (1 + p.z);
-        tmp[k] = x / p.lineE;
-      }
-      e = tmp.data();
-    } else {
-      e = g.conv_ener.data();
-    }
-    // only bins that a window can reach: [min gmin, max gmax] of the table
-    int jlo = 0, jhi = p.n_e - 1;
-    while (jlo < p.n_e && !(e[jlo + 1] > g.rel_gmin_all * 0.999)) jlo++;
-    while (jhi >= 0 && !(e[jhi] < g.rel_gmax_all * 1.001)) jhi--;
-    b.h_req[i].tile_j0 = 0;
-    b.h_req[i].tile_j1 = 0;
-    if (jlo > jhi) continue;
-    const int c0 = jlo / W, c1 = jhi / W;
-    b.h_req[i].tile_j0 = c0 * W;
-    b.h_req[i].tile_j1 = std::min(p.n_e, (c1 + 1) * W);
-    for (int c = c0; c <= c1; c++) {
-      const int j0 = c * W, j1 = std::min(p.n_e, j0 + W);
-      const double lo = std::max(e[j0], 0.05), hi = std::min(e[j1], 1.6);
-      float cost = 0.01f;
-      if (hi > lo) cost += (float)(log(hi / lo) * (e[j1] > 0.95 ? 4.0 : 1.0) / log(e[j1] / e[j0]));
-      for (int r = 0; r < n_rch; r++) tc.push_back({{i, c, r, 0}, cost});
-    }
-  }
-  // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
-  // L2): put all tiles of a request in one XCD's queue (request % 8), each
-  // queue heavy-first, and interleave the queues so that tile index % 8 is
-  // the XCD.
-  constexpr int kXcd = 8;
-  std::vector<std::vector<TC>> qs(kXcd);
-  for (auto &x : tc) qs[x.t.req % kXcd].push_back(x);
-  for (auto &q : qs) std::stable_sort(q.begin(), q.end(), [](const TC &x, const TC &y) { return x.cost > y.cost; });
-  b.h_tiles.clear();
-  b.h_tiles.reserve(tc.size());
-  std::vector<size_t> pos(kXcd, 0);
-  bool more = true;
-  while (more) {
-    more = false;
-    for (int x = 0; x < kXcd; x++) {
-      if (pos[x] < qs[x].size()) {
-        b.h_tiles.push_back(qs[x][pos[x]++].t);
-        more = true;
-      }
-    }
-  }
-  int64_t o_part = 0;
-  for (int i = 0; i < b.n_req; i++) {
-    b.h_req[i].part_off = o_part;
-    o_part += (int64_t)n_rch * b.h_req[i].n_e;
-  }
-  b.tot_part = o_part;
+    if (b.h_status[i] == RX_OK) cap += (int64_t)n_rc * ((b.h_req[i].n_e + seg_bins - 1) / seg_bins);
+  b.item_cap = (int)std::min<int64_t>(std::max<int64_t>(cap, 1), INT32_MAX);
 }
 
 int batch_upload(Batch &b) {
@@ -524,29 +450,23 @@ int batch_upload(Batch &b) {
   e |= b.d_pgrid.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_pgrid));
   e |= b.d_node.ensure(sizeof(double) * nreq * 2 * (size_t)g.rel.n_r * g.rel.n_g * 2);
   e |= b.d_rad.ensure(sizeof(RadRec) * nreq * g.nr_fine);
-  e |= b.d_part.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_part));
   e |= b.d_prof.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_prof));
   e |= b.d_dist.ensure(sizeof(double) * nreq * MAX_INCL);
   e |= b.d_refl.ensure(sizeof(LpRefl) * nreq);
   e |= b.d_status.ensure(sizeof(int) * nreq);
-  e |= b.d_tiles.ensure(sizeof(Tile) * std::max<size_t>(1, b.h_tiles.size()));
   e |= b.d_conv.ensure(sizeof(int) * std::max<size_t>(1, b.h_conv.size()));
-  const size_t nfine = nreq * (size_t)g.nr_fine;
-  const size_t nq = 2 * (size_t)g.rel.n_g;
-  e |= b.d_irec.ensure(sizeof(IntRec) * nfine);
-  e |= b.d_f32.ensure(sizeof(float) * nfine * nq);
-  if (b.any_cosne) e |= b.d_f32c.ensure(sizeof(float) * nfine * nq);
-  e |= b.d_chunk.ensure(sizeof(int) * 2 * std::max<int64_t>(1, b.tot_chunks));
-  b.fix_cap = (int)std::min<int64_t>(1 << 22, std::max<int64_t>(1 << 16, b.tot_chunks * 64 / 8));
+  e |= b.d_items.ensure(sizeof(WorkItem) * (size_t)b.item_cap);
+  e |= b.d_ctr.ensure(sizeof(int) * N_COUNTERS);
+  // FP64 re-check queue: a few percent of the bins (overflow redoes the batch in FP64)
+  int64_t bins = 0;
+  for (int i = 0; i < b.n_req; i++) bins += b.h_req[i].n_e;
+  b.fix_cap = (int)std::min<int64_t>(1 << 22, std::max<int64_t>(1 << 16, bins / 8));
   e |= b.d_fix.ensure(sizeof(FixEntry) * (size_t)b.fix_cap);
-  e |= b.d_fixc.ensure(sizeof(int) * 2);
   if (e) return RX_ERR_NOMEM;
   e |= (int)hipMemcpyAsync(b.d_req.p, b.h_req.data(), sizeof(ReqParams) * nreq, hipMemcpyHostToDevice, b.stream);
   e |= (int)hipMemcpyAsync(b.d_ener.p, b.h_ener.data(), sizeof(double) * b.tot_ener, hipMemcpyHostToDevice, b.stream);
   e |= (int)hipMemcpyAsync(b.d_flux_in.p, b.h_flux_in.data(), sizeof(double) * b.tot_flux, hipMemcpyHostToDevice, b.stream);
   e |= (int)hipMemcpyAsync(b.d_status.p, b.h_status.data(), sizeof(int) * nreq, hipMemcpyHostToDevice, b.stream);
-  if (!b.h_tiles.empty())
-    e |= (int)hipMemcpyAsync(b.d_tiles.p, b.h_tiles.data(), sizeof(Tile) * b.h_tiles.size(), hipMemcpyHostToDevice, b.stream);
   if (!b.h_conv.empty())
     e |= (int)hipMemcpyAsync(b.d_conv.p, b.h_conv.data(), sizeof(int) * b.h_conv.size(), hipMemcpyHostToDevice, b.stream);
   e |= (int)hipStreamSynchronize(b.stream);
@@ -559,8 +479,7 @@ static BatchDev dev_view(Batch &b) {
   memset(&bd, 0, sizeof(bd));
   bd.n_req = b.n_req;
   bd.nr_fine = g.nr_fine;
-  bd.n_rch = b.n_rch;
-  bd.groups = b.groups;
+  bd.seg_bins = b.seg_bins;
   bd.node_stride = 2 * g.rel.n_r * g.rel.n_g * 2;
   bd.req = b.d_req.as<ReqParams>();
   bd.ener_in = b.d_ener.as<double>();
@@ -569,19 +488,14 @@ static BatchDev dev_view(Batch &b) {
   bd.pgrid = b.d_pgrid.as<double>();
   bd.node = b.d_node.as<double>();
   bd.rad = b.d_rad.as<RadRec>();
-  bd.irec = b.d_irec.as<IntRec>();
-  bd.part = b.d_part.as<double>();
   bd.prof = b.d_prof.as<double>();
   bd.dist = b.d_dist.as<double>();
   bd.refl = b.d_refl.as<LpRefl>();
   bd.status = b.d_status.as<int>();
-  bd.fine32 = b.d_f32.as<float>();
-  bd.fine32c = b.d_f32c.as<float>();
-  bd.fine64 = b.d_f64.as<double>();
-  bd.fine64c = b.d_f64c.as<double>();
-  bd.chunk_rng = b.d_chunk.as<int>();
+  bd.items = b.d_items.as<WorkItem>();
+  bd.item_cap = b.item_cap;
+  bd.ctr = b.d_ctr.as<int>();
   bd.fix = b.d_fix.as<FixEntry>();
-  bd.fix_count = b.d_fixc.as<int>();
   bd.fix_cap = b.fix_cap;
   bd.mode = b.last_mode;
   bd.guard = g.guard;
@@ -592,31 +506,23 @@ static BatchDev dev_view(Batch &b) {
 int batch_run(Batch &b, int precision, std::string &err) {
   Global &g = G();
   b.last_mode = (precision == RX_FP32) ? 1 : 0;
-  if (b.last_mode == 0) {
-    const size_t nfine = (size_t)b.n_req * g.nr_fine;
-    const size_t nq = 2 * (size_t)g.rel.n_g;
-    int e = b.d_f64.ensure(sizeof(double) * nfine * nq);
-    if (b.any_limb) e |= b.d_f64c.ensure(sizeof(double) * nfine * nq);
-    if (e) {
-      err = "device allocation failed";
-      return RX_ERR_NOMEM;
-    }
-  }
   BatchDev bd = dev_view(b);
   hipStream_t s = b.stream;
   const bool prof = g.profile;
+  const bool cos_rows = b.any_limb;
+  const int n_blocks = g.n_cu * b.waves_per_cu;
   int e = 0;
   if (prof) e |= (int)hipEventRecord(b.ev[0], s);
   e |= launch_setup(bd, s);
-  e |= launch_fine(bd, s);
   if (prof) e |= (int)hipEventRecord(b.ev[1], s);
-  e |= launch_integrate(bd, b.d_tiles.as<Tile>(), (int)b.h_tiles.size(), b.last_mode, b.groups, s);
-  e |= launch_fixup(bd, s);
+  e |= launch_integrate(bd, n_blocks, cos_rows, s);
   if (prof) e |= (int)hipEventRecord(b.ev[2], s);
-  e |= launch_finalize(bd, s);
+  e |= launch_fixup(bd, cos_rows, s);
   if (prof) e |= (int)hipEventRecord(b.ev[3], s);
-  e |= launch_conv(bd, b.d_conv.as<int>(), (int)b.h_conv.size(), s);
+  e |= launch_finalize(bd, s);
   if (prof) e |= (int)hipEventRecord(b.ev[4], s);
+  e |= launch_conv(bd, b.d_conv.as<int>(), (int)b.h_conv.size(), s);
+  if (prof) e |= (int)hipEventRecord(b.ev[5], s);
   b.profiled = prof;
   if (e) {
     err = "kernel launch failed";
@@ -628,25 +534,29 @@ int batch_run(Batch &b, int precision, std::string &err) {
 int batch_sync(Batch &b) {
   if (!b.stream) return RX_OK;
   if (hipStreamSynchronize(b.stream) != hipSuccess) return RX_ERR_DEVICE;
-  if (b.last_mode == 1) {
-    int fc[2] = {0, 0};
-    if (hipMemcpy(fc, b.d_fixc.p, sizeof fc, hipMemcpyDeviceToHost) != hipSuccess) return RX_ERR_DEVICE;
-    b.fix_overflow = fc[1] != 0;
-    if (b.fix_overflow) {
-      // the FP64 re-check queue overflowed: redo the batch in FP64
-      std::string err;
-      if (batch_run(b, RX_FP64, err) != RX_OK) return RX_ERR_DEVICE;
-      if (hipStreamSynchronize(b.stream) != hipSuccess) return RX_ERR_DEVICE;
-    }
+  int ctr[N_COUNTERS] = {};
+  if (hipMemcpy(ctr, b.d_ctr.p, sizeof ctr, hipMemcpyDeviceToHost) != hipSuccess) return RX_ERR_DEVICE;
+  b.fix_overflow = b.last_mode == 1 && ctr[CTR_FIX_OVF] != 0;
+  if (b.fix_overflow) {
+    // the FP64 re-check queue overflowed: redo the batch in FP64
+    std::string err;
+    if (batch_run(b, RX_FP64, err) != RX_OK) return RX_ERR_DEVICE;
+    if (hipStreamSynchronize(b.stream) != hipSuccess) return RX_ERR_DEVICE;
   }
   if (b.profiled) {
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < RX_N_TIMED; k++) {
       float ms = 0;
       (void)hipEventElapsedTime(&ms, b.ev[k], b.ev[k + 1]);
       b.kernel_ms[k] = ms;
     }
   }
   return RX_OK;
+}
+
+int public_status(int dev_status) {
+  if (dev_status == ST_WORKLIST) return RX_ERR_DEVICE;
+  if (dev_status >= 10 && dev_status < 50) return RX_ERR_TABLE;
+  return dev_status;
 }
 
 }  // namespace rx

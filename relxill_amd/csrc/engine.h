@@ -28,47 +28,47 @@ int n_model_params(int model);
 
 struct Batch {
   int n_req = 0;
-  int n_rch = 1;
-  int n_rch_forced = 0;
-  int groups = 1;         // radius groups per wavefront
-  int groups_forced = 0;
+  int seg_bins = 64;        // bins per work item
+  int seg_forced = 0;       // 0: automatic / autotuned
+  int waves_per_cu = 8;     // persistent integration wavefronts per CU
+  int waves_forced = 0;
   std::vector<ReqParams> h_req;
   std::vector<int> h_status;  // host-side parse status
   std::vector<double> h_ener, h_flux_in;
-  std::vector<Tile> h_tiles;
   std::vector<int> h_conv;
-  std::vector<int> n_tiles_base;  // per request bin chunks
-  int64_t tot_ener = 0, tot_flux = 0, tot_pgrid = 0, tot_part = 0, tot_prof = 0;
-  DevBuf d_req, d_ener, d_flux_in, d_out, d_pgrid, d_node, d_rad, d_part, d_prof, d_dist,
-      d_refl, d_status, d_tiles, d_conv;
-  DevBuf d_irec, d_f32, d_f32c, d_f64, d_f64c, d_chunk, d_fix, d_fixc;
-  int64_t tot_chunks = 0;
+  int64_t tot_ener = 0, tot_flux = 0, tot_pgrid = 0, tot_prof = 0;
+  DevBuf d_req, d_ener, d_flux_in, d_out, d_pgrid, d_node, d_rad, d_prof, d_dist, d_refl, d_status, d_conv;
+  DevBuf d_items, d_ctr, d_fix;
+  int item_cap = 0;
   bool any_cosne = false, any_limb = false;
   int fix_cap = 0;
   int last_mode = 1;
   bool fix_overflow = false;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[RX_N_TIMED + 1] = {};
   bool profiled = false;
-  double kernel_ms[4] = {0, 0, 0, 0};
+  double kernel_ms[RX_N_TIMED] = {};
   int shape_class = 0;
   ~Batch();
 };
 
 // Fills b (re-using device buffers when large enough) from the requests.
 int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err);
-// Plans tiles for n_rch radius chunks and `groups` radius groups per
-// wavefront (0: automatic / autotuned).
-void batch_plan(Batch &b, int n_rch, int groups);
+// Chooses the work-item segment length and the persistent wavefronts per CU
+// (0: automatic / autotuned) and sizes the work list.
+void batch_plan(Batch &b, int seg_bins, int waves_per_cu);
 int batch_upload(Batch &b);
 int batch_run(Batch &b, int precision, std::string &err);
 int batch_sync(Batch &b);
+// Public status of a request from its device status.
+int public_status(int dev_status);
 
 // global state
 struct Global {
   std::recursive_mutex mu;
   bool device_ready = false;
   int device = 0;
+  int n_cu = 256;
   RelTableHost rel;
   LpTableHost lp;
   bool rel_loaded = false, lp_loaded = false;
@@ -91,9 +91,9 @@ struct Global {
   bool refl_dirty = false;
   bool profile = false;
   float guard = 2e-3f;
-  // autotune cache: shape class -> (n_rch, groups)
+  // autotune cache: shape class -> (segment bins, waves per CU)
   struct Tuned {
-    int shape, n_rch, groups;
+    int shape, seg_bins, waves_per_cu;
   };
   std::vector<Tuned> tune;
 };

@@ -1,16 +1,17 @@
 // kernels.hip -- CDNA4 (gfx950) kernels of the relativistic smearing path.
 //
+//   k_reset      clears the run's counters
 //   k_setup      one workgroup per request: (a, mu0) bilinear table
 //                interpolation (interpol_relTable, src/Relprofile.cpp:141-303),
 //                fine radial grid, emissivity (src/Rellp.cpp:518-564), energy
-//                windows, angle distribution (src/Relprofile.cpp:908-938)
-//   k_integrate  one 64-lane wavefront per tile (request, 64-bin chunk,
-//                radius chunk): every lane owns one energy bin and sums the
-//                bin integral over the tile's radii (calc_relline_profile,
-//                src/Relprofile.cpp:857-900); the per-radius transfer
-//                function is lerped from the staged node into LDS
-//   k_finalize   one workgroup per request: radius-chunk reduction, photons
-//                per bin and renormalisation (src/Relprofile.cpp:749-796)
+//                windows, angle distribution (src/Relprofile.cpp:908-938), and
+//                the request's work items (radius chunk x bin segment)
+//   k_integrate  persistent 64-lane wavefronts over the work items: lane =
+//                fine radius, loop over the segment's bins, fixed-order wave
+//                sum per bin (calc_relline_profile, src/Relprofile.cpp:857-900)
+//   k_fixup      FP64 re-check of FP32 Romberg stop tests in the guard band
+//   k_finalize   one workgroup per request: photons per bin and
+//                renormalisation (src/Relprofile.cpp:749-796)
 //   k_conv       one workgroup per convolution request: rebin onto the
 //                convolution grid, real FFT convolution held in LDS with the
 //                1 keV rotation and E/dE weighting, band normalisation, rebin
@@ -19,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "rx_engine.h"
 #include "rx_math.h"
@@ -91,6 +93,12 @@ __device__ inline double relat_abberation(double del, double beta) {
   return acos((cos(del) - beta) / (1 - beta * cos(del)));
 }
 
+// Where a request's profile accumulates: the caller output for line models,
+// the convolution-grid profile for convolution models.
+__device__ inline double *profile_acc(const BatchDev &b, const ReqParams &rq) {
+  return (rq.prof_off >= 0) ? b.prof + rq.prof_off : b.out + rq.flux_off;
+}
+
 // block-wide deterministic sum (fixed tree order)
 __device__ inline double block_sum(double v, double *red) {
   const int tid = threadIdx.x;
@@ -117,10 +125,6 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b) {
   const DevRelTable &T = b.rt;
   const int nr = b.nr_fine;
   const int n_r = T.n_r, ng = T.n_g;
-  if (req == 0 && tid == 0) {  // reset the FP64 re-check queue of this run
-    b.fix_count[0] = 0;
-    b.fix_count[1] = 0;
-  }
   if (rq.pre_status != 0) return;  // status already set by the host
 
   __shared__ double s_tre[MAX_NR_TAB], s_tgmin[MAX_NR_TAB], s_tgmax[MAX_NR_TAB];
@@ -129,6 +133,14 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b) {
   __shared__ double s_dist[MAX_INCL];
   __shared__ int s_st, s_ia, s_im, s_rmin, s_lia, s_lih0, s_lih1;
   __shared__ double s_fa, s_fm, s_rms, s_lfa, s_lfh0, s_lfh1;
+  constexpr int MAX_RC = MAX_NR_FINE / RAD_CHUNK;
+  __shared__ int s_ulo[MAX_RC], s_uhi[MAX_RC], s_pref[MAX_RC + 1];
+  __shared__ int s_base;
+  const int n_rc = (b.nr_fine + RAD_CHUNK - 1) / RAD_CHUNK;
+  for (int c = tid; c < n_rc; c += SETUP_THREADS) {
+    s_ulo[c] = 0x7fffffff;
+    s_uhi[c] = -1;
+  }
 
   // profile grid: caller grid shifted to line-energy units (XspecSpectrum.h:61-76)
   const double *E;
@@ -318,6 +330,10 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b) {
     r.iehi = iehi;
     r.pad = 0;
     rad[ii] = r;
+    if (ielo <= iehi) {  // energy window of the radius chunk
+      atomicMin(&s_ulo[ii / RAD_CHUNK], ielo);
+      atomicMax(&s_uhi[ii / RAD_CHUNK], iehi);
+    }
   }
   if (err) atomicMax(&s_st, err);
   // BKN normalisation: sum emis * 2 pi r dr (norm_emis_profile, src/Rellp.cpp:91-109)
@@ -334,16 +350,6 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b) {
     r.emisw = emis * weight;
     r.coef = emis / (r.gmax - r.gmin) * weight;
     rad[ii] = r;
-  }
-
-  // chunk radius ranges start empty; k_fine fills them
-  {
-    int *cr = b.chunk_rng + 2 * rq.chunk_off;
-    const int n_chunks = (rq.n_e + 63) / 64;
-    for (int c = tid; c < n_chunks; c += SETUP_THREADS) {
-      cr[2 * c] = 0x7fffffff;
-      cr[2 * c + 1] = -1;
-    }
   }
 
   // reflection fractions (calc_refl_frac, src/Rellp.cpp:36-89)
@@ -417,323 +423,228 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b) {
     for (int k = tid; k < rq.n_incl; k += SETUP_THREADS) b.dist[(size_t)req * MAX_INCL + k] = s_dist[k];
   }
   __syncthreads();
+  if (s_st == 0) {
+    // the profile accumulates with atomic adds: clear it
+    double *acc = profile_acc(b, rq);
+    for (int j = tid; j < rq.n_e; j += SETUP_THREADS) acc[j] = 0.0;
+    // work items: each radius chunk's window cut into segments of seg_bins
+    const int S = b.seg_bins;
+    if (tid == 0) {
+      int tot = 0;
+      for (int c = 0; c < n_rc; c++) {
+        s_pref[c] = tot;
+        if (s_uhi[c] >= s_ulo[c]) tot += (s_uhi[c] - s_ulo[c] + S) / S;
+      }
+      s_pref[n_rc] = tot;
+      int base = atomicAdd(&b.ctr[CTR_ITEMS], tot);
+      if (base + tot > b.item_cap) {  // cannot happen: the host sizes the list for every bin
+        atomicOr(&b.ctr[CTR_ITEMS_OVF], 1);
+        s_st = ST_WORKLIST;
+        base = -1;
+      }
+      s_base = base;
+    }
+    __syncthreads();
+    if (s_base >= 0) {
+      for (int c = tid; c < n_rc; c += SETUP_THREADS) {
+        int k = s_base + s_pref[c];
+        for (int j0 = s_ulo[c]; j0 <= s_uhi[c]; j0 += S) {
+          WorkItem it;
+          it.req = req;
+          it.rc = c;
+          it.j0 = j0;
+          it.j1 = min(s_uhi[c] + 1, j0 + S);
+          b.items[k++] = it;
+        }
+      }
+    }
+    __syncthreads();
+  }
   if (tid == 0) b.status[req] = s_st;
 }
 
-// ---------------------------------------------------------------------------
-// integrate: the hot loop
-// ---------------------------------------------------------------------------
-// ---------------------------------------------------------------------------
-// fine: per fine radius, the transfer function (and cosines) lerped from the
-// table node onto the fine grid (src/Relprofile.cpp:280-293), the FP64 values
-// of the two edge brackets, the compact integration record, and the chunk
-// radius ranges.  One wavefront per radius (lanes over the g*-branch values).
-// ---------------------------------------------------------------------------
-constexpr int FINE_RADII_PER_BLOCK = 64;
+// Clears the run's counters; launched before k_setup on the batch's stream.
+__global__ __launch_bounds__(64) void k_reset(int *ctr) {
+  if (threadIdx.x < N_COUNTERS) ctr[threadIdx.x] = 0;
+}
 
-__global__ __launch_bounds__(256) void k_fine(BatchDev b, int blocks_per_req) {
-  __shared__ double sT[4][2 * MAX_NG];
-  __shared__ double sC[4][2 * MAX_NG];
-  const int req = blockIdx.x / blocks_per_req;
-  const int rb = blockIdx.x - req * blocks_per_req;
-  if (b.status[req] != 0) return;
-  const ReqParams &rq = b.req[req];
-  const int nr = b.nr_fine;
-  const int ng = b.rt.n_g;
-  const int nq = 2 * ng;
-  const size_t nq2 = (size_t)b.rt.n_r * nq;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+// ---------------------------------------------------------------------------
+// integrate: the hot loop (calc_relline_profile, src/Relprofile.cpp:857-900)
+// ---------------------------------------------------------------------------
+namespace {
+
+template <int CTRL>
+__device__ inline int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+
+template <int CTRL>
+__device__ inline double dpp_d(double v) {
+  const int lo = dpp_i<CTRL>(__double2loint(v));
+  const int hi = dpp_i<CTRL>(__double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+
+__device__ inline double readlane_d(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+// Sum over the 64 lanes in a fixed order (pairs, quads, half rows, rows, then
+// the four rows); every lane must be active.
+__device__ inline double wave_sum(double v) {
+  v += dpp_d<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_d<0x141>(v);  // row_half_mirror
+  v += dpp_d<0x140>(v);  // row_mirror
+  return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+}
+
+// The transfer function (and cosines) of one fine radius, lerped from its two
+// table-node rows into a lane-private LDS row, and the FP64 edge norms.
+template <class TV>
+__device__ inline void load_radius(const BatchDev &b, int req, const RadRec &r, int limb, TV *myT, TV *myC,
+                                   RadiusView &v) {
+  const int nq = 2 * b.rt.n_g;
   const double *node = b.node + (size_t)req * b.node_stride;
-  const size_t fbase = (size_t)req * nr * nq;
-  const bool cos_needed = rq.need_cosne != 0;
-  const int limb = rq.limb;
-  int *cr = b.chunk_rng + 2 * rq.chunk_off;
-  const int r_beg = rb * FINE_RADII_PER_BLOCK;
-  const int r_end = min(nr, r_beg + FINE_RADII_PER_BLOCK);
-  const double step = (1.0 - 2 * GFAC_H) / (ng - 1);
-  for (int base = r_beg; base < r_end; base += 4) {  // same trip count in every wave
-    const int ii = base + wave;
-    const bool act = ii < r_end;
-    RadRec r;
-    if (act) r = b.rad[(size_t)req * nr + ii];
-    if (act) {
-      for (int q = lane; q < nq; q += 64) {
-        const double t = lerp1(r.ifac_r, node[(size_t)(r.K + 1) * nq + q], node[(size_t)r.K * nq + q]);
-        sT[wave][q] = t;
-        b.fine32[fbase + (size_t)ii * nq + q] = (float)t;
-        if (b.mode == 0) b.fine64[fbase + (size_t)ii * nq + q] = t;
-        if (cos_needed) {
-          const double c = lerp1(r.ifac_r, node[nq2 + (size_t)(r.K + 1) * nq + q], node[nq2 + (size_t)r.K * nq + q]);
-          sC[wave][q] = c;
-          b.fine32c[fbase + (size_t)ii * nq + q] = (float)c;
-          if (b.mode == 0 && limb) b.fine64c[fbase + (size_t)ii * nq + q] = c;
-        }
-      }
-    }
-    __syncthreads();
-    if (act && lane == 0) {
-      RadiusView v;
-      v.gmin = r.gmin;
-      v.gmax = r.gmax;
-      v.del_g = r.del_g;
-      v.T64 = sT[wave];
-      v.C64 = sC[wave];
-      v.T32 = nullptr;
-      v.C32 = nullptr;
-      v.ng = ng;
-      v.limb = limb;
-      v.gs_step = step;
-      v.inv_step = 1.0 / step;
-      v.guard = 0.0f;
-      IntRec w;
-      w.gmin = r.gmin;
-      w.gmax = r.gmax;
-      w.del_g = r.del_g;
-      w.coef = r.coef;
-      w.normL = edge_norm64(false, v);
-      w.normU = edge_norm64(true, v);
-      w.ielo = r.ielo;
-      w.iehi = r.iehi;
-      b.irec[(size_t)req * nr + ii] = w;
-      if (r.ielo <= r.iehi) {
-        for (int c = r.ielo >> 6; c <= (r.iehi >> 6); c++) {
-          atomicMin(&cr[2 * c], ii);
-          atomicMax(&cr[2 * c + 1], ii);
-        }
-      }
-    }
-    __syncthreads();
-  }
+  const size_t nq2 = (size_t)b.rt.n_r * nq;
+  const double *tlo = node + (size_t)(r.K + 1) * nq;
+  const double *thi = node + (size_t)r.K * nq;
+  const double *clo = tlo + nq2;
+  const double *chi = thi + nq2;
+  for (int q = 0; q < nq; q++) myT[q] = (TV)node_lerp(r.ifac_r, tlo, thi, q);
+  if (limb)
+    for (int q = 0; q < nq; q++) myC[q] = (TV)node_lerp(r.ifac_r, clo, chi, q);
+  v.gmin = r.gmin;
+  v.gmax = r.gmax;
+  v.del_g = r.del_g;
+  v.limb = limb;
+  v.normL = edge_norm_node(false, v, r.ifac_r, tlo, thi, clo, chi);
+  v.normU = edge_norm_node(true, v, r.ifac_r, tlo, thi, clo, chi);
 }
 
-template <class T>
-__device__ inline T ld_const(const T *p) {
-  // uniform read-only data: scalar loads through the constant address space
-#if defined(__HIP_DEVICE_COMPILE__)
-  return *(const __attribute__((address_space(4))) T *)(p);
-#else
-  return *p;
-#endif
+__device__ inline const double *profile_grid(const BatchDev &b, const ReqParams *rq) {
+  const int64_t off = rq->pgrid_off;
+  return (off >= 0) ? b.pgrid + off : b.cg.ener;
 }
 
-// Tile = (request, W-bin chunk, radius chunk), W = 64/G.  Lane l works on bin
-// chunk*W + l%W for the radius group l/W: per iteration the wavefront stages G
-// consecutive fine radii (their transfer-function values are contiguous in
-// memory) and each group integrates its radius for its W bins.  The next
-// iteration's values and records are prefetched while the current one is
-// integrated.  Sums: FP64 (MODE 0) or Kahan-compensated FP32 (MODE 1) per lane,
-// combined over the groups in a fixed order at the end.
-template <int MODE, int G>
-__global__ __launch_bounds__(64) void k_integrate(BatchDev b, const Tile *__restrict__ tiles) {
-  constexpr int W = 64 / G;
-  constexpr int UQ = (G * 2 * MAX_NG + 63) / 64;  // value slots per lane
+}  // namespace
+
+// Persistent wavefronts pull work items (request, chunk of 64 fine radii, bin
+// segment) from a shared counter until the list is exhausted.  Lane l owns
+// fine radius 64 rc + l: its transfer-function row sits in LDS and its window,
+// emissivity factor and edge norms in registers.  For every bin of the
+// segment each lane integrates the bin at its radius (zero outside its
+// window), the wavefront sums the 64 values in FP64 in a fixed order and adds
+// the sum to the request's profile.  MODE 0: FP64; MODE 1: FP32 integrand with
+// FP64 decisions and edges; bins whose Romberg stop test fell in the guard
+// band are queued for the FP64 re-check (k_fixup).
+template <int MODE>
+__global__ __launch_bounds__(64) void k_integrate(BatchDev b, int cos_rows) {
+  using TV = typename std::conditional<MODE == 0, double, float>::type;
   extern __shared__ double s_dyn[];
-
-  const Tile tl = ld_const(tiles + blockIdx.x);
-  const int req = tl.req;
   const int lane = threadIdx.x;
-  const int grp = lane / W;
-  const int bl = lane - grp * W;
-  const ReqParams *rqp = b.req + req;
-  const int n_e = ld_const(&rqp->n_e);
-  const int limb = ld_const(&rqp->limb);
-  const int64_t part_off = ld_const(&rqp->part_off);
-  const int64_t pgrid_off = ld_const(&rqp->pgrid_off);
-  const int64_t chunk_off = ld_const(&rqp->chunk_off);
-  const int j0 = tl.bchunk * W;
-  const int j = j0 + bl;
-  const bool valid = j < n_e;
-  double *part = b.part + part_off + (size_t)tl.rch * n_e;
-  const int nr = b.nr_fine;
-  const int rlen = (nr + b.n_rch - 1) / b.n_rch;
-  int i0 = tl.rch * rlen;
-  int i1 = min(nr, i0 + rlen);
-  if (ld_const(b.status + req) != 0) {
-    i1 = i0;
-  } else {
-    const int c64 = j0 >> 6;
-    const int first = ld_const(b.chunk_rng + 2 * (chunk_off + c64));
-    const int last = ld_const(b.chunk_rng + 2 * (chunk_off + c64) + 1);
-    i0 = max(i0, first);
-    i1 = min(i1, last + 1);
-  }
-  if (i0 >= i1) {
-    if (valid && grp == 0) part[j] = 0.0;
-    return;
-  }
   const int ng = b.rt.n_g;
-  const int nq = 2 * ng;
-  const int gq = G * nq;
-  const double *E = (pgrid_off >= 0) ? b.pgrid + pgrid_off : b.cg.ener;
-  const double elo = valid ? E[j] : 0.0;
-  const double ehi = valid ? E[j + 1] : 0.0;
-  const IntRec *irec = b.irec + (size_t)req * nr;
-  const size_t fbase = (size_t)req * nr * nq;
-  const float *f32 = b.fine32 + fbase;
-  const float *f32c = limb ? b.fine32c + fbase : nullptr;
-  const double *f64 = (MODE == 0) ? b.fine64 + fbase : nullptr;
-  const double *f64c = (MODE == 0 && limb) ? b.fine64c + fbase : nullptr;
-
-  // LDS per buffer: doubles first, then floats
-  const int nd = (MODE == 0) ? gq * (limb ? 2 : 1) : 0;
-  const int nf = (MODE == 1) ? gq * (limb ? 2 : 1) : 0;
-  const int buf_doubles = nd + (nf + 1) / 2;
-
-  const double step = (1.0 - 2 * GFAC_H) / (ng - 1);
+  const int RS = 2 * ng + 1;  // odd row stride: lanes' rows start in distinct banks
+  TV *rows = reinterpret_cast<TV *>(s_dyn);
+  TV *myT = rows + lane * RS;
+  TV *myC = cos_rows ? rows + (64 + lane) * RS : myT;
+  const int n_items = min(b.ctr[CTR_ITEMS], b.item_cap);
+  const int nr = b.nr_fine;
   RadiusView v;
   v.ng = ng;
-  v.limb = limb;
-  v.gs_step = step;
-  v.inv_step = 1.0 / step;
+  v.gs_step = (1.0 - 2 * GFAC_H) / (ng - 1);
+  v.inv_step = 1.0 / v.gs_step;
   v.guard = b.guard;
-
-  float pt[UQ], pc[UQ];
-  double pd[UQ], pdc[UQ];
-  IntRec nrec;
-  auto fetch = [&](int base) {
-    const size_t o = (size_t)base * nq;
-    const int lim = (i1 - base) * nq;  // values of radii < i1
-#pragma unroll
-    for (int u = 0; u < UQ; u++) {
-      const int k = lane + 64 * u;
-      if (k < gq && k < lim) {
-        if (MODE == 1) {
-          pt[u] = f32[o + k];
-          if (limb) pc[u] = f32c[o + k];
-        } else {
-          pd[u] = f64[o + k];
-          if (limb) pdc[u] = f64c[o + k];
-        }
-      }
+  v.T64 = nullptr;
+  v.C64 = nullptr;
+  v.T32 = nullptr;
+  v.C32 = nullptr;
+  if (MODE == 0) {
+    v.T64 = reinterpret_cast<const double *>(myT);
+    v.C64 = reinterpret_cast<const double *>(myC);
+  } else {
+    v.T32 = reinterpret_cast<const float *>(myT);
+    v.C32 = reinterpret_cast<const float *>(myC);
+  }
+  for (;;) {
+    int k = 0;
+    if (lane == 0) k = atomicAdd(&b.ctr[CTR_NEXT], 1);
+    k = __builtin_amdgcn_readfirstlane(k);
+    if (k >= n_items) break;
+    const WorkItem it = b.items[k];
+    const ReqParams *rq = b.req + it.req;
+    const int limb = rq->limb;
+    const int i = it.rc * RAD_CHUNK + lane;
+    RadRec r;
+    r.ielo = 1;
+    r.iehi = 0;
+    r.coef = 0.0;
+    if (i < nr) {
+      r = b.rad[(size_t)it.req * nr + i];
+      if (r.ielo <= r.iehi) load_radius<TV>(b, it.req, r, limb, myT, myC, v);
     }
-    const int mine = base + grp;
-    if (mine < i1) nrec = irec[mine];
-  };
-  fetch(i0);
-
-  double acc = 0.0;
-  float accf = 0.0f, comp = 0.0f;
-  int buf = 0;
-  for (int base = i0; base < i1; base += G) {
-    double *bd = s_dyn + buf * buf_doubles;
-    float *bf = reinterpret_cast<float *>(bd + nd);
-#pragma unroll
-    for (int u = 0; u < UQ; u++) {
-      const int k = lane + 64 * u;
-      if (k < gq) {
-        if (MODE == 1) {
-          bf[k] = pt[u];
-          if (limb) bf[gq + k] = pc[u];
+    const double *E = profile_grid(b, rq);
+    double *acc = profile_acc(b, *rq);
+    for (int j = it.j0; j < it.j1; j++) {
+      const double elo = E[j], ehi = E[j + 1];
+      TV c = (TV)0;
+      if (j >= r.ielo && j <= r.iehi) {
+        if (MODE == 0) {
+          c = (TV)(integ_bin<0>(elo, ehi, v) * r.coef);
         } else {
-          bd[k] = pd[u];
-          if (limb) bd[gq + k] = pdc[u];
-        }
-      }
-    }
-    const IntRec r = nrec;
-    const int i = base + grp;
-    __syncthreads();
-    if (base + G < i1) fetch(base + G);
-    if (i < i1 && valid && j >= r.ielo && j <= r.iehi) {
-      v.gmin = r.gmin;
-      v.gmax = r.gmax;
-      v.del_g = r.del_g;
-      v.normL = r.normL;
-      v.normU = r.normU;
-      if (MODE == 0) {
-        v.T64 = bd + grp * nq;
-        v.C64 = bd + gq + grp * nq;
-        v.T32 = nullptr;
-        v.C32 = nullptr;
-        acc += integ_bin<0>(elo, ehi, v) * r.coef;
-      } else {
-        v.T64 = nullptr;
-        v.C64 = nullptr;
-        v.T32 = bf + grp * nq;
-        v.C32 = bf + gq + grp * nq;
-        bool need64 = false;
-        const float c32 = (float)(integ_bin<1>(elo, ehi, v, &need64) * r.coef);
-        const float y = c32 - comp;  // Kahan-compensated FP32 sum over radii
-        const float t = accf + y;
-        comp = (t - accf) - y;
-        accf = t;
-        if (need64) {
-          const int slot = atomicAdd(&b.fix_count[0], 1);
-          if (slot < b.fix_cap) {
-            FixEntry fe;
-            fe.req = req;
-            fe.rad = i;
-            fe.bin = j;
-            fe.rch = tl.rch;
-            fe.v32 = (double)c32;
-            b.fix[slot] = fe;
-          } else {
-            atomicOr(&b.fix_count[1], 1);
+          bool need64 = false;
+          c = (TV)(integ_bin<1>(elo, ehi, v, &need64) * r.coef);
+          if (need64) {
+            const int slot = atomicAdd(&b.ctr[CTR_FIX], 1);
+            if (slot < b.fix_cap) {
+              FixEntry fe;
+              fe.req = it.req;
+              fe.rad = i;
+              fe.bin = j;
+              fe.pad = 0;
+              fe.v32 = (double)c;
+              b.fix[slot] = fe;
+            } else {
+              atomicOr(&b.ctr[CTR_FIX_OVF], 1);
+            }
           }
         }
       }
-    }
-    buf ^= 1;
-  }
-  // combine the radius groups of each bin in group order
-  double mine = (MODE == 0) ? acc : ((double)accf - (double)comp);
-  if (G > 1) {
-    __syncthreads();
-    s_dyn[lane] = mine;
-    __syncthreads();
-    if (grp == 0) {
-      mine = 0.0;
-#pragma unroll
-      for (int gg = 0; gg < G; gg++) mine += s_dyn[gg * W + bl];
+      // FP64 sum of the 64 per-radius values: the rounding of the sum over
+      // radii stays at FP64 level whatever the number of radii
+      const double sum = wave_sum((double)c);
+      if (lane == 0) unsafeAtomicAdd(&acc[j], sum);
     }
   }
-  if (valid && grp == 0) part[j] = mine;
 }
 
-// FP64 re-check of guarded FP32 bins: one wavefront per entry, grid-stride over
-// the queue filled by k_integrate<1>.
-__global__ __launch_bounds__(64) void k_fixup(BatchDev b) {
-  __shared__ double sT64[2 * MAX_NG];
-  __shared__ double sC64[2 * MAX_NG];
-  const int n = min(b.fix_count[0], b.fix_cap);
+// FP64 re-check of guarded FP32 bins, one entry per lane.
+__global__ __launch_bounds__(64) void k_fixup(BatchDev b, int cos_rows) {
+  extern __shared__ double s_dyn[];
   const int lane = threadIdx.x;
   const int ng = b.rt.n_g;
-  const int nq = 2 * ng;
-  const size_t nq2 = (size_t)b.rt.n_r * nq;
-  const double step = (1.0 - 2 * GFAC_H) / (ng - 1);
-  for (int e = blockIdx.x; e < n; e += gridDim.x) {
+  const int RS = 2 * ng + 1;
+  double *myT = s_dyn + lane * RS;
+  double *myC = cos_rows ? s_dyn + (64 + lane) * RS : myT;
+  const int n = min(b.ctr[CTR_FIX], b.fix_cap);
+  RadiusView v;
+  v.ng = ng;
+  v.gs_step = (1.0 - 2 * GFAC_H) / (ng - 1);
+  v.inv_step = 1.0 / v.gs_step;
+  v.guard = b.guard;
+  v.T64 = myT;
+  v.C64 = myC;
+  v.T32 = nullptr;
+  v.C32 = nullptr;
+  for (int e = blockIdx.x * 64 + lane; e < n; e += gridDim.x * 64) {
     const FixEntry fe = b.fix[e];
-    const ReqParams &rq = b.req[fe.req];
+    const ReqParams *rq = b.req + fe.req;
     const RadRec r = b.rad[(size_t)fe.req * b.nr_fine + fe.rad];
-    const IntRec ir = b.irec[(size_t)fe.req * b.nr_fine + fe.rad];
-    const double *node = b.node + (size_t)fe.req * b.node_stride;
-    for (int q = lane; q < nq; q += 64) {
-      sT64[q] = lerp1(r.ifac_r, node[(size_t)(r.K + 1) * nq + q], node[(size_t)r.K * nq + q]);
-      if (rq.limb) sC64[q] = lerp1(r.ifac_r, node[nq2 + (size_t)(r.K + 1) * nq + q], node[nq2 + (size_t)r.K * nq + q]);
-    }
-    __syncthreads();
-    if (lane == 0) {
-      const double *E = (rq.pgrid_off >= 0) ? b.pgrid + rq.pgrid_off : b.cg.ener;
-      RadiusView v;
-      v.ng = ng;
-      v.limb = rq.limb;
-      v.gs_step = step;
-      v.inv_step = 1.0 / step;
-      v.guard = b.guard;
-      v.gmin = r.gmin;
-      v.gmax = r.gmax;
-      v.del_g = r.del_g;
-      v.normL = ir.normL;
-      v.normU = ir.normU;
-      v.T64 = sT64;
-      v.C64 = sC64;
-      v.T32 = nullptr;
-      v.C32 = nullptr;
-      const double val = integ_bin<0>(E[fe.bin], E[fe.bin + 1], v) * r.coef;
-      atomicAdd(&b.part[rq.part_off + (size_t)fe.rch * rq.n_e + fe.bin], val - fe.v32);
-    }
-    __syncthreads();
+    load_radius<double>(b, fe.req, r, rq->limb, myT, myC, v);
+    const double *E = profile_grid(b, rq);
+    const double val = integ_bin<0>(E[fe.bin], E[fe.bin + 1], v) * r.coef;
+    unsafeAtomicAdd(&profile_acc(b, *rq)[fe.bin], val - fe.v32);
   }
 }
 
@@ -746,21 +657,16 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_finalize(BatchDev b) {
   const int tid = threadIdx.x;
   const ReqParams rq = b.req[req];
   const int n_e = rq.n_e;
-  double *dst = (rq.prof_off >= 0) ? b.prof + rq.prof_off : b.out + rq.flux_off;
+  double *dst = profile_acc(b, rq);
   const int st = b.status[req];
   if (st != 0) {
     for (int j = tid; j < n_e; j += SETUP_THREADS) dst[j] = 0.0;
     return;
   }
   const double *E = (rq.pgrid_off >= 0) ? b.pgrid + rq.pgrid_off : b.cg.ener;
-  const double *part = b.part + rq.part_off;
   double loc = 0.0;
   for (int j = tid; j < n_e; j += SETUP_THREADS) {
-    double v = 0.0;
-    if (j >= rq.tile_j0 && j < rq.tile_j1) {
-      for (int c = 0; c < b.n_rch; c++) v += part[(size_t)c * n_e + j];
-    }
-    v /= 0.5 * (E[j] + E[j + 1]);
+    const double v = dst[j] / (0.5 * (E[j] + E[j + 1]));
     dst[j] = v;
     loc += v;
   }
@@ -1014,47 +920,34 @@ __global__ __launch_bounds__(256) void k_copy(const double4 *__restrict__ src, d
 // ---------------------------------------------------------------------------
 int launch_setup(const BatchDev &b, void *stream) {
   if (b.n_req <= 0) return 0;
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, (hipStream_t)stream, b.ctr);
   hipLaunchKernelGGL(k_setup, dim3(b.n_req), dim3(SETUP_THREADS), 0, (hipStream_t)stream, b);
   return (int)hipGetLastError();
 }
 
-template <int MODE, int G>
-static void launch_int(const BatchDev &b, const Tile *tiles, int n_tiles, hipStream_t st) {
-  const int nq = 2 * b.rt.n_g;
-  const int gq = G * nq;
-  const bool limb = true;  // size for the largest layout (cosines present)
-  const int nd = (MODE == 0) ? gq * (limb ? 2 : 1) : 0;
-  const int nf = (MODE == 1) ? gq * (limb ? 2 : 1) : 0;
-  const size_t per_buf = (size_t)(nd + (nf + 1) / 2);
-  const size_t lds = std::max<size_t>(2 * per_buf, 64) * sizeof(double);
-  hipLaunchKernelGGL((k_integrate<MODE, G>), dim3(n_tiles), dim3(64), lds, st, b, tiles);
+static size_t row_lds(const BatchDev &b, bool cos_rows, size_t elem) {
+  return (size_t)(cos_rows ? 2 : 1) * 64 * (2 * b.rt.n_g + 1) * elem;
 }
 
-int launch_integrate(const BatchDev &b, const Tile *tiles, int n_tiles, int mode, int groups, void *stream) {
-  if (n_tiles <= 0) return 0;
+int launch_integrate(const BatchDev &b, int n_blocks, bool cos_rows, void *stream) {
+  if (b.n_req <= 0 || n_blocks <= 0) return 0;
   hipStream_t st = (hipStream_t)stream;
-  if (mode == 0) {
-    if (groups == 4) launch_int<0, 4>(b, tiles, n_tiles, st);
-    else if (groups == 2) launch_int<0, 2>(b, tiles, n_tiles, st);
-    else launch_int<0, 1>(b, tiles, n_tiles, st);
+  if (b.mode == 0) {
+    const size_t lds = row_lds(b, cos_rows, sizeof(double));
+    (void)hipFuncSetAttribute((const void *)k_integrate<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_integrate<0>, dim3(n_blocks), dim3(64), lds, st, b, (int)cos_rows);
   } else {
-    if (groups == 4) launch_int<1, 4>(b, tiles, n_tiles, st);
-    else if (groups == 2) launch_int<1, 2>(b, tiles, n_tiles, st);
-    else launch_int<1, 1>(b, tiles, n_tiles, st);
+    const size_t lds = row_lds(b, cos_rows, sizeof(float));
+    hipLaunchKernelGGL(k_integrate<1>, dim3(n_blocks), dim3(64), lds, st, b, (int)cos_rows);
   }
   return (int)hipGetLastError();
 }
 
-int launch_fine(const BatchDev &b, void *stream) {
-  if (b.n_req <= 0) return 0;
-  const int bpr = (b.nr_fine + FINE_RADII_PER_BLOCK - 1) / FINE_RADII_PER_BLOCK;
-  hipLaunchKernelGGL(k_fine, dim3(b.n_req * bpr), dim3(256), 0, (hipStream_t)stream, b, bpr);
-  return (int)hipGetLastError();
-}
-
-int launch_fixup(const BatchDev &b, void *stream) {
+int launch_fixup(const BatchDev &b, bool cos_rows, void *stream) {
   if (b.mode != 1 || b.n_req <= 0) return 0;
-  hipLaunchKernelGGL(k_fixup, dim3(1024), dim3(64), 0, (hipStream_t)stream, b);
+  const size_t lds = row_lds(b, cos_rows, sizeof(double));
+  (void)hipFuncSetAttribute((const void *)k_fixup, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(k_fixup, dim3(256), dim3(64), lds, (hipStream_t)stream, b, (int)cos_rows);
   return (int)hipGetLastError();
 }
 

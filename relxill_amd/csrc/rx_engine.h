@@ -6,15 +6,17 @@
 //                       get_rel_params, src/ModelDefinition.cpp:278-334)
 //   ener_in             caller energy grids, n_in+1 edges each
 //   flux_in             caller input spectra (convolution models)
-//   out                 caller-grid outputs, n_in each
+//   out                 caller-grid outputs, n_in each (line models accumulate
+//                       their profile here directly)
 //   pgrid               profile grids (line models: the caller grid shifted
 //                       to line-energy units; conv models use the shared
 //                       convolution grid instead)
 //   node[n]             (a,mu0)-interpolated table node, FP64:
 //                       trff[n_r][n_g][2] followed by cosne[n_r][n_g][2]
 //   rad[n][nr_fine]     per fine radius record (RadRec)
-//   part                per request n_rch partial profiles of n_e bins
-//   prof                per request final profile on the conv grid (conv models)
+//   prof                per request profile on the conv grid (conv models)
+//   items               work list written by k_setup: (request, chunk of 64
+//                       fine radii, segment of at most seg_bins bins)
 //   dist[n][n_incl]     emission-angle distribution (relxill-style requests)
 #pragma once
 #include <stdint.h>
@@ -51,10 +53,7 @@ struct ReqParams {
   int64_t ener_off;   // caller grid edges in ener_in
   int64_t flux_off;   // caller input / output spectrum offset (n_in)
   int64_t pgrid_off;  // profile grid edges in pgrid (-1: shared conv grid)
-  int64_t part_off;   // partial profiles (n_rch * n_e)
-  int64_t prof_off;   // final profile (n_e) for conv models, -1 otherwise
-  int64_t chunk_off;  // per 64-bin chunk radius ranges (int2) in chunk_rng
-  int tile_j0, tile_j1;  // bins covered by integration tiles (others are 0)
+  int64_t prof_off;   // profile (n_e) for conv models, -1 otherwise
   int refl_set;       // relxill-style: which resident reflection set
   int pre_status;     // host-side parameter status; nonzero requests are skipped
 };
@@ -67,13 +66,6 @@ struct RadRec {
   int ielo;    // first bin of the window (ielo > iehi: radius does not contribute)
   int iehi;    // last bin of the window (inclusive)
   int pad;
-};
-
-// What the integration reads per fine radius (56 bytes): the window, the
-// factor emis*weight/(gmax-gmin) and the two int_edge norms.
-struct IntRec {
-  double gmin, gmax, del_g, coef, normL, normU;
-  int ielo, iehi;
 };
 
 struct LpRefl {
@@ -108,22 +100,42 @@ struct DevConvGrid {
   const double *tw;    // twiddles exp(-2 pi i k/n), k < n/2, interleaved re/im
 };
 
-struct Tile {
-  int req, bchunk, rch, pad;
+// Fine radii are integrated in chunks of one wavefront: lane l of chunk c
+// owns fine radius 64 c + l.
+constexpr int RAD_CHUNK = 64;
+constexpr int MAX_NR_FINE = 64 * RAD_CHUNK;
+
+// One unit of integration work: bins [j0, j1) of one request for the fine
+// radii of one chunk.  k_setup emits the segments of each chunk's energy
+// window (union over its radii), k_integrate pulls them from a shared counter.
+struct WorkItem {
+  int req, rc, j0, j1;
 };
 
 // FP32 bins whose Romberg stop test fell in the guard band: recomputed in FP64
-// by k_fixup, which adds (FP64 value - FP32 value) to the partial profile.
+// by k_fixup, which adds (FP64 value - FP32 value) to the profile.
 struct FixEntry {
-  int req, rad, bin, rch;
+  int req, rad, bin, pad;
   double v32;  // contribution already accumulated (float value, widened)
+};
+
+// k_setup status of a request whose work items did not fit the work list.
+constexpr int ST_WORKLIST = 60;
+
+// Device counters of one batch run (cleared by k_reset before k_setup).
+enum CounterSlot : int {
+  CTR_FIX = 0,        // FixEntry entries written
+  CTR_FIX_OVF = 1,    // FixEntry queue overflowed
+  CTR_ITEMS = 2,      // work items written
+  CTR_ITEMS_OVF = 3,  // work list overflowed
+  CTR_NEXT = 4,       // next work item to integrate
+  N_COUNTERS = 8
 };
 
 struct BatchDev {
   int n_req;
   int nr_fine;
-  int n_rch;      // radius chunks per request
-  int groups;     // radius groups per wavefront (chunk width 64/groups bins)
+  int seg_bins;     // bins per work item (at most)
   int node_stride;  // doubles per request in node (2 * n_r * n_g * 2)
   ReqParams *req;
   const double *ener_in;
@@ -132,20 +144,14 @@ struct BatchDev {
   double *pgrid;
   double *node;
   RadRec *rad;
-  IntRec *irec;
-  double *part;
   double *prof;
   double *dist;
   LpRefl *refl;
   int *status;
-  // fine-radius transfer function [req][nr_fine][2*ng] (FP32 always, FP64 in
-  // FP64 mode); the *c arrays hold the emission-angle cosines (limb
-  // darkening / angle distribution), else null
-  float *fine32, *fine32c;
-  double *fine64, *fine64c;
-  int *chunk_rng;     // [sum of chunks][2]: first/last radius touching the chunk
+  WorkItem *items;
+  int item_cap;
+  int *ctr;           // CounterSlot
   FixEntry *fix;
-  int *fix_count;     // [0] entries, [1] overflow flag
   int fix_cap;
   int mode;           // 0 FP64, 1 FP32
   float guard;        // FP32 Romberg guard band (relative)
@@ -157,9 +163,8 @@ struct BatchDev {
 
 // ---- launchers (kernels.hip) ----
 int launch_setup(const BatchDev &b, void *stream);
-int launch_fine(const BatchDev &b, void *stream);
-int launch_integrate(const BatchDev &b, const Tile *tiles, int n_tiles, int mode, int groups, void *stream);
-int launch_fixup(const BatchDev &b, void *stream);
+int launch_integrate(const BatchDev &b, int n_blocks, bool limb, void *stream);
+int launch_fixup(const BatchDev &b, bool cos_rows, void *stream);
 int launch_finalize(const BatchDev &b, void *stream);
 int launch_conv(const BatchDev &b, const int *conv_reqs, int n_conv_reqs, void *stream);
 int launch_copy(const void *src, void *dst, size_t bytes, void *stream);

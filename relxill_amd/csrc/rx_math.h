@@ -65,6 +65,11 @@ struct RadiusView {
   float guard;      // FP32 mode: relative guard band around the Romberg threshold
 };
 
+RX_HD inline double lerp1_(double ifac, double lo, double hi) {
+#pragma clang fp contract(off)
+  return ifac * hi + (1.0 - ifac) * lo;  // interp_lin_1d, src/relutility.c:24-26
+}
+
 RX_HD inline double gstar_node(double step, int i) {
 #pragma clang fp contract(off)
   return GFAC_H + step * (double)((float)i);
@@ -142,6 +147,34 @@ RX_HD inline double edge_norm64(bool upper, const RadiusView &v) {
   const double e = (hex * (v.gmax - v.gmin) + v.gmin) * 1.0;
   double f0, f1;
   eval64_pair(e, v, f0, f1);
+  double norm = 0.0;
+  norm = norm + f0;
+  norm = norm + f1;
+  return norm * sqrt(GFAC_H);
+}
+
+// Transfer-function value of one fine radius lerped from its two table-node
+// rows (src/Relprofile.cpp:280-293): lo = node K+1, hi = node K.
+RX_HD inline double node_lerp(double ifac, const double *lo, const double *hi, int q) {
+  return lerp1_(ifac, lo[q], hi[q]);
+}
+
+// edge_norm64 with the FP64 values taken from the node rows directly (the
+// same arithmetic as a lerped row, so the same bits).  c_lo/c_hi: cosines.
+RX_HD inline double edge_norm_node(bool upper, const RadiusView &v, double ifac, const double *t_lo,
+                                   const double *t_hi, const double *c_lo, const double *c_hi) {
+#pragma clang fp contract(off)
+  const double hex = upper ? 1.0 - GFAC_H : GFAC_H;
+  const double e = (hex * (v.gmax - v.gmin) + v.gmin) * 1.0;
+  const double egs = (e - v.gmin) * v.del_g;
+  const int ind = g_bracket(egs, v.ng, v.gs_step, v.inv_step);
+  double t4[4], c4[4];
+  for (int k = 0; k < 4; k++) {
+    t4[k] = node_lerp(ifac, t_lo, t_hi, ind * 2 + k);
+    c4[k] = v.limb ? node_lerp(ifac, c_lo, c_hi, ind * 2 + k) : 0.0;
+  }
+  double f0, f1;
+  eval64_at(e, egs, ind, v, t4, c4, f0, f1);
   double norm = 0.0;
   norm = norm + f0;
   norm = norm + f1;
@@ -307,6 +340,20 @@ RX_HD inline void eval32s(float s, const BinF &bf, const RadiusView &v, float &f
   }
 }
 
+// Stop test of the reference, ob = |R - top| / R > prec (continue), written
+// without the division: ob > prec  <=>  (R > 0 and |d| > prec R) or (R == 0
+// and d != 0, ob = inf); R < 0 or NaN gives ob <= 0 or NaN (stop).  A test
+// inside the guard band lo <= ob <= hi (R > 0) sets *need64.
+RX_HD inline bool romb_continue32(float R, float top, float lo, float hi, int n, bool *need64) {
+  const float d = fabsf(R - top);
+  if (n > ROMB_ITERMAX) return false;
+  if (R > 0.0f) {
+    if (d >= lo * R && d <= hi * R) *need64 = true;
+    return d > (float)ROMB_PREC * R;
+  }
+  return R == 0.0f && d > 0.0f;
+}
+
 // Romberg for both branches, FP32 values.  Sets *need64 when a stop decision
 // fell inside the guard band; the caller then recomputes in FP64.
 RX_HD inline float romberg32_pair(const BinF &bf, const RadiusView &v, bool *need64) {
@@ -330,11 +377,13 @@ RX_HD inline float romberg32_pair(const BinF &bf, const RadiusView &v, bool *nee
     const int nnew = 1 << (n - 1);
     const float ds = 1.0f / (float)(2 * nnew);
     float add0 = 0.0f, add1 = 0.0f;
+    float sm = ds;  // (2m+1) ds, exact: dyadic with at most 7 bits
     for (int m = 0; m < nnew; m++) {
       float g0v, g1v;
-      eval32s((float)(2 * m + 1) * ds, bf, v, g0v, g1v);
+      eval32s(sm, bf, v, g0v, g1v);
       add0 += g0v;
       add1 += g1v;
+      sm += 2.0f * ds;
     }
     s0 += add0;
     s1 += add1;
@@ -359,15 +408,11 @@ RX_HD inline float romberg32_pair(const BinF &bf, const RadiusView &v, bool *nee
     }
     if (run0) {
       res0 = R0[n];
-      const float ob = fabsf(R0[n] - top0) / R0[n];
-      if (ob >= lo && ob <= hi && n <= ROMB_ITERMAX) *need64 = true;
-      run0 = (ob > (float)ROMB_PREC) && (n <= ROMB_ITERMAX);
+      run0 = romb_continue32(R0[n], top0, lo, hi, n, need64);
     }
     if (run1) {
       res1 = R1[n];
-      const float ob = fabsf(R1[n] - top1) / R1[n];
-      if (ob >= lo && ob <= hi && n <= ROMB_ITERMAX) *need64 = true;
-      run1 = (ob > (float)ROMB_PREC) && (n <= ROMB_ITERMAX);
+      run1 = romb_continue32(R1[n], top1, lo, hi, n, need64);
     }
   }
   return res0 + res1;
@@ -439,10 +484,7 @@ RX_HD inline double kerr_rms_hd(double a) {
   return 3.0 + z2 - sgn * sqrt((3.0 - z1) * (3.0 + z1 + (2 * z2)));
 }
 
-RX_HD inline double lerp1(double ifac, double lo, double hi) {
-#pragma clang fp contract(off)
-  return ifac * hi + (1.0 - ifac) * lo;  // interp_lin_1d, src/relutility.c:24-26
-}
+RX_HD inline double lerp1(double ifac, double lo, double hi) { return lerp1_(ifac, lo, hi); }
 
 RX_HD inline double lerp2f(double i1, double i2, float r11, float r12, float r21, float r22) {
 #pragma clang fp contract(off)

@@ -10,8 +10,9 @@
 // done with their latency (submit -> completion seen).  Tables and the
 // convolution grid stay resident for the server's lifetime.
 //
-// Each request is computed with a fixed tile shape (radius chunks), so its
-// result does not depend on which other requests share its batch.
+// Each request's work items depend only on the request itself (fixed segment
+// length); per-bin sums over radius chunks are added atomically in FP64, so
+// a request's result equals its solo run up to FP64 rounding of that order.
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -30,8 +31,8 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-constexpr int SERVER_RCHUNKS = 4;
-constexpr int SERVER_GROUPS = 2;
+constexpr int SERVER_SEG_BINS = 64;
+constexpr int SERVER_WAVES_PER_CU = 8;
 
 struct Pending {
   int64_t ticket;
@@ -61,8 +62,11 @@ int64_t request_bytes(const Pending &p) {
   const bool conv = (p.model == MOD_RELCONV || p.model == MOD_RELCONVLP || p.model == MOD_RELXILL_CONV);
   const int64_t n_in = (int64_t)p.energy.size() - 1;
   const int64_t n_e = conv ? g.conv_n : n_in;
+  const int64_t n_rc = (g.nr_fine + RAD_CHUNK - 1) / RAD_CHUNK;
   int64_t b = sizeof(ReqParams) + 8 * (2 * (n_in + 1) + 2 * n_in);
-  b += 8 * (n_e + 1) + 8 * n_e * (SERVER_RCHUNKS + 1);
+  b += 8 * (n_e + 1) + 8 * n_e;
+  b += (int64_t)sizeof(WorkItem) * n_rc * ((n_e + SERVER_SEG_BINS - 1) / SERVER_SEG_BINS);
+  b += (int64_t)sizeof(FixEntry) * (n_e / 8 + 1);
   b += (int64_t)sizeof(RadRec) * g.nr_fine;
   b += 8 * 2 * (int64_t)g.rel.n_r * g.rel.n_g * 2;
   b += 8 * MAX_INCL + sizeof(LpRefl) + 64;
@@ -123,8 +127,8 @@ void rx_server::launch_ready() {
       std::lock_guard<std::recursive_mutex> lk(G().mu);
       st = ensure_device(err);
       if (st == RX_OK) {
-        s.batch.n_rch_forced = SERVER_RCHUNKS;
-        s.batch.groups_forced = SERVER_GROUPS;
+        s.batch.seg_forced = SERVER_SEG_BINS;
+        s.batch.waves_forced = SERVER_WAVES_PER_CU;
         st = batch_build(s.batch, rq.data(), (int)rq.size(), err);
       }
       if (st == RX_OK) st = batch_run(s.batch, precision, err);
@@ -174,8 +178,7 @@ bool rx_server::reap() {
       const ReqParams &p = s.batch.h_req[i];
       Result &r = results[s.reqs[i].ticket];
       r.done = true;
-      int code = st[i];
-      if (code >= 10 && code < 50) code = RX_ERR_TABLE;
+      int code = public_status(st[i]);
       r.status = code;
       r.flux.assign(out.begin() + p.flux_off, out.begin() + p.flux_off + p.n_in);
       if (code != RX_OK) std::fill(r.flux.begin(), r.flux.end(), 0.0);

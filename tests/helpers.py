@@ -107,3 +107,95 @@ def flat_table(gmin, gmax, trff_sum=1.0, cosne=0.5):
     out["cosne1"] = np.full_like(rel["cosne1"], cosne)
     out["cosne2"] = np.full_like(rel["cosne2"], cosne)
     return out
+
+
+def write_rxtb(path, rel):
+    """Writes a relline table dict (tables_view layout) in the library's RXTB
+    format: int32 magic 'RXTB', kind 1, n_a, n_mu0, n_r, n_g, then the float32
+    arrays a, mu0, r, gmin, gmax, trff1, trff2, cosne1, cosne2."""
+    hdr = np.array([0x42545852, 1, rel["n_a"], rel["n_mu0"], rel["n_r"], rel["n_g"]], dtype=np.int32)
+    with open(path, "wb") as f:
+        f.write(hdr.tobytes())
+        for k in ("a", "mu0", "r", "gmin", "gmax", "trff1", "trff2", "cosne1", "cosne2"):
+            f.write(np.ascontiguousarray(rel[k], dtype=np.float32).tobytes())
+
+
+def sawtooth_table(c_rel=0.5):
+    """Flat window [0.999, 1.001] with trff_k[g] = (1 + c g/(n_g-1))/2 for every
+    node and radius: relb_func's swapped lerp (src/Relprofile.cpp:502-504) turns
+    this into a sawtooth in g* with known per-bin integrals."""
+    rel = flat_table(0.999, 1.001)
+    ng = rel["n_g"]
+    nn = rel["n_a"] * rel["n_mu0"] * rel["n_r"]
+    saw = 0.5 * (1.0 + c_rel / (ng - 1) * np.arange(ng))
+    rel["trff1"] = np.tile(saw, nn).astype(np.float32)
+    rel["trff2"] = rel["trff1"].copy()
+    return rel
+
+
+def sawtooth_bins(e, rel, c_rel=0.5, swapped=True):
+    """Closed-form photons per bin of relline on sawtooth_table() with the
+    physical normalisation: F_j = 1/4 int E^3 f(g*) / sqrt(g*(1-g*)) dg* / E_mid
+    (1/4 = sum of emis * pi r dr / 2 for a normalised emissivity), f the
+    branch sum of the lerped transfer function, for the bins in 0.02 < g* <
+    0.98 that contain no g* node.  Returns (indices, values)."""
+    from scipy.integrate import quad
+    ng = rel["n_g"]
+    H = 5e-3
+    step = (1 - 2 * H) / (ng - 1)
+    c = c_rel / (ng - 1)
+    gmin, gmax = float(rel["gmin"][0]), float(rel["gmax"][0])
+
+    def f(g):
+        q = (g - H) / step
+        i = min(max(np.floor(q), 0), ng - 2)
+        inte = q - i
+        return 1 + c * (i + 1 - inte) if swapped else 1 + c * (i + inte)
+
+    def integrand(g):
+        E = gmin + g * (gmax - gmin)
+        return E ** 3 * f(g) / np.sqrt(g * (1 - g))
+
+    nodes = H + step * np.arange(ng)
+    idx, val = [], []
+    for j in range(len(e) - 1):
+        glo = (e[j] - gmin) / (gmax - gmin)
+        ghi = (e[j + 1] - gmin) / (gmax - gmin)
+        if glo < 0.02 or ghi > 0.98 or np.any((nodes > glo) & (nodes < ghi)):
+            continue
+        idx.append(j)
+        val.append(0.25 * quad(integrand, glo, ghi, epsabs=0, epsrel=1e-12)[0] / (0.5 * (e[j] + e[j + 1])))
+    return np.array(idx), np.array(val)
+
+
+def affine_table(alpha=1.0, beta=0.3, gamma=0.5):
+    """Flat window with trff_1 = trff_2 = (alpha + beta a + gamma mu0)/2 at
+    every node: the bilinear (a, mu0) interpolation (src/Relprofile.cpp:39-80,
+    src/relutility.c:53-58) reproduces an affine field exactly, so the flux is
+    pi/4 * (alpha + beta a_f + gamma mu0_f) with a_f, mu0_f the grid points
+    moved by the float interpolation factors."""
+    rel = flat_table(0.999, 1.001)
+    na, nm, nr, ng = rel["n_a"], rel["n_mu0"], rel["n_r"], rel["n_g"]
+    val = 0.5 * (alpha + beta * rel["a"][:, None] + gamma * rel["mu0"][None, :])
+    t = np.repeat(val.reshape(na * nm), nr * ng).astype(np.float32)
+    rel["trff1"] = t
+    rel["trff2"] = t.copy()
+    return rel
+
+
+def affine_expected(rel, a, incl_deg, alpha=1.0, beta=0.3, gamma=0.5, swap_a=False):
+    """alpha + beta a_f + gamma mu0_f: an affine field is reproduced exactly by
+    bilinear interpolation, at the point a_f = a_i + f_a (a_i+1 - a_i) (same
+    for mu0) with the factors f_a, f_mu0 computed in float as the reference
+    does (src/Relprofile.cpp:171-177)."""
+    ag, mg = rel["a"], rel["mu0"]
+    af, mf = np.float32(a), np.float32(np.cos(np.radians(incl_deg)))
+    ia = np.searchsorted(ag, af, side="right") - 1
+    im = np.searchsorted(mg, mf, side="right") - 1
+    fa = float(np.float32((af - ag[ia]) / (ag[ia + 1] - ag[ia])))
+    fm = float(np.float32((mf - mg[im]) / (mg[im + 1] - mg[im])))
+    if swap_a:  # a plausible mistake: the a weights exchanged
+        fa = 1.0 - fa
+    a_f = float(ag[ia]) + fa * (float(ag[ia + 1]) - float(ag[ia]))
+    m_f = float(mg[im]) + fm * (float(mg[im + 1]) - float(mg[im]))
+    return alpha + beta * a_f + gamma * m_f, (fa, fm)

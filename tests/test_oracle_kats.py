@@ -288,3 +288,42 @@ def test_work_counters_cfg0():
     assert 1e5 < s["relb_func_calls"] < 1e7
     # every bin integral is a midpoint, a Romberg pair (two branches) or edge-only
     assert s["bin_integrals"] >= s["midpoint_calls"] + s["romberg_calls"] // 2
+
+
+def test_sawtooth_transfer_function_per_bin_closed_form():
+    """relb_func's swapped lerp weights (src/Relprofile.cpp:502-504), the g*
+    grid (:104-116), the bracket search, E^3/sqrt(g*(1-g*)) and Romberg, bin by
+    bin against quadrature of the closed form; the usual (unswapped) lerp
+    would miss by percents."""
+    from tests.helpers import sawtooth_bins, sawtooth_table
+    rel = sawtooth_table()
+    T = oracle.Tables(rel)
+    e = np.linspace(0.998, 1.002, 4001)
+    f, st = oracle.eval_model(oracle.MOD_RELLINE, T, [1.0, 3.0, 3.0, 15.0, 0.5, 40.0, -1.0, 400.0, 0.0, 0.0],
+                              e, phys_norm=True)
+    assert st == 0
+    idx, ref = sawtooth_bins(e, rel)
+    assert idx.size > 1500
+    assert np.max(np.abs(f[idx] / ref - 1)) < 1e-6
+    _, ref_u = sawtooth_bins(e, rel, swapped=False)
+    assert np.max(np.abs(f[idx] / ref_u - 1)) > 1e-2  # the check can tell the two lerps apart
+
+
+@pytest.mark.parametrize("a,incl", [(0.5, 40.0), (-0.63, 71.0), (0.93, 12.0)])
+def test_affine_table_bilinear_closed_form(a, incl):
+    """Bilinear (a, mu0) node interpolation with float factors
+    (src/Relprofile.cpp:39-80, :171-177) on a field affine in a and mu0:
+    per-bin flux = closed form of the flat window times the affine value."""
+    from tests.helpers import affine_expected, affine_table, sawtooth_bins
+    rel = affine_table()
+    T = oracle.Tables(rel)
+    e = np.linspace(0.998, 1.002, 4001)
+    f, st = oracle.eval_model(oracle.MOD_RELLINE, T, [1.0, 3.0, 3.0, 15.0, a, incl, -1.0, 400.0, 0.0, 0.0],
+                              e, phys_norm=True)
+    assert st == 0
+    k, (fa, fm) = affine_expected(rel, a, incl)
+    idx, unit = sawtooth_bins(e, rel, c_rel=0.0)
+    assert np.max(np.abs(f[idx] / (k * unit) - 1)) < 1e-6
+    # exchanged a weights would be visible at this tolerance
+    k_swap, _ = affine_expected(rel, a, incl, swap_a=True)
+    assert abs(k_swap / k - 1) > 1e-4
