@@ -239,8 +239,8 @@ int rx_tables_view(rx_reltable_view *rel, rx_lptable_view *lp) {
 int rx_set_conv_grid(int n, double emin, double emax) {
   Global &g = G();
   std::lock_guard<std::recursive_mutex> lk(g.mu);
-  if (n < 64 || n > MAX_FFT || (n & (n - 1)) != 0 || !(emin > 0) || !(emax > emin)) {
-    set_last(RX_ERR_ARG, "convolution grid must have a power-of-two size in [64, 8192]");
+  if (n < 256 || n > MAX_FFT || (n & (n - 1)) != 0 || !(emin > 0) || !(emax > emin)) {
+    set_last(RX_ERR_ARG, "convolution grid must have a power-of-two size in [256, 8192]");
     return RX_ERR_ARG;
   }
   if (!(emin < 1.0 && emax > 1.0)) {

@@ -115,6 +115,7 @@ int parse_request(int model, const double *par, bool phys_norm, ReqParams &p, st
   if (p.rbr < 0) p.rbr = -1.0 * p.rbr * kerr_rms(p.a);
   if (p.rout <= p.rin) status = RX_ERR_PARAM;
   const double rms = kerr_rms(p.a);
+  p.rms = rms;  // the device uses this value, not its own pow/sqrt rounding
   if (p.rin < rms) p.rin = rms;
   if (p.a > 0.9982) { err = "spin a > 0.9982"; return RX_ERR_PARAM; }
   if (p.a < -1) { err = "spin a < -1"; return RX_ERR_PARAM; }

@@ -30,7 +30,8 @@ namespace rx {
 namespace {
 
 constexpr int SETUP_THREADS = 256;
-constexpr int CONV_THREADS = 256;
+constexpr int CONV_THREADS = 256;      // convolution workgroup for grids up to 4096 bins
+constexpr int CONV_THREADS_MAX = 512;  // n/16 threads for larger grids (8192)
 
 __device__ inline double lp_ifac(float value, const float *arr, int n, int &ind) {
   // get_ipol_factor: float search and float quotient (src/relutility.c:419-423)
@@ -159,7 +160,7 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b) {
 
   if (tid == 0) {
     int st = 0;
-    const double rms = kerr_rms_hd(rq.a);
+    const double rms = rq.rms;
     if (!(rq.rout > rq.rin) || !(rq.rin >= rms)) st = 10;
     const double mu0 = cos(rq.incl);
     const int ia = bsearch_asc(T.a, T.n_a, (float)rq.a);
@@ -689,141 +690,314 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_finalize(BatchDev b) {
 // ---------------------------------------------------------------------------
 namespace {
 
-// value of output bin [elo, ehi] of _rebin_spectrum (src/relutility.c:549-601)
-// from the input grid e0 (n0+1 edges) / spectrum f0 (n0 bins).  The
-// sequential walk of the reference reduces to upper-bound counts.
-__device__ inline double rebin_one(double elo_, double ehi_, const double *e0, const double *f0, int n0) {
-  if (!((e0[0] <= ehi_) && (e0[n0] >= elo_))) return 0.0;
-  int lo = 0, hi = n0 + 1;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (e0[mid] <= elo_) lo = mid + 1; else hi = mid;
-  }
-  int imin = lo > 0 ? lo - 1 : 0;
-  if (imin >= n0) return 0.0;  // zero-width overlap at the top edge
-  lo = 0;
-  hi = n0 + 1;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (e0[mid] <= ehi_) lo = mid + 1; else hi = mid;
-  }
-  int imax = lo < n0 ? lo : n0;
-  if (imax > 0) imax--;
-  double elo = elo_, ehi = ehi_;
-  if (elo < e0[imin]) elo = e0[imin];
-  if (ehi > e0[imax + 1]) ehi = e0[imax + 1];
-  if (imax == imin) {
-    return (ehi - elo) / (e0[imin + 1] - e0[imin]) * f0[imin];
-  }
-  const double dmin = (e0[imin + 1] - elo) / (e0[imin + 1] - e0[imin]);
-  const double dmax = (ehi - e0[imax]) / (e0[imax + 1] - e0[imax]);
-  double flu = 0.0;
-  flu += f0[imin] * dmin + f0[imax] * dmax;
-  for (int jj = imin + 1; jj <= imax - 1; jj++) flu += f0[jj];
-  return flu;
-}
+// LDS index of complex element i: one pad slot every 16 elements, so that the
+// stride-16 writes of a radix-16 pass fall into distinct banks.
+__device__ inline int P(int i) { return i + (i >> 4); }
 
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ inline double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+// a * (S i), S = -1 (forward) or +1 (inverse)
+template <int S>
+__device__ inline double2 mul_si(double2 a) {
+  return S > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
 
-__device__ inline int bitrev(int x, int logn) { return (int)(__brev((unsigned)x) >> (32 - logn)); }
+// exp(S 2 pi i M / 16), M known at compile time
+template <int S, int M>
+__device__ inline double2 w16() {
+  constexpr double c[5] = {1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173, 0.0};
+  constexpr int q = (M & 15) >> 2, r = M & 3;
+  constexpr double cs = q == 0 ? c[r] : q == 1 ? -c[4 - r] : q == 2 ? -c[r] : c[4 - r];
+  constexpr double sn = q == 0 ? c[4 - r] : q == 1 ? c[r] : q == 2 ? -c[4 - r] : -c[r];
+  return make_double2(cs, S * sn);
+}
 
-// in-place radix-2 DIT FFT of length n (power of two) held in LDS, input
-// already in bit-reversed order.  sign -1: forward, +1: inverse (unnormalised).
-// tw[t] = exp(-2 pi i t / N) for t < N/2, N = stride_n * n.
-__device__ inline void lds_fft(double2 *x, int n, int logn, const double2 *tw, int tw_stride, int sign) {
-  for (int s = 1; s <= logn; s++) {
-    const int half = 1 << (s - 1);
-    const int tstride = tw_stride * (n >> s);
-    for (int bf = threadIdx.x; bf < (n >> 1); bf += blockDim.x) {
-      const int grp = bf >> (s - 1);
-      const int k = bf & (half - 1);
-      const int i = grp * (half << 1) + k;
-      const int jx = i + half;
-      double2 w = tw[k * tstride];
-      if (sign > 0) w.y = -w.y;
-      const double2 u = x[i];
-      const double2 t = cmul(w, x[jx]);
-      x[i] = make_double2(u.x + t.x, u.y + t.y);
-      x[jx] = make_double2(u.x - t.x, u.y - t.y);
-    }
-    __syncthreads();
+template <int S>
+__device__ __forceinline__ void dft4(double2 &a0, double2 &a1, double2 &a2, double2 &a3) {
+  const double2 s02 = cadd(a0, a2), d02 = csub(a0, a2);
+  const double2 s13 = cadd(a1, a3), d13 = mul_si<S>(csub(a1, a3));
+  a0 = cadd(s02, s13);
+  a2 = csub(s02, s13);
+  a1 = cadd(d02, d13);
+  a3 = csub(d02, d13);
+}
+
+template <int S, int N2, int K1>
+__device__ inline void tw16(double2 &x) {
+  if (N2 * K1 != 0) x = cmul(x, w16<S, N2 * K1>());
+}
+
+// In-register DFT of length R (2, 4, 8 or 16), natural order in and out.
+template <int R, int S>
+__device__ __forceinline__ void dft(double2 *v) {
+  if (R == 2) {
+    const double2 t = v[0];
+    v[0] = cadd(t, v[1]);
+    v[1] = csub(t, v[1]);
+  } else if (R == 4) {
+    dft4<S>(v[0], v[1], v[2], v[3]);
+  } else if (R == 8) {
+    // n = 2 n1 + n2, k = k1 + 4 k2: two DFT4 (even, odd), twiddle, butterflies
+    dft4<S>(v[0], v[2], v[4], v[6]);
+    dft4<S>(v[1], v[3], v[5], v[7]);
+    tw16<S, 1, 2>(v[3]);
+    tw16<S, 1, 4>(v[5]);
+    tw16<S, 1, 6>(v[7]);
+    const double2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    const double2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    v[0] = cadd(e0, o0);
+    v[4] = csub(e0, o0);
+    v[1] = cadd(e1, o1);
+    v[5] = csub(e1, o1);
+    v[2] = cadd(e2, o2);
+    v[6] = csub(e2, o2);
+    v[3] = cadd(e3, o3);
+    v[7] = csub(e3, o3);
+  } else {
+    // n = 4 n1 + n2, k = k1 + 4 k2: DFT4 over n1 for each n2 (result k1 in
+    // slot n2 + 4 k1), twiddle w16^(n2 k1), DFT4 over n2 for each k1 (result
+    // k2 in slot n2 = k2 of row k1), then the transpose to natural order.
+    dft4<S>(v[0], v[4], v[8], v[12]);
+    dft4<S>(v[1], v[5], v[9], v[13]);
+    dft4<S>(v[2], v[6], v[10], v[14]);
+    dft4<S>(v[3], v[7], v[11], v[15]);
+    tw16<S, 1, 1>(v[5]);
+    tw16<S, 1, 2>(v[9]);
+    tw16<S, 1, 3>(v[13]);
+    tw16<S, 2, 1>(v[6]);
+    tw16<S, 2, 2>(v[10]);
+    tw16<S, 2, 3>(v[14]);
+    tw16<S, 3, 1>(v[7]);
+    tw16<S, 3, 2>(v[11]);
+    tw16<S, 3, 3>(v[15]);
+    dft4<S>(v[0], v[1], v[2], v[3]);
+    dft4<S>(v[4], v[5], v[6], v[7]);
+    dft4<S>(v[8], v[9], v[10], v[11]);
+    dft4<S>(v[12], v[13], v[14], v[15]);
+    // slot 4 k1 + k2 holds X[k1 + 4 k2]
+    double2 t[16];
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++)
+#pragma unroll
+      for (int k2 = 0; k2 < 4; k2++) t[k1 + 4 * k2] = v[4 * k1 + k2];
+#pragma unroll
+    for (int k = 0; k < 16; k++) v[k] = t[k];
   }
+}
+
+// exp(S 2 pi i idx / ntab) from the half table tw[k] = exp(-2 pi i k / ntab)
+template <int S>
+__device__ inline double2 twiddle(const double2 *tw, int idx, int ntab) {
+  double2 w = (idx < (ntab >> 1)) ? tw[idx] : make_double2(-tw[idx - (ntab >> 1)].x, -tw[idx - (ntab >> 1)].y);
+  if (S > 0) w.y = -w.y;
+  return w;
+}
+
+// One Stockham radix-R pass over N complex values in LDS (padded layout):
+// thread j loads x[j + r N/R] r = 0..R-1, twiddles by w^(r (j mod Ns)),
+// w = exp(S 2 pi i / (Ns R)), transforms, and stores at
+// (j / Ns) Ns R + j mod Ns + r Ns.  PER j values per thread; reads finish
+// before any write.
+template <int R, int S, int PER>
+__device__ __forceinline__ void fft_pass(double2 *buf, int N, int Ns, const double2 *tw, int ntab) {
+  const int J = N / R;
+  double2 v[PER][R];
+  const int tstep = ntab / (Ns * R);
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    const int j = threadIdx.x + u * blockDim.x;
+    if (j < J) {
+      const int k = j % Ns;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        double2 x = buf[P(j + r * J)];
+        if (r > 0 && k > 0) x = cmul(x, twiddle<S>(tw, r * k * tstep, ntab));
+        v[u][r] = x;
+      }
+      dft<R, S>(v[u]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    const int j = threadIdx.x + u * blockDim.x;
+    if (j < J) {
+      const int k = j % Ns;
+      const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; r++) buf[P(base + r * Ns)] = v[u][r];
+    }
+  }
+  __syncthreads();
+}
+
+// The block has at least N/16 threads, so a pass holds at most 16 values per
+// thread: PER = 1 (R = 16), <= 2 (R = 8), <= 4 (R = 4), <= 8 (R = 2).
+template <int R, int S>
+__device__ __forceinline__ void fft_pass_any(double2 *buf, int N, int Ns, const double2 *tw, int ntab) {
+  const int per = (N / R + blockDim.x - 1) / blockDim.x;
+  if (per <= 1) fft_pass<R, S, 1>(buf, N, Ns, tw, ntab);
+  else if (R <= 8 && per <= 2) fft_pass<R, S, (R <= 8 ? 2 : 1)>(buf, N, Ns, tw, ntab);
+  else if (R <= 4 && per <= 4) fft_pass<R, S, (R <= 4 ? 4 : 1)>(buf, N, Ns, tw, ntab);
+  else fft_pass<R, S, (R <= 2 ? 8 : 1)>(buf, N, Ns, tw, ntab);
+}
+
+// Unnormalised complex FFT of N (power of two, 2..8192) values in LDS,
+// natural order in and out; radix-16 passes, then one of 8, 4 or 2.
+template <int S>
+__device__ __forceinline__ void lds_fft(double2 *buf, int N, const double2 *tw, int ntab) {
+  int Ns = 1;
+  while (Ns < N) {
+    const int left = N / Ns;
+    if (left % 16 == 0) {
+      fft_pass_any<16, S>(buf, N, Ns, tw, ntab);
+      Ns *= 16;
+    } else if (left % 8 == 0) {
+      fft_pass_any<8, S>(buf, N, Ns, tw, ntab);
+      Ns *= 8;
+    } else if (left % 4 == 0) {
+      fft_pass_any<4, S>(buf, N, Ns, tw, ntab);
+      Ns *= 4;
+    } else {
+      fft_pass_any<2, S>(buf, N, Ns, tw, ntab);
+      Ns *= 2;
+    }
+  }
+}
+
+// Block sum in a fixed order: FP64 wave trees, then the waves in order.
+__device__ inline double block_sum4(double v, double *s_w) {
+  const double w = wave_sum(v);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) s_w[wave] = w;
+  __syncthreads();
+  double t = 0.0;
+  for (int k = 0; k < (int)(blockDim.x >> 6); k++) t += s_w[k];
+  __syncthreads();
+  return t;
 }
 
 __device__ inline bool in_band(const double *e, int i) {
   return e[i] >= 0.01 && e[i + 1] < 1000.0;  // calcFFTNormFactor band (src/Relbase.cpp:205)
 }
 
+// _rebin_spectrum (src/relutility.c:549-601) for output bins [b0, b1) of the
+// grid eo onto which input (grid e0 with n0+1 edges, values f0(i)) is
+// rebinned.  The reference's per-bin sequential search becomes a walking count
+// cnt(x) = #{k : e0[k] <= x}: imin = max(cnt(lo)-1, 0), imax = max(min(cnt(hi),
+// n0)-1, 0); an overlap of zero width at the top edge gives 0.
+template <class F, class G>
+__device__ inline void rebin_range(const double *eo, int b0, int b1, const double *e0, int n0, F f0, G put) {
+  if (b0 >= b1) return;
+  // cnt at eo[b0] by binary search
+  int lo = 0, hi = n0 + 1;
+  const double x0 = eo[b0];
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (e0[mid] <= x0) lo = mid + 1; else hi = mid;
+  }
+  int c = lo;
+  const double e0lo = e0[0], e0hi = e0[n0];
+  double elo_ = x0;
+  for (int ib = b0; ib < b1; ib++) {
+    const int clo = c;
+    const double ehi_ = eo[ib + 1];
+    while (c <= n0 && e0[c] <= ehi_) c++;
+    double v = 0.0;
+    const int imin = clo > 0 ? clo - 1 : 0;
+    if ((e0lo <= ehi_) && (e0hi >= elo_) && imin < n0) {
+      int imax = c < n0 ? c : n0;
+      if (imax > 0) imax--;
+      double elo = elo_, ehi = ehi_;
+      if (elo < e0[imin]) elo = e0[imin];
+      if (ehi > e0[imax + 1]) ehi = e0[imax + 1];
+      if (imax == imin) {
+        v = (ehi - elo) / (e0[imin + 1] - e0[imin]) * f0(imin);
+      } else {
+        const double dmin = (e0[imin + 1] - elo) / (e0[imin + 1] - e0[imin]);
+        const double dmax = (ehi - e0[imax]) / (e0[imax + 1] - e0[imax]);
+        v += f0(imin) * dmin + f0(imax) * dmax;
+        for (int jj = imin + 1; jj <= imax - 1; jj++) v += f0(jj);
+      }
+    }
+    put(ib, v);
+    elo_ = ehi_;
+  }
+}
+
 }  // namespace
 
 // Convolution of one request (relconv_kernel minus relbase, src/Relbase.cpp:257-289).
-__global__ __launch_bounds__(CONV_THREADS) void k_conv(BatchDev b, const int *__restrict__ reqs) {
-  extern __shared__ double2 s_buf[];  // n complex
-  __shared__ double s_red[CONV_THREADS];
+// One complex FFT carries both real inputs (input spectrum in the real part,
+// rotated profile in the imaginary part); their spectra are separated, multiplied
+// and inverted with a half-length complex FFT.
+__global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int *__restrict__ reqs) {
+  extern __shared__ double2 s_buf[];  // P(n) complex
+  __shared__ double s_w[CONV_THREADS_MAX / 64];
+  const int NT = blockDim.x;
   const int req = reqs[blockIdx.x];
   const int tid = threadIdx.x;
   const ReqParams rq = b.req[req];
   const DevConvGrid cg = b.cg;
   const int n = cg.n;
   const int m = n >> 1;
-  int logn = 0;
-  while ((1 << logn) < n) logn++;
   double *out = b.out + rq.flux_off;
   if (b.status[req] != 0) {
-    for (int j = tid; j < rq.n_in; j += CONV_THREADS) out[j] = 0.0;
+    for (int j = tid; j < rq.n_in; j += NT) out[j] = 0.0;
     return;
   }
   const double *Ec = cg.ener;
   const double *cf = cg.cfac;
   const double *prof = b.prof + rq.prof_off;
   const double *ein = b.ener_in + rq.ener_off;
+  const double *fin = b.flux_in + rq.flux_off;
   const double2 *tw = reinterpret_cast<const double2 *>(cg.tw);
   const bool angular = (rq.model == MOD_RELXILL_CONV);
-
-  // 1. x = rebinned input * E/dE (real part), y = rotated profile * E/dE (imag part)
-  double sx = 0.0, sr = 0.0;
   const double *dist = b.dist + (size_t)req * MAX_INCL;
   const DevReflSet *rs = angular ? &b.refl_sets[rq.refl_set] : nullptr;
-  for (int i = tid; i < n; i += CONV_THREADS) {
-    double xr = 0.0;
-    if (!angular) {
-      xr = rebin_one(Ec[i], Ec[i + 1], ein, b.flux_in + rq.flux_off, rq.n_in);
+
+  // 1. x = rebinned input * E/dE (real part), y = rotated profile * E/dE (imag part)
+  const int per = n / NT;
+  const int i0 = tid * per;
+  double sx = 0.0, sr = 0.0;
+  if (!angular) {
+    rebin_range(Ec, i0, i0 + per, ein, rq.n_in, [&](int k) { return fin[k]; }, [&](int i, double xr) {
       if (in_band(Ec, i)) sx += xr;
-    }
+      s_buf[P(i)].x = xr * cf[i];
+    });
+  } else {
+    for (int i = i0; i < i0 + per; i++) s_buf[P(i)].x = 0.0;
+  }
+  for (int i = i0; i < i0 + per; i++) {
     const double pv = prof[i];
     if (in_band(Ec, i)) sr += pv;
     const int irot = (i - cg.i1kev + n) % n;
-    s_buf[bitrev(i, logn)].x = xr * cf[i];
-    s_buf[bitrev(irot, logn)].y = pv * cf[i];
+    s_buf[P(irot)].y = pv * cf[i];
   }
-  const double sum_xill_local = block_sum(sx, s_red);
-  const double sum_rel = block_sum(sr, s_red);
-  double sum_xill = sum_xill_local;
+  double sum_xill = block_sum4(sx, s_w);  // (barrier inside: the buffer is complete)
+  const double sum_rel = block_sum4(sr, s_w);
   if (angular) {
     sum_xill = 0.0;
     for (int k = 0; k < rs->n_incl; k++) sum_xill += dist[k] * rs->bandsum[k];
   }
-  __syncthreads();
   // 2. forward complex FFT of x + i y
-  lds_fft(s_buf, n, logn, tw, 1, -1);
+  lds_fft<-1>(s_buf, n, tw, n);
   // 3. X = (W_k + conj W_{n-k})/2, Y = (W_k - conj W_{n-k})/(2i), P = X Y (k = 0..m);
   //    then pack for the half-length inverse real FFT:
-  //    Z_k = (P_k + conj P_{m-k}) + i (P_k - conj P_{m-k}) e^{+2 pi i k/n}
-  constexpr int MAXC = (MAX_FFT / 2) / CONV_THREADS;
-  double2 zk[MAXC];
-#pragma unroll
-  for (int c = 0; c < MAXC; c++) {
-    const int k = tid + c * CONV_THREADS;
-    if (k >= m) continue;
-    double2 P[2];
+  //    Z_k = (P_k + conj P_{m-k}) + i (P_k - conj P_{m-k}) e^{+2 pi i k/n}.
+  //    The pair {k, m-k} reads slots k, n-k, m-k, m+k and writes slots k and
+  //    m-k, which no other pair touches: no staging across a barrier.
+  for (int k = tid; k <= (m >> 1); k += NT) {
+    double2 Pk[2];
     const int ks[2] = {k, m - k};
 #pragma unroll
     for (int u = 0; u < 2; u++) {
       const int kk = ks[u];
-      const double2 wk = s_buf[kk];
-      const double2 wn = s_buf[(n - kk) & (n - 1)];
+      const double2 wk = s_buf[P(kk)];
+      const double2 wn = s_buf[P((n - kk) & (n - 1))];
       double2 Y = make_double2(0.5 * (wk.y + wn.y), -0.5 * (wk.x - wn.x));
       double2 X;
       if (angular) {
@@ -836,75 +1010,76 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv(BatchDev b, const int *__
       } else {
         X = make_double2(0.5 * (wk.x + wn.x), 0.5 * (wk.y - wn.y));
       }
-      P[u] = cmul(X, Y);
+      Pk[u] = cmul(X, Y);
     }
     if (k == 0) {  // c2r ignores the imaginary parts of P_0 and P_m
-      P[0].y = 0.0;
-      P[1].y = 0.0;
+      Pk[0].y = 0.0;
+      Pk[1].y = 0.0;
     }
-    const double2 A = make_double2(P[0].x + P[1].x, P[0].y - P[1].y);
-    const double2 Bd = make_double2(P[0].x - P[1].x, P[0].y + P[1].y);
-    double2 w = tw[k];
-    w.y = -w.y;  // e^{+2 pi i k/n}
-    const double2 Bw = cmul(Bd, w);
-    zk[c] = make_double2(A.x - Bw.y, A.y + Bw.x);
-  }
-  __syncthreads();
-  const int logm = logn - 1;
 #pragma unroll
-  for (int c = 0; c < MAXC; c++) {
-    const int k = tid + c * CONV_THREADS;
-    if (k < m) s_buf[bitrev(k, logm)] = zk[c];
+    for (int u = 0; u < 2; u++) {
+      const int kk = ks[u];
+      if (kk >= m || (u == 1 && kk == k)) continue;  // Z_m is not needed; k = m/2 once
+      const double2 p0 = Pk[u], p1 = Pk[1 - u];
+      const double2 A = make_double2(p0.x + p1.x, p0.y - p1.y);
+      const double2 Bd = make_double2(p0.x - p1.x, p0.y + p1.y);
+      double2 w = tw[kk];
+      w.y = -w.y;  // e^{+2 pi i kk/n}
+      const double2 Bw = cmul(Bd, w);
+      s_buf[P(kk)] = make_double2(A.x - Bw.y, A.y + Bw.x);
+    }
   }
   __syncthreads();
   // 4. half-length inverse FFT: z_j = out_{2j} + i out_{2j+1}
-  lds_fft(s_buf, m, logm, tw, 2, +1);
-  double *o = reinterpret_cast<double *>(s_buf);
+  lds_fft<+1>(s_buf, m, tw, n);
+  auto o_at = [&](int i) -> double & {
+    double2 &z = s_buf[P(i >> 1)];
+    return (i & 1) ? z.y : z.x;
+  };
   double sc = 0.0;
-  for (int i = tid; i < n; i += CONV_THREADS) {
-    const double v = o[i] / cf[i];
-    o[i] = v;
+  for (int i = i0; i < i0 + per; i++) {
+    const double v = o_at(i) / cf[i];
+    o_at(i) = v;
     if (in_band(Ec, i)) sc += v;
   }
-  const double sum_conv = block_sum(sc, s_red);
+  const double sum_conv = block_sum4(sc, s_w);
   const double fnorm = sum_rel * sum_xill / sum_conv;
-  for (int i = tid; i < n; i += CONV_THREADS) o[i] *= fnorm;
+  for (int i = i0; i < i0 + per; i++) o_at(i) *= fnorm;
   __syncthreads();
   // 5. rebin onto the caller grid; relconv zeroes outside 0.01-1000 keV
-  for (int jb = tid; jb < rq.n_in; jb += CONV_THREADS) {
-    double v = rebin_one(ein[jb], ein[jb + 1], Ec, o, n);
+  const int pin = (rq.n_in + NT - 1) / NT;
+  const int jb0 = min(rq.n_in, tid * pin), jb1 = min(rq.n_in, jb0 + pin);
+  rebin_range(ein, jb0, jb1, Ec, n, [&](int i) { return o_at(i); }, [&](int jb, double v) {
     if (!angular && (ein[jb + 1] < 0.01 || ein[jb] > 1000.0)) v = 0.0;
     out[jb] = v;
-  }
+  });
 }
 
 // Resident reflection set: per angle, rebin onto the convolution grid, band
 // sum, and the real FFT of (spectrum * E/dE), stored for k = 0..n/2.
-__global__ __launch_bounds__(CONV_THREADS) void k_refl_prepare(DevConvGrid cg, const double *refl_ener,
-                                                              const double *refl, int n_refl,
-                                                              double *spec, double *bandsum, double *fft) {
+__global__ __launch_bounds__(CONV_THREADS_MAX) void k_refl_prepare(DevConvGrid cg, const double *refl_ener,
+                                                                  const double *refl, int n_refl,
+                                                                  double *spec, double *bandsum, double *fft) {
   extern __shared__ double2 s_buf[];
-  __shared__ double s_red[CONV_THREADS];
+  __shared__ double s_w[CONV_THREADS_MAX / 64];
   const int a = blockIdx.x;
   const int tid = threadIdx.x;
   const int n = cg.n, m = n >> 1;
-  int logn = 0;
-  while ((1 << logn) < n) logn++;
   const double *f0 = refl + (size_t)a * n_refl;
+  const int per = n / blockDim.x;
+  const int i0 = tid * per;
   double sx = 0.0;
-  for (int i = tid; i < n; i += CONV_THREADS) {
-    const double xr = rebin_one(cg.ener[i], cg.ener[i + 1], refl_ener, f0, n_refl);
+  rebin_range(cg.ener, i0, i0 + per, refl_ener, n_refl, [&](int k) { return f0[k]; }, [&](int i, double xr) {
     spec[(size_t)a * n + i] = xr;
     if (in_band(cg.ener, i)) sx += xr;
-    s_buf[bitrev(i, logn)] = make_double2(xr * cg.cfac[i], 0.0);
-  }
-  const double s = block_sum(sx, s_red);
+    s_buf[P(i)] = make_double2(xr * cg.cfac[i], 0.0);
+  });
+  const double s = block_sum4(sx, s_w);
   if (tid == 0) bandsum[a] = s;
-  __syncthreads();
-  lds_fft(s_buf, n, logn, reinterpret_cast<const double2 *>(cg.tw), 1, -1);
-  for (int k = tid; k <= m; k += CONV_THREADS) {
-    fft[((size_t)a * (m + 1) + k) * 2 + 0] = s_buf[k].x;
-    fft[((size_t)a * (m + 1) + k) * 2 + 1] = s_buf[k].y;
+  lds_fft<-1>(s_buf, n, reinterpret_cast<const double2 *>(cg.tw), n);
+  for (int k = tid; k <= m; k += blockDim.x) {
+    fft[((size_t)a * (m + 1) + k) * 2 + 0] = s_buf[P(k)].x;
+    fft[((size_t)a * (m + 1) + k) * 2 + 1] = s_buf[P(k)].y;
   }
 }
 
@@ -957,11 +1132,14 @@ int launch_finalize(const BatchDev &b, void *stream) {
   return (int)hipGetLastError();
 }
 
+static size_t conv_lds(int n) { return (size_t)(n + n / 16) * sizeof(double2); }
+static int conv_threads(int n) { return std::max(CONV_THREADS, n / 16); }
+
 int launch_conv(const BatchDev &b, const int *conv_reqs, int n_conv_reqs, void *stream) {
   if (n_conv_reqs <= 0) return 0;
-  const size_t lds = (size_t)b.cg.n * sizeof(double2);
+  const size_t lds = conv_lds(b.cg.n);
   (void)hipFuncSetAttribute((const void *)k_conv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(k_conv, dim3(n_conv_reqs), dim3(CONV_THREADS), lds, (hipStream_t)stream, b, conv_reqs);
+  hipLaunchKernelGGL(k_conv, dim3(n_conv_reqs), dim3(conv_threads(b.cg.n)), lds, (hipStream_t)stream, b, conv_reqs);
   return (int)hipGetLastError();
 }
 
@@ -973,9 +1151,9 @@ int launch_copy(const void *src, void *dst, size_t bytes, void *stream) {
 
 int launch_refl_prepare(const DevConvGrid &cg, const double *refl_ener, const double *refl, int n_refl,
                         int n_incl, double *spec, double *bandsum, double *fft, void *stream) {
-  const size_t lds = (size_t)cg.n * sizeof(double2);
+  const size_t lds = conv_lds(cg.n);
   (void)hipFuncSetAttribute((const void *)k_refl_prepare, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(k_refl_prepare, dim3(n_incl), dim3(CONV_THREADS), lds, (hipStream_t)stream, cg, refl_ener,
+  hipLaunchKernelGGL(k_refl_prepare, dim3(n_incl), dim3(conv_threads(cg.n)), lds, (hipStream_t)stream, cg, refl_ener,
                      refl, n_refl, spec, bandsum, fft);
   return (int)hipGetLastError();
 }

@@ -50,6 +50,7 @@ struct ReqParams {
   int n_incl;     // >0: compute the angle distribution with n_incl bins
   double a, incl, emis1, emis2, rbr, rin, rout, lineE, z, height, gamma, beta;
   double norm;    // renormalisation target (1, or 0.5 cos i for relxill BKN)
+  double rms;     // ISCO of a (host kerr_rms: Rin was reset to exactly this value)
   int64_t ener_off;   // caller grid edges in ener_in
   int64_t flux_off;   // caller input / output spectrum offset (n_in)
   int64_t pgrid_off;  // profile grid edges in pgrid (-1: shared conv grid)

@@ -477,13 +477,6 @@ RX_HD inline double integ_bin(double rlo0, double rhi0, const RadiusView &v, boo
 
 // ---- small helpers shared by setup code ---------------------------------
 
-RX_HD inline double kerr_rms_hd(double a) {
-  double sgn = (a < 0) ? -1.0 : 1.0;
-  double z1 = 1.0 + pow(1.0 - a * a, 1.0 / 3.0) * (pow(1.0 + a, 1.0 / 3.0) + pow(1.0 - a, 1.0 / 3.0));
-  double z2 = sqrt((3.0 * a * a) + (z1 * z1));
-  return 3.0 + z2 - sgn * sqrt((3.0 - z1) * (3.0 + z1 + (2 * z2)));
-}
-
 RX_HD inline double lerp1(double ifac, double lo, double hi) { return lerp1_(ifac, lo, hi); }
 
 RX_HD inline double lerp2f(double i1, double i2, float r11, float r12, float r21, float r22) {

@@ -199,3 +199,35 @@ def affine_expected(rel, a, incl_deg, alpha=1.0, beta=0.3, gamma=0.5, swap_a=Fal
     a_f = float(ag[ia]) + fa * (float(ag[ia + 1]) - float(ag[ia]))
     m_f = float(mg[im]) + fm * (float(mg[im + 1]) - float(mg[im]))
     return alpha + beta * a_f + gamma * m_f, (fa, fm)
+
+
+def radial_table(u=0.01, v=1.0):
+    """Flat window with trff_1 = trff_2 = (u + v r/1000)/2 at every table
+    radius r: the node radii are the spin-lerped table radii and the fine-grid
+    values lerp between two nodes (src/Relprofile.cpp:190-193, :249-300), so a
+    correct mapping reproduces (u + v re/1000)/2 at every fine radius re."""
+    rel = flat_table(0.999, 1.001)
+    t = 0.5 * (u + v * rel["r"].astype(np.float64) / 1000.0)
+    rel["trff1"] = np.repeat(t, rel["n_g"]).astype(np.float32)
+    rel["trff2"] = rel["trff1"].copy()
+    return rel
+
+
+def radial_expected(rin, rout, u=0.01, v=1.0, q=3.0, nr=1000, node_r=None):
+    """Flux ratio radial_table / flat table = emissivity-and-area weighted mean
+    of (u + v re/1000) over the fine grid (re descending, weights re dr with
+    half intervals at the ends, emissivity re^-q).  With node_r (descending
+    node radii) the fine values are taken with the node bracket's lerp
+    direction reversed: a plausible mistake the check must catch."""
+    import oracle
+    re = oracle.fine_radial_grid(rin, rout, nr)
+    dr = np.empty_like(re)
+    dr[0] = 0.5 * (re[0] - re[1])
+    dr[-1] = 0.5 * (re[-2] - re[-1])
+    dr[1:-1] = 0.5 * (re[:-2] - re[2:])
+    w = re * dr * re ** -q
+    x = re
+    if node_r is not None:
+        k = np.searchsorted(-node_r, -re, side="right") - 1  # node_r[k] >= re > node_r[k+1]
+        x = node_r[k] + node_r[k + 1] - re
+    return u + v / 1000.0 * np.sum(w * x) / np.sum(w)

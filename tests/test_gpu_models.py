@@ -193,3 +193,19 @@ def test_affine_table_closed_form(tabs, tmp_path, a, incl):
             f, st = rx.eval_model(rx.MODEL_RELLINE, p, e, precision=prec)
             assert np.max(np.abs(f[idx] / (k * unit) - 1)) < tol
     _with_table(tmp_path, rel, run)
+
+
+def test_radial_mapping_closed_form(tabs, tmp_path):
+    """Node radii and fine-grid lerp on the GPU: flux ratio of the radial
+    table to the flat table equals the weighted mean of (u + v re/1000)."""
+    from tests.helpers import radial_expected, radial_table
+    e = np.linspace(0.998, 1.002, 4001)
+    p = [1.0, 3.0, 3.0, 15.0, 0.5, 40.0, -2.0, 400.0, 0.0, 0.0]
+    expect = radial_expected(2 * oracle.lib.orc_kerr_rms(0.5), 400.0)
+    for prec, tol in ((rx.FP64, 1e-7), (rx.FP32, 1e-4)):
+        fr = _with_table(tmp_path, radial_table(), lambda: rx.eval_model(rx.MODEL_RELLINE, p, e, precision=prec)[0])
+        ff = _with_table(tmp_path, flat_table(0.999, 1.001),
+                         lambda: rx.eval_model(rx.MODEL_RELLINE, p, e, precision=prec)[0])
+        got = fr.sum() / ff.sum()
+        print(f"radial mapping prec {prec}: {got:.10f} expected {expect:.10f}", flush=True)
+        assert abs(got / expect - 1) < tol

@@ -51,7 +51,8 @@ def test_lmodrelline_symbol_matches(tabs):
     e = log_grid(3000, 0.1, 10.0)
     y1 = rx.lmodrelline(e, RELLINE_CFG0)
     y2, _ = rx.eval_model(rx.MODEL_RELLINE, RELLINE_CFG0, e)
-    assert np.array_equal(y1, y2)
+    # equal up to the FP64 order of the atomic per-chunk sums
+    assert rel_err(y1, y2) < 1e-13
 
 
 @pytest.mark.parametrize("limb", [1, 2])

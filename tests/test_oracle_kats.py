@@ -327,3 +327,27 @@ def test_affine_table_bilinear_closed_form(a, incl):
     # exchanged a weights would be visible at this tolerance
     k_swap, _ = affine_expected(rel, a, incl, swap_a=True)
     assert abs(k_swap / k - 1) > 1e-4
+
+
+@pytest.mark.parametrize("a,incl", [(0.5, 40.0), (-0.3, 60.0), (0.95, 20.0)])
+def test_radial_mapping_closed_form(a, incl):
+    """Node radii lerped in spin and fine-grid values lerped between node
+    brackets (src/Relprofile.cpp:190-193, :249-300): a transfer function
+    affine in the table radius gives, relative to the flat table, the weighted
+    mean of (u + v re/1000) over the fine grid; a reversed lerp would not."""
+    from tests.helpers import flat_table, radial_expected, radial_table
+    e = np.linspace(0.998, 1.002, 4001)
+    p = [1.0, 3.0, 3.0, 15.0, a, incl, -2.0, 400.0, 0.0, 0.0]  # Rin = 2 ISCO: clear of the ISCO snap
+    fr, st = oracle.eval_model(oracle.MOD_RELLINE, oracle.Tables(radial_table()), p, e, phys_norm=True)
+    ff, st2 = oracle.eval_model(oracle.MOD_RELLINE, oracle.Tables(flat_table(0.999, 1.001)), p, e,
+                                phys_norm=True)
+    assert st == 0 and st2 == 0
+    rin = 2 * oracle.lib.orc_kerr_rms(a)
+    got = fr.sum() / ff.sum()
+    assert abs(got / radial_expected(rin, 400.0) - 1) < 1e-7
+    rel = radial_table()
+    r = rel["r"].reshape(rel["n_a"], rel["n_mu0"], rel["n_r"])[:, 0, :].astype(np.float64)
+    ia = np.searchsorted(rel["a"], np.float32(a), side="right") - 1
+    fa = float(np.float32((np.float32(a) - rel["a"][ia]) / (rel["a"][ia + 1] - rel["a"][ia])))
+    node_r = fa * r[ia + 1] + (1 - fa) * r[ia]
+    assert abs(got / radial_expected(rin, 400.0, node_r=node_r) - 1) > 1e-5
