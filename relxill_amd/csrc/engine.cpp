@@ -193,11 +193,36 @@ void ensure_tables_host() {
       g.conv_ener[ii] = exp(e);
     }
     g.conv_cfac.resize(n);
+    g.conv_cfinv.resize(n);
+    g.conv_inv_emid.resize(n);
     for (int ii = 0; ii < n; ii++) {
       // calculate_energyflux_conversion, src/Relbase.cpp:93-103
-      g.conv_cfac[ii] = 0.5 * (g.conv_ener[ii] + g.conv_ener[ii + 1]) / (g.conv_ener[ii + 1] - g.conv_ener[ii]);
+      const double emid = 0.5 * (g.conv_ener[ii] + g.conv_ener[ii + 1]);
+      const double de = g.conv_ener[ii + 1] - g.conv_ener[ii];
+      g.conv_cfac[ii] = emid / de;
+      g.conv_cfinv[ii] = de / emid;
+      g.conv_inv_emid[ii] = 1.0 / emid;
     }
     g.conv_i1kev = bsearch_asc(g.conv_ener.data(), n + 1, 1.0);
+    // per-pass twiddles of the device FFTs (forward n points, inverse n/2):
+    // pass (Ns, R) stores exp(-2 pi i r k / (Ns R)) at [r][k], k < Ns
+    for (int which = 0; which < 2; which++) {
+      std::vector<double> &t = which == 0 ? g.conv_twp_fwd : g.conv_twp_inv;
+      const int N = which == 0 ? n : n / 2;
+      t.clear();
+      for (int Ns = 1; Ns < N;) {
+        const int left = N / Ns;
+        const int R = left % 16 == 0 ? 16 : left % 8 == 0 ? 8 : left % 4 == 0 ? 4 : 2;
+        for (int r = 0; r < R; r++) {
+          for (int k = 0; k < Ns; k++) {
+            const long double ang = -2.0L * 3.141592653589793238462643383279502884L * r * k / ((long double)Ns * R);
+            t.push_back((double)cosl(ang));
+            t.push_back((double)sinl(ang));
+          }
+        }
+        Ns *= R;
+      }
+    }
     g.conv_tw.resize(n);  // n/2 complex
     for (int t = 0; t < n / 2; t++) {
       long double ang = -2.0L * 3.141592653589793238462643383279502884L * t / n;
@@ -265,6 +290,8 @@ int ensure_device(std::string &err) {
   }
   if (g.conv_dirty) {
     e |= upload_vec(g.c_ener, g.conv_ener) | upload_vec(g.c_cfac, g.conv_cfac) | upload_vec(g.c_tw, g.conv_tw);
+    e |= upload_vec(g.c_cfinv, g.conv_cfinv) | upload_vec(g.c_inv_emid, g.conv_inv_emid);
+    e |= upload_vec(g.c_twp_fwd, g.conv_twp_fwd) | upload_vec(g.c_twp_inv, g.conv_twp_inv);
     g.conv_dirty = false;
   }
   if (g.refl_dirty) {
@@ -306,6 +333,10 @@ void fill_dev_views(BatchDev &bd) {
   bd.cg.i1kev = g.conv_i1kev;
   bd.cg.ener = g.c_ener.as<double>();
   bd.cg.cfac = g.c_cfac.as<double>();
+  bd.cg.cfinv = g.c_cfinv.as<double>();
+  bd.cg.inv_emid = g.c_inv_emid.as<double>();
+  bd.cg.twp_fwd = g.c_twp_fwd.as<double>();
+  bd.cg.twp_inv = g.c_twp_inv.as<double>();
   bd.cg.tw = g.c_tw.as<double>();
   bd.refl_sets = g.d_refl_sets.as<DevReflSet>();
 }
@@ -458,6 +489,7 @@ int batch_upload(Batch &b) {
   e |= b.d_conv.ensure(sizeof(int) * std::max<size_t>(1, b.h_conv.size()));
   e |= b.d_items.ensure(sizeof(WorkItem) * (size_t)b.item_cap);
   e |= b.d_ctr.ensure(sizeof(int) * N_COUNTERS);
+  e |= b.d_reach.ensure(sizeof(BinRange) * nreq);
   // FP64 re-check queue: a few percent of the bins (overflow redoes the batch in FP64)
   int64_t bins = 0;
   for (int i = 0; i < b.n_req; i++) bins += b.h_req[i].n_e;
@@ -496,6 +528,7 @@ static BatchDev dev_view(Batch &b) {
   bd.items = b.d_items.as<WorkItem>();
   bd.item_cap = b.item_cap;
   bd.ctr = b.d_ctr.as<int>();
+  bd.reach = b.d_reach.as<BinRange>();
   bd.fix = b.d_fix.as<FixEntry>();
   bd.fix_cap = b.fix_cap;
   bd.mode = b.last_mode;

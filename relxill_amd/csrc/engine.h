@@ -38,7 +38,7 @@ struct Batch {
   std::vector<int> h_conv;
   int64_t tot_ener = 0, tot_flux = 0, tot_pgrid = 0, tot_prof = 0;
   DevBuf d_req, d_ener, d_flux_in, d_out, d_pgrid, d_node, d_rad, d_prof, d_dist, d_refl, d_status, d_conv;
-  DevBuf d_items, d_ctr, d_fix;
+  DevBuf d_items, d_ctr, d_fix, d_reach;
   int item_cap = 0;
   bool any_cosne = false, any_limb = false;
   int fix_cap = 0;
@@ -79,10 +79,10 @@ struct Global {
   // convolution grid
   int conv_n = 4096;
   double conv_emin = 0.00035, conv_emax = 2000.0;
-  std::vector<double> conv_ener, conv_cfac, conv_tw;
+  std::vector<double> conv_ener, conv_cfac, conv_cfinv, conv_inv_emid, conv_tw, conv_twp_fwd, conv_twp_inv;
   int conv_i1kev = 0;
   bool conv_dirty = true;
-  DevBuf c_ener, c_cfac, c_tw;
+  DevBuf c_ener, c_cfac, c_cfinv, c_inv_emid, c_tw, c_twp_fwd, c_twp_inv;
   int nr_fine = 1000;
   // reflection sets
   std::vector<DevReflSet> refl_sets;

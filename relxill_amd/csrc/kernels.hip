@@ -425,9 +425,16 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b) {
   }
   __syncthreads();
   if (s_st == 0) {
-    // the profile accumulates with atomic adds: clear it
+    // the profile accumulates with atomic adds: clear the bins any window
+    // reaches (k_finalize writes zeros elsewhere)
+    int jlo = 0x7fffffff, jhi = -1;
+    for (int c = 0; c < n_rc; c++) {
+      jlo = min(jlo, s_ulo[c]);
+      jhi = max(jhi, s_uhi[c]);
+    }
+    if (tid == 0) b.reach[req] = BinRange{jlo, jhi};
     double *acc = profile_acc(b, rq);
-    for (int j = tid; j < rq.n_e; j += SETUP_THREADS) acc[j] = 0.0;
+    for (int j = jlo + tid; j <= jhi; j += SETUP_THREADS) acc[j] = 0.0;
     // work items: each radius chunk's window cut into segments of seg_bins
     const int S = b.seg_bins;
     if (tid == 0) {
@@ -664,17 +671,21 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_finalize(BatchDev b) {
     for (int j = tid; j < n_e; j += SETUP_THREADS) dst[j] = 0.0;
     return;
   }
-  const double *E = (rq.pgrid_off >= 0) ? b.pgrid + rq.pgrid_off : b.cg.ener;
-  double loc = 0.0;
-  for (int j = tid; j < n_e; j += SETUP_THREADS) {
-    const double v = dst[j] / (0.5 * (E[j] + E[j + 1]));
-    dst[j] = v;
-    loc += v;
-  }
-  const double sum = block_sum(loc, s_red);
-  if (rq.do_renorm) {
-    const double f = rq.norm / sum;
-    for (int j = tid; j < n_e; j += SETUP_THREADS) dst[j] *= f;
+  if (rq.prof_off < 0) {  // line models; convolution profiles are finished by k_conv
+    const double *E = b.pgrid + rq.pgrid_off;
+    const BinRange rc = b.reach[req];
+    double loc = 0.0;
+    for (int j = tid; j < n_e; j += SETUP_THREADS) {
+      double v = 0.0;
+      if (j >= rc.lo && j <= rc.hi) v = dst[j] / (0.5 * (E[j] + E[j + 1]));
+      dst[j] = v;
+      loc += v;
+    }
+    const double sum = block_sum(loc, s_red);
+    if (rq.do_renorm) {
+      const double f = rq.norm / sum;
+      for (int j = tid; j < n_e; j += SETUP_THREADS) dst[j] *= f;
+    }
   }
   if (rq.n_incl > 0) {
     double *d = b.dist + (size_t)req * MAX_INCL;
@@ -788,33 +799,29 @@ __device__ __forceinline__ void dft(double2 *v) {
   }
 }
 
-// exp(S 2 pi i idx / ntab) from the half table tw[k] = exp(-2 pi i k / ntab)
-template <int S>
-__device__ inline double2 twiddle(const double2 *tw, int idx, int ntab) {
-  double2 w = (idx < (ntab >> 1)) ? tw[idx] : make_double2(-tw[idx - (ntab >> 1)].x, -tw[idx - (ntab >> 1)].y);
-  if (S > 0) w.y = -w.y;
-  return w;
-}
-
 // One Stockham radix-R pass over N complex values in LDS (padded layout):
 // thread j loads x[j + r N/R] r = 0..R-1, twiddles by w^(r (j mod Ns)),
-// w = exp(S 2 pi i / (Ns R)), transforms, and stores at
-// (j / Ns) Ns R + j mod Ns + r Ns.  PER j values per thread; reads finish
-// before any write.
+// w = exp(S 2 pi i / (Ns R)) (table twp[r Ns + k], forward sign, conjugated
+// for S = +1), transforms, and stores at (j / Ns) Ns R + j mod Ns + r Ns.
+// PER j values per thread; reads finish before any write.
 template <int R, int S, int PER>
-__device__ __forceinline__ void fft_pass(double2 *buf, int N, int Ns, const double2 *tw, int ntab) {
+__device__ __forceinline__ void fft_pass(double2 *buf, int N, int Ns, const double2 *twp) {
   const int J = N / R;
   double2 v[PER][R];
-  const int tstep = ntab / (Ns * R);
+  const int lgNs = __ffs(Ns) - 1;  // Ns is a power of two
 #pragma unroll
   for (int u = 0; u < PER; u++) {
     const int j = threadIdx.x + u * blockDim.x;
     if (j < J) {
-      const int k = j % Ns;
+      const int k = j & (Ns - 1);
 #pragma unroll
       for (int r = 0; r < R; r++) {
         double2 x = buf[P(j + r * J)];
-        if (r > 0 && k > 0) x = cmul(x, twiddle<S>(tw, r * k * tstep, ntab));
+        if (r > 0) {
+          double2 w = twp[r * Ns + k];
+          if (S > 0) w.y = -w.y;
+          x = cmul(x, w);
+        }
         v[u][r] = x;
       }
       dft<R, S>(v[u]);
@@ -825,8 +832,8 @@ __device__ __forceinline__ void fft_pass(double2 *buf, int N, int Ns, const doub
   for (int u = 0; u < PER; u++) {
     const int j = threadIdx.x + u * blockDim.x;
     if (j < J) {
-      const int k = j % Ns;
-      const int base = (j / Ns) * Ns * R + k;
+      const int k = j & (Ns - 1);
+      const int base = ((j >> lgNs) << lgNs) * R + k;
 #pragma unroll
       for (int r = 0; r < R; r++) buf[P(base + r * Ns)] = v[u][r];
     }
@@ -837,32 +844,37 @@ __device__ __forceinline__ void fft_pass(double2 *buf, int N, int Ns, const doub
 // The block has at least N/16 threads, so a pass holds at most 16 values per
 // thread: PER = 1 (R = 16), <= 2 (R = 8), <= 4 (R = 4), <= 8 (R = 2).
 template <int R, int S>
-__device__ __forceinline__ void fft_pass_any(double2 *buf, int N, int Ns, const double2 *tw, int ntab) {
+__device__ __forceinline__ void fft_pass_any(double2 *buf, int N, int Ns, const double2 *twp) {
   const int per = (N / R + blockDim.x - 1) / blockDim.x;
-  if (per <= 1) fft_pass<R, S, 1>(buf, N, Ns, tw, ntab);
-  else if (R <= 8 && per <= 2) fft_pass<R, S, (R <= 8 ? 2 : 1)>(buf, N, Ns, tw, ntab);
-  else if (R <= 4 && per <= 4) fft_pass<R, S, (R <= 4 ? 4 : 1)>(buf, N, Ns, tw, ntab);
-  else fft_pass<R, S, (R <= 2 ? 8 : 1)>(buf, N, Ns, tw, ntab);
+  if (per <= 1) fft_pass<R, S, 1>(buf, N, Ns, twp);
+  else if (R <= 8 && per <= 2) fft_pass<R, S, (R <= 8 ? 2 : 1)>(buf, N, Ns, twp);
+  else if (R <= 4 && per <= 4) fft_pass<R, S, (R <= 4 ? 4 : 1)>(buf, N, Ns, twp);
+  else fft_pass<R, S, (R <= 2 ? 8 : 1)>(buf, N, Ns, twp);
 }
 
 // Unnormalised complex FFT of N (power of two, 2..8192) values in LDS,
-// natural order in and out; radix-16 passes, then one of 8, 4 or 2.
+// natural order in and out; radix-16 passes, then one of 8, 4 or 2.  twp
+// holds the passes' tables one after the other (R Ns entries each).
 template <int S>
-__device__ __forceinline__ void lds_fft(double2 *buf, int N, const double2 *tw, int ntab) {
+__device__ __forceinline__ void lds_fft(double2 *buf, int N, const double2 *twp) {
   int Ns = 1;
   while (Ns < N) {
     const int left = N / Ns;
     if (left % 16 == 0) {
-      fft_pass_any<16, S>(buf, N, Ns, tw, ntab);
+      fft_pass_any<16, S>(buf, N, Ns, twp);
+      twp += 16 * Ns;
       Ns *= 16;
     } else if (left % 8 == 0) {
-      fft_pass_any<8, S>(buf, N, Ns, tw, ntab);
+      fft_pass_any<8, S>(buf, N, Ns, twp);
+      twp += 8 * Ns;
       Ns *= 8;
     } else if (left % 4 == 0) {
-      fft_pass_any<4, S>(buf, N, Ns, tw, ntab);
+      fft_pass_any<4, S>(buf, N, Ns, twp);
+      twp += 4 * Ns;
       Ns *= 4;
     } else {
-      fft_pass_any<2, S>(buf, N, Ns, tw, ntab);
+      fft_pass_any<2, S>(buf, N, Ns, twp);
+      twp += 2 * Ns;
       Ns *= 2;
     }
   }
@@ -880,66 +892,127 @@ __device__ inline double block_sum4(double v, double *s_w) {
   return t;
 }
 
+// Two block sums at once (one pair of barriers).
+__device__ inline void block_sum2(double a, double c, double *out, double (*s_w)[CONV_THREADS_MAX / 64]) {
+  const double wa = wave_sum(a), wc = wave_sum(c);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_w[0][wave] = wa;
+    s_w[1][wave] = wc;
+  }
+  __syncthreads();
+  double ta = 0.0, tc = 0.0;
+  for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
+    ta += s_w[0][k];
+    tc += s_w[1][k];
+  }
+  __syncthreads();
+  out[0] = ta;
+  out[1] = tc;
+}
+
 __device__ inline bool in_band(const double *e, int i) {
   return e[i] >= 0.01 && e[i + 1] < 1000.0;  // calcFFTNormFactor band (src/Relbase.cpp:205)
 }
 
-// _rebin_spectrum (src/relutility.c:549-601) for output bins [b0, b1) of the
-// grid eo onto which input (grid e0 with n0+1 edges, values f0(i)) is
-// rebinned.  The reference's per-bin sequential search becomes a walking count
-// cnt(x) = #{k : e0[k] <= x}: imin = max(cnt(lo)-1, 0), imax = max(min(cnt(hi),
-// n0)-1, 0); an overlap of zero width at the top edge gives 0.
-template <class F, class G>
-__device__ inline void rebin_range(const double *eo, int b0, int b1, const double *e0, int n0, F f0, G put) {
-  if (b0 >= b1) return;
-  // cnt at eo[b0] by binary search
+// 1/x to within an ulp or two: hardware estimate and two Newton steps
+// (cheaper than the IEEE division sequence; the rebin weights need ~1e-15).
+__device__ __forceinline__ double rcp64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// _rebin_spectrum (src/relutility.c:549-601), value of one output bin
+// [elo_, ehi_] from the counts clo = cnt(elo_), chi = cnt(ehi_) of input
+// edges at or below each end, cnt(x) = #{k : e0(k) <= x}: the reference's
+// sequential search gives imin = max(clo-1, 0), imax = max(min(chi, n0)-1, 0);
+// an overlap of zero width at the top edge gives 0.
+template <class E, class F>
+__device__ __forceinline__ double rebin_bin(double elo_, double ehi_, int clo, int chi, E e0, int n0, F f0) {
+  const int imin = clo > 0 ? clo - 1 : 0;
+  if (!((e0(0) <= ehi_) && (e0(n0) >= elo_)) || imin >= n0) return 0.0;
+  int imax = chi < n0 ? chi : n0;
+  if (imax > 0) imax--;
+  double elo = elo_, ehi = ehi_;
+  const double a0 = e0(imin), a1 = e0(imin + 1);
+  if (elo < a0) elo = a0;
+  const double b0 = e0(imax), b1 = e0(imax + 1);
+  if (ehi > b1) ehi = b1;
+  const double ra = rcp64(a1 - a0);
+  if (imax == imin) return (ehi - elo) * ra * f0(imin);
+  const double dmin = (a1 - elo) * ra;
+  const double dmax = (ehi - b0) * rcp64(b1 - b0);
+  double v = 0.0;
+  v += f0(imin) * dmin + f0(imax) * dmax;
+  for (int jj = imin + 1; jj <= imax - 1; jj++) v += f0(jj);
+  return v;
+}
+
+// Index guess for an ascending grid that is close to log-spaced (every grid
+// here is): e0[k] ~ e0[0] exp(k dlog).
+struct GridGuess {
+  float lg0, scale;  // log2(e0[0]), n0 / log2(e0[n0] / e0[0])
+  double first, last;
+};
+
+__device__ __forceinline__ GridGuess grid_guess(const double *e0, int n0) {
+  GridGuess g;
+  g.first = e0[0];
+  g.last = e0[n0];
+  g.lg0 = __log2f((float)g.first);
+  g.scale = (float)n0 / (__log2f((float)g.last) - g.lg0);
+  return g;
+}
+
+// cnt(x) = #{k in [0, n0] : e0(k) <= x}: the log-spaced guess, exact
+// correction by at most two steps, else bisection (any ascending grid).
+template <class E>
+__device__ __forceinline__ int count_le(E e0, int n0, double x, const GridGuess &g) {
+  if (!(x >= g.first)) return 0;
+  if (x >= g.last) return n0 + 1;
+  int k = (int)((__log2f((float)x) - g.lg0) * g.scale);
+  k = min(max(k, 0), n0 - 1);
+#pragma unroll
+  for (int it = 0; it < 3; it++) {
+    if (e0(k) > x) {
+      if (k == 0) return 0;  // cannot happen: x >= e0[0]
+      k--;
+    } else if (e0(k + 1) <= x) {
+      k++;
+    } else {
+      return k + 1;
+    }
+  }
   int lo = 0, hi = n0 + 1;
-  const double x0 = eo[b0];
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
-    if (e0[mid] <= x0) lo = mid + 1; else hi = mid;
+    if (e0(mid) <= x) lo = mid + 1; else hi = mid;
   }
-  int c = lo;
-  const double e0lo = e0[0], e0hi = e0[n0];
-  double elo_ = x0;
-  for (int ib = b0; ib < b1; ib++) {
-    const int clo = c;
-    const double ehi_ = eo[ib + 1];
-    while (c <= n0 && e0[c] <= ehi_) c++;
-    double v = 0.0;
-    const int imin = clo > 0 ? clo - 1 : 0;
-    if ((e0lo <= ehi_) && (e0hi >= elo_) && imin < n0) {
-      int imax = c < n0 ? c : n0;
-      if (imax > 0) imax--;
-      double elo = elo_, ehi = ehi_;
-      if (elo < e0[imin]) elo = e0[imin];
-      if (ehi > e0[imax + 1]) ehi = e0[imax + 1];
-      if (imax == imin) {
-        v = (ehi - elo) / (e0[imin + 1] - e0[imin]) * f0(imin);
-      } else {
-        const double dmin = (e0[imin + 1] - elo) / (e0[imin + 1] - e0[imin]);
-        const double dmax = (ehi - e0[imax]) / (e0[imax + 1] - e0[imax]);
-        v += f0(imin) * dmin + f0(imax) * dmax;
-        for (int jj = imin + 1; jj <= imax - 1; jj++) v += f0(jj);
-      }
-    }
-    put(ib, v);
-    elo_ = ehi_;
-  }
+  return lo;
 }
+
+}  // namespace
+
+namespace {
 
 }  // namespace
 
 // Convolution of one request (relconv_kernel minus relbase, src/Relbase.cpp:257-289).
 // One complex FFT carries both real inputs (input spectrum in the real part,
 // rotated profile in the imaginary part); their spectra are separated, multiplied
-// and inverted with a half-length complex FFT.
+// and inverted with a half-length complex FFT.  Grid-walking phases give each
+// wavefront a contiguous run of bins, one bin per lane, so that neighbouring
+// lanes read neighbouring addresses.
 __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int *__restrict__ reqs) {
   extern __shared__ double2 s_buf[];  // P(n) complex
-  __shared__ double s_w[CONV_THREADS_MAX / 64];
+  __shared__ double s_w[2][CONV_THREADS_MAX / 64];
   const int NT = blockDim.x;
   const int req = reqs[blockIdx.x];
   const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6, nwave = NT >> 6;
   const ReqParams rq = b.req[req];
   const DevConvGrid cg = b.cg;
   const int n = cg.n;
@@ -958,33 +1031,60 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
   const bool angular = (rq.model == MOD_RELXILL_CONV);
   const double *dist = b.dist + (size_t)req * MAX_INCL;
   const DevReflSet *rs = angular ? &b.refl_sets[rq.refl_set] : nullptr;
+  const int n_in = rq.n_in;
+  auto ge_in = [&](int k) { return ein[k]; };
 
   // 1. x = rebinned input * E/dE (real part), y = rotated profile * E/dE (imag part)
-  const int per = n / NT;
-  const int i0 = tid * per;
-  double sx = 0.0, sr = 0.0;
+  double sx = 0.0;
+  const int nb = n / nwave;  // bins per wavefront (a multiple of 64)
   if (!angular) {
-    rebin_range(Ec, i0, i0 + per, ein, rq.n_in, [&](int k) { return fin[k]; }, [&](int i, double xr) {
-      if (in_band(Ec, i)) sx += xr;
+    const GridGuess gi = grid_guess(ein, n_in);
+    for (int q = 0; q < nb; q += 64) {
+      const int i = wave * nb + q + lane;
+      const double elo = Ec[i], ehi = Ec[i + 1];
+      const int clo = count_le(ge_in, n_in, elo, gi);
+      const int chi = count_le(ge_in, n_in, ehi, gi);
+      const double xr = rebin_bin(elo, ehi, clo, chi, ge_in, n_in, [&](int k) { return fin[k]; });
+      if (elo >= 0.01 && ehi < 1000.0) sx += xr;  // calcFFTNormFactor band (src/Relbase.cpp:205)
       s_buf[P(i)].x = xr * cf[i];
-    });
+    }
   } else {
-    for (int i = i0; i < i0 + per; i++) s_buf[P(i)].x = 0.0;
+    for (int i = tid; i < n; i += NT) s_buf[P(i)].x = 0.0;
   }
-  for (int i = i0; i < i0 + per; i++) {
-    const double pv = prof[i];
-    if (in_band(Ec, i)) sr += pv;
-    const int irot = (i - cg.i1kev + n) % n;
-    s_buf[P(irot)].y = pv * cf[i];
+  // the profile as photons per bin, renormalised (renorm_relline_profile,
+  // src/Relprofile.cpp:757-779); bins no window reaches are zero
+  const BinRange rc = b.reach[req];
+  const int per = n / NT;
+  double ph[16];
+  double sph = 0.0;
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int i = tid + q * NT;
+    ph[q] = (q < per && i >= rc.lo && i <= rc.hi) ? prof[i] * cg.inv_emid[i] : 0.0;
+    sph += ph[q];
   }
-  double sum_xill = block_sum4(sx, s_w);  // (barrier inside: the buffer is complete)
-  const double sum_rel = block_sum4(sr, s_w);
+  double s2[2];
+  block_sum2(sx, sph, s2, s_w);
+  double sum_xill = s2[0];
+  const double f = rq.do_renorm ? rq.norm / s2[1] : 1.0;
+  double sr = 0.0;
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int i = tid + q * NT;
+    if (q < per) {
+      const double pv = ph[q] * f;
+      if (in_band(Ec, i)) sr += pv;
+      const int irot = (i - cg.i1kev + n) & (n - 1);
+      s_buf[P(irot)].y = pv * cf[i];
+    }
+  }
+  const double sum_rel = block_sum4(sr, s_w[0]);  // (barrier inside: the buffer is complete)
   if (angular) {
     sum_xill = 0.0;
     for (int k = 0; k < rs->n_incl; k++) sum_xill += dist[k] * rs->bandsum[k];
   }
   // 2. forward complex FFT of x + i y
-  lds_fft<-1>(s_buf, n, tw, n);
+  lds_fft<-1>(s_buf, n, reinterpret_cast<const double2 *>(cg.twp_fwd));
   // 3. X = (W_k + conj W_{n-k})/2, Y = (W_k - conj W_{n-k})/(2i), P = X Y (k = 0..m);
   //    then pack for the half-length inverse real FFT:
   //    Z_k = (P_k + conj P_{m-k}) + i (P_k - conj P_{m-k}) e^{+2 pi i k/n}.
@@ -1031,28 +1131,41 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
   }
   __syncthreads();
   // 4. half-length inverse FFT: z_j = out_{2j} + i out_{2j+1}
-  lds_fft<+1>(s_buf, m, tw, n);
+  lds_fft<+1>(s_buf, m, reinterpret_cast<const double2 *>(cg.twp_inv));
   auto o_at = [&](int i) -> double & {
     double2 &z = s_buf[P(i >> 1)];
     return (i & 1) ? z.y : z.x;
   };
   double sc = 0.0;
-  for (int i = i0; i < i0 + per; i++) {
-    const double v = o_at(i) / cf[i];
+  for (int i = tid; i < n; i += NT) {
+    const double v = o_at(i) * cg.cfinv[i];
     o_at(i) = v;
     if (in_band(Ec, i)) sc += v;
   }
-  const double sum_conv = block_sum4(sc, s_w);
+  const double sum_conv = block_sum4(sc, s_w[0]);
   const double fnorm = sum_rel * sum_xill / sum_conv;
-  for (int i = i0; i < i0 + per; i++) o_at(i) *= fnorm;
+  for (int i = tid; i < n; i += NT) o_at(i) *= fnorm;
+  // the conv grid edges go to the upper part of the buffer, free after the
+  // half-length FFT (its values occupy slots below P(m))
+  double *s_e = reinterpret_cast<double *>(s_buf + P(m));
+  for (int i = tid; i <= n; i += NT) s_e[i] = Ec[i];
   __syncthreads();
   // 5. rebin onto the caller grid; relconv zeroes outside 0.01-1000 keV
-  const int pin = (rq.n_in + NT - 1) / NT;
-  const int jb0 = min(rq.n_in, tid * pin), jb1 = min(rq.n_in, jb0 + pin);
-  rebin_range(ein, jb0, jb1, Ec, n, [&](int i) { return o_at(i); }, [&](int jb, double v) {
-    if (!angular && (ein[jb + 1] < 0.01 || ein[jb] > 1000.0)) v = 0.0;
-    out[jb] = v;
-  });
+  const GridGuess gc = grid_guess(Ec, n);
+  auto ge_c = [&](int k) { return s_e[k]; };
+  const int nbo = (n_in + nwave - 1) / nwave;
+  const int jend = min(n_in, (wave + 1) * nbo);
+  for (int q = 0; q < nbo; q += 64) {
+    const int jb = wave * nbo + q + lane;
+    if (jb < jend) {
+      const double elo = ein[jb], ehi = ein[jb + 1];
+      const int clo = count_le(ge_c, n, elo, gc);
+      const int chi = count_le(ge_c, n, ehi, gc);
+      double v = rebin_bin(elo, ehi, clo, chi, ge_c, n, [&](int i) { return o_at(i); });
+      if (!angular && (ehi < 0.01 || elo > 1000.0)) v = 0.0;
+      out[jb] = v;
+    }
+  }
 }
 
 // Resident reflection set: per angle, rebin onto the convolution grid, band
@@ -1064,19 +1177,26 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_refl_prepare(DevConvGrid c
   __shared__ double s_w[CONV_THREADS_MAX / 64];
   const int a = blockIdx.x;
   const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
   const int n = cg.n, m = n >> 1;
   const double *f0 = refl + (size_t)a * n_refl;
-  const int per = n / blockDim.x;
-  const int i0 = tid * per;
+  const GridGuess gi = grid_guess(refl_ener, n_refl);
+  auto ge = [&](int k) { return refl_ener[k]; };
+  const int nb = n / nwave;
   double sx = 0.0;
-  rebin_range(cg.ener, i0, i0 + per, refl_ener, n_refl, [&](int k) { return f0[k]; }, [&](int i, double xr) {
+  for (int q = 0; q < nb; q += 64) {
+    const int i = wave * nb + q + lane;
+    const double elo = cg.ener[i], ehi = cg.ener[i + 1];
+    const int clo = count_le(ge, n_refl, elo, gi);
+    const int chi = count_le(ge, n_refl, ehi, gi);
+    const double xr = rebin_bin(elo, ehi, clo, chi, ge, n_refl, [&](int k) { return f0[k]; });
     spec[(size_t)a * n + i] = xr;
-    if (in_band(cg.ener, i)) sx += xr;
+    if (elo >= 0.01 && ehi < 1000.0) sx += xr;
     s_buf[P(i)] = make_double2(xr * cg.cfac[i], 0.0);
-  });
+  }
   const double s = block_sum4(sx, s_w);
   if (tid == 0) bandsum[a] = s;
-  lds_fft<-1>(s_buf, n, reinterpret_cast<const double2 *>(cg.tw), n);
+  lds_fft<-1>(s_buf, n, reinterpret_cast<const double2 *>(cg.twp_fwd));
   for (int k = tid; k <= m; k += blockDim.x) {
     fft[((size_t)a * (m + 1) + k) * 2 + 0] = s_buf[P(k)].x;
     fft[((size_t)a * (m + 1) + k) * 2 + 1] = s_buf[P(k)].y;

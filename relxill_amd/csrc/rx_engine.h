@@ -98,7 +98,11 @@ struct DevConvGrid {
   int i1kev;           // binary_search(ener, n+1, 1.0)
   const double *ener;  // n+1 edges
   const double *cfac;  // E/dE conversion, n
+  const double *cfinv; // dE/E, n
+  const double *inv_emid;  // 1/E_mid, n
   const double *tw;    // twiddles exp(-2 pi i k/n), k < n/2, interleaved re/im
+  const double *twp_fwd;  // per-pass twiddles of the n-point FFT ([r][k] per pass)
+  const double *twp_inv;  // per-pass twiddles of the n/2-point FFT
 };
 
 // Fine radii are integrated in chunks of one wavefront: lane l of chunk c
@@ -111,6 +115,10 @@ constexpr int MAX_NR_FINE = 64 * RAD_CHUNK;
 // window (union over its radii), k_integrate pulls them from a shared counter.
 struct WorkItem {
   int req, rc, j0, j1;
+};
+
+struct BinRange {
+  int lo, hi;
 };
 
 // FP32 bins whose Romberg stop test fell in the guard band: recomputed in FP64
@@ -151,6 +159,7 @@ struct BatchDev {
   int *status;
   WorkItem *items;
   int item_cap;
+  BinRange *reach;    // per request: bins [lo, hi] that some window reaches (lo > hi: none)
   int *ctr;           // CounterSlot
   FixEntry *fix;
   int fix_cap;
