@@ -204,6 +204,14 @@ void ensure_tables_host() {
       g.conv_inv_emid[ii] = 1.0 / emid;
     }
     g.conv_i1kev = bsearch_asc(g.conv_ener.data(), n + 1, 1.0);
+    g.conv_band_lo = n;
+    g.conv_band_hi = 0;
+    for (int ii = 0; ii < n; ii++) {
+      if (g.conv_ener[ii] >= 0.01 && g.conv_ener[ii + 1] < 1000.0) {
+        g.conv_band_lo = std::min(g.conv_band_lo, ii);
+        g.conv_band_hi = std::max(g.conv_band_hi, ii + 1);
+      }
+    }
     // per-pass twiddles of the device FFTs (forward n points, inverse n/2):
     // pass (Ns, R) stores exp(-2 pi i r k / (Ns R)) at [r][k], k < Ns
     for (int which = 0; which < 2; which++) {
@@ -331,6 +339,8 @@ void fill_dev_views(BatchDev &bd) {
   bd.lp.del_inc = g.l_inc.as<float>();
   bd.cg.n = g.conv_n;
   bd.cg.i1kev = g.conv_i1kev;
+  bd.cg.band_lo = g.conv_band_lo;
+  bd.cg.band_hi = g.conv_band_hi;
   bd.cg.ener = g.c_ener.as<double>();
   bd.cg.cfac = g.c_cfac.as<double>();
   bd.cg.cfinv = g.c_cfinv.as<double>();

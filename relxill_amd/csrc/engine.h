@@ -81,6 +81,7 @@ struct Global {
   double conv_emin = 0.00035, conv_emax = 2000.0;
   std::vector<double> conv_ener, conv_cfac, conv_cfinv, conv_inv_emid, conv_tw, conv_twp_fwd, conv_twp_inv;
   int conv_i1kev = 0;
+  int conv_band_lo = 0, conv_band_hi = 0;
   bool conv_dirty = true;
   DevBuf c_ener, c_cfac, c_cfinv, c_inv_emid, c_tw, c_twp_fwd, c_twp_inv;
   int nr_fine = 1000;

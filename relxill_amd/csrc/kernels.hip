@@ -100,6 +100,49 @@ __device__ inline double *profile_acc(const BatchDev &b, const ReqParams &rq) {
   return (rq.prof_off >= 0) ? b.prof + rq.prof_off : b.out + rq.flux_off;
 }
 
+// Index guess for an ascending grid that is close to log-spaced (every grid
+// here is): e0[k] ~ e0[0] exp(k dlog).
+struct GridGuess {
+  float lg0, scale;  // log2(e0[0]), n0 / log2(e0[n0] / e0[0])
+  double first, last;
+};
+
+__device__ __forceinline__ GridGuess grid_guess(const double *e0, int n0) {
+  GridGuess g;
+  g.first = e0[0];
+  g.last = e0[n0];
+  g.lg0 = __log2f((float)g.first);
+  g.scale = (float)n0 / (__log2f((float)g.last) - g.lg0);
+  return g;
+}
+
+// cnt(x) = #{k in [0, n0] : e0(k) <= x}: the log-spaced guess, exact
+// correction by at most two steps, else bisection (any ascending grid).
+template <class E>
+__device__ __forceinline__ int count_le(E e0, int n0, double x, const GridGuess &g) {
+  if (!(x >= g.first)) return 0;
+  if (x >= g.last) return n0 + 1;
+  int k = (int)((__log2f((float)x) - g.lg0) * g.scale);
+  k = min(max(k, 0), n0 - 1);
+#pragma unroll
+  for (int it = 0; it < 3; it++) {
+    if (e0(k) > x) {
+      if (k == 0) return 0;  // cannot happen: x >= e0[0]
+      k--;
+    } else if (e0(k + 1) <= x) {
+      k++;
+    } else {
+      return k + 1;
+    }
+  }
+  int lo = 0, hi = n0 + 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (e0(mid) <= x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
 // block-wide deterministic sum (fixed tree order)
 __device__ inline double block_sum(double v, double *red) {
   const int tid = threadIdx.x;
@@ -269,6 +312,8 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b) {
 
   // fine radii: node -> fine map (:249-300), emissivity, energy window (:859-878)
   RadRec *rad = b.rad + (size_t)req * nr;
+  const GridGuess gE = grid_guess(E, rq.n_e);
+  auto ge = [&](int k) { return E[k]; };
   double area = 0.0;
   int err = 0;
   for (int ii = tid; ii < nr; ii += SETUP_THREADS) {
@@ -316,8 +361,9 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b) {
     if ((egmax > E[0]) && (egmin < E[rq.n_e])) {
       if (egmin < E[0]) egmin = E[0];
       if (egmax > E[rq.n_e]) egmax = E[rq.n_e];
-      ielo = bsearch_asc(E, rq.n_e + 1, egmin);
-      iehi = bsearch_asc(E, rq.n_e + 1, egmax);
+      // binary_search (src/relutility.c:155-171) on [E0, E_n]: count - 1, at most n-1
+      ielo = min(count_le(ge, rq.n_e, egmin, gE) - 1, rq.n_e - 1);
+      iehi = min(count_le(ge, rq.n_e, egmax, gE) - 1, rq.n_e - 1);
     }
     RadRec r;
     r.gmin = gmin;
@@ -911,9 +957,9 @@ __device__ inline void block_sum2(double a, double c, double *out, double (*s_w)
   out[1] = tc;
 }
 
-__device__ inline bool in_band(const double *e, int i) {
-  return e[i] >= 0.01 && e[i + 1] < 1000.0;  // calcFFTNormFactor band (src/Relbase.cpp:205)
-}
+// calcFFTNormFactor band (src/Relbase.cpp:205): bins with E_lo >= 0.01 and
+// E_hi < 1000, a contiguous index range of the convolution grid
+__device__ inline bool in_band(const DevConvGrid &cg, int i) { return i >= cg.band_lo && i < cg.band_hi; }
 
 // 1/x to within an ulp or two: hardware estimate and two Newton steps
 // (cheaper than the IEEE division sequence; the rebin weights need ~1e-15).
@@ -951,52 +997,6 @@ __device__ __forceinline__ double rebin_bin(double elo_, double ehi_, int clo, i
   return v;
 }
 
-// Index guess for an ascending grid that is close to log-spaced (every grid
-// here is): e0[k] ~ e0[0] exp(k dlog).
-struct GridGuess {
-  float lg0, scale;  // log2(e0[0]), n0 / log2(e0[n0] / e0[0])
-  double first, last;
-};
-
-__device__ __forceinline__ GridGuess grid_guess(const double *e0, int n0) {
-  GridGuess g;
-  g.first = e0[0];
-  g.last = e0[n0];
-  g.lg0 = __log2f((float)g.first);
-  g.scale = (float)n0 / (__log2f((float)g.last) - g.lg0);
-  return g;
-}
-
-// cnt(x) = #{k in [0, n0] : e0(k) <= x}: the log-spaced guess, exact
-// correction by at most two steps, else bisection (any ascending grid).
-template <class E>
-__device__ __forceinline__ int count_le(E e0, int n0, double x, const GridGuess &g) {
-  if (!(x >= g.first)) return 0;
-  if (x >= g.last) return n0 + 1;
-  int k = (int)((__log2f((float)x) - g.lg0) * g.scale);
-  k = min(max(k, 0), n0 - 1);
-#pragma unroll
-  for (int it = 0; it < 3; it++) {
-    if (e0(k) > x) {
-      if (k == 0) return 0;  // cannot happen: x >= e0[0]
-      k--;
-    } else if (e0(k + 1) <= x) {
-      k++;
-    } else {
-      return k + 1;
-    }
-  }
-  int lo = 0, hi = n0 + 1;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (e0(mid) <= x) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
-
-}  // namespace
-
-namespace {
 
 }  // namespace
 
@@ -1039,6 +1039,7 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
   const int nb = n / nwave;  // bins per wavefront (a multiple of 64)
   if (!angular) {
     const GridGuess gi = grid_guess(ein, n_in);
+#pragma unroll 4
     for (int q = 0; q < nb; q += 64) {
       const int i = wave * nb + q + lane;
       const double elo = Ec[i], ehi = Ec[i + 1];
@@ -1073,7 +1074,7 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
     const int i = tid + q * NT;
     if (q < per) {
       const double pv = ph[q] * f;
-      if (in_band(Ec, i)) sr += pv;
+      if (in_band(cg, i)) sr += pv;
       const int irot = (i - cg.i1kev + n) & (n - 1);
       s_buf[P(irot)].y = pv * cf[i];
     }
@@ -1137,10 +1138,11 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
     return (i & 1) ? z.y : z.x;
   };
   double sc = 0.0;
+#pragma unroll 8
   for (int i = tid; i < n; i += NT) {
     const double v = o_at(i) * cg.cfinv[i];
     o_at(i) = v;
-    if (in_band(Ec, i)) sc += v;
+    if (in_band(cg, i)) sc += v;
   }
   const double sum_conv = block_sum4(sc, s_w[0]);
   const double fnorm = sum_rel * sum_xill / sum_conv;
@@ -1148,6 +1150,7 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
   // the conv grid edges go to the upper part of the buffer, free after the
   // half-length FFT (its values occupy slots below P(m))
   double *s_e = reinterpret_cast<double *>(s_buf + P(m));
+#pragma unroll 8
   for (int i = tid; i <= n; i += NT) s_e[i] = Ec[i];
   __syncthreads();
   // 5. rebin onto the caller grid; relconv zeroes outside 0.01-1000 keV
@@ -1155,6 +1158,7 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
   auto ge_c = [&](int k) { return s_e[k]; };
   const int nbo = (n_in + nwave - 1) / nwave;
   const int jend = min(n_in, (wave + 1) * nbo);
+#pragma unroll 4
   for (int q = 0; q < nbo; q += 64) {
     const int jb = wave * nbo + q + lane;
     if (jb < jend) {

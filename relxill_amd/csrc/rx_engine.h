@@ -96,6 +96,7 @@ struct DevReflSet {
 struct DevConvGrid {
   int n;               // bins (power of two)
   int i1kev;           // binary_search(ener, n+1, 1.0)
+  int band_lo, band_hi;  // bins with E_lo >= 0.01 and E_hi < 1000 keV: [band_lo, band_hi)
   const double *ener;  // n+1 edges
   const double *cfac;  // E/dE conversion, n
   const double *cfinv; // dE/E, n
