@@ -275,8 +275,19 @@ def main():
     alg_bytes = float(np.sum(68800 + 16 * nbins + 8))
     t_int = kms32[1] * 1e-3
     achieved = alg_bytes / t_int / 1e9
+    # measured HBM bytes per launch of the same kernel and work shape, from the
+    # committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated on a copy)
+    traffic, traffic_src = None, None
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic_relconv_batch4096.json")
+    if os.path.exists(tpath):
+        tj = json.load(open(tpath))
+        ki = tj.get("kernels", {}).get("k_integrate<1>")
+        if ki and tj.get("shape") == f"{seg},{wpc}" and len(reqs) == 4096:
+            traffic = round(ki["read_bytes"] + ki["write_bytes"])
+            traffic_src = tj.get("source")
     roofline = dict(bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=achieved / HBM_PEAK_GBS, traffic=None,
+                    frac=achieved / HBM_PEAK_GBS, traffic=traffic, traffic_source=traffic_src,
+                    algorithmic_bytes=round(alg_bytes),
                     kernel="k_integrate<1>", kernel_ms=round(kms32[1], 4),
                     kernel_ms_all={k: round(v, 4) for k, v in zip(KERNELS, kms32)})
 
