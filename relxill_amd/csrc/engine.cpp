@@ -220,7 +220,7 @@ void ensure_tables_host() {
       t.clear();
       for (int Ns = 1; Ns < N;) {
         const int left = N / Ns;
-        const int R = left % 16 == 0 ? 16 : left % 8 == 0 ? 8 : left % 4 == 0 ? 4 : 2;
+        const int R = left % 8 == 0 ? 8 : left % 4 == 0 ? 4 : 2;  // the device plan (lds_fft)
         for (int r = 0; r < R; r++) {
           for (int k = 0; k < Ns; k++) {
             const long double ang = -2.0L * 3.141592653589793238462643383279502884L * r * k / ((long double)Ns * R);

@@ -30,8 +30,8 @@ namespace rx {
 namespace {
 
 constexpr int SETUP_THREADS = 256;
-constexpr int CONV_THREADS = 256;      // convolution workgroup for grids up to 4096 bins
-constexpr int CONV_THREADS_MAX = 512;  // n/16 threads for larger grids (8192)
+constexpr int CONV_THREADS = 256;      // smallest convolution workgroup
+constexpr int CONV_THREADS_MAX = 1024;  // n/8 threads: 512 for 4096 bins, 1024 for 8192
 
 __device__ inline double lp_ifac(float value, const float *arr, int n, int &ind) {
   // get_ipol_factor: float search and float quotient (src/relutility.c:419-423)
@@ -887,8 +887,8 @@ __device__ __forceinline__ void fft_pass(double2 *buf, int N, int Ns, const doub
   __syncthreads();
 }
 
-// The block has at least N/16 threads, so a pass holds at most 16 values per
-// thread: PER = 1 (R = 16), <= 2 (R = 8), <= 4 (R = 4), <= 8 (R = 2).
+// The block has at least N/8 threads, so a pass holds at most 8 values per
+// thread: PER = 1 (R = 8), <= 2 (R = 4), <= 4 (R = 2); radix 16 keeps PER = 1.
 template <int R, int S>
 __device__ __forceinline__ void fft_pass_any(double2 *buf, int N, int Ns, const double2 *twp) {
   const int per = (N / R + blockDim.x - 1) / blockDim.x;
@@ -899,18 +899,15 @@ __device__ __forceinline__ void fft_pass_any(double2 *buf, int N, int Ns, const 
 }
 
 // Unnormalised complex FFT of N (power of two, 2..8192) values in LDS,
-// natural order in and out; radix-16 passes, then one of 8, 4 or 2.  twp
-// holds the passes' tables one after the other (R Ns entries each).
+// natural order in and out; radix-8 passes, then one of radix 4 or 2 (the
+// block has at least N/8 threads).  twp holds the passes' tables one after
+// the other (R Ns entries each), in the same plan as the host builds them.
 template <int S>
 __device__ __forceinline__ void lds_fft(double2 *buf, int N, const double2 *twp) {
   int Ns = 1;
   while (Ns < N) {
     const int left = N / Ns;
-    if (left % 16 == 0) {
-      fft_pass_any<16, S>(buf, N, Ns, twp);
-      twp += 16 * Ns;
-      Ns *= 16;
-    } else if (left % 8 == 0) {
+    if (left % 8 == 0) {
       fft_pass_any<8, S>(buf, N, Ns, twp);
       twp += 8 * Ns;
       Ns *= 8;
@@ -1039,7 +1036,7 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
   const int nb = n / nwave;  // bins per wavefront (a multiple of 64)
   if (!angular) {
     const GridGuess gi = grid_guess(ein, n_in);
-#pragma unroll 4
+#pragma unroll 2
     for (int q = 0; q < nb; q += 64) {
       const int i = wave * nb + q + lane;
       const double elo = Ec[i], ehi = Ec[i + 1];
@@ -1055,11 +1052,11 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
   // the profile as photons per bin, renormalised (renorm_relline_profile,
   // src/Relprofile.cpp:757-779); bins no window reaches are zero
   const BinRange rc = b.reach[req];
-  const int per = n / NT;
-  double ph[16];
+  const int per = n / NT;  // <= 8: the block has at least n/8 threads
+  double ph[8];
   double sph = 0.0;
 #pragma unroll
-  for (int q = 0; q < 16; q++) {
+  for (int q = 0; q < 8; q++) {
     const int i = tid + q * NT;
     ph[q] = (q < per && i >= rc.lo && i <= rc.hi) ? prof[i] * cg.inv_emid[i] : 0.0;
     sph += ph[q];
@@ -1070,7 +1067,7 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
   const double f = rq.do_renorm ? rq.norm / s2[1] : 1.0;
   double sr = 0.0;
 #pragma unroll
-  for (int q = 0; q < 16; q++) {
+  for (int q = 0; q < 8; q++) {
     const int i = tid + q * NT;
     if (q < per) {
       const double pv = ph[q] * f;
@@ -1158,7 +1155,7 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
   auto ge_c = [&](int k) { return s_e[k]; };
   const int nbo = (n_in + nwave - 1) / nwave;
   const int jend = min(n_in, (wave + 1) * nbo);
-#pragma unroll 4
+#pragma unroll 2
   for (int q = 0; q < nbo; q += 64) {
     const int jb = wave * nbo + q + lane;
     if (jb < jend) {
@@ -1257,7 +1254,7 @@ int launch_finalize(const BatchDev &b, void *stream) {
 }
 
 static size_t conv_lds(int n) { return (size_t)(n + n / 16) * sizeof(double2); }
-static int conv_threads(int n) { return std::max(CONV_THREADS, n / 16); }
+static int conv_threads(int n) { return std::max(CONV_THREADS, n / 8); }
 
 int launch_conv(const BatchDev &b, const int *conv_reqs, int n_conv_reqs, void *stream) {
   if (n_conv_reqs <= 0) return 0;
