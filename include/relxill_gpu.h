@@ -170,6 +170,9 @@ void rx_batch_destroy(rx_batch *b);
  * n_refl bins (photons/bin) on refl_energy (n_refl+1 edges).  Returns the set
  * id (>= 0) or a negative status. */
 int rx_reflset_create(const double *refl_energy, int n_refl, const double *refl, int n_incl);
+/* Frees every resident reflection set (after all batches using them are done);
+ * set ids start again from 0.  Needed before changing the conv grid size. */
+int rx_reflsets_clear(void);
 
 /* Kernel timing of the last rx_batch_run with profiling enabled: per kernel
  * (0 k_setup (with k_reset), 1 k_integrate, 2 k_fixup, 3 k_finalize, 4 k_conv)

@@ -104,6 +104,7 @@ _EXPORTS = {
     "rx_batch_dist": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_int]),
     "rx_batch_reflfrac": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D]),
     "rx_batch_set_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "rx_reflsets_clear": (ctypes.c_int, []),
     "rx_batch_destroy": (None, [ctypes.c_void_p]),
     "rx_reflset_create": (ctypes.c_int, [_D, ctypes.c_int, _D, ctypes.c_int]),
     "rx_profile_enable": (ctypes.c_int, [ctypes.c_int]),

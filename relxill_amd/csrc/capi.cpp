@@ -511,6 +511,18 @@ int rx_reflset_create(const double *refl_energy, int n_refl, const double *refl,
   return (int)g.refl_sets.size() - 1;
 }
 
+int rx_reflsets_clear(void) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (g.device_ready) (void)hipDeviceSynchronize();  // no running batch may still read them
+  for (DevBuf *bufs : g.refl_bufs) delete[] bufs;
+  g.refl_bufs.clear();
+  g.refl_sets.clear();
+  g.refl_dirty = true;
+  set_last(RX_OK, "");
+  return RX_OK;
+}
+
 /* ---- parameter processing (host mirror of get_rel_params), for checking ---- */
 int rx_model_params(int model, const double *parameter, double *out12) {
   if (!parameter || !out12) return RX_ERR_ARG;

@@ -53,6 +53,7 @@ def test_relxill_style_angle_resolved_conv(tabs):
                 assert rel_err(d, dist_ref) < tol
             b.close()
     finally:
+        rx.lib.rx_reflsets_clear()
         rx.set_conv_grid(4096, 0.00035, 2000.0)
 
 

@@ -148,3 +148,24 @@ def test_fp32_guard_fixup_path(tabs):
     _report("relline fp32 wide guard", y, ref)
     assert rel_err(y, ref) < TOL32
     assert rel_err(y, ref) <= rel_err(y_default, ref) * 1.5
+
+
+@pytest.mark.parametrize("n_conv", [1024, 2048, 8192])
+def test_relconv_other_conv_grids(tabs, n_conv):
+    """Other convolution grid sizes take other FFT plans (radix 8/4/2 passes)
+    and workgroup sizes (256 to 1024 threads)."""
+    e_ref, refl = reflection_spectrum(n_incl=1)
+    e = log_grid(1500, 0.1, 100.0)
+    fin = oracle.rebin(e, e_ref, refl[0])
+    cg = oracle.log_grid(n_conv + 1, 0.00035, 2000.0)
+    p = [3.0, 3.0, 15.0, 0.9, 45.0, -1.0, 400.0, 0.0]
+    ref, st = oracle.eval_model(oracle.MOD_RELCONV, tabs, p, e, flux_in=fin, conv_grid=cg)
+    assert st == 0
+    rx.set_conv_grid(n_conv, 0.00035, 2000.0)
+    try:
+        for prec, tol in ((rx.FP64, TOL64), (rx.FP32, TOL32)):
+            y, st = rx.eval_model(rx.MODEL_RELCONV, p, e, flux_in=fin, precision=prec)
+            _report(f"relconv n_conv={n_conv} prec {prec}", y, ref)
+            assert st == 0 and rel_err(y, ref) < tol
+    finally:
+        rx.set_conv_grid(4096, 0.00035, 2000.0)
