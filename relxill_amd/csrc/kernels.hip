@@ -640,15 +640,19 @@ __global__ __launch_bounds__(64) void k_integrate(BatchDev b, int cos_rows) {
     }
     const double *E = profile_grid(b, rq);
     double *acc = profile_acc(b, *rq);
+    double ehi = E[it.j0];
+    double ghi = (ehi - v.gmin) * v.del_g;  // unclamped g* of the bin's upper edge, carried
     for (int j = it.j0; j < it.j1; j++) {
-      const double elo = E[j], ehi = E[j + 1];
-      TV c = (TV)0;
+      const double elo = ehi, glo = ghi;
+      ehi = E[j + 1];
+      ghi = (ehi - v.gmin) * v.del_g;
+      double c = 0.0;
       if (j >= r.ielo && j <= r.iehi) {
         if (MODE == 0) {
-          c = (TV)(integ_bin<0>(elo, ehi, v) * r.coef);
+          c = integ_bin_g<0>(elo, ehi, glo, ghi, v) * r.coef;
         } else {
           bool need64 = false;
-          c = (TV)(integ_bin<1>(elo, ehi, v, &need64) * r.coef);
+          c = integ_bin_g<1>(elo, ehi, glo, ghi, v, &need64) * r.coef;
           if (need64) {
             const int slot = atomicAdd(&b.ctr[CTR_FIX], 1);
             if (slot < b.fix_cap) {
@@ -657,7 +661,7 @@ __global__ __launch_bounds__(64) void k_integrate(BatchDev b, int cos_rows) {
               fe.rad = i;
               fe.bin = j;
               fe.pad = 0;
-              fe.v32 = (double)c;
+              fe.v32 = c;
               b.fix[slot] = fe;
             } else {
               atomicOr(&b.ctr[CTR_FIX_OVF], 1);
@@ -667,7 +671,7 @@ __global__ __launch_bounds__(64) void k_integrate(BatchDev b, int cos_rows) {
       }
       // FP64 sum of the 64 per-radius values: the rounding of the sum over
       // radii stays at FP64 level whatever the number of radii
-      const double sum = wave_sum((double)c);
+      const double sum = wave_sum(c);
       if (lane == 0) unsafeAtomicAdd(&acc[j], sum);
     }
   }

@@ -279,12 +279,11 @@ struct BinF {
   float ga, gb, ua, ub, dg, e0, de;
 };
 
-RX_HD inline BinF make_binf(double rlo, double rhi, const RadiusView &v) {
+// ga, gb: the ends' g* = (E - gmin) del_g, computed by the caller
+RX_HD inline BinF make_binf(double rlo, double rhi, double ga, double gb) {
   BinF b;
   b.rlo = rlo;
   b.rhi = rhi;
-  const double ga = (rlo - v.gmin) * v.del_g;
-  const double gb = (rhi - v.gmin) * v.del_g;
   b.ga = (float)ga;
   b.gb = (float)gb;
   b.ua = (float)(1.0 - ga);
@@ -357,9 +356,11 @@ RX_HD inline bool romb_continue32(float R, float top, float lo, float hi, int n,
 // Romberg for both branches, FP32 values.  Sets *need64 when a stop decision
 // fell inside the guard band; the caller then recomputes in FP64.
 RX_HD inline float romberg32_pair(const BinF &bf, const RadiusView &v, bool *need64) {
-  float fa0, fa1, fb0, fb1;
+  float fa0, fa1, fb0, fb1, fm0, fm1;
+  // the three points of levels 0 and 1 are always needed: evaluate them together
   eval32s(0.0f, bf, v, fa0, fa1);
   eval32s(1.0f, bf, v, fb0, fb1);
+  eval32s(0.5f, bf, v, fm0, fm1);
   const float ta0 = (fa0 + fb0) * 0.5f, ta1 = (fa1 + fb1) * 0.5f;
   float pas = bf.de;
   float s0 = ta0, s1 = ta1;
@@ -377,13 +378,18 @@ RX_HD inline float romberg32_pair(const BinF &bf, const RadiusView &v, bool *nee
     const int nnew = 1 << (n - 1);
     const float ds = 1.0f / (float)(2 * nnew);
     float add0 = 0.0f, add1 = 0.0f;
-    float sm = ds;  // (2m+1) ds, exact: dyadic with at most 7 bits
-    for (int m = 0; m < nnew; m++) {
-      float g0v, g1v;
-      eval32s(sm, bf, v, g0v, g1v);
-      add0 += g0v;
-      add1 += g1v;
-      sm += 2.0f * ds;
+    if (n == 1) {
+      add0 += fm0;
+      add1 += fm1;
+    } else {
+      float sm = ds;  // (2m+1) ds, exact: dyadic with at most 7 bits
+      for (int m = 0; m < nnew; m++) {
+        float g0v, g1v;
+        eval32s(sm, bf, v, g0v, g1v);
+        add0 += g0v;
+        add1 += g1v;
+        sm += 2.0f * ds;
+      }
     }
     s0 += add0;
     s1 += add1;
@@ -420,20 +426,25 @@ RX_HD inline float romberg32_pair(const BinF &bf, const RadiusView &v, bool *nee
 
 // Integral of one energy bin [rlo0, rhi0] (line-energy units) at one radius,
 // in reduced units (multiply by emis/(gmax-gmin) for relb_func units).
+// glo_u, ghi_u: the ends' unclamped g* = (E - gmin) del_g (a caller walking
+// the bins of one radius carries the shared edge from bin to bin).
 // MODE 0: FP64; MODE 1: FP32 interior with FP64 decisions and edges.
 template <int MODE>
-RX_HD inline double integ_bin(double rlo0, double rhi0, const RadiusView &v, bool *need64 = nullptr) {
+RX_HD inline double integ_bin_g(double rlo0, double rhi0, double glo_u, double ghi_u, const RadiusView &v,
+                                bool *need64 = nullptr) {
 #pragma clang fp contract(off)
-  double gblo = (rlo0 / 1.0 - v.gmin) * v.del_g;
+  double gblo = glo_u;
   if (gblo < 0.0) gblo = 0.0; else if (gblo > 1.0) gblo = 1.0;
-  double gbhi = (rhi0 / 1.0 - v.gmin) * v.del_g;
+  double gbhi = ghi_u;
   if (gbhi < 0.0) gbhi = 0.0; else if (gbhi > 1.0) gbhi = 1.0;
   if (gbhi == 0) return 0.0;
   double rlo = rlo0, rhi = rhi0;
+  double ga = glo_u, gb = ghi_u;
   double flu = 0.0;
   if (gblo <= GFAC_H) {
     double hlo = gblo, hhi = GFAC_H;
     rlo = (GFAC_H * (v.gmax - v.gmin) + v.gmin) * 1.0;
+    ga = (rlo - v.gmin) * v.del_g;
     if (gbhi <= GFAC_H) {
       hhi = gbhi;
       rlo = -1.0;
@@ -443,6 +454,7 @@ RX_HD inline double integ_bin(double rlo0, double rhi0, const RadiusView &v, boo
   if (gbhi >= (1.0 - GFAC_H)) {
     double hhi = gbhi, hlo = 1.0 - GFAC_H;
     rhi = ((1 - GFAC_H) * (v.gmax - v.gmin) + v.gmin) * 1.0;
+    gb = (rhi - v.gmin) * v.del_g;
     if (gblo >= (1.0 - GFAC_H)) {
       hlo = gblo;
       rhi = -1.0;
@@ -455,7 +467,7 @@ RX_HD inline double integ_bin(double rlo0, double rhi0, const RadiusView &v, boo
         flu = flu + romberg64_pair(rlo, rhi, v);
       } else {
         // a stop decision inside the guard band is re-done in FP64 by the caller
-        const BinF bf = make_binf(rlo, rhi, v);
+        const BinF bf = make_binf(rlo, rhi, ga, gb);
         flu = flu + (double)romberg32_pair(bf, v, need64);
       }
     } else {
@@ -465,7 +477,7 @@ RX_HD inline double integ_bin(double rlo0, double rhi0, const RadiusView &v, boo
         eval64_pair(mid, v, f0, f1);
         flu = flu + (f0 * (rhi - rlo) + f1 * (rhi - rlo));
       } else {
-        const BinF bf = make_binf(rlo, rhi, v);
+        const BinF bf = make_binf(rlo, rhi, ga, gb);
         float f0, f1;
         eval32s(0.5f, bf, v, f0, f1);
         flu = flu + (double)((f0 + f1) * bf.de);
@@ -473,6 +485,13 @@ RX_HD inline double integ_bin(double rlo0, double rhi0, const RadiusView &v, boo
     }
   }
   return flu;
+}
+
+template <int MODE>
+RX_HD inline double integ_bin(double rlo0, double rhi0, const RadiusView &v, bool *need64 = nullptr) {
+#pragma clang fp contract(off)
+  return integ_bin_g<MODE>(rlo0, rhi0, (rlo0 / 1.0 - v.gmin) * v.del_g, (rhi0 / 1.0 - v.gmin) * v.del_g, v,
+                           need64);
 }
 
 // ---- small helpers shared by setup code ---------------------------------
