@@ -70,6 +70,59 @@ def make_cfg3(n_req, seed=5):
     return reqs
 
 
+def refl_angles(n_incl=10, seed=2):
+    """per-angle reflection spectra on 2999 log bins over 0.07-1000 keV: the cfg3
+    spectrum with a smooth angle modulation (seeded)"""
+    rng = np.random.default_rng(seed)
+    e = log_grid(2999, 0.07, 1000.0)
+    em = 0.5 * (e[1:] + e[:-1])
+    de = np.diff(e)
+    base = em ** -2.0 * np.exp(-em / 300.0)
+    lines = 0.02 * np.exp(-0.5 * ((em - 6.4) / 0.05) ** 2) + 0.008 * np.exp(-0.5 * ((em - 6.97) / 0.05) ** 2)
+    out = np.empty((n_incl, e.size - 1))
+    for k in range(n_incl):
+        mu = 1.0 - (k + 0.5) / n_incl
+        amp = 1.0 + 0.3 * (mu - 0.5) + 0.05 * rng.standard_normal()
+        out[k] = (base * em ** (0.05 * (mu - 0.5)) + lines * (1.0 + 0.5 * mu)) * amp * de
+    return e, out
+
+
+def relxill_style(rx, steps):
+    """relxill-style angle-resolved relconv: profile on 4096 log bins over
+    0.01-1000 keV with 10 angle bins, resident reflection set, output on 3000
+    bins over 0.1-100 keV.  Plain FP64 (one request per launch) and a batch of
+    1024 parameter sets (FP32 and FP64)."""
+    rx.set_conv_grid(4096, 0.01, 1000.0)
+    e_ref, refl = refl_angles()
+    sid = rx.reflset_create(e_ref, refl)
+    e_out = log_grid(3000, 0.1, 100.0)
+    rng = np.random.default_rng(7)
+    reqs = []
+    for _ in range(1024):
+        a = rng.uniform(-0.998, 0.998)
+        incl = rng.uniform(5.0, 85.0)
+        rin = rng.uniform(kerr_rms(a), 50.0)
+        reqs.append(dict(model=-1, energy=e_out, params=[3.0, 3.0, 15.0, a, incl, rin, 400.0, 0.0], refl_set=sid))
+    lat = []
+    for r in reqs[:8]:
+        pb = rx.Batch([r])
+        d, _ = timed_steps(rx, pb, rx.FP64, 3, 1)
+        lat.append(d)
+        pb.close()
+    b = rx.Batch(reqs)
+    b.autotune(rx.FP32)
+    dt32, _ = timed_steps(rx, b, rx.FP32, max(3, steps // 2), 1)
+    o32 = [b.output(i)[0] for i in range(len(reqs))]
+    dt64, _ = timed_steps(rx, b, rx.FP64, 2, 1)
+    o64 = [b.output(i)[0] for i in range(len(reqs))]
+    b.close()
+    rx.lib.rx_reflsets_clear()
+    rx.set_conv_grid(4096, 0.00035, 2000.0)
+    return dict(plain_fp64_ms_per_spectrum=float(np.mean(lat)) * 1e3,
+                batched_fp32_spectra_per_s=len(reqs) / dt32, batched_fp64_spectra_per_s=len(reqs) / dt64,
+                max_rel_err_fp32_vs_fp64=max(rel_err(o32[i], o64[i]) for i in range(len(reqs))))
+
+
 def make_cfg2(n_req, seed=4):
     rng = np.random.default_rng(seed)
     e = log_grid(3000, 0.1, 10.0)
@@ -264,11 +317,17 @@ def main():
     log(f"cfg2 lp relline fp32: {cfg2}")
     b2.close()
 
-    serving = None
+    # cfg1: relxill-style angle-resolved convolution
+    cfg1 = relxill_style(rx, args.steps)
+    log(f"cfg1 relxill-style: {cfg1}")
+
+    serving = serving_high = None
     if not args.no_serve:
         qps = float(min(4000.0, 0.4 * value))
         serving = serve(rx, args.serve_seconds, qps)
         log(f"serving: {serving}")
+        serving_high = serve(rx, args.serve_seconds, 20000.0, seed=16)
+        log(f"serving at 20k QPS offered: {serving_high}")
 
     # roofline of the dominant kernel (relline integration): algorithmic bytes
     # per spectrum = table corners 68,800 B + grid/in/out 16 N + 8 B
@@ -333,7 +392,9 @@ def main():
         "max_rel_err_fp32_vs_fp64_gpu": err32_64,
         "max_rel_err_fp32_vs_oracle": max_err_oracle,
         "lp_relline_batch": cfg2,
+        "relxill_style_conv": cfg1,
         "serving": serving,
+        "serving_20k_qps": serving_high,
         "p99_latency_ms": serving["p99_ms"] if serving else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
