@@ -8,6 +8,12 @@
  * cpu_baseline leg, always as the checker or the timed CPU baseline, never as
  * part of the product path.
  *
+ * Pinning: tests/test_oracle_kats.py checks the functions against the
+ * reference's known-answer tests and against closed forms.  Parity partially
+ * pinned (exercised, no dedicated closed form): the ISCO / 1000 rg snaps of
+ * the table interpolation and the 75 deg lin/log switch of the lamp-post
+ * radius interpolation.
+ *
  * Conventions follow the reference:
  *   - energy grids have n+1 ascending edges, fluxes n values (photons/bin);
  *   - relline tables are float32, node-major [ia][imu0][ir](.. [ig]);
