@@ -143,6 +143,31 @@ __device__ __forceinline__ int count_le(E e0, int n0, double x, const GridGuess 
   return lo;
 }
 
+template <int CTRL>
+__device__ inline int dpp_int(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+
+// min / max over the 64 lanes (quad, half-row and row exchanges, then the rows);
+// every lane must be active
+__device__ inline int wave_min_i(int v) {
+  v = min(v, dpp_int<0xB1>(v));
+  v = min(v, dpp_int<0x4E>(v));
+  v = min(v, dpp_int<0x141>(v));
+  v = min(v, dpp_int<0x140>(v));
+  return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+             min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
+__device__ inline int wave_max_i(int v) {
+  v = max(v, dpp_int<0xB1>(v));
+  v = max(v, dpp_int<0x4E>(v));
+  v = max(v, dpp_int<0x141>(v));
+  v = max(v, dpp_int<0x140>(v));
+  return max(max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+             max(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
 // block-wide deterministic sum (fixed tree order)
 __device__ inline double block_sum(double v, double *red) {
   const int tid = threadIdx.x;
@@ -316,7 +341,10 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b) {
   auto ge = [&](int k) { return E[k]; };
   double area = 0.0;
   int err = 0;
-  for (int ii = tid; ii < nr; ii += SETUP_THREADS) {
+  for (int ii0 = 0; ii0 < nr; ii0 += SETUP_THREADS) {  // same trip count in every wavefront
+    const int ii = ii0 + tid;
+    int wlo = 0x7fffffff, whi = -1;
+    if (ii < nr) {
     const double re = fine_radius(rq.rin, rq.rout, ii, nr);
     const double rprev = (ii > 0) ? fine_radius(rq.rin, rq.rout, ii - 1, nr) : 0.0;
     const double rnext = (ii < nr - 1) ? fine_radius(rq.rin, rq.rout, ii + 1, nr) : 0.0;
@@ -377,9 +405,17 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b) {
     r.iehi = iehi;
     r.pad = 0;
     rad[ii] = r;
-    if (ielo <= iehi) {  // energy window of the radius chunk
-      atomicMin(&s_ulo[ii / RAD_CHUNK], ielo);
-      atomicMax(&s_uhi[ii / RAD_CHUNK], iehi);
+    if (ielo <= iehi) {
+      wlo = ielo;
+      whi = iehi;
+    }
+    }  // ii < nr
+    // energy window of the radius chunk: this wavefront's 64 consecutive radii
+    wlo = wave_min_i(wlo);
+    whi = wave_max_i(whi);
+    if ((tid & 63) == 0 && ii < nr) {
+      s_ulo[ii / RAD_CHUNK] = wlo;
+      s_uhi[ii / RAD_CHUNK] = whi;
     }
   }
   if (err) atomicMax(&s_st, err);
@@ -1247,7 +1283,7 @@ int launch_fixup(const BatchDev &b, bool cos_rows, void *stream) {
   if (b.mode != 1 || b.n_req <= 0) return 0;
   const size_t lds = row_lds(b, cos_rows, sizeof(double));
   (void)hipFuncSetAttribute((const void *)k_fixup, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(k_fixup, dim3(256), dim3(64), lds, (hipStream_t)stream, b, (int)cos_rows);
+  hipLaunchKernelGGL(k_fixup, dim3(1024), dim3(64), lds, (hipStream_t)stream, b, (int)cos_rows);
   return (int)hipGetLastError();
 }
 
