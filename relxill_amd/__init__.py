@@ -13,7 +13,8 @@ _LIB_NAMES = {
     "MODEL_RELXILL_CONV", "DEFAULTS", "N_PARAMS", "Batch", "RxError", "eval_model", "lib",
     "lmodrelconv", "lmodrelconvlp", "lmodrelline", "lmodrellinelp", "model_params",
     "reflset_create", "set_conv_grid", "tables_synthetic", "lptable_synthetic", "tables_view",
-    "Server",
+    "Server", "RX_OK", "RX_ERR_PARAM", "RX_ERR_UNSUPPORTED", "RX_ERR_TABLE", "RX_ERR_INPUT",
+    "RX_ERR_DEVICE", "RX_ERR_ARG",
 }
 
 __all__ = sorted(_LIB_NAMES)

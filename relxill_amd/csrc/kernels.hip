@@ -516,7 +516,10 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b) {
     }
     if (tid == 0) b.reach[req] = BinRange{jlo, jhi};
     double *acc = profile_acc(b, rq);
-    for (int j = jlo + tid; j <= jhi; j += SETUP_THREADS) acc[j] = 0.0;
+    // (no window reaches the grid: jlo = INT_MAX > jhi = -1, nothing to clear;
+    // jlo + tid must not be formed then, it would overflow)
+    if (jlo <= jhi)
+      for (int j = jlo + tid; j <= jhi; j += SETUP_THREADS) acc[j] = 0.0;
     // work items: each radius chunk's window cut into segments of seg_bins
     const int S = b.seg_bins;
     if (tid == 0) {
