@@ -149,6 +149,7 @@ int rx_tables_synthetic(uint64_t seed, int n_a, int n_mu0, int n_r, int n_g) {
   }
   g.rel_loaded = true;
   g.rel_dirty = true;
+  g.config_gen++;  // batches built before hold buffers sized for the old table
   std::string e = validate_reltable(g.rel);
   set_last(e.empty() ? RX_OK : RX_ERR_TABLE, e);
   return e.empty() ? RX_OK : RX_ERR_TABLE;
@@ -164,6 +165,7 @@ int rx_lptable_synthetic(uint64_t seed, int n_a, int n_h, int n_r) {
   g.lp = make_synthetic_lptable(seed, n_a, n_h, n_r);
   g.lp_loaded = true;
   g.lp_dirty = true;
+  g.config_gen++;
   std::string e = validate_lptable(g.lp);
   set_last(e.empty() ? RX_OK : RX_ERR_TABLE, e);
   return e.empty() ? RX_OK : RX_ERR_TABLE;
@@ -182,6 +184,7 @@ int rx_tables_load(const char *path) {
   g.rel = std::move(t);
   g.rel_loaded = true;
   g.rel_dirty = true;
+  g.config_gen++;  // batches built before hold buffers sized for the old table
   return RX_OK;
 }
 
@@ -198,6 +201,7 @@ int rx_lptable_load(const char *path) {
   g.lp = std::move(t);
   g.lp_loaded = true;
   g.lp_dirty = true;
+  g.config_gen++;
   return RX_OK;
 }
 
@@ -255,6 +259,7 @@ int rx_set_conv_grid(int n, double emin, double emax) {
   g.conv_emin = emin;
   g.conv_emax = emax;
   g.conv_dirty = true;
+  g.config_gen++;
   ensure_tables_host();
   return RX_OK;
 }
@@ -263,6 +268,7 @@ int rx_set_nr_fine(int nr_fine) {
   Global &g = G();
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   if (nr_fine < 3 || nr_fine > MAX_NR_FINE) return RX_ERR_ARG;
+  if (nr_fine != g.nr_fine) g.config_gen++;
   g.nr_fine = nr_fine;
   return RX_OK;
 }
@@ -519,6 +525,7 @@ int rx_reflsets_clear(void) {
   g.refl_bufs.clear();
   g.refl_sets.clear();
   g.refl_dirty = true;
+  g.config_gen++;  // set ids of existing batches are void
   set_last(RX_OK, "");
   return RX_OK;
 }

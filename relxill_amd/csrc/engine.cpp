@@ -275,6 +275,14 @@ int ensure_device(std::string &err) {
     (void)hipSetDevice(g.device);
   }
   int e = 0;
+  if (g.rel_dirty || g.lp_dirty || g.conv_dirty || g.refl_dirty) {
+    // resident buffers are about to be overwritten or reallocated: no kernel
+    // of any stream may still read them
+    if (hipDeviceSynchronize() != hipSuccess) {
+      err = "device synchronisation failed";
+      return RX_ERR_DEVICE;
+    }
+  }
   if (g.rel_dirty) {
     const RelTableHost &t = g.rel;
     if (t.n_r > MAX_NR_TAB || t.n_g > MAX_NG) {
@@ -452,6 +460,7 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
   while ((1LL << lg) < bins) lg++;
   b.shape_class = lg * 2 + (b.h_conv.empty() ? 0 : 1);
   batch_plan(b, b.seg_forced, b.waves_forced);
+  b.config_gen = g.config_gen;
   return batch_upload(b);
 }
 
@@ -549,6 +558,10 @@ static BatchDev dev_view(Batch &b) {
 
 int batch_run(Batch &b, int precision, std::string &err) {
   Global &g = G();
+  if (b.config_gen != g.config_gen) {
+    err = "tables, convolution grid, fine-grid size or reflection sets changed since the batch was built";
+    return RX_ERR_ARG;
+  }
   b.last_mode = (precision == RX_FP32) ? 1 : 0;
   BatchDev bd = dev_view(b);
   hipStream_t s = b.stream;

@@ -49,6 +49,7 @@ struct Batch {
   bool profiled = false;
   double kernel_ms[RX_N_TIMED] = {};
   int shape_class = 0;
+  uint64_t config_gen = 0;  // Global::config_gen at build time
   ~Batch();
 };
 
@@ -91,6 +92,10 @@ struct Global {
   DevBuf d_refl_sets;
   bool refl_dirty = false;
   bool profile = false;
+  // bumped by every change that invalidates built batches (tables, conv grid,
+  // fine-grid size, clearing reflection sets); a batch refuses to run on a
+  // newer configuration than it was built for
+  uint64_t config_gen = 1;
   float guard = 2e-3f;
   // autotune cache: shape class -> (segment bins, waves per CU)
   struct Tuned {

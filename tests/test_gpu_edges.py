@@ -98,3 +98,27 @@ def test_batch_statuses_are_per_request(tabs):
     assert s2 == rx.RX_ERR_INPUT and np.all(y2 == 0)
     assert s3 == 0 and np.isnan(y3).all()
     b.close()
+
+
+def test_stale_batch_refuses_to_run(tabs):
+    """A batch's device buffers are sized for the configuration it was built
+    for; after a change of the fine-grid size it reports an error instead of
+    running, and a rebuilt batch runs."""
+    e = log_grid(600, 0.1, 10.0)
+    b = rx.Batch([dict(model=1, energy=e, params=RELLINE_CFG0)])
+    b.run(rx.FP32)
+    b.sync()
+    assert rx.lib.rx_set_nr_fine(500) == 0
+    try:
+        with pytest.raises(rx.RxError):
+            b.run(rx.FP32)
+    finally:
+        assert rx.lib.rx_set_nr_fine(1000) == 0
+    b.close()
+    b = rx.Batch([dict(model=1, energy=e, params=RELLINE_CFG0)])
+    b.run(rx.FP32)
+    b.sync()
+    ref, _ = oracle.eval_model(1, tabs, RELLINE_CFG0, e)
+    y, st = b.output(0)
+    assert st == 0 and rel_err(y, ref) < 1e-4
+    b.close()
