@@ -187,7 +187,7 @@ __device__ inline double block_sum(double v, double *red) {
 // ---------------------------------------------------------------------------
 // setup
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b) {
+__global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
   const int req = blockIdx.x;
   const int tid = threadIdx.x;
   const ReqParams rq = b.req[req];
