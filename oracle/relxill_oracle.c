@@ -429,9 +429,9 @@ void orc_syspar_free(orc_syspar *s) {
 /* interpol_relTable; src/Relprofile.cpp:141-303 (with interpol_a_mu0 :39-80)
  * Pinned by closed forms (tests/test_oracle_kats.py): the (a, mu0) bilinear
  * node on an affine field, the spin-lerped node radii and the node -> fine
- * mapping on a radius-affine field.  Parity partially pinned: the ISCO and
- * 1000 rg snaps (:195-200, :235-238) have no dedicated closed form; they run in
- * every Rin = ISCO evaluation of the parity tests. */
+ * mapping on a radius-affine field; the ISCO and 1000 rg snaps (:195-200,
+ * :235-238) by the ramps they put into the innermost and outermost node
+ * intervals of a radius-valued table. */
 int orc_interpol_reltable(const orc_reltable *tab, double a, double incl, double rin,
                           double rout, orc_syspar *s) {
   const int nr = tab->n_r, ng = tab->n_g;
@@ -605,8 +605,9 @@ static int calc_refl_frac(const orc_syspar *s, double rin, double rout, double d
  * interpol_emisprofile_spin_height (:190-210), rebin_emisprofile_on_radial_grid
  * (:120-175), calc_refl_frac (:36-89), apply_emis_fluxboost_source_disk
  * (:249-253), all from calc_emis_jet_point_source (:266-285).
- * Parity partially pinned: the 75 deg lin/log switch of the radius factor has
- * no dedicated closed form; the lamp-post parity draws cross it. */
+ * Pinned by closed forms: the 75 deg lin/log switch of the radius factor on a
+ * table whose fields are exact under one factor on each side of the switch;
+ * the boost g^gamma by the u^t ratio. */
 static int emis_lamp_post(const orc_lptable *tab, const orc_relparam *p, orc_syspar *s) {
   const int nh = tab->n_h, ntr = tab->n_r;
   int ind_a;

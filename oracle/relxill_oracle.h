@@ -9,10 +9,9 @@
  * part of the product path.
  *
  * Pinning: tests/test_oracle_kats.py checks the functions against the
- * reference's known-answer tests and against closed forms.  Parity partially
- * pinned (exercised, no dedicated closed form): the ISCO / 1000 rg snaps of
- * the table interpolation and the 75 deg lin/log switch of the lamp-post
- * radius interpolation.
+ * reference's known-answer tests and against closed forms, including the
+ * ISCO / 1000 rg snaps of the table interpolation and the 75 deg lin/log
+ * switch of the lamp-post radius interpolation.
  *
  * Conventions follow the reference:
  *   - energy grids have n+1 ascending edges, fluxes n values (photons/bin);

@@ -231,3 +231,57 @@ def radial_expected(rin, rout, u=0.01, v=1.0, q=3.0, nr=1000, node_r=None):
         k = np.searchsorted(-node_r, -re, side="right") - 1  # node_r[k] >= re > node_r[k+1]
         x = node_r[k] + node_r[k + 1] - re
     return u + v / 1000.0 * np.sum(w * x) / np.sum(w)
+
+
+def switch_lp_table(r_s=10.0):
+    """The default lamp-post grids with fields that are exact under one of the
+    two radius factors of rebin_emisprofile_on_radial_grid: below r_s the
+    emission angle exceeds 75 deg and intensity is a power law, emission and
+    incidence angles affine in log r (exact under the log factor); from r_s on
+    the angle is below 75 deg and intensity exp(-lambda r), the angles affine
+    in r (exact under the linear factor).  Same values at every spin and
+    height."""
+    rel, lp, T = tables()
+    out = dict(lp)
+    rad = lp["rad"].reshape(lp["n_a"], 1, lp["n_r"]).astype(np.float64)
+    rad = np.broadcast_to(rad, (lp["n_a"], lp["n_h"], lp["n_r"]))
+    lo = rad < r_s
+    out["intens"] = np.where(lo, rad ** -3.0, 5e-3 * np.exp(-5e-3 * rad)).astype(np.float32).ravel()
+    out["del_"] = np.where(lo, 1.40 + 0.05 * np.log(rad), 0.3 + 9e-4 * rad).astype(np.float32).ravel()
+    out["del_inc"] = np.where(lo, 0.2 + 0.03 * np.log(rad), 0.1 + 5e-4 * rad).astype(np.float32).ravel()
+    return rel, out
+
+
+def switch_fields(r, lo):
+    return (np.where(lo, r ** -3.0, 5e-3 * np.exp(-5e-3 * r)),
+            np.where(lo, 1.40 + 0.05 * np.log(r), 0.3 + 9e-4 * r),
+            np.where(lo, 0.2 + 0.03 * np.log(r), 0.1 + 5e-4 * r))
+
+
+
+def isco_snap_table(above=5e-4):
+    """Flat window, innermost table radius ISCO (1 + above) at every spin node
+    (inside the 1e-3 snap band of src/Relprofile.cpp:195-200) and a transfer
+    function equal to the table radius: trff_1 = trff_2 = r/2."""
+    rel = flat_table(0.999, 1.001)
+    na, nm, nr, ng = rel["n_a"], rel["n_mu0"], rel["n_r"], rel["n_g"]
+    r = rel["r"].reshape(na, nm, nr).astype(np.float64).copy()
+    for ia in range(na):
+        r[ia, :, -1] = oracle.lib.orc_kerr_rms(float(rel["a"][ia])) * (1 + above)
+    rel["r"] = r.astype(np.float32).ravel()
+    t = 0.5 * rel["r"].astype(np.float64)
+    rel["trff1"] = np.repeat(t, ng).astype(np.float32)
+    rel["trff2"] = rel["trff1"].copy()
+    return rel
+
+
+def write_lptb(path, lp):
+    """Writes a lamp-post table dict (tables_view layout) in the library's RXTB
+    format: int32 magic 'RXTB', kind 2, n_a, n_h, n_r, then the float32 arrays
+    a, h, rad, intens, del, del_inc."""
+    hdr = np.array([0x42545852, 2, lp["n_a"], lp["n_h"], lp["n_r"]], dtype=np.int32)
+    with open(path, "wb") as f:
+        f.write(hdr.tobytes())
+        for k in ("a", "h", "rad", "intens", "del_", "del_inc"):
+            f.write(np.ascontiguousarray(lp[k], dtype=np.float32).tobytes())
+

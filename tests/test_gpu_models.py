@@ -210,3 +210,49 @@ def test_radial_mapping_closed_form(tabs, tmp_path):
         got = fr.sum() / ff.sum()
         print(f"radial mapping prec {prec}: {got:.10f} expected {expect:.10f}", flush=True)
         assert abs(got / expect - 1) < tol
+
+
+def test_isco_snap_table_parity(tabs, tmp_path):
+    """The ISCO snap (src/Relprofile.cpp:195-200) on the GPU: a table whose
+    innermost radius sits 5e-4 above the ISCO, at a table spin, bin by bin
+    against the oracle."""
+    from tests.helpers import isco_snap_table, rel_err
+    rel = isco_snap_table()
+    a = float(rel["a"][int(np.searchsorted(rel["a"], 0.5))])
+    e = np.linspace(0.998, 1.002, 4001)
+    p = [1.0, 3.0, 3.0, 15.0, a, 40.0, -1.0, 20.0, 0.0, 0.0]
+    ref, st = oracle.eval_model(oracle.MOD_RELLINE, oracle.Tables(rel), p, e, phys_norm=True)
+    assert st == 0
+
+    def run():
+        for prec, tol in ((rx.FP64, 1e-8), (rx.FP32, 1e-4)):
+            f, st = rx.eval_model(rx.MODEL_RELLINE, p, e, precision=prec)
+            err = rel_err(f, ref)
+            print(f"isco snap table prec {prec}: status {st} max rel err {err:.3e}", flush=True)
+            assert st == 0 and err < tol
+    _with_table(tmp_path, rel, run)
+
+
+def test_lp_switch_table_parity(tabs, tmp_path):
+    """The 75 deg lin/log radius factor of the lamp-post emissivity
+    (src/relutility.c:407-417, src/Rellp.cpp:120-175) on the GPU: a lamp-post table whose
+    emission angle crosses 75 deg at 10 rg, relline_lp bin by bin against the
+    oracle."""
+    from tests.helpers import rel_err, switch_lp_table, write_lptb
+    rel, lp = switch_lp_table(10.0)
+    a = float(lp["a"][11])
+    e = log_grid(3000, 0.1, 10.0)
+    p = [6.4, 6.0, a, 30.0, -1.0, 100.0, 0.0, 0.0, 2.0, 0.0]
+    ref, st = oracle.eval_model(oracle.MOD_RELLINELP, oracle.Tables(rel, lp), p, e)
+    assert st == 0
+    path = str(tmp_path / "lp.rxtb")
+    write_lptb(path, lp)
+    try:
+        assert rx.lib.rx_lptable_load(path.encode()) == 0, rx.lib.rx_last_error()
+        for prec, tol in ((rx.FP64, 1e-8), (rx.FP32, 1e-4)):
+            f, st = rx.eval_model(rx.MODEL_RELLINELP, p, e, precision=prec)
+            err = rel_err(f, ref)
+            print(f"lp switch table prec {prec}: status {st} max rel err {err:.3e}", flush=True)
+            assert st == 0 and err < tol
+    finally:
+        rx.lptable_synthetic(3, 20, 100, 100)

@@ -351,3 +351,129 @@ def test_radial_mapping_closed_form(a, incl):
     fa = float(np.float32((np.float32(a) - rel["a"][ia]) / (rel["a"][ia + 1] - rel["a"][ia])))
     node_r = fa * r[ia + 1] + (1 - fa) * r[ia]
     assert abs(got / radial_expected(rin, 400.0, node_r=node_r) - 1) > 1e-5
+
+
+def test_outer_radius_snap_closed_form():
+    """The 1000 rg snap (src/Relprofile.cpp:235-238): the table's outermost node
+    radius 999 is moved to 1000, so on a radius-affine transfer function with a
+    flat emissivity (q = 0, outer radii dominate) the fine values in the outer
+    node interval follow the ramp from (r_1, u + v r_1/1000) to (1000,
+    u + v 999/1000) instead of u + v re/1000."""
+    from tests.helpers import flat_table, radial_table
+    e = np.linspace(0.998, 1.002, 4001)
+    a, incl = 0.5, 40.0
+    p = [1.0, 0.0, 0.0, 15.0, a, incl, -2.0, 1000.0, 0.0, 0.0]
+    rel = radial_table()
+    fr, st = oracle.eval_model(oracle.MOD_RELLINE, oracle.Tables(rel), p, e, phys_norm=True)
+    ff, st2 = oracle.eval_model(oracle.MOD_RELLINE, oracle.Tables(flat_table(0.999, 1.001)), p, e, phys_norm=True)
+    assert st == 0 and st2 == 0
+    got = fr.sum() / ff.sum()
+    rin = 2 * oracle.lib.orc_kerr_rms(a)
+    re = oracle.fine_radial_grid(rin, 1000.0, 1000)
+    dr = np.empty_like(re)
+    dr[0] = 0.5 * (re[0] - re[1])
+    dr[-1] = 0.5 * (re[-2] - re[-1])
+    dr[1:-1] = 0.5 * (re[:-2] - re[2:])
+    w = re * dr  # q = 0
+    r = rel["r"].reshape(rel["n_a"], rel["n_mu0"], rel["n_r"])[:, 0, :].astype(np.float64)
+    ia = np.searchsorted(rel["a"], np.float32(a), side="right") - 1
+    fa = float(np.float32((np.float32(a) - rel["a"][ia]) / (rel["a"][ia + 1] - rel["a"][ia])))
+    r0, r1 = fa * r[ia + 1, 0] + (1 - fa) * r[ia, 0], fa * r[ia + 1, 1] + (1 - fa) * r[ia, 1]
+    assert r0 < 1000.0 < r0 * 1.01  # the snap applies
+    u, v = 0.01, 1.0
+    x = u + v * re / 1000.0
+    outer = re > r1
+    ramp = (re[outer] - r1) / (1000.0 - r1)
+    x_snap = x.copy()
+    x_snap[outer] = (1 - ramp) * (u + v * r1 / 1000.0) + ramp * (u + v * r0 / 1000.0)
+    expect_snap = np.sum(w * x_snap) / np.sum(w)
+    expect_plain = np.sum(w * x) / np.sum(w)
+    assert abs(got / expect_snap - 1) < 1e-7
+    assert abs(got / expect_plain - 1) > 1e-5
+
+
+def test_lp_radius_factor_switch_closed_form():
+    """rebin_emisprofile_on_radial_grid (src/Rellp.cpp:120-175) interpolates in
+    log r where the lower node's emission angle exceeds 75 deg and linearly in
+    r elsewhere (get_ipol_factor_radius, src/relutility.c:407-417), with intensity log-
+    interpolated.  On _switch_lp_table each region's fields are reproduced
+    exactly by its own factor and missed by the other one (by 1e-5 .. 1e-3);
+    the node interval that straddles r_s follows the lower node's (log) factor:
+    a power law through its two intensity nodes."""
+    r_s = 10.0
+    from tests.helpers import switch_fields, switch_lp_table
+    rel, lp = switch_lp_table(r_s)
+    T = oracle.Tables(rel, lp)
+    a = float(lp["a"][11])  # a table spin: no spin interpolation
+    gamma, h = 2.0, 6.0
+    rp, st = oracle.rel_params(oracle.MOD_RELLINELP, [6.4, h, a, 30.0, -1.0, 100.0, 0.0, 0.0, gamma, 0.0])
+    assert st == 0
+    sp, st = oracle.system_parameters(T, rp)
+    assert st == 0
+    re = sp["re"]
+    ut_d = (re * np.sqrt(re) + a) / (np.sqrt(re) * np.sqrt(re * re - 3 * re + 2 * a * np.sqrt(re)))
+    ut_h = np.sqrt((h * h + a * a) / (h * h - 2 * h + a * a))
+    emis = sp["emis"] / (ut_d / ut_h) ** gamma  # beta = 0: the boost is g^gamma
+    rad = lp["rad"].reshape(lp["n_a"], lp["n_r"])[11].astype(np.float64)
+    k = np.searchsorted(rad, re, side="right") - 1
+    lo_node = rad[k] < r_s
+    straddle = lo_node & (rad[k + 1] >= r_s)
+    inside = ~straddle
+    i_x, d_x, di_x = switch_fields(re, re < r_s)
+    assert inside.sum() > 500 and straddle.sum() > 5 and (re[inside] < r_s).sum() > 100
+    assert np.max(np.abs(emis[inside] / i_x[inside] - 1)) < 2e-6
+    assert np.max(np.abs(sp["del_emit"][inside] - d_x[inside])) < 1e-6
+    assert np.max(np.abs(sp["del_inc"][inside] - di_x[inside])) < 1e-6
+    # the other factor in each region misses
+    f_lin = (re - rad[k]) / (rad[k + 1] - rad[k])
+    i_lo, d_lo, _ = switch_fields(rad[k], rad[k] < r_s)
+    i_hi, d_hi, _ = switch_fields(rad[k + 1], rad[k + 1] < r_s)
+    f_log = np.log(re / rad[k]) / np.log(rad[k + 1] / rad[k])
+    f_wrong = np.where(lo_node, f_lin, f_log)
+    d_wrong = (1 - f_wrong) * d_lo + f_wrong * d_hi
+    i_wrong = i_lo * (i_hi / i_lo) ** f_wrong
+    miss_d = np.abs(d_wrong - d_x)[inside]
+    miss_i = np.abs(i_wrong / i_x - 1)[inside]
+    mid = inside & (f_lin > 0.3) & (f_lin < 0.7)
+    assert np.min(np.abs(d_wrong - d_x)[mid]) > 3e-6 and np.max(miss_d) > 2e-5
+    assert np.min(np.abs(i_wrong / i_x - 1)[mid]) > 5e-6 and np.max(miss_i) > 1e-4
+    # straddling interval: lower node > 75 deg -> power law through the nodes
+    pl = i_lo * (re / rad[k]) ** (np.log(i_hi / i_lo) / np.log(rad[k + 1] / rad[k]))
+    assert np.max(np.abs(emis[straddle] / pl[straddle] - 1)) < 2e-6
+    lin = i_lo * (i_hi / i_lo) ** f_lin
+    assert np.max(np.abs(lin[straddle] / pl[straddle] - 1)) > 5e-3
+
+
+def test_isco_snap_closed_form():
+    """The ISCO snap (src/Relprofile.cpp:195-200): an innermost node radius
+    within 1e-3 above the ISCO is moved onto it.  Table: innermost radius
+    ISCO (1 + 5e-4) at every spin node, transfer function equal to the table
+    radius (trff_1 = trff_2 = r/2).  At a table spin the fine values equal re
+    except in the innermost node interval, which ramps from (r_1, r_1) to
+    (ISCO, r_0 (1 + 5e-4)); without the snap the mean would be that of re."""
+    from tests.helpers import flat_table, isco_snap_table
+    rel = isco_snap_table()
+    ia = int(np.searchsorted(rel["a"], 0.5))
+    a = float(rel["a"][ia])
+    rms = oracle.lib.orc_kerr_rms(a)
+    nodes = rel["r"].reshape(rel["n_a"], rel["n_mu0"], rel["n_r"])[ia, 0].astype(np.float64)
+    assert nodes[-1] > rms and (nodes[-1] - rms) / nodes[-1] < 1e-3
+    e = np.linspace(0.998, 1.002, 4001)
+    p = [1.0, 3.0, 3.0, 15.0, a, 40.0, -1.0, 20.0, 0.0, 0.0]
+    fr, st = oracle.eval_model(oracle.MOD_RELLINE, oracle.Tables(rel), p, e, phys_norm=True)
+    ff, st2 = oracle.eval_model(oracle.MOD_RELLINE, oracle.Tables(flat_table(0.999, 1.001)), p, e, phys_norm=True)
+    assert st == 0 and st2 == 0
+    got = fr.sum() / ff.sum()
+    re = oracle.fine_radial_grid(rms, 20.0, 1000)
+    dr = np.empty_like(re)
+    dr[0] = 0.5 * (re[0] - re[1])
+    dr[-1] = 0.5 * (re[-2] - re[-1])
+    dr[1:-1] = 0.5 * (re[:-2] - re[2:])
+    w = re * dr * re ** -3.0
+    x = re.copy()
+    last = re < nodes[-2]
+    ramp = (nodes[-2] - re[last]) / (nodes[-2] - rms)
+    x[last] = (1 - ramp) * nodes[-2] + ramp * nodes[-1]
+    assert last.sum() > 10
+    assert abs(got / (np.sum(w * x) / np.sum(w)) - 1) < 1e-7
+    assert abs(got / (np.sum(w * re) / np.sum(w)) - 1) > 1e-6
