@@ -283,6 +283,8 @@ def main():
         print(json.dumps({"ms_per_step": dt32 * 1e3, "kernel_ms": kms32.tolist(), "value": value}))
         return
     out32 = [b.output(i)[0] for i in range(len(reqs))]
+    work32 = b.counters()
+    log(f"cfg3 fp32 work: {work32}")
 
     dt64b, kms64b = timed_steps(rx, b, rx.FP64, max(2, args.steps // 3), 1)
     out64 = [b.output(i)[0] for i in range(len(reqs))]
@@ -390,6 +392,7 @@ def main():
         "batched_fp64_spectra_per_s": round(len(reqs) / dt64b, 2),
         "lmodrelconv_ms_per_call": round(lmod_ms, 3),
         "max_rel_err_fp32_vs_fp64_gpu": err32_64,
+        "fp32_work": work32,
         "max_rel_err_fp32_vs_oracle": max_err_oracle,
         "lp_relline_batch": cfg2,
         "relxill_style_conv": cfg1,

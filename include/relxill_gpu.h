@@ -186,6 +186,11 @@ int rx_copy_test(int64_t bytes, int reps, double *ms);
 /* Waits for all work of the library on its device. */
 int rx_device_synchronize(void);
 int rx_batch_kernel_ms(rx_batch *b, double *ms);
+/* Work counters of the last completed rx_batch_run (after rx_batch_sync):
+ * out[0] work items integrated, out[1] FP32 bins re-checked in FP64,
+ * out[2] 1 if the re-check queue overflowed (the batch was then redone in
+ * FP64).  No counterpart in the reference (diagnostics of the batched path). */
+int rx_batch_counters(rx_batch *b, int64_t *out);
 
 /* Autotuner: times candidate work shapes (segment bins x waves per CU) for
  * this batch on the device and keeps the fastest for batches of the same

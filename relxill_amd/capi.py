@@ -111,6 +111,7 @@ _EXPORTS = {
     "rx_device_synchronize": (ctypes.c_int, []),
     "rx_copy_test": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _D]),
     "rx_batch_kernel_ms": (ctypes.c_int, [ctypes.c_void_p, _D]),
+    "rx_batch_counters": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "rx_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "rx_server_start": (ctypes.c_void_p, [ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int]),
     "rx_server_submit": (ctypes.c_int64, [ctypes.c_void_p, ctypes.POINTER(_Request)]),
@@ -311,6 +312,12 @@ class Batch:
         o = np.zeros(5)
         _check(lib.rx_batch_kernel_ms(self.h, _dp(o)))
         return o
+
+    def counters(self):
+        """work items, FP64 re-checks and re-check overflow of the last run"""
+        o = (ctypes.c_int64 * 3)()
+        _check(lib.rx_batch_counters(self.h, o))
+        return dict(items=o[0], fp64_rechecks=o[1], recheck_overflow=o[2])
 
     def close(self):
         if getattr(self, "h", None):

@@ -419,6 +419,12 @@ int rx_batch_kernel_ms(rx_batch *b, double *ms) {
   return b->b.profiled ? RX_OK : RX_ERR_ARG;
 }
 
+int rx_batch_counters(rx_batch *b, int64_t *out) {
+  if (!b || !out) return RX_ERR_ARG;
+  for (int k = 0; k < 3; k++) out[k] = b->b.counters[k];
+  return RX_OK;
+}
+
 // Times every (segment bins, waves per CU) candidate on this batch and keeps
 // the fastest for batches of the same shape class (log2 of the profile bins
 // and whether convolutions are present).

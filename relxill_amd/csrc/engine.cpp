@@ -501,6 +501,7 @@ int batch_upload(Batch &b) {
   e |= b.d_pgrid.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_pgrid));
   e |= b.d_node.ensure(sizeof(double) * nreq * 2 * (size_t)g.rel.n_r * g.rel.n_g * 2);
   e |= b.d_rad.ensure(sizeof(RadRec) * nreq * g.nr_fine);
+  e |= b.d_rnorm.ensure(sizeof(ReqNorm) * nreq);
   e |= b.d_prof.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_prof));
   e |= b.d_dist.ensure(sizeof(double) * nreq * MAX_INCL);
   e |= b.d_refl.ensure(sizeof(LpRefl) * nreq);
@@ -540,6 +541,7 @@ static BatchDev dev_view(Batch &b) {
   bd.pgrid = b.d_pgrid.as<double>();
   bd.node = b.d_node.as<double>();
   bd.rad = b.d_rad.as<RadRec>();
+  bd.rnorm = b.d_rnorm.as<ReqNorm>();
   bd.prof = b.d_prof.as<double>();
   bd.dist = b.d_dist.as<double>();
   bd.refl = b.d_refl.as<LpRefl>();
@@ -594,6 +596,9 @@ int batch_sync(Batch &b) {
   int ctr[N_COUNTERS] = {};
   if (hipMemcpy(ctr, b.d_ctr.p, sizeof ctr, hipMemcpyDeviceToHost) != hipSuccess) return RX_ERR_DEVICE;
   b.fix_overflow = b.last_mode == 1 && ctr[CTR_FIX_OVF] != 0;
+  b.counters[0] = std::min(ctr[CTR_ITEMS], b.item_cap);
+  b.counters[1] = (b.last_mode == 1) ? ctr[CTR_FIX] : 0;
+  b.counters[2] = b.fix_overflow ? 1 : 0;
   if (b.fix_overflow) {
     // the FP64 re-check queue overflowed: redo the batch in FP64
     std::string err;

@@ -38,7 +38,7 @@ struct Batch {
   std::vector<int> h_conv;
   int64_t tot_ener = 0, tot_flux = 0, tot_pgrid = 0, tot_prof = 0;
   DevBuf d_req, d_ener, d_flux_in, d_out, d_pgrid, d_node, d_rad, d_prof, d_dist, d_refl, d_status, d_conv;
-  DevBuf d_items, d_ctr, d_fix, d_reach;
+  DevBuf d_items, d_ctr, d_fix, d_reach, d_rnorm;
   int item_cap = 0;
   bool any_cosne = false, any_limb = false;
   int fix_cap = 0;
@@ -48,6 +48,7 @@ struct Batch {
   hipEvent_t ev[RX_N_TIMED + 1] = {};
   bool profiled = false;
   double kernel_ms[RX_N_TIMED] = {};
+  int64_t counters[3] = {};  // items, FP64 re-checks, re-check overflow (last run)
   int shape_class = 0;
   uint64_t config_gen = 0;  // Global::config_gen at build time
   ~Batch();

@@ -168,6 +168,20 @@ __device__ inline int wave_max_i(int v) {
              max(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
 }
 
+// block-wide maximum
+__device__ inline double block_max(double v, double *red) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (tid < s) red[tid] = fmax(red[tid], red[tid + s]);
+    __syncthreads();
+  }
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+
 // block-wide deterministic sum (fixed tree order)
 __device__ inline double block_sum(double v, double *red) {
   const int tid = threadIdx.x;
@@ -340,6 +354,7 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
   const GridGuess gE = grid_guess(E, rq.n_e);
   auto ge = [&](int k) { return E[k]; };
   double area = 0.0;
+  double cmax = 0.0;  // largest weight of a radius with a window
   int err = 0;
   for (int ii0 = 0; ii0 < nr; ii0 += SETUP_THREADS) {  // same trip count in every wavefront
     const int ii = ii0 + tid;
@@ -348,13 +363,15 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     const double re = fine_radius(rq.rin, rq.rout, ii, nr);
     const double rprev = (ii > 0) ? fine_radius(rq.rin, rq.rout, ii - 1, nr) : 0.0;
     const double rnext = (ii < nr - 1) ? fine_radius(rq.rin, rq.rout, ii + 1, nr) : 0.0;
-    // K = largest k with tre[k] > re (the reference's downward walk)
+    // K = largest k <= ind_rmin with tre[k] > re: the walk of :250-265 starts
+    // at the Rin bracket and only moves outwards, so a fine radius rounded
+    // just below the innermost node extrapolates that bracket (ifac_r < 0)
     int lo = 0, hi = n_r;
     while (lo < hi) {
       int mid = (lo + hi) >> 1;
       if (s_tre[mid] > re) lo = mid + 1; else hi = mid;
     }
-    int K = lo - 1;
+    int K = min(lo - 1, s_rmin);
     if (K < 0) {
       if (re - 1000.0 <= 1e-6) K = 0; else err = 13;
     }
@@ -397,9 +414,9 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     r.gmin = gmin;
     r.gmax = gmax;
     r.del_g = 1. / (gmax - gmin);
-    r.coef = weight;   // scaled below once the emissivity norm is known
+    r.coef = emis / (gmax - gmin) * weight;  // times ReqNorm.fac
     r.ifac_r = ifac_r;
-    r.emisw = emis;
+    r.emisw = emis * weight;                 // times ReqNorm.fac
     r.K = K;
     r.ielo = ielo;
     r.iehi = iehi;
@@ -408,6 +425,7 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     if (ielo <= iehi) {
       wlo = ielo;
       whi = iehi;
+      cmax = fmax(cmax, fabs(r.coef));
     }
     }  // ii < nr
     // energy window of the radius chunk: this wavefront's 64 consecutive radii
@@ -419,20 +437,24 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     }
   }
   if (err) atomicMax(&s_st, err);
-  // BKN normalisation: sum emis * 2 pi r dr (norm_emis_profile, src/Rellp.cpp:91-109)
+  // BKN normalisation: sum emis * 2 pi r dr (norm_emis_profile, src/Rellp.cpp:91-109),
+  // applied through ReqNorm instead of a second pass over the records
   const double total = block_sum(area, s_red);
+  const double cmax_all = block_max(cmax, s_red);
   if (s_st) {
     if (tid == 0) b.status[req] = s_st;
     return;
   }
-  const double inv_area_den = lp ? 1.0 : total;
-  for (int ii = tid; ii < nr; ii += SETUP_THREADS) {
-    RadRec r = rad[ii];
-    const double emis = r.emisw / inv_area_den;
-    const double weight = r.coef;
-    r.emisw = emis * weight;
-    r.coef = emis / (r.gmax - r.gmin) * weight;
-    rad[ii] = r;
+  const double fac = lp ? 1.0 : 1.0 / total;
+  if (tid == 0) {
+    ReqNorm n;
+    n.fac = fac;
+    const double m = cmax_all * fac;
+    int k = 0;
+    if (m > 0.0 && m < INFINITY) k = min(1000, max(-1000, -ilogb(m)));
+    n.cmul = ldexp(fac, k);
+    n.cinv = ldexp(1.0, -k);
+    b.rnorm[req] = n;
   }
 
   // reflection fractions (calc_refl_frac, src/Rellp.cpp:36-89)
@@ -493,7 +515,7 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
       const int imu = ((int)(rq.n_incl * (1 - cosne) + 1)) - 1;
       const double dgs = (g == 0 || g == ng - 1) ? 0.5 * (gs1 - gs0) + GFAC_H : gs1 - gs0;
       const double pr = 2 * PI * gv * re;
-      const double tmp = re * (pr * pr) / sqrt(gs - gs * gs) * trff * r.emisw * dgs;
+      const double tmp = re * (pr * pr) / sqrt(gs - gs * gs) * trff * (r.emisw * fac) * dgs;
       if (imu < 0 || imu >= rq.n_incl) {
         derr = 40;
         continue;
@@ -623,17 +645,31 @@ __device__ inline const double *profile_grid(const BatchDev &b, const ReqParams 
 
 }  // namespace
 
+// Byte offset of k_integrate's coefficient slots and bin buffer (FP64 path)
+// behind the lane rows (rows: 64 or 128 of 2 n_g + 1 values).
+__host__ __device__ inline size_t integrate_rows_bytes(int ng, bool cos_rows, size_t elem) {
+  const size_t b = (size_t)(cos_rows ? 2 : 1) * 64 * (2 * ng + 1) * elem;
+  return (b + 15) & ~(size_t)15;
+}
+
 // Persistent wavefronts pull work items (request, chunk of 64 fine radii, bin
 // segment) from a shared counter until the list is exhausted.  Lane l owns
 // fine radius 64 rc + l: its transfer-function row sits in LDS and its window,
-// emissivity factor and edge norms in registers.  For every bin of the
-// segment each lane integrates the bin at its radius (zero outside its
-// window), the wavefront sums the 64 values in FP64 in a fixed order and adds
-// the sum to the request's profile.  MODE 0: FP64; MODE 1: FP32 integrand with
-// FP64 decisions and edges; bins whose Romberg stop test fell in the guard
-// band are queued for the FP64 re-check (k_fixup).
+// weight and edge norms in registers.  For every bin of the segment each lane
+// integrates the bin at its radius (zero outside its window) and parks the
+// value in LDS; every SUM_BINS bins the wavefront turns round: 8 lanes per bin
+// each sum 8 radii in FP64 (fixed order), a 3-step DPP tree adds the 8 partial
+// sums and one lane per bin adds the result to the request's profile with one
+// FP64 atomic.
+//   MODE 0 (FP64): the parked values are the FP64 bin integrals; the
+//   turn-round multiplies them by the radii's weights (64 slots in LDS).
+//   MODE 1 (FP32 integrand, FP64 decisions and edges): each lane parks its
+//   integral times its weight times the request's power of two
+//   (ReqNorm.cmul) as FP32, and the atomic adds the sum times ReqNorm.cinv.
+//   Bins whose Romberg stop test fell in the guard band are queued for the
+//   FP64 re-check (k_fixup).
 template <int MODE>
-__global__ __launch_bounds__(64) void k_integrate(BatchDev b, int cos_rows) {
+__global__ __launch_bounds__(64, MODE == 0 ? 1 : 4) void k_integrate(BatchDev b, int cos_rows) {
   using TV = typename std::conditional<MODE == 0, double, float>::type;
   extern __shared__ double s_dyn[];
   const int lane = threadIdx.x;
@@ -642,8 +678,13 @@ __global__ __launch_bounds__(64) void k_integrate(BatchDev b, int cos_rows) {
   TV *rows = reinterpret_cast<TV *>(s_dyn);
   TV *myT = rows + lane * RS;
   TV *myC = cos_rows ? rows + (64 + lane) * RS : myT;
+  // behind the rows: MODE 0 the 64 weights then the parked doubles, MODE 1 the parked floats
+  double *s_coef = reinterpret_cast<double *>(reinterpret_cast<char *>(s_dyn) +
+                                              integrate_rows_bytes(ng, cos_rows != 0, sizeof(TV)));
+  TV *s_bins = (MODE == 0) ? reinterpret_cast<TV *>(s_coef + 64) : reinterpret_cast<TV *>(s_coef);
   const int n_items = min(b.ctr[CTR_ITEMS], b.item_cap);
   const int nr = b.nr_fine;
+  const int col = lane >> 3, part = lane & 7;  // turn-round roles: bin col, radii 8 part ..
   RadiusView v;
   v.ng = ng;
   v.gs_step = (1.0 - 2 * GFAC_H) / (ng - 1);
@@ -667,6 +708,7 @@ __global__ __launch_bounds__(64) void k_integrate(BatchDev b, int cos_rows) {
     if (k >= n_items) break;
     const WorkItem it = b.items[k];
     const ReqParams *rq = b.req + it.req;
+    const ReqNorm rn = b.rnorm[it.req];
     const int limb = rq->limb;
     const int i = it.rc * RAD_CHUNK + lane;
     RadRec r;
@@ -677,41 +719,71 @@ __global__ __launch_bounds__(64) void k_integrate(BatchDev b, int cos_rows) {
       r = b.rad[(size_t)it.req * nr + i];
       if (r.ielo <= r.iehi) load_radius<TV>(b, it.req, r, limb, myT, myC, v);
     }
+    const bool has_win = r.ielo <= r.iehi;
+    const double w64 = has_win ? r.coef * rn.fac : 0.0;   // MODE 0 weight
+    const double w32 = has_win ? r.coef * rn.cmul : 0.0;  // MODE 1 scaled weight
+    if constexpr (MODE == 0) s_coef[lane] = w64;
     const double *E = profile_grid(b, rq);
     double *acc = profile_acc(b, *rq);
     double ehi = E[it.j0];
+    double enext = E[min(it.j0 + 1, it.j1)];
     double ghi = (ehi - v.gmin) * v.del_g;  // unclamped g* of the bin's upper edge, carried
-    for (int j = it.j0; j < it.j1; j++) {
-      const double elo = ehi, glo = ghi;
-      ehi = E[j + 1];
-      ghi = (ehi - v.gmin) * v.del_g;
-      double c = 0.0;
-      if (j >= r.ielo && j <= r.iehi) {
-        if (MODE == 0) {
-          c = integ_bin_g<0>(elo, ehi, glo, ghi, v) * r.coef;
-        } else {
-          bool need64 = false;
-          c = integ_bin_g<1>(elo, ehi, glo, ghi, v, &need64) * r.coef;
-          if (need64) {
-            const int slot = atomicAdd(&b.ctr[CTR_FIX], 1);
-            if (slot < b.fix_cap) {
-              FixEntry fe;
-              fe.req = it.req;
-              fe.rad = i;
-              fe.bin = j;
-              fe.pad = 0;
-              fe.v32 = c;
-              b.fix[slot] = fe;
-            } else {
-              atomicOr(&b.ctr[CTR_FIX_OVF], 1);
+    EdgePair<TV> cy;  // integrand pair at the next bin's lower edge, when a Romberg bin left it
+    cy.ok = false;
+    for (int jg = it.j0; jg < it.j1; jg += SUM_BINS) {
+      const int jn = min(SUM_BINS, it.j1 - jg);
+      for (int jj = 0; jj < jn; jj++) {
+        const int j = jg + jj;
+        const double elo = ehi, glo = ghi;
+        ehi = enext;
+        enext = E[min(j + 2, it.j1)];  // next bin's upper edge, loaded a bin ahead
+        ghi = (ehi - v.gmin) * v.del_g;
+        TV val = 0;
+        if (j >= r.ielo && j <= r.iehi) {
+          if constexpr (MODE == 0) {
+            val = integ_bin_g<0>(elo, ehi, glo, ghi, v, nullptr, &cy);
+          } else {
+            bool need64 = false;
+            val = (float)(integ_bin_g<1>(elo, ehi, glo, ghi, v, &need64, &cy) * w32);
+            if (need64) {
+              const int slot = atomicAdd(&b.ctr[CTR_FIX], 1);
+              if (slot < b.fix_cap) {
+                FixEntry fe;
+                fe.req = it.req;
+                fe.rad = i;
+                fe.bin = j;
+                fe.pad = 0;
+                fe.v32 = (double)val * rn.cinv;  // this radius' share of the bin's atomic
+                b.fix[slot] = fe;
+              } else {
+                atomicOr(&b.ctr[CTR_FIX_OVF], 1);
+              }
             }
           }
+        } else {
+          cy.ok = false;
         }
+        s_bins[jj * SUM_STRIDE + lane] = val;
       }
-      // FP64 sum of the 64 per-radius values: the rounding of the sum over
-      // radii stays at FP64 level whatever the number of radii
-      const double sum = wave_sum(c);
-      if (lane == 0) unsafeAtomicAdd(&acc[j], sum);
+      __syncthreads();  // one wavefront: orders the parked values before the turn-round
+      // FP64 sum over the 64 radii of bin jg + col: 8 radii per lane, then the
+      // 8 lanes of the bin in a fixed DPP tree
+      const TV *src = s_bins + col * SUM_STRIDE + part * 8;
+      double sum = 0.0;
+      if constexpr (MODE == 0) {
+        const double *cf = s_coef + part * 8;
+#pragma unroll
+        for (int q = 0; q < 8; q++) sum = fma(src[q], cf[q], sum);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 8; q++) sum += (double)src[q];
+      }
+      sum += dpp_d<0xB1>(sum);   // quad_perm [1,0,3,2]
+      sum += dpp_d<0x4E>(sum);   // quad_perm [2,3,0,1]
+      sum += dpp_d<0x141>(sum);  // row_half_mirror: every lane of the 8 holds the total
+      if (MODE == 1) sum *= rn.cinv;
+      if (part == 0 && col < jn) unsafeAtomicAdd(&acc[jg + col], sum);
+      __syncthreads();  // the next bins overwrite the parked values
     }
   }
 }
@@ -740,7 +812,7 @@ __global__ __launch_bounds__(64) void k_fixup(BatchDev b, int cos_rows) {
     const RadRec r = b.rad[(size_t)fe.req * b.nr_fine + fe.rad];
     load_radius<double>(b, fe.req, r, rq->limb, myT, myC, v);
     const double *E = profile_grid(b, rq);
-    const double val = integ_bin<0>(E[fe.bin], E[fe.bin + 1], v) * r.coef;
+    const double val = integ_bin<0>(E[fe.bin], E[fe.bin + 1], v) * (r.coef * b.rnorm[fe.req].fac);
     unsafeAtomicAdd(&profile_acc(b, *rq)[fe.bin], val - fe.v32);
   }
 }
@@ -1268,15 +1340,23 @@ static size_t row_lds(const BatchDev &b, bool cos_rows, size_t elem) {
   return (size_t)(cos_rows ? 2 : 1) * 64 * (2 * b.rt.n_g + 1) * elem;
 }
 
+// lane rows, then (FP64 path) 64 weight slots, then the SUM_BINS x SUM_STRIDE parked values
+static size_t integrate_lds(const BatchDev &b, bool cos_rows, size_t elem) {
+  const size_t rows = integrate_rows_bytes(b.rt.n_g, cos_rows, elem);
+  const size_t w = (elem == sizeof(double)) ? 64 * sizeof(double) : 0;
+  return rows + w + (size_t)SUM_BINS * SUM_STRIDE * elem;
+}
+
 int launch_integrate(const BatchDev &b, int n_blocks, bool cos_rows, void *stream) {
   if (b.n_req <= 0 || n_blocks <= 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   if (b.mode == 0) {
-    const size_t lds = row_lds(b, cos_rows, sizeof(double));
+    const size_t lds = integrate_lds(b, cos_rows, sizeof(double));
     (void)hipFuncSetAttribute((const void *)k_integrate<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k_integrate<0>, dim3(n_blocks), dim3(64), lds, st, b, (int)cos_rows);
   } else {
-    const size_t lds = row_lds(b, cos_rows, sizeof(float));
+    const size_t lds = integrate_lds(b, cos_rows, sizeof(float));
+    (void)hipFuncSetAttribute((const void *)k_integrate<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k_integrate<1>, dim3(n_blocks), dim3(64), lds, st, b, (int)cos_rows);
   }
   return (int)hipGetLastError();

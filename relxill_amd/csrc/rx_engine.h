@@ -69,6 +69,16 @@ struct RadRec {
   int pad;
 };
 
+// Per-request factors of the radius weights.  RadRec.coef holds the weight
+// without the emissivity normalisation; fac is that normalisation (1/area for
+// the broken power law, 1 for the lamp post).  The FP32 path sums the 64
+// per-radius values of a bin in FP32; cmul = fac 2^k brings the request's
+// largest weight near 1 for that (far from FP32 under- and overflow) and
+// cinv = 2^-k undoes it exactly.
+struct ReqNorm {
+  double fac, cmul, cinv;
+};
+
 struct LpRefl {
   double refl_frac, f_bh, f_ad, f_inf, f_inf_rest;
 };
@@ -110,6 +120,14 @@ struct DevConvGrid {
 // owns fine radius 64 c + l.
 constexpr int RAD_CHUNK = 64;
 constexpr int MAX_NR_FINE = 64 * RAD_CHUNK;
+
+// k_integrate parks the per-radius values of SUM_BINS consecutive bins in LDS
+// ([bin][lane], row stride SUM_STRIDE: lane (c, p) of the turn-round reads
+// values 8 p .. 8 p + 7 of bin c, and the odd stride spreads the 8 bins over
+// the banks) and then sums each bin with 8 lanes (8 radii each, then a 3-step
+// DPP tree).
+constexpr int SUM_BINS = 8;
+constexpr int SUM_STRIDE = 65;
 
 // One unit of integration work: bins [j0, j1) of one request for the fine
 // radii of one chunk.  k_setup emits the segments of each chunk's energy
@@ -154,6 +172,7 @@ struct BatchDev {
   double *pgrid;
   double *node;
   RadRec *rad;
+  ReqNorm *rnorm;
   double *prof;
   double *dist;
   LpRefl *refl;

@@ -41,6 +41,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace rx {
 
 constexpr double GFAC_H = 5e-3;          // src/Relprofile.cpp:36
@@ -197,13 +199,24 @@ RX_HD inline double int_edge_pre(double blo, double bhi, const RadiusView &v) {
   return 2 * norm * (sqrt(hi) - sqrt(lo)) * 1.0 * (v.gmax - v.gmin);
 }
 
+// Integrand values at a bin end, handed from one bin to the next: a bin's
+// upper end is the next bin's lower end, so a Romberg bin following a Romberg
+// bin reuses the pair instead of evaluating it again (same arguments, same
+// bits).  ok: f0/f1 hold the pair at the current bin's lower edge.
+template <class T>
+struct EdgePair {
+  T f0, f1;
+  bool ok;
+};
+
 // Romberg for both branches on [a,b], FP64.  Returns t_0[niter0][0] + t_1[niter1][0].
 // R[ii] holds the current anti-diagonal t[ii][n-ii] of the reference's table
-// (src/Relprofile.cpp:559-576), updated in place level by level.
-RX_HD inline double romberg64_pair(double a, double b, const RadiusView &v) {
+// (src/Relprofile.cpp:559-576), updated in place level by level.  With
+// have_a, (fa0, fa1) is the pair at a; the pair at b is returned in (fb0, fb1).
+RX_HD inline double romberg64_pair(double a, double b, const RadiusView &v, bool have_a, double fa0,
+                                   double fa1, double &fb0, double &fb1) {
 #pragma clang fp contract(off)
-  double fa0, fa1, fb0, fb1;
-  eval64_pair(a, v, fa0, fa1);
+  if (!have_a) eval64_pair(a, v, fa0, fa1);
   eval64_pair(b, v, fb0, fb1);
   const double ta0 = (fa0 + fb0) / 2.0, ta1 = (fa1 + fb1) / 2.0;
   double pas = b - a;
@@ -276,7 +289,7 @@ RX_HD inline float rx_rsqrtf(float x) {
 // of g* and 1-g* at FP32 rounding), E(s) = rlo + s dE.
 struct BinF {
   double rlo, rhi;  // FP64 ends (fallback path)
-  float ga, gb, ua, ub, dg, e0, de;
+  float ga, gb, ua, ub, dg, e0, eb, de;
 };
 
 // ga, gb: the ends' g* = (E - gmin) del_g, computed by the caller
@@ -290,20 +303,24 @@ RX_HD inline BinF make_binf(double rlo, double rhi, double ga, double gb) {
   b.ub = (float)(1.0 - gb);
   b.dg = (float)(gb - ga);
   b.e0 = (float)rlo;
+  b.eb = (float)rhi;
   b.de = (float)(rhi - rlo);
   return b;
 }
 
-// Both branches at s (exact dyadic in [0,1]).
+// Both branches at s (exact dyadic in [0,1]); coordinates from the nearer end,
+// so s = 1 gives exactly the values the next bin sees at its s = 0.
 RX_HD inline void eval32s(float s, const BinF &bf, const RadiusView &v, float &f0, float &f1) {
-  float t, u;
+  float t, u, e;
   if (s <= 0.5f) {
     t = fmaf(s, bf.dg, bf.ga);
     u = fmaf(-s, bf.dg, bf.ua);
+    e = fmaf(s, bf.de, bf.e0);
   } else {
     const float sr = 1.0f - s;
     t = fmaf(-sr, bf.dg, bf.gb);
     u = fmaf(sr, bf.dg, bf.ub);
+    e = fmaf(-sr, bf.de, bf.eb);
   }
   const float inv_step = (float)v.inv_step;
   const float q = (t - (float)GFAC_H) * inv_step;
@@ -311,7 +328,6 @@ RX_HD inline void eval32s(float s, const BinF &bf, const RadiusView &v, float &f
   const float thr = 1e-6f * (inv_step + 1.0f);
   int ind;
   float inte;
-  float e = fmaf(s, bf.de, bf.e0);
   if (fabsf(q - qi) < thr && qi >= 1.0f && qi <= (float)(v.ng - 2)) {
     // close to a g* node: decide the bracket with the FP64 coordinates
     const double eg = bf.rlo + (bf.rhi - bf.rlo) * (double)s;
@@ -354,11 +370,14 @@ RX_HD inline bool romb_continue32(float R, float top, float lo, float hi, int n,
 }
 
 // Romberg for both branches, FP32 values.  Sets *need64 when a stop decision
-// fell inside the guard band; the caller then recomputes in FP64.
-RX_HD inline float romberg32_pair(const BinF &bf, const RadiusView &v, bool *need64) {
-  float fa0, fa1, fb0, fb1, fm0, fm1;
-  // the three points of levels 0 and 1 are always needed: evaluate them together
-  eval32s(0.0f, bf, v, fa0, fa1);
+// fell inside the guard band; the caller then recomputes in FP64.  With
+// have_a, (fa0, fa1) is the pair at s = 0; the pair at s = 1 is returned in
+// (fb0, fb1).
+RX_HD inline float romberg32_pair(const BinF &bf, const RadiusView &v, bool *need64, bool have_a, float fa0,
+                                  float fa1, float &fb0, float &fb1) {
+  float fm0, fm1;
+  // the points of levels 0 and 1 are always needed: evaluate them together
+  if (!have_a) eval32s(0.0f, bf, v, fa0, fa1);
   eval32s(1.0f, bf, v, fb0, fb1);
   eval32s(0.5f, bf, v, fm0, fm1);
   const float ta0 = (fa0 + fb0) * 0.5f, ta1 = (fa1 + fb1) * 0.5f;
@@ -429,10 +448,15 @@ RX_HD inline float romberg32_pair(const BinF &bf, const RadiusView &v, bool *nee
 // glo_u, ghi_u: the ends' unclamped g* = (E - gmin) del_g (a caller walking
 // the bins of one radius carries the shared edge from bin to bin).
 // MODE 0: FP64; MODE 1: FP32 interior with FP64 decisions and edges.
-template <int MODE>
+// cy (optional): the integrand pair at rlo0 when cy->ok; on return cy holds the
+// pair at rhi0 when this bin evaluated it there (a Romberg bin whose upper end
+// is the bin edge), else cy->ok is false.
+template <int MODE, class TV = typename std::conditional<MODE == 0, double, float>::type>
 RX_HD inline double integ_bin_g(double rlo0, double rhi0, double glo_u, double ghi_u, const RadiusView &v,
-                                bool *need64 = nullptr) {
+                                bool *need64 = nullptr, EdgePair<TV> *cy = nullptr) {
 #pragma clang fp contract(off)
+  const bool have_a = cy && cy->ok;
+  if (cy) cy->ok = false;
   double gblo = glo_u;
   if (gblo < 0.0) gblo = 0.0; else if (gblo > 1.0) gblo = 1.0;
   double gbhi = ghi_u;
@@ -441,6 +465,7 @@ RX_HD inline double integ_bin_g(double rlo0, double rhi0, double glo_u, double g
   double rlo = rlo0, rhi = rhi0;
   double ga = glo_u, gb = ghi_u;
   double flu = 0.0;
+  const bool lo_edge = (gblo <= GFAC_H), hi_edge = (gbhi >= (1.0 - GFAC_H));
   if (gblo <= GFAC_H) {
     double hlo = gblo, hhi = GFAC_H;
     rlo = (GFAC_H * (v.gmax - v.gmin) + v.gmin) * 1.0;
@@ -463,12 +488,19 @@ RX_HD inline double integ_bin_g(double rlo0, double rhi0, double glo_u, double g
   }
   if ((rhi >= 0) && (rlo >= 0)) {
     if (rlo >= 1.0 * RED_WING) {
-      if (MODE == 0) {
-        flu = flu + romberg64_pair(rlo, rhi, v);
+      const bool use_a = have_a && !lo_edge;  // the lower end is the shared bin edge
+      TV fa0 = use_a ? cy->f0 : (TV)0, fa1 = use_a ? cy->f1 : (TV)0, fb0, fb1;
+      if constexpr (MODE == 0) {
+        flu = flu + romberg64_pair(rlo, rhi, v, use_a, fa0, fa1, fb0, fb1);
       } else {
         // a stop decision inside the guard band is re-done in FP64 by the caller
         const BinF bf = make_binf(rlo, rhi, ga, gb);
-        flu = flu + (double)romberg32_pair(bf, v, need64);
+        flu = flu + (double)romberg32_pair(bf, v, need64, use_a, fa0, fa1, fb0, fb1);
+      }
+      if (cy && !hi_edge) {
+        cy->f0 = fb0;
+        cy->f1 = fb1;
+        cy->ok = true;
       }
     } else {
       if (MODE == 0) {

@@ -69,7 +69,7 @@ int64_t request_bytes(const Pending &p) {
   b += (int64_t)sizeof(FixEntry) * (n_e / 8 + 1);
   b += (int64_t)sizeof(RadRec) * g.nr_fine;
   b += 8 * 2 * (int64_t)g.rel.n_r * g.rel.n_g * 2;
-  b += 8 * MAX_INCL + sizeof(LpRefl) + 64;
+  b += 8 * MAX_INCL + sizeof(LpRefl) + sizeof(ReqNorm) + 64;
   return b;
 }
 

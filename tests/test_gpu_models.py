@@ -9,7 +9,7 @@ import pytest
 
 import oracle
 import relxill_amd as rx
-from tests.helpers import (RELLINE_CFG0, draw_lp_params, draw_relconv_params, flat_table, log_grid,
+from tests.helpers import (RELLINE_CFG0, abs_err, draw_lp_params, draw_relconv_params, flat_table, log_grid,
                            reflection_spectrum, rel_err, tables, write_rxtb)
 
 pytestmark = pytest.mark.gpu
@@ -94,6 +94,16 @@ def _mixed_requests(n, seed):
     return reqs
 
 
+def _same_up_to_sum_order(y, solo, model):
+    """A request's output against its solo run: only the order of the FP64
+    atomic adds of the per-chunk sums differs (~1e-16 of each bin).  Line
+    models keep that per bin; the convolution's FFT spreads it over the
+    output, so there the bound is ~1e-16 of the spectrum's maximum."""
+    if model in (rx.MODEL_RELLINE, rx.MODEL_RELLINELP):
+        return rel_err(y, solo) < 1e-12
+    return abs_err(y, solo) < 1e-13
+
+
 def test_batch_equals_solo_runs(tabs):
     """A request's output in a mixed batch equals its solo evaluation: its work
     items depend only on itself; only the FP64 order of the per-chunk sums may
@@ -107,8 +117,31 @@ def test_batch_equals_solo_runs(tabs):
             y, st = b.output(i)
             assert st == 0
             solo, st = rx.eval_model(r["model"], r["params"], r["energy"], flux_in=r.get("flux_in"), precision=prec)
-            assert rel_err(y, solo) < 1e-12, (prec, i, rel_err(y, solo))
+            assert _same_up_to_sum_order(y, solo, r["model"]), (prec, i, rel_err(y, solo), abs_err(y, solo))
     b.close()
+
+
+def test_work_item_shape_does_not_change_results(tabs):
+    """Bins are cut into work items of S bins; a Romberg bin reuses the
+    integrand pair its predecessor left at the shared edge, except at an
+    item's first bin, which evaluates it afresh.  Both give the same bits, so
+    line-model outputs agree per bin up to the FP64 order of the atomic adds
+    whatever S is."""
+    reqs = [r for r in _mixed_requests(30, 57) if r["model"] in (rx.MODEL_RELLINE, rx.MODEL_RELLINELP)]
+    b = rx.Batch(reqs)
+    try:
+        for prec in (rx.FP64, rx.FP32):
+            outs = []
+            for seg, wpc in ((32, 8), (256, 4), (128, 16)):
+                b.set_shape(seg, wpc)
+                b.run(prec)
+                b.sync()
+                outs.append([b.output(i)[0].copy() for i in range(len(reqs))])
+            worst = max(rel_err(o[i], outs[0][i]) for o in outs[1:] for i in range(len(reqs)))
+            print(f"work item shapes prec {prec}: worst per-bin difference {worst:.3e}", flush=True)
+            assert worst < 1e-12
+    finally:
+        b.close()
 
 
 def test_server_isolation_and_latency(tabs):
@@ -126,7 +159,7 @@ def test_server_isolation_and_latency(tabs):
             y, status, lat = srv.poll(t)
             assert status == 0 and lat > 0
             solo, _ = rx.eval_model(r["model"], r["params"], r["energy"], flux_in=r.get("flux_in"), precision=rx.FP32)
-            assert rel_err(y, solo) < 1e-12
+            assert _same_up_to_sum_order(y, solo, r["model"]), (rel_err(y, solo), abs_err(y, solo))
     finally:
         srv.stop()
 
