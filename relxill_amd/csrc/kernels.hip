@@ -617,6 +617,24 @@ __device__ inline double wave_sum(double v) {
 
 // The transfer function (and cosines) of one fine radius, lerped from its two
 // table-node rows into a lane-private LDS row, and the FP64 edge norms.
+// One row of a fine radius (2 n_g values), lerped from its two node rows into
+// LDS; 8 values per round, so that a round's 16 loads are in flight together.
+template <class TV>
+__device__ inline void lerp_row(double ifac, const double *lo, const double *hi, int nq, TV *dst) {
+  for (int q0 = 0; q0 < nq; q0 += 8) {
+    double a[8], c[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int q = min(q0 + k, nq - 1);
+      a[k] = lo[q];
+      c[k] = hi[q];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (q0 + k < nq) dst[q0 + k] = (TV)lerp1_(ifac, a[k], c[k]);
+  }
+}
+
 template <class TV>
 __device__ inline void load_radius(const BatchDev &b, int req, const RadRec &r, int limb, TV *myT, TV *myC,
                                    RadiusView &v) {
@@ -627,9 +645,8 @@ __device__ inline void load_radius(const BatchDev &b, int req, const RadRec &r, 
   const double *thi = node + (size_t)r.K * nq;
   const double *clo = tlo + nq2;
   const double *chi = thi + nq2;
-  for (int q = 0; q < nq; q++) myT[q] = (TV)node_lerp(r.ifac_r, tlo, thi, q);
-  if (limb)
-    for (int q = 0; q < nq; q++) myC[q] = (TV)node_lerp(r.ifac_r, clo, chi, q);
+  lerp_row<TV>(r.ifac_r, tlo, thi, nq, myT);
+  if (limb) lerp_row<TV>(r.ifac_r, clo, chi, nq, myC);
   v.gmin = r.gmin;
   v.gmax = r.gmax;
   v.del_g = r.del_g;

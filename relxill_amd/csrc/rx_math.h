@@ -170,11 +170,17 @@ RX_HD inline double edge_norm_node(bool upper, const RadiusView &v, double ifac,
   const double e = (hex * (v.gmax - v.gmin) + v.gmin) * 1.0;
   const double egs = (e - v.gmin) * v.del_g;
   const int ind = g_bracket(egs, v.ng, v.gs_step, v.inv_step);
-  double t4[4], c4[4];
-  for (int k = 0; k < 4; k++) {
-    t4[k] = node_lerp(ifac, t_lo, t_hi, ind * 2 + k);
-    c4[k] = v.limb ? node_lerp(ifac, c_lo, c_hi, ind * 2 + k) : 0.0;
+  double t4[4], c4[4], tl[4], th[4];
+  for (int k = 0; k < 4; k++) {  // the 8 loads first, so that they are in flight together
+    tl[k] = t_lo[ind * 2 + k];
+    th[k] = t_hi[ind * 2 + k];
   }
+  for (int k = 0; k < 4; k++) {
+    t4[k] = lerp1_(ifac, tl[k], th[k]);
+    c4[k] = 0.0;
+  }
+  if (v.limb)
+    for (int k = 0; k < 4; k++) c4[k] = node_lerp(ifac, c_lo, c_hi, ind * 2 + k);
   double f0, f1;
   eval64_at(e, egs, ind, v, t4, c4, f0, f1);
   double norm = 0.0;
@@ -208,6 +214,34 @@ struct EdgePair {
   T f0, f1;
   bool ok;
 };
+
+RX_HD inline float rx_rcpf(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+
+// int_edge for the FP32 path: the bounds and their difference stay FP64 (no
+// cancellation: sqrt(hi) - sqrt(lo) = (hi - lo) / (sqrt(hi) + sqrt(lo))), the
+// square roots and the quotient are FP32.
+RX_HD inline double int_edge_pre32(double blo, double bhi, const RadiusView &v) {
+  double lo, hi, norm;
+  if (blo <= 0.5) {
+    lo = blo;
+    hi = bhi;
+    norm = v.normL;
+  } else {
+    lo = 1.0 - bhi;
+    hi = 1.0 - blo;
+    norm = v.normU;
+  }
+  const float d = (float)(hi - lo);
+  const float sq = sqrtf((float)hi) + sqrtf((float)lo);
+  const float q = (sq > 0.0f) ? d * rx_rcpf(sq) : 0.0f;
+  return 2 * norm * (double)q * (v.gmax - v.gmin);
+}
 
 // Romberg for both branches on [a,b], FP64.  Returns t_0[niter0][0] + t_1[niter1][0].
 // R[ii] holds the current anti-diagonal t[ii][n-ii] of the reference's table
@@ -308,9 +342,11 @@ RX_HD inline BinF make_binf(double rlo, double rhi, double ga, double gb) {
   return b;
 }
 
-// Both branches at s (exact dyadic in [0,1]); coordinates from the nearer end,
-// so s = 1 gives exactly the values the next bin sees at its s = 0.
-RX_HD inline void eval32s(float s, const BinF &bf, const RadiusView &v, float &f0, float &f1) {
+// Bracket, interpolation factor and E^3 / sqrt(g* (1 - g*)) at s (exact dyadic
+// in [0,1]); coordinates from the nearer end, so s = 1 gives exactly the
+// values the next bin sees at its s = 0.
+RX_HD inline void eval32_point(float s, const BinF &bf, const RadiusView &v, int &ind, float &inte,
+                               float &base) {
   float t, u, e;
   if (s <= 0.5f) {
     t = fmaf(s, bf.dg, bf.ga);
@@ -326,8 +362,8 @@ RX_HD inline void eval32s(float s, const BinF &bf, const RadiusView &v, float &f
   const float q = (t - (float)GFAC_H) * inv_step;
   const float qi = rintf(q);
   const float thr = 1e-6f * (inv_step + 1.0f);
-  int ind;
-  float inte;
+  ind = min(max((int)floorf(q), 0), v.ng - 2);
+  inte = q - (float)ind;
   if (fabsf(q - qi) < thr && qi >= 1.0f && qi <= (float)(v.ng - 2)) {
     // close to a g* node: decide the bracket with the FP64 coordinates
     const double eg = bf.rlo + (bf.rhi - bf.rlo) * (double)s;
@@ -337,22 +373,44 @@ RX_HD inline void eval32s(float s, const BinF &bf, const RadiusView &v, float &f
     t = (float)egs;
     u = (float)(1.0 - egs);
     e = (float)eg;
-  } else {
-    ind = (int)floorf(q);
-    ind = ind < 0 ? 0 : (ind > v.ng - 2 ? v.ng - 2 : ind);
-    inte = q - (float)ind;
   }
-  const float base = e * e * e * rx_rsqrtf(t * u);
-  const float a0 = v.T32[ind * 2 + 0], a1 = v.T32[ind * 2 + 1];
-  const float b0 = v.T32[ind * 2 + 2], b1 = v.T32[ind * 2 + 3];
-  f0 = base * fmaf(inte, a0 - b0, b0);
-  f1 = base * fmaf(inte, a1 - b1, b1);
+  base = e * e * e * rx_rsqrtf(t * u);
+}
+
+// Both branches at K points: all brackets first, then the row reads and the
+// interpolation in one straight run (the LDS reads of the K points overlap).
+template <int K>
+RX_HD inline void eval32_batch(const float (&s)[K], const BinF &bf, const RadiusView &v, float (&f0)[K],
+                               float (&f1)[K]) {
+  int ind[K];
+  float inte[K], base[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) eval32_point(s[k], bf, v, ind[k], inte[k], base[k]);
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const float a0 = v.T32[ind[k] * 2 + 0], a1 = v.T32[ind[k] * 2 + 1];
+    const float b0 = v.T32[ind[k] * 2 + 2], b1 = v.T32[ind[k] * 2 + 3];
+    f0[k] = base[k] * fmaf(inte[k], a0 - b0, b0);
+    f1[k] = base[k] * fmaf(inte[k], a1 - b1, b1);
+  }
   if (v.limb) {
-    const float c0 = v.C32[ind * 2 + 0], c1 = v.C32[ind * 2 + 1];
-    const float d0 = v.C32[ind * 2 + 2], d1 = v.C32[ind * 2 + 3];
-    f0 *= limb_factor32(v.limb, fmaf(inte, c0 - d0, d0));
-    f1 *= limb_factor32(v.limb, fmaf(inte, c1 - d1, d1));
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const float c0 = v.C32[ind[k] * 2 + 0], c1 = v.C32[ind[k] * 2 + 1];
+      const float d0 = v.C32[ind[k] * 2 + 2], d1 = v.C32[ind[k] * 2 + 3];
+      f0[k] *= limb_factor32(v.limb, fmaf(inte[k], c0 - d0, d0));
+      f1[k] *= limb_factor32(v.limb, fmaf(inte[k], c1 - d1, d1));
+    }
   }
+}
+
+// Both branches at one point.
+RX_HD inline void eval32s(float s, const BinF &bf, const RadiusView &v, float &f0, float &f1) {
+  const float sk[1] = {s};
+  float g0[1], g1[1];
+  eval32_batch<1>(sk, bf, v, g0, g1);
+  f0 = g0[0];
+  f1 = g1[0];
 }
 
 // Stop test of the reference, ob = |R - top| / R > prec (continue), written
@@ -378,8 +436,15 @@ RX_HD inline float romberg32_pair(const BinF &bf, const RadiusView &v, bool *nee
   float fm0, fm1;
   // the points of levels 0 and 1 are always needed: evaluate them together
   if (!have_a) eval32s(0.0f, bf, v, fa0, fa1);
-  eval32s(1.0f, bf, v, fb0, fb1);
-  eval32s(0.5f, bf, v, fm0, fm1);
+  {
+    const float sk[2] = {1.0f, 0.5f};
+    float g0[2], g1[2];
+    eval32_batch<2>(sk, bf, v, g0, g1);
+    fb0 = g0[0];
+    fb1 = g1[0];
+    fm0 = g0[1];
+    fm1 = g1[1];
+  }
   const float ta0 = (fa0 + fb0) * 0.5f, ta1 = (fa1 + fb1) * 0.5f;
   float pas = bf.de;
   float s0 = ta0, s1 = ta1;
@@ -400,14 +465,24 @@ RX_HD inline float romberg32_pair(const BinF &bf, const RadiusView &v, bool *nee
     if (n == 1) {
       add0 += fm0;
       add1 += fm1;
+    } else if (n == 2) {
+      const float sk[2] = {0.25f, 0.75f};
+      float g0[2], g1[2];
+      eval32_batch<2>(sk, bf, v, g0, g1);
+      add0 = (add0 + g0[0]) + g0[1];
+      add1 = (add1 + g1[0]) + g1[1];
     } else {
-      float sm = ds;  // (2m+1) ds, exact: dyadic with at most 7 bits
-      for (int m = 0; m < nnew; m++) {
-        float g0v, g1v;
-        eval32s(sm, bf, v, g0v, g1v);
-        add0 += g0v;
-        add1 += g1v;
-        sm += 2.0f * ds;
+      // (2m+1) ds, exact: dyadic with at most 7 bits; 4 points per batch
+      for (int m0 = 0; m0 < nnew; m0 += 4) {
+        float sk[4], g0[4], g1[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) sk[k] = (float)(2 * (m0 + k) + 1) * ds;
+        eval32_batch<4>(sk, bf, v, g0, g1);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          add0 += g0[k];
+          add1 += g1[k];
+        }
       }
     }
     s0 += add0;
@@ -457,10 +532,9 @@ RX_HD inline double integ_bin_g(double rlo0, double rhi0, double glo_u, double g
 #pragma clang fp contract(off)
   const bool have_a = cy && cy->ok;
   if (cy) cy->ok = false;
-  double gblo = glo_u;
-  if (gblo < 0.0) gblo = 0.0; else if (gblo > 1.0) gblo = 1.0;
-  double gbhi = ghi_u;
-  if (gbhi < 0.0) gbhi = 0.0; else if (gbhi > 1.0) gbhi = 1.0;
+  // clamp to [0, 1] (the ends are finite)
+  const double gblo = fmin(fmax(glo_u, 0.0), 1.0);
+  const double gbhi = fmin(fmax(ghi_u, 0.0), 1.0);
   if (gbhi == 0) return 0.0;
   double rlo = rlo0, rhi = rhi0;
   double ga = glo_u, gb = ghi_u;
@@ -474,7 +548,7 @@ RX_HD inline double integ_bin_g(double rlo0, double rhi0, double glo_u, double g
       hhi = gbhi;
       rlo = -1.0;
     }
-    flu = flu + int_edge_pre(hlo, hhi, v);
+    flu = flu + ((MODE == 0) ? int_edge_pre(hlo, hhi, v) : int_edge_pre32(hlo, hhi, v));
   }
   if (gbhi >= (1.0 - GFAC_H)) {
     double hhi = gbhi, hlo = 1.0 - GFAC_H;
@@ -484,7 +558,7 @@ RX_HD inline double integ_bin_g(double rlo0, double rhi0, double glo_u, double g
       hlo = gblo;
       rhi = -1.0;
     }
-    flu = flu + int_edge_pre(hlo, hhi, v);
+    flu = flu + ((MODE == 0) ? int_edge_pre(hlo, hhi, v) : int_edge_pre32(hlo, hhi, v));
   }
   if ((rhi >= 0) && (rlo >= 0)) {
     if (rlo >= 1.0 * RED_WING) {
