@@ -338,7 +338,9 @@ def main():
     achieved = alg_bytes / t_int / 1e9
     # measured HBM bytes per launch of the same kernel and work shape, from the
     # committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated on a copy)
-    traffic, traffic_src = None, None
+    # the kernel is bound by VALU issue, not HBM: the same summary's VALU-busy
+    # fraction (SQ_ACTIVE_INST_VALU over GPU-busy cycles) is reported beside it
+    traffic, traffic_src, valu_busy = None, None, None
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic_relconv_batch4096.json")
     if os.path.exists(tpath):
         tj = json.load(open(tpath))
@@ -346,8 +348,10 @@ def main():
         if ki and tj.get("shape") == f"{seg},{wpc}" and len(reqs) == 4096:
             traffic = round(ki["read_bytes"] + ki["write_bytes"])
             traffic_src = tj.get("source")
+            valu_busy = ki.get("valu_busy")
     roofline = dict(bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=achieved / HBM_PEAK_GBS, traffic=traffic, traffic_source=traffic_src,
+                    valu_busy=valu_busy,
                     algorithmic_bytes=round(alg_bytes),
                     kernel="k_integrate<1>", kernel_ms=round(kms32[1], 4),
                     kernel_ms_all={k: round(v, 4) for k, v in zip(KERNELS, kms32)})
