@@ -143,6 +143,22 @@ __device__ __forceinline__ int count_le(E e0, int n0, double x, const GridGuess 
   return lo;
 }
 
+// Guessed bracket of x (the first guess of count_le), clamped to [0, n0 - 1].
+__device__ __forceinline__ int guess_index(double x, const GridGuess &g, int n0) {
+  const int k = (int)((__log2f((float)x) - g.lg0) * g.scale);
+  return min(max(k, 0), n0 - 1);
+}
+
+// count_le when the edges a = e0(k), b = e0(k + 1) around the guess k are
+// already loaded: the exact search only when the guess misses.
+template <class E>
+__device__ __forceinline__ int count_le_at(E e0, int n0, double x, const GridGuess &g, int k, double a, double b) {
+  if (!(x >= g.first)) return 0;
+  if (x >= g.last) return n0 + 1;
+  if (a <= x && x < b) return k + 1;
+  return count_le(e0, n0, x, g);
+}
+
 template <int CTRL>
 __device__ inline int dpp_int(int v) {
   return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
@@ -1167,16 +1183,80 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
   double sx = 0.0;
   const int nb = n / nwave;  // bins per wavefront (a multiple of 64)
   if (!angular) {
+    // _rebin_spectrum of the input, 4 conv bins per lane at a time: the edges
+    // around the guessed brackets of all 4 are loaded together, then their
+    // end fluxes; the exact search runs only where a guess misses
     const GridGuess gi = grid_guess(ein, n_in);
-#pragma unroll 2
-    for (int q = 0; q < nb; q += 64) {
-      const int i = wave * nb + q + lane;
-      const double elo = Ec[i], ehi = Ec[i + 1];
-      const int clo = count_le(ge_in, n_in, elo, gi);
-      const int chi = count_le(ge_in, n_in, ehi, gi);
-      const double xr = rebin_bin(elo, ehi, clo, chi, ge_in, n_in, [&](int k) { return fin[k]; });
-      if (elo >= 0.01 && ehi < 1000.0) sx += xr;  // calcFFTNormFactor band (src/Relbase.cpp:205)
-      s_buf[P(i)].x = xr * cf[i];
+    for (int q0 = 0; q0 < nb; q0 += 4 * 64) {
+      double elo[4], ehi[4], al[4], bl[4], ah[4], bh[4];
+      int kl[4], kh[4], i4[4];
+      bool use[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const bool valid = q0 + 64 * u < nb;
+        i4[u] = wave * nb + (valid ? q0 + 64 * u : 0) + lane;
+        elo[u] = Ec[i4[u]];
+        ehi[u] = Ec[i4[u] + 1];
+        // overlap with the input grid (the first test of _rebin_spectrum)
+        use[u] = valid && gi.first <= ehi[u] && gi.last >= elo[u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        kl[u] = guess_index(elo[u], gi, n_in);
+        kh[u] = guess_index(ehi[u], gi, n_in);
+        if (use[u]) {
+          al[u] = ein[kl[u]];
+          bl[u] = ein[kl[u] + 1];
+          ah[u] = ein[kh[u]];
+          bh[u] = ein[kh[u] + 1];
+        }
+      }
+      int imin[4], imax[4];
+      double fa[4], fb[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (!use[u]) continue;
+        const int clo = count_le_at(ge_in, n_in, elo[u], gi, kl[u], al[u], bl[u]);
+        const int chi = count_le_at(ge_in, n_in, ehi[u], gi, kh[u], ah[u], bh[u]);
+        imin[u] = clo > 0 ? clo - 1 : 0;
+        imax[u] = chi < n_in ? chi : n_in;
+        if (imax[u] > 0) imax[u]--;
+        if (imin[u] >= n_in) {
+          use[u] = false;
+          continue;
+        }
+        if (imin[u] != kl[u]) {  // a missed guess or a clipped end: the edges of imin
+          al[u] = ein[imin[u]];
+          bl[u] = ein[imin[u] + 1];
+        }
+        if (imax[u] != kh[u]) {
+          ah[u] = ein[imax[u]];
+          bh[u] = ein[imax[u] + 1];
+        }
+        fa[u] = fin[imin[u]];
+        fb[u] = fin[imax[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (q0 + 64 * u >= nb) continue;
+        double xr = 0.0;
+        if (use[u]) {  // rebin_bin with the edges and end fluxes in registers
+          const double el = (elo[u] < al[u]) ? al[u] : elo[u];
+          const double eh = (ehi[u] > bh[u]) ? bh[u] : ehi[u];
+          const double ra = rcp64(bl[u] - al[u]);
+          if (imax[u] == imin[u]) {
+            xr = (eh - el) * ra * fa[u];
+          } else {
+            const double dmin = (bl[u] - el) * ra;
+            const double dmax = (eh - ah[u]) * rcp64(bh[u] - ah[u]);
+            xr += fa[u] * dmin + fb[u] * dmax;
+            for (int jj = imin[u] + 1; jj <= imax[u] - 1; jj++) xr += fin[jj];
+          }
+        }
+        const int i = i4[u];
+        if (elo[u] >= 0.01 && ehi[u] < 1000.0) sx += xr;  // calcFFTNormFactor band (src/Relbase.cpp:205)
+        s_buf[P(i)].x = xr * cf[i];
+      }
     }
   } else {
     for (int i = tid; i < n; i += NT) s_buf[P(i)].x = 0.0;
