@@ -92,10 +92,12 @@ int rx_model_params(int model, const double *parameter, double *out12);
 /* ===================================================================== */
 
 /* Selects the device and uploads the tables (lazily done by the first call
- * otherwise).  Tables: RELXILL_TABLE_PATH/rel_table.rxtb and rel_lp_table.rxtb
- * when present (own binary format), else the synthetic tables of the given
- * seeds at the reference dims used by the benchmarks (25x22x100x20 relline,
- * 20x100x100 lamp post). */
+ * otherwise).  Tables: in RELXILL_TABLE_PATH the reference's FITS files
+ * rel_table_v0.5a.fits and rel_lp_table_v0.5b.fits (src/common.h:73,80) when
+ * present and cfitsio can be loaded, else rel_table.rxtb and
+ * rel_lp_table.rxtb (own binary format), else the synthetic tables of the
+ * given seeds at the reference dims used by the benchmarks (25x22x100x20
+ * relline, 20x100x100 lamp post). */
 int rx_init(int device);
 void rx_shutdown(void);
 
@@ -108,6 +110,17 @@ int rx_tables_load(const char *path);
 int rx_lptable_load(const char *path);
 int rx_tables_save(const char *path);
 int rx_lptable_save(const char *path);
+/* Load the reference's FITS tables: the relline table in the layout read by
+ * read_relline_table (src/reltable.c:121-304: extensions "a", "mu0", node
+ * (i, j) at HDU i*n_mu0 + j + 4 with columns r, gmin, gmax, trff1, trff2,
+ * cosne1, cosne2) and the lamp-post table in the layout read by read_lp_table
+ * (src/reltable.c:313-448: extension "I_h", one row per spin with a, hgrid,
+ * r, h<k>, del<k>, del_inc<k>; angles stored as absolute values, :370-377).
+ * Dimensions come from the file.  cfitsio is loaded at run time from
+ * RELXILL_CFITSIO_LIB, $HEADAS/lib, the loader path or $CONDA_PREFIX/lib;
+ * without it, or on a malformed file, RX_ERR_TABLE and rx_last_error(). */
+int rx_tables_load_fits(const char *path);
+int rx_lptable_load_fits(const char *path);
 
 /* Read-only host view of the resident tables (for checking). */
 typedef struct {

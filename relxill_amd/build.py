@@ -21,7 +21,7 @@ CFLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall",
           "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable",
           "-I", os.path.join(ROOT, "include"), "-I", CSRC] + NO_PK
 
-SOURCES = ["tables.cpp", "engine.cpp", "capi.cpp", "server.cpp", "kernels.hip"]
+SOURCES = ["tables.cpp", "fits.cpp", "engine.cpp", "capi.cpp", "server.cpp", "kernels.hip"]
 
 
 def _compile(src):
@@ -42,7 +42,7 @@ def build_library(verbose=False):
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-o", tmp] + objs + [f"--offload-arch={ARCH}", "-lpthread"]
+    cmd = [HIPCC, "-shared", "-o", tmp] + objs + [f"--offload-arch={ARCH}", "-lpthread", "-ldl"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -91,6 +91,8 @@ _EXPORTS = {
     "rx_tables_load": (ctypes.c_int, [ctypes.c_char_p]),
     "rx_lptable_load": (ctypes.c_int, [ctypes.c_char_p]),
     "rx_tables_save": (ctypes.c_int, [ctypes.c_char_p]),
+    "rx_tables_load_fits": (ctypes.c_int, [ctypes.c_char_p]),
+    "rx_lptable_load_fits": (ctypes.c_int, [ctypes.c_char_p]),
     "rx_lptable_save": (ctypes.c_int, [ctypes.c_char_p]),
     "rx_tables_view": (ctypes.c_int, [ctypes.POINTER(_RelView), ctypes.POINTER(_LpView)]),
     "rx_set_conv_grid": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double]),

@@ -205,6 +205,40 @@ int rx_lptable_load(const char *path) {
   return RX_OK;
 }
 
+int rx_tables_load_fits(const char *path) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!path) return RX_ERR_ARG;
+  RelTableHost t;
+  std::string e = load_reltable_fits(t, path);
+  if (!e.empty()) {
+    set_last(RX_ERR_TABLE, e);
+    return RX_ERR_TABLE;
+  }
+  g.rel = std::move(t);
+  g.rel_loaded = true;
+  g.rel_dirty = true;
+  g.config_gen++;
+  return RX_OK;
+}
+
+int rx_lptable_load_fits(const char *path) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!path) return RX_ERR_ARG;
+  LpTableHost t;
+  std::string e = load_lptable_fits(t, path);
+  if (!e.empty()) {
+    set_last(RX_ERR_TABLE, e);
+    return RX_ERR_TABLE;
+  }
+  g.lp = std::move(t);
+  g.lp_loaded = true;
+  g.lp_dirty = true;
+  g.config_gen++;
+  return RX_OK;
+}
+
 int rx_tables_save(const char *path) {
   Global &g = G();
   std::lock_guard<std::recursive_mutex> lk(g.mu);

@@ -162,8 +162,9 @@ void ensure_tables_host() {
   if (!g.rel_loaded) {
     bool ok = false;
     if (const char *path = getenv("RELXILL_TABLE_PATH")) {
-      std::string e = load_reltable(g.rel, std::string(path) + "/rel_table.rxtb");
-      ok = e.empty();
+      // the reference's table file and name first (src/common.h:73), then our own format
+      ok = load_reltable_fits(g.rel, std::string(path) + "/rel_table_v0.5a.fits").empty() ||
+           load_reltable(g.rel, std::string(path) + "/rel_table.rxtb").empty();
     }
     if (!ok) g.rel = make_synthetic_reltable(1, 25, 22, 100, 20);
     g.rel_loaded = true;
@@ -176,8 +177,8 @@ void ensure_tables_host() {
   if (!g.lp_loaded) {
     bool ok = false;
     if (const char *path = getenv("RELXILL_TABLE_PATH")) {
-      std::string e = load_lptable(g.lp, std::string(path) + "/rel_lp_table.rxtb");
-      ok = e.empty();
+      ok = load_lptable_fits(g.lp, std::string(path) + "/rel_lp_table_v0.5b.fits").empty() ||
+           load_lptable(g.lp, std::string(path) + "/rel_lp_table.rxtb").empty();
     }
     if (!ok) g.lp = make_synthetic_lptable(3, 20, 100, 100);
     g.lp_loaded = true;

@@ -42,6 +42,11 @@ std::string load_reltable(RelTableHost &t, const std::string &path);
 std::string save_lptable(const LpTableHost &t, const std::string &path);
 std::string load_lptable(LpTableHost &t, const std::string &path);
 
+// The reference's FITS tables (rel_table_v0.5a.fits / rel_lp_table_v0.5b.fits
+// layout, dimensions from the file); cfitsio is loaded at run time (fits.cpp).
+std::string load_reltable_fits(RelTableHost &t, const std::string &path);
+std::string load_lptable_fits(LpTableHost &t, const std::string &path);
+
 // Checks the grid invariants; returns an empty string when they hold.
 std::string validate_reltable(const RelTableHost &t);
 std::string validate_lptable(const LpTableHost &t);

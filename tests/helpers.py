@@ -285,3 +285,81 @@ def write_lptb(path, lp):
         for k in ("a", "h", "rad", "intens", "del_", "del_inc"):
             f.write(np.ascontiguousarray(lp[k], dtype=np.float32).tobytes())
 
+
+
+# ---- FITS writer (binary tables, big-endian float32 columns) ----------------
+# An independent writer of the layout the reference's loaders read
+# (src/reltable.c:121-304 relline, :313-448 lamp post), so that the library's
+# FITS loaders are checked against a file they did not write.
+
+def _fits_card(key, value=None):
+    if value is None:
+        s = f"{key:<8}"
+    elif isinstance(value, bool):
+        s = f"{key:<8}= {'T' if value else 'F':>20}"
+    elif isinstance(value, int):
+        s = f"{key:<8}= {value:>20}"
+    else:
+        s = f"{key:<8}= '{str(value):<8}'"
+    return s.ljust(80)[:80]
+
+
+def _fits_header(cards):
+    h = "".join(cards) + "END".ljust(80)
+    return (h + " " * ((-len(h)) % 2880)).encode("ascii")
+
+
+def _fits_bintable(name, cols):
+    """cols: list of (column name, float array of shape (rows, repeat))."""
+    cols = [(c, np.asarray(v, dtype=">f4").reshape(len(v), -1)) for c, v in cols]
+    nrows = cols[0][1].shape[0]
+    assert all(v.shape[0] == nrows for _, v in cols)
+    rowbytes = sum(4 * v.shape[1] for _, v in cols)
+    cards = [_fits_card("XTENSION", "BINTABLE"), _fits_card("BITPIX", 8), _fits_card("NAXIS", 2),
+             _fits_card("NAXIS1", rowbytes), _fits_card("NAXIS2", nrows), _fits_card("PCOUNT", 0),
+             _fits_card("GCOUNT", 1), _fits_card("TFIELDS", len(cols))]
+    for i, (c, v) in enumerate(cols):
+        cards += [_fits_card(f"TTYPE{i + 1}", c), _fits_card(f"TFORM{i + 1}", f"{v.shape[1]}E")]
+    cards.append(_fits_card("EXTNAME", name))
+    data = np.concatenate([v for _, v in cols], axis=1).astype(">f4").tobytes()
+    return _fits_header(cards) + data + b"\0" * ((-len(data)) % 2880)
+
+
+def _fits_primary():
+    return _fits_header([_fits_card("SIMPLE", True), _fits_card("BITPIX", 8), _fits_card("NAXIS", 0),
+                         _fits_card("EXTEND", True)])
+
+
+def write_fits_reltable(path, rel):
+    """The relline table (tables_view layout) as rel_table_v0.5a.fits lays it
+    out: extensions "a" and "mu0", then node (i, j) as HDU i*n_mu0 + j + 4 named
+    "<i+1>_<j+1>" with rows r, gmin, gmax and vectors trff1, trff2, cosne1, cosne2."""
+    na, nm, nr, ng = rel["n_a"], rel["n_mu0"], rel["n_r"], rel["n_g"]
+    out = [_fits_primary(), _fits_bintable("a", [("a", rel["a"])]), _fits_bintable("mu0", [("mu0", rel["mu0"])])]
+    r = rel["r"].reshape(na, nm, nr)
+    gl, gh = rel["gmin"].reshape(na, nm, nr), rel["gmax"].reshape(na, nm, nr)
+    t1, t2 = rel["trff1"].reshape(na, nm, nr, ng), rel["trff2"].reshape(na, nm, nr, ng)
+    c1, c2 = rel["cosne1"].reshape(na, nm, nr, ng), rel["cosne2"].reshape(na, nm, nr, ng)
+    for i in range(na):
+        for j in range(nm):
+            out.append(_fits_bintable(f"{i + 1}_{j + 1}", [("r", r[i, j]), ("gmin", gl[i, j]), ("gmax", gh[i, j]),
+                                                            ("trff1", t1[i, j]), ("trff2", t2[i, j]),
+                                                            ("cosne1", c1[i, j]), ("cosne2", c2[i, j])]))
+    with open(path, "wb") as f:
+        for blk in out:
+            f.write(blk)
+
+
+def write_fits_lptable(path, lp):
+    """The lamp-post table as rel_lp_table_v0.5b.fits lays it out: extension
+    "I_h", one row per spin with a, hgrid, r, and h<k>, del<k>, del_inc<k> for
+    every height k."""
+    na, nh, nr = lp["n_a"], lp["n_h"], lp["n_r"]
+    inten = lp["intens"].reshape(na, nh, nr)
+    de, di = lp["del_"].reshape(na, nh, nr), lp["del_inc"].reshape(na, nh, nr)
+    cols = [("a", lp["a"]), ("hgrid", lp["h"].reshape(na, nh)), ("r", lp["rad"].reshape(na, nr))]
+    for k in range(nh):
+        cols += [(f"h{k + 1}", inten[:, k]), (f"del{k + 1}", de[:, k]), (f"del_inc{k + 1}", di[:, k])]
+    with open(path, "wb") as f:
+        f.write(_fits_primary())
+        f.write(_fits_bintable("I_h", cols))

@@ -169,3 +169,35 @@ def test_relconv_other_conv_grids(tabs, n_conv):
             assert st == 0 and rel_err(y, ref) < tol
     finally:
         rx.set_conv_grid(4096, 0.00035, 2000.0)
+
+
+def test_reference_table_dims(tabs):
+    """The released tables' dimensions (src/common.h:69-81): relline 25 spins x
+    30 mu0 x 100 radii x 40 g*, lamp post 20 spins x 250 heights x 100 radii
+    (40 g* doubles the lane rows in LDS).  relline, relconv and relline_lp
+    against the oracle on synthetic tables of those dims, both precisions."""
+    rx.tables_synthetic(11, 25, 30, 100, 40)
+    rx.lptable_synthetic(13, 20, 250, 100)
+    try:
+        rel, lp = rx.tables_view()
+        T = oracle.Tables({k: (np.array(v) if hasattr(v, "shape") else v) for k, v in rel.items()},
+                          {k: (np.array(v) if hasattr(v, "shape") else v) for k, v in lp.items()})
+        e = log_grid(3000, 0.1, 10.0)
+        e_ref, refl = reflection_spectrum(n_incl=1)
+        e_c = log_grid(2000, 0.1, 100.0)
+        fin = oracle.rebin(e_c, e_ref, refl[0])
+        cases = [(rx.MODEL_RELLINE, oracle.MOD_RELLINE, RELLINE_CFG0, e, None),
+                 (rx.MODEL_RELLINE, oracle.MOD_RELLINE, [6.4, 3.0, 3.0, 15.0, 0.3, 75.0, -1.0, 400.0, 0.0, 2.0], e, None),
+                 (rx.MODEL_RELCONV, oracle.MOD_RELCONV, [3.0, 3.0, 15.0, -0.4, 45.0, -1.0, 400.0, 0.0], e_c, fin),
+                 (rx.MODEL_RELLINELP, oracle.MOD_RELLINELP, [6.4, 8.0, 0.9, 30.0, -1.0, 400.0, 0.0, 0.0, 2.0, 0.0],
+                  e, None)]
+        for model, omodel, p, grid, flux_in in cases:
+            ref, st = oracle.eval_model(omodel, T, p, grid, flux_in=flux_in)
+            assert st == 0
+            for prec, tol in ((rx.FP64, TOL64), (rx.FP32, TOL32)):
+                y, st = rx.eval_model(model, p, grid, flux_in=flux_in, precision=prec)
+                _report(f"reference dims model {model} prec {prec}", y, ref)
+                assert st == 0 and rel_err(y, ref) < tol
+    finally:
+        rx.tables_synthetic(1, 25, 22, 100, 20)
+        rx.lptable_synthetic(3, 20, 100, 100)
