@@ -363,3 +363,23 @@ def write_fits_lptable(path, lp):
     with open(path, "wb") as f:
         f.write(_fits_primary())
         f.write(_fits_bintable("I_h", cols))
+
+
+def draw_wide_params(model, rng):
+    """Parameter arrays over the full lmodel.dat ranges, including values the
+    reference rejects (Rout <= Rin) or rescales (negative Rin, Rout, h)."""
+    a = rng.uniform(-0.998, 0.998)
+    incl = rng.uniform(3.0, 87.0)
+    rin = rng.choice([-1.0, -rng.uniform(1.0, 5.0), rng.uniform(1.0, 60.0)])
+    rout = rng.choice([400.0, -rng.uniform(10.0, 300.0), rng.uniform(10.0, 1000.0)])
+    limb = float(rng.integers(0, 3))
+    if model == rx.MODEL_RELLINE:
+        return [rng.uniform(1, 10), rng.uniform(-2, 6), rng.uniform(-2, 6), rng.uniform(1, 100), a, incl,
+                rin, rout, rng.uniform(0, 1), limb]
+    if model in (rx.MODEL_RELCONV, rx.MODEL_RELXILL_CONV):
+        return [rng.uniform(-2, 6), rng.uniform(-2, 6), rng.uniform(1, 100), a, incl, rin, rout, limb]
+    if model == rx.MODEL_RELLINELP:
+        return [rng.uniform(1, 10), rng.uniform(-10, 120), a, incl, rin, rout, rng.uniform(0, 1), limb,
+                rng.uniform(1, 3.4), float(rng.integers(0, 2))]
+    return [rng.uniform(-10, 120), rng.uniform(-0.5, 1.0), a, incl, rin, rout, limb, rng.uniform(1, 3.4),
+            float(rng.integers(0, 2))]

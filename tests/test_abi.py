@@ -13,6 +13,7 @@ import pytest
 import oracle
 import relxill_amd as rx
 import relxill_amd.capi as capi
+from tests.helpers import draw_wide_params
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "relxill_gpu.h")
@@ -50,30 +51,12 @@ def test_python_mirror_binds_only_declared_functions():
     assert declared <= bound, declared - bound
 
 
-def _draw(model, rng):
-    a = rng.uniform(-0.998, 0.998)
-    incl = rng.uniform(3.0, 87.0)
-    rin = rng.choice([-1.0, -rng.uniform(1.0, 5.0), rng.uniform(1.0, 60.0)])
-    rout = rng.choice([400.0, -rng.uniform(10.0, 300.0), rng.uniform(10.0, 1000.0)])
-    limb = float(rng.integers(0, 3))
-    if model == rx.MODEL_RELLINE:
-        return [rng.uniform(1, 10), rng.uniform(-2, 6), rng.uniform(-2, 6), rng.uniform(1, 100), a, incl,
-                rin, rout, rng.uniform(0, 1), limb]
-    if model in (rx.MODEL_RELCONV, rx.MODEL_RELXILL_CONV):
-        return [rng.uniform(-2, 6), rng.uniform(-2, 6), rng.uniform(1, 100), a, incl, rin, rout, limb]
-    if model == rx.MODEL_RELLINELP:
-        return [rng.uniform(1, 10), rng.uniform(-10, 120), a, incl, rin, rout, rng.uniform(0, 1), limb,
-                rng.uniform(1, 3.4), float(rng.integers(0, 2))]
-    return [rng.uniform(-10, 120), rng.uniform(-0.5, 1.0), a, incl, rin, rout, limb, rng.uniform(1, 3.4),
-            float(rng.integers(0, 2))]
-
-
 @pytest.mark.parametrize("model", [1, 2, 11, 12])
 def test_model_params_equal_oracle(model):
     rng = np.random.default_rng(100 + model)
     n_ok = n_rej = 0
     for _ in range(400):
-        par = _draw(model, rng)
+        par = draw_wide_params(model, rng)
         out, st = rx.model_params(model, par)
         rp, st_o = oracle.rel_params(model, par)
         assert st == st_o, (par, st, st_o)

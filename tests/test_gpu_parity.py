@@ -201,3 +201,46 @@ def test_reference_table_dims(tabs):
     finally:
         rx.tables_synthetic(1, 25, 22, 100, 20)
         rx.lptable_synthetic(3, 20, 100, 100)
+
+
+@pytest.mark.parametrize("model", [rx.MODEL_RELLINE, rx.MODEL_RELCONV, rx.MODEL_RELLINELP, rx.MODEL_RELCONVLP])
+def test_random_parameter_sweep(tabs, model):
+    """48 parameter sets over the full lmodel.dat ranges (limb laws, negative
+    radii and heights as multiples of the ISCO / horizon, emissivity indices
+    -2..6, rejected sets), each on its own log grid of 300-3000 bins, in one
+    batch: accepted where the oracle accepts (same status class otherwise), and
+    within the FP64 / FP32 bars where both evaluate."""
+    from tests.helpers import draw_wide_params
+    rng = np.random.default_rng(1000 + model)
+    conv = model in (rx.MODEL_RELCONV, rx.MODEL_RELCONVLP)
+    e_ref, refl = reflection_spectrum(n_incl=1)
+    reqs, jobs = [], []
+    for _ in range(48):
+        p = draw_wide_params(model, rng)
+        n = int(np.exp(rng.uniform(np.log(300), np.log(3000))))
+        e = log_grid(n, 0.1, 100.0 if conv else 10.0)
+        fin = oracle.rebin(e, e_ref, refl[0]) if conv else None
+        reqs.append(dict(model=model, energy=e, params=p, flux_in=fin))
+        jobs.append((model, p, e, fin))
+    refs, sts = oracle.eval_batch_mt(tabs, jobs, 16)
+    cls = lambda s: 10 if 10 <= s < 50 else s
+    b = rx.Batch(reqs)
+    n_eval = 0
+    worst = {rx.FP64: 0.0, rx.FP32: 0.0}
+    try:
+        for prec, tol in ((rx.FP64, TOL64), (rx.FP32, TOL32)):
+            b.run(prec)
+            b.sync()
+            for i, r in enumerate(reqs):
+                y, st = b.output(i)
+                assert cls(st) == cls(sts[i]), (i, r["params"], st, sts[i])
+                if st == 0:
+                    err = rel_err(y, refs[i])
+                    worst[prec] = max(worst[prec], err)
+                    assert err < tol, (prec, i, r["params"], err)
+                    n_eval += prec == rx.FP64
+    finally:
+        b.close()
+    print(f"sweep model {model}: {n_eval} evaluated, worst rel err FP64 {worst[rx.FP64]:.3e} "
+          f"FP32 {worst[rx.FP32]:.3e}", flush=True)
+    assert n_eval >= 20
