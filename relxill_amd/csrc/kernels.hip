@@ -717,7 +717,10 @@ __global__ __launch_bounds__(64, MODE == 0 ? 1 : 4) void k_integrate(BatchDev b,
   TV *s_bins = (MODE == 0) ? reinterpret_cast<TV *>(s_coef + 64) : reinterpret_cast<TV *>(s_coef);
   const int n_items = min(b.ctr[CTR_ITEMS], b.item_cap);
   const int nr = b.nr_fine;
-  const int col = lane >> 3, part = lane & 7;  // turn-round roles: bin col, radii 8 part ..
+  // turn-round roles: SUM_LANES lanes per bin; lane (col, part) sums radii
+  // SUM_BINS part .. SUM_BINS part + SUM_BINS - 1 of bin col
+  constexpr int SUM_LANES = 64 / SUM_BINS;
+  const int col = lane / SUM_LANES, part = lane % SUM_LANES;
   RadiusView v;
   v.ng = ng;
   v.gs_step = (1.0 - 2 * GFAC_H) / (ng - 1);
@@ -799,21 +802,22 @@ __global__ __launch_bounds__(64, MODE == 0 ? 1 : 4) void k_integrate(BatchDev b,
         s_bins[jj * SUM_STRIDE + lane] = val;
       }
       __syncthreads();  // one wavefront: orders the parked values before the turn-round
-      // FP64 sum over the 64 radii of bin jg + col: 8 radii per lane, then the
-      // 8 lanes of the bin in a fixed DPP tree
-      const TV *src = s_bins + col * SUM_STRIDE + part * 8;
+      // FP64 sum over the 64 radii of bin jg + col: SUM_BINS radii per lane,
+      // then the SUM_LANES lanes of the bin in a fixed DPP tree
+      const TV *src = s_bins + col * SUM_STRIDE + part * SUM_BINS;
       double sum = 0.0;
       if constexpr (MODE == 0) {
-        const double *cf = s_coef + part * 8;
+        const double *cf = s_coef + part * SUM_BINS;
 #pragma unroll
-        for (int q = 0; q < 8; q++) sum = fma(src[q], cf[q], sum);
+        for (int q = 0; q < SUM_BINS; q++) sum = fma(src[q], cf[q], sum);
       } else {
 #pragma unroll
-        for (int q = 0; q < 8; q++) sum += (double)src[q];
+        for (int q = 0; q < SUM_BINS; q++) sum += (double)src[q];
       }
       sum += dpp_d<0xB1>(sum);   // quad_perm [1,0,3,2]
       sum += dpp_d<0x4E>(sum);   // quad_perm [2,3,0,1]
-      sum += dpp_d<0x141>(sum);  // row_half_mirror: every lane of the 8 holds the total
+      sum += dpp_d<0x141>(sum);  // row_half_mirror: every lane of a half row holds its sum
+      if constexpr (SUM_LANES == 16) sum += dpp_d<0x140>(sum);  // row_mirror: the whole row
       if (MODE == 1) sum *= rn.cinv;
       if (part == 0 && col < jn) unsafeAtomicAdd(&acc[jg + col], sum);
       __syncthreads();  // the next bins overwrite the parked values

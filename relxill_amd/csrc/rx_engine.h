@@ -123,11 +123,12 @@ constexpr int MAX_NR_FINE = 64 * RAD_CHUNK;
 
 // k_integrate parks the per-radius values of SUM_BINS consecutive bins in LDS
 // ([bin][lane], row stride SUM_STRIDE: lane (c, p) of the turn-round reads
-// values 8 p .. 8 p + 7 of bin c, and the odd stride spreads the 8 bins over
-// the banks) and then sums each bin with 8 lanes (8 radii each, then a 3-step
-// DPP tree).
+// values SUM_BINS p .. SUM_BINS p + SUM_BINS - 1 of bin c, and the odd stride
+// spreads the bins over the banks) and then sums each bin with 64 / SUM_BINS
+// lanes (SUM_BINS radii each, then a DPP tree); SUM_BINS is 8 or 4.
 constexpr int SUM_BINS = 8;
 constexpr int SUM_STRIDE = 65;
+static_assert(SUM_BINS == 8 || SUM_BINS == 4, "turn-round DPP tree covers 8 or 16 lanes");
 
 // One unit of integration work: bins [j0, j1) of one request for the fine
 // radii of one chunk.  k_setup emits the segments of each chunk's energy
