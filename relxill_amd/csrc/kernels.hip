@@ -429,10 +429,8 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     RadRec r;
     r.gmin = gmin;
     r.gmax = gmax;
-    r.del_g = 1. / (gmax - gmin);
     r.coef = emis / (gmax - gmin) * weight;  // times ReqNorm.fac
     r.ifac_r = ifac_r;
-    r.emisw = emis * weight;                 // times ReqNorm.fac
     r.K = K;
     r.ielo = ielo;
     r.iehi = iehi;
@@ -531,7 +529,8 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
       const int imu = ((int)(rq.n_incl * (1 - cosne) + 1)) - 1;
       const double dgs = (g == 0 || g == ng - 1) ? 0.5 * (gs1 - gs0) + GFAC_H : gs1 - gs0;
       const double pr = 2 * PI * gv * re;
-      const double tmp = re * (pr * pr) / sqrt(gs - gs * gs) * trff * (r.emisw * fac) * dgs;
+      const double emisw = r.coef * (r.gmax - r.gmin) * fac;  // emis * weight
+      const double tmp = re * (pr * pr) / sqrt(gs - gs * gs) * trff * emisw * dgs;
       if (imu < 0 || imu >= rq.n_incl) {
         derr = 40;
         continue;
@@ -665,7 +664,7 @@ __device__ inline void load_radius(const BatchDev &b, int req, const RadRec &r, 
   if (limb) lerp_row<TV>(r.ifac_r, clo, chi, nq, myC);
   v.gmin = r.gmin;
   v.gmax = r.gmax;
-  v.del_g = r.del_g;
+  v.del_g = 1. / (r.gmax - r.gmin);  // set_str_relbf, src/Relprofile.cpp:733
   v.limb = limb;
   v.normL = edge_norm_node(false, v, r.ifac_r, tlo, thi, clo, chi);
   v.normU = edge_norm_node(true, v, r.ifac_r, tlo, thi, clo, chi);

@@ -62,7 +62,9 @@ struct ReqParams {
 // One fine radius (64 bytes).  The bin integral at this radius is
 //   flux[j] += integ_bin(E_j, E_j+1) * coef,   coef = emis*weight/(gmax-gmin)
 struct RadRec {
-  double gmin, gmax, del_g, coef, ifac_r, emisw;
+  double gmin, gmax;  // energy window (line-energy units); del_g = 1 / (gmax - gmin)
+  double coef;        // emis / (gmax - gmin) * weight, times ReqNorm.fac
+  double ifac_r;      // lerp factor between the node radii K+1 (lo) and K (hi)
   int K;       // table-node bracket: values lerp between node K+1 (lo) and K (hi)
   int ielo;    // first bin of the window (ielo > iehi: radius does not contribute)
   int iehi;    // last bin of the window (inclusive)
