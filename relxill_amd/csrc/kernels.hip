@@ -1357,8 +1357,9 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
     if (in_band(cg, i)) sc += v;
   }
   const double sum_conv = block_sum4(sc, s_w[0]);
+  // normalizeFFTOutput (src/Relbase.cpp:199-222): applied to the rebinned
+  // values below (the rebin is linear), which saves a pass over the buffer
   const double fnorm = sum_rel * sum_xill / sum_conv;
-  for (int i = tid; i < n; i += NT) o_at(i) *= fnorm;
   // the conv grid edges go to the upper part of the buffer, free after the
   // half-length FFT (its values occupy slots below P(m))
   double *s_e = reinterpret_cast<double *>(s_buf + P(m));
@@ -1377,7 +1378,7 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
       const double elo = ein[jb], ehi = ein[jb + 1];
       const int clo = count_le(ge_c, n, elo, gc);
       const int chi = count_le(ge_c, n, ehi, gc);
-      double v = rebin_bin(elo, ehi, clo, chi, ge_c, n, [&](int i) { return o_at(i); });
+      double v = fnorm * rebin_bin(elo, ehi, clo, chi, ge_c, n, [&](int i) { return o_at(i); });
       if (!angular && (ehi < 0.01 || elo > 1000.0)) v = 0.0;
       out[jb] = v;
     }
