@@ -29,7 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
-KERNELS = ("k_setup", "k_integrate", "k_fixup", "k_finalize", "k_conv")
+KERNELS = ("k_setup", "k_integrate", "k_fixup", "k_finalize", "k_conv", "k_xill")
 
 
 def log(*a):
@@ -102,7 +102,7 @@ def relxill_style(rx, steps):
         a = rng.uniform(-0.998, 0.998)
         incl = rng.uniform(5.0, 85.0)
         rin = rng.uniform(kerr_rms(a), 50.0)
-        reqs.append(dict(model=-1, energy=e_out, params=[3.0, 3.0, 15.0, a, incl, rin, 400.0, 0.0], refl_set=sid))
+        reqs.append(dict(model=rx.MODEL_RELXILL_CONV, energy=e_out, params=[3.0, 3.0, 15.0, a, incl, rin, 400.0, 0.0], refl_set=sid))
     lat = []
     for r in reqs[:8]:
         pb = rx.Batch([r])

@@ -41,7 +41,10 @@ extern "C" {
 #define RX_MODEL_RELLINELP 2
 #define RX_MODEL_RELCONV 11
 #define RX_MODEL_RELCONVLP 12
-#define RX_MODEL_RELXILL_CONV (-1) /* relxill-style: angle-resolved reflection, one zone */
+#define RX_MODEL_RELXILL (-1) /* relxill: broken power law, one zone, xillver table */
+/* project id: relxill-style convolution of a resident angle-resolved
+ * reflection set (rx_reflset_create) with relconv's parameters */
+#define RX_MODEL_RELXILL_CONV (-1001)
 
 /* ---- precision of the batched path ---- */
 #define RX_FP64 0 /* plain form: FP64 throughout */
@@ -72,6 +75,16 @@ void lmodrelconv(const double *energy, int Nflux, const double *parameter, int s
  * (lmodel_relxill_public.dat:35-44); switch_returnrad must be 0 */
 void lmodrelconvlp(const double *energy, int Nflux, const double *parameter, int spectrum,
                    double *flux, double *fluxError, const char *init);
+
+/* relxill: Index1 Index2 Rbr a Incl Rin Rout z gamma logxi Afe Ecut refl_frac
+ * (lmodel_relxill_public.dat:46-59; the reference's c_lmodrelxill,
+ * LocalModel::relxill_model -> relxill_kernel, src/LocalModel.cpp:53-72,
+ * src/Relxill.cpp:289-414): the xillver spectra of the table interpolated in
+ * Gamma, A_Fe, logXi, Ecut for every emission angle, weighted by the angle
+ * distribution, convolved with the relativistic profile, times |refl_frac|,
+ * plus the cut-off power law for refl_frac >= 0. */
+void lmodrelxill(const double *energy, int Nflux, const double *parameter, int spectrum,
+                 double *flux, double *fluxError, const char *init);
 
 /* Status-returning evaluation of one model (the body of xspec_C_wrapper_eval_model,
  * src/LocalModel.cpp:143-160, and LocalModel::eval_model, src/LocalModel.h:108-128).
@@ -121,6 +134,28 @@ int rx_lptable_save(const char *path);
  * without it, or on a malformed file, RX_ERR_TABLE and rx_last_error(). */
 int rx_tables_load_fits(const char *path);
 int rx_lptable_load_fits(const char *path);
+
+/* xillver reflection table (five parameters, the last the inclination;
+ * the reference's xillver-a-Ec5.fits, src/common.h:163).  Loaded from
+ * RELXILL_TABLE_PATH/xillver-a-Ec5.fits on the first relxill request when
+ * present, else a synthetic table (13 Gamma x 5 A_Fe x 8 logXi x 6 Ecut x 10
+ * inclinations, 2999 log bins over 0.07-1000 keV).  The FITS loader reads
+ * the layout of src/xilltable.c:169-277, :513-566 (extensions PARAMETERS with
+ * NAME/NUMBVALS/VALUE in columns 1/9/10, ENERGIES, SPECTRA with the spectrum
+ * in column 2); spectra are divided by 10^logXi when uploaded
+ * (renorm_xill_spec, src/xilltable.c:478-511). */
+int rx_xilltable_synthetic(uint64_t seed, int n_gam, int n_afe, int n_lxi, int n_ect, int n_incl, int n_ener);
+int rx_xilltable_load_fits(const char *path);
+typedef struct {
+  int n_par;
+  int pid[5];   /* parameter ids: 0 Gamma, 1 A_Fe, 2 logXi, 3 Ecut, 4 Dens, 7 Incl */
+  int n_val[5];
+  const float *val[5];
+  int n_ener;
+  const float *elo, *ehi;
+  const float *spec;  /* rows as stored in the file (before the 10^logXi division) */
+} rx_xilltable_view;
+int rx_xilltable_get_view(rx_xilltable_view *v);
 
 /* Read-only host view of the resident tables (for checking). */
 typedef struct {
@@ -188,9 +223,9 @@ int rx_reflset_create(const double *refl_energy, int n_refl, const double *refl,
 int rx_reflsets_clear(void);
 
 /* Kernel timing of the last rx_batch_run with profiling enabled: per kernel
- * (0 k_setup (with k_reset), 1 k_integrate, 2 k_fixup, 3 k_finalize, 4 k_conv)
- * the duration in ms measured with HIP events on the batch's stream. */
-#define RX_N_TIMED 5
+ * (0 k_setup (with k_reset), 1 k_integrate, 2 k_fixup, 3 k_finalize, 4 k_conv,
+ * 5 k_xill) the duration in ms measured with HIP events on the batch's stream. */
+#define RX_N_TIMED 6
 int rx_profile_enable(int on);
 /* Streams `bytes` through a plain device copy kernel `reps` times (after one
  * warm-up) and returns the average kernel time; used to calibrate HBM byte

@@ -20,6 +20,7 @@ MOD_RELLINELP = 2
 MOD_RELCONV = 11
 MOD_RELCONVLP = 12
 MOD_RELXILL = -1
+MOD_RELXILL_CONV = -1001
 
 _D = ctypes.POINTER(ctypes.c_double)
 _F = ctypes.POINTER(ctypes.c_float)
@@ -57,6 +58,17 @@ class SysPar(ctypes.Structure):
                 ("d_gstar", _D), ("trff", _D), ("cosne", _D), ("emis", _D), ("del_emit", _D),
                 ("del_inc", _D), ("limb_law", _I),
                 ("refl", ctypes.c_double * 5)]
+
+
+class XillTable(ctypes.Structure):
+    _fields_ = [("n_par", _I), ("pid", _I * 5), ("n_val", _I * 5), ("val", _F * 5), ("n_ener", _I),
+                ("elo", _F), ("ehi", _F), ("spec", _F)]
+
+
+class XillParam(ctypes.Structure):
+    _fields_ = [("gam", ctypes.c_double), ("afe", ctypes.c_double), ("lxi", ctypes.c_double),
+                ("ect", ctypes.c_double), ("dens", ctypes.c_double), ("incl", ctypes.c_double),
+                ("refl_frac", ctypes.c_double)]
 
 
 class Job(ctypes.Structure):
@@ -102,6 +114,17 @@ def _load():
                                     ctypes.POINTER(Stats)]),
         "orc_relxill_conv": (_I, [ctypes.POINTER(RelTable), ctypes.POINTER(RelParam), _D, _I,
                                   _D, _D, _I, _I, _D, _D, _I, _I, _D, ctypes.POINTER(Stats)]),
+        "orc_xill_indices": (None, [ctypes.POINTER(XillTable), ctypes.POINTER(XillParam), ctypes.POINTER(_I)]),
+        "orc_xill_factors": (None, [ctypes.POINTER(XillTable), ctypes.POINTER(XillParam), ctypes.POINTER(_I), _D]),
+        "orc_xill_renorm": (None, [_F, _I, ctypes.c_double, ctypes.c_double]),
+        "orc_xill_spectra": (None, [ctypes.POINTER(XillTable), ctypes.POINTER(XillParam), _D, _D]),
+        "orc_xill_angdep": (None, [_D, _D, _D, _I, _I]),
+        "orc_spec_cutoffpl": (None, [_D, _D, _I, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+        "orc_norm_wrt_xillver": (ctypes.c_double, [_D, _D, _I, ctypes.POINTER(_I)]),
+        "orc_primary_norm_factor": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
+        "orc_get_relxill_params": (_I, [_D, _I, ctypes.POINTER(RelParam), ctypes.POINTER(XillParam)]),
+        "orc_relxill": (_I, [ctypes.POINTER(RelTable), ctypes.POINTER(XillTable), _D, _D, _D, _I, _D, _I, _I,
+                             _D, ctypes.POINTER(Stats)]),
         "orc_eval_model": (_I, [_I, ctypes.POINTER(RelTable), ctypes.POINTER(LpTable), _D, _D,
                                 _D, _I, _D, _I, _I, ctypes.POINTER(Stats)]),
         "orc_eval_batch_mt": (_I, [ctypes.POINTER(RelTable), ctypes.POINTER(LpTable),
@@ -268,7 +291,7 @@ def rzone_grid(rmin, rmax, nzones, h):
 
 def relxill_conv(tables, params, conv_grid, refl_ener, refl, out_ener, nr_fine=1000,
                  phys_norm=False):
-    p, st = rel_params(MOD_RELXILL, params, phys_norm)
+    p, st = rel_params(MOD_RELXILL_CONV, params, phys_norm)
     if st:
         return None, None, st
     cg = _arr(conv_grid)
@@ -300,3 +323,78 @@ def eval_batch_mt(tables, jobs, n_threads, conv_grid=None):
                           _dp(cg) if cg is not None else None,
                           (cg.size - 1) if cg is not None else 0, n_threads)
     return outs, [arr[i].status for i in range(len(jobs))]
+
+
+class Xill:
+    """Oracle view of a xillver table given as the dict of
+    relxill_amd.xilltable_view() (rows as stored in the file)."""
+
+    def __init__(self, x):
+        self._keep = []
+
+        def f32(a):
+            a = np.ascontiguousarray(a, dtype=np.float32)
+            self._keep.append(a)
+            return a
+        t = XillTable()
+        t.n_par = x["n_par"]
+        for k in range(x["n_par"]):
+            t.pid[k] = x["pid"][k]
+            t.n_val[k] = x["n_val"][k]
+            t.val[k] = _fp(f32(x["val"][k]))
+        t.n_ener = x["n_ener"]
+        t.elo = _fp(f32(x["elo"]))
+        t.ehi = _fp(f32(x["ehi"]))
+        t.spec = _fp(f32(np.asarray(x["spec"]).reshape(-1)))
+        self.t = t
+        self.arrays = x
+
+    @property
+    def n_incl(self):
+        return self.arrays["n_val"][-1]
+
+
+def xill_param(gam, afe, lxi, ect, dens=15.0, incl=30.0, refl_frac=0.0):
+    return XillParam(gam, afe, lxi, ect, dens, incl, refl_frac)
+
+
+def xill_corner(xt, xp):
+    """(indices, factors) of get_xilltab_indices_for_paramvals / interp_xill_table"""
+    ind = (_I * 5)()
+    fac = np.zeros(5)
+    lib.orc_xill_indices(ctypes.byref(xt.t), ctypes.byref(xp), ind)
+    lib.orc_xill_factors(ctypes.byref(xt.t), ctypes.byref(xp), ind, _dp(fac))
+    return np.array(ind[:xt.t.n_par]), fac[:xt.t.n_par]
+
+
+def xill_spectra(xt, xp):
+    """(ener[n+1], flu[n_incl, n]) of the table for the relxill model"""
+    n = xt.t.n_ener
+    e = np.zeros(n + 1)
+    flu = np.zeros(xt.n_incl * n)
+    lib.orc_xill_spectra(ctypes.byref(xt.t), ctypes.byref(xp), _dp(e), _dp(flu))
+    return e, flu.reshape(xt.n_incl, n)
+
+
+def cutoffpl(ener, ect, gam, shift=1.0):
+    e = _arr(ener)
+    out = np.zeros(e.size - 1)
+    lib.orc_spec_cutoffpl(_dp(out), _dp(e), e.size - 1, ect, gam, shift)
+    return out
+
+
+def primary_norm_factor(gam, ect):
+    return lib.orc_primary_norm_factor(gam, ect)
+
+
+def relxill(tables, xt, params, energy, conv_grid=None, phys_norm=False):
+    """relxill on the caller grid; returns (flux, dist, status)."""
+    e = _arr(energy)
+    n = e.size - 1
+    out = np.zeros(n)
+    dist = np.zeros(xt.n_incl)
+    cg = _arr(conv_grid) if conv_grid is not None else None
+    st = lib.orc_relxill(ctypes.byref(tables.rel), ctypes.byref(xt.t), _dp(_arr(params)), _dp(e), _dp(out), n,
+                         _dp(cg) if cg is not None else None, (cg.size - 1) if cg is not None else 0,
+                         int(phys_norm), _dp(dist), None)
+    return out, dist, st

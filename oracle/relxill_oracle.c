@@ -306,7 +306,8 @@ int orc_n_params(int model_type) {
     case ORC_MOD_RELCONV: return 8;
     case ORC_MOD_RELLINELP: return 10;
     case ORC_MOD_RELCONVLP: return 9;
-    case ORC_MOD_RELXILL: return 8;  /* relxill-style conv: relconv's parameters */
+    case ORC_MOD_RELXILL: return 13;
+    case ORC_MOD_RELXILL_CONV: return 8;  /* relxill-style conv: relconv's parameters */
     default: return -1;
   }
 }
@@ -352,8 +353,13 @@ int orc_get_rel_params(int model_type, const double *par, int phys_norm_env, orc
       p->a = par[4]; incl_deg = par[5]; p->rin = par[6]; p->rout = par[7]; p->z = par[8];
       p->limb = (int)lround(par[9]);
       break;
+    case ORC_MOD_RELXILL:  /* Index1 Index2 Rbr a Incl Rin Rout z gamma logxi Afe Ecut refl_frac */
+      p->emis_type = ORC_EMIS_BKN;
+      p->emis1 = par[0]; p->emis2 = par[1]; p->rbr = par[2]; p->a = par[3]; incl_deg = par[4];
+      p->rin = par[5]; p->rout = par[6]; p->z = par[7]; p->gamma = par[8];
+      break;
     case ORC_MOD_RELCONV:  /* Index1 Index2 Rbr a Incl Rin Rout limb */
-    case ORC_MOD_RELXILL:
+    case ORC_MOD_RELXILL_CONV:
       p->emis_type = ORC_EMIS_BKN;
       p->emis1 = par[0]; p->emis2 = par[1]; p->rbr = par[2]; p->a = par[3]; incl_deg = par[4];
       p->rin = par[5]; p->rout = par[6]; p->limb = (int)lround(par[7]);
@@ -1104,6 +1110,258 @@ int orc_relxill_conv(const orc_reltable *tab, const orc_relparam *p, const doubl
     if (dist_out) memcpy(dist_out, dist, sizeof(double) * n_incl);
   }
   free(rel); free(dist); free(xill); free(xill_rb); free(conv_out); free(total);
+  return status;
+}
+
+/* ------------------------------------------------------------------------ */
+/* xillver table and the relxill model                                       */
+/* ------------------------------------------------------------------------ */
+
+static const double EMIN_XILL_NORM = 0.1;     /* src/Xillspec.h:28 */
+static const double EMAX_XILL_NORM = 1000.0;  /* src/Xillspec.h:29 */
+static const int N_ENER_COARSE = 500;         /* src/Xillspec.h:32 */
+
+/* the input value of each table parameter, as float (get_xilltab_paramvals,
+ * src/xilltable.c:278-295; kTbb and Frac are 0 for relxill) */
+static void xill_paramvals(const orc_xillparam *x, float *v) {
+  for (int k = 0; k < 8; k++) v[k] = 0.0f;
+  v[ORC_XP_GAM] = (float)x->gam;
+  v[ORC_XP_AFE] = (float)x->afe;
+  v[ORC_XP_LXI] = (float)x->lxi;
+  v[ORC_XP_ECT] = (float)x->ect;
+  v[ORC_XP_DNS] = (float)x->dens;
+  v[ORC_XP_INC] = (float)x->incl;
+}
+
+void orc_xill_indices(const orc_xilltable *t, const orc_xillparam *x, int *ind) {
+  float v[8];
+  xill_paramvals(x, v);
+  for (int k = 0; k < t->n_par; k++) {
+    ind[k] = orc_binary_search_float(t->val[k], t->n_val[k], v[t->pid[k]]);
+    if (ind[k] < 0) ind[k] = 0;
+    else if (ind[k] > t->n_val[k] - 2) ind[k] = t->n_val[k] - 2;
+  }
+}
+
+static int xill_dim_of(const orc_xilltable *t, int id) {  /* get_xilltab_param_index, :455-465 */
+  for (int k = 0; k < t->n_par; k++)
+    if (t->pid[k] == id) return k;
+  return -1;
+}
+
+void orc_xill_factors(const orc_xilltable *t, const orc_xillparam *x, const int *ind, double *fac) {
+  float v[8];
+  xill_paramvals(x, v);
+  for (int k = 0; k < t->n_par; k++) {
+    const int pk = t->pid[k];
+    /* relxill models do not clamp the inclination (it is not interpolated) */
+    if (pk != ORC_XP_INC) {
+      const float lo = t->val[k][0], hi = t->val[k][t->n_val[k] - 1];
+      if (v[pk] < lo) v[pk] = lo;
+      else if (v[pk] > hi) v[pk] = hi;
+    }
+    /* float arithmetic throughout, as in the reference */
+    const float num = v[pk] - t->val[k][ind[k]];
+    const float den = t->val[k][ind[k] + 1] - t->val[k][ind[k]];
+    fac[k] = (float)(num / den);
+  }
+  /* ensure_ecut_within_boundarys: the reference reads the Ecut axis as
+   * param_vals[param_index[k_ect]], i.e. the axis whose number is the id of
+   * Ecut (3); for tables with Ecut as fourth parameter that is its own axis */
+  const int k_ect = xill_dim_of(t, ORC_XP_ECT);
+  if (k_ect >= 0) {
+    const int q = t->pid[k_ect];
+    if (q < t->n_par) {
+      if (x->ect <= t->val[q][0]) fac[k_ect] = 0.0;
+      if (x->ect >= t->val[q][t->n_val[q] - 1]) fac[k_ect] = 1.0;
+    }
+  }
+}
+
+void orc_xill_renorm(float *spec, int n, double lxi, double dens) {
+  for (int ii = 0; ii < n; ii++) {
+    spec[ii] /= pow(10, lxi);
+    if (fabs(dens - 15) > 1e-6) spec[ii] /= pow(10, dens - 15);
+  }
+}
+
+/* one spectrum of the table, renormalised as when it is loaded
+ * (xilltable_fits_load_single_spec, src/xilltable.c:513-566, with
+ * normalizeXillverSpecLogxiDensity, :488-511) */
+static void xill_row(const orc_xilltable *t, const orc_xillparam *x, const int *idx, float *dst) {
+  size_t row = 0;
+  for (int k = 0; k < t->n_par; k++) row = row * (size_t)t->n_val[k] + (size_t)idx[k];
+  memcpy(dst, t->spec + row * (size_t)t->n_ener, sizeof(float) * t->n_ener);
+  double dens = x->dens, lxi = x->lxi;
+  const int k_dns = xill_dim_of(t, ORC_XP_DNS), k_lxi = xill_dim_of(t, ORC_XP_LXI);
+  if (k_dns >= 0) dens = t->val[k_dns][idx[k_dns]];
+  if (k_lxi >= 0) lxi = t->val[k_lxi][idx[k_lxi]];
+  orc_xill_renorm(dst, t->n_ener, lxi, dens);
+}
+
+void orc_xill_spectra(const orc_xilltable *t, const orc_xillparam *x, double *ener, double *flu) {
+  const int n = t->n_ener, n_incl = t->n_val[t->n_par - 1];
+  for (int ii = 0; ii < n; ii++) ener[ii] = t->elo[ii];
+  ener[n] = t->ehi[n - 1];
+  int ind[5];
+  double fac[5];
+  orc_xill_indices(t, x, ind);
+  orc_xill_factors(t, x, ind, fac);
+  /* interp_5d_tab_incl (src/xilltable.c:812-876), for every inclination:
+   * corners in the reference's order, weights as its products */
+  static const char corner[16][4] = {
+      {0, 0, 0, 0}, {1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {1, 1, 0, 0}, {1, 0, 1, 0},
+      {0, 1, 1, 0}, {1, 1, 1, 0}, {0, 0, 0, 1}, {1, 0, 0, 1}, {0, 1, 0, 1}, {0, 0, 1, 1},
+      {1, 1, 0, 1}, {1, 0, 1, 1}, {0, 1, 1, 1}, {1, 1, 1, 1}};
+  double w[16];
+  for (int c = 0; c < 16; c++) {
+    double f[4];
+    for (int k = 0; k < 4; k++) f[k] = corner[c][k] ? fac[k] : (1.0 - fac[k]);
+    w[c] = f[0] * f[1] * f[2] * f[3];
+  }
+  float *rows = (float *)malloc(sizeof(float) * 16 * (size_t)n);
+  for (int ii = 0; ii < n_incl; ii++) {
+    for (int c = 0; c < 16; c++) {
+      int idx[5];
+      for (int k = 0; k < 4; k++) idx[k] = ind[k] + corner[c][k];
+      idx[4] = ii;
+      xill_row(t, x, idx, rows + (size_t)c * n);
+    }
+    double *out = flu + (size_t)ii * n;
+    for (int e = 0; e < n; e++) {
+      double v = 0.0;
+      for (int c = 0; c < 16; c++) v = v + w[c] * (double)rows[(size_t)c * n + e];
+      out[e] = v;
+    }
+  }
+  free(rows);
+}
+
+void orc_xill_angdep(double *out, const double *flu, const double *dist, int n_incl, int n_ener) {
+  for (int jj = 0; jj < n_ener; jj++) out[jj] = 0.0;
+  for (int ii = 0; ii < n_incl; ii++)
+    for (int jj = 0; jj < n_ener; jj++) out[jj] += dist[ii] * flu[(size_t)ii * n_ener + jj];
+}
+
+void orc_spec_cutoffpl(double *flux, const double *ener, int n, double ect, double gam, double shift) {
+  const double ecut = ect * shift;
+  for (int ii = 0; ii < n; ii++) {
+    const double en = 0.5 * (ener[ii] + ener[ii + 1]);
+    flux[ii] = exp(1.0 / ecut) * pow(en, -gam) * exp(-en / ecut) * (ener[ii + 1] - ener[ii]);
+  }
+}
+
+double orc_norm_wrt_xillver(const double *flux, const double *ener, int n, int *status) {
+  const double keV2erg = 1.602177e-09;
+  const double tol = 1e-6;
+  if (ener[n] < (EMAX_XILL_NORM - tol) || ener[0] > (EMIN_XILL_NORM + tol)) {
+    if (status) *status = 1;
+    return 0.0;
+  }
+  double sum_pl = 0.0;
+  for (int ii = 0; ii < n; ii++) {
+    if (ener[ii] >= EMIN_XILL_NORM && ener[ii] <= EMAX_XILL_NORM)
+      sum_pl += flux[ii] * 0.5 * (ener[ii] + ener[ii + 1]) * 1e20 * keV2erg;
+  }
+  const double norm_xillver_table = 1e15 / 4.0 / M_PI;
+  return sum_pl / norm_xillver_table;
+}
+
+double orc_primary_norm_factor(double gam, double ect) {
+  double ener[N_ENER_COARSE + 1], flux[N_ENER_COARSE];
+  orc_get_log_grid(ener, N_ENER_COARSE + 1, EMIN_XILL_NORM, EMAX_XILL_NORM);
+  orc_spec_cutoffpl(flux, ener, N_ENER_COARSE, ect, gam, 1.0);
+  return 1. / orc_norm_wrt_xillver(flux, ener, N_ENER_COARSE, NULL);
+}
+
+int orc_get_relxill_params(const double *par, int phys_norm_env, orc_relparam *p, orc_xillparam *x) {
+  int status = orc_get_rel_params(ORC_MOD_RELXILL, par, phys_norm_env, p);
+  /* get_xill_params (src/ModelDefinition.cpp:339-390): Ecut in keV, the
+   * density at its default 15; logXi kept within 0-4.7 for the zones
+   * (adjust_parameter_to_be_within_xillver_bounds, src/IonGradient.cpp:36-44,
+   * :388) */
+  memset(x, 0, sizeof(*x));
+  x->gam = par[8];
+  x->lxi = par[9];
+  x->afe = par[10];
+  x->ect = par[11];
+  x->refl_frac = par[12];
+  x->dens = 15;
+  x->incl = par[4];
+  if (x->lxi < 0.0) x->lxi = 0.0;
+  else if (x->lxi > 4.7) x->lxi = 4.7;
+  return status;
+}
+
+int orc_relxill(const orc_reltable *tab, const orc_xilltable *xt, const double *parameter,
+                const double *energy, double *flux, int n_flux, const double *conv_ener,
+                int n_conv, int phys_norm_env, double *dist_out, orc_stats *st) {
+  orc_relparam p;
+  orc_xillparam x;
+  int status = orc_get_relxill_params(parameter, phys_norm_env, &p, &x);
+  if (status) {
+    memset(flux, 0, sizeof(double) * n_flux);
+    return status;
+  }
+  double *grid = NULL;
+  if (!conv_ener) {  /* get_relxill_conv_energy_grid, src/Xillspec.cpp:44-54 */
+    n_conv = 4096;
+    grid = (double *)malloc(sizeof(double) * (n_conv + 1));
+    orc_get_log_grid(grid, n_conv + 1, 0.00035, 2000.0);
+    conv_ener = grid;
+  }
+  const int nx = xt->n_ener, n_incl = xt->n_val[xt->n_par - 1];
+  double *ener = (double *)malloc(sizeof(double) * (n_flux + 1));
+  double *xe = (double *)malloc(sizeof(double) * (nx + 1));
+  double *xflu = (double *)malloc(sizeof(double) * (size_t)n_incl * nx);
+  double *xill = (double *)malloc(sizeof(double) * nx);
+  double *rel = (double *)malloc(sizeof(double) * n_conv);
+  double *dist = (double *)calloc(n_incl, sizeof(double));
+  double *xill_rb = (double *)malloc(sizeof(double) * n_conv);
+  double *conv_out = (double *)malloc(sizeof(double) * n_conv);
+  double *total = (double *)calloc(n_conv, sizeof(double));
+  double *prim = (double *)malloc(sizeof(double) * n_conv);
+  /* shift_energy_grid_redshift (src/XspecSpectrum.h:70-76) */
+  memcpy(ener, energy, sizeof(double) * (n_flux + 1));
+  if (p.z > 0)
+    for (int ii = 0; ii <= n_flux; ii++) ener[ii] *= (1 + p.z);
+  /* 1-2: one zone, constant ionisation: the zone's xillver parameters are
+   * those of the source (IonGradient, src/IonGradient.cpp:196-209, :340-389) */
+  orc_xill_spectra(xt, &x, xe, xflu);
+  /* 4: the relline profile with its angle distribution on the relxill grid */
+  status = orc_relbase(tab, NULL, &p, conv_ener, n_conv, 1000, rel, dist, n_incl, st);
+  if (status == 0) {
+    /* 5: angle weighting; the normalisation change source -> disk is 1 for
+     * the broken power law (energy shift 1, src/IonGradient.cpp:182-193,
+     * src/Xillspec.cpp:408-435) */
+    orc_xill_angdep(xill, xflu, dist, n_incl, nx);
+    /* 6: relxill_convolution_multizone (src/Relxill.cpp:432-482) */
+    double relsum = 0.0;
+    for (int jj = 0; jj < n_conv; jj++) relsum += rel[jj];
+    if (!(relsum < 1e-12)) {
+      orc_rebin_spectrum(conv_ener, xill_rb, n_conv, xe, xill, nx);
+      orc_conv_normalized(conv_ener, xill_rb, rel, conv_out, n_conv);
+      for (int jj = 0; jj < n_conv; jj++) total[jj] += conv_out[jj];
+    }
+    /* add_primary_spectrum (src/PrimarySource.cpp:66-125), not the lamp
+     * post: the reflection times |refl_frac|, the primary added for
+     * refl_frac >= 0; the primary is the cut-off power law normalised as the
+     * xillver incident spectrum (src/PrimarySource.h:238-248) */
+    for (int jj = 0; jj < n_conv; jj++) total[jj] *= fabs(x.refl_frac);
+    if (x.refl_frac >= 0) {
+      orc_spec_cutoffpl(prim, conv_ener, n_conv, x.ect, x.gam, 1.0);
+      const double nf = orc_primary_norm_factor(x.gam, x.ect);
+      for (int jj = 0; jj < n_conv; jj++) prim[jj] *= nf;
+      for (int jj = 0; jj < n_conv; jj++) total[jj] += prim[jj];
+    }
+    /* rebin_and_normalize_relxill_for_xspec (src/Relxill.cpp:267-278) */
+    orc_rebin_spectrum(ener, flux, n_flux, conv_ener, total, n_conv);
+    if (dist_out) memcpy(dist_out, dist, sizeof(double) * n_incl);
+  } else {
+    memset(flux, 0, sizeof(double) * n_flux);
+  }
+  free(grid); free(ener); free(xe); free(xflu); free(xill); free(rel); free(dist);
+  free(xill_rb); free(conv_out); free(total); free(prim);
   return status;
 }
 

@@ -33,6 +33,9 @@ extern "C" {
 #define ORC_MOD_RELCONV 11
 #define ORC_MOD_RELCONVLP 12
 #define ORC_MOD_RELXILL (-1)
+/* project id: relxill-style convolution of a given angle-resolved reflection
+ * set, with relconv's 8 parameters and relxill's profile normalisation */
+#define ORC_MOD_RELXILL_CONV (-1001)
 
 /* emissivity types (src/common.h:50-55) */
 #define ORC_EMIS_BKN 1
@@ -174,6 +177,68 @@ int orc_relxill_conv(const orc_reltable *tab, const orc_relparam *p, const doubl
                      int n_conv, const double *refl_ener, const double *refl, int n_refl,
                      int n_incl, const double *out_ener, double *out, int n_out, int nr_fine,
                      double *dist_out, orc_stats *st);
+
+/* ---- xillver table and the relxill model ---- */
+/* xillver parameter ids (PARAM_*, src/common.h:141-149) */
+#define ORC_XP_GAM 0
+#define ORC_XP_AFE 1
+#define ORC_XP_LXI 2
+#define ORC_XP_ECT 3
+#define ORC_XP_DNS 4
+#define ORC_XP_INC 7
+
+/* a five-parameter xillver table (xillTable, src/xilltable.h), the last
+ * parameter the inclination; spec holds the rows as stored in the file
+ * (row r at spec[r * n_ener], r = ((((i0 n1 + i1) n2 + i2) n3 + i3) n4 + i4),
+ * src/xilltable.c:326-337), before the division by 10^logXi */
+typedef struct {
+  int n_par;
+  int pid[5];
+  int n_val[5];
+  const float *val[5];
+  int n_ener;
+  const float *elo, *ehi;
+  const float *spec;
+} orc_xilltable;
+
+/* xillver parameters of one evaluation (xillTableParam, src/xilltable.h) */
+typedef struct {
+  double gam, afe, lxi, ect, dens, incl;  /* incl in degrees */
+  double refl_frac;
+} orc_xillparam;
+
+/* get_xilltab_indices_for_paramvals (src/xilltable.c:297-324) */
+void orc_xill_indices(const orc_xilltable *t, const orc_xillparam *x, int *ind);
+/* the interpolation factors of interp_xill_table (src/xilltable.c:1090-1125)
+ * with resetInpvalsToBoundaries (:1069-1088) and ensure_ecut_within_boundarys
+ * (:1054-1067) */
+void orc_xill_factors(const orc_xilltable *t, const orc_xillparam *x, const int *ind, double *fac);
+/* renorm_xill_spec (src/xilltable.c:478-486) */
+void orc_xill_renorm(float *spec, int n, double lxi, double dens);
+/* get_xillver_spectra_table for a relxill model (src/Xillspec.cpp:58-88):
+ * ener[n_ener+1] (elo, then the last ehi), flu[n_incl][n_ener] */
+void orc_xill_spectra(const orc_xilltable *t, const orc_xillparam *x, double *ener, double *flu);
+/* calc_xillver_angdep (src/Xillspec.cpp:557-573) */
+void orc_xill_angdep(double *out, const double *flu, const double *dist, int n_incl, int n_ener);
+/* spec_cutoffpl (src/Xillspec.cpp:215-241) */
+void orc_spec_cutoffpl(double *flux, const double *ener, int n, double ect, double gam, double shift);
+/* calcNormWrtXillverTableSpec (src/Xillspec.cpp:179-205); *status = 1 when
+ * the grid does not cover 0.1-1000 keV */
+double orc_norm_wrt_xillver(const double *flux, const double *ener, int n, int *status);
+/* 1 / calcNormWrtXillverTableSpec of the cut-off power law on the coarse
+ * 500-bin grid (PrimarySource::calc_normalization_factor_source,
+ * src/PrimarySource.h:279-292) */
+double orc_primary_norm_factor(double gam, double ect);
+/* relxill parameters: Index1 Index2 Rbr a Incl Rin Rout z gamma logxi Afe
+ * Ecut refl_frac (src/modelfiles/lmodel_relxill_public.dat:46-59) */
+int orc_get_relxill_params(const double *parameter, int phys_norm_env, orc_relparam *p,
+                           orc_xillparam *x);
+/* relxill (src/Relxill.cpp:289-414 with src/PrimarySource.cpp:66-125): the
+ * broken power-law model with one zone and constant ionisation.
+ * dist_out (optional): the emission-angle distribution (n_incl values). */
+int orc_relxill(const orc_reltable *tab, const orc_xilltable *xt, const double *parameter,
+                const double *energy, double *flux, int n_flux, const double *conv_ener,
+                int n_conv, int phys_norm_env, double *dist_out, orc_stats *st);
 
 /* ---- model level: what lmod<name> computes (src/LocalModel.cpp:32-98) ---- */
 /* energy: n_flux+1 edges; flux: n_flux (input spectrum for conv models).

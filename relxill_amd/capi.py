@@ -26,17 +26,19 @@ MODEL_RELLINE = 1
 MODEL_RELLINELP = 2
 MODEL_RELCONV = 11
 MODEL_RELCONVLP = 12
-MODEL_RELXILL_CONV = -1
+MODEL_RELXILL = -1
+MODEL_RELXILL_CONV = -1001  # project id: relxill-style conv of a resident reflection set
 
 FP64 = 0
 FP32 = 1
 
 N_PARAMS = {MODEL_RELLINE: 10, MODEL_RELCONV: 8, MODEL_RELLINELP: 10, MODEL_RELCONVLP: 9,
-            MODEL_RELXILL_CONV: 8}
+            MODEL_RELXILL_CONV: 8, MODEL_RELXILL: 13}
 
 # default parameter values of the reference's lmodel.dat
-# (src/modelfiles/lmodel_relxill_public.dat:1-44)
+# (src/modelfiles/lmodel_relxill_public.dat:1-59)
 DEFAULTS = {
+    MODEL_RELXILL: [3.0, 3.0, 15.0, 0.998, 30.0, -1.0, 400.0, 0.0, 2.0, 3.1, 1.0, 300.0, 3.0],
     MODEL_RELLINE: [6.4, 3.0, 3.0, 15.0, 0.998, 30.0, -1.0, 400.0, 0.0, 0.0],
     MODEL_RELCONV: [3.0, 3.0, 15.0, 0.998, 30.0, -1.0, 400.0, 0.0],
     MODEL_RELLINELP: [6.4, 6.0, 0.998, 30.0, -1.0, 400.0, 0.0, 0.0, 2.0, 1.0],
@@ -66,6 +68,11 @@ class _RelView(ctypes.Structure):
                 ("gmax", _F), ("trff1", _F), ("trff2", _F), ("cosne1", _F), ("cosne2", _F)]
 
 
+class _XillView(ctypes.Structure):
+    _fields_ = [("n_par", ctypes.c_int), ("pid", ctypes.c_int * 5), ("n_val", ctypes.c_int * 5),
+                ("val", _F * 5), ("n_ener", ctypes.c_int), ("elo", _F), ("ehi", _F), ("spec", _F)]
+
+
 class _LpView(ctypes.Structure):
     _fields_ = [("n_a", ctypes.c_int), ("n_h", ctypes.c_int), ("n_r", ctypes.c_int),
                 ("a", _F), ("h", _F), ("rad", _F), ("intens", _F), ("del_", _F),
@@ -78,6 +85,10 @@ _EXPORTS = {
     "lmodrellinelp": (None, [_D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_char_p]),
     "lmodrelconv": (None, [_D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_char_p]),
     "lmodrelconvlp": (None, [_D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_char_p]),
+    "lmodrelxill": (None, [_D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_char_p]),
+    "rx_xilltable_synthetic": (ctypes.c_int, [ctypes.c_uint64] + [ctypes.c_int] * 6),
+    "rx_xilltable_load_fits": (ctypes.c_int, [ctypes.c_char_p]),
+    "rx_xilltable_get_view": (ctypes.c_int, [ctypes.POINTER(_XillView)]),
     "rx_eval_model": (ctypes.c_int, [ctypes.c_int, _D, _D, _D, ctypes.c_int, ctypes.c_int]),
     "rx_last_status": (ctypes.c_int, []),
     "rx_last_error": (ctypes.c_char_p, []),
@@ -199,6 +210,29 @@ def lmodrelconvlp(energy, params, flux):
     return _lmod(lib.lmodrelconvlp, energy, params, flux)
 
 
+def lmodrelxill(energy, params):
+    return _lmod(lib.lmodrelxill, energy, params)
+
+
+def xilltable_synthetic(seed=5, n_gam=13, n_afe=5, n_lxi=8, n_ect=6, n_incl=10, n_ener=2999):
+    _check(lib.rx_xilltable_synthetic(seed, n_gam, n_afe, n_lxi, n_ect, n_incl, n_ener))
+
+
+def xilltable_view():
+    """Copy of the resident xillver table: dict with n_par, pid, n_val, val
+    (list of arrays), n_ener, elo, ehi, spec (rows x n_ener, as stored in the
+    file)."""
+    v = _XillView()
+    _check(lib.rx_xilltable_get_view(ctypes.byref(v)))
+    f = lambda ptr, n: np.ctypeslib.as_array(ptr, shape=(n,)).astype(np.float32).copy()
+    n_val = [v.n_val[k] for k in range(v.n_par)]
+    rows = int(np.prod(n_val))
+    return dict(n_par=v.n_par, pid=[v.pid[k] for k in range(v.n_par)], n_val=n_val,
+                val=[f(v.val[k], n_val[k]) for k in range(v.n_par)], n_ener=v.n_ener,
+                elo=f(v.elo, v.n_ener), ehi=f(v.ehi, v.n_ener),
+                spec=f(v.spec, rows * v.n_ener).reshape(rows, v.n_ener))
+
+
 def tables_synthetic(seed=1, n_a=25, n_mu0=22, n_r=100, n_g=20):
     _check(lib.rx_tables_synthetic(seed, n_a, n_mu0, n_r, n_g))
 
@@ -310,8 +344,8 @@ class Batch:
         return r // 256, r % 256
 
     def kernel_ms(self):
-        """per kernel ms of the last profiled run: setup, integrate, fixup, finalize, conv"""
-        o = np.zeros(5)
+        """per kernel ms of the last profiled run: setup, integrate, fixup, finalize, conv, xill"""
+        o = np.zeros(6)
         _check(lib.rx_batch_kernel_ms(self.h, _dp(o)))
         return o
 

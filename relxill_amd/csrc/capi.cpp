@@ -111,6 +111,11 @@ void lmodrelconvlp(const double *energy, int Nflux, const double *parameter, int
   rx_eval_model(RX_MODEL_RELCONVLP, parameter, energy, flux, Nflux, RX_FP64);
 }
 
+void lmodrelxill(const double *energy, int Nflux, const double *parameter, int, double *flux, double *,
+                 const char *) {
+  rx_eval_model(RX_MODEL_RELXILL, parameter, energy, flux, Nflux, RX_FP64);
+}
+
 int rx_last_status(void) { return last_status(); }
 const char *rx_last_error(void) { return last_error(); }
 
@@ -236,6 +241,73 @@ int rx_lptable_load_fits(const char *path) {
   g.lp_loaded = true;
   g.lp_dirty = true;
   g.config_gen++;
+  return RX_OK;
+}
+
+int rx_xilltable_synthetic(uint64_t seed, int n_gam, int n_afe, int n_lxi, int n_ect, int n_incl, int n_ener) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (n_gam < 2 || n_afe < 2 || n_lxi < 2 || n_ect < 2 || n_incl < 2 || n_incl > MAX_INCL || n_ener < 2) {
+    set_last(RX_ERR_ARG, "xillver table dims out of range");
+    return RX_ERR_ARG;
+  }
+  try {
+    g.xill = make_synthetic_xilltable(seed, n_gam, n_afe, n_lxi, n_ect, n_incl, n_ener);
+  } catch (...) {
+    set_last(RX_ERR_NOMEM, "table allocation failed");
+    return RX_ERR_NOMEM;
+  }
+  g.xill_loaded = true;
+  g.xill_dirty = true;
+  g.config_gen++;
+  std::string e = validate_xilltable(g.xill);
+  set_last(e.empty() ? RX_OK : RX_ERR_TABLE, e);
+  return e.empty() ? RX_OK : RX_ERR_TABLE;
+}
+
+int rx_xilltable_load_fits(const char *path) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!path) return RX_ERR_ARG;
+  XillTableHost t;
+  std::string e;
+  try {
+    e = load_xilltable_fits(t, path);
+  } catch (...) {
+    e = "table allocation failed";
+  }
+  if (!e.empty()) {
+    set_last(RX_ERR_TABLE, e);
+    return RX_ERR_TABLE;
+  }
+  if (t.n_incl() > MAX_INCL) {
+    set_last(RX_ERR_TABLE, "xillver table has more inclinations than the kernels support");
+    return RX_ERR_TABLE;
+  }
+  g.xill = std::move(t);
+  g.xill_loaded = true;
+  g.xill_dirty = true;
+  g.config_gen++;
+  return RX_OK;
+}
+
+int rx_xilltable_get_view(rx_xilltable_view *v) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!v) return RX_ERR_ARG;
+  ensure_xilltable_host();
+  const XillTableHost &t = g.xill;
+  memset(v, 0, sizeof(*v));
+  v->n_par = t.n_par;
+  for (int k = 0; k < t.n_par && k < 5; k++) {
+    v->pid[k] = t.pid[k];
+    v->n_val[k] = t.n_val[k];
+    v->val[k] = t.val[k].data();
+  }
+  v->n_ener = t.n_ener;
+  v->elo = t.elo.data();
+  v->ehi = t.ehi.data();
+  v->spec = t.spec.data();
   return RX_OK;
 }
 

@@ -60,6 +60,7 @@ int n_model_params(int model) {
     case MOD_RELLINELP: return 10;
     case MOD_RELCONVLP: return 9;
     case MOD_RELXILL_CONV: return 8;
+    case MOD_RELXILL: return 13;
     default: return -1;
   }
 }
@@ -78,6 +79,16 @@ int parse_request(int model, const double *par, bool phys_norm, ReqParams &p, st
       p.lineE = par[0]; p.emis1 = par[1]; p.emis2 = par[2]; p.rbr = par[3]; p.a = par[4];
       incl_deg = par[5]; p.rin = par[6]; p.rout = par[7]; p.z = par[8];
       p.limb = (int)lround(par[9]);
+      break;
+    case MOD_RELXILL:  // Index1 Index2 Rbr a Incl Rin Rout z gamma logxi Afe Ecut refl_frac
+      p.emis_type = EMIS_BKN;
+      p.emis1 = par[0]; p.emis2 = par[1]; p.rbr = par[2]; p.a = par[3]; incl_deg = par[4];
+      p.rin = par[5]; p.rout = par[6]; p.z = par[7]; p.gamma = par[8];
+      // get_xill_params (src/ModelDefinition.cpp:339-390); the cut-off power
+      // law of the broken power-law model keeps Ecut as given
+      p.xgam = par[8];
+      p.xect = par[11];
+      p.refl_frac = par[12];
       break;
     case MOD_RELCONV:  // Index1 Index2 Rbr a Incl Rin Rout limb
     case MOD_RELXILL_CONV:
@@ -153,6 +164,59 @@ int parse_request(int model, const double *par, bool phys_norm, ReqParams &p, st
   return RX_OK;
 }
 
+// binary_search_float (src/relutility.c:135-151)
+static int bsearch_float(const float *arr, int n, float val) {
+  if (n <= 1) return -1;
+  int klo = 0, khi = n - 1;
+  while (khi - klo > 1) {
+    const int k = (khi + klo) / 2;
+    if (arr[k] > val) khi = k;
+    else klo = k;
+  }
+  return klo;
+}
+
+void xill_corner(const XillTableHost &t, double gam, double afe, double lxi, double ect, double dens,
+                 double incl_deg, int *ind, double *fac) {
+  // the input value of every table parameter as float (get_xilltab_paramvals,
+  // src/xilltable.c:278-295); kTbb and Frac are 0 for relxill
+  float v[8] = {(float)gam, (float)afe, (float)lxi, (float)ect, (float)dens, 0.0f, 0.0f, (float)incl_deg};
+  int k_ind[XILL_NPAR];
+  for (int k = 0; k < t.n_par; k++) {
+    int i = bsearch_float(t.val[k].data(), t.n_val[k], v[t.pid[k]]);
+    if (i < 0) i = 0;
+    else if (i > t.n_val[k] - 2) i = t.n_val[k] - 2;
+    k_ind[k] = i;
+  }
+  // factors in float after clamping the value into the table (relxill
+  // models leave the inclination alone: it is not interpolated)
+  double k_fac[XILL_NPAR];
+  for (int k = 0; k < t.n_par; k++) {
+    float &x = v[t.pid[k]];
+    const std::vector<float> &vals = t.val[k];
+    if (t.pid[k] != XP_INC) {
+      if (x < vals[0]) x = vals[0];
+      else if (x > vals[t.n_val[k] - 1]) x = vals[t.n_val[k] - 1];
+    }
+    const float num = x - vals[k_ind[k]];
+    const float den = vals[k_ind[k] + 1] - vals[k_ind[k]];
+    k_fac[k] = (float)(num / den);
+  }
+  // ensure_ecut_within_boundarys (src/xilltable.c:1054-1067) reads the axis
+  // numbered by the id of Ecut (3): Ecut's own axis in tables that list it
+  // fourth, as xillver-a-Ec5.fits does
+  const int k_ect = t.dim_of(XP_ECT);
+  if (k_ect >= 0 && t.pid[k_ect] < t.n_par) {
+    const std::vector<float> &vals = t.val[t.pid[k_ect]];
+    if (ect <= vals[0]) k_fac[k_ect] = 0.0;
+    if (ect >= vals[vals.size() - 1]) k_fac[k_ect] = 1.0;
+  }
+  for (int k = 0; k < 4; k++) {
+    ind[k] = k_ind[k];
+    fac[k] = k_fac[k];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // tables and grids
 // ---------------------------------------------------------------------------
@@ -185,6 +249,15 @@ void ensure_tables_host() {
     g.lp_dirty = true;
   }
   if (g.conv_ener.empty()) g.conv_dirty = true;
+  if (g.coarse_ener.empty()) {
+    // get_coarse_xillver_energrid (src/Xillspec.cpp:164-177) via get_log_grid
+    const int n = N_ENER_COARSE;
+    g.coarse_ener.resize(n + 1);
+    for (int ii = 0; ii < n + 1; ii++) {
+      double e = 1.0 * ii / (n + 1 - 1) * (log(1000.0) - log(0.1)) + log(0.1);
+      g.coarse_ener[ii] = exp(e);
+    }
+  }
   if (g.conv_dirty) {
     const int n = g.conv_n;
     g.conv_ener.resize(n + 1);
@@ -306,6 +379,7 @@ int ensure_device(std::string &err) {
     g.lp_dirty = false;
   }
   if (g.conv_dirty) {
+    e |= upload_vec(g.c_coarse, g.coarse_ener);
     e |= upload_vec(g.c_ener, g.conv_ener) | upload_vec(g.c_cfac, g.conv_cfac) | upload_vec(g.c_tw, g.conv_tw);
     e |= upload_vec(g.c_cfinv, g.conv_cfinv) | upload_vec(g.c_inv_emid, g.conv_inv_emid);
     e |= upload_vec(g.c_twp_fwd, g.conv_twp_fwd) | upload_vec(g.c_twp_inv, g.conv_twp_inv);
@@ -319,6 +393,45 @@ int ensure_device(std::string &err) {
     err = "device upload failed";
     return RX_ERR_DEVICE;
   }
+  return RX_OK;
+}
+
+void ensure_xilltable_host() {
+  Global &g = G();
+  if (g.xill_loaded) return;
+  bool ok = false;
+  if (const char *path = getenv("RELXILL_TABLE_PATH")) {
+    // the reference's table name (src/common.h:163)
+    ok = load_xilltable_fits(g.xill, std::string(path) + "/xillver-a-Ec5.fits").empty();
+  }
+  if (!ok) g.xill = make_synthetic_xilltable(5, 13, 5, 8, 6, 10, 2999);
+  g.xill_loaded = true;
+  g.xill_dirty = true;
+}
+
+int ensure_xilltable_device(std::string &err) {
+  Global &g = G();
+  ensure_xilltable_host();
+  if (!g.xill_dirty) return RX_OK;
+  const XillTableHost &t = g.xill;
+  if (t.n_incl() > MAX_INCL) {
+    err = "xillver table has more inclinations than the kernels support";
+    return RX_ERR_ARG;
+  }
+  if (hipDeviceSynchronize() != hipSuccess) {
+    err = "device synchronisation failed";
+    return RX_ERR_DEVICE;
+  }
+  // interp_xill_table's energy grid (src/xilltable.c:1104-1108)
+  g.xill_ener.resize(t.n_ener + 1);
+  for (int i = 0; i < t.n_ener; i++) g.xill_ener[i] = t.elo[i];
+  g.xill_ener[t.n_ener] = t.ehi[t.n_ener - 1];
+  int e = upload_vec(g.x_spec, xilltable_renormed(t)) | upload_vec(g.x_ener, g.xill_ener);
+  if (e) {
+    err = "xillver table upload failed";
+    return RX_ERR_DEVICE;
+  }
+  g.xill_dirty = false;
   return RX_OK;
 }
 
@@ -357,7 +470,13 @@ void fill_dev_views(BatchDev &bd) {
   bd.cg.twp_fwd = g.c_twp_fwd.as<double>();
   bd.cg.twp_inv = g.c_twp_inv.as<double>();
   bd.cg.tw = g.c_tw.as<double>();
+  bd.cg.coarse = g.c_coarse.as<double>();
   bd.refl_sets = g.d_refl_sets.as<DevReflSet>();
+  bd.xt.n_ener = g.xill.n_ener;
+  bd.xt.n_incl = g.xill.n_incl();
+  for (int k = 0; k < 4; k++) bd.xt.n_val[k] = g.xill.n_par ? g.xill.n_val[k] : 0;
+  bd.xt.spec = g.x_spec.as<float>();
+  bd.xt.ener = g.x_ener.as<double>();
 }
 
 // ---------------------------------------------------------------------------
@@ -371,7 +490,7 @@ Batch::~Batch() {
 }
 
 static bool is_conv(int model) {
-  return model == MOD_RELCONV || model == MOD_RELCONVLP || model == MOD_RELXILL_CONV;
+  return model == MOD_RELCONV || model == MOD_RELCONVLP || model == MOD_RELXILL_CONV || model == MOD_RELXILL;
 }
 
 int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
@@ -385,7 +504,15 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
   b.h_req.assign(n_req, ReqParams());
   b.h_status.assign(n_req, 0);
   b.h_conv.clear();
-  int64_t o_ener = 0, o_flux = 0, o_pgrid = 0, o_prof = 0;
+  b.h_xill.clear();
+  for (int i = 0; i < n_req; i++) {
+    if (req[i].model == MOD_RELXILL) {
+      int st = ensure_xilltable_device(err);
+      if (st != RX_OK) return st;
+      break;
+    }
+  }
+  int64_t o_ener = 0, o_flux = 0, o_pgrid = 0, o_prof = 0, o_xspec = 0;
   for (int i = 0; i < n_req; i++) {
     const rx_request &r = req[i];
     if (r.n_flux <= 0 || !r.energy || !r.parameter) {
@@ -411,7 +538,20 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
       p.prof_off = o_prof;
       o_prof += g.conv_n;
       b.h_conv.push_back(i);
-      if (st == RX_OK && r.model != MOD_RELXILL_CONV) {
+      if (r.model == MOD_RELXILL) {
+        // the zone's xillver parameters (src/IonGradient.cpp:196-209, :340-389):
+        // logXi kept within 0-4.7, density 15, Ecut unchanged for the broken
+        // power law
+        double lxi = r.parameter[9];
+        if (lxi < 0.0) lxi = 0.0;
+        else if (lxi > 4.7) lxi = 4.7;
+        xill_corner(g.xill, r.parameter[8], r.parameter[10], lxi, r.parameter[11], 15.0, r.parameter[4],
+                    p.xind, p.xfac);
+        p.n_incl = g.xill.n_incl();
+        p.xspec_off = o_xspec;
+        o_xspec += g.xill.n_ener;
+        b.h_xill.push_back(i);
+      } else if (st == RX_OK && r.model != MOD_RELXILL_CONV) {
         if (!r.flux_in) {
           err = "convolution request without input spectrum";
           return RX_ERR_ARG;
@@ -440,13 +580,20 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
   b.tot_flux = o_flux;
   b.tot_pgrid = o_pgrid;
   b.tot_prof = o_prof;
+  b.tot_xspec = o_xspec;
   b.h_ener.resize(o_ener);
   b.h_flux_in.assign(o_flux, 0.0);
   for (int i = 0; i < n_req; i++) {
     const rx_request &r = req[i];
     const ReqParams &p = b.h_req[i];
     memcpy(&b.h_ener[p.ener_off], r.energy, sizeof(double) * (r.n_flux + 1));
-    if (r.flux_in) memcpy(&b.h_flux_in[p.flux_off], r.flux_in, sizeof(double) * r.n_flux);
+    if (r.model == MOD_RELXILL && p.z > 0) {
+      // relxill evaluates on the redshifted grid (shift_energy_grid_redshift,
+      // src/XspecSpectrum.h:70-76)
+      for (int k = 0; k <= r.n_flux; k++) b.h_ener[p.ener_off + k] *= (1 + p.z);
+    }
+    if (r.flux_in && is_conv(r.model) && r.model != MOD_RELXILL)
+      memcpy(&b.h_flux_in[p.flux_off], r.flux_in, sizeof(double) * r.n_flux);
   }
   b.any_cosne = false;
   b.any_limb = false;
@@ -508,6 +655,8 @@ int batch_upload(Batch &b) {
   e |= b.d_refl.ensure(sizeof(LpRefl) * nreq);
   e |= b.d_status.ensure(sizeof(int) * nreq);
   e |= b.d_conv.ensure(sizeof(int) * std::max<size_t>(1, b.h_conv.size()));
+  e |= b.d_xill.ensure(sizeof(int) * std::max<size_t>(1, b.h_xill.size()));
+  e |= b.d_xspec.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_xspec));
   e |= b.d_items.ensure(sizeof(WorkItem) * (size_t)b.item_cap);
   e |= b.d_ctr.ensure(sizeof(int) * N_COUNTERS);
   e |= b.d_reach.ensure(sizeof(BinRange) * nreq);
@@ -523,6 +672,8 @@ int batch_upload(Batch &b) {
   e |= (int)hipMemcpyAsync(b.d_status.p, b.h_status.data(), sizeof(int) * nreq, hipMemcpyHostToDevice, b.stream);
   if (!b.h_conv.empty())
     e |= (int)hipMemcpyAsync(b.d_conv.p, b.h_conv.data(), sizeof(int) * b.h_conv.size(), hipMemcpyHostToDevice, b.stream);
+  if (!b.h_xill.empty())
+    e |= (int)hipMemcpyAsync(b.d_xill.p, b.h_xill.data(), sizeof(int) * b.h_xill.size(), hipMemcpyHostToDevice, b.stream);
   e |= (int)hipStreamSynchronize(b.stream);
   return e ? RX_ERR_DEVICE : RX_OK;
 }
@@ -556,6 +707,7 @@ static BatchDev dev_view(Batch &b) {
   bd.mode = b.last_mode;
   bd.guard = g.guard;
   fill_dev_views(bd);
+  bd.xspec = b.d_xspec.as<double>();
   return bd;
 }
 
@@ -581,8 +733,10 @@ int batch_run(Batch &b, int precision, std::string &err) {
   if (prof) e |= (int)hipEventRecord(b.ev[3], s);
   e |= launch_finalize(bd, s);
   if (prof) e |= (int)hipEventRecord(b.ev[4], s);
-  e |= launch_conv(bd, b.d_conv.as<int>(), (int)b.h_conv.size(), s);
+  e |= launch_xill(bd, b.d_xill.as<int>(), (int)b.h_xill.size(), s);
   if (prof) e |= (int)hipEventRecord(b.ev[5], s);
+  e |= launch_conv(bd, b.d_conv.as<int>(), (int)b.h_conv.size(), s);
+  if (prof) e |= (int)hipEventRecord(b.ev[6], s);
   b.profiled = prof;
   if (e) {
     err = "kernel launch failed";
@@ -607,10 +761,13 @@ int batch_sync(Batch &b) {
     if (hipStreamSynchronize(b.stream) != hipSuccess) return RX_ERR_DEVICE;
   }
   if (b.profiled) {
+    // events: setup | integrate | fixup | finalize | xill | conv; the timing
+    // slots keep conv fourth and add xill as the sixth
+    static const int slot[RX_N_TIMED] = {0, 1, 2, 3, 5, 4};
     for (int k = 0; k < RX_N_TIMED; k++) {
       float ms = 0;
       (void)hipEventElapsedTime(&ms, b.ev[k], b.ev[k + 1]);
-      b.kernel_ms[k] = ms;
+      b.kernel_ms[slot[k]] = ms;
     }
   }
   return RX_OK;

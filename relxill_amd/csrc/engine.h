@@ -24,6 +24,11 @@ struct DevBuf {
 // Host mirror of get_rel_params + check_parameter_bounds
 // (src/ModelDefinition.cpp:153-334) for one request.  Returns RX_OK or a status.
 int parse_request(int model, const double *par, bool phys_norm, ReqParams &p, std::string &err);
+// relxill requests: the xillver table corner and factors of the request's
+// parameters (get_xilltab_indices_for_paramvals and interp_xill_table,
+// src/xilltable.c:297-324, :1090-1125).
+void xill_corner(const XillTableHost &t, double gam, double afe, double lxi, double ect, double dens,
+                 double incl_deg, int *ind, double *fac);
 int n_model_params(int model);
 
 struct Batch {
@@ -35,9 +40,10 @@ struct Batch {
   std::vector<ReqParams> h_req;
   std::vector<int> h_status;  // host-side parse status
   std::vector<double> h_ener, h_flux_in;
-  std::vector<int> h_conv;
-  int64_t tot_ener = 0, tot_flux = 0, tot_pgrid = 0, tot_prof = 0;
+  std::vector<int> h_conv, h_xill;
+  int64_t tot_ener = 0, tot_flux = 0, tot_pgrid = 0, tot_prof = 0, tot_xspec = 0;
   DevBuf d_req, d_ener, d_flux_in, d_out, d_pgrid, d_node, d_rad, d_prof, d_dist, d_refl, d_status, d_conv;
+  DevBuf d_xill, d_xspec;
   DevBuf d_items, d_ctr, d_fix, d_reach, d_rnorm;
   int item_cap = 0;
   bool any_cosne = false, any_limb = false;
@@ -78,6 +84,11 @@ struct Global {
   double rel_gmin_all = 0.0, rel_gmax_all = 1e30;  // g range reachable by any window
   DevBuf t_a, t_mu0, t_r, t_gmin, t_gmax, t_trff1, t_trff2, t_cosne1, t_cosne2;
   DevBuf l_a, l_h, l_rad, l_int, l_del, l_inc;
+  // xillver table (loaded and uploaded on the first relxill request)
+  XillTableHost xill;
+  bool xill_loaded = false, xill_dirty = true;
+  std::vector<double> xill_ener;
+  DevBuf x_spec, x_ener;
   // convolution grid
   int conv_n = 4096;
   double conv_emin = 0.00035, conv_emax = 2000.0;
@@ -85,6 +96,8 @@ struct Global {
   int conv_i1kev = 0;
   int conv_band_lo = 0, conv_band_hi = 0;
   bool conv_dirty = true;
+  std::vector<double> coarse_ener;
+  DevBuf c_coarse;
   DevBuf c_ener, c_cfac, c_cfinv, c_inv_emid, c_tw, c_twp_fwd, c_twp_inv;
   int nr_fine = 1000;
   // reflection sets
@@ -108,6 +121,8 @@ struct Global {
 Global &G();
 void ensure_tables_host();
 int ensure_device(std::string &err);  // device + uploads (tables, conv grid, refl sets)
+void ensure_xilltable_host();
+int ensure_xilltable_device(std::string &err);
 void fill_dev_views(BatchDev &bd);
 void set_last(int status, const std::string &msg);
 int last_status();

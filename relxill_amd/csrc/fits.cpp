@@ -42,7 +42,7 @@ struct Fits {
   std::string err;
 };
 
-constexpr int kReadOnly = 0, kBinaryTbl = 2, kCaseInsen = 0, kTFloat = 42;
+constexpr int kReadOnly = 0, kBinaryTbl = 2, kCaseInsen = 0, kTFloat = 42, kTInt = 31, kTString = 16;
 
 // Candidate locations: RELXILL_CFITSIO_LIB, HEASoft's $HEADAS/lib (where an
 // XSPEC installation keeps it), the loader's search path, $CONDA_PREFIX/lib.
@@ -133,6 +133,21 @@ struct FitsFile {
     float nul = 0.0f;
     int anynul = 0;
     if (ok() && f.read_col(fp, kTFloat, col, row, first, n, &nul, dst, &anynul, &st)) fail("cannot read a column");
+  }
+  void read_int(int col, long row, long n, int *dst) {
+    int nul = 0, anynul = 0;
+    if (ok() && f.read_col(fp, kTInt, col, row, 1, n, &nul, dst, &anynul, &st)) fail("cannot read an integer column");
+  }
+  // one string cell (at most 71 characters), trailing blanks removed
+  std::string read_str(int col, long row) {
+    char buf[72] = {0};
+    char *cell[1] = {buf};
+    char nul[2] = " ";
+    int anynul = 0;
+    if (ok() && f.read_col(fp, kTString, col, row, 1, 1, nul, cell, &anynul, &st)) fail("cannot read a string column");
+    std::string v(buf);
+    while (!v.empty() && v.back() == ' ') v.pop_back();
+    return v;
   }
 };
 
@@ -247,6 +262,68 @@ std::string load_lptable_fits(LpTableHost &t, const std::string &path) {
   for (float &x : out.del_inc) x = fabsf(x);
   std::string v = validate_lptable(out);
   if (!v.empty()) return "lamp-post table " + path + ": " + v;
+  t = std::move(out);
+  return "";
+}
+
+// Parameter names of xillver tables and their ids (src/common.h:141-161,
+// src/xilltable.c:24-27; kTe shares the id of Ecut, A_CO that of A_Fe).
+static int xill_param_id(const std::string &name) {
+  static const struct {
+    const char *name;
+    int id;
+  } kNames[] = {{"Gamma", XP_GAM}, {"A_Fe", XP_AFE}, {"logXi", XP_LXI}, {"Ecut", XP_ECT}, {"kTe", XP_ECT},
+                {"Dens", XP_DNS}, {"kTbb", XP_KTB}, {"A_CO", XP_AFE}, {"Frac", XP_FRA}, {"Incl", XP_INC}};
+  for (const auto &n : kNames)
+    if (name == n.name) return n.id;
+  return -1;
+}
+
+std::string load_xilltable_fits(XillTableHost &t, const std::string &path) {
+  Fits &f = fits_lib();
+  if (!f.err.empty()) return f.err;
+  FitsFile ff(f, path);
+  XillTableHost out;
+  // PARAMETERS: one row per parameter; the column numbers the reference reads
+  // (src/xilltable.c:182-183, :205-206, :219-220)
+  ff.move_named("PARAMETERS");
+  const long n_par = ff.rows();
+  if (ff.ok() && n_par != XILL_NPAR) return "xillver table " + path + ": only five-parameter tables are supported";
+  out.n_par = (int)n_par;
+  if (ff.ok()) ff.read_int(9, 1, n_par, out.n_val);
+  for (int k = 0; k < out.n_par && ff.ok(); k++) {
+    const std::string name = ff.read_str(1, k + 1);
+    out.pid[k] = xill_param_id(name);
+    if (ff.ok() && out.pid[k] < 0) ff.fail("unknown parameter " + name);
+    if (ff.ok() && (out.n_val[k] < 2 || out.n_val[k] > ff.repeat(10))) ff.fail("parameter " + name + " has a bad NUMBVALS");
+    if (!ff.ok()) break;
+    out.val[k].resize(out.n_val[k]);
+    ff.read(10, k + 1, 1, out.n_val[k], out.val[k].data());
+  }
+  // ENERGIES: ENERG_LO, ENERG_HI (src/xilltable.c:240-276)
+  ff.move_named("ENERGIES");
+  const long n_e = ff.rows();
+  if (ff.ok()) {
+    out.n_ener = (int)n_e;
+    out.elo.resize(n_e);
+    out.ehi.resize(n_e);
+    ff.read(1, 1, 1, n_e, out.elo.data());
+    ff.read(2, 1, 1, n_e, out.ehi.data());
+  }
+  // SPECTRA: row r + 1 holds spectrum r in column 2 (src/xilltable.c:532-545)
+  ff.move_named("SPECTRA");
+  if (ff.ok()) {
+    const size_t rows = out.n_rows();
+    if ((size_t)ff.rows() < rows || ff.repeat(2) < n_e) {
+      ff.fail("SPECTRA has fewer rows or bins than the parameter grid needs");
+    } else {
+      out.spec.resize(rows * (size_t)n_e);
+      for (size_t r = 0; r < rows && ff.ok(); r++) ff.read(2, (long)r + 1, 1, n_e, &out.spec[r * n_e]);
+    }
+  }
+  if (!ff.ok()) return "xillver table " + path + ": " + ff.err;
+  std::string v = validate_xilltable(out);
+  if (!v.empty()) return "xillver table " + path + ": " + v;
   t = std::move(out);
   return "";
 }

@@ -31,6 +31,7 @@ namespace {
 
 constexpr int SETUP_THREADS = 256;
 constexpr int CONV_THREADS = 256;      // smallest convolution workgroup
+constexpr int XILL_THREADS = 256;
 constexpr int CONV_THREADS_MAX = 1024;  // n/8 threads: 512 for 4096 bins, 1024 for 8192
 
 __device__ inline double lp_ifac(float value, const float *arr, int n, int &ind) {
@@ -1166,20 +1167,26 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
   const int n = cg.n;
   const int m = n >> 1;
   double *out = b.out + rq.flux_off;
+  // output grid: the caller's (for relxill already redshifted on the host)
+  const double *eo = b.ener_in + rq.ener_off;
+  const int n_o = rq.n_in;
   if (b.status[req] != 0) {
-    for (int j = tid; j < rq.n_in; j += NT) out[j] = 0.0;
+    for (int j = tid; j < n_o; j += NT) out[j] = 0.0;
     return;
   }
   const double *Ec = cg.ener;
   const double *cf = cg.cfac;
   const double *prof = b.prof + rq.prof_off;
-  const double *ein = b.ener_in + rq.ener_off;
-  const double *fin = b.flux_in + rq.flux_off;
   const double2 *tw = reinterpret_cast<const double2 *>(cg.tw);
   const bool angular = (rq.model == MOD_RELXILL_CONV);
+  const bool relxill = (rq.model == MOD_RELXILL);
+  // input: the caller's spectrum, or for relxill the angle-weighted xillver
+  // spectrum on the table grid (k_xill)
+  const double *ein = relxill ? b.xt.ener : eo;
+  const double *fin = relxill ? b.xspec + rq.xspec_off : b.flux_in + rq.flux_off;
+  const int n_in = relxill ? b.xt.n_ener : n_o;
   const double *dist = b.dist + (size_t)req * MAX_INCL;
   const DevReflSet *rs = angular ? &b.refl_sets[rq.refl_set] : nullptr;
-  const int n_in = rq.n_in;
   auto ge_in = [&](int k) { return ein[k]; };
 
   // 1. x = rebinned input * E/dE (real part), y = rotated profile * E/dE (imag part)
@@ -1280,18 +1287,28 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
   block_sum2(sx, sph, s2, s_w);
   double sum_xill = s2[0];
   const double f = rq.do_renorm ? rq.norm / s2[1] : 1.0;
-  double sr = 0.0;
+  double sr = 0.0, sa = 0.0;
 #pragma unroll
   for (int q = 0; q < 8; q++) {
     const int i = tid + q * NT;
     if (q < per) {
       const double pv = ph[q] * f;
       if (in_band(cg, i)) sr += pv;
+      sa += pv;
       const int irot = (i - cg.i1kev + n) & (n - 1);
       s_buf[P(irot)].y = pv * cf[i];
     }
   }
-  const double sum_rel = block_sum4(sr, s_w[0]);  // (barrier inside: the buffer is complete)
+  // (barrier inside: the buffer is complete)
+  double sum_rel, sum_prof = 1.0;
+  if (angular || relxill) {
+    double s3[2];
+    block_sum2(sr, sa, s3, s_w);
+    sum_rel = s3[0];
+    sum_prof = s3[1];
+  } else {
+    sum_rel = block_sum4(sr, s_w[0]);
+  }
   if (angular) {
     sum_xill = 0.0;
     for (int k = 0; k < rs->n_incl; k++) sum_xill += dist[k] * rs->bandsum[k];
@@ -1359,7 +1376,42 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
   const double sum_conv = block_sum4(sc, s_w[0]);
   // normalizeFFTOutput (src/Relbase.cpp:199-222): applied to the rebinned
   // values below (the rebin is linear), which saves a pass over the buffer
-  const double fnorm = sum_rel * sum_xill / sum_conv;
+  double fnorm = sum_rel * sum_xill / sum_conv;
+  // a zone whose profile sums below 1e-12 is left out (src/Relxill.cpp:453-456)
+  const bool zone_used = !(sum_prof < 1e-12);
+  if (angular && !zone_used) fnorm = 0.0;
+  if (relxill) {
+    // add_primary_spectrum for the broken power law (src/PrimarySource.cpp:66-125):
+    // reflection times |refl_frac|, plus the cut-off power law (spec_cutoffpl,
+    // src/Xillspec.cpp:215-241) normalised as the xillver incident spectrum
+    // (calcNormWrtXillverTableSpec on the 500-bin 0.1-1000 keV grid,
+    // src/Xillspec.cpp:179-205, src/PrimarySource.h:279-292) for refl_frac >= 0
+    const double ecut = rq.xect, gam = rq.xgam;
+    const double e1c = exp(1.0 / ecut);
+    double sp = 0.0;
+    for (int k = tid; k < N_ENER_COARSE; k += NT) {
+      const double e0 = cg.coarse[k], e1 = cg.coarse[k + 1];
+      if (e0 >= 0.1 && e0 <= 1000.0) {
+        const double en = 0.5 * (e0 + e1);
+        const double fl = e1c * pow(en, -gam) * exp(-en / ecut) * (e1 - e0);
+        sp += fl * 0.5 * (e0 + e1) * 1e20 * 1.602177e-09;
+      }
+    }
+    const double nf = 1. / (block_sum4(sp, s_w[0]) / (1e15 / 4.0 / M_PI));
+    const double arf = fabs(rq.refl_frac);
+    const bool add_prim = rq.refl_frac >= 0;
+#pragma unroll 8
+    for (int i = tid; i < n; i += NT) {
+      double v = zone_used ? (o_at(i) * fnorm) * arf : 0.0;
+      if (add_prim) {
+        const double e0 = Ec[i], e1 = Ec[i + 1];
+        const double en = 0.5 * (e0 + e1);
+        v += (e1c * pow(en, -gam) * exp(-en / ecut) * (e1 - e0)) * nf;
+      }
+      o_at(i) = v;
+    }
+    fnorm = 1.0;
+  }
   // the conv grid edges go to the upper part of the buffer, free after the
   // half-length FFT (its values occupy slots below P(m))
   double *s_e = reinterpret_cast<double *>(s_buf + P(m));
@@ -1369,20 +1421,79 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
   // 5. rebin onto the caller grid; relconv zeroes outside 0.01-1000 keV
   const GridGuess gc = grid_guess(Ec, n);
   auto ge_c = [&](int k) { return s_e[k]; };
-  const int nbo = (n_in + nwave - 1) / nwave;
-  const int jend = min(n_in, (wave + 1) * nbo);
+  const int nbo = (n_o + nwave - 1) / nwave;
+  const int jend = min(n_o, (wave + 1) * nbo);
+  const bool relconv = !angular && !relxill;
 #pragma unroll 2
   for (int q = 0; q < nbo; q += 64) {
     const int jb = wave * nbo + q + lane;
     if (jb < jend) {
-      const double elo = ein[jb], ehi = ein[jb + 1];
+      const double elo = eo[jb], ehi = eo[jb + 1];
       const int clo = count_le(ge_c, n, elo, gc);
       const int chi = count_le(ge_c, n, ehi, gc);
       double v = fnorm * rebin_bin(elo, ehi, clo, chi, ge_c, n, [&](int i) { return o_at(i); });
-      if (!angular && (ehi < 0.01 || elo > 1000.0)) v = 0.0;
+      if (relconv && (ehi < 0.01 || elo > 1000.0)) v = 0.0;
       out[jb] = v;
     }
   }
+}
+
+// relxill: the xillver spectrum of the request's parameters for every
+// emission angle (interp_5d_tab_incl, src/xilltable.c:812-876: 16 corners of
+// the four interpolated parameters, in the reference's order and with its
+// weight products) weighted by the angle distribution (calc_xillver_angdep,
+// src/Xillspec.cpp:557-573).  One thread per bin of the table grid; every
+// table read is coalesced across the block.  Angles with zero weight are
+// skipped (they add exactly 0).
+__global__ __launch_bounds__(XILL_THREADS) void k_xill(BatchDev b, const int *__restrict__ reqs, int chunks) {
+  const int req = reqs[blockIdx.x / chunks];
+  if (b.status[req] != 0) return;
+  const DevXillTable xt = b.xt;
+  const int e = (blockIdx.x % chunks) * XILL_THREADS + threadIdx.x;
+  if (e >= xt.n_ener) return;
+  const ReqParams &rq = b.req[req];
+  int i0[4];
+  double f[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    i0[k] = rq.xind[k];
+    f[k] = rq.xfac[k];
+  }
+  // corner c has the offsets of bits 3, 2, 1, 0 of kCorner[c] on axes 0..3
+  constexpr unsigned char kCorner[16] = {0x0, 0x8, 0x4, 0x2, 0xC, 0xA, 0x6, 0xE,
+                                         0x1, 0x9, 0x5, 0x3, 0xD, 0xB, 0x7, 0xF};
+  double w[16];
+  size_t row[16];
+#pragma unroll
+  for (int c = 0; c < 16; c++) {
+    int o[4];
+    double fk[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      o[k] = (kCorner[c] >> (3 - k)) & 1;
+      fk[k] = o[k] ? f[k] : (1.0 - f[k]);
+    }
+    w[c] = fk[0] * fk[1] * fk[2] * fk[3];
+    const size_t r = (((size_t)(i0[0] + o[0]) * xt.n_val[1] + (i0[1] + o[1])) * xt.n_val[2] + (i0[2] + o[2])) *
+                         xt.n_val[3] + (i0[3] + o[3]);
+    row[c] = r * xt.n_incl;
+  }
+  const double *dist = b.dist + (size_t)req * MAX_INCL;
+  const float *spec = xt.spec + e;
+  const size_t ne = (size_t)xt.n_ener;
+  double acc = 0.0;
+  for (int a = 0; a < xt.n_incl; a++) {
+    const double d = dist[a];
+    if (d == 0.0) continue;
+    float t[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) t[c] = spec[(row[c] + a) * ne];
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < 16; c++) s += w[c] * (double)t[c];
+    acc += d * s;
+  }
+  b.xspec[rq.xspec_off + e] = acc;
 }
 
 // Resident reflection set: per angle, rebin onto the convolution grid, band
@@ -1485,6 +1596,14 @@ int launch_conv(const BatchDev &b, const int *conv_reqs, int n_conv_reqs, void *
   const size_t lds = conv_lds(b.cg.n);
   (void)hipFuncSetAttribute((const void *)k_conv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL(k_conv, dim3(n_conv_reqs), dim3(conv_threads(b.cg.n)), lds, (hipStream_t)stream, b, conv_reqs);
+  return (int)hipGetLastError();
+}
+
+int launch_xill(const BatchDev &b, const int *xill_reqs, int n_xill_reqs, void *stream) {
+  if (n_xill_reqs <= 0) return 0;
+  const int chunks = (b.xt.n_ener + XILL_THREADS - 1) / XILL_THREADS;
+  hipLaunchKernelGGL(k_xill, dim3(n_xill_reqs * chunks), dim3(XILL_THREADS), 0, (hipStream_t)stream, b, xill_reqs,
+                     chunks);
   return (int)hipGetLastError();
 }
 

@@ -18,6 +18,8 @@
 //   items               work list written by k_setup: (request, chunk of 64
 //                       fine radii, segment of at most seg_bins bins)
 //   dist[n][n_incl]     emission-angle distribution (relxill-style requests)
+//   xspec               relxill requests: the angle-weighted xillver spectrum
+//                       (n_ener of the xillver table each), written by k_xill
 #pragma once
 #include <stdint.h>
 
@@ -34,7 +36,8 @@ enum ModelKind : int {
   MOD_RELLINELP = 2,
   MOD_RELCONV = 11,
   MOD_RELCONVLP = 12,
-  MOD_RELXILL_CONV = -1,  // relxill-style: angle-resolved reflection, one zone
+  MOD_RELXILL = -1,          // relxill: broken power law, one zone, xillver table
+  MOD_RELXILL_CONV = -1001,  // relxill-style conv of a resident reflection set
 };
 
 enum EmisKind : int { EMIS_BKN = 1, EMIS_LP = 2 };
@@ -57,6 +60,13 @@ struct ReqParams {
   int64_t prof_off;   // profile (n_e) for conv models, -1 otherwise
   int refl_set;       // relxill-style: which resident reflection set
   int pre_status;     // host-side parameter status; nonzero requests are skipped
+  // relxill: xillver corner (lower indices of the four interpolated table
+  // parameters, in table order) and interpolation factors, the primary
+  // continuum and the offset of the request's xillver spectrum in xspec
+  int xind[4];
+  double xfac[4];
+  double xgam, xect, refl_frac;
+  int64_t xspec_off;
 };
 
 // One fine radius (64 bytes).  The bin integral at this radius is
@@ -105,6 +115,17 @@ struct DevReflSet {
   const double *fft;     // [n_incl][n_conv/2+1][2]
 };
 
+// Resident xillver table: the spectra divided by 10^logXi at load time,
+// row ((((i0 n1 + i1) n2 + i2) n3 + i3) n_incl + ia) at spec[row * n_ener].
+struct DevXillTable {
+  int n_ener, n_incl;
+  int n_val[4];         // values of the four interpolated parameters
+  const float *spec;
+  const double *ener;   // n_ener + 1 edges (elo, then the last ehi)
+};
+
+constexpr int N_ENER_COARSE = 500;  // primary-continuum normalisation grid (src/Xillspec.h:32)
+
 struct DevConvGrid {
   int n;               // bins (power of two)
   int i1kev;           // binary_search(ener, n+1, 1.0)
@@ -116,6 +137,7 @@ struct DevConvGrid {
   const double *tw;    // twiddles exp(-2 pi i k/n), k < n/2, interleaved re/im
   const double *twp_fwd;  // per-pass twiddles of the n-point FFT ([r][k] per pass)
   const double *twp_inv;  // per-pass twiddles of the n/2-point FFT
+  const double *coarse;   // N_ENER_COARSE + 1 log edges over 0.1-1000 keV
 };
 
 // Fine radii are integrated in chunks of one wavefront: lane l of chunk c
@@ -192,6 +214,8 @@ struct BatchDev {
   DevLpTable lp;
   DevConvGrid cg;
   const DevReflSet *refl_sets;
+  DevXillTable xt;
+  double *xspec;
 };
 
 // ---- launchers (kernels.hip) ----
@@ -200,6 +224,7 @@ int launch_integrate(const BatchDev &b, int n_blocks, bool limb, void *stream);
 int launch_fixup(const BatchDev &b, bool cos_rows, void *stream);
 int launch_finalize(const BatchDev &b, void *stream);
 int launch_conv(const BatchDev &b, const int *conv_reqs, int n_conv_reqs, void *stream);
+int launch_xill(const BatchDev &b, const int *xill_reqs, int n_xill_reqs, void *stream);
 int launch_copy(const void *src, void *dst, size_t bytes, void *stream);
 int launch_refl_prepare(const DevConvGrid &cg, const double *refl_ener, const double *refl,
                         int n_refl, int n_incl, double *spec, double *bandsum, double *fft,

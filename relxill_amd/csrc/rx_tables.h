@@ -51,4 +51,52 @@ std::string load_lptable_fits(LpTableHost &t, const std::string &path);
 std::string validate_reltable(const RelTableHost &t);
 std::string validate_lptable(const LpTableHost &t);
 
+// xillver reflection table (the xillTable of src/xilltable.h, five-parameter
+// tables as xillver-a-Ec5.fits): n_par parameters, the last one the
+// inclination; parameter k has the id pid[k] (PARAM_* of src/common.h:141-149)
+// and n_val[k] ascending values in val[k]; the energy grid has n_ener bins
+// [elo, ehi]; spec holds row r = ((((i0 n1 + i1) n2 + i2) n3 + i3) n4 + i4) at
+// spec[r * n_ener] (row number r + 1 of the SPECTRA extension,
+// src/xilltable.c:326-337), as stored in the file (not yet divided by
+// 10^logXi).
+enum XillParamId : int {
+  XP_GAM = 0, XP_AFE = 1, XP_LXI = 2, XP_ECT = 3, XP_DNS = 4, XP_KTB = 5, XP_FRA = 6, XP_INC = 7
+};
+constexpr int XILL_NPAR = 5;
+
+struct XillTableHost {
+  int n_par = 0;
+  int pid[XILL_NPAR] = {};
+  int n_val[XILL_NPAR] = {};
+  std::vector<float> val[XILL_NPAR];
+  int n_ener = 0;
+  std::vector<float> elo, ehi;
+  std::vector<float> spec;
+  size_t n_rows() const {
+    size_t n = 1;
+    for (int k = 0; k < n_par; k++) n *= (size_t)n_val[k];
+    return n;
+  }
+  int n_incl() const { return n_par ? n_val[n_par - 1] : 0; }
+  int dim_of(int id) const {
+    for (int k = 0; k < n_par; k++)
+      if (pid[k] == id) return k;
+    return -1;
+  }
+};
+
+// Seeded synthetic table with the parameters Gamma, A_Fe, logXi, Ecut, Incl
+// (in that order) and n_ener log bins over 0.07-1000 keV.
+XillTableHost make_synthetic_xilltable(uint64_t seed, int n_gam, int n_afe, int n_lxi, int n_ect,
+                                       int n_incl, int n_ener);
+std::string validate_xilltable(const XillTableHost &t);
+// The spectra divided by 10^logXi of their row and, for a table with a
+// density axis, by 10^(logN - 15) (renorm_xill_spec, src/xilltable.c:478-511,
+// applied when a spectrum is loaded, :513-566).
+std::vector<float> xilltable_renormed(const XillTableHost &t);
+// xillver-a-Ec5.fits layout: extensions PARAMETERS (NAME in column 1,
+// NUMBVALS in column 9, VALUE in column 10), ENERGIES (ENERG_LO, ENERG_HI),
+// SPECTRA (spectrum in column 2), src/xilltable.c:169-277, :513-566.
+std::string load_xilltable_fits(XillTableHost &t, const std::string &path);
+
 }  // namespace rx

@@ -59,7 +59,8 @@ struct Slot {
 
 int64_t request_bytes(const Pending &p) {
   Global &g = G();
-  const bool conv = (p.model == MOD_RELCONV || p.model == MOD_RELCONVLP || p.model == MOD_RELXILL_CONV);
+  const bool conv = (p.model == MOD_RELCONV || p.model == MOD_RELCONVLP || p.model == MOD_RELXILL_CONV ||
+                     p.model == MOD_RELXILL);
   const int64_t n_in = (int64_t)p.energy.size() - 1;
   const int64_t n_e = conv ? g.conv_n : n_in;
   const int64_t n_rc = (g.nr_fine + RAD_CHUNK - 1) / RAD_CHUNK;
@@ -70,6 +71,7 @@ int64_t request_bytes(const Pending &p) {
   b += (int64_t)sizeof(RadRec) * g.nr_fine;
   b += 8 * 2 * (int64_t)g.rel.n_r * g.rel.n_g * 2;
   b += 8 * MAX_INCL + sizeof(LpRefl) + sizeof(ReqNorm) + 64;
+  if (p.model == MOD_RELXILL) b += 8 * (int64_t)g.xill.n_ener + 4;  // xillver spectrum, request index
   return b;
 }
 

@@ -257,6 +257,150 @@ std::string validate_lptable(const LpTableHost &t) {
   return "";
 }
 
+// ---- xillver reflection table ----
+
+namespace {
+// values of one synthetic parameter axis: n points on [lo, hi], linear or log
+std::vector<float> axis(int n, double lo, double hi, bool log_spaced) {
+  std::vector<float> v(n);
+  for (int i = 0; i < n; i++) {
+    const double t = (n > 1) ? (double)i / (n - 1) : 0.0;
+    v[i] = (float)(log_spaced ? lo * std::pow(hi / lo, t) : lo + (hi - lo) * t);
+  }
+  return v;
+}
+}  // namespace
+
+// Synthetic xillver spectra (photons per bin): a cut-off power law
+// E^(1-Gamma) exp(-E/Ecut) dE with a seeded smooth soft-band modulation that
+// changes with logXi, a limb-brightening factor in the emission angle and a
+// Gaussian Fe K line at 6.4 keV whose strength scales with A_Fe; every row is
+// multiplied by 10^logXi (the scale in which xillver tables store spectra).
+// Inclinations are the centres of n_incl equal bins in cos(i), as the angle
+// bins of the relativistic profile (src/Relprofile.cpp:799-801).
+XillTableHost make_synthetic_xilltable(uint64_t seed, int n_gam, int n_afe, int n_lxi, int n_ect,
+                                       int n_incl, int n_ener) {
+  XillTableHost t;
+  t.n_par = XILL_NPAR;
+  const int ids[XILL_NPAR] = {XP_GAM, XP_AFE, XP_LXI, XP_ECT, XP_INC};
+  const int nv[XILL_NPAR] = {n_gam, n_afe, n_lxi, n_ect, n_incl};
+  for (int k = 0; k < XILL_NPAR; k++) {
+    t.pid[k] = ids[k];
+    t.n_val[k] = nv[k];
+  }
+  t.val[0] = axis(n_gam, 1.0, 3.4, false);
+  t.val[1] = axis(n_afe, 0.5, 10.0, true);
+  t.val[2] = axis(n_lxi, 0.0, 4.7, false);
+  t.val[3] = axis(n_ect, 5.0, 1000.0, true);
+  t.val[4].resize(n_incl);
+  for (int i = 0; i < n_incl; i++) t.val[4][i] = (float)(std::acos(1.0 - (i + 0.5) / n_incl) * 180.0 / kPi);
+  t.n_ener = n_ener;
+  t.elo.resize(n_ener);
+  t.ehi.resize(n_ener);
+  std::vector<double> lne(n_ener), emid(n_ener), de(n_ener), line(n_ener), soft(n_ener);
+  SplitMix rng(seed);
+  Smooth s_soft(rng), s_pl(rng);
+  for (int e = 0; e < n_ener; e++) {
+    const double lo = 0.07 * std::pow(1000.0 / 0.07, (double)e / n_ener);
+    const double hi = 0.07 * std::pow(1000.0 / 0.07, (double)(e + 1) / n_ener);
+    t.elo[e] = (float)lo;
+    t.ehi[e] = (float)hi;
+    emid[e] = 0.5 * (lo + hi);
+    de[e] = hi - lo;
+    lne[e] = std::log(emid[e]);
+    const double x = (lne[e] - std::log(6.4)) / 0.03;  // sigma 3% of 6.4 keV
+    line[e] = 3.0 * std::exp(-0.5 * x * x);
+    soft[e] = std::exp(-emid[e] / 2.0);
+  }
+  t.spec.resize(t.n_rows() * (size_t)n_ener);
+  std::vector<double> base(n_ener);
+  size_t row = 0;
+  for (int ig = 0; ig < n_gam; ig++) {
+    const double gam = t.val[0][ig];
+    for (int ia = 0; ia < n_afe; ia++) {
+      const double afe = t.val[1][ia];
+      for (int ix = 0; ix < n_lxi; ix++) {
+        const double lxi = t.val[2][ix];
+        const double scale = std::pow(10.0, lxi);
+        for (int ic = 0; ic < n_ect; ic++) {
+          const double ect = t.val[3][ic];
+          for (int e = 0; e < n_ener; e++) {
+            const double u = lne[e] / std::log(1000.0);
+            base[e] = (1.0 - gam) * lne[e] - emid[e] / ect + 0.08 * s_pl(u, gam / 3.4) +
+                      0.6 * (lxi / 4.7) * soft[e] * (1.0 + 0.5 * s_soft(u, lxi / 4.7));
+          }
+          for (int ii = 0; ii < n_incl; ii++, row++) {
+            const double mu = std::cos(t.val[4][ii] * kPi / 180.0);
+            const double limb = std::log(0.35 + 0.65 * mu);
+            float *dst = &t.spec[row * n_ener];
+            for (int e = 0; e < n_ener; e++) {
+              const double v = scale * de[e] * std::exp(base[e] + limb * (1.0 + 0.05 * lne[e])) *
+                               (1.0 + afe * line[e] * (0.6 + 0.4 * mu));
+              dst[e] = (float)v;
+            }
+          }
+        }
+      }
+    }
+  }
+  return t;
+}
+
+std::string validate_xilltable(const XillTableHost &t) {
+  if (t.n_par != XILL_NPAR) return "xillver table must have five parameters";
+  if (t.pid[XILL_NPAR - 1] != XP_INC) return "the last xillver parameter must be the inclination";
+  bool seen[8] = {};
+  for (int k = 0; k < t.n_par; k++) {
+    if (t.pid[k] < 0 || t.pid[k] > XP_INC || seen[t.pid[k]]) return "xillver parameter ids invalid or repeated";
+    seen[t.pid[k]] = true;
+    if (t.n_val[k] < 2 || (int)t.val[k].size() != t.n_val[k]) return "xillver parameter axis too short";
+    for (int i = 0; i + 1 < t.n_val[k]; i++)
+      if (!(t.val[k][i] < t.val[k][i + 1])) return "xillver parameter values not ascending";
+  }
+  if (!seen[XP_LXI]) return "xillver table without a logXi axis";
+  if (t.n_ener < 2 || (int)t.elo.size() != t.n_ener || (int)t.ehi.size() != t.n_ener) return "xillver energy grid invalid";
+  for (int e = 0; e < t.n_ener; e++) {
+    if (!(t.elo[e] > 0.0f) || !(t.ehi[e] > t.elo[e])) return "xillver energy bins invalid";
+    if (e + 1 < t.n_ener && !(t.elo[e + 1] >= t.ehi[e])) return "xillver energy bins overlap";
+  }
+  if (t.spec.size() != t.n_rows() * (size_t)t.n_ener) return "xillver spectra size mismatch";
+  for (float v : t.spec)
+    if (!(v >= 0.0f) || !std::isfinite(v)) return "xillver spectrum negative or not finite";
+  return "";
+}
+
+// renorm_xill_spec (src/xilltable.c:478-486): each division is rounded to
+// float (the spectrum is a float array); the logXi and density of a row are
+// the table values of its indices (normalizeXillverSpecLogxiDensity, :488-511);
+// a table without a density axis uses the default 15, which skips that step.
+std::vector<float> xilltable_renormed(const XillTableHost &t) {
+  std::vector<float> out(t.spec.size());
+  const int d_lxi = t.dim_of(XP_LXI), d_dns = t.dim_of(XP_DNS);
+  const size_t n_rows = t.n_rows();
+  for (size_t row = 0; row < n_rows; row++) {
+    size_t rest = row;
+    int idx[XILL_NPAR] = {};
+    for (int k = t.n_par - 1; k >= 0; k--) {
+      idx[k] = (int)(rest % t.n_val[k]);
+      rest /= t.n_val[k];
+    }
+    const double lxi = t.val[d_lxi][idx[d_lxi]];
+    const double dens = d_dns >= 0 ? (double)t.val[d_dns][idx[d_dns]] : 15.0;
+    const double f_lxi = std::pow(10, lxi);
+    const bool do_dens = std::fabs(dens - 15) > 1e-6;
+    const double f_dens = do_dens ? std::pow(10, dens - 15) : 1.0;
+    const float *src = &t.spec[row * t.n_ener];
+    float *dst = &out[row * t.n_ener];
+    for (int e = 0; e < t.n_ener; e++) {
+      float v = src[e];
+      v = (float)(v / f_lxi);
+      if (do_dens) v = (float)(v / f_dens);
+      dst[e] = v;
+    }
+  }
+  return out;
+}
+
 // ---- RXTB binary format: magic, kind, dims, then the float arrays ----
 
 namespace {

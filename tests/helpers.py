@@ -310,18 +310,34 @@ def _fits_header(cards):
 
 
 def _fits_bintable(name, cols):
-    """cols: list of (column name, float array of shape (rows, repeat))."""
-    cols = [(c, np.asarray(v, dtype=">f4").reshape(len(v), -1)) for c, v in cols]
-    nrows = cols[0][1].shape[0]
-    assert all(v.shape[0] == nrows for _, v in cols)
-    rowbytes = sum(4 * v.shape[1] for _, v in cols)
+    """cols: list of (column name, array of shape (rows, repeat)[, tform]) with
+    tform "E" (float32, the default), "J" (int32) or "<w>A" (strings of width w)."""
+    prepared = []
+    for c in cols:
+        cname, v = c[0], c[1]
+        form = c[2] if len(c) > 2 else "E"
+        if form.endswith("A"):
+            w = int(form[:-1])
+            data = np.array([s.ljust(w)[:w].encode("ascii") for s in v], dtype=f"S{w}").reshape(len(v), 1)
+            prepared.append((cname, form, data, w))
+        else:
+            dt = ">f4" if form == "E" else ">i4"
+            a = np.asarray(v, dtype=dt).reshape(len(v), -1)
+            prepared.append((cname, f"{a.shape[1]}{form}", a, 4 * a.shape[1]))
+    nrows = prepared[0][2].shape[0]
+    assert all(d.shape[0] == nrows for _, _, d, _ in prepared)
+    rowbytes = sum(w for _, _, _, w in prepared)
     cards = [_fits_card("XTENSION", "BINTABLE"), _fits_card("BITPIX", 8), _fits_card("NAXIS", 2),
              _fits_card("NAXIS1", rowbytes), _fits_card("NAXIS2", nrows), _fits_card("PCOUNT", 0),
-             _fits_card("GCOUNT", 1), _fits_card("TFIELDS", len(cols))]
-    for i, (c, v) in enumerate(cols):
-        cards += [_fits_card(f"TTYPE{i + 1}", c), _fits_card(f"TFORM{i + 1}", f"{v.shape[1]}E")]
+             _fits_card("GCOUNT", 1), _fits_card("TFIELDS", len(prepared))]
+    for i, (cname, form, _, _) in enumerate(prepared):
+        cards += [_fits_card(f"TTYPE{i + 1}", cname), _fits_card(f"TFORM{i + 1}", form)]
     cards.append(_fits_card("EXTNAME", name))
-    data = np.concatenate([v for _, v in cols], axis=1).astype(">f4").tobytes()
+    rows = bytearray()
+    for r in range(nrows):
+        for _, _, d, _ in prepared:
+            rows += d[r].tobytes()
+    data = bytes(rows)
     return _fits_header(cards) + data + b"\0" * ((-len(data)) % 2880)
 
 
@@ -383,3 +399,79 @@ def draw_wide_params(model, rng):
                 rng.uniform(1, 3.4), float(rng.integers(0, 2))]
     return [rng.uniform(-10, 120), rng.uniform(-0.5, 1.0), a, incl, rin, rout, limb, rng.uniform(1, 3.4),
             float(rng.integers(0, 2))]
+
+
+XILL_NAMES = {0: "Gamma", 1: "A_Fe", 2: "logXi", 3: "Ecut", 4: "Dens", 7: "Incl"}
+
+
+def write_fits_xilltable(path, x):
+    """A xillver table (xilltable_view layout) as an OGIP table model file like
+    xillver-a-Ec5.fits: PARAMETERS (NAME, METHOD, INITIAL, DELTA, MINIMUM,
+    BOTTOM, TOP, MAXIMUM, NUMBVALS, VALUE), ENERGIES (ENERG_LO, ENERG_HI),
+    SPECTRA (PARAMVAL, INTPSPEC), the reader's column numbers 1, 9, 10 and 2."""
+    n_par, n_val = x["n_par"], list(x["n_val"])
+    nmax = max(n_val)
+    vals = np.zeros((n_par, nmax), dtype=np.float32)
+    for k in range(n_par):
+        vals[k, :n_val[k]] = x["val"][k]
+    first = np.array([x["val"][k][0] for k in range(n_par)], dtype=np.float32)
+    last = np.array([x["val"][k][-1] for k in range(n_par)], dtype=np.float32)
+    params = _fits_bintable("PARAMETERS", [
+        ("NAME", [XILL_NAMES[p] for p in x["pid"]], "12A"), ("METHOD", np.zeros(n_par), "J"),
+        ("INITIAL", first), ("DELTA", np.full(n_par, 0.01)), ("MINIMUM", first), ("BOTTOM", first),
+        ("TOP", last), ("MAXIMUM", last), ("NUMBVALS", n_val, "J"), ("VALUE", vals)])
+    energies = _fits_bintable("ENERGIES", [("ENERG_LO", x["elo"]), ("ENERG_HI", x["ehi"])])
+    spec = np.asarray(x["spec"], dtype=np.float32).reshape(-1, x["n_ener"])
+    grid = np.stack(np.meshgrid(*[x["val"][k] for k in range(n_par)], indexing="ij"), -1).reshape(-1, n_par)
+    spectra = _fits_bintable("SPECTRA", [("PARAMVAL", grid[:spec.shape[0]]), ("INTPSPEC", spec)])
+    with open(path, "wb") as f:
+        for blk in (_fits_primary(), params, energies, spectra):
+            f.write(blk)
+
+
+def xill_axes(n_gam=4, n_afe=3, n_lxi=5, n_ect=4, n_incl=6):
+    """Parameter axes of a small xillver table (Gamma, A_Fe, logXi, Ecut, Incl)."""
+    return [np.linspace(1.0, 3.4, n_gam), np.geomspace(0.5, 10.0, n_afe), np.linspace(0.0, 4.7, n_lxi),
+            np.geomspace(5.0, 1000.0, n_ect),
+            np.degrees(np.arccos(1.0 - (np.arange(n_incl) + 0.5) / n_incl))]
+
+
+AFFINE_XILL_C = (0.7, 0.21, 0.037, 0.083, 4.1e-4, 0.059)
+
+
+def affine_xilltable(n_ener=300, c=AFFINE_XILL_C, **dims):
+    """A xillver table whose stored spectra are 10^logXi times a shape g(E)
+    times c0 + c1 Gamma + c2 A_Fe + c3 logXi + c4 Ecut + c5 k (k the index of
+    the inclination): after the division by 10^logXi the values are affine in
+    the four interpolated parameters, so the interpolation reproduces them."""
+    ax = xill_axes(**dims)
+    e = np.geomspace(0.07, 1000.0, n_ener + 1)
+    g = 1e-3 * (0.5 * (e[1:] + e[:-1])) ** -1.0 * np.diff(e)
+    G, A, L, C, I = np.meshgrid(*ax, indexing="ij")
+    K = np.broadcast_to(np.arange(len(ax[4])), G.shape)
+    lin = c[0] + c[1] * G + c[2] * A + c[3] * L + c[4] * C + c[5] * K
+    spec = (10.0 ** L * lin)[..., None] * g
+    return dict(n_par=5, pid=[0, 1, 2, 3, 7], n_val=[len(a) for a in ax],
+                val=[np.asarray(a, dtype=np.float32) for a in ax], n_ener=n_ener,
+                elo=e[:-1].astype(np.float32), ehi=e[1:].astype(np.float32),
+                spec=spec.reshape(-1, n_ener).astype(np.float32)), g
+
+
+def affine_xill_expected(x, g, gam, afe, lxi, ect, c=AFFINE_XILL_C):
+    """[n_incl, n_ener] spectra the interpolation must give (parameters clamped
+    into the table)."""
+    v = x["val"]
+    clamp = lambda q, k: min(max(q, float(v[k][0])), float(v[k][-1]))
+    gam, afe, lxi, ect = clamp(gam, 0), clamp(afe, 1), clamp(lxi, 2), clamp(ect, 3)
+    k = np.arange(x["n_val"][4])
+    lin = c[0] + c[1] * gam + c[2] * afe + c[3] * lxi + c[4] * ect + c[5] * k
+    return lin[:, None] * g[None, :]
+
+
+def draw_relxill_params(rng):
+    """relxill parameters over the lmodel.dat ranges (refl_frac of both signs)."""
+    a = rng.uniform(-0.998, 0.998)
+    return [rng.uniform(-1, 6), rng.uniform(-1, 6), rng.uniform(2, 100), a, rng.uniform(5, 85),
+            rng.choice([-1.0, rng.uniform(1.0, 30.0)]), rng.uniform(50, 1000), rng.choice([0.0, rng.uniform(0, 2)]),
+            rng.uniform(1.0, 3.4), rng.uniform(0.0, 4.7), rng.uniform(0.5, 10.0), rng.uniform(5.0, 1000.0),
+            rng.choice([-1.0, rng.uniform(-5, 10)])]

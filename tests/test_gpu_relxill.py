@@ -1,0 +1,201 @@
+"""relxill on the GPU (k_setup angle distribution, k_xill table interpolation
+and angle weighting, k_conv convolution with the primary continuum) against
+the oracle: the reference's default parameters, a randomized sweep over the
+lmodel.dat ranges, parameters outside the table, both signs of refl_frac,
+redshift, a table read from an OGIP FITS file, the XSPEC entry point,
+batching with the other models and the serving loop."""
+import numpy as np
+import pytest
+
+import oracle
+import relxill_amd as rx
+from tests.helpers import (affine_xilltable, draw_relxill_params, log_grid, rel_err, tables,
+                           write_fits_xilltable)
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-8
+TOL32 = 1e-4
+XDIMS = (5, 7, 4, 6, 5, 10, 2999)  # seed, Gamma, A_Fe, logXi, Ecut, Incl, energy bins
+
+
+@pytest.fixture(scope="module")
+def relx():
+    _, _, T = tables()
+    assert rx.lib.rx_init(0) == 0, rx.lib.rx_last_error()
+    rx.xilltable_synthetic(*XDIMS)
+    xt = oracle.Xill(rx.xilltable_view())
+    yield T, xt
+    rx.xilltable_synthetic()
+
+
+def _gpu(params, e, precision, dist=False):
+    b = rx.Batch([dict(model=rx.MODEL_RELXILL, energy=e, params=params)])
+    b.run(precision)
+    b.sync()
+    y, st = b.output(0)
+    d = b.dist(0, 10) if dist else None
+    b.close()
+    return y, st, d
+
+
+def test_relxill_default_parameters(relx):
+    """lmodel.dat defaults (Index 3/3, Rbr 15, a 0.998, 30 deg, Rin ISCO,
+    Rout 400, z 0, Gamma 2, logxi 3.1, Afe 1, Ecut 300, refl_frac 3) on 3000
+    bins over 0.1-100 keV; the angle distribution too."""
+    T, xt = relx
+    p = rx.DEFAULTS[rx.MODEL_RELXILL]
+    e = log_grid(3000, 0.1, 100.0)
+    ref, dist_ref, st = oracle.relxill(T, xt, p, e)
+    assert st == 0
+    for prec, tol in ((rx.FP64, TOL64), (rx.FP32, TOL32)):
+        y, st, d = _gpu(p, e, prec, dist=True)
+        print(f"relxill default prec {prec}: rel {rel_err(y, ref):.3e} dist {rel_err(d, dist_ref):.3e}")
+        assert st == 0
+        assert rel_err(y, ref) < tol
+        assert rel_err(d, dist_ref) < tol
+
+
+def test_relxill_random_sweep(relx):
+    """12 parameter sets over the full ranges (both signs of refl_frac,
+    redshift, Rin beyond the ISCO, xillver parameters across the table), one
+    batch, FP64 and FP32."""
+    T, xt = relx
+    rng = np.random.default_rng(11)
+    e = log_grid(1500, 0.1, 100.0)
+    ps = [draw_relxill_params(rng) for _ in range(12)]
+    refs = []
+    for p in ps:
+        ref, _, st = oracle.relxill(T, xt, p, e)
+        refs.append((ref, st))
+    b = rx.Batch([dict(model=rx.MODEL_RELXILL, energy=e, params=p) for p in ps])
+    for prec, tol in ((rx.FP64, TOL64), (rx.FP32, TOL32)):
+        b.run(prec)
+        b.sync()
+        worst = 0.0
+        for i, (ref, st) in enumerate(refs):
+            y, st_g = b.output(i)
+            assert st_g == (0 if st == 0 else rx.RX_ERR_PARAM), (i, ps[i], st)
+            if st == 0:
+                worst = max(worst, rel_err(y, ref))
+        print(f"relxill sweep prec {prec}: worst rel {worst:.3e}")
+        assert worst < tol
+    b.close()
+
+
+@pytest.mark.parametrize("xpar", [(0.5, 0.1, -2.0, 2.0), (4.0, 30.0, 9.0, 5000.0), (1.0, 0.5, 0.0, 5.0)])
+def test_relxill_xillver_parameters_outside_the_table(relx, xpar):
+    """Gamma, A_Fe, logXi, Ecut below / above / on the table ends: the values
+    are clamped (resetInpvalsToBoundaries, logXi within 0-4.7), Ecut's factor
+    pinned to 0 / 1."""
+    T, xt = relx
+    gam, afe, lxi, ect = xpar
+    p = [3.0, 3.0, 15.0, 0.9, 45.0, -1.0, 400.0, 0.0, gam, lxi, afe, ect, 1.0]
+    e = log_grid(800, 0.1, 100.0)
+    ref, _, st = oracle.relxill(T, xt, p, e)
+    assert st == 0
+    y, st, _ = _gpu(p, e, rx.FP64)
+    assert st == 0
+    assert rel_err(y, ref) < TOL64
+
+
+@pytest.mark.parametrize("rf,z", [(-1.0, 0.0), (0.0, 0.0), (3.0, 0.0), (-2.5, 0.7), (1.0, 2.0)])
+def test_relxill_refl_frac_and_redshift(relx, rf, z):
+    T, xt = relx
+    p = [3.0, 3.0, 15.0, 0.5, 60.0, -1.0, 400.0, z, 1.9, 2.7, 2.0, 150.0, rf]
+    e = log_grid(1000, 0.1, 100.0)
+    ref, _, st = oracle.relxill(T, xt, p, e)
+    assert st == 0
+    for prec, tol in ((rx.FP64, TOL64), (rx.FP32, TOL32)):
+        y, st, _ = _gpu(p, e, prec)
+        assert st == 0
+        assert rel_err(y, ref) < tol, (rf, z, prec)
+
+
+def test_relxill_table_from_fits(relx, tmp_path):
+    """A table written as an OGIP file (xillver-a-Ec5 layout) and loaded with
+    rx_xilltable_load_fits: the GPU result equals the oracle on the same
+    table."""
+    T, _ = relx
+    x, _ = affine_xilltable(n_ener=500)
+    path = tmp_path / "xillver-a-Ec5.fits"
+    write_fits_xilltable(path, x)
+    try:
+        assert rx.lib.rx_xilltable_load_fits(str(path).encode()) == 0, rx.lib.rx_last_error()
+        xt = oracle.Xill(rx.xilltable_view())
+        e = log_grid(600, 0.1, 100.0)
+        for p in ([3.0, 3.0, 15.0, 0.998, 30.0, -1.0, 400.0, 0.0, 2.0, 3.1, 1.0, 300.0, -1.0],
+                  [2.0, 4.0, 8.0, -0.5, 70.0, 3.0, 300.0, 0.1, 2.9, 0.7, 6.0, 40.0, 2.0]):
+            ref, _, st = oracle.relxill(T, xt, p, e)
+            assert st == 0
+            y, st, _ = _gpu(p, e, rx.FP64)
+            assert st == 0
+            assert rel_err(y, ref) < TOL64
+    finally:
+        rx.xilltable_synthetic(*XDIMS)
+
+
+def test_lmodrelxill_entry_point(relx):
+    """The XSPEC entry point gives the FP64 result; rejected parameters give
+    zeros and a status (Rout <= Rin)."""
+    T, xt = relx
+    p = [3.0, 2.5, 20.0, 0.7, 40.0, -1.0, 400.0, 0.05, 2.2, 1.5, 3.0, 100.0, 2.0]
+    e = log_grid(700, 0.2, 80.0)
+    ref, _, st = oracle.relxill(T, xt, p, e)
+    y = rx.lmodrelxill(e, p)
+    assert rel_err(y, ref) < TOL64
+    bad = list(p)
+    bad[5], bad[6] = 50.0, 20.0
+    y = rx.lmodrelxill(e, bad)
+    assert not y.any() and rx.lib.rx_last_status() == rx.RX_ERR_PARAM
+
+
+def test_relxill_batched_with_other_models(relx):
+    """relxill requests batched with relline and relconv requests give what
+    they give alone (to the order of the FP64 atomic sums)."""
+    T, xt = relx
+    e = log_grid(900, 0.1, 100.0)
+    rng = np.random.default_rng(3)
+    px = [draw_relxill_params(rng) for _ in range(4)]
+    px = [p for p in px if oracle.relxill(T, xt, p, e)[2] == 0][:3]
+    alone = [_gpu(p, e, rx.FP64)[0] for p in px]
+    el = log_grid(500, 0.5, 10.0)
+    flux_in = np.diff(e) * (0.5 * (e[1:] + e[:-1])) ** -2.0
+    reqs = []
+    for p in px:
+        reqs.append(dict(model=rx.MODEL_RELLINE, energy=el, params=rx.DEFAULTS[rx.MODEL_RELLINE]))
+        reqs.append(dict(model=rx.MODEL_RELXILL, energy=e, params=p))
+        reqs.append(dict(model=rx.MODEL_RELCONV, energy=e, params=rx.DEFAULTS[rx.MODEL_RELCONV], flux_in=flux_in))
+    b = rx.Batch(reqs)
+    rx.lib.rx_profile_enable(1)
+    try:
+        b.run(rx.FP64)
+        b.sync()
+        ms = b.kernel_ms()
+    finally:
+        rx.lib.rx_profile_enable(0)
+    for k, y0 in enumerate(alone):
+        y, st = b.output(3 * k + 1)
+        assert st == 0
+        assert rel_err(y, y0) < 1e-12
+    b.close()
+    # setup, integrate, fixup, finalize, conv, xill
+    assert len(ms) == 6 and ms[4] > 0 and ms[5] > 0
+
+
+def test_relxill_through_the_server(relx):
+    """relxill requests admitted by the serving loop return the batch result."""
+    T, xt = relx
+    e = log_grid(1200, 0.1, 100.0)
+    p = [3.0, 3.0, 15.0, 0.9, 35.0, -1.0, 400.0, 0.0, 2.1, 3.0, 1.0, 200.0, 1.5]
+    y0, _, _ = _gpu(p, e, rx.FP32)
+    s = rx.Server(max_inflight=8, hbm_budget=1 << 30, n_streams=2, precision=rx.FP32)
+    try:
+        tickets = [s.submit(rx.MODEL_RELXILL, e, p) for _ in range(5)]
+        s.drain()
+        for t in tickets:
+            y, st, _ = s.poll(t)
+            assert st == 0
+            assert rel_err(y, y0) < 1e-12
+    finally:
+        s.stop()
