@@ -10,7 +10,9 @@ device path (setup, relline integration, renormalisation, FFT convolution,
 rebin) over the batch, FP32 with Kahan sums, inputs resident in HBM.
 
 Also reported: the plain FP64 path (one request per launch), the batched
-lamp-post relline (1024 requests, 3000 bins), the serving loop's p50/p99
+lamp-post relline (1024 requests, 3000 bins), the relxill-style convolution of
+a resident reflection set, relxill with the xillver table interpolated on the
+device (1024 requests), the serving loop's p50/p99
 latency under Poisson arrivals (relline/relconv/lamp-post mix 40/40/20), the
 maximum relative error of the FP32 path against the FP64 oracle, the dominant
 kernel's roofline figures and the FP64 CPU oracle timed on the host's cores.
@@ -121,6 +123,65 @@ def relxill_style(rx, steps):
     return dict(plain_fp64_ms_per_spectrum=float(np.mean(lat)) * 1e3,
                 batched_fp32_spectra_per_s=len(reqs) / dt32, batched_fp64_spectra_per_s=len(reqs) / dt64,
                 max_rel_err_fp32_vs_fp64=max(rel_err(o32[i], o64[i]) for i in range(len(reqs))))
+
+
+def relxill_batch(rx, steps, n_req=1024, seed=9, cpu=True):
+    """relxill (broken power law, one zone) on the default synthetic xillver
+    table (13 Gamma x 5 A_Fe x 8 logXi x 6 Ecut x 10 inclinations x 2999 bins):
+    a batch of parameter sets drawn over the lmodel.dat ranges, output on 3000
+    bins over 0.1-100 keV.  FP32 and FP64 batches, plain FP64 (one request per
+    launch), the k_xill roofline and the oracle on one host core."""
+    rx.xilltable_synthetic()
+    rng = np.random.default_rng(seed)
+    e_out = log_grid(3000, 0.1, 100.0)
+    reqs = []
+    for _ in range(n_req):
+        a = rng.uniform(-0.998, 0.998)
+        incl = rng.uniform(5.0, 85.0)
+        rin = rng.uniform(kerr_rms(a), 50.0)
+        reqs.append(dict(model=rx.MODEL_RELXILL, energy=e_out,
+                         params=[3.0, 3.0, 15.0, a, incl, rin, 400.0, 0.0, rng.uniform(1.0, 3.4),
+                                 rng.uniform(0.0, 4.7), rng.uniform(0.5, 10.0),
+                                 float(np.exp(rng.uniform(np.log(5.0), np.log(1000.0)))), 3.0]))
+    b = rx.Batch(reqs)
+    b.autotune(rx.FP32)
+    dt32, kms32 = timed_steps(rx, b, rx.FP32, max(3, steps // 2), 1)
+    o32 = [b.output(i)[0] for i in range(n_req)]
+    xv = rx.xilltable_view()
+    n_e, n_incl = xv["n_ener"], xv["n_val"][-1]
+    n_angles = sum(int(np.count_nonzero(b.dist(i, n_incl))) for i in range(n_req))
+    dt64, _ = timed_steps(rx, b, rx.FP64, 2, 1)
+    o64 = [b.output(i)[0] for i in range(n_req)]
+    b.close()
+    lat = []
+    for r in reqs[:8]:
+        pb = rx.Batch([r])
+        d, _ = timed_steps(rx, pb, rx.FP64, 3, 1)
+        lat.append(d)
+        pb.close()
+    # k_xill algorithmic bytes: per request and emission angle of nonzero
+    # weight, 16 table corners of n_e float bins read; the FP64 result written
+    alg = 16 * n_angles * n_e * 4 + n_req * 8 * n_e
+    t_x = kms32[5] * 1e-3
+    out = dict(n_requests=n_req, batched_fp32_spectra_per_s=n_req / dt32, batched_fp64_spectra_per_s=n_req / dt64,
+               plain_fp64_ms_per_spectrum=float(np.mean(lat)) * 1e3,
+               max_rel_err_fp32_vs_fp64=max(rel_err(o32[i], o64[i]) for i in range(n_req)),
+               kernel_ms={k: round(v, 4) for k, v in zip(KERNELS, kms32)},
+               mean_angles_used=n_angles / n_req,
+               k_xill_roofline=dict(bound="hbm", algorithmic_bytes=alg, achieved=alg / t_x / 1e9,
+                                    peak=HBM_PEAK_GBS, unit="GB/s", frac=alg / t_x / 1e9 / HBM_PEAK_GBS))
+    if cpu:
+        import oracle
+        rel, lp = rx.tables_view()
+        T = oracle.Tables(rel, lp)
+        xt = oracle.Xill(xv)
+        t0 = time.perf_counter()
+        n_cpu = 8
+        ref = [oracle.relxill(T, xt, r["params"], e_out)[0] for r in reqs[:n_cpu]]
+        t1 = time.perf_counter()
+        out["cpu_oracle_single_core_spectra_per_s"] = n_cpu / (t1 - t0)
+        out["max_rel_err_fp32_vs_oracle"] = max(rel_err(o32[i], ref[i]) for i in range(n_cpu))
+    return out
 
 
 def make_cfg2(n_req, seed=4):
@@ -243,6 +304,8 @@ def main():
                     help="fixed work shape 'seg_bins,waves_per_cu' instead of autotuning")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the headline batch (FP32), for profiler runs")
+    ap.add_argument("--relxill-only", action="store_true",
+                    help="only the relxill batch (for profiler runs)")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="run a plain copy kernel of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -264,6 +327,11 @@ def main():
         assert rx.lib.rx_copy_test(nbytes, 5, ms.ctypes.data_as(rx.capi._D)) == 0
         print(json.dumps({"copy_bytes": nbytes, "copy_ms": float(ms[0]),
                           "copy_GBps": 2 * nbytes / ms[0] / 1e6}))
+        return
+
+    if args.relxill_only:
+        r = relxill_batch(rx, args.steps, cpu=False)
+        print(json.dumps(r))
         return
 
     log("building cfg3 batch ...")
@@ -322,6 +390,9 @@ def main():
     # cfg1: relxill-style angle-resolved convolution
     cfg1 = relxill_style(rx, args.steps)
     log(f"cfg1 relxill-style: {cfg1}")
+    # relxill with the xillver table on the device
+    relxill = relxill_batch(rx, args.steps, cpu=not args.no_cpu)
+    log(f"relxill batch: {relxill}")
 
     serving = serving_high = None
     if not args.no_serve:
@@ -400,6 +471,7 @@ def main():
         "max_rel_err_fp32_vs_oracle": max_err_oracle,
         "lp_relline_batch": cfg2,
         "relxill_style_conv": cfg1,
+        "relxill_batch": relxill,
         "serving": serving,
         "serving_20k_qps": serving_high,
         "p99_latency_ms": serving["p99_ms"] if serving else None,

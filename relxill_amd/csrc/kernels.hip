@@ -165,6 +165,33 @@ __device__ inline int dpp_int(int v) {
   return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
 }
 
+template <int CTRL>
+__device__ inline int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+
+template <int CTRL>
+__device__ inline double dpp_d(double v) {
+  const int lo = dpp_i<CTRL>(__double2loint(v));
+  const int hi = dpp_i<CTRL>(__double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+
+__device__ inline double readlane_d(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+// Sum over the 64 lanes in a fixed order (pairs, quads, half rows, rows, then
+// the four rows); every lane must be active.
+__device__ inline double wave_sum(double v) {
+  v += dpp_d<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_d<0x141>(v);  // row_half_mirror
+  v += dpp_d<0x140>(v);  // row_mirror
+  return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+}
+
 // min / max over the 64 lanes (quad, half-row and row exchanges, then the rows);
 // every lane must be active
 __device__ inline int wave_min_i(int v) {
@@ -506,38 +533,71 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     b.refl[req] = o;
   }
 
-  // emission-angle distribution (:908-938)
+  // emission-angle distribution (:908-938).  Lane = fine radius, loop over
+  // the 2 n_g (g*, branch) columns: a radius' record is read once and its
+  // node rows are contiguous.  Neighbouring radii mostly fall into the same
+  // angle bin, so a wavefront whose valid lanes share the bin adds one wave
+  // sum to it; otherwise every lane adds its own term.
   if (rq.n_incl > 0) {
     for (int k = tid; k < rq.n_incl; k += SETUP_THREADS) s_dist[k] = 0.0;
     __syncthreads();
     const double *node = b.node + (size_t)req * b.node_stride;
     const size_t nq2 = (size_t)n_r * ng * 2;
+    const int cols = ng * 2;
     const double step = (1.0 - 2 * GFAC_H) / (ng - 1);
     const double gs0 = gstar_node(step, 0), gs1 = gstar_node(step, 1);
-    const int items = nr * ng * 2;
-    int derr = 0;
-    for (int it = tid; it < items; it += SETUP_THREADS) {
-      const int ii = it / (ng * 2);
-      const int q = it - ii * ng * 2;
-      const int g = q >> 1;
-      const RadRec r = rad[ii];
-      if (r.ielo > r.iehi) continue;
-      const double re = fine_radius(rq.rin, rq.rout, ii, nr);
-      const double trff = lerp1(r.ifac_r, node[(size_t)(r.K + 1) * ng * 2 + q], node[(size_t)r.K * ng * 2 + q]);
-      const double cosne = lerp1(r.ifac_r, node[nq2 + (size_t)(r.K + 1) * ng * 2 + q], node[nq2 + (size_t)r.K * ng * 2 + q]);
+    // per g*: the node value and d_gstar / sqrt(g* (1 - g*)) (s_red is free here)
+    static_assert(2 * MAX_NG <= SETUP_THREADS, "g* factors must fit the reduction buffer");
+    double *s_gs = s_red, *s_gw = s_red + MAX_NG;
+    for (int g = tid; g < ng; g += SETUP_THREADS) {
       const double gs = gstar_node(step, g);
-      const double gv = gs * (r.gmax - r.gmin) + r.gmin;
-      const int imu = ((int)(rq.n_incl * (1 - cosne) + 1)) - 1;
       const double dgs = (g == 0 || g == ng - 1) ? 0.5 * (gs1 - gs0) + GFAC_H : gs1 - gs0;
-      const double pr = 2 * PI * gv * re;
-      const double emisw = r.coef * (r.gmax - r.gmin) * fac;  // emis * weight
-      const double tmp = re * (pr * pr) / sqrt(gs - gs * gs) * trff * emisw * dgs;
-      if (imu < 0 || imu >= rq.n_incl) {
-        derr = 40;
-        continue;
+      s_gs[g] = gs;
+      s_gw[g] = dgs / sqrt(gs - gs * gs);
+    }
+    __syncthreads();
+    const int lane = tid & 63;
+    int derr = 0;
+    for (int i0 = 0; i0 < nr; i0 += SETUP_THREADS) {
+      const int ii = i0 + tid;
+      bool live = ii < nr;
+      RadRec r = {};
+      if (live) {
+        r = rad[ii];
+        live = r.ielo <= r.iehi;
       }
-      if (tmp != tmp) derr = 41;
-      atomicAdd(&s_dist[imu], tmp);
+      if (__ballot(live) == 0) continue;
+      const double re = live ? fine_radius(rq.rin, rq.rout, ii, nr) : 0.0;
+      const double emisw = r.coef * (r.gmax - r.gmin) * fac;  // emis * weight
+      const double *tlo = node + (size_t)(r.K + 1) * cols, *thi = node + (size_t)r.K * cols;
+      for (int q = 0; q < cols; q++) {
+        const int g = q >> 1;
+        const double gs = s_gs[g];
+        double tmp = 0.0;
+        int imu = -1;
+        bool ok = false;
+        if (live) {
+          const double trff = lerp1(r.ifac_r, tlo[q], thi[q]);
+          const double cosne = lerp1(r.ifac_r, tlo[nq2 + q], thi[nq2 + q]);
+          const double gv = gs * (r.gmax - r.gmin) + r.gmin;
+          imu = ((int)(rq.n_incl * (1 - cosne) + 1)) - 1;
+          const double pr = 2 * PI * gv * re;
+          tmp = re * (pr * pr) * trff * emisw * s_gw[g];
+          if (imu < 0 || imu >= rq.n_incl) derr = 40;
+          else if (tmp != tmp) derr = 41;
+          else ok = true;
+        }
+        const unsigned long long m_ok = __ballot(ok);
+        if (m_ok == 0) continue;
+        const int first = __ffsll((long long)m_ok) - 1;
+        const int k0 = __builtin_amdgcn_readlane(imu, first);
+        if (__ballot(ok && imu != k0) == 0) {
+          const double sum = wave_sum(ok ? tmp : 0.0);
+          if (lane == first) atomicAdd(&s_dist[k0], sum);
+        } else if (ok) {
+          atomicAdd(&s_dist[imu], tmp);
+        }
+      }
     }
     if (derr) atomicMax(&s_st, derr);
     __syncthreads();
@@ -603,33 +663,6 @@ __global__ __launch_bounds__(64) void k_reset(int *ctr) {
 // integrate: the hot loop (calc_relline_profile, src/Relprofile.cpp:857-900)
 // ---------------------------------------------------------------------------
 namespace {
-
-template <int CTRL>
-__device__ inline int dpp_i(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
-}
-
-template <int CTRL>
-__device__ inline double dpp_d(double v) {
-  const int lo = dpp_i<CTRL>(__double2loint(v));
-  const int hi = dpp_i<CTRL>(__double2hiint(v));
-  return __hiloint2double(hi, lo);
-}
-
-__device__ inline double readlane_d(double v, int l) {
-  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
-                          __builtin_amdgcn_readlane(__double2loint(v), l));
-}
-
-// Sum over the 64 lanes in a fixed order (pairs, quads, half rows, rows, then
-// the four rows); every lane must be active.
-__device__ inline double wave_sum(double v) {
-  v += dpp_d<0xB1>(v);   // quad_perm [1,0,3,2]
-  v += dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]
-  v += dpp_d<0x141>(v);  // row_half_mirror
-  v += dpp_d<0x140>(v);  // row_mirror
-  return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
-}
 
 // The transfer function (and cosines) of one fine radius, lerped from its two
 // table-node rows into a lane-private LDS row, and the FP64 edge norms.
