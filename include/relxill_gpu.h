@@ -42,6 +42,7 @@ extern "C" {
 #define RX_MODEL_RELCONV 11
 #define RX_MODEL_RELCONVLP 12
 #define RX_MODEL_RELXILL (-1) /* relxill: broken power law, one zone, xillver table */
+#define RX_MODEL_RELXILLLP (-2) /* relxilllp: lamp post, 10 radial zones, xillver table */
 /* project id: relxill-style convolution of a resident angle-resolved
  * reflection set (rx_reflset_create) with relconv's parameters */
 #define RX_MODEL_RELXILL_CONV (-1001)
@@ -85,6 +86,20 @@ void lmodrelconvlp(const double *energy, int Nflux, const double *parameter, int
  * plus the cut-off power law for refl_frac >= 0. */
 void lmodrelxill(const double *energy, int Nflux, const double *parameter, int spectrum,
                  double *flux, double *fluxError, const char *init);
+
+/* relxilllp: h beta a Incl Rin Rout z gamma logxi Afe Ecut refl_frac
+ * switch_returnrad switch_reflfrac_boost (lmodel_relxill_public.dat:61-75; the
+ * reference's c_lmodrelxilllp -> relxill_kernel with 10 radial zones,
+ * src/relutility.c:506-544): per zone the relativistic profile and angle
+ * distribution, the xillver spectra for the Ecut the zone sees (source ->
+ * disk energy shift, src/IonGradient.cpp:182-209) divided by the change of
+ * the incident normalisation (src/Xillspec.cpp:408-435), one convolution per
+ * zone; the reflection scaled by |refl_frac| over the lamp post's reflection
+ * fraction (by |refl_frac| alone with switch_reflfrac_boost = 1) and the
+ * primary boosted by f_inf / 0.5 g^Gamma (src/PrimarySource.cpp:66-125).
+ * switch_returnrad must be 0 (RX_ERR_UNSUPPORTED otherwise). */
+void lmodrelxilllp(const double *energy, int Nflux, const double *parameter, int spectrum,
+                   double *flux, double *fluxError, const char *init);
 
 /* Status-returning evaluation of one model (the body of xspec_C_wrapper_eval_model,
  * src/LocalModel.cpp:143-160, and LocalModel::eval_model, src/LocalModel.h:108-128).
@@ -204,8 +219,9 @@ int rx_batch_sync(rx_batch *b);
  * returns its status. */
 int rx_batch_output(rx_batch *b, int i, double *flux);
 int rx_batch_status(rx_batch *b, int i);
-/* emission-angle distribution of a relxill-style request (n_incl values) */
-int rx_batch_dist(rx_batch *b, int i, double *dist, int n_incl);
+/* emission-angle distribution of a relxill-style or relxill request: the
+ * first n values of [zone][n_incl] (one zone except relxilllp's 10) */
+int rx_batch_dist(rx_batch *b, int i, double *dist, int n);
 /* lamp-post reflection fractions of request i: refl_frac, f_bh, f_ad, f_inf, f_inf_rest */
 int rx_batch_reflfrac(rx_batch *b, int i, double *out5);
 /* Integration work shape: bins per work item (segment of one chunk of 64

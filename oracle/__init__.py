@@ -20,6 +20,7 @@ MOD_RELLINELP = 2
 MOD_RELCONV = 11
 MOD_RELCONVLP = 12
 MOD_RELXILL = -1
+MOD_RELXILLLP = -2
 MOD_RELXILL_CONV = -1001
 
 _D = ctypes.POINTER(ctypes.c_double)
@@ -68,7 +69,7 @@ class XillTable(ctypes.Structure):
 class XillParam(ctypes.Structure):
     _fields_ = [("gam", ctypes.c_double), ("afe", ctypes.c_double), ("lxi", ctypes.c_double),
                 ("ect", ctypes.c_double), ("dens", ctypes.c_double), ("incl", ctypes.c_double),
-                ("refl_frac", ctypes.c_double)]
+                ("refl_frac", ctypes.c_double), ("boost", _I)]
 
 
 class Job(ctypes.Structure):
@@ -122,7 +123,11 @@ def _load():
         "orc_spec_cutoffpl": (None, [_D, _D, _I, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
         "orc_norm_wrt_xillver": (ctypes.c_double, [_D, _D, _I, ctypes.POINTER(_I)]),
         "orc_primary_norm_factor": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
-        "orc_get_relxill_params": (_I, [_D, _I, ctypes.POINTER(RelParam), ctypes.POINTER(XillParam)]),
+        "orc_get_relxill_params": (_I, [_I, _D, _I, ctypes.POINTER(RelParam), ctypes.POINTER(XillParam)]),
+        "orc_energy_shift_source_obs": (ctypes.c_double, [ctypes.POINTER(RelParam)]),
+        "orc_relxill_zones": (_I, [_I, ctypes.POINTER(RelTable), ctypes.POINTER(LpTable), _D, _D]),
+        "orc_relxill_model": (_I, [_I, ctypes.POINTER(RelTable), ctypes.POINTER(LpTable), ctypes.POINTER(XillTable),
+                                   _D, _D, _D, _I, _D, _I, _I, _D, ctypes.POINTER(Stats)]),
         "orc_relxill": (_I, [ctypes.POINTER(RelTable), ctypes.POINTER(XillTable), _D, _D, _D, _I, _D, _I, _I,
                              _D, ctypes.POINTER(Stats)]),
         "orc_eval_model": (_I, [_I, ctypes.POINTER(RelTable), ctypes.POINTER(LpTable), _D, _D,
@@ -355,7 +360,7 @@ class Xill:
 
 
 def xill_param(gam, afe, lxi, ect, dens=15.0, incl=30.0, refl_frac=0.0):
-    return XillParam(gam, afe, lxi, ect, dens, incl, refl_frac)
+    return XillParam(gam, afe, lxi, ect, dens, incl, refl_frac, 0)
 
 
 def xill_corner(xt, xp):
@@ -398,3 +403,32 @@ def relxill(tables, xt, params, energy, conv_grid=None, phys_norm=False):
                          _dp(cg) if cg is not None else None, (cg.size - 1) if cg is not None else 0,
                          int(phys_norm), _dp(dist), None)
     return out, dist, st
+
+
+def relxill_model(model, tables, xt, params, energy, conv_grid=None, phys_norm=False):
+    """relxill (MOD_RELXILL) or relxilllp (MOD_RELXILLLP) on the caller grid;
+    returns (flux, dist[num_zones, n_incl], status)."""
+    e = _arr(energy)
+    n = e.size - 1
+    out = np.zeros(n)
+    nz = 10 if model == MOD_RELXILLLP else 1
+    dist = np.zeros(nz * xt.n_incl)
+    cg = _arr(conv_grid) if conv_grid is not None else None
+    st = lib.orc_relxill_model(model, ctypes.byref(tables.rel), tables.lp_ptr, ctypes.byref(xt.t),
+                               _dp(_arr(params)), _dp(e), _dp(out), n, _dp(cg) if cg is not None else None,
+                               (cg.size - 1) if cg is not None else 0, int(phys_norm), _dp(dist), None)
+    return out, dist.reshape(nz, xt.n_incl), st
+
+
+def relxill_zones(model, tables, params):
+    """per zone: r_lo, r_hi, del_emit(centre), energy shift source -> disk,
+    Ecut seen by the zone, normalisation change; (array[nz, 6], status)"""
+    nz = 10 if model == MOD_RELXILLLP else 1
+    out = np.zeros(6 * nz)
+    st = lib.orc_relxill_zones(model, ctypes.byref(tables.rel), tables.lp_ptr, _dp(_arr(params)), _dp(out))
+    return out.reshape(nz, 6), st
+
+
+def energy_shift_source_obs(model, params):
+    p, st = rel_params(model, params)
+    return lib.orc_energy_shift_source_obs(ctypes.byref(p)), st

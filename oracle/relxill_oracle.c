@@ -307,6 +307,7 @@ int orc_n_params(int model_type) {
     case ORC_MOD_RELLINELP: return 10;
     case ORC_MOD_RELCONVLP: return 9;
     case ORC_MOD_RELXILL: return 13;
+    case ORC_MOD_RELXILLLP: return 14;
     case ORC_MOD_RELXILL_CONV: return 8;  /* relxill-style conv: relconv's parameters */
     default: return -1;
   }
@@ -357,6 +358,14 @@ int orc_get_rel_params(int model_type, const double *par, int phys_norm_env, orc
       p->emis_type = ORC_EMIS_BKN;
       p->emis1 = par[0]; p->emis2 = par[1]; p->rbr = par[2]; p->a = par[3]; incl_deg = par[4];
       p->rin = par[5]; p->rout = par[6]; p->z = par[7]; p->gamma = par[8];
+      break;
+    case ORC_MOD_RELXILLLP:  /* h beta a Incl Rin Rout z gamma logxi Afe Ecut refl_frac
+                                switch_returnrad switch_reflfrac_boost */
+      p->emis_type = ORC_EMIS_LP;
+      p->height = par[0]; p->beta = par[1]; p->a = par[2]; incl_deg = par[3]; p->rin = par[4];
+      p->rout = par[5]; p->z = par[6]; p->gamma = par[7];
+      p->return_rad = (int)lround(par[12]);
+      p->num_zones = 10;  /* N_ZONES, src/common.h:134 (get_num_zones, src/relutility.c:506-544) */
       break;
     case ORC_MOD_RELCONV:  /* Index1 Index2 Rbr a Incl Rin Rout limb */
     case ORC_MOD_RELXILL_CONV:
@@ -1274,31 +1283,59 @@ double orc_primary_norm_factor(double gam, double ect) {
   return 1. / orc_norm_wrt_xillver(flux, ener, N_ENER_COARSE, NULL);
 }
 
-int orc_get_relxill_params(const double *par, int phys_norm_env, orc_relparam *p, orc_xillparam *x) {
-  int status = orc_get_rel_params(ORC_MOD_RELXILL, par, phys_norm_env, p);
+int orc_get_relxill_params(int model_type, const double *par, int phys_norm_env, orc_relparam *p,
+                           orc_xillparam *x) {
+  int status = orc_get_rel_params(model_type, par, phys_norm_env, p);
   /* get_xill_params (src/ModelDefinition.cpp:339-390): Ecut in keV, the
    * density at its default 15; logXi kept within 0-4.7 for the zones
    * (adjust_parameter_to_be_within_xillver_bounds, src/IonGradient.cpp:36-44,
    * :388) */
   memset(x, 0, sizeof(*x));
-  x->gam = par[8];
-  x->lxi = par[9];
-  x->afe = par[10];
-  x->ect = par[11];
-  x->refl_frac = par[12];
+  if (model_type == ORC_MOD_RELXILLLP) {
+    x->gam = par[7]; x->lxi = par[8]; x->afe = par[9]; x->ect = par[10]; x->refl_frac = par[11];
+    x->incl = par[3];
+    x->boost = (int)lround(par[13]);
+  } else {
+    x->gam = par[8]; x->lxi = par[9]; x->afe = par[10]; x->ect = par[11]; x->refl_frac = par[12];
+    x->incl = par[4];
+  }
   x->dens = 15;
-  x->incl = par[4];
   if (x->lxi < 0.0) x->lxi = 0.0;
   else if (x->lxi > 4.7) x->lxi = 4.7;
   return status;
 }
 
-int orc_relxill(const orc_reltable *tab, const orc_xilltable *xt, const double *parameter,
-                const double *energy, double *flux, int n_flux, const double *conv_ener,
-                int n_conv, int phys_norm_env, double *dist_out, orc_stats *st) {
+/* energy_shift_source_obs (src/Relphysics.cpp:239-254) with calc_g_inf
+ * (:134-136) and doppler_factor_source_obs (:217-230) */
+double orc_energy_shift_source_obs(const orc_relparam *p) {
+  if (p->emis_type != ORC_EMIS_LP) return 1;
+  const double g_inf_0 = sqrt(1.0 - (2 * p->height / (p->height * p->height + p->a * p->a)));
+  if (p->beta < 1e-4) return g_inf_0;
+  return g_inf_0 * doppler_factor(M_PI - p->incl, p->beta);
+}
+
+/* inv_rebin_mean (src/relutility.c:636-663): y0 on the descending x0,
+ * interpolated at the ascending xn */
+static void inv_rebin_mean(const double *x0, const double *y0, int n0, const double *xn, double *yn, int nn) {
+  int in = nn - 1;
+  for (int ii = 0; ii < n0 - 1; ii++) {
+    if (x0[ii] > xn[in] && x0[ii + 1] <= xn[in]) {
+      const double ifac_r = (xn[in] - x0[ii + 1]) / (x0[ii] - x0[ii + 1]);
+      yn[in] = orc_interp_lin_1d(ifac_r, y0[ii + 1], y0[ii]);
+      in--;
+      if (in < 0) break;
+    }
+  }
+}
+
+int orc_relxill_model(int model_type, const orc_reltable *tab, const orc_lptable *lp,
+                      const orc_xilltable *xt, const double *parameter, const double *energy, double *flux,
+                      int n_flux, const double *conv_ener, int n_conv, int phys_norm_env, double *dist_out,
+                      orc_stats *st) {
   orc_relparam p;
   orc_xillparam x;
-  int status = orc_get_relxill_params(parameter, phys_norm_env, &p, &x);
+  int status = orc_get_relxill_params(model_type, parameter, phys_norm_env, &p, &x);
+  if (status == 0 && p.return_rad != 0) status = 2;  /* returning radiation is out of scope */
   if (status) {
     memset(flux, 0, sizeof(double) * n_flux);
     return status;
@@ -1310,59 +1347,143 @@ int orc_relxill(const orc_reltable *tab, const orc_xilltable *xt, const double *
     orc_get_log_grid(grid, n_conv + 1, 0.00035, 2000.0);
     conv_ener = grid;
   }
+  const int nz = p.num_zones;
   const int nx = xt->n_ener, n_incl = xt->n_val[xt->n_par - 1];
   double *ener = (double *)malloc(sizeof(double) * (n_flux + 1));
   double *xe = (double *)malloc(sizeof(double) * (nx + 1));
   double *xflu = (double *)malloc(sizeof(double) * (size_t)n_incl * nx);
   double *xill = (double *)malloc(sizeof(double) * nx);
-  double *rel = (double *)malloc(sizeof(double) * n_conv);
-  double *dist = (double *)calloc(n_incl, sizeof(double));
+  double *rel = (double *)malloc(sizeof(double) * (size_t)nz * n_conv);
+  double *dist = (double *)calloc((size_t)nz * n_incl, sizeof(double));
   double *xill_rb = (double *)malloc(sizeof(double) * n_conv);
   double *conv_out = (double *)malloc(sizeof(double) * n_conv);
   double *total = (double *)calloc(n_conv, sizeof(double));
   double *prim = (double *)malloc(sizeof(double) * n_conv);
+  double *rgrid = (double *)malloc(sizeof(double) * (nz + 1));
+  double *rmean = (double *)malloc(sizeof(double) * nz);
+  double *del_emit = (double *)malloc(sizeof(double) * nz);
+  double *shift = (double *)malloc(sizeof(double) * nz);
   /* shift_energy_grid_redshift (src/XspecSpectrum.h:70-76) */
   memcpy(ener, energy, sizeof(double) * (n_flux + 1));
   if (p.z > 0)
     for (int ii = 0; ii <= n_flux; ii++) ener[ii] *= (1 + p.z);
-  /* 1-2: one zone, constant ionisation: the zone's xillver parameters are
-   * those of the source (IonGradient, src/IonGradient.cpp:196-209, :340-389) */
-  orc_xill_spectra(xt, &x, xe, xflu);
-  /* 4: the relline profile with its angle distribution on the relxill grid */
-  status = orc_relbase(tab, NULL, &p, conv_ener, n_conv, 1000, rel, dist, n_incl, st);
+  /* the lamp post gives Ecut in the observer frame: the source's is Ecut / g
+   * (src/Relxill.cpp:197-200, src/PrimarySource.h:204-216) */
+  const double g_so = orc_energy_shift_source_obs(&p);
+  orc_xillparam xs = x;
+  if (p.emis_type == ORC_EMIS_LP) xs.ect /= g_so;
+  orc_syspar *sp = orc_get_system_parameters(tab, lp, &p, 1000, &status);
+  if (sp) {
+    /* 1: zones (RadialGrid, src/IonGradient.cpp:457-489; IonGradient.h:63-70)
+     * with their del_emit and source -> disk energy shift (:182-193, :421-430) */
+    orc_get_rzone_grid(p.rin, p.rout, nz, p.height, rgrid);
+    for (int iz = 0; iz < nz; iz++) rmean[iz] = 0.5 * (rgrid[iz] + rgrid[iz + 1]);
+    for (int iz = 0; iz < nz; iz++) {
+      shift[iz] = 1.0;
+      del_emit[iz] = 0.0;
+    }
+    if (p.emis_type == ORC_EMIS_LP) {
+      inv_rebin_mean(sp->re, sp->del_emit, sp->nr, rmean, del_emit, nz);
+      for (int iz = 0; iz < nz; iz++) shift[iz] = orc_gi_potential_lp(rmean[iz], p.a, p.height, p.beta, del_emit[iz]);
+    }
+    /* 4: the relline profile of every zone with its angle distribution */
+    status = orc_calc_relline_profile(sp, conv_ener, n_conv, rgrid, nz, rel, dist, n_incl, st);
+    if (status == 0) orc_renorm_relline_profile(conv_ener, n_conv, nz, rel, dist, n_incl, &p);
+  }
   if (status == 0) {
-    /* 5: angle weighting; the normalisation change source -> disk is 1 for
-     * the broken power law (energy shift 1, src/IonGradient.cpp:182-193,
+    /* normalisation change source -> disk (calc_xillver_normalization_change_source_to_disk,
      * src/Xillspec.cpp:408-435) */
-    orc_xill_angdep(xill, xflu, dist, n_incl, nx);
-    /* 6: relxill_convolution_multizone (src/Relxill.cpp:432-482) */
-    double relsum = 0.0;
-    for (int jj = 0; jj < n_conv; jj++) relsum += rel[jj];
-    if (!(relsum < 1e-12)) {
+    const double nf_src = orc_primary_norm_factor(xs.gam, xs.ect);
+    for (int iz = 0; iz < nz; iz++) {
+      /* 2 + 5: the zone's xillver spectra (Ecut seen by the zone,
+       * get_xill_param_zone, src/IonGradient.cpp:196-209), angle weighting */
+      orc_xillparam xz = xs;
+      xz.ect = xs.ect * shift[iz];
+      orc_xill_spectra(xt, &xz, xe, xflu);
+      orc_xill_angdep(xill, xflu, dist + (size_t)iz * n_incl, n_incl, nx);
+      const double ncf = orc_primary_norm_factor(xs.gam, xs.ect * shift[iz]) / nf_src;
+      for (int jj = 0; jj < nx; jj++) xill[jj] /= ncf;
+      /* 6: relxill_convolution_multizone (src/Relxill.cpp:432-482) */
+      const double *rz = rel + (size_t)iz * n_conv;
+      double relsum = 0.0;
+      for (int jj = 0; jj < n_conv; jj++) relsum += rz[jj];
+      if (relsum < 1e-12) continue;
       orc_rebin_spectrum(conv_ener, xill_rb, n_conv, xe, xill, nx);
-      orc_conv_normalized(conv_ener, xill_rb, rel, conv_out, n_conv);
+      orc_conv_normalized(conv_ener, xill_rb, rz, conv_out, n_conv);
       for (int jj = 0; jj < n_conv; jj++) total[jj] += conv_out[jj];
     }
-    /* add_primary_spectrum (src/PrimarySource.cpp:66-125), not the lamp
-     * post: the reflection times |refl_frac|, the primary added for
-     * refl_frac >= 0; the primary is the cut-off power law normalised as the
-     * xillver incident spectrum (src/PrimarySource.h:238-248) */
-    for (int jj = 0; jj < n_conv; jj++) total[jj] *= fabs(x.refl_frac);
-    if (x.refl_frac >= 0) {
-      orc_spec_cutoffpl(prim, conv_ener, n_conv, x.ect, x.gam, 1.0);
-      const double nf = orc_primary_norm_factor(x.gam, x.ect);
-      for (int jj = 0; jj < n_conv; jj++) prim[jj] *= nf;
-      for (int jj = 0; jj < n_conv; jj++) total[jj] += prim[jj];
+    /* add_primary_spectrum (src/PrimarySource.cpp:66-125): the observed
+     * cut-off power law (Ecut of the source shifted by g, normalised as the
+     * xillver incident spectrum at the source, src/PrimarySource.h:238-292) */
+    orc_spec_cutoffpl(prim, conv_ener, n_conv, xs.ect, xs.gam, g_so);
+    for (int jj = 0; jj < n_conv; jj++) prim[jj] *= nf_src;
+    if (p.emis_type != ORC_EMIS_LP) {
+      for (int jj = 0; jj < n_conv; jj++) total[jj] *= fabs(x.refl_frac);
+    } else {
+      const orc_reflfrac *lr = &sp->refl;
+      double refl_frac_input = x.refl_frac;
+      if (x.boost) refl_frac_input *= lr->refl_frac;
+      /* flux_boost_source_to_observer (src/PrimarySource.h:84-93) */
+      double prime_fac = lr->f_inf_rest / 0.5 * pow(g_so, xs.gam);
+      if (p.beta > 1e-4) prime_fac *= pow(doppler_factor(M_PI - p.incl, p.beta), 2);
+      const double norm_fac_refl = (fabs(refl_frac_input)) / lr->refl_frac;
+      for (int jj = 0; jj < n_conv; jj++) total[jj] *= norm_fac_refl;
+      for (int jj = 0; jj < n_conv; jj++) prim[jj] *= prime_fac;
     }
+    if (x.refl_frac >= 0)
+      for (int jj = 0; jj < n_conv; jj++) total[jj] += prim[jj];
     /* rebin_and_normalize_relxill_for_xspec (src/Relxill.cpp:267-278) */
     orc_rebin_spectrum(ener, flux, n_flux, conv_ener, total, n_conv);
-    if (dist_out) memcpy(dist_out, dist, sizeof(double) * n_incl);
+    if (dist_out) memcpy(dist_out, dist, sizeof(double) * (size_t)nz * n_incl);
   } else {
     memset(flux, 0, sizeof(double) * n_flux);
   }
+  if (sp) orc_syspar_free(sp);
   free(grid); free(ener); free(xe); free(xflu); free(xill); free(rel); free(dist);
-  free(xill_rb); free(conv_out); free(total); free(prim);
+  free(xill_rb); free(conv_out); free(total); free(prim); free(rgrid); free(rmean); free(del_emit);
+  free(shift);
   return status;
+}
+
+int orc_relxill_zones(int model_type, const orc_reltable *tab, const orc_lptable *lp, const double *parameter,
+                      double *out) {
+  orc_relparam p;
+  orc_xillparam x;
+  int status = orc_get_relxill_params(model_type, parameter, 0, &p, &x);
+  if (status) return status;
+  const int nz = p.num_zones;
+  orc_syspar *sp = orc_get_system_parameters(tab, lp, &p, 1000, &status);
+  if (!sp) return status;
+  double rgrid[64], rmean[64], del[64];
+  orc_get_rzone_grid(p.rin, p.rout, nz, p.height, rgrid);
+  for (int iz = 0; iz < nz; iz++) {
+    rmean[iz] = 0.5 * (rgrid[iz] + rgrid[iz + 1]);
+    del[iz] = 0.0;
+  }
+  if (p.emis_type == ORC_EMIS_LP) inv_rebin_mean(sp->re, sp->del_emit, sp->nr, rmean, del, nz);
+  const double g_so = orc_energy_shift_source_obs(&p);
+  const double ect_src = (p.emis_type == ORC_EMIS_LP) ? x.ect / g_so : x.ect;
+  const double nf_src = orc_primary_norm_factor(x.gam, ect_src);
+  for (int iz = 0; iz < nz; iz++) {
+    const double shift =
+        (p.emis_type == ORC_EMIS_LP) ? orc_gi_potential_lp(rmean[iz], p.a, p.height, p.beta, del[iz]) : 1.0;
+    double *o = out + 6 * iz;
+    o[0] = rgrid[iz];
+    o[1] = rgrid[iz + 1];
+    o[2] = del[iz];
+    o[3] = shift;
+    o[4] = ect_src * shift;
+    o[5] = orc_primary_norm_factor(x.gam, ect_src * shift) / nf_src;
+  }
+  orc_syspar_free(sp);
+  return 0;
+}
+
+int orc_relxill(const orc_reltable *tab, const orc_xilltable *xt, const double *parameter,
+                const double *energy, double *flux, int n_flux, const double *conv_ener,
+                int n_conv, int phys_norm_env, double *dist_out, orc_stats *st) {
+  return orc_relxill_model(ORC_MOD_RELXILL, tab, NULL, xt, parameter, energy, flux, n_flux, conv_ener, n_conv,
+                           phys_norm_env, dist_out, st);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -33,6 +33,7 @@ extern "C" {
 #define ORC_MOD_RELCONV 11
 #define ORC_MOD_RELCONVLP 12
 #define ORC_MOD_RELXILL (-1)
+#define ORC_MOD_RELXILLLP (-2)
 /* project id: relxill-style convolution of a given angle-resolved reflection
  * set, with relconv's 8 parameters and relxill's profile normalisation */
 #define ORC_MOD_RELXILL_CONV (-1001)
@@ -205,6 +206,7 @@ typedef struct {
 typedef struct {
   double gam, afe, lxi, ect, dens, incl;  /* incl in degrees */
   double refl_frac;
+  int boost;  /* switch_reflfrac_boost: refl_frac relative to the lamp post's */
 } orc_xillparam;
 
 /* get_xilltab_indices_for_paramvals (src/xilltable.c:297-324) */
@@ -230,12 +232,26 @@ double orc_norm_wrt_xillver(const double *flux, const double *ener, int n, int *
  * src/PrimarySource.h:279-292) */
 double orc_primary_norm_factor(double gam, double ect);
 /* relxill parameters: Index1 Index2 Rbr a Incl Rin Rout z gamma logxi Afe
- * Ecut refl_frac (src/modelfiles/lmodel_relxill_public.dat:46-59) */
-int orc_get_relxill_params(const double *parameter, int phys_norm_env, orc_relparam *p,
+ * Ecut refl_frac (src/modelfiles/lmodel_relxill_public.dat:46-59); relxilllp:
+ * h beta a Incl Rin Rout z gamma logxi Afe Ecut refl_frac switch_returnrad
+ * switch_reflfrac_boost (:61-75) */
+int orc_get_relxill_params(int model_type, const double *parameter, int phys_norm_env, orc_relparam *p,
                            orc_xillparam *x);
-/* relxill (src/Relxill.cpp:289-414 with src/PrimarySource.cpp:66-125): the
- * broken power-law model with one zone and constant ionisation.
- * dist_out (optional): the emission-angle distribution (n_incl values). */
+/* energy shift primary source -> observer (src/Relphysics.cpp:239-254) */
+double orc_energy_shift_source_obs(const orc_relparam *p);
+/* relxill_kernel (src/Relxill.cpp:289-414 with src/PrimarySource.cpp:66-125)
+ * for relxill (broken power law, one zone) and relxilllp (lamp post, 10
+ * zones with the source -> disk energy shift of Ecut and the normalisation
+ * change, lamp-post reflection fraction and primary boost); constant
+ * ionisation.  dist_out (optional): [num_zones][n_incl]. */
+int orc_relxill_model(int model_type, const orc_reltable *tab, const orc_lptable *lp,
+                      const orc_xilltable *xt, const double *parameter, const double *energy, double *flux,
+                      int n_flux, const double *conv_ener, int n_conv, int phys_norm_env, double *dist_out,
+                      orc_stats *st);
+/* the zones of a relxill model: per zone {r_lo, r_hi, del_emit at the centre,
+ * source -> disk energy shift, Ecut seen by the zone, normalisation change} */
+int orc_relxill_zones(int model_type, const orc_reltable *tab, const orc_lptable *lp, const double *parameter,
+                      double *out);
 int orc_relxill(const orc_reltable *tab, const orc_xilltable *xt, const double *parameter,
                 const double *energy, double *flux, int n_flux, const double *conv_ener,
                 int n_conv, int phys_norm_env, double *dist_out, orc_stats *st);

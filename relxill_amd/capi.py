@@ -27,18 +27,20 @@ MODEL_RELLINELP = 2
 MODEL_RELCONV = 11
 MODEL_RELCONVLP = 12
 MODEL_RELXILL = -1
+MODEL_RELXILLLP = -2
 MODEL_RELXILL_CONV = -1001  # project id: relxill-style conv of a resident reflection set
 
 FP64 = 0
 FP32 = 1
 
 N_PARAMS = {MODEL_RELLINE: 10, MODEL_RELCONV: 8, MODEL_RELLINELP: 10, MODEL_RELCONVLP: 9,
-            MODEL_RELXILL_CONV: 8, MODEL_RELXILL: 13}
+            MODEL_RELXILL_CONV: 8, MODEL_RELXILL: 13, MODEL_RELXILLLP: 14}
 
 # default parameter values of the reference's lmodel.dat
 # (src/modelfiles/lmodel_relxill_public.dat:1-59)
 DEFAULTS = {
     MODEL_RELXILL: [3.0, 3.0, 15.0, 0.998, 30.0, -1.0, 400.0, 0.0, 2.0, 3.1, 1.0, 300.0, 3.0],
+    MODEL_RELXILLLP: [6.0, 0.0, 0.998, 30.0, -1.0, 400.0, 0.0, 2.0, 3.1, 1.0, 300.0, 1.0, 1.0, 0.0],
     MODEL_RELLINE: [6.4, 3.0, 3.0, 15.0, 0.998, 30.0, -1.0, 400.0, 0.0, 0.0],
     MODEL_RELCONV: [3.0, 3.0, 15.0, 0.998, 30.0, -1.0, 400.0, 0.0],
     MODEL_RELLINELP: [6.4, 6.0, 0.998, 30.0, -1.0, 400.0, 0.0, 0.0, 2.0, 1.0],
@@ -86,6 +88,7 @@ _EXPORTS = {
     "lmodrelconv": (None, [_D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_char_p]),
     "lmodrelconvlp": (None, [_D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_char_p]),
     "lmodrelxill": (None, [_D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_char_p]),
+    "lmodrelxilllp": (None, [_D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_char_p]),
     "rx_xilltable_synthetic": (ctypes.c_int, [ctypes.c_uint64] + [ctypes.c_int] * 6),
     "rx_xilltable_load_fits": (ctypes.c_int, [ctypes.c_char_p]),
     "rx_xilltable_get_view": (ctypes.c_int, [ctypes.POINTER(_XillView)]),
@@ -212,6 +215,10 @@ def lmodrelconvlp(energy, params, flux):
 
 def lmodrelxill(energy, params):
     return _lmod(lib.lmodrelxill, energy, params)
+
+
+def lmodrelxilllp(energy, params):
+    return _lmod(lib.lmodrelxilllp, energy, params)
 
 
 def xilltable_synthetic(seed=5, n_gam=13, n_afe=5, n_lxi=8, n_ect=6, n_incl=10, n_ener=2999):

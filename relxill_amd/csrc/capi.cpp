@@ -116,6 +116,11 @@ void lmodrelxill(const double *energy, int Nflux, const double *parameter, int, 
   rx_eval_model(RX_MODEL_RELXILL, parameter, energy, flux, Nflux, RX_FP64);
 }
 
+void lmodrelxilllp(const double *energy, int Nflux, const double *parameter, int, double *flux, double *,
+                   const char *) {
+  rx_eval_model(RX_MODEL_RELXILLLP, parameter, energy, flux, Nflux, RX_FP64);
+}
+
 int rx_last_status(void) { return last_status(); }
 const char *rx_last_error(void) { return last_error(); }
 
@@ -439,8 +444,11 @@ int rx_batch_output(rx_batch *b, int i, double *flux) {
 }
 
 int rx_batch_dist(rx_batch *b, int i, double *dist, int n_incl) {
-  if (!b || !dist || i < 0 || i >= b->b.n_req || n_incl > MAX_INCL) return RX_ERR_ARG;
-  if (hipMemcpy(dist, b->b.d_dist.as<double>() + (size_t)i * MAX_INCL, sizeof(double) * n_incl,
+  if (!b || !dist || i < 0 || i >= b->b.n_req || n_incl < 0) return RX_ERR_ARG;
+  const ReqParams &p = b->b.h_req[i];
+  if (n_incl > p.n_incl * p.n_zones) return RX_ERR_ARG;
+  if (n_incl == 0) return RX_OK;
+  if (hipMemcpy(dist, b->b.d_dist.as<double>() + p.dist_off, sizeof(double) * n_incl,
                 hipMemcpyDeviceToHost) != hipSuccess)
     return RX_ERR_DEVICE;
   return RX_OK;

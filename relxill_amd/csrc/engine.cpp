@@ -61,6 +61,7 @@ int n_model_params(int model) {
     case MOD_RELCONVLP: return 9;
     case MOD_RELXILL_CONV: return 8;
     case MOD_RELXILL: return 13;
+    case MOD_RELXILLLP: return 14;
     default: return -1;
   }
 }
@@ -89,6 +90,17 @@ int parse_request(int model, const double *par, bool phys_norm, ReqParams &p, st
       p.xgam = par[8];
       p.xect = par[11];
       p.refl_frac = par[12];
+      break;
+    case MOD_RELXILLLP:  // h beta a Incl Rin Rout z gamma logxi Afe Ecut refl_frac
+                         // switch_returnrad switch_reflfrac_boost
+      p.emis_type = EMIS_LP;
+      p.height = par[0]; p.beta = par[1]; p.a = par[2]; incl_deg = par[3]; p.rin = par[4];
+      p.rout = par[5]; p.z = par[6]; p.gamma = par[7];
+      return_rad = (int)lround(par[12]);
+      p.xgam = par[7];
+      p.xect = par[10];
+      p.refl_frac = par[11];
+      p.xboost = (int)lround(par[13]);
       break;
     case MOD_RELCONV:  // Index1 Index2 Rbr a Incl Rin Rout limb
     case MOD_RELXILL_CONV:
@@ -161,7 +173,51 @@ int parse_request(int model, const double *par, bool phys_norm, ReqParams &p, st
     // norm_factor_semi_infinite_slab, src/Xillspec.cpp:528-530
     p.norm = 0.5 * cos((p.incl * 180.0 / kPi) * kPi / 180);
   }
+  p.n_zones = 1;
+  p.g_so = 1.0;
+  if (model == MOD_RELXILL || model == MOD_RELXILLLP) {
+    // zones (get_num_zones, src/relutility.c:506-544: N_ZONES for the lamp
+    // post, one otherwise) and their edges
+    p.n_zones = (model == MOD_RELXILLLP) ? MAX_ZONES : 1;
+    rzone_grid(p.rin, p.rout, p.n_zones, p.height, p.zgrid);
+    if (p.emis_type == EMIS_LP) {
+      // energy_shift_source_obs (src/Relphysics.cpp:239-254, :134-136, :217-230)
+      const double g_inf_0 = sqrt(1.0 - (2 * p.height / (p.height * p.height + p.a * p.a)));
+      p.g_so = g_inf_0;
+      if (!(p.beta < 1e-4)) p.g_so = g_inf_0 * (sqrt(1.0 - p.beta * p.beta) / (1.0 + p.beta * cos(kPi - p.incl)));
+    }
+    // the lamp post gives Ecut in the observer frame; the xillver tables and
+    // the normalisation use the source's (src/Relxill.cpp:197-200)
+    p.xect_src = (p.emis_type == EMIS_LP) ? p.xect / p.g_so : p.xect;
+  }
   return RX_OK;
+}
+
+void rzone_grid(double rmin, double rmax, int nzones, double h, double *rgrid) {
+  // RadialGrid::calculate_radial_grid (src/IonGradient.cpp:457-489): log
+  // spacing below the source height, 1/r spacing above
+  if (nzones == 1) {
+    rgrid[0] = rmin;
+    rgrid[1] = rmax;
+    return;
+  }
+  double r_transition = rmin;
+  int indr = 0;
+  if (h > rmin) {
+    r_transition = h;
+    for (int ii = 0; ii < nzones + 1; ii++) {  // get_log_grid, src/relutility.c:399-405
+      rgrid[ii] = exp(1.0 * ii / (nzones + 1 - 1) * (log(rmax) - log(rmin)) + log(rmin));
+    }
+    indr = bsearch_asc(rgrid, nzones + 1, r_transition);
+    r_transition = rgrid[indr];
+  }
+  if (indr < nzones) {
+    const double rlo = r_transition, rhi = rmax;
+    for (int ii = indr; ii < nzones + 1; ii++) {
+      rgrid[ii] = 1.0 * (ii - indr) / (nzones - indr) * (1.0 / rhi - 1.0 / rlo) + 1.0 / rlo;
+      rgrid[ii] = fabs(1.0 / rgrid[ii]);
+    }
+  }
 }
 
 // binary_search_float (src/relutility.c:135-151)
@@ -427,6 +483,14 @@ int ensure_xilltable_device(std::string &err) {
   for (int i = 0; i < t.n_ener; i++) g.xill_ener[i] = t.elo[i];
   g.xill_ener[t.n_ener] = t.ehi[t.n_ener - 1];
   int e = upload_vec(g.x_spec, xilltable_renormed(t)) | upload_vec(g.x_ener, g.xill_ener);
+  // the Ecut axis (the lamp post's zones bracket their own Ecut on the device)
+  // and the axis ensure_ecut_within_boundarys reads
+  const int k_ect = t.dim_of(XP_ECT);
+  if (k_ect >= 0) {
+    e |= upload_vec(g.x_ect, t.val[k_ect]);
+    const int q = t.pid[k_ect] < t.n_par ? t.pid[k_ect] : k_ect;
+    e |= upload_vec(g.x_q, t.val[q]);
+  }
   if (e) {
     err = "xillver table upload failed";
     return RX_ERR_DEVICE;
@@ -477,6 +541,17 @@ void fill_dev_views(BatchDev &bd) {
   for (int k = 0; k < 4; k++) bd.xt.n_val[k] = g.xill.n_par ? g.xill.n_val[k] : 0;
   bd.xt.spec = g.x_spec.as<float>();
   bd.xt.ener = g.x_ener.as<double>();
+  bd.xt.ect_dim = g.xill.n_par ? g.xill.dim_of(XP_ECT) : -1;
+  bd.xt.ect_q = -1;
+  bd.xt.n_ect = bd.xt.n_q = 0;
+  if (bd.xt.ect_dim >= 0) {
+    const int q = g.xill.pid[bd.xt.ect_dim] < g.xill.n_par ? g.xill.pid[bd.xt.ect_dim] : bd.xt.ect_dim;
+    bd.xt.ect_q = q;
+    bd.xt.n_ect = g.xill.n_val[bd.xt.ect_dim];
+    bd.xt.n_q = g.xill.n_val[q];
+  }
+  bd.xt.ect_vals = g.x_ect.as<float>();
+  bd.xt.q_vals = g.x_q.as<float>();
 }
 
 // ---------------------------------------------------------------------------
@@ -489,8 +564,13 @@ Batch::~Batch() {
   if (stream) (void)hipStreamDestroy(stream);
 }
 
+static bool is_relxill(int model) { return model == MOD_RELXILL || model == MOD_RELXILLLP; }
+
+// the inclination parameter (degrees) of a relxill model
+static double incl_deg_of(int model, const double *par) { return par[model == MOD_RELXILLLP ? 3 : 4]; }
+
 static bool is_conv(int model) {
-  return model == MOD_RELCONV || model == MOD_RELCONVLP || model == MOD_RELXILL_CONV || model == MOD_RELXILL;
+  return model == MOD_RELCONV || model == MOD_RELCONVLP || model == MOD_RELXILL_CONV || is_relxill(model);
 }
 
 int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
@@ -506,13 +586,13 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
   b.h_conv.clear();
   b.h_xill.clear();
   for (int i = 0; i < n_req; i++) {
-    if (req[i].model == MOD_RELXILL) {
+    if (is_relxill(req[i].model)) {
       int st = ensure_xilltable_device(err);
       if (st != RX_OK) return st;
       break;
     }
   }
-  int64_t o_ener = 0, o_flux = 0, o_pgrid = 0, o_prof = 0, o_xspec = 0;
+  int64_t o_ener = 0, o_flux = 0, o_pgrid = 0, o_prof = 0, o_xspec = 0, o_dist = 0, o_zone = 0;
   for (int i = 0; i < n_req; i++) {
     const rx_request &r = req[i];
     if (r.n_flux <= 0 || !r.energy || !r.parameter) {
@@ -532,25 +612,31 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
     p.flux_off = o_flux;
     o_flux += r.n_flux;
     p.refl_set = r.refl_set;
+    if (p.n_zones < 1) p.n_zones = 1;
     if (is_conv(r.model)) {
       p.n_e = g.conv_n;
       p.pgrid_off = -1;
       p.prof_off = o_prof;
-      o_prof += g.conv_n;
+      o_prof += (int64_t)g.conv_n * p.n_zones;
       b.h_conv.push_back(i);
-      if (r.model == MOD_RELXILL) {
-        // the zone's xillver parameters (src/IonGradient.cpp:196-209, :340-389):
-        // logXi kept within 0-4.7, density 15, Ecut unchanged for the broken
-        // power law
-        double lxi = r.parameter[9];
+      if (is_relxill(r.model)) {
+        // the zones' xillver parameters (src/IonGradient.cpp:196-209, :340-389):
+        // logXi kept within 0-4.7, density 15; the Ecut axis of the lamp
+        // post's zones is bracketed on the device (it depends on the zone's
+        // energy shift)
+        const bool lp = r.model == MOD_RELXILLLP;
+        double lxi = r.parameter[lp ? 8 : 9];
         if (lxi < 0.0) lxi = 0.0;
         else if (lxi > 4.7) lxi = 4.7;
-        xill_corner(g.xill, r.parameter[8], r.parameter[10], lxi, r.parameter[11], 15.0, r.parameter[4],
-                    p.xind, p.xfac);
+        const double afe = r.parameter[lp ? 9 : 10];
+        xill_corner(g.xill, p.xgam, afe, lxi, p.xect_src, 15.0, incl_deg_of(r.model, r.parameter), p.xind,
+                    p.xfac);
         p.n_incl = g.xill.n_incl();
         p.xspec_off = o_xspec;
-        o_xspec += g.xill.n_ener;
-        b.h_xill.push_back(i);
+        o_xspec += (int64_t)g.xill.n_ener * p.n_zones;
+        p.zone_off = o_zone;
+        o_zone += p.n_zones;
+        for (int z = 0; z < p.n_zones; z++) b.h_xill.push_back(XillWork{i, z});
       } else if (st == RX_OK && r.model != MOD_RELXILL_CONV) {
         if (!r.flux_in) {
           err = "convolution request without input spectrum";
@@ -573,6 +659,8 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
       o_pgrid += r.n_flux + 1;
     }
     p.need_cosne = (p.limb != 0 || p.n_incl > 0) ? 1 : 0;
+    p.dist_off = o_dist;
+    o_dist += (int64_t)p.n_incl * p.n_zones;
     p.pre_status = st;
     b.h_status[i] = st;
   }
@@ -581,18 +669,20 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
   b.tot_pgrid = o_pgrid;
   b.tot_prof = o_prof;
   b.tot_xspec = o_xspec;
+  b.tot_dist = o_dist;
+  b.tot_zone = o_zone;
   b.h_ener.resize(o_ener);
   b.h_flux_in.assign(o_flux, 0.0);
   for (int i = 0; i < n_req; i++) {
     const rx_request &r = req[i];
     const ReqParams &p = b.h_req[i];
     memcpy(&b.h_ener[p.ener_off], r.energy, sizeof(double) * (r.n_flux + 1));
-    if (r.model == MOD_RELXILL && p.z > 0) {
+    if (is_relxill(r.model) && p.z > 0) {
       // relxill evaluates on the redshifted grid (shift_energy_grid_redshift,
       // src/XspecSpectrum.h:70-76)
       for (int k = 0; k <= r.n_flux; k++) b.h_ener[p.ener_off + k] *= (1 + p.z);
     }
-    if (r.flux_in && is_conv(r.model) && r.model != MOD_RELXILL)
+    if (r.flux_in && is_conv(r.model) && !is_relxill(r.model))
       memcpy(&b.h_flux_in[p.flux_off], r.flux_in, sizeof(double) * r.n_flux);
   }
   b.any_cosne = false;
@@ -626,11 +716,14 @@ void batch_plan(Batch &b, int seg_bins, int waves_per_cu) {
   if (waves_per_cu <= 0) waves_per_cu = 8;
   b.seg_bins = seg_bins;
   b.waves_per_cu = waves_per_cu;
-  // work list bound: every radius chunk of every request covering all its bins
+  // work list bound: every (radius chunk, zone) pair of every request covering
+  // all its bins; the zones are contiguous radius ranges, so a request has at
+  // most n_rc + n_zones - 1 such pairs
   const int n_rc = (g.nr_fine + RAD_CHUNK - 1) / RAD_CHUNK;
   int64_t cap = 0;
   for (int i = 0; i < b.n_req; i++)
-    if (b.h_status[i] == RX_OK) cap += (int64_t)n_rc * ((b.h_req[i].n_e + seg_bins - 1) / seg_bins);
+    if (b.h_status[i] == RX_OK)
+      cap += (int64_t)(n_rc + b.h_req[i].n_zones - 1) * ((b.h_req[i].n_e + seg_bins - 1) / seg_bins);
   b.item_cap = (int)std::min<int64_t>(std::max<int64_t>(cap, 1), INT32_MAX);
 }
 
@@ -651,11 +744,12 @@ int batch_upload(Batch &b) {
   e |= b.d_rad.ensure(sizeof(RadRec) * nreq * g.nr_fine);
   e |= b.d_rnorm.ensure(sizeof(ReqNorm) * nreq);
   e |= b.d_prof.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_prof));
-  e |= b.d_dist.ensure(sizeof(double) * nreq * MAX_INCL);
+  e |= b.d_dist.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_dist));
+  e |= b.d_zone.ensure(sizeof(ZoneRec) * std::max<int64_t>(1, b.tot_zone));
   e |= b.d_refl.ensure(sizeof(LpRefl) * nreq);
   e |= b.d_status.ensure(sizeof(int) * nreq);
   e |= b.d_conv.ensure(sizeof(int) * std::max<size_t>(1, b.h_conv.size()));
-  e |= b.d_xill.ensure(sizeof(int) * std::max<size_t>(1, b.h_xill.size()));
+  e |= b.d_xill.ensure(sizeof(XillWork) * std::max<size_t>(1, b.h_xill.size()));
   e |= b.d_xspec.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_xspec));
   e |= b.d_items.ensure(sizeof(WorkItem) * (size_t)b.item_cap);
   e |= b.d_ctr.ensure(sizeof(int) * N_COUNTERS);
@@ -673,7 +767,8 @@ int batch_upload(Batch &b) {
   if (!b.h_conv.empty())
     e |= (int)hipMemcpyAsync(b.d_conv.p, b.h_conv.data(), sizeof(int) * b.h_conv.size(), hipMemcpyHostToDevice, b.stream);
   if (!b.h_xill.empty())
-    e |= (int)hipMemcpyAsync(b.d_xill.p, b.h_xill.data(), sizeof(int) * b.h_xill.size(), hipMemcpyHostToDevice, b.stream);
+    e |= (int)hipMemcpyAsync(b.d_xill.p, b.h_xill.data(), sizeof(XillWork) * b.h_xill.size(), hipMemcpyHostToDevice,
+                             b.stream);
   e |= (int)hipStreamSynchronize(b.stream);
   return e ? RX_ERR_DEVICE : RX_OK;
 }
@@ -708,6 +803,7 @@ static BatchDev dev_view(Batch &b) {
   bd.guard = g.guard;
   fill_dev_views(bd);
   bd.xspec = b.d_xspec.as<double>();
+  bd.zone = b.d_zone.as<ZoneRec>();
   return bd;
 }
 
@@ -733,7 +829,7 @@ int batch_run(Batch &b, int precision, std::string &err) {
   if (prof) e |= (int)hipEventRecord(b.ev[3], s);
   e |= launch_finalize(bd, s);
   if (prof) e |= (int)hipEventRecord(b.ev[4], s);
-  e |= launch_xill(bd, b.d_xill.as<int>(), (int)b.h_xill.size(), s);
+  e |= launch_xill(bd, b.d_xill.as<XillWork>(), (int)b.h_xill.size(), s);
   if (prof) e |= (int)hipEventRecord(b.ev[5], s);
   e |= launch_conv(bd, b.d_conv.as<int>(), (int)b.h_conv.size(), s);
   if (prof) e |= (int)hipEventRecord(b.ev[6], s);

@@ -30,6 +30,8 @@ int parse_request(int model, const double *par, bool phys_norm, ReqParams &p, st
 void xill_corner(const XillTableHost &t, double gam, double afe, double lxi, double ect, double dens,
                  double incl_deg, int *ind, double *fac);
 int n_model_params(int model);
+// Edges of the radial zones (RadialGrid::calculate_radial_grid, src/IonGradient.cpp:457-489).
+void rzone_grid(double rmin, double rmax, int nzones, double h, double *rgrid);
 
 struct Batch {
   int n_req = 0;
@@ -40,10 +42,11 @@ struct Batch {
   std::vector<ReqParams> h_req;
   std::vector<int> h_status;  // host-side parse status
   std::vector<double> h_ener, h_flux_in;
-  std::vector<int> h_conv, h_xill;
-  int64_t tot_ener = 0, tot_flux = 0, tot_pgrid = 0, tot_prof = 0, tot_xspec = 0;
+  std::vector<int> h_conv;
+  std::vector<XillWork> h_xill;
+  int64_t tot_ener = 0, tot_flux = 0, tot_pgrid = 0, tot_prof = 0, tot_xspec = 0, tot_dist = 0, tot_zone = 0;
   DevBuf d_req, d_ener, d_flux_in, d_out, d_pgrid, d_node, d_rad, d_prof, d_dist, d_refl, d_status, d_conv;
-  DevBuf d_xill, d_xspec;
+  DevBuf d_xill, d_xspec, d_zone;
   DevBuf d_items, d_ctr, d_fix, d_reach, d_rnorm;
   int item_cap = 0;
   bool any_cosne = false, any_limb = false;
@@ -88,7 +91,7 @@ struct Global {
   XillTableHost xill;
   bool xill_loaded = false, xill_dirty = true;
   std::vector<double> xill_ener;
-  DevBuf x_spec, x_ener;
+  DevBuf x_spec, x_ener, x_ect, x_q;
   // convolution grid
   int conv_n = 4096;
   double conv_emin = 0.00035, conv_emax = 2000.0;

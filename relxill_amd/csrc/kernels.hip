@@ -40,6 +40,26 @@ __device__ inline double lp_ifac(float value, const float *arr, int n, int &ind)
   return (double)((value - arr[ind]) / (arr[ind + 1] - arr[ind]));
 }
 
+// spec_cutoffpl (src/Xillspec.cpp:215-241): photons of the cut-off power law in [e0, e1]
+__device__ inline double cutoffpl_bin(double e0, double e1, double ecut, double gam) {
+  const double en = 0.5 * (e0 + e1);
+  return exp(1.0 / ecut) * pow(en, -gam) * exp(-en / ecut) * (e1 - e0);
+}
+
+// 1e15 / (4 pi): the xillver tables' incident flux normalisation (src/Xillspec.cpp:203)
+constexpr double NORM_XILLVER_TABLE = 1e15 / 4.0 / 3.14159265358979323846;
+
+// This thread's part of calcNormWrtXillverTableSpec's sum (src/Xillspec.cpp:179-205)
+// for the cut-off power law on the 500-bin 0.1-1000 keV grid ce.
+__device__ inline double xill_norm_part(const double *ce, double ecut, double gam, int tid, int nt) {
+  double sp = 0.0;
+  for (int k = tid; k < N_ENER_COARSE; k += nt) {
+    const double e0 = ce[k], e1 = ce[k + 1];
+    if (e0 >= 0.1 && e0 <= 1000.0) sp += cutoffpl_bin(e0, e1, ecut, gam) * 0.5 * (e0 + e1) * 1e20 * 1.602177e-09;
+  }
+  return sp;
+}
+
 struct LpFine {
   double emis, del_emit, del_inc;
   int err;
@@ -257,14 +277,17 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
   __shared__ double s_tre[MAX_NR_TAB], s_tgmin[MAX_NR_TAB], s_tgmax[MAX_NR_TAB];
   __shared__ double s_jr[MAX_LP_R], s_je[MAX_LP_R], s_jd[MAX_LP_R], s_ji[MAX_LP_R];
   __shared__ double s_red[SETUP_THREADS];
-  __shared__ double s_dist[MAX_INCL];
+  __shared__ double s_dist[MAX_ZONES * MAX_INCL];
   __shared__ int s_st, s_ia, s_im, s_rmin, s_lia, s_lih0, s_lih1;
   __shared__ double s_fa, s_fm, s_rms, s_lfa, s_lfh0, s_lfh1;
   constexpr int MAX_RC = MAX_NR_FINE / RAD_CHUNK;
-  __shared__ int s_ulo[MAX_RC], s_uhi[MAX_RC], s_pref[MAX_RC + 1];
+  // energy window of each (radius chunk, zone): bins [s_ulo, s_uhi]
+  __shared__ int s_ulo[MAX_RC * MAX_ZONES], s_uhi[MAX_RC * MAX_ZONES], s_pref[MAX_RC * MAX_ZONES + 1];
   __shared__ int s_base;
   const int n_rc = (b.nr_fine + RAD_CHUNK - 1) / RAD_CHUNK;
-  for (int c = tid; c < n_rc; c += SETUP_THREADS) {
+  const int nz = rq.n_zones;
+  const double *zg = b.req[req].zgrid;
+  for (int c = tid; c < n_rc * MAX_ZONES; c += SETUP_THREADS) {
     s_ulo[c] = 0x7fffffff;
     s_uhi[c] = -1;
   }
@@ -403,6 +426,7 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
   for (int ii0 = 0; ii0 < nr; ii0 += SETUP_THREADS) {  // same trip count in every wavefront
     const int ii = ii0 + tid;
     int wlo = 0x7fffffff, whi = -1;
+    int izone = -1;
     if (ii < nr) {
     const double re = fine_radius(rq.rin, rq.rout, ii, nr);
     const double rprev = (ii > 0) ? fine_radius(rq.rin, rq.rout, ii - 1, nr) : 0.0;
@@ -454,6 +478,8 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
       ielo = min(count_le(ge, rq.n_e, egmin, gE) - 1, rq.n_e - 1);
       iehi = min(count_le(ge, rq.n_e, egmax, gE) - 1, rq.n_e - 1);
     }
+    // zone of the radius (binary_search on the zone edges, src/Relprofile.cpp:874)
+    izone = (nz > 1) ? bsearch_asc(zg, nz + 1, re) : 0;
     RadRec r;
     r.gmin = gmin;
     r.gmax = gmax;
@@ -462,7 +488,7 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     r.K = K;
     r.ielo = ielo;
     r.iehi = iehi;
-    r.pad = 0;
+    r.pad = izone;
     rad[ii] = r;
     if (ielo <= iehi) {
       wlo = ielo;
@@ -470,12 +496,17 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
       cmax = fmax(cmax, fabs(r.coef));
     }
     }  // ii < nr
-    // energy window of the radius chunk: this wavefront's 64 consecutive radii
-    wlo = wave_min_i(wlo);
-    whi = wave_max_i(whi);
-    if ((tid & 63) == 0 && ii < nr) {
-      s_ulo[ii / RAD_CHUNK] = wlo;
-      s_uhi[ii / RAD_CHUNK] = whi;
+    // energy window of the radius chunk (this wavefront's 64 consecutive
+    // radii), per zone present in the chunk
+    const int zlo = wave_min_i(ii < nr ? izone : 0x7fffffff);
+    const int zhi = wave_max_i(ii < nr ? izone : -1);
+    for (int z = zlo; z <= zhi; z++) {
+      const int zl = wave_min_i(izone == z ? wlo : 0x7fffffff);
+      const int zh = wave_max_i(izone == z ? whi : -1);
+      if ((tid & 63) == 0) {
+        s_ulo[(ii / RAD_CHUNK) * MAX_ZONES + z] = zl;
+        s_uhi[(ii / RAD_CHUNK) * MAX_ZONES + z] = zh;
+      }
     }
   }
   if (err) atomicMax(&s_st, err);
@@ -533,13 +564,81 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     b.refl[req] = o;
   }
 
+  // zones of a relxill request (IonGradient, src/IonGradient.cpp:182-209,
+  // :421-430): the centre radius, its emission angle (inv_rebin_mean of the
+  // fine-grid angles, src/relutility.c:636-663), the source -> disk energy
+  // shift, the Ecut the zone sees with its xillver bracket (float factor,
+  // clamps, ensure_ecut_within_boundarys, src/xilltable.c:1054-1125) and the
+  // normalisation change of the incident spectrum (src/Xillspec.cpp:408-435)
+  const bool relxill_model = (rq.model == MOD_RELXILL || rq.model == MOD_RELXILLLP);
+  if (relxill_model && s_st == 0) {
+    ZoneRec *zr = b.zone + rq.zone_off;
+    const DevXillTable &xt = b.xt;
+    __shared__ double s_zect[MAX_ZONES];
+    for (int z = tid; z < nz; z += SETUP_THREADS) {
+      ZoneRec o;
+      o.rmean = 0.5 * (zg[z] + zg[z + 1]);
+      o.del_emit = 0.0;
+      o.shift = 1.0;
+      o.ncf = 1.0;
+      o.ect_ind = xt.ect_dim >= 0 ? b.req[req].xind[xt.ect_dim] : 0;
+      o.ect_fac = xt.ect_dim >= 0 ? b.req[req].xfac[xt.ect_dim] : 0.0;
+      o.pad = 0;
+      if (lp) {
+        int lo = 0, hi = nr - 1;  // re(lo) > rmean >= re(hi)
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (fine_radius(rq.rin, rq.rout, mid, nr) > o.rmean) lo = mid; else hi = mid;
+        }
+        const double r0 = fine_radius(rq.rin, rq.rout, lo, nr), r1 = fine_radius(rq.rin, rq.rout, lo + 1, nr);
+        const LpFine f0 = lp_on_fine(r0, s_jr, s_je, s_jd, s_ji, ntr);
+        const LpFine f1 = lp_on_fine(r1, s_jr, s_je, s_jd, s_ji, ntr);
+        o.del_emit = lerp1((o.rmean - r1) / (r0 - r1), f1.del_emit, f0.del_emit);
+        o.shift = gi_potential_lp(o.rmean, rq.a, rq.height, rq.beta, o.del_emit);
+        const double ect = rq.xect_src * o.shift;
+        s_zect[z] = ect;
+        if (xt.ect_dim >= 0) {
+          const float *vals = xt.ect_vals;
+          const int n = xt.n_ect;
+          float v = (float)ect;
+          int ind = bsearch_asc(vals, n, v);
+          if (ind < 0) ind = 0;
+          else if (ind > n - 2) ind = n - 2;
+          if (v < vals[0]) v = vals[0];
+          else if (v > vals[n - 1]) v = vals[n - 1];
+          const float num = v - vals[ind];
+          const float den = vals[ind + 1] - vals[ind];
+          double fac = (float)(num / den);
+          if (ect <= xt.q_vals[0]) fac = 0.0;
+          if (ect >= xt.q_vals[xt.n_q - 1]) fac = 1.0;
+          o.ect_ind = ind;
+          o.ect_fac = fac;
+        }
+      }
+      zr[z] = o;
+    }
+    __syncthreads();
+    if (lp) {
+      const double src = 1. / (block_sum(xill_norm_part(b.cg.coarse, rq.xect_src, rq.xgam, tid, SETUP_THREADS),
+                                         s_red) / NORM_XILLVER_TABLE);
+      for (int z = 0; z < nz; z++) {
+        const double ez = s_zect[z];
+        const double disk =
+            1. / (block_sum(xill_norm_part(b.cg.coarse, ez, rq.xgam, tid, SETUP_THREADS), s_red) / NORM_XILLVER_TABLE);
+        if (tid == 0) zr[z].ncf = disk / src;
+      }
+    }
+    __syncthreads();
+  }
+
   // emission-angle distribution (:908-938).  Lane = fine radius, loop over
   // the 2 n_g (g*, branch) columns: a radius' record is read once and its
   // node rows are contiguous.  Neighbouring radii mostly fall into the same
   // angle bin, so a wavefront whose valid lanes share the bin adds one wave
   // sum to it; otherwise every lane adds its own term.
   if (rq.n_incl > 0) {
-    for (int k = tid; k < rq.n_incl; k += SETUP_THREADS) s_dist[k] = 0.0;
+    const int nd = rq.n_incl * nz;  // [zone][angle bin]
+    for (int k = tid; k < nd; k += SETUP_THREADS) s_dist[k] = 0.0;
     __syncthreads();
     const double *node = b.node + (size_t)req * b.node_stride;
     const size_t nq2 = (size_t)n_r * ng * 2;
@@ -586,6 +685,7 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
           if (imu < 0 || imu >= rq.n_incl) derr = 40;
           else if (tmp != tmp) derr = 41;
           else ok = true;
+          imu += r.pad * rq.n_incl;  // the radius' zone
         }
         const unsigned long long m_ok = __ballot(ok);
         if (m_ok == 0) continue;
@@ -601,7 +701,7 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     }
     if (derr) atomicMax(&s_st, derr);
     __syncthreads();
-    for (int k = tid; k < rq.n_incl; k += SETUP_THREADS) b.dist[(size_t)req * MAX_INCL + k] = s_dist[k];
+    for (int k = tid; k < nd; k += SETUP_THREADS) b.dist[rq.dist_off + k] = s_dist[k];
   }
   __syncthreads();
   if (s_st == 0) {
@@ -609,24 +709,31 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     // reaches (k_finalize writes zeros elsewhere)
     int jlo = 0x7fffffff, jhi = -1;
     for (int c = 0; c < n_rc; c++) {
-      jlo = min(jlo, s_ulo[c]);
-      jhi = max(jhi, s_uhi[c]);
+      for (int z = 0; z < nz; z++) {
+        jlo = min(jlo, s_ulo[c * MAX_ZONES + z]);
+        jhi = max(jhi, s_uhi[c * MAX_ZONES + z]);
+      }
     }
     if (tid == 0) b.reach[req] = BinRange{jlo, jhi};
     double *acc = profile_acc(b, rq);
     // (no window reaches the grid: jlo = INT_MAX > jhi = -1, nothing to clear;
-    // jlo + tid must not be formed then, it would overflow)
+    // jlo + tid must not be formed then, it would overflow); every zone's
+    // profile over the union of the windows
     if (jlo <= jhi)
-      for (int j = jlo + tid; j <= jhi; j += SETUP_THREADS) acc[j] = 0.0;
-    // work items: each radius chunk's window cut into segments of seg_bins
+      for (int z = 0; z < nz; z++)
+        for (int j = jlo + tid; j <= jhi; j += SETUP_THREADS) acc[(size_t)z * rq.n_e + j] = 0.0;
+    // work items: the window of each (radius chunk, zone) cut into segments
+    // of seg_bins; the item's chunk and zone share its rc field
     const int S = b.seg_bins;
+    const int n_cz = n_rc * nz;  // pairs q = c nz + z; their windows sit at c MAX_ZONES + z
     if (tid == 0) {
       int tot = 0;
-      for (int c = 0; c < n_rc; c++) {
-        s_pref[c] = tot;
-        if (s_uhi[c] >= s_ulo[c]) tot += (s_uhi[c] - s_ulo[c] + S) / S;
+      for (int q = 0; q < n_cz; q++) {
+        const int w = (q / nz) * MAX_ZONES + q % nz;
+        s_pref[q] = tot;
+        if (s_uhi[w] >= s_ulo[w]) tot += (s_uhi[w] - s_ulo[w] + S) / S;
       }
-      s_pref[n_rc] = tot;
+      s_pref[n_cz] = tot;
       int base = atomicAdd(&b.ctr[CTR_ITEMS], tot);
       if (base + tot > b.item_cap) {  // cannot happen: the host sizes the list for every bin
         atomicOr(&b.ctr[CTR_ITEMS_OVF], 1);
@@ -637,14 +744,15 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     }
     __syncthreads();
     if (s_base >= 0) {
-      for (int c = tid; c < n_rc; c += SETUP_THREADS) {
-        int k = s_base + s_pref[c];
-        for (int j0 = s_ulo[c]; j0 <= s_uhi[c]; j0 += S) {
+      for (int q = tid; q < n_cz; q += SETUP_THREADS) {
+        const int c = q / nz, z = q % nz, w = c * MAX_ZONES + z;
+        int k = s_base + s_pref[q];
+        for (int j0 = s_ulo[w]; j0 <= s_uhi[w]; j0 += S) {
           WorkItem it;
           it.req = req;
-          it.rc = c;
+          it.rc = c | (z << 16);
           it.j0 = j0;
-          it.j1 = min(s_uhi[c] + 1, j0 + S);
+          it.j1 = min(s_uhi[w] + 1, j0 + S);
           b.items[k++] = it;
         }
       }
@@ -779,13 +887,18 @@ __global__ __launch_bounds__(64, MODE == 0 ? 1 : 4) void k_integrate(BatchDev b,
     const ReqParams *rq = b.req + it.req;
     const ReqNorm rn = b.rnorm[it.req];
     const int limb = rq->limb;
-    const int i = it.rc * RAD_CHUNK + lane;
+    const int i = (it.rc & 0xffff) * RAD_CHUNK + lane;
+    const int zone = it.rc >> 16;
     RadRec r;
     r.ielo = 1;
     r.iehi = 0;
     r.coef = 0.0;
     if (i < nr) {
       r = b.rad[(size_t)it.req * nr + i];
+      if (r.pad != zone) {  // a radius of another zone of the chunk
+        r.ielo = 1;
+        r.iehi = 0;
+      }
       if (r.ielo <= r.iehi) load_radius<TV>(b, it.req, r, limb, myT, myC, v);
     }
     const bool has_win = r.ielo <= r.iehi;
@@ -793,7 +906,7 @@ __global__ __launch_bounds__(64, MODE == 0 ? 1 : 4) void k_integrate(BatchDev b,
     const double w32 = has_win ? r.coef * rn.cmul : 0.0;  // MODE 1 scaled weight
     if constexpr (MODE == 0) s_coef[lane] = w64;
     const double *E = profile_grid(b, rq);
-    double *acc = profile_acc(b, *rq);
+    double *acc = profile_acc(b, *rq) + (size_t)zone * rq->n_e;
     double ehi = E[it.j0];
     double enext = E[min(it.j0 + 1, it.j1)];
     double ghi = (ehi - v.gmin) * v.del_g;  // unclamped g* of the bin's upper edge, carried
@@ -883,7 +996,7 @@ __global__ __launch_bounds__(64) void k_fixup(BatchDev b, int cos_rows) {
     load_radius<double>(b, fe.req, r, rq->limb, myT, myC, v);
     const double *E = profile_grid(b, rq);
     const double val = integ_bin<0>(E[fe.bin], E[fe.bin + 1], v) * (r.coef * b.rnorm[fe.req].fac);
-    unsafeAtomicAdd(&profile_acc(b, *rq)[fe.bin], val - fe.v32);
+    unsafeAtomicAdd(&profile_acc(b, *rq)[(size_t)r.pad * rq->n_e + fe.bin], val - fe.v32);
   }
 }
 
@@ -899,7 +1012,7 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_finalize(BatchDev b) {
   double *dst = profile_acc(b, rq);
   const int st = b.status[req];
   if (st != 0) {
-    for (int j = tid; j < n_e; j += SETUP_THREADS) dst[j] = 0.0;
+    for (int j = tid; j < n_e * rq.n_zones; j += SETUP_THREADS) dst[j] = 0.0;
     return;
   }
   if (rq.prof_off < 0) {  // line models; convolution profiles are finished by k_conv
@@ -918,8 +1031,8 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_finalize(BatchDev b) {
       for (int j = tid; j < n_e; j += SETUP_THREADS) dst[j] *= f;
     }
   }
-  if (rq.n_incl > 0) {
-    double *d = b.dist + (size_t)req * MAX_INCL;
+  for (int z = 0; rq.n_incl > 0 && z < rq.n_zones; z++) {  // each zone's distribution sums to 1
+    double *d = b.dist + rq.dist_off + (size_t)z * rq.n_incl;
     double ds = 0.0;
     for (int k = tid; k < rq.n_incl; k += SETUP_THREADS) ds += d[k];
     const double dsum = block_sum(ds, s_red);
@@ -1209,19 +1322,56 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
   }
   const double *Ec = cg.ener;
   const double *cf = cg.cfac;
-  const double *prof = b.prof + rq.prof_off;
   const double2 *tw = reinterpret_cast<const double2 *>(cg.tw);
   const bool angular = (rq.model == MOD_RELXILL_CONV);
-  const bool relxill = (rq.model == MOD_RELXILL);
-  // input: the caller's spectrum, or for relxill the angle-weighted xillver
-  // spectrum on the table grid (k_xill)
+  const bool relxill = (rq.model == MOD_RELXILL || rq.model == MOD_RELXILLLP);
+  // input: the caller's spectrum, or for relxill each zone's angle-weighted
+  // xillver spectrum on the table grid (k_xill)
   const double *ein = relxill ? b.xt.ener : eo;
-  const double *fin = relxill ? b.xspec + rq.xspec_off : b.flux_in + rq.flux_off;
   const int n_in = relxill ? b.xt.n_ener : n_o;
-  const double *dist = b.dist + (size_t)req * MAX_INCL;
   const DevReflSet *rs = angular ? &b.refl_sets[rq.refl_set] : nullptr;
   auto ge_in = [&](int k) { return ein[k]; };
+  const BinRange rc = b.reach[req];
+  const int n_zones = relxill ? rq.n_zones : 1;
+  // relxill: reflection scale and primary factors (add_primary_spectrum,
+  // src/PrimarySource.cpp:66-125): |refl_frac| for the broken power law; for
+  // the lamp post |refl_frac| (times the lamp post's own reflection fraction
+  // when refl_frac is a boost) over that fraction, and the primary's flux
+  // boost f_inf_rest / 0.5 g^Gamma (times the Doppler factor squared for a
+  // moving source, src/PrimarySource.h:84-93)
+  double refl_scale = 1.0, prim_scale = 1.0;
+  if (relxill) {
+    if (rq.emis_type != EMIS_LP) {
+      refl_scale = fabs(rq.refl_frac);
+    } else {
+      const LpRefl lr = b.refl[req];
+      double rfi = rq.refl_frac;
+      if (rq.xboost) rfi *= lr.refl_frac;
+      refl_scale = (fabs(rfi)) / lr.refl_frac;
+      prim_scale = lr.f_inf_rest / 0.5 * pow(rq.g_so, rq.xgam);
+      if (rq.beta > 1e-4) {
+        const double dop = sqrt(1.0 - rq.beta * rq.beta) / (1.0 + rq.beta * cos(M_PI - rq.incl));
+        prim_scale *= pow(dop, 2);
+      }
+    }
+  }
+  const int nbo = (n_o + nwave - 1) / nwave;  // caller bins per wavefront (phase 5)
+  const int jend = min(n_o, (wave + 1) * nbo);
+  bool wrote = false;  // relxill: a zone's result is in out already
+  auto o_at = [&](int i) -> double & {
+    double2 &z = s_buf[P(i >> 1)];
+    return (i & 1) ? z.y : z.x;
+  };
+  // the conv grid edges go to the upper part of the buffer, free after the
+  // half-length FFT (its values occupy slots below P(m))
+  double *s_e = reinterpret_cast<double *>(s_buf + P(m));
+  const GridGuess gc = grid_guess(Ec, n);
+  auto ge_c = [&](int k) { return s_e[k]; };
 
+  for (int zone = 0; zone < n_zones; zone++) {
+  const double *prof = b.prof + rq.prof_off + (size_t)zone * n;
+  const double *fin = relxill ? b.xspec + rq.xspec_off + (size_t)zone * n_in : b.flux_in + rq.flux_off;
+  const double *dist = b.dist + rq.dist_off + (size_t)zone * rq.n_incl;
   // 1. x = rebinned input * E/dE (real part), y = rotated profile * E/dE (imag part)
   double sx = 0.0;
   const int nb = n / nwave;  // bins per wavefront (a multiple of 64)
@@ -1306,7 +1456,6 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
   }
   // the profile as photons per bin, renormalised (renorm_relline_profile,
   // src/Relprofile.cpp:757-779); bins no window reaches are zero
-  const BinRange rc = b.reach[req];
   const int per = n / NT;  // <= 8: the block has at least n/8 threads
   double ph[8];
   double sph = 0.0;
@@ -1346,6 +1495,9 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
     sum_xill = 0.0;
     for (int k = 0; k < rs->n_incl; k++) sum_xill += dist[k] * rs->bandsum[k];
   }
+  // a zone whose profile sums below 1e-12 is left out (src/Relxill.cpp:453-456)
+  const bool zone_used = !(sum_prof < 1e-12);
+  if (relxill && !zone_used) continue;  // (block-uniform; the buffer is not read after the last barrier)
   // 2. forward complex FFT of x + i y
   lds_fft<-1>(s_buf, n, reinterpret_cast<const double2 *>(cg.twp_fwd));
   // 3. X = (W_k + conj W_{n-k})/2, Y = (W_k - conj W_{n-k})/(2i), P = X Y (k = 0..m);
@@ -1395,10 +1547,6 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
   __syncthreads();
   // 4. half-length inverse FFT: z_j = out_{2j} + i out_{2j+1}
   lds_fft<+1>(s_buf, m, reinterpret_cast<const double2 *>(cg.twp_inv));
-  auto o_at = [&](int i) -> double & {
-    double2 &z = s_buf[P(i >> 1)];
-    return (i & 1) ? z.y : z.x;
-  };
   double sc = 0.0;
 #pragma unroll 8
   for (int i = tid; i < n; i += NT) {
@@ -1410,52 +1558,13 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
   // normalizeFFTOutput (src/Relbase.cpp:199-222): applied to the rebinned
   // values below (the rebin is linear), which saves a pass over the buffer
   double fnorm = sum_rel * sum_xill / sum_conv;
-  // a zone whose profile sums below 1e-12 is left out (src/Relxill.cpp:453-456)
-  const bool zone_used = !(sum_prof < 1e-12);
   if (angular && !zone_used) fnorm = 0.0;
-  if (relxill) {
-    // add_primary_spectrum for the broken power law (src/PrimarySource.cpp:66-125):
-    // reflection times |refl_frac|, plus the cut-off power law (spec_cutoffpl,
-    // src/Xillspec.cpp:215-241) normalised as the xillver incident spectrum
-    // (calcNormWrtXillverTableSpec on the 500-bin 0.1-1000 keV grid,
-    // src/Xillspec.cpp:179-205, src/PrimarySource.h:279-292) for refl_frac >= 0
-    const double ecut = rq.xect, gam = rq.xgam;
-    const double e1c = exp(1.0 / ecut);
-    double sp = 0.0;
-    for (int k = tid; k < N_ENER_COARSE; k += NT) {
-      const double e0 = cg.coarse[k], e1 = cg.coarse[k + 1];
-      if (e0 >= 0.1 && e0 <= 1000.0) {
-        const double en = 0.5 * (e0 + e1);
-        const double fl = e1c * pow(en, -gam) * exp(-en / ecut) * (e1 - e0);
-        sp += fl * 0.5 * (e0 + e1) * 1e20 * 1.602177e-09;
-      }
-    }
-    const double nf = 1. / (block_sum4(sp, s_w[0]) / (1e15 / 4.0 / M_PI));
-    const double arf = fabs(rq.refl_frac);
-    const bool add_prim = rq.refl_frac >= 0;
-#pragma unroll 8
-    for (int i = tid; i < n; i += NT) {
-      double v = zone_used ? (o_at(i) * fnorm) * arf : 0.0;
-      if (add_prim) {
-        const double e0 = Ec[i], e1 = Ec[i + 1];
-        const double en = 0.5 * (e0 + e1);
-        v += (e1c * pow(en, -gam) * exp(-en / ecut) * (e1 - e0)) * nf;
-      }
-      o_at(i) = v;
-    }
-    fnorm = 1.0;
-  }
-  // the conv grid edges go to the upper part of the buffer, free after the
-  // half-length FFT (its values occupy slots below P(m))
-  double *s_e = reinterpret_cast<double *>(s_buf + P(m));
+  if (relxill) fnorm *= refl_scale;
 #pragma unroll 8
   for (int i = tid; i <= n; i += NT) s_e[i] = Ec[i];
   __syncthreads();
-  // 5. rebin onto the caller grid; relconv zeroes outside 0.01-1000 keV
-  const GridGuess gc = grid_guess(Ec, n);
-  auto ge_c = [&](int k) { return s_e[k]; };
-  const int nbo = (n_o + nwave - 1) / nwave;
-  const int jend = min(n_o, (wave + 1) * nbo);
+  // 5. rebin onto the caller grid; relconv zeroes outside 0.01-1000 keV;
+  // relxill adds the zones up (each output bin stays with its thread)
   const bool relconv = !angular && !relxill;
 #pragma unroll 2
   for (int q = 0; q < nbo; q += 64) {
@@ -1466,7 +1575,37 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
       const int chi = count_le(ge_c, n, ehi, gc);
       double v = fnorm * rebin_bin(elo, ehi, clo, chi, ge_c, n, [&](int i) { return o_at(i); });
       if (relconv && (ehi < 0.01 || elo > 1000.0)) v = 0.0;
-      out[jb] = v;
+      out[jb] = wrote ? out[jb] + v : v;
+    }
+  }
+  wrote = true;
+  __syncthreads();  // the next zone overwrites the buffer
+  }  // zones
+
+  if (relxill) {
+    // the primary: the cut-off power law with the source's Ecut shifted to the
+    // observer (spec_cutoffpl, src/Xillspec.cpp:215-241), normalised as the
+    // xillver incident spectrum at the source (calcNormWrtXillverTableSpec on
+    // the 500-bin grid, src/PrimarySource.h:238-292), added for refl_frac >= 0
+    const double ect = rq.xect_src, gam = rq.xgam;
+    const double nf = 1. / (block_sum4(xill_norm_part(cg.coarse, ect, gam, tid, NT), s_w[0]) / NORM_XILLVER_TABLE);
+    const bool add_prim = rq.refl_frac >= 0;
+#pragma unroll 8
+    for (int i = tid; i < n; i += NT)
+      o_at(i) = add_prim ? (cutoffpl_bin(Ec[i], Ec[i + 1], ect * rq.g_so, gam) * nf) * prim_scale : 0.0;
+#pragma unroll 8
+    for (int i = tid; i <= n; i += NT) s_e[i] = Ec[i];
+    __syncthreads();
+#pragma unroll 2
+    for (int q = 0; q < nbo; q += 64) {
+      const int jb = wave * nbo + q + lane;
+      if (jb < jend) {
+        const double elo = eo[jb], ehi = eo[jb + 1];
+        const int clo = count_le(ge_c, n, elo, gc);
+        const int chi = count_le(ge_c, n, ehi, gc);
+        const double v = add_prim ? rebin_bin(elo, ehi, clo, chi, ge_c, n, [&](int i) { return o_at(i); }) : 0.0;
+        out[jb] = wrote ? out[jb] + v : v;
+      }
     }
   }
 }
@@ -1477,20 +1616,25 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
 // weight products) weighted by the angle distribution (calc_xillver_angdep,
 // src/Xillspec.cpp:557-573).  One thread per bin of the table grid; every
 // table read is coalesced across the block.  Angles with zero weight are
-// skipped (they add exactly 0).
-__global__ __launch_bounds__(XILL_THREADS) void k_xill(BatchDev b, const int *__restrict__ reqs, int chunks) {
-  const int req = reqs[blockIdx.x / chunks];
+// skipped (they add exactly 0).  One unit per (request, zone): the zone's Ecut
+// bracket, its angle distribution and its normalisation change
+// (src/Relxill.cpp:357-388).
+__global__ __launch_bounds__(XILL_THREADS) void k_xill(BatchDev b, const XillWork *__restrict__ work, int chunks) {
+  const XillWork wk = work[blockIdx.x / chunks];
+  const int req = wk.req;
   if (b.status[req] != 0) return;
   const DevXillTable xt = b.xt;
   const int e = (blockIdx.x % chunks) * XILL_THREADS + threadIdx.x;
   if (e >= xt.n_ener) return;
   const ReqParams &rq = b.req[req];
+  const ZoneRec zr = b.zone[rq.zone_off + wk.zone];
   int i0[4];
   double f[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    i0[k] = rq.xind[k];
-    f[k] = rq.xfac[k];
+    const bool ect = (k == xt.ect_dim);
+    i0[k] = ect ? zr.ect_ind : rq.xind[k];
+    f[k] = ect ? zr.ect_fac : rq.xfac[k];
   }
   // corner c has the offsets of bits 3, 2, 1, 0 of kCorner[c] on axes 0..3
   constexpr unsigned char kCorner[16] = {0x0, 0x8, 0x4, 0x2, 0xC, 0xA, 0x6, 0xE,
@@ -1511,7 +1655,7 @@ __global__ __launch_bounds__(XILL_THREADS) void k_xill(BatchDev b, const int *__
                          xt.n_val[3] + (i0[3] + o[3]);
     row[c] = r * xt.n_incl;
   }
-  const double *dist = b.dist + (size_t)req * MAX_INCL;
+  const double *dist = b.dist + rq.dist_off + (size_t)wk.zone * xt.n_incl;
   const float *spec = xt.spec + e;
   const size_t ne = (size_t)xt.n_ener;
   double acc = 0.0;
@@ -1526,7 +1670,7 @@ __global__ __launch_bounds__(XILL_THREADS) void k_xill(BatchDev b, const int *__
     for (int c = 0; c < 16; c++) s += w[c] * (double)t[c];
     acc += d * s;
   }
-  b.xspec[rq.xspec_off + e] = acc;
+  b.xspec[rq.xspec_off + (size_t)wk.zone * xt.n_ener + e] = acc / zr.ncf;
 }
 
 // Resident reflection set: per angle, rebin onto the convolution grid, band
@@ -1632,11 +1776,10 @@ int launch_conv(const BatchDev &b, const int *conv_reqs, int n_conv_reqs, void *
   return (int)hipGetLastError();
 }
 
-int launch_xill(const BatchDev &b, const int *xill_reqs, int n_xill_reqs, void *stream) {
-  if (n_xill_reqs <= 0) return 0;
+int launch_xill(const BatchDev &b, const XillWork *work, int n_work, void *stream) {
+  if (n_work <= 0) return 0;
   const int chunks = (b.xt.n_ener + XILL_THREADS - 1) / XILL_THREADS;
-  hipLaunchKernelGGL(k_xill, dim3(n_xill_reqs * chunks), dim3(XILL_THREADS), 0, (hipStream_t)stream, b, xill_reqs,
-                     chunks);
+  hipLaunchKernelGGL(k_xill, dim3(n_work * chunks), dim3(XILL_THREADS), 0, (hipStream_t)stream, b, work, chunks);
   return (int)hipGetLastError();
 }
 

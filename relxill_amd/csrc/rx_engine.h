@@ -30,6 +30,7 @@ constexpr int MAX_NG = 64;        // g* points (trff values per radius: 2*ng)
 constexpr int MAX_LP_R = 256;     // lamp-post table radii
 constexpr int MAX_INCL = 64;      // emission-angle bins
 constexpr int MAX_FFT = 8192;     // convolution grid size limit
+constexpr int MAX_ZONES = 10;     // radial zones of relxilllp (N_ZONES, src/common.h:134)
 
 enum ModelKind : int {
   MOD_RELLINE = 1,
@@ -37,6 +38,7 @@ enum ModelKind : int {
   MOD_RELCONV = 11,
   MOD_RELCONVLP = 12,
   MOD_RELXILL = -1,          // relxill: broken power law, one zone, xillver table
+  MOD_RELXILLLP = -2,        // relxilllp: lamp post, MAX_ZONES radial zones
   MOD_RELXILL_CONV = -1001,  // relxill-style conv of a resident reflection set
 };
 
@@ -66,7 +68,30 @@ struct ReqParams {
   int xind[4];
   double xfac[4];
   double xgam, xect, refl_frac;
-  int64_t xspec_off;
+  int64_t xspec_off;  // n_zones spectra of the table's n_ener
+  // radial zones (relxill models): edges (RadialGrid, src/IonGradient.cpp:457-489);
+  // dist holds n_zones x n_incl values from dist_off, prof n_zones x n_e
+  int n_zones;
+  int xboost;         // relxilllp switch_reflfrac_boost
+  double zgrid[MAX_ZONES + 1];
+  double g_so;        // energy shift primary source -> observer (1 without a lamp post)
+  double xect_src;    // Ecut in the source frame
+  int64_t dist_off;
+  int64_t zone_off;   // ZoneRec per zone
+};
+
+// Per zone of a relxill request (IonGradient, src/IonGradient.cpp:182-209):
+// the source -> disk energy shift of the zone centre, the Ecut seen there,
+// its xillver axis bracket and factor, and the normalisation change of the
+// incident spectrum (src/Xillspec.cpp:408-435).
+struct ZoneRec {
+  double rmean, del_emit, shift, ncf, ect_fac;
+  int ect_ind, pad;
+};
+
+// One k_xill unit: a relxill request and one of its zones.
+struct XillWork {
+  int req, zone;
 };
 
 // One fine radius (64 bytes).  The bin integral at this radius is
@@ -122,6 +147,11 @@ struct DevXillTable {
   int n_val[4];         // values of the four interpolated parameters
   const float *spec;
   const double *ener;   // n_ener + 1 edges (elo, then the last ehi)
+  int ect_dim;          // table axis of Ecut (-1: none)
+  int ect_q;            // the axis ensure_ecut_within_boundarys reads (src/xilltable.c:1058-1066)
+  const float *ect_vals;  // values of axis ect_dim
+  const float *q_vals;    // values of axis ect_q
+  int n_ect, n_q;
 };
 
 constexpr int N_ENER_COARSE = 500;  // primary-continuum normalisation grid (src/Xillspec.h:32)
@@ -216,6 +246,7 @@ struct BatchDev {
   const DevReflSet *refl_sets;
   DevXillTable xt;
   double *xspec;
+  ZoneRec *zone;
 };
 
 // ---- launchers (kernels.hip) ----
@@ -224,7 +255,7 @@ int launch_integrate(const BatchDev &b, int n_blocks, bool limb, void *stream);
 int launch_fixup(const BatchDev &b, bool cos_rows, void *stream);
 int launch_finalize(const BatchDev &b, void *stream);
 int launch_conv(const BatchDev &b, const int *conv_reqs, int n_conv_reqs, void *stream);
-int launch_xill(const BatchDev &b, const int *xill_reqs, int n_xill_reqs, void *stream);
+int launch_xill(const BatchDev &b, const XillWork *work, int n_work, void *stream);
 int launch_copy(const void *src, void *dst, size_t bytes, void *stream);
 int launch_refl_prepare(const DevConvGrid &cg, const double *refl_ener, const double *refl,
                         int n_refl, int n_incl, double *spec, double *bandsum, double *fft,

@@ -59,19 +59,20 @@ struct Slot {
 
 int64_t request_bytes(const Pending &p) {
   Global &g = G();
-  const bool conv = (p.model == MOD_RELCONV || p.model == MOD_RELCONVLP || p.model == MOD_RELXILL_CONV ||
-                     p.model == MOD_RELXILL);
+  const bool relxill = (p.model == MOD_RELXILL || p.model == MOD_RELXILLLP);
+  const bool conv = (p.model == MOD_RELCONV || p.model == MOD_RELCONVLP || p.model == MOD_RELXILL_CONV || relxill);
+  const int64_t nz = (p.model == MOD_RELXILLLP) ? MAX_ZONES : 1;
   const int64_t n_in = (int64_t)p.energy.size() - 1;
   const int64_t n_e = conv ? g.conv_n : n_in;
   const int64_t n_rc = (g.nr_fine + RAD_CHUNK - 1) / RAD_CHUNK;
   int64_t b = sizeof(ReqParams) + 8 * (2 * (n_in + 1) + 2 * n_in);
-  b += 8 * (n_e + 1) + 8 * n_e;
-  b += (int64_t)sizeof(WorkItem) * n_rc * ((n_e + SERVER_SEG_BINS - 1) / SERVER_SEG_BINS);
+  b += 8 * (n_e + 1) + 8 * n_e * nz;
+  b += (int64_t)sizeof(WorkItem) * (n_rc + nz - 1) * ((n_e + SERVER_SEG_BINS - 1) / SERVER_SEG_BINS);
   b += (int64_t)sizeof(FixEntry) * (n_e / 8 + 1);
   b += (int64_t)sizeof(RadRec) * g.nr_fine;
   b += 8 * 2 * (int64_t)g.rel.n_r * g.rel.n_g * 2;
-  b += 8 * MAX_INCL + sizeof(LpRefl) + sizeof(ReqNorm) + 64;
-  if (p.model == MOD_RELXILL) b += 8 * (int64_t)g.xill.n_ener + 4;  // xillver spectrum, request index
+  b += 8 * MAX_INCL * nz + sizeof(LpRefl) + sizeof(ReqNorm) + 64;
+  if (relxill) b += (8 * (int64_t)g.xill.n_ener + (int64_t)(sizeof(XillWork) + sizeof(ZoneRec))) * nz;
   return b;
 }
 

@@ -199,3 +199,104 @@ def test_relxill_through_the_server(relx):
             assert rel_err(y, y0) < 1e-12
     finally:
         s.stop()
+
+
+# ---- relxilllp: lamp post, 10 radial zones ----
+
+def draw_relxilllp_params(rng):
+    a = rng.uniform(-0.998, 0.998)
+    return [rng.uniform(3.0, 100.0), rng.choice([0.0, rng.uniform(0.0, 0.7)]), a, rng.uniform(5, 85),
+            rng.choice([-1.0, rng.uniform(1.0, 20.0)]), rng.uniform(50, 1000), rng.choice([0.0, rng.uniform(0, 2)]),
+            rng.uniform(1.0, 3.4), rng.uniform(0.0, 4.7), rng.uniform(0.5, 10.0), rng.uniform(5.0, 1000.0),
+            rng.choice([-1.0, rng.uniform(-5, 10)]), 0.0, float(rng.integers(0, 2))]
+
+
+def _gpu_lp(params, e, precision, n_dist=0):
+    b = rx.Batch([dict(model=rx.MODEL_RELXILLLP, energy=e, params=params)])
+    b.run(precision)
+    b.sync()
+    y, st = b.output(0)
+    d = b.dist(0, n_dist) if n_dist else None
+    b.close()
+    return y, st, d
+
+
+def test_relxilllp_default_parameters(relx):
+    """lmodel.dat defaults (h 6, beta 0, a 0.998, 30 deg, Rin ISCO, Rout 400,
+    Gamma 2, logxi 3.1, Afe 1, Ecut 300, refl_frac 1) without returning
+    radiation, on 3000 bins over 0.1-100 keV; the ten zones' angle
+    distributions too."""
+    T, xt = relx
+    p = list(rx.DEFAULTS[rx.MODEL_RELXILLLP])
+    p[12] = 0.0
+    e = log_grid(3000, 0.1, 100.0)
+    ref, dist_ref, st = oracle.relxill_model(oracle.MOD_RELXILLLP, T, xt, p, e)
+    assert st == 0
+    for prec, tol in ((rx.FP64, TOL64), (rx.FP32, TOL32)):
+        y, st, d = _gpu_lp(p, e, prec, n_dist=100)
+        print(f"relxilllp default prec {prec}: rel {rel_err(y, ref):.3e} dist {rel_err(d, dist_ref.reshape(-1)):.3e}")
+        assert st == 0
+        assert rel_err(y, ref) < tol
+        assert rel_err(d, dist_ref.reshape(-1)) < tol
+
+
+def test_relxilllp_random_sweep(relx):
+    """12 lamp-post parameter sets over the full ranges (heights 3-100, moving
+    sources, both signs of refl_frac, the boost switch, redshift), one batch,
+    FP64 and FP32."""
+    T, xt = relx
+    rng = np.random.default_rng(21)
+    e = log_grid(1500, 0.1, 100.0)
+    ps = [draw_relxilllp_params(rng) for _ in range(12)]
+    refs = [oracle.relxill_model(oracle.MOD_RELXILLLP, T, xt, p, e) for p in ps]
+    b = rx.Batch([dict(model=rx.MODEL_RELXILLLP, energy=e, params=p) for p in ps])
+    for prec, tol in ((rx.FP64, TOL64), (rx.FP32, TOL32)):
+        b.run(prec)
+        b.sync()
+        worst = 0.0
+        for i, (ref, _, st) in enumerate(refs):
+            y, st_g = b.output(i)
+            assert (st_g == 0) == (st == 0), (i, ps[i], st, st_g)
+            if st == 0:
+                worst = max(worst, rel_err(y, ref))
+        print(f"relxilllp sweep prec {prec}: worst rel {worst:.3e}")
+        assert worst < tol
+    b.close()
+
+
+def test_relxilllp_entry_point_and_returning_radiation(relx):
+    """lmodrelxilllp gives the FP64 result; switch_returnrad = 1 (the lmodel.dat
+    default) is refused with RX_ERR_UNSUPPORTED and zero flux."""
+    T, xt = relx
+    p = [10.0, 0.1, 0.7, 50.0, -1.0, 300.0, 0.02, 2.3, 2.0, 2.0, 150.0, 2.0, 0.0, 1.0]
+    e = log_grid(700, 0.2, 80.0)
+    ref, _, st = oracle.relxill_model(oracle.MOD_RELXILLLP, T, xt, p, e)
+    assert st == 0
+    y = rx.lmodrelxilllp(e, p)
+    assert rel_err(y, ref) < TOL64
+    y = rx.lmodrelxilllp(e, rx.DEFAULTS[rx.MODEL_RELXILLLP])
+    assert not y.any() and rx.lib.rx_last_status() == rx.RX_ERR_UNSUPPORTED
+
+
+def test_relxilllp_batched_with_relxill(relx):
+    """relxilllp requests batched with relxill and relconv requests give what
+    they give alone (to the order of the FP64 atomic sums)."""
+    T, xt = relx
+    e = log_grid(900, 0.1, 100.0)
+    rng = np.random.default_rng(8)
+    pl = [draw_relxilllp_params(rng) for _ in range(3)]
+    alone = [_gpu_lp(p, e, rx.FP64)[0] for p in pl]
+    flux_in = np.diff(e) * (0.5 * (e[1:] + e[:-1])) ** -2.0
+    reqs = []
+    for p in pl:
+        reqs.append(dict(model=rx.MODEL_RELXILL, energy=e, params=rx.DEFAULTS[rx.MODEL_RELXILL]))
+        reqs.append(dict(model=rx.MODEL_RELXILLLP, energy=e, params=p))
+        reqs.append(dict(model=rx.MODEL_RELCONV, energy=e, params=rx.DEFAULTS[rx.MODEL_RELCONV], flux_in=flux_in))
+    b = rx.Batch(reqs)
+    b.run(rx.FP64)
+    b.sync()
+    for k, y0 in enumerate(alone):
+        y, st = b.output(3 * k + 1)
+        assert st == 0
+        assert rel_err(y, y0) < 1e-12
+    b.close()

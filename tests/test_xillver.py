@@ -224,3 +224,138 @@ def test_synthetic_xilltable_shape():
         assert 0.2 < np.median(r2[300:] / r0[300:]) < 5
     finally:
         rx.xilltable_synthetic()
+
+
+# ---- relxilllp: lamp post, 10 radial zones ----
+
+RELXILLLP_P = [6.0, 0.0, 0.9, 30.0, -1.0, 400.0, 0.0, 2.0, 3.1, 1.0, 300.0, 1.0, 0.0, 0.0]
+
+
+@pytest.fixture(scope="module")
+def relxlp():
+    from tests.helpers import tables as _tables
+    _, _, T = _tables()
+    rx.xilltable_synthetic(5, 5, 4, 6, 5, 10, 600)
+    return T, oracle.Xill(rx.xilltable_view())
+
+
+def _ut_disk(r, a):
+    return (r * math.sqrt(r) + a) / (math.sqrt(r) * math.sqrt(r * r - 3 * r + 2 * a * math.sqrt(r)))
+
+
+def _norm_integral(gam, ect):
+    return quad(lambda e: math.exp(1 / ect) * e ** (1 - gam) * math.exp(-e / ect), 0.1, 1000.0,
+                limit=400, epsabs=0, epsrel=1e-12)[0]
+
+
+@pytest.mark.parametrize("h,a,beta,incl", [(6.0, 0.9, 0.0, 30.0), (3.0, 0.998, 0.0, 60.0), (20.0, -0.5, 0.3, 45.0)])
+def test_relxilllp_zones_closed_forms(relxlp, h, a, beta, incl):
+    """The zones (RadialGrid, IonGradient): edges from Rin to Rout, log-spaced
+    below the source height and uniform in 1/r above; the source -> disk energy
+    shift of a static source is u^t(disk, r) / u^t(axis, h) at the zone centre;
+    the observer shift sqrt(1 - 2h/(h^2+a^2)) times the Doppler factor
+    sqrt(1-beta^2)/(1 - beta cos i); the zone's Ecut is the observed Ecut
+    over the observer shift times the disk shift; the normalisation change is
+    the ratio of the incident energy-flux integrals (source over zone), 2e-4."""
+    T, _ = relxlp
+    p = list(RELXILLLP_P)
+    p[0], p[1], p[2], p[3] = h, beta, a, incl
+    z, st = oracle.relxill_zones(oracle.MOD_RELXILLLP, T, p)
+    assert st == 0
+    rin = float(oracle.lib.orc_kerr_rms(a))
+    edges = np.append(z[:, 0], z[-1, 1])
+    assert edges[0] == rin and edges[-1] == pytest.approx(400.0, rel=1e-14) and np.all(np.diff(edges) > 0)
+    # log grid up to its last edge at or below h, then uniform in 1/r
+    logg = rin * (400.0 / rin) ** (np.arange(11) / 10)
+    indr = int(np.searchsorted(logg, h, side="right") - 1) if h > rin else 0
+    assert np.allclose(edges[:indr + 1], logg[:indr + 1], rtol=1e-12)
+    inv = 1.0 / edges[indr:]
+    assert np.allclose(np.diff(inv), np.diff(inv)[0], rtol=1e-9)
+    g_so = math.sqrt(1 - 2 * h / (h * h + a * a))
+    if beta >= 1e-4:
+        g_so *= math.sqrt(1 - beta * beta) / (1 - beta * math.cos(math.radians(incl)))
+    assert oracle.energy_shift_source_obs(oracle.MOD_RELXILLLP, p)[0] == pytest.approx(g_so, rel=1e-13)
+    ut_h = math.sqrt((h * h + a * a) / (h * h - 2 * h + a * a))
+    ect_src = p[10] / g_so
+    for iz in range(10):
+        rm = 0.5 * (z[iz, 0] + z[iz, 1])
+        if beta == 0.0:
+            assert z[iz, 3] == pytest.approx(_ut_disk(rm, a) / ut_h, rel=1e-12)
+        assert z[iz, 4] == pytest.approx(ect_src * z[iz, 3], rel=1e-13)
+        want = _norm_integral(2.0, ect_src) / _norm_integral(2.0, z[iz, 4])
+        assert z[iz, 5] / want - 1 == pytest.approx(0.0, abs=2e-4)
+
+
+def test_relxilllp_zone_profiles_partition_the_profile(relxlp):
+    """The zones' profiles add up to the one-zone profile (the lamp post is not
+    renormalised), and each zone's profile lies inside the energy windows of
+    the fine radii between its edges."""
+    T, _ = relxlp
+    p, st = oracle.rel_params(oracle.MOD_RELXILLLP, RELXILLLP_P)
+    assert st == 0
+    cg = log_grid(4096, 0.00035, 2000.0)
+    f10, d10, st = oracle.relbase(T, p, cg, n_incl=10)
+    assert st == 0
+    p.num_zones = 1
+    f1, _, st = oracle.relbase(T, p, cg, n_incl=10)
+    f10 = f10.reshape(10, -1)
+    assert rel_err(f10.sum(axis=0), f1) < 1e-13
+    p.num_zones = 10
+    sp, st = oracle.system_parameters(T, p)
+    z, _ = oracle.relxill_zones(oracle.MOD_RELXILLLP, T, RELXILLLP_P)
+    emid = 0.5 * (cg[1:] + cg[:-1])
+    for iz in range(10):
+        sel = (sp["re"] >= z[iz, 0]) & (sp["re"] < z[iz, 1]) if iz < 9 else (sp["re"] >= z[iz, 0])
+        lo, hi = sp["gmin"][sel].min(), sp["gmax"][sel].max()
+        outside = (cg[1:] < lo * (1 - 1e-12)) | (cg[:-1] > hi * (1 + 1e-12))
+        assert f10[iz][outside].max() == 0.0 and f10[iz].max() > 0, iz
+
+
+@pytest.mark.parametrize("h,a,beta,gam,ect", [(6.0, 0.9, 0.0, 2.0, 300.0), (3.0, 0.998, 0.2, 1.6, 60.0)])
+def test_relxilllp_primary_only_closed_form(relxlp, h, a, beta, gam, ect):
+    """refl_frac = 0: the observed cut-off power law (the source's Ecut = Ecut /
+    g shifted back by g), normalised at the source, times f_inf_rest / 0.5
+    g^Gamma (and the Doppler factor squared for a moving source): the integral
+    over each caller bin, 1e-3."""
+    T, xt = relxlp
+    p = list(RELXILLLP_P)
+    p[0], p[1], p[2], p[7], p[10], p[11] = h, beta, a, gam, ect, 0.0
+    e = log_grid(200, 0.1, 100.0)
+    y, _, st = oracle.relxill_model(oracle.MOD_RELXILLLP, T, xt, p, e)
+    assert st == 0
+    rp, _ = oracle.rel_params(oracle.MOD_RELXILLLP, p)
+    sp, _ = oracle.system_parameters(T, rp)
+    f_inf_rest = sp["refl"][4]
+    g = math.sqrt(1 - 2 * h / (h * h + a * a))
+    dop = 1.0
+    if beta >= 1e-4:
+        dop = math.sqrt(1 - beta * beta) / (1 - beta * math.cos(math.radians(p[3])))
+        g *= dop
+    nf = oracle.primary_norm_factor(gam, ect / g)
+    fac = f_inf_rest / 0.5 * g ** gam * dop ** 2 * nf
+    want = np.array([fac * quad(lambda x: math.exp(1 / ect) * x ** -gam * math.exp(-x / ect), a0, b0,
+                                epsrel=1e-10)[0] for a0, b0 in zip(e[:-1], e[1:])])
+    assert rel_err(y, want) < 1e-3
+
+
+def test_relxilllp_refl_frac_and_boost(relxlp):
+    """The reflection scales with |refl_frac| over the lamp post's reflection
+    fraction; with switch_reflfrac_boost = 1, with |refl_frac| alone; the
+    primary only for refl_frac >= 0."""
+    T, xt = relxlp
+    e = log_grid(300, 0.1, 100.0)
+
+    def run(rf, boost=0.0):
+        p = list(RELXILLLP_P)
+        p[11], p[13] = rf, boost
+        return oracle.relxill_model(oracle.MOD_RELXILLLP, T, xt, p, e)[0]
+    r1, r2, p0, p3 = run(-1.0), run(-2.0), run(0.0), run(3.0)
+    assert r1.max() > 0 and p0.max() > 0
+    assert rel_err(r2, 2 * r1) < 1e-14
+    assert rel_err(p3, 3 * r1 + p0) < 1e-13
+    rp, _ = oracle.rel_params(oracle.MOD_RELXILLLP, RELXILLLP_P)
+    sp, _ = oracle.system_parameters(T, rp)
+    assert rel_err(run(-1.0, 1.0), sp["refl"][0] * r1) < 1e-13
+    p = list(RELXILLLP_P)
+    p[12] = 1.0  # returning radiation is not built
+    assert oracle.relxill_model(oracle.MOD_RELXILLLP, T, xt, p, e)[2] == 2
