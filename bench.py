@@ -11,8 +11,8 @@ rebin) over the batch, FP32 with Kahan sums, inputs resident in HBM.
 
 Also reported: the plain FP64 path (one request per launch), the batched
 lamp-post relline (1024 requests, 3000 bins), the relxill-style convolution of
-a resident reflection set, relxill with the xillver table interpolated on the
-device (1024 requests), the serving loop's p50/p99
+a resident reflection set, relxill and relxilllp (10 zones) with the xillver
+table interpolated on the device (1024 requests each), the serving loop's p50/p99
 latency under Poisson arrivals (relline/relconv/lamp-post mix 40/40/20), the
 maximum relative error of the FP32 path against the FP64 oracle, the dominant
 kernel's roofline figures and the FP64 CPU oracle timed on the host's cores.
@@ -31,7 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
-KERNELS = ("k_setup", "k_integrate", "k_fixup", "k_finalize", "k_conv", "k_xill")
+KERNELS = ("k_setup", "k_integrate", "k_fixup", "k_finalize", "k_conv", "k_xill", "k_primary")
 
 
 def log(*a):
@@ -123,6 +123,40 @@ def relxill_style(rx, steps):
     return dict(plain_fp64_ms_per_spectrum=float(np.mean(lat)) * 1e3,
                 batched_fp32_spectra_per_s=len(reqs) / dt32, batched_fp64_spectra_per_s=len(reqs) / dt64,
                 max_rel_err_fp32_vs_fp64=max(rel_err(o32[i], o64[i]) for i in range(len(reqs))))
+
+
+def relxilllp_batch(rx, steps, n_req=1024, seed=10):
+    """relxilllp (lamp post, 10 zones) on the same table and grids: h U[3,100],
+    a, incl, Rin over their ranges, the xillver parameters drawn as for relxill,
+    refl_frac 1, no returning radiation; FP32 and FP64 batches, plain FP64, the
+    FP32-vs-FP64 error and the per-kernel times."""
+    rng = np.random.default_rng(seed)
+    e_out = log_grid(3000, 0.1, 100.0)
+    reqs = []
+    for _ in range(n_req):
+        a = rng.uniform(-0.998, 0.998)
+        reqs.append(dict(model=rx.MODEL_RELXILLLP, energy=e_out,
+                         params=[rng.uniform(3.0, 100.0), 0.0, a, rng.uniform(5.0, 85.0),
+                                 rng.uniform(kerr_rms(a), 50.0), 400.0, 0.0, rng.uniform(1.0, 3.4),
+                                 rng.uniform(0.0, 4.7), rng.uniform(0.5, 10.0),
+                                 float(np.exp(rng.uniform(np.log(5.0), np.log(1000.0)))), 1.0, 0.0, 0.0]))
+    b = rx.Batch(reqs)
+    b.autotune(rx.FP32)
+    dt32, kms32 = timed_steps(rx, b, rx.FP32, max(3, steps // 2), 1)
+    o32 = [b.output(i)[0] for i in range(n_req)]
+    dt64, _ = timed_steps(rx, b, rx.FP64, 2, 1)
+    o64 = [b.output(i)[0] for i in range(n_req)]
+    b.close()
+    lat = []
+    for r in reqs[:8]:
+        pb = rx.Batch([r])
+        d, _ = timed_steps(rx, pb, rx.FP64, 3, 1)
+        lat.append(d)
+        pb.close()
+    return dict(n_requests=n_req, zones=10, batched_fp32_spectra_per_s=n_req / dt32,
+                batched_fp64_spectra_per_s=n_req / dt64, plain_fp64_ms_per_spectrum=float(np.mean(lat)) * 1e3,
+                max_rel_err_fp32_vs_fp64=max(rel_err(o32[i], o64[i]) for i in range(n_req)),
+                kernel_ms={k: round(v, 4) for k, v in zip(KERNELS, kms32)})
 
 
 def relxill_batch(rx, steps, n_req=1024, seed=9, cpu=True):
@@ -331,6 +365,7 @@ def main():
 
     if args.relxill_only:
         r = relxill_batch(rx, args.steps, cpu=False)
+        r["relxilllp"] = relxilllp_batch(rx, args.steps)
         print(json.dumps(r))
         return
 
@@ -393,6 +428,8 @@ def main():
     # relxill with the xillver table on the device
     relxill = relxill_batch(rx, args.steps, cpu=not args.no_cpu)
     log(f"relxill batch: {relxill}")
+    relxilllp = relxilllp_batch(rx, args.steps)
+    log(f"relxilllp batch: {relxilllp}")
 
     serving = serving_high = None
     if not args.no_serve:
@@ -472,6 +509,7 @@ def main():
         "lp_relline_batch": cfg2,
         "relxill_style_conv": cfg1,
         "relxill_batch": relxill,
+        "relxilllp_batch": relxilllp,
         "serving": serving,
         "serving_20k_qps": serving_high,
         "p99_latency_ms": serving["p99_ms"] if serving else None,

@@ -240,8 +240,9 @@ int rx_reflsets_clear(void);
 
 /* Kernel timing of the last rx_batch_run with profiling enabled: per kernel
  * (0 k_setup (with k_reset), 1 k_integrate, 2 k_fixup, 3 k_finalize, 4 k_conv,
- * 5 k_xill) the duration in ms measured with HIP events on the batch's stream. */
-#define RX_N_TIMED 6
+ * 5 k_xill, 6 k_primary) the duration in ms measured with HIP events on the
+ * batch's stream. */
+#define RX_N_TIMED 7
 int rx_profile_enable(int on);
 /* Streams `bytes` through a plain device copy kernel `reps` times (after one
  * warm-up) and returns the average kernel time; used to calibrate HBM byte

@@ -351,8 +351,8 @@ class Batch:
         return r // 256, r % 256
 
     def kernel_ms(self):
-        """per kernel ms of the last profiled run: setup, integrate, fixup, finalize, conv, xill"""
-        o = np.zeros(6)
+        """per kernel ms of the last profiled run: setup, integrate, fixup, finalize, conv, xill, primary"""
+        o = np.zeros(7)
         _check(lib.rx_batch_kernel_ms(self.h, _dp(o)))
         return o
 

@@ -585,6 +585,7 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
   b.h_status.assign(n_req, 0);
   b.h_conv.clear();
   b.h_xill.clear();
+  b.h_prim.clear();
   for (int i = 0; i < n_req; i++) {
     if (is_relxill(req[i].model)) {
       int st = ensure_xilltable_device(err);
@@ -618,7 +619,7 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
       p.pgrid_off = -1;
       p.prof_off = o_prof;
       o_prof += (int64_t)g.conv_n * p.n_zones;
-      b.h_conv.push_back(i);
+      for (int z = 0; z < p.n_zones; z++) b.h_conv.push_back(XillWork{i, z});
       if (is_relxill(r.model)) {
         // the zones' xillver parameters (src/IonGradient.cpp:196-209, :340-389):
         // logXi kept within 0-4.7, density 15; the Ecut axis of the lamp
@@ -637,6 +638,7 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
         p.zone_off = o_zone;
         o_zone += p.n_zones;
         for (int z = 0; z < p.n_zones; z++) b.h_xill.push_back(XillWork{i, z});
+        b.h_prim.push_back(i);
       } else if (st == RX_OK && r.model != MOD_RELXILL_CONV) {
         if (!r.flux_in) {
           err = "convolution request without input spectrum";
@@ -748,8 +750,9 @@ int batch_upload(Batch &b) {
   e |= b.d_zone.ensure(sizeof(ZoneRec) * std::max<int64_t>(1, b.tot_zone));
   e |= b.d_refl.ensure(sizeof(LpRefl) * nreq);
   e |= b.d_status.ensure(sizeof(int) * nreq);
-  e |= b.d_conv.ensure(sizeof(int) * std::max<size_t>(1, b.h_conv.size()));
+  e |= b.d_conv.ensure(sizeof(XillWork) * std::max<size_t>(1, b.h_conv.size()));
   e |= b.d_xill.ensure(sizeof(XillWork) * std::max<size_t>(1, b.h_xill.size()));
+  e |= b.d_prim.ensure(sizeof(int) * std::max<size_t>(1, b.h_prim.size()));
   e |= b.d_xspec.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_xspec));
   e |= b.d_items.ensure(sizeof(WorkItem) * (size_t)b.item_cap);
   e |= b.d_ctr.ensure(sizeof(int) * N_COUNTERS);
@@ -765,7 +768,10 @@ int batch_upload(Batch &b) {
   e |= (int)hipMemcpyAsync(b.d_flux_in.p, b.h_flux_in.data(), sizeof(double) * b.tot_flux, hipMemcpyHostToDevice, b.stream);
   e |= (int)hipMemcpyAsync(b.d_status.p, b.h_status.data(), sizeof(int) * nreq, hipMemcpyHostToDevice, b.stream);
   if (!b.h_conv.empty())
-    e |= (int)hipMemcpyAsync(b.d_conv.p, b.h_conv.data(), sizeof(int) * b.h_conv.size(), hipMemcpyHostToDevice, b.stream);
+    e |= (int)hipMemcpyAsync(b.d_conv.p, b.h_conv.data(), sizeof(XillWork) * b.h_conv.size(), hipMemcpyHostToDevice,
+                             b.stream);
+  if (!b.h_prim.empty())
+    e |= (int)hipMemcpyAsync(b.d_prim.p, b.h_prim.data(), sizeof(int) * b.h_prim.size(), hipMemcpyHostToDevice, b.stream);
   if (!b.h_xill.empty())
     e |= (int)hipMemcpyAsync(b.d_xill.p, b.h_xill.data(), sizeof(XillWork) * b.h_xill.size(), hipMemcpyHostToDevice,
                              b.stream);
@@ -831,8 +837,10 @@ int batch_run(Batch &b, int precision, std::string &err) {
   if (prof) e |= (int)hipEventRecord(b.ev[4], s);
   e |= launch_xill(bd, b.d_xill.as<XillWork>(), (int)b.h_xill.size(), s);
   if (prof) e |= (int)hipEventRecord(b.ev[5], s);
-  e |= launch_conv(bd, b.d_conv.as<int>(), (int)b.h_conv.size(), s);
+  e |= launch_conv(bd, b.d_conv.as<XillWork>(), (int)b.h_conv.size(), s);
   if (prof) e |= (int)hipEventRecord(b.ev[6], s);
+  e |= launch_primary(bd, b.d_prim.as<int>(), (int)b.h_prim.size(), s);
+  if (prof) e |= (int)hipEventRecord(b.ev[7], s);
   b.profiled = prof;
   if (e) {
     err = "kernel launch failed";
@@ -857,9 +865,9 @@ int batch_sync(Batch &b) {
     if (hipStreamSynchronize(b.stream) != hipSuccess) return RX_ERR_DEVICE;
   }
   if (b.profiled) {
-    // events: setup | integrate | fixup | finalize | xill | conv; the timing
-    // slots keep conv fourth and add xill as the sixth
-    static const int slot[RX_N_TIMED] = {0, 1, 2, 3, 5, 4};
+    // events: setup | integrate | fixup | finalize | xill | conv | primary; the
+    // timing slots keep conv fourth and add xill and primary after it
+    static const int slot[RX_N_TIMED] = {0, 1, 2, 3, 5, 4, 6};
     for (int k = 0; k < RX_N_TIMED; k++) {
       float ms = 0;
       (void)hipEventElapsedTime(&ms, b.ev[k], b.ev[k + 1]);

@@ -32,6 +32,7 @@ namespace {
 constexpr int SETUP_THREADS = 256;
 constexpr int CONV_THREADS = 256;      // smallest convolution workgroup
 constexpr int XILL_THREADS = 256;
+constexpr int PRIM_THREADS = 256;
 constexpr int CONV_THREADS_MAX = 1024;  // n/8 threads: 512 for 4096 bins, 1024 for 8192
 
 __device__ inline double lp_ifac(float value, const float *arr, int n, int &ind) {
@@ -273,6 +274,11 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
   const int nr = b.nr_fine;
   const int n_r = T.n_r, ng = T.n_g;
   if (rq.pre_status != 0) return;  // status already set by the host
+  if (rq.model == MOD_RELXILL || rq.model == MOD_RELXILLLP) {
+    // the zones' convolutions add into the output (k_conv)
+    double *out = b.out + rq.flux_off;
+    for (int j = tid; j < rq.n_in; j += SETUP_THREADS) out[j] = 0.0;
+  }
 
   __shared__ double s_tre[MAX_NR_TAB], s_tgmin[MAX_NR_TAB], s_tgmax[MAX_NR_TAB];
   __shared__ double s_jr[MAX_LP_R], s_je[MAX_LP_R], s_jd[MAX_LP_R], s_ji[MAX_LP_R];
@@ -1301,11 +1307,12 @@ __device__ __forceinline__ double rebin_bin(double elo_, double ehi_, int clo, i
 // and inverted with a half-length complex FFT.  Grid-walking phases give each
 // wavefront a contiguous run of bins, one bin per lane, so that neighbouring
 // lanes read neighbouring addresses.
-__global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int *__restrict__ reqs) {
+__global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const XillWork *__restrict__ work) {
   extern __shared__ double2 s_buf[];  // P(n) complex
   __shared__ double s_w[2][CONV_THREADS_MAX / 64];
   const int NT = blockDim.x;
-  const int req = reqs[blockIdx.x];
+  const int req = work[blockIdx.x].req;
+  const int zone = work[blockIdx.x].zone;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6, nwave = NT >> 6;
   const ReqParams rq = b.req[req];
@@ -1332,14 +1339,12 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
   const DevReflSet *rs = angular ? &b.refl_sets[rq.refl_set] : nullptr;
   auto ge_in = [&](int k) { return ein[k]; };
   const BinRange rc = b.reach[req];
-  const int n_zones = relxill ? rq.n_zones : 1;
-  // relxill: reflection scale and primary factors (add_primary_spectrum,
+  // relxill: the reflection's scale (add_primary_spectrum,
   // src/PrimarySource.cpp:66-125): |refl_frac| for the broken power law; for
   // the lamp post |refl_frac| (times the lamp post's own reflection fraction
-  // when refl_frac is a boost) over that fraction, and the primary's flux
-  // boost f_inf_rest / 0.5 g^Gamma (times the Doppler factor squared for a
-  // moving source, src/PrimarySource.h:84-93)
-  double refl_scale = 1.0, prim_scale = 1.0;
+  // when refl_frac is a boost) over that fraction.  The primary is added by
+  // k_primary.
+  double refl_scale = 1.0;
   if (relxill) {
     if (rq.emis_type != EMIS_LP) {
       refl_scale = fabs(rq.refl_frac);
@@ -1348,27 +1353,14 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
       double rfi = rq.refl_frac;
       if (rq.xboost) rfi *= lr.refl_frac;
       refl_scale = (fabs(rfi)) / lr.refl_frac;
-      prim_scale = lr.f_inf_rest / 0.5 * pow(rq.g_so, rq.xgam);
-      if (rq.beta > 1e-4) {
-        const double dop = sqrt(1.0 - rq.beta * rq.beta) / (1.0 + rq.beta * cos(M_PI - rq.incl));
-        prim_scale *= pow(dop, 2);
-      }
     }
   }
-  const int nbo = (n_o + nwave - 1) / nwave;  // caller bins per wavefront (phase 5)
-  const int jend = min(n_o, (wave + 1) * nbo);
-  bool wrote = false;  // relxill: a zone's result is in out already
   auto o_at = [&](int i) -> double & {
     double2 &z = s_buf[P(i >> 1)];
     return (i & 1) ? z.y : z.x;
   };
-  // the conv grid edges go to the upper part of the buffer, free after the
-  // half-length FFT (its values occupy slots below P(m))
-  double *s_e = reinterpret_cast<double *>(s_buf + P(m));
-  const GridGuess gc = grid_guess(Ec, n);
-  auto ge_c = [&](int k) { return s_e[k]; };
 
-  for (int zone = 0; zone < n_zones; zone++) {
+  // this block's zone (relxill requests have one block per zone)
   const double *prof = b.prof + rq.prof_off + (size_t)zone * n;
   const double *fin = relxill ? b.xspec + rq.xspec_off + (size_t)zone * n_in : b.flux_in + rq.flux_off;
   const double *dist = b.dist + rq.dist_off + (size_t)zone * rq.n_incl;
@@ -1497,7 +1489,7 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
   }
   // a zone whose profile sums below 1e-12 is left out (src/Relxill.cpp:453-456)
   const bool zone_used = !(sum_prof < 1e-12);
-  if (relxill && !zone_used) continue;  // (block-uniform; the buffer is not read after the last barrier)
+  if (relxill && !zone_used) return;  // (block-uniform)
   // 2. forward complex FFT of x + i y
   lds_fft<-1>(s_buf, n, reinterpret_cast<const double2 *>(cg.twp_fwd));
   // 3. X = (W_k + conj W_{n-k})/2, Y = (W_k - conj W_{n-k})/(2i), P = X Y (k = 0..m);
@@ -1560,11 +1552,18 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
   double fnorm = sum_rel * sum_xill / sum_conv;
   if (angular && !zone_used) fnorm = 0.0;
   if (relxill) fnorm *= refl_scale;
+  // the conv grid edges go to the upper part of the buffer, free after the
+  // half-length FFT (its values occupy slots below P(m))
+  double *s_e = reinterpret_cast<double *>(s_buf + P(m));
 #pragma unroll 8
   for (int i = tid; i <= n; i += NT) s_e[i] = Ec[i];
   __syncthreads();
   // 5. rebin onto the caller grid; relconv zeroes outside 0.01-1000 keV;
-  // relxill adds the zones up (each output bin stays with its thread)
+  // relxill adds its zones up
+  const GridGuess gc = grid_guess(Ec, n);
+  auto ge_c = [&](int k) { return s_e[k]; };
+  const int nbo = (n_o + nwave - 1) / nwave;
+  const int jend = min(n_o, (wave + 1) * nbo);
   const bool relconv = !angular && !relxill;
 #pragma unroll 2
   for (int q = 0; q < nbo; q += 64) {
@@ -1575,37 +1574,58 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const int
       const int chi = count_le(ge_c, n, ehi, gc);
       double v = fnorm * rebin_bin(elo, ehi, clo, chi, ge_c, n, [&](int i) { return o_at(i); });
       if (relconv && (ehi < 0.01 || elo > 1000.0)) v = 0.0;
-      out[jb] = wrote ? out[jb] + v : v;
+      if (relxill) unsafeAtomicAdd(&out[jb], v);  // the zones add up (k_setup cleared out)
+      else out[jb] = v;
     }
   }
-  wrote = true;
-  __syncthreads();  // the next zone overwrites the buffer
-  }  // zones
+}
 
-  if (relxill) {
-    // the primary: the cut-off power law with the source's Ecut shifted to the
-    // observer (spec_cutoffpl, src/Xillspec.cpp:215-241), normalised as the
-    // xillver incident spectrum at the source (calcNormWrtXillverTableSpec on
-    // the 500-bin grid, src/PrimarySource.h:238-292), added for refl_frac >= 0
-    const double ect = rq.xect_src, gam = rq.xgam;
-    const double nf = 1. / (block_sum4(xill_norm_part(cg.coarse, ect, gam, tid, NT), s_w[0]) / NORM_XILLVER_TABLE);
-    const bool add_prim = rq.refl_frac >= 0;
-#pragma unroll 8
-    for (int i = tid; i < n; i += NT)
-      o_at(i) = add_prim ? (cutoffpl_bin(Ec[i], Ec[i + 1], ect * rq.g_so, gam) * nf) * prim_scale : 0.0;
-#pragma unroll 8
-    for (int i = tid; i <= n; i += NT) s_e[i] = Ec[i];
-    __syncthreads();
-#pragma unroll 2
-    for (int q = 0; q < nbo; q += 64) {
-      const int jb = wave * nbo + q + lane;
-      if (jb < jend) {
-        const double elo = eo[jb], ehi = eo[jb + 1];
-        const int clo = count_le(ge_c, n, elo, gc);
-        const int chi = count_le(ge_c, n, ehi, gc);
-        const double v = add_prim ? rebin_bin(elo, ehi, clo, chi, ge_c, n, [&](int i) { return o_at(i); }) : 0.0;
-        out[jb] = wrote ? out[jb] + v : v;
-      }
+// relxill: the primary added to the reflection (add_primary_spectrum,
+// src/PrimarySource.cpp:66-125) for refl_frac >= 0: the cut-off power law with
+// the source's Ecut shifted to the observer (spec_cutoffpl,
+// src/Xillspec.cpp:215-241) on the convolution grid, normalised as the
+// xillver incident spectrum at the source (calcNormWrtXillverTableSpec on the
+// 500-bin grid, src/PrimarySource.h:238-292), times the lamp post's flux boost,
+// rebinned onto the caller grid.  One workgroup per request, after k_conv.
+__global__ __launch_bounds__(PRIM_THREADS) void k_primary(BatchDev b, const int *__restrict__ reqs) {
+  extern __shared__ double s_p[];  // n values, then n + 1 edges
+  __shared__ double s_w[PRIM_THREADS / 64];
+  const int req = reqs[blockIdx.x];
+  const int tid = threadIdx.x, NT = blockDim.x;
+  const int lane = tid & 63, wave = tid >> 6, nwave = NT >> 6;
+  const ReqParams &rq = b.req[req];
+  if (b.status[req] != 0 || !(rq.refl_frac >= 0)) return;
+  const DevConvGrid &cg = b.cg;
+  const int n = cg.n;
+  const double ect = rq.xect_src, gam = rq.xgam;
+  double prim_scale = 1.0;
+  if (rq.emis_type == EMIS_LP) {  // flux_boost_source_to_observer, src/PrimarySource.h:84-93
+    prim_scale = b.refl[req].f_inf_rest / 0.5 * pow(rq.g_so, gam);
+    if (rq.beta > 1e-4) {
+      const double dop = sqrt(1.0 - rq.beta * rq.beta) / (1.0 + rq.beta * cos(M_PI - rq.incl));
+      prim_scale *= pow(dop, 2);
+    }
+  }
+  const double nf = 1. / (block_sum4(xill_norm_part(cg.coarse, ect, gam, tid, NT), s_w) / NORM_XILLVER_TABLE);
+  double *s_e = s_p + n;
+  for (int i = tid; i <= n; i += NT) s_e[i] = cg.ener[i];
+  for (int i = tid; i < n; i += NT)
+    s_p[i] = (cutoffpl_bin(cg.ener[i], cg.ener[i + 1], ect * rq.g_so, gam) * nf) * prim_scale;
+  __syncthreads();
+  const GridGuess gc = grid_guess(cg.ener, n);
+  auto ge_c = [&](int k) { return s_e[k]; };
+  const double *eo = b.ener_in + rq.ener_off;
+  double *out = b.out + rq.flux_off;
+  const int n_o = rq.n_in;
+  const int nbo = (n_o + nwave - 1) / nwave;
+  const int jend = min(n_o, (wave + 1) * nbo);
+  for (int q = 0; q < nbo; q += 64) {
+    const int jb = wave * nbo + q + lane;
+    if (jb < jend) {
+      const double elo = eo[jb], ehi = eo[jb + 1];
+      const int clo = count_le(ge_c, n, elo, gc);
+      const int chi = count_le(ge_c, n, ehi, gc);
+      out[jb] += rebin_bin(elo, ehi, clo, chi, ge_c, n, [&](int i) { return s_p[i]; });
     }
   }
 }
@@ -1768,11 +1788,11 @@ int launch_finalize(const BatchDev &b, void *stream) {
 static size_t conv_lds(int n) { return (size_t)(n + n / 16) * sizeof(double2); }
 static int conv_threads(int n) { return std::max(CONV_THREADS, n / 8); }
 
-int launch_conv(const BatchDev &b, const int *conv_reqs, int n_conv_reqs, void *stream) {
-  if (n_conv_reqs <= 0) return 0;
+int launch_conv(const BatchDev &b, const XillWork *conv_work, int n_conv_work, void *stream) {
+  if (n_conv_work <= 0) return 0;
   const size_t lds = conv_lds(b.cg.n);
   (void)hipFuncSetAttribute((const void *)k_conv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(k_conv, dim3(n_conv_reqs), dim3(conv_threads(b.cg.n)), lds, (hipStream_t)stream, b, conv_reqs);
+  hipLaunchKernelGGL(k_conv, dim3(n_conv_work), dim3(conv_threads(b.cg.n)), lds, (hipStream_t)stream, b, conv_work);
   return (int)hipGetLastError();
 }
 
@@ -1780,6 +1800,14 @@ int launch_xill(const BatchDev &b, const XillWork *work, int n_work, void *strea
   if (n_work <= 0) return 0;
   const int chunks = (b.xt.n_ener + XILL_THREADS - 1) / XILL_THREADS;
   hipLaunchKernelGGL(k_xill, dim3(n_work * chunks), dim3(XILL_THREADS), 0, (hipStream_t)stream, b, work, chunks);
+  return (int)hipGetLastError();
+}
+
+int launch_primary(const BatchDev &b, const int *reqs, int n_reqs, void *stream) {
+  if (n_reqs <= 0) return 0;
+  const size_t lds = (size_t)(2 * b.cg.n + 1) * sizeof(double);
+  (void)hipFuncSetAttribute((const void *)k_primary, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(k_primary, dim3(n_reqs), dim3(PRIM_THREADS), lds, (hipStream_t)stream, b, reqs);
   return (int)hipGetLastError();
 }
 

@@ -254,8 +254,11 @@ int launch_setup(const BatchDev &b, void *stream);
 int launch_integrate(const BatchDev &b, int n_blocks, bool limb, void *stream);
 int launch_fixup(const BatchDev &b, bool cos_rows, void *stream);
 int launch_finalize(const BatchDev &b, void *stream);
-int launch_conv(const BatchDev &b, const int *conv_reqs, int n_conv_reqs, void *stream);
+// conv_work: (request, zone) pairs; relxill requests have one per zone
+int launch_conv(const BatchDev &b, const XillWork *conv_work, int n_conv_work, void *stream);
 int launch_xill(const BatchDev &b, const XillWork *work, int n_work, void *stream);
+// relxill requests: the primary continuum added to the output (after launch_conv)
+int launch_primary(const BatchDev &b, const int *reqs, int n_reqs, void *stream);
 int launch_copy(const void *src, void *dst, size_t bytes, void *stream);
 int launch_refl_prepare(const DevConvGrid &cg, const double *refl_ener, const double *refl,
                         int n_refl, int n_incl, double *spec, double *bandsum, double *fft,

@@ -179,8 +179,8 @@ def test_relxill_batched_with_other_models(relx):
         assert st == 0
         assert rel_err(y, y0) < 1e-12
     b.close()
-    # setup, integrate, fixup, finalize, conv, xill
-    assert len(ms) == 6 and ms[4] > 0 and ms[5] > 0
+    # setup, integrate, fixup, finalize, conv, xill, primary
+    assert len(ms) == 7 and ms[4] > 0 and ms[5] > 0 and ms[6] > 0
 
 
 def test_relxill_through_the_server(relx):
