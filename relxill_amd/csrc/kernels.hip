@@ -32,6 +32,7 @@ namespace {
 constexpr int SETUP_THREADS = 256;
 constexpr int CONV_THREADS = 256;      // smallest convolution workgroup
 constexpr int XILL_THREADS = 256;
+constexpr int XILL_BINS = 4;  // table bins per k_xill thread
 constexpr int PRIM_THREADS = 256;
 constexpr int CONV_THREADS_MAX = 1024;  // n/8 threads: 512 for 4096 bins, 1024 for 8192
 
@@ -1639,13 +1640,15 @@ __global__ __launch_bounds__(PRIM_THREADS) void k_primary(BatchDev b, const int 
 // skipped (they add exactly 0).  One unit per (request, zone): the zone's Ecut
 // bracket, its angle distribution and its normalisation change
 // (src/Relxill.cpp:357-388).
-__global__ __launch_bounds__(XILL_THREADS) void k_xill(BatchDev b, const XillWork *__restrict__ work, int chunks) {
+__global__ __launch_bounds__(XILL_THREADS, 4) void k_xill(BatchDev b, const XillWork *__restrict__ work, int chunks) {
   const XillWork wk = work[blockIdx.x / chunks];
   const int req = wk.req;
   if (b.status[req] != 0) return;
   const DevXillTable xt = b.xt;
-  const int e = (blockIdx.x % chunks) * XILL_THREADS + threadIdx.x;
-  if (e >= xt.n_ener) return;
+  // XILL_BINS bins per thread, XILL_THREADS apart: the corner weights and rows
+  // are set up once for all of them, and their loads are in flight together
+  const int e0 = (blockIdx.x % chunks) * (XILL_THREADS * XILL_BINS) + threadIdx.x;
+  if (e0 >= xt.n_ener) return;
   const ReqParams &rq = b.req[req];
   const ZoneRec zr = b.zone[rq.zone_off + wk.zone];
   int i0[4];
@@ -1660,7 +1663,7 @@ __global__ __launch_bounds__(XILL_THREADS) void k_xill(BatchDev b, const XillWor
   constexpr unsigned char kCorner[16] = {0x0, 0x8, 0x4, 0x2, 0xC, 0xA, 0x6, 0xE,
                                          0x1, 0x9, 0x5, 0x3, 0xD, 0xB, 0x7, 0xF};
   double w[16];
-  size_t row[16];
+  unsigned row[16];  // first row (inclination 0) of each corner
 #pragma unroll
   for (int c = 0; c < 16; c++) {
     int o[4];
@@ -1671,26 +1674,42 @@ __global__ __launch_bounds__(XILL_THREADS) void k_xill(BatchDev b, const XillWor
       fk[k] = o[k] ? f[k] : (1.0 - f[k]);
     }
     w[c] = fk[0] * fk[1] * fk[2] * fk[3];
-    const size_t r = (((size_t)(i0[0] + o[0]) * xt.n_val[1] + (i0[1] + o[1])) * xt.n_val[2] + (i0[2] + o[2])) *
-                         xt.n_val[3] + (i0[3] + o[3]);
+    const unsigned r = (((unsigned)(i0[0] + o[0]) * xt.n_val[1] + (i0[1] + o[1])) * xt.n_val[2] + (i0[2] + o[2])) *
+                           xt.n_val[3] + (i0[3] + o[3]);
     row[c] = r * xt.n_incl;
   }
   const double *dist = b.dist + rq.dist_off + (size_t)wk.zone * xt.n_incl;
-  const float *spec = xt.spec + e;
   const size_t ne = (size_t)xt.n_ener;
-  double acc = 0.0;
+  int eb[XILL_BINS];
+  bool in[XILL_BINS];
+#pragma unroll
+  for (int u = 0; u < XILL_BINS; u++) {
+    eb[u] = e0 + u * XILL_THREADS;
+    in[u] = eb[u] < xt.n_ener;
+    if (!in[u]) eb[u] = e0;  // a valid address; the result is not stored
+  }
+  double acc[XILL_BINS];
+#pragma unroll
+  for (int u = 0; u < XILL_BINS; u++) acc[u] = 0.0;
   for (int a = 0; a < xt.n_incl; a++) {
     const double d = dist[a];
     if (d == 0.0) continue;
-    float t[16];
+    double s[XILL_BINS];
 #pragma unroll
-    for (int c = 0; c < 16; c++) t[c] = spec[(row[c] + a) * ne];
-    double s = 0.0;
+    for (int u = 0; u < XILL_BINS; u++) s[u] = 0.0;
 #pragma unroll
-    for (int c = 0; c < 16; c++) s += w[c] * (double)t[c];
-    acc += d * s;
+    for (int c = 0; c < 16; c++) {
+      const float *src = xt.spec + (size_t)(row[c] + a) * ne;
+#pragma unroll
+      for (int u = 0; u < XILL_BINS; u++) s[u] += w[c] * (double)src[eb[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < XILL_BINS; u++) acc[u] += d * s[u];
   }
-  b.xspec[rq.xspec_off + (size_t)wk.zone * xt.n_ener + e] = acc / zr.ncf;
+  double *dst = b.xspec + rq.xspec_off + (size_t)wk.zone * xt.n_ener;
+#pragma unroll
+  for (int u = 0; u < XILL_BINS; u++)
+    if (in[u]) dst[eb[u]] = acc[u] / zr.ncf;
 }
 
 // Resident reflection set: per angle, rebin onto the convolution grid, band
@@ -1798,7 +1817,7 @@ int launch_conv(const BatchDev &b, const XillWork *conv_work, int n_conv_work, v
 
 int launch_xill(const BatchDev &b, const XillWork *work, int n_work, void *stream) {
   if (n_work <= 0) return 0;
-  const int chunks = (b.xt.n_ener + XILL_THREADS - 1) / XILL_THREADS;
+  const int chunks = (b.xt.n_ener + XILL_THREADS * XILL_BINS - 1) / (XILL_THREADS * XILL_BINS);
   hipLaunchKernelGGL(k_xill, dim3(n_work * chunks), dim3(XILL_THREADS), 0, (hipStream_t)stream, b, work, chunks);
   return (int)hipGetLastError();
 }
