@@ -313,6 +313,8 @@ void ensure_tables_host() {
       double e = 1.0 * ii / (n + 1 - 1) * (log(1000.0) - log(0.1)) + log(0.1);
       g.coarse_ener[ii] = exp(e);
     }
+    g.coarse_ln.resize(n);
+    for (int ii = 0; ii < n; ii++) g.coarse_ln[ii] = log(0.5 * (g.coarse_ener[ii] + g.coarse_ener[ii + 1]));
   }
   if (g.conv_dirty) {
     const int n = g.conv_n;
@@ -325,6 +327,7 @@ void ensure_tables_host() {
     g.conv_cfac.resize(n);
     g.conv_cfinv.resize(n);
     g.conv_inv_emid.resize(n);
+    g.conv_ln_emid.resize(n);
     for (int ii = 0; ii < n; ii++) {
       // calculate_energyflux_conversion, src/Relbase.cpp:93-103
       const double emid = 0.5 * (g.conv_ener[ii] + g.conv_ener[ii + 1]);
@@ -332,6 +335,7 @@ void ensure_tables_host() {
       g.conv_cfac[ii] = emid / de;
       g.conv_cfinv[ii] = de / emid;
       g.conv_inv_emid[ii] = 1.0 / emid;
+      g.conv_ln_emid[ii] = log(emid);
     }
     g.conv_i1kev = bsearch_asc(g.conv_ener.data(), n + 1, 1.0);
     g.conv_band_lo = n;
@@ -435,7 +439,8 @@ int ensure_device(std::string &err) {
     g.lp_dirty = false;
   }
   if (g.conv_dirty) {
-    e |= upload_vec(g.c_coarse, g.coarse_ener);
+    e |= upload_vec(g.c_coarse, g.coarse_ener) | upload_vec(g.c_coarse_ln, g.coarse_ln);
+    e |= upload_vec(g.c_ln_emid, g.conv_ln_emid);
     e |= upload_vec(g.c_ener, g.conv_ener) | upload_vec(g.c_cfac, g.conv_cfac) | upload_vec(g.c_tw, g.conv_tw);
     e |= upload_vec(g.c_cfinv, g.conv_cfinv) | upload_vec(g.c_inv_emid, g.conv_inv_emid);
     e |= upload_vec(g.c_twp_fwd, g.conv_twp_fwd) | upload_vec(g.c_twp_inv, g.conv_twp_inv);
@@ -535,6 +540,8 @@ void fill_dev_views(BatchDev &bd) {
   bd.cg.twp_inv = g.c_twp_inv.as<double>();
   bd.cg.tw = g.c_tw.as<double>();
   bd.cg.coarse = g.c_coarse.as<double>();
+  bd.cg.coarse_ln = g.c_coarse_ln.as<double>();
+  bd.cg.ln_emid = g.c_ln_emid.as<double>();
   bd.refl_sets = g.d_refl_sets.as<DevReflSet>();
   bd.xt.n_ener = g.xill.n_ener;
   bd.xt.n_incl = g.xill.n_incl();

@@ -99,8 +99,8 @@ struct Global {
   int conv_i1kev = 0;
   int conv_band_lo = 0, conv_band_hi = 0;
   bool conv_dirty = true;
-  std::vector<double> coarse_ener;
-  DevBuf c_coarse;
+  std::vector<double> coarse_ener, coarse_ln, conv_ln_emid;
+  DevBuf c_coarse, c_coarse_ln, c_ln_emid;
   DevBuf c_ener, c_cfac, c_cfinv, c_inv_emid, c_tw, c_twp_fwd, c_twp_inv;
   int nr_fine = 1000;
   // reflection sets

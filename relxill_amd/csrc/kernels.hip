@@ -42,10 +42,13 @@ __device__ inline double lp_ifac(float value, const float *arr, int n, int &ind)
   return (double)((value - arr[ind]) / (arr[ind + 1] - arr[ind]));
 }
 
-// spec_cutoffpl (src/Xillspec.cpp:215-241): photons of the cut-off power law in [e0, e1]
-__device__ inline double cutoffpl_bin(double e0, double e1, double ecut, double gam) {
+// spec_cutoffpl (src/Xillspec.cpp:215-241): photons of the cut-off power law
+// exp(1/Ecut) E^-Gamma exp(-E/Ecut) dE in [e0, e1] at E = (e0 + e1) / 2, as
+// one exponential of ln_en = ln E (precomputed per grid bin): it differs from
+// the product of the three factors by a few ulp of the exponent
+__device__ inline double cutoffpl_bin(double e0, double e1, double ln_en, double ecut, double gam) {
   const double en = 0.5 * (e0 + e1);
-  return exp(1.0 / ecut) * pow(en, -gam) * exp(-en / ecut) * (e1 - e0);
+  return exp(1.0 / ecut - gam * ln_en - en / ecut) * (e1 - e0);
 }
 
 // 1e15 / (4 pi): the xillver tables' incident flux normalisation (src/Xillspec.cpp:203)
@@ -53,11 +56,12 @@ constexpr double NORM_XILLVER_TABLE = 1e15 / 4.0 / 3.14159265358979323846;
 
 // This thread's part of calcNormWrtXillverTableSpec's sum (src/Xillspec.cpp:179-205)
 // for the cut-off power law on the 500-bin 0.1-1000 keV grid ce.
-__device__ inline double xill_norm_part(const double *ce, double ecut, double gam, int tid, int nt) {
+__device__ inline double xill_norm_part(const DevConvGrid &cg, double ecut, double gam, int tid, int nt) {
   double sp = 0.0;
   for (int k = tid; k < N_ENER_COARSE; k += nt) {
-    const double e0 = ce[k], e1 = ce[k + 1];
-    if (e0 >= 0.1 && e0 <= 1000.0) sp += cutoffpl_bin(e0, e1, ecut, gam) * 0.5 * (e0 + e1) * 1e20 * 1.602177e-09;
+    const double e0 = cg.coarse[k], e1 = cg.coarse[k + 1];
+    if (e0 >= 0.1 && e0 <= 1000.0)
+      sp += cutoffpl_bin(e0, e1, cg.coarse_ln[k], ecut, gam) * 0.5 * (e0 + e1) * 1e20 * 1.602177e-09;
   }
   return sp;
 }
@@ -626,12 +630,12 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     }
     __syncthreads();
     if (lp) {
-      const double src = 1. / (block_sum(xill_norm_part(b.cg.coarse, rq.xect_src, rq.xgam, tid, SETUP_THREADS),
+      const double src = 1. / (block_sum(xill_norm_part(b.cg, rq.xect_src, rq.xgam, tid, SETUP_THREADS),
                                          s_red) / NORM_XILLVER_TABLE);
       for (int z = 0; z < nz; z++) {
         const double ez = s_zect[z];
         const double disk =
-            1. / (block_sum(xill_norm_part(b.cg.coarse, ez, rq.xgam, tid, SETUP_THREADS), s_red) / NORM_XILLVER_TABLE);
+            1. / (block_sum(xill_norm_part(b.cg, ez, rq.xgam, tid, SETUP_THREADS), s_red) / NORM_XILLVER_TABLE);
         if (tid == 0) zr[z].ncf = disk / src;
       }
     }
@@ -1607,11 +1611,11 @@ __global__ __launch_bounds__(PRIM_THREADS) void k_primary(BatchDev b, const int 
       prim_scale *= pow(dop, 2);
     }
   }
-  const double nf = 1. / (block_sum4(xill_norm_part(cg.coarse, ect, gam, tid, NT), s_w) / NORM_XILLVER_TABLE);
+  const double nf = 1. / (block_sum4(xill_norm_part(cg, ect, gam, tid, NT), s_w) / NORM_XILLVER_TABLE);
   double *s_e = s_p + n;
   for (int i = tid; i <= n; i += NT) s_e[i] = cg.ener[i];
   for (int i = tid; i < n; i += NT)
-    s_p[i] = (cutoffpl_bin(cg.ener[i], cg.ener[i + 1], ect * rq.g_so, gam) * nf) * prim_scale;
+    s_p[i] = (cutoffpl_bin(cg.ener[i], cg.ener[i + 1], cg.ln_emid[i], ect * rq.g_so, gam) * nf) * prim_scale;
   __syncthreads();
   const GridGuess gc = grid_guess(cg.ener, n);
   auto ge_c = [&](int k) { return s_e[k]; };

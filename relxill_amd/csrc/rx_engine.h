@@ -168,6 +168,8 @@ struct DevConvGrid {
   const double *twp_fwd;  // per-pass twiddles of the n-point FFT ([r][k] per pass)
   const double *twp_inv;  // per-pass twiddles of the n/2-point FFT
   const double *coarse;   // N_ENER_COARSE + 1 log edges over 0.1-1000 keV
+  const double *coarse_ln;  // ln of the coarse bin centres
+  const double *ln_emid;    // ln of the bin centres, n
 };
 
 // Fine radii are integrated in chunks of one wavefront: lane l of chunk c
