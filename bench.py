@@ -144,6 +144,9 @@ def relxilllp_batch(rx, steps, n_req=1024, seed=10):
     b.autotune(rx.FP32)
     dt32, kms32 = timed_steps(rx, b, rx.FP32, max(3, steps // 2), 1)
     o32 = [b.output(i)[0] for i in range(n_req)]
+    xv = rx.xilltable_view()
+    n_e, n_incl = xv["n_ener"], xv["n_val"][-1]
+    n_angles = sum(int(np.count_nonzero(b.dist(i, 10 * n_incl))) for i in range(n_req))
     dt64, _ = timed_steps(rx, b, rx.FP64, 2, 1)
     o64 = [b.output(i)[0] for i in range(n_req)]
     b.close()
@@ -153,7 +156,14 @@ def relxilllp_batch(rx, steps, n_req=1024, seed=10):
         d, _ = timed_steps(rx, pb, rx.FP64, 3, 1)
         lat.append(d)
         pb.close()
-    return dict(n_requests=n_req, zones=10, batched_fp32_spectra_per_s=n_req / dt32,
+    # k_xill algorithmic bytes: per zone and angle of nonzero weight 16 corner
+    # rows of n_e float bins; every zone's FP64 spectrum written
+    alg = 16 * n_angles * n_e * 4 + n_req * 10 * 8 * n_e
+    t_x = kms32[5] * 1e-3
+    return dict(n_requests=n_req, zones=10, mean_angles_used_per_zone=n_angles / n_req / 10,
+                k_xill_roofline=dict(bound="hbm", algorithmic_bytes=alg, achieved=alg / t_x / 1e9, peak=HBM_PEAK_GBS,
+                                     unit="GB/s", frac=alg / t_x / 1e9 / HBM_PEAK_GBS),
+                batched_fp32_spectra_per_s=n_req / dt32,
                 batched_fp64_spectra_per_s=n_req / dt64, plain_fp64_ms_per_spectrum=float(np.mean(lat)) * 1e3,
                 max_rel_err_fp32_vs_fp64=max(rel_err(o32[i], o64[i]) for i in range(n_req)),
                 kernel_ms={k: round(v, 4) for k, v in zip(KERNELS, kms32)})
