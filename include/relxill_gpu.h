@@ -29,7 +29,7 @@ extern "C" {
 /* ---- status codes ---- */
 #define RX_OK 0
 #define RX_ERR_PARAM 1        /* parameters rejected (check_parameter_bounds) */
-#define RX_ERR_UNSUPPORTED 2  /* feature out of scope (returning radiation, extended jet) */
+#define RX_ERR_UNSUPPORTED 2  /* feature out of scope (relxilllp's returning radiation, extended jet) */
 #define RX_ERR_TABLE 10       /* table interpolation failed (grid invariants) */
 #define RX_ERR_INPUT 50       /* relconv input flux with sum <= 0 (src/LocalModel.cpp:84-86) */
 #define RX_ERR_DEVICE 100     /* HIP error / no device */
@@ -63,7 +63,8 @@ void lmodrelline(const double *energy, int Nflux, const double *parameter, int s
                  double *flux, double *fluxError, const char *init);
 
 /* relline_lp: lineE h a Incl Rin Rout z limb gamma switch_returnrad
- * (lmodel_relxill_public.dat:23-33); switch_returnrad must be 0 */
+ * (lmodel_relxill_public.dat:23-33); switch_returnrad 1 adds the returning
+ * radiation to the lamp-post emissivity (src/Rellp.cpp:483-507) */
 void lmodrellinelp(const double *energy, int Nflux, const double *parameter, int spectrum,
                    double *flux, double *fluxError, const char *init);
 
@@ -73,7 +74,7 @@ void lmodrelconv(const double *energy, int Nflux, const double *parameter, int s
                  double *flux, double *fluxError, const char *init);
 
 /* relconv_lp: h beta a Incl Rin Rout limb gamma switch_returnrad
- * (lmodel_relxill_public.dat:35-44); switch_returnrad must be 0 */
+ * (lmodel_relxill_public.dat:35-44); switch_returnrad as for relline_lp */
 void lmodrelconvlp(const double *energy, int Nflux, const double *parameter, int spectrum,
                    double *flux, double *fluxError, const char *init);
 
@@ -171,6 +172,20 @@ typedef struct {
   const float *spec;  /* rows as stored in the file (before the 10^logXi division) */
 } rx_xilltable_view;
 int rx_xilltable_get_view(rx_xilltable_view *v);
+
+/* Returning-radiation table of the lamp-post models (switch_returnrad = 1;
+ * the reference's table_returnRad_v20220301.fits, src/common.h:126-128):
+ * loaded from RELXILL_TABLE_PATH on the first request that needs it, else a
+ * synthetic table (21 spins x 50 radial zones x 20 energy-shift bins).  The
+ * FITS loader reads the layout of src/Relreturn_Table.cpp:126-340 (extension
+ * SPIN, then FRAC01, FRAC02, ... with rlo, rhi, tf_r, gmin, gmax, frac_g). */
+int rx_rradtable_synthetic(uint64_t seed, int n_spin, int n_rad, int n_g);
+int rx_rradtable_load_fits(const char *path);
+typedef struct {
+  int n_spin, n_rad, n_g;
+  const double *spin, *rlo, *rhi, *tf_r, *gmin, *gmax, *frac_g;
+} rx_rradtable_view;
+int rx_rradtable_get_view(rx_rradtable_view *v);
 
 /* Read-only host view of the resident tables (for checking). */
 typedef struct {

@@ -34,9 +34,14 @@ class RelTable(ctypes.Structure):
                 ("cosne1", _F), ("cosne2", _F)]
 
 
+class RradTable(ctypes.Structure):
+    _fields_ = [("n_spin", _I), ("n_rad", _I), ("n_g", _I)] + \
+               [(k, _D) for k in ("spin", "rlo", "rhi", "tf_r", "gmin", "gmax", "frac_g")]
+
+
 class LpTable(ctypes.Structure):
     _fields_ = [("n_a", _I), ("n_h", _I), ("n_r", _I), ("a", _F), ("h", _F), ("rad", _F),
-                ("intens", _F), ("del_", _F), ("del_inc", _F)]
+                ("intens", _F), ("del_", _F), ("del_inc", _F), ("rrad", ctypes.POINTER(RradTable))]
 
 
 class RelParam(ctypes.Structure):
@@ -126,6 +131,7 @@ def _load():
         "orc_get_relxill_params": (_I, [_I, _D, _I, ctypes.POINTER(RelParam), ctypes.POINTER(XillParam)]),
         "orc_energy_shift_source_obs": (ctypes.c_double, [ctypes.POINTER(RelParam)]),
         "orc_relxill_zones": (_I, [_I, ctypes.POINTER(RelTable), ctypes.POINTER(LpTable), _D, _D]),
+        "orc_rrad_emis": (_I, [ctypes.POINTER(RradTable), ctypes.POINTER(RelParam), _D, _D, _I, _D]),
         "orc_relxill_model": (_I, [_I, ctypes.POINTER(RelTable), ctypes.POINTER(LpTable), ctypes.POINTER(XillTable),
                                    _D, _D, _D, _I, _D, _I, _I, _D, ctypes.POINTER(Stats)]),
         "orc_relxill": (_I, [ctypes.POINTER(RelTable), ctypes.POINTER(XillTable), _D, _D, _D, _I, _D, _I, _I,
@@ -161,7 +167,7 @@ class Tables:
     """Oracle view of a relline table (and optional lamp-post table) given as
     dicts of numpy arrays in the layout of relxill_amd.tables_view()."""
 
-    def __init__(self, rel, lp=None):
+    def __init__(self, rel, lp=None, rrad=None):
         self._keep = []
 
         def f32(k, d):
@@ -172,10 +178,19 @@ class Tables:
                             *[_fp(f32(k, rel)) for k in ("a", "mu0", "r", "gmin", "gmax",
                                                          "trff1", "trff2", "cosne1", "cosne2")])
         self.lp = None
+        self.rr = None
         if lp is not None:
             self.lp = LpTable(lp["n_a"], lp["n_h"], lp["n_r"],
                               *[_fp(f32(k, lp)) for k in ("a", "h", "rad", "intens", "del_",
                                                           "del_inc")])
+            if rrad is not None:
+                def f64(k):
+                    a = np.ascontiguousarray(rrad[k], dtype=np.float64)
+                    self._keep.append(a)
+                    return _dp(a)
+                self.rr = RradTable(rrad["n_spin"], rrad["n_rad"], rrad["n_g"],
+                                    *[f64(k) for k in ("spin", "rlo", "rhi", "tf_r", "gmin", "gmax", "frac_g")])
+                self.lp.rrad = ctypes.pointer(self.rr)
         self.rel_arrays = rel
         self.lp_arrays = lp
 
@@ -432,3 +447,21 @@ def relxill_zones(model, tables, params):
 def energy_shift_source_obs(model, params):
     p, st = rel_params(model, params)
     return lib.orc_energy_shift_source_obs(ctypes.byref(p)), st
+
+
+def rrad_emis(rrad, relparam, re, emis0):
+    """returning-radiation emissivity on the fine grid re (descending); (array, status)"""
+    t = Tables.__new__(Tables)
+    keep = []
+
+    def f64(k):
+        a = np.ascontiguousarray(rrad[k], dtype=np.float64)
+        keep.append(a)
+        return _dp(a)
+    rt = RradTable(rrad["n_spin"], rrad["n_rad"], rrad["n_g"],
+                   *[f64(k) for k in ("spin", "rlo", "rhi", "tf_r", "gmin", "gmax", "frac_g")])
+    re = _arr(re)
+    e0 = _arr(emis0)
+    out = np.zeros(re.size)
+    st = lib.orc_rrad_emis(ctypes.byref(rt), ctypes.byref(relparam), _dp(re), _dp(e0), re.size, _dp(out))
+    return out, st

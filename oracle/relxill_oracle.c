@@ -396,7 +396,6 @@ int orc_get_rel_params(int model_type, const double *par, int phys_norm_env, orc
     p->do_renorm = phys_norm_env ? 0 : 1;
   }
   if (check_parameter_bounds(p)) return 1;
-  if (p->return_rad != 0) return 2;  /* returning radiation is out of scope */
   return 0;
 }
 
@@ -698,6 +697,120 @@ done:
   return status;
 }
 
+/* ---- returning radiation (src/Relreturn_Corona.cpp, src/Relreturn_Table.cpp) ---- */
+
+/* inv_rebin_mean (src/relutility.c:636-663): y0 on the descending x0,
+ * interpolated at the ascending xn */
+static void inv_rebin_mean(const double *x0, const double *y0, int n0, const double *xn, double *yn, int nn) {
+  int in = nn - 1;
+  for (int ii = 0; ii < n0 - 1; ii++) {
+    if (x0[ii] > xn[in] && x0[ii + 1] <= xn[in]) {
+      const double ifac_r = (xn[in] - x0[ii + 1]) / (x0[ii] - x0[ii + 1]);
+      yn[in] = orc_interp_lin_1d(ifac_r, y0[ii + 1], y0[ii]);
+      in--;
+      if (in < 0) break;
+    }
+  }
+}
+
+
+
+int orc_rrad_emis(const orc_rradtable *rt, const orc_relparam *p, const double *re, const double *emis0, int nr,
+                  double *emis_ret) {
+  const double a = p->a;
+  /* determine_rlo_rhi (:201-213): the fine grid's ends */
+  const double rin = re[nr - 1], rout = re[0];
+  /* select_spinIndexForTable (src/Relreturn_Table.cpp:343-357): the first
+   * tabulated spin at or above a */
+  int k = orc_binary_search(rt->spin, rt->n_spin, a);
+  if (k < 0) k = 0;
+  if (rt->spin[k] < a) k++;
+  if (k >= rt->n_spin) return 35;
+  const int nt = rt->n_rad, ng = rt->n_g;
+  const double *rlo_t = rt->rlo + (size_t)k * nt, *rhi_t = rt->rhi + (size_t)k * nt;
+  /* allocate_radial_grid (:359-404): the table zones covering [rin, rout],
+   * the outer ends replaced by rin and rout */
+  int klo = orc_binary_search(rlo_t, nt, rin);
+  int khi = orc_binary_search(rhi_t, nt, rout);
+  if (fabs(rout - rhi_t[nt - 1]) < 1e-6) khi = nt - 1; else khi++;
+  if (fabs(rin - rlo_t[0]) < 1e-6) klo = 0;
+  const int n = (khi + 1) - klo;
+  if (n < 2 || n > nt || klo < 0) return 36;
+  if (!(rin >= rlo_t[klo] - 1e-4)) return 37;
+  double *rlo = (double *)malloc(sizeof(double) * n), *rhi = (double *)malloc(sizeof(double) * n);
+  double *rad = (double *)malloc(sizeof(double) * n), *emis_in = (double *)malloc(sizeof(double) * n);
+  double *ret = (double *)malloc(sizeof(double) * n), *tfr = (double *)malloc(sizeof(double) * n * n);
+  double *term = (double *)malloc(sizeof(double) * n);
+  for (int i = 0; i < n; i++) {
+    rlo[i] = rlo_t[klo + i];
+    rhi[i] = rhi_t[klo + i];
+  }
+  rlo[0] = rin;
+  rhi[n - 1] = rout;
+  for (int i = 0; i < n; i++) rad[i] = 0.5 * (rlo[i] + rhi[i]);
+  /* get_interpolated_tfr (:437-480) with the ring-area corrections of the end
+   * zones (get_ring_area_correction_factor, :415-435) */
+  const double *tf = rt->tf_r + (size_t)k * nt * nt;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) tfr[i * n + j] = tf[(size_t)(klo + i) * nt + (klo + j)];
+  double corr[2];
+  for (int e = 0; e < 2; e++) {
+    const int i = e == 0 ? 0 : n - 1;
+    double rl = rlo_t[klo + i];
+    if (klo + i == 0 && rl > orc_kerr_rms(a)) rl = orc_kerr_rms(a);
+    const double rh = rhi_t[klo + i];
+    const double area_table = 0.5 * (rl + rh) * (rh - rl);
+    const double area_model = 0.5 * (rlo[i] + rhi[i]) * (rhi[i] - rlo[i]);
+    corr[e] = area_model / area_table;
+  }
+  for (int i = 0; i < n - 1; i++) tfr[i * n + 0] *= corr[0];
+  for (int i = 0; i < n - 2; i++) tfr[i * n + n - 1] *= corr[1];
+  /* the direct emissivity at the zone centres (inv_rebin_mean, src/relutility.c:636-663) */
+  inv_rebin_mean(re, emis0, nr, rad, emis_in, n);
+  /* calc_rrad_emis_corona (src/Relreturn_Corona.cpp:119-151) with
+   * calc_rrad_emis_zone (:87-108): sum over g of frac_g g^(Gamma-1) */
+  for (int i = 0; i < n; i++) {
+    for (int j = 0; j < n; j++) {
+      const size_t q = (size_t)k * nt * nt + (size_t)(klo + i) * nt + (klo + j);
+      const double gmin = rt->gmin[q], gmax = rt->gmax[q];
+      double ez = 0.0;
+      for (int jj = 0; jj < ng; jj++) {
+        const double g = (jj + 0.5) / ng * (gmax - gmin) + gmin;  /* get_gfac_grid, src/Relreturn_Datastruct.cpp:51-59 */
+        double v = rt->frac_g[q * ng + jj];
+        if (fabs(g - 1) > 1e-3) v *= pow(g, p->gamma - 1);
+        ez += v;
+      }
+      term[j] = ez * tfr[i * n + j] * emis_in[j];
+    }
+    double sum = 0.0;
+    for (int j = 0; j < n; j++) sum += term[j];
+    ret[i] = sum;
+  }
+  /* back to the fine grid (rebin_emisprofile_on_radial_grid, src/Rellp.cpp:120-175;
+   * the profile's emission angles are 0, so the radius factor is linear and
+   * the emissivity log-interpolated) */
+  int status = 0;
+  int kk = orc_binary_search(rad, n, re[nr - 1]);
+  if (kk < 0 || kk >= n - 1) status = 38;
+  for (int ii = nr - 1; ii >= 0 && status == 0; ii--) {
+    while (re[ii] >= rad[kk + 1]) {
+      kk++;
+      if (kk >= n - 1) {
+        if (re[ii] - RELTABLE_MAX_R <= 1e-6) {
+          kk = n - 2;
+          break;
+        }
+        status = 39;
+        break;
+      }
+    }
+    const double inter_r = get_ipol_factor_radius(rad[kk], rad[kk + 1], 0.0, re[ii]);
+    emis_ret[ii] = interp_log_1d(inter_r, ret[kk], ret[kk + 1]);
+  }
+  free(rlo); free(rhi); free(rad); free(emis_in); free(ret); free(tfr); free(term);
+  return status;
+}
+
 /* calc_emis_profile; src/Rellp.cpp:518-564 (BKN and LP point source) */
 int orc_calc_emis_profile(const orc_lptable *lp, const orc_relparam *p, orc_syspar *s) {
   if (p->emis_type == ORC_EMIS_BKN) {
@@ -712,8 +825,19 @@ int orc_calc_emis_profile(const orc_lptable *lp, const orc_relparam *p, orc_sysp
     if (!lp) return 30;
     /* point source unless htop describes an extended jet (:287-294) */
     if (!((fabs(p->htop) <= 1.0) || (p->htop - 1e-3 <= p->height))) return 31;
-    if (p->return_rad != 0) return 32;
-    return emis_lamp_post(lp, p, s);
+    int st = emis_lamp_post(lp, p, s);
+    if (st == 0 && p->return_rad != 0) {
+      /* add_returnrad_emis (src/Rellp.cpp:483-507) */
+      if (!lp->rrad) return 32;
+      if (!(p->return_rad == 1 || p->return_rad == -1 || p->return_rad == 2)) return 34;
+      double *er = (double *)malloc(sizeof(double) * s->nr);
+      st = orc_rrad_emis(lp->rrad, p, s->re, s->emis, s->nr, er);
+      if (st == 0) {
+        for (int ii = 0; ii < s->nr; ii++) s->emis[ii] = (p->return_rad == 1) ? s->emis[ii] + er[ii] : er[ii];
+      }
+      free(er);
+    }
+    return st;
   }
   return 33;
 }
@@ -1314,20 +1438,6 @@ double orc_energy_shift_source_obs(const orc_relparam *p) {
   return g_inf_0 * doppler_factor(M_PI - p->incl, p->beta);
 }
 
-/* inv_rebin_mean (src/relutility.c:636-663): y0 on the descending x0,
- * interpolated at the ascending xn */
-static void inv_rebin_mean(const double *x0, const double *y0, int n0, const double *xn, double *yn, int nn) {
-  int in = nn - 1;
-  for (int ii = 0; ii < n0 - 1; ii++) {
-    if (x0[ii] > xn[in] && x0[ii + 1] <= xn[in]) {
-      const double ifac_r = (xn[in] - x0[ii + 1]) / (x0[ii] - x0[ii + 1]);
-      yn[in] = orc_interp_lin_1d(ifac_r, y0[ii + 1], y0[ii]);
-      in--;
-      if (in < 0) break;
-    }
-  }
-}
-
 int orc_relxill_model(int model_type, const orc_reltable *tab, const orc_lptable *lp,
                       const orc_xilltable *xt, const double *parameter, const double *energy, double *flux,
                       int n_flux, const double *conv_ener, int n_conv, int phys_norm_env, double *dist_out,
@@ -1335,7 +1445,9 @@ int orc_relxill_model(int model_type, const orc_reltable *tab, const orc_lptable
   orc_relparam p;
   orc_xillparam x;
   int status = orc_get_relxill_params(model_type, parameter, phys_norm_env, &p, &x);
-  if (status == 0 && p.return_rad != 0) status = 2;  /* returning radiation is out of scope */
+  /* relxill models' returning radiation needs the xillver correction factors
+   * (calc_rrad_corr_factors, src/Relxill.cpp:337-341), which are not restated */
+  if (status == 0 && p.return_rad != 0) status = 2;
   if (status) {
     memset(flux, 0, sizeof(double) * n_flux);
     return status;

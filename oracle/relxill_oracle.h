@@ -56,6 +56,17 @@ typedef struct {
   const float *cosne2;
 } orc_reltable;
 
+/* returning-radiation table (src/Relreturn_Table.h): per spin n_rad zones
+ * [rlo, rhi] and per (incident ro, emitting re) pair tf_r, gmin, gmax and
+ * frac_g[n_g]; flat, spin-major */
+typedef struct orc_rradtable {
+  int n_spin, n_rad, n_g;
+  const double *spin;            /* [n_spin] ascending */
+  const double *rlo, *rhi;       /* [n_spin][n_rad] */
+  const double *tf_r, *gmin, *gmax;  /* [n_spin][n_rad][n_rad] */
+  const double *frac_g;          /* [n_spin][n_rad][n_rad][n_g] */
+} orc_rradtable;
+
 /* lamp-post table (src/reltable.h:51-69): the h grid is stored per spin */
 typedef struct {
   int n_a, n_h, n_r;
@@ -65,6 +76,7 @@ typedef struct {
   const float *intens;   /* [n_a][n_h][n_r] */
   const float *del;      /* [n_a][n_h][n_r], already |.| as at load time */
   const float *del_inc;  /* [n_a][n_h][n_r] */
+  const orc_rradtable *rrad;  /* returning radiation (may be NULL: not available) */
 } orc_lptable;
 
 /* relativistic parameters (src/common.h:191-214, subset on this path) */
@@ -142,6 +154,11 @@ void orc_syspar_free(orc_syspar *s);
 int orc_interpol_reltable(const orc_reltable *tab, double a, double incl, double rin,
                           double rout, orc_syspar *out);
 int orc_calc_emis_profile(const orc_lptable *lp, const orc_relparam *p, orc_syspar *s);
+/* returning-radiation emissivity on the fine grid re (descending) for the
+ * direct emissivity emis0 (get_rrad_emis_corona, src/Relreturn_Corona.cpp:291-321,
+ * without the relxill correction factors) */
+int orc_rrad_emis(const orc_rradtable *rt, const orc_relparam *p, const double *re, const double *emis0, int nr,
+                  double *emis_ret);
 /* everything get_system_parameters does (src/Relprofile.cpp:310-358) */
 orc_syspar *orc_get_system_parameters(const orc_reltable *tab, const orc_lptable *lp,
                                       const orc_relparam *p, int nr_fine, int *status);

@@ -75,6 +75,11 @@ class _XillView(ctypes.Structure):
                 ("val", _F * 5), ("n_ener", ctypes.c_int), ("elo", _F), ("ehi", _F), ("spec", _F)]
 
 
+class _RradView(ctypes.Structure):
+    _fields_ = [("n_spin", ctypes.c_int), ("n_rad", ctypes.c_int), ("n_g", ctypes.c_int)] + \
+               [(k, _D) for k in ("spin", "rlo", "rhi", "tf_r", "gmin", "gmax", "frac_g")]
+
+
 class _LpView(ctypes.Structure):
     _fields_ = [("n_a", ctypes.c_int), ("n_h", ctypes.c_int), ("n_r", ctypes.c_int),
                 ("a", _F), ("h", _F), ("rad", _F), ("intens", _F), ("del_", _F),
@@ -92,6 +97,9 @@ _EXPORTS = {
     "rx_xilltable_synthetic": (ctypes.c_int, [ctypes.c_uint64] + [ctypes.c_int] * 6),
     "rx_xilltable_load_fits": (ctypes.c_int, [ctypes.c_char_p]),
     "rx_xilltable_get_view": (ctypes.c_int, [ctypes.POINTER(_XillView)]),
+    "rx_rradtable_synthetic": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "rx_rradtable_load_fits": (ctypes.c_int, [ctypes.c_char_p]),
+    "rx_rradtable_get_view": (ctypes.c_int, [ctypes.POINTER(_RradView)]),
     "rx_eval_model": (ctypes.c_int, [ctypes.c_int, _D, _D, _D, ctypes.c_int, ctypes.c_int]),
     "rx_last_status": (ctypes.c_int, []),
     "rx_last_error": (ctypes.c_char_p, []),
@@ -238,6 +246,21 @@ def xilltable_view():
                 val=[f(v.val[k], n_val[k]) for k in range(v.n_par)], n_ener=v.n_ener,
                 elo=f(v.elo, v.n_ener), ehi=f(v.ehi, v.n_ener),
                 spec=f(v.spec, rows * v.n_ener).reshape(rows, v.n_ener))
+
+
+def rradtable_synthetic(seed=7, n_spin=21, n_rad=50, n_g=20):
+    _check(lib.rx_rradtable_synthetic(seed, n_spin, n_rad, n_g))
+
+
+def rradtable_view():
+    """Copy of the resident returning-radiation table (spin-major arrays)."""
+    v = _RradView()
+    _check(lib.rx_rradtable_get_view(ctypes.byref(v)))
+    ns, nr, ng = v.n_spin, v.n_rad, v.n_g
+    f = lambda ptr, n: np.ctypeslib.as_array(ptr, shape=(n,)).copy()
+    return dict(n_spin=ns, n_rad=nr, n_g=ng, spin=f(v.spin, ns), rlo=f(v.rlo, ns * nr), rhi=f(v.rhi, ns * nr),
+                tf_r=f(v.tf_r, ns * nr * nr), gmin=f(v.gmin, ns * nr * nr), gmax=f(v.gmax, ns * nr * nr),
+                frac_g=f(v.frac_g, ns * nr * nr * ng))
 
 
 def tables_synthetic(seed=1, n_a=25, n_mu0=22, n_r=100, n_g=20):

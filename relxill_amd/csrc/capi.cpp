@@ -316,6 +316,50 @@ int rx_xilltable_get_view(rx_xilltable_view *v) {
   return RX_OK;
 }
 
+int rx_rradtable_synthetic(uint64_t seed, int n_spin, int n_rad, int n_g) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (n_spin < 1 || n_rad < 2 || n_g < 1 || n_rad > 256 || n_g > 256) {
+    set_last(RX_ERR_ARG, "returning-radiation table dims out of range");
+    return RX_ERR_ARG;
+  }
+  g.rrad = make_synthetic_rradtable(seed, n_spin, n_rad, n_g);
+  g.rrad_loaded = true;
+  g.rrad_dirty = true;
+  g.config_gen++;
+  std::string e = validate_rradtable(g.rrad);
+  set_last(e.empty() ? RX_OK : RX_ERR_TABLE, e);
+  return e.empty() ? RX_OK : RX_ERR_TABLE;
+}
+
+int rx_rradtable_load_fits(const char *path) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!path) return RX_ERR_ARG;
+  RradTableHost t;
+  std::string e = load_rradtable_fits(t, path);
+  if (!e.empty()) {
+    set_last(RX_ERR_TABLE, e);
+    return RX_ERR_TABLE;
+  }
+  g.rrad = std::move(t);
+  g.rrad_loaded = true;
+  g.rrad_dirty = true;
+  g.config_gen++;
+  return RX_OK;
+}
+
+int rx_rradtable_get_view(rx_rradtable_view *v) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!v) return RX_ERR_ARG;
+  ensure_rradtable_host();
+  const RradTableHost &t = g.rrad;
+  *v = {t.n_spin, t.n_rad, t.n_g, t.spin.data(), t.rlo.data(), t.rhi.data(), t.tf_r.data(), t.gmin.data(),
+        t.gmax.data(), t.frac_g.data()};
+  return RX_OK;
+}
+
 int rx_tables_save(const char *path) {
   Global &g = G();
   std::lock_guard<std::recursive_mutex> lk(g.mu);

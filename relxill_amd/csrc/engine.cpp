@@ -165,9 +165,18 @@ int parse_request(int model, const double *par, bool phys_norm, ReqParams &p, st
     return status;
   }
   if (return_rad != 0) {
-    err = "returning radiation (switch_returnrad != 0) is not supported";
-    return RX_ERR_UNSUPPORTED;
+    // add_returnrad_emis (src/Rellp.cpp:483-507): 1 adds the returning
+    // radiation, -1 and 2 replace the direct emissivity by it
+    if (model == MOD_RELXILLLP) {
+      err = "returning radiation of relxilllp (its xillver correction factors) is not supported";
+      return RX_ERR_UNSUPPORTED;
+    }
+    if (!(return_rad == 1 || return_rad == -1 || return_rad == 2)) {
+      err = "switch_returnrad must be 0, 1, -1 or 2";
+      return RX_ERR_PARAM;
+    }
   }
+  p.return_rad = return_rad;
   p.norm = 1.0;
   if (model < 0 && p.emis_type == EMIS_BKN) {
     // norm_factor_semi_infinite_slab, src/Xillspec.cpp:528-530
@@ -504,6 +513,61 @@ int ensure_xilltable_device(std::string &err) {
   return RX_OK;
 }
 
+void ensure_rradtable_host() {
+  Global &g = G();
+  if (g.rrad_loaded) return;
+  bool ok = false;
+  if (const char *path = getenv("RELXILL_TABLE_PATH")) {
+    // the reference's table name (src/common.h:128)
+    ok = load_rradtable_fits(g.rrad, std::string(path) + "/table_returnRad_v20220301.fits").empty();
+  }
+  if (!ok) g.rrad = make_synthetic_rradtable(7, 21, 50, 20);
+  g.rrad_loaded = true;
+  g.rrad_dirty = true;
+}
+
+int ensure_rradtable_device(std::string &err) {
+  Global &g = G();
+  ensure_rradtable_host();
+  if (!g.rrad_dirty) return RX_OK;
+  if (hipDeviceSynchronize() != hipSuccess) {
+    err = "device synchronisation failed";
+    return RX_ERR_DEVICE;
+  }
+  const RradTableHost &t = g.rrad;
+  int e = upload_vec(g.r_rlo, t.rlo) | upload_vec(g.r_rhi, t.rhi) | upload_vec(g.r_tf, t.tf_r) |
+          upload_vec(g.r_gmin, t.gmin) | upload_vec(g.r_gmax, t.gmax) | upload_vec(g.r_fg, t.frac_g);
+  if (e) {
+    err = "returning-radiation table upload failed";
+    return RX_ERR_DEVICE;
+  }
+  g.rrad_dirty = false;
+  return RX_OK;
+}
+
+int rrad_zones(const RradTableHost &t, ReqParams &p, int nr_fine) {
+  // the fine grid's ends (determine_rlo_rhi, src/Relreturn_Corona.cpp:201-213)
+  const double rin = fine_radius(p.rin, p.rout, nr_fine - 1, nr_fine);
+  const double rout = fine_radius(p.rin, p.rout, 0, nr_fine);
+  int k = bsearch_asc(t.spin.data(), t.n_spin, p.a);
+  if (k < 0) k = 0;
+  if (t.spin[k] < p.a) k++;
+  if (k >= t.n_spin) return 35;
+  const int nt = t.n_rad;
+  const double *rlo = &t.rlo[(size_t)k * nt], *rhi = &t.rhi[(size_t)k * nt];
+  int klo = bsearch_asc(rlo, nt, rin);
+  int khi = bsearch_asc(rhi, nt, rout);
+  if (fabs(rout - rhi[nt - 1]) < 1e-6) khi = nt - 1; else khi++;
+  if (fabs(rin - rlo[0]) < 1e-6) klo = 0;
+  const int n = (khi + 1) - klo;
+  if (n < 2 || n > nt || n > MAX_RR || klo < 0) return 36;
+  if (!(rin >= rlo[klo] - 1e-4)) return 37;
+  p.rr_k = k;
+  p.rr_klo = klo;
+  p.rr_n = n;
+  return 0;
+}
+
 void fill_dev_views(BatchDev &bd) {
   Global &g = G();
   bd.rt.n_a = g.rel.n_a;
@@ -559,6 +623,15 @@ void fill_dev_views(BatchDev &bd) {
   }
   bd.xt.ect_vals = g.x_ect.as<float>();
   bd.xt.q_vals = g.x_q.as<float>();
+  bd.rr.n_spin = g.rrad.n_spin;
+  bd.rr.n_rad = g.rrad.n_rad;
+  bd.rr.n_g = g.rrad.n_g;
+  bd.rr.rlo = g.r_rlo.as<double>();
+  bd.rr.rhi = g.r_rhi.as<double>();
+  bd.rr.tf_r = g.r_tf.as<double>();
+  bd.rr.gmin = g.r_gmin.as<double>();
+  bd.rr.gmax = g.r_gmax.as<double>();
+  bd.rr.frac_g = g.r_fg.as<double>();
 }
 
 // ---------------------------------------------------------------------------
@@ -596,6 +669,15 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
   for (int i = 0; i < n_req; i++) {
     if (is_relxill(req[i].model)) {
       int st = ensure_xilltable_device(err);
+      if (st != RX_OK) return st;
+      break;
+    }
+  }
+  for (int i = 0; i < n_req; i++) {
+    const int m = req[i].model;
+    const int k_rr = (m == MOD_RELLINELP) ? 9 : (m == MOD_RELCONVLP) ? 8 : -1;
+    if (k_rr >= 0 && req[i].parameter && lround(req[i].parameter[k_rr]) != 0) {
+      int st = ensure_rradtable_device(err);
       if (st != RX_OK) return st;
       break;
     }
@@ -667,6 +749,7 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
       p.pgrid_off = o_pgrid;
       o_pgrid += r.n_flux + 1;
     }
+    if (st == RX_OK && p.return_rad != 0) st = rrad_zones(g.rrad, p, g.nr_fine);
     p.need_cosne = (p.limb != 0 || p.n_incl > 0) ? 1 : 0;
     p.dist_off = o_dist;
     o_dist += (int64_t)p.n_incl * p.n_zones;

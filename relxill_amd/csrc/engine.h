@@ -92,6 +92,10 @@ struct Global {
   bool xill_loaded = false, xill_dirty = true;
   std::vector<double> xill_ener;
   DevBuf x_spec, x_ener, x_ect, x_q;
+  // returning-radiation table (loaded and uploaded on the first request with it)
+  RradTableHost rrad;
+  bool rrad_loaded = false, rrad_dirty = true;
+  DevBuf r_rlo, r_rhi, r_tf, r_gmin, r_gmax, r_fg;
   // convolution grid
   int conv_n = 4096;
   double conv_emin = 0.00035, conv_emax = 2000.0;
@@ -125,6 +129,12 @@ Global &G();
 void ensure_tables_host();
 int ensure_device(std::string &err);  // device + uploads (tables, conv grid, refl sets)
 void ensure_xilltable_host();
+void ensure_rradtable_host();
+int ensure_rradtable_device(std::string &err);
+// the returning-radiation table zones of an LP request (select_spinIndexForTable
+// and allocate_radial_grid, src/Relreturn_Table.cpp:343-404); a status != 0 if
+// the table does not cover the disk
+int rrad_zones(const RradTableHost &t, ReqParams &p, int nr_fine);
 int ensure_xilltable_device(std::string &err);
 void fill_dev_views(BatchDev &bd);
 void set_last(int status, const std::string &msg);

@@ -5,6 +5,9 @@
 //     of spin i and mu0 j is HDU number i * n_mu0 + j + 4 (1-based, :287-288),
 //     with n_r rows of r, gmin, gmax and the n_g-vectors trff1, trff2, cosne1,
 //     cosne2;
+//   xillver tables (xillver-a-Ec5.fits, src/xilltable.c:169-277, :513-566) and
+//     the returning-radiation table (table_returnRad_v20220301.fits,
+//     src/Relreturn_Table.cpp:126-340) are read further below;
 //   lamp-post table (rel_lp_table_v0.5b.fits, :313-448): extension "I_h", one
 //     row per spin with columns a, hgrid[n_h], r[n_rad] and, for every height
 //     k = 1..n_h, h<k>[n_rad] (intensity), del<k>, del_inc<k> (angles, stored
@@ -42,7 +45,7 @@ struct Fits {
   std::string err;
 };
 
-constexpr int kReadOnly = 0, kBinaryTbl = 2, kCaseInsen = 0, kTFloat = 42, kTInt = 31, kTString = 16;
+constexpr int kReadOnly = 0, kBinaryTbl = 2, kCaseInsen = 0, kTFloat = 42, kTInt = 31, kTString = 16, kTDouble = 82;
 
 // Candidate locations: RELXILL_CFITSIO_LIB, HEASoft's $HEADAS/lib (where an
 // XSPEC installation keeps it), the loader's search path, $CONDA_PREFIX/lib.
@@ -133,6 +136,11 @@ struct FitsFile {
     float nul = 0.0f;
     int anynul = 0;
     if (ok() && f.read_col(fp, kTFloat, col, row, first, n, &nul, dst, &anynul, &st)) fail("cannot read a column");
+  }
+  void read_dbl(int col, long row, long first, long n, double *dst) {
+    double nul = 0.0;
+    int anynul = 0;
+    if (ok() && f.read_col(fp, kTDouble, col, row, first, n, &nul, dst, &anynul, &st)) fail("cannot read a column");
   }
   void read_int(int col, long row, long n, int *dst) {
     int nul = 0, anynul = 0;
@@ -262,6 +270,64 @@ std::string load_lptable_fits(LpTableHost &t, const std::string &path) {
   for (float &x : out.del_inc) x = fabsf(x);
   std::string v = validate_lptable(out);
   if (!v.empty()) return "lamp-post table " + path + ": " + v;
+  t = std::move(out);
+  return "";
+}
+
+std::string load_rradtable_fits(RradTableHost &t, const std::string &path) {
+  Fits &f = fits_lib();
+  if (!f.err.empty()) return f.err;
+  FitsFile ff(f, path);
+  RradTableHost out;
+  // SPIN: the spins (get_reltable_axis_double, src/Relreturn_Table.cpp:126-150)
+  ff.move_named("SPIN");
+  const int ca = ff.column("a");
+  const long ns = ff.rows();
+  if (ff.ok()) {
+    out.n_spin = (int)ns;
+    out.spin.resize(ns);
+    ff.read_dbl(ca, 1, 1, ns, out.spin.data());
+  }
+  // FRAC<k>: one extension per spin (:262-300, extension names :331)
+  for (long k = 0; k < ns && ff.ok(); k++) {
+    char ext[16];
+    snprintf(ext, sizeof ext, "FRAC%02ld", k + 1);
+    ff.move_named(ext);
+    const int c_rlo = ff.column("rlo"), c_rhi = ff.column("rhi"), c_tf = ff.column("tf_r");
+    const int c_gmin = ff.column("gmin"), c_gmax = ff.column("gmax"), c_fg = ff.column("frac_g");
+    const int c_fret = ff.column("f_ret");
+    const long nr = ff.rows();
+    if (!ff.ok()) break;
+    if (k == 0) {
+      out.n_rad = (int)nr;
+      out.n_g = nr > 0 ? (int)(ff.repeat(c_fg) / nr) : 0;
+      const size_t nr2 = (size_t)nr * nr;
+      out.rlo.resize(ns * nr);
+      out.rhi.resize(ns * nr);
+      out.f_ret.resize(ns * nr);
+      out.tf_r.resize(ns * nr2);
+      out.gmin.resize(ns * nr2);
+      out.gmax.resize(ns * nr2);
+      out.frac_g.resize(ns * nr2 * out.n_g);
+    }
+    if (nr != out.n_rad || ff.repeat(c_tf) != nr || ff.repeat(c_fg) != nr * out.n_g || out.n_g < 1) {
+      ff.fail(std::string(ext) + " has other dimensions");
+      break;
+    }
+    const size_t o1 = (size_t)k * nr, o2 = (size_t)k * nr * nr;
+    ff.read_dbl(c_rlo, 1, 1, nr, &out.rlo[o1]);
+    ff.read_dbl(c_rhi, 1, 1, nr, &out.rhi[o1]);
+    ff.read_dbl(c_fret, 1, 1, nr, &out.f_ret[o1]);
+    for (long i = 0; i < nr && ff.ok(); i++) {  // row i: incident zone i
+      ff.read_dbl(c_tf, i + 1, 1, nr, &out.tf_r[o2 + (size_t)i * nr]);
+      ff.read_dbl(c_gmin, i + 1, 1, nr, &out.gmin[o2 + (size_t)i * nr]);
+      ff.read_dbl(c_gmax, i + 1, 1, nr, &out.gmax[o2 + (size_t)i * nr]);
+      ff.read_dbl(c_fg, i + 1, 1, nr * out.n_g, &out.frac_g[(o2 + (size_t)i * nr) * out.n_g]);
+    }
+  }
+  if (!ff.ok()) return "returning-radiation table " + path + ": " + ff.err;
+  std::string v = validate_rradtable(out);
+  if (!v.empty()) return "returning-radiation table " + path + ": " + v;
   t = std::move(out);
   return "";
 }

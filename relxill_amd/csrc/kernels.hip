@@ -434,6 +434,80 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
   double area = 0.0;
   double cmax = 0.0;  // largest weight of a radius with a window
   int err = 0;
+  // lamp-post emissivity of a fine radius with the flux boost (src/Rellp.cpp:249-253,
+  // src/Relphysics.cpp:301-311)
+  auto lp_emis = [&](double re) {
+    LpFine f = lp_on_fine(re, s_jr, s_je, s_jd, s_ji, ntr);
+    if (f.err) err = f.err;
+    double boost = pow(gi_potential_lp(re, rq.a, rq.height, rq.beta, f.del_emit), rq.gamma);
+    if (rq.beta > 1e-6) {
+      const double d = sqrt(1.0 - rq.beta * rq.beta) / (1.0 + rq.beta * cos(f.del_emit));
+      boost *= d * d;
+    }
+    return f.emis * boost;
+  };
+  // returning radiation (add_returnrad_emis, src/Rellp.cpp:483-507, with
+  // get_rrad_emis_corona, src/Relreturn_Corona.cpp:291-321): the direct
+  // emissivity of every fine radius (parked in the radius records), at the
+  // table zones' centres (inv_rebin_mean), the zones' returning emissivity
+  // sum_re frac_g g^(Gamma-1) tf_r emis(re) (calc_rrad_emis_corona, :119-151),
+  // log-interpolated back onto the fine radii below
+  __shared__ double s_rr_rad[MAX_RR], s_rr_in[MAX_RR], s_rr_ret[MAX_RR], s_rr_corr[2];
+  const bool rrad = lp && rq.return_rad != 0 && s_st == 0;
+  const int rr_n = rq.rr_n;
+  if (rrad) {
+    for (int ii = tid; ii < nr; ii += SETUP_THREADS) rad[ii].coef = lp_emis(fine_radius(rq.rin, rq.rout, ii, nr));
+    const DevRradTable &R = b.rr;
+    const int nt = R.n_rad, k = rq.rr_k, klo = rq.rr_klo;
+    const double *rlo_t = R.rlo + (size_t)k * nt, *rhi_t = R.rhi + (size_t)k * nt;
+    const double rin_f = fine_radius(rq.rin, rq.rout, nr - 1, nr), rout_f = fine_radius(rq.rin, rq.rout, 0, nr);
+    for (int i = tid; i < rr_n; i += SETUP_THREADS) {
+      const double lo = (i == 0) ? rin_f : rlo_t[klo + i];
+      const double hi = (i == rr_n - 1) ? rout_f : rhi_t[klo + i];
+      s_rr_rad[i] = 0.5 * (lo + hi);
+      s_rr_ret[i] = 0.0;
+    }
+    if (tid < 2) {  // ring-area corrections of the end zones (src/Relreturn_Table.cpp:415-435)
+      const int i = (tid == 0) ? 0 : rr_n - 1;
+      double rl = rlo_t[klo + i];
+      if (klo + i == 0 && rl > rq.rms) rl = rq.rms;
+      const double rh = rhi_t[klo + i];
+      const double lo = (i == 0) ? rin_f : rlo_t[klo + i];
+      const double hi = (i == rr_n - 1) ? rout_f : rhi_t[klo + i];
+      s_rr_corr[tid] = (0.5 * (lo + hi) * (hi - lo)) / (0.5 * (rl + rh) * (rh - rl));
+    }
+    __syncthreads();
+    for (int i = tid; i < rr_n; i += SETUP_THREADS) {
+      // inv_rebin_mean (src/relutility.c:636-663): the fine radii re(ii) > rad >= re(ii+1)
+      const double x = s_rr_rad[i];
+      int lo = 0, hi = nr - 1;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (fine_radius(rq.rin, rq.rout, mid, nr) > x) lo = mid; else hi = mid;
+      }
+      const double r0 = fine_radius(rq.rin, rq.rout, lo, nr), r1 = fine_radius(rq.rin, rq.rout, lo + 1, nr);
+      s_rr_in[i] = lerp1((x - r1) / (r0 - r1), rad[lo + 1].coef, rad[lo].coef);
+    }
+    __syncthreads();
+    const int ng = R.n_g;
+    for (int q = tid; q < rr_n * rr_n; q += SETUP_THREADS) {
+      const int i = q / rr_n, j = q % rr_n;  // incident zone i, emitting zone j
+      const size_t t = (size_t)k * nt * nt + (size_t)(klo + i) * nt + (klo + j);
+      const double gmin = R.gmin[t], gmax = R.gmax[t];
+      double ez = 0.0;
+      for (int jj = 0; jj < ng; jj++) {
+        const double g = (jj + 0.5) / ng * (gmax - gmin) + gmin;  // get_gfac_grid (src/Relreturn_Datastruct.cpp:51-59)
+        double v = R.frac_g[t * ng + jj];
+        if (fabs(g - 1) > 1e-3) v *= pow(g, rq.gamma - 1);
+        ez += v;
+      }
+      double tf = R.tf_r[t];
+      if (j == 0 && i < rr_n - 1) tf *= s_rr_corr[0];
+      if (j == rr_n - 1 && i < rr_n - 2) tf *= s_rr_corr[1];
+      atomicAdd(&s_rr_ret[i], ez * tf * s_rr_in[j]);
+    }
+    __syncthreads();
+  }
   for (int ii0 = 0; ii0 < nr; ii0 += SETUP_THREADS) {  // same trip count in every wavefront
     const int ii = ii0 + tid;
     int wlo = 0x7fffffff, whi = -1;
@@ -469,15 +543,15 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
       const double alpha = (re > rq.rbr) ? rq.emis2 : rq.emis1;
       emis = pow(re / rq.rbr, -alpha);
       area += emis * (tz * 2);
-    } else {  // lamp post + flux boost (src/Rellp.cpp:249-253, src/Relphysics.cpp:301-311)
-      LpFine f = lp_on_fine(re, s_jr, s_je, s_jd, s_ji, ntr);
-      if (f.err) err = f.err;
-      double boost = pow(gi_potential_lp(re, rq.a, rq.height, rq.beta, f.del_emit), rq.gamma);
-      if (rq.beta > 1e-6) {
-        const double d = sqrt(1.0 - rq.beta * rq.beta) / (1.0 + rq.beta * cos(f.del_emit));
-        boost *= d * d;
-      }
-      emis = f.emis * boost;
+    } else if (!rrad) {
+      emis = lp_emis(re);
+    } else {
+      // the returning emissivity at this radius (rebin_emisprofile_on_radial_grid,
+      // src/Rellp.cpp:120-175: linear radius factor, log-interpolated values)
+      const int kk = min(max(bsearch_asc(s_rr_rad, rr_n, re), 0), rr_n - 2);
+      const double fr = (re - s_rr_rad[kk]) / (s_rr_rad[kk + 1] - s_rr_rad[kk]);
+      const double er = exp(fr * log(s_rr_ret[kk + 1]) + (1.0 - fr) * log(s_rr_ret[kk]));
+      emis = (rq.return_rad == 1) ? rad[ii].coef + er : er;
     }
     // energy window
     double egmin = gmin, egmax = gmax;

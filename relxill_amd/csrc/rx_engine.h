@@ -78,6 +78,9 @@ struct ReqParams {
   double xect_src;    // Ecut in the source frame
   int64_t dist_off;
   int64_t zone_off;   // ZoneRec per zone
+  // returning radiation (lamp post): 1 add, -1 or 2 replace the direct
+  // emissivity; the table spin, its first zone and the number of zones used
+  int return_rad, rr_k, rr_klo, rr_n;
 };
 
 // Per zone of a relxill request (IonGradient, src/IonGradient.cpp:182-209):
@@ -123,6 +126,13 @@ struct LpRefl {
 struct DevRelTable {
   int n_a, n_mu0, n_r, n_g;
   const float *a, *mu0, *r, *gmin, *gmax, *trff1, *trff2, *cosne1, *cosne2;
+};
+
+// Returning-radiation table resident on the device (flat, spin-major).
+constexpr int MAX_RR = 64;  // table zones a request may use
+struct DevRradTable {
+  int n_spin, n_rad, n_g;
+  const double *rlo, *rhi, *tf_r, *gmin, *gmax, *frac_g;
 };
 
 struct DevLpTable {
@@ -249,6 +259,7 @@ struct BatchDev {
   DevXillTable xt;
   double *xspec;
   ZoneRec *zone;
+  DevRradTable rr;
 };
 
 // ---- launchers (kernels.hip) ----

@@ -85,6 +85,27 @@ struct XillTableHost {
   }
 };
 
+// Returning-radiation table (returnTable / tabulatedReturnFractions,
+// src/Relreturn_Table.h; table_returnRad_v20220301.fits): per spin (ascending)
+// n_rad radial zones [rlo, rhi] (ascending, the first starting at the spin's
+// ISCO) and, for every pair (incident zone ro, emitting zone re), the fraction
+// tf_r[ro][re] of the radiation emitted at re that returns to ro, the g range
+// [gmin, gmax][ro][re] and the distribution frac_g[ro][re][n_g] of the energy
+// shift over n_g bins of that range.  Flat, spin-major arrays.
+struct RradTableHost {
+  int n_spin = 0, n_rad = 0, n_g = 0;
+  std::vector<double> spin;
+  std::vector<double> rlo, rhi;                  // [spin][rad]
+  std::vector<double> tf_r, gmin, gmax;          // [spin][ro][re]
+  std::vector<double> frac_g;                    // [spin][ro][re][g]
+  std::vector<double> f_ret;                     // [spin][rad]
+};
+RradTableHost make_synthetic_rradtable(uint64_t seed, int n_spin, int n_rad, int n_g);
+std::string validate_rradtable(const RradTableHost &t);
+// layout of src/Relreturn_Table.cpp:126-300: extension SPIN (column a), then
+// FRAC01, FRAC02, ... (one per spin) with rlo, rhi, tf_r, gmin, gmax, frac_g, f_ret
+std::string load_rradtable_fits(RradTableHost &t, const std::string &path);
+
 // Seeded synthetic table with the parameters Gamma, A_Fe, logXi, Ecut, Incl
 // (in that order) and n_ener log bins over 0.07-1000 keV.
 XillTableHost make_synthetic_xilltable(uint64_t seed, int n_gam, int n_afe, int n_lxi, int n_ect,

@@ -401,6 +401,86 @@ std::vector<float> xilltable_renormed(const XillTableHost &t) {
   return out;
 }
 
+// ---- returning-radiation table ----
+
+// Synthetic returning fractions: n_rad zones log-spaced from the spin's ISCO to
+// 1000 rg; the energy shift between two radii centres on the ratio of the disk
+// u^t, with a width that grows towards small radii; a smooth, seeded
+// distribution over the g bins (sums to 1); the returning fraction falls with
+// the distance between the zones and with radius.
+RradTableHost make_synthetic_rradtable(uint64_t seed, int n_spin, int n_rad, int n_g) {
+  RradTableHost t;
+  t.n_spin = n_spin;
+  t.n_rad = n_rad;
+  t.n_g = n_g;
+  t.spin.resize(n_spin);
+  for (int k = 0; k < n_spin; k++) t.spin[k] = -0.998 + 1.996 * k / (n_spin - 1);
+  const size_t nr2 = (size_t)n_rad * n_rad;
+  t.rlo.resize((size_t)n_spin * n_rad);
+  t.rhi.resize((size_t)n_spin * n_rad);
+  t.f_ret.resize((size_t)n_spin * n_rad);
+  t.tf_r.resize(n_spin * nr2);
+  t.gmin.resize(n_spin * nr2);
+  t.gmax.resize(n_spin * nr2);
+  t.frac_g.resize(n_spin * nr2 * n_g);
+  SplitMix rng(seed);
+  Smooth s_tf(rng), s_g(rng);
+  for (int k = 0; k < n_spin; k++) {
+    const double a = t.spin[k];
+    const double r0 = kerr_rms(a);
+    std::vector<double> rm(n_rad);
+    for (int i = 0; i < n_rad; i++) {
+      t.rlo[(size_t)k * n_rad + i] = r0 * std::pow(1000.0 / r0, (double)i / n_rad);
+      t.rhi[(size_t)k * n_rad + i] = r0 * std::pow(1000.0 / r0, (double)(i + 1) / n_rad);
+      rm[i] = 0.5 * (t.rlo[(size_t)k * n_rad + i] + t.rhi[(size_t)k * n_rad + i]);
+    }
+    for (int io = 0; io < n_rad; io++) {
+      double fret = 0.0;
+      for (int ie = 0; ie < n_rad; ie++) {
+        const size_t q = (size_t)k * nr2 + (size_t)io * n_rad + ie;
+        const double d = std::log(rm[io] / rm[ie]);
+        const double tf = 0.08 * std::exp(-0.8 * std::fabs(d)) * std::pow(rm[ie] / r0, -0.7) *
+                          (1.0 + 0.2 * s_tf((double)io / n_rad, (double)ie / n_rad, (a + 1) / 2));
+        t.tf_r[q] = tf;
+        fret += tf;
+        const double g0 = ut_disk(rm[io], a) / ut_disk(rm[ie], a);
+        const double w = 0.1 + 0.5 / std::sqrt(rm[ie]);
+        t.gmin[q] = g0 * (1.0 - w);
+        t.gmax[q] = g0 * (1.0 + w);
+        double sum = 0.0;
+        for (int g = 0; g < n_g; g++) {
+          const double x = (g + 0.5) / n_g;
+          const double v = (0.3 + std::sqrt(x * (1.0 - x))) * (1.0 + 0.3 * s_g(x, (double)ie / n_rad, (double)io / n_rad));
+          t.frac_g[q * n_g + g] = v;
+          sum += v;
+        }
+        for (int g = 0; g < n_g; g++) t.frac_g[q * n_g + g] /= sum;
+      }
+      t.f_ret[(size_t)k * n_rad + io] = fret;
+    }
+  }
+  return t;
+}
+
+std::string validate_rradtable(const RradTableHost &t) {
+  if (t.n_spin < 1 || t.n_rad < 2 || t.n_g < 1) return "returning-radiation table dims too small";
+  const size_t nr2 = (size_t)t.n_rad * t.n_rad;
+  if (t.spin.size() != (size_t)t.n_spin || t.rlo.size() != (size_t)t.n_spin * t.n_rad ||
+      t.tf_r.size() != t.n_spin * nr2 || t.frac_g.size() != t.n_spin * nr2 * t.n_g)
+    return "returning-radiation table size mismatch";
+  for (int k = 0; k + 1 < t.n_spin; k++)
+    if (!(t.spin[k] < t.spin[k + 1])) return "returning-radiation spins not ascending";
+  for (int k = 0; k < t.n_spin; k++)
+    for (int i = 0; i < t.n_rad; i++) {
+      const size_t o = (size_t)k * t.n_rad + i;
+      if (!(t.rhi[o] > t.rlo[o]) || (i + 1 < t.n_rad && !(t.rlo[o + 1] >= t.rlo[o])))
+        return "returning-radiation radii not ascending";
+    }
+  for (size_t q = 0; q < t.tf_r.size(); q++)
+    if (!(t.tf_r[q] >= 0) || !(t.gmax[q] > t.gmin[q]) || !(t.gmin[q] > 0)) return "returning fractions invalid";
+  return "";
+}
+
 // ---- RXTB binary format: magic, kind, dims, then the float arrays ----
 
 namespace {

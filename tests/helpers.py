@@ -22,12 +22,14 @@ _TABLES = {}
 
 
 def tables():
-    """(rx tables view, oracle.Tables) of the resident default tables."""
+    """(rx tables view, oracle.Tables) of the resident default tables (the
+    oracle's with the default returning-radiation table too)."""
     if "t" not in _TABLES:
         rx.tables_synthetic(1, 25, 22, 100, 20)
         rx.lptable_synthetic(3, 20, 100, 100)
+        rx.rradtable_synthetic(7, 21, 50, 20)
         rel, lp = rx.tables_view()
-        _TABLES["t"] = (rel, lp, oracle.Tables(rel, lp))
+        _TABLES["t"] = (rel, lp, oracle.Tables(rel, lp, rx.rradtable_view()))
     return _TABLES["t"]
 
 
@@ -311,7 +313,8 @@ def _fits_header(cards):
 
 def _fits_bintable(name, cols):
     """cols: list of (column name, array of shape (rows, repeat)[, tform]) with
-    tform "E" (float32, the default), "J" (int32) or "<w>A" (strings of width w)."""
+    tform "E" (float32, the default), "D" (float64), "J" (int32) or "<w>A"
+    (strings of width w)."""
     prepared = []
     for c in cols:
         cname, v = c[0], c[1]
@@ -321,9 +324,9 @@ def _fits_bintable(name, cols):
             data = np.array([s.ljust(w)[:w].encode("ascii") for s in v], dtype=f"S{w}").reshape(len(v), 1)
             prepared.append((cname, form, data, w))
         else:
-            dt = ">f4" if form == "E" else ">i4"
+            dt = {"E": ">f4", "J": ">i4", "D": ">f8"}[form]
             a = np.asarray(v, dtype=dt).reshape(len(v), -1)
-            prepared.append((cname, f"{a.shape[1]}{form}", a, 4 * a.shape[1]))
+            prepared.append((cname, f"{a.shape[1]}{form}", a, a.itemsize * a.shape[1]))
     nrows = prepared[0][2].shape[0]
     assert all(d.shape[0] == nrows for _, _, d, _ in prepared)
     rowbytes = sum(w for _, _, _, w in prepared)
@@ -475,3 +478,56 @@ def draw_relxill_params(rng):
             rng.choice([-1.0, rng.uniform(1.0, 30.0)]), rng.uniform(50, 1000), rng.choice([0.0, rng.uniform(0, 2)]),
             rng.uniform(1.0, 3.4), rng.uniform(0.0, 4.7), rng.uniform(0.5, 10.0), rng.uniform(5.0, 1000.0),
             rng.choice([-1.0, rng.uniform(-5, 10)])]
+
+
+def rrad_table(n_spin=3, n_rad=12, n_g=6, spins=(-0.5, 0.5, 0.998), c=0.05, shift=0, g0=1.0, spin_scale=None):
+    """A returning-radiation table: zones log-spaced from each spin's ISCO to
+    1000 rg; radiation emitted in zone j returns to zone j + shift with the
+    fraction c (times spin_scale[k] for spin k); all of it in one energy-shift
+    bin at g = g0 (the middle of n_g bins, n_g even: g0 is the centre of bin
+    n_g/2 when gmin, gmax are placed so)."""
+    spins = np.asarray(spins[:n_spin], dtype=np.float64)
+    nr2 = n_rad * n_rad
+    rlo = np.zeros((n_spin, n_rad))
+    rhi = np.zeros((n_spin, n_rad))
+    tf = np.zeros((n_spin, n_rad, n_rad))
+    gmin = np.zeros((n_spin, n_rad, n_rad))
+    gmax = np.zeros((n_spin, n_rad, n_rad))
+    fg = np.zeros((n_spin, n_rad, n_rad, n_g))
+    for k, a in enumerate(spins):
+        r0 = oracle.lib.orc_kerr_rms(float(a))
+        edges = r0 * (1000.0 / r0) ** (np.arange(n_rad + 1) / n_rad)
+        rlo[k], rhi[k] = edges[:-1], edges[1:]
+        for j in range(n_rad):
+            i = j + shift
+            if 0 <= i < n_rad:
+                tf[k, i, j] = c * (spin_scale[k] if spin_scale is not None else 1.0)
+        # bin m = n_g // 2 has its centre at g0: (m + 0.5)/n_g (gmax - gmin) + gmin = g0
+        m = n_g // 2
+        width = 0.4
+        gmin[k] = g0 - (m + 0.5) / n_g * width
+        gmax[k] = gmin[k] + width
+        fg[k, :, :, m] = 1.0
+    return dict(n_spin=n_spin, n_rad=n_rad, n_g=n_g, spin=spins, rlo=rlo.ravel(), rhi=rhi.ravel(),
+                tf_r=tf.ravel(), gmin=gmin.ravel(), gmax=gmax.ravel(), frac_g=fg.ravel())
+
+
+def write_fits_rradtable(path, t):
+    """The returning-radiation table as table_returnRad_v20220301.fits lays it
+    out: extension SPIN (column a), then FRAC01, ... with rlo, rhi, frac_e, tf_r,
+    gmin, gmax, frac_g, f_ret (double columns; row = incident zone)."""
+    ns, nr, ng = t["n_spin"], t["n_rad"], t["n_g"]
+    out = [_fits_primary(), _fits_bintable("SPIN", [("a", np.asarray(t["spin"]), "D")])]
+    rlo = np.asarray(t["rlo"]).reshape(ns, nr)
+    rhi = np.asarray(t["rhi"]).reshape(ns, nr)
+    tf = np.asarray(t["tf_r"]).reshape(ns, nr, nr)
+    gl = np.asarray(t["gmin"]).reshape(ns, nr, nr)
+    gh = np.asarray(t["gmax"]).reshape(ns, nr, nr)
+    fg = np.asarray(t["frac_g"]).reshape(ns, nr, nr * ng)
+    for k in range(ns):
+        out.append(_fits_bintable(f"FRAC{k + 1:02d}", [
+            ("rlo", rlo[k], "D"), ("rhi", rhi[k], "D"), ("frac_e", np.zeros((nr, nr)), "D"), ("tf_r", tf[k], "D"),
+            ("gmin", gl[k], "D"), ("gmax", gh[k], "D"), ("frac_g", fg[k], "D"), ("f_ret", tf[k].sum(axis=1), "D")]))
+    with open(path, "wb") as f:
+        for blk in out:
+            f.write(blk)

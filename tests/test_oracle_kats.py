@@ -266,7 +266,7 @@ def test_parameter_clamps():
     p, st = oracle.rel_params(oracle.MOD_RELCONVLP, [-3.0, 1.5, 0.9, 30, -1, 400, 0, 2, 0])
     assert st == 0 and p.beta == 0.99 and abs(p.height - 3 * oracle.lib.orc_kerr_rplus(0.9)) < 1e-12
     p, st = oracle.rel_params(oracle.MOD_RELLINELP, [6.4, 6.0, 0.9, 30, -1, 400, 0, 0, 2, 1])
-    assert st == 2  # returning radiation is out of scope
+    assert st == 0 and p.return_rad == 1
 
 
 def test_lp_closed_forms():
