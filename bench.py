@@ -146,7 +146,16 @@ def relxilllp_batch(rx, steps, n_req=1024, seed=10):
     o32 = [b.output(i)[0] for i in range(n_req)]
     xv = rx.xilltable_view()
     n_e, n_incl = xv["n_ener"], xv["n_val"][-1]
-    n_angles = sum(int(np.count_nonzero(b.dist(i, 10 * n_incl))) for i in range(n_req))
+    n_angles = 0
+    n_rows = 0
+    for i in range(n_req):
+        d = b.dist(i, 10 * n_incl).reshape(10, n_incl)
+        ect = b.zones(i, 10)[:, 4].astype(int)
+        n_angles += int(np.count_nonzero(d))
+        # distinct (Ecut node, angle) pairs over the zones: zones whose shifted
+        # Ecut falls between the same table nodes need the same rows
+        n_rows += len({(e + k, a) for z in range(10) for a in np.flatnonzero(d[z]) for e in (ect[z],)
+                       for k in (0, 1)})
     dt64, _ = timed_steps(rx, b, rx.FP64, 2, 1)
     o64 = [b.output(i)[0] for i in range(n_req)]
     b.close()
@@ -156,11 +165,14 @@ def relxilllp_batch(rx, steps, n_req=1024, seed=10):
         d, _ = timed_steps(rx, pb, rx.FP64, 3, 1)
         lat.append(d)
         pb.close()
-    # k_xill algorithmic bytes: per zone and angle of nonzero weight 16 corner
-    # rows of n_e float bins; every zone's FP64 spectrum written
-    alg = 16 * n_angles * n_e * 4 + n_req * 10 * 8 * n_e
+    # k_xill algorithmic bytes: per request the distinct table rows its zones
+    # need (8 Gamma / A_Fe / logXi corners times the distinct (Ecut node, angle)
+    # pairs, n_e float bins each: zones sharing rows read them once from HBM,
+    # the repeats come from L2); every zone's FP64 spectrum written
+    alg = 8 * n_rows * n_e * 4 + n_req * 10 * 8 * n_e
     t_x = kms32[5] * 1e-3
     return dict(n_requests=n_req, zones=10, mean_angles_used_per_zone=n_angles / n_req / 10,
+                rows_read_per_zone_angle=16.0, distinct_rows_per_request=8 * n_rows / n_req,
                 k_xill_roofline=dict(bound="hbm", algorithmic_bytes=alg, achieved=alg / t_x / 1e9, peak=HBM_PEAK_GBS,
                                      unit="GB/s", frac=alg / t_x / 1e9 / HBM_PEAK_GBS),
                 batched_fp32_spectra_per_s=n_req / dt32,

@@ -127,6 +127,7 @@ _EXPORTS = {
     "rx_batch_status": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "rx_batch_dist": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_int]),
     "rx_batch_reflfrac": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D]),
+    "rx_batch_zones": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_int]),
     "rx_batch_set_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "rx_reflsets_clear": (ctypes.c_int, []),
     "rx_batch_destroy": (None, [ctypes.c_void_p]),
@@ -356,6 +357,12 @@ class Batch:
         d = np.zeros(n_incl)
         _check(lib.rx_batch_dist(self.h, i, _dp(d), n_incl))
         return d
+
+    def zones(self, i, n_zones):
+        """(n_zones, 6): rmean, del_emit, shift, ncf, Ecut index, Ecut factor"""
+        o = np.zeros(6 * n_zones)
+        _check(lib.rx_batch_zones(self.h, i, _dp(o), n_zones))
+        return o.reshape(n_zones, 6)
 
     def reflfrac(self, i):
         o = np.zeros(5)

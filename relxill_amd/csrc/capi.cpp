@@ -498,6 +498,28 @@ int rx_batch_dist(rx_batch *b, int i, double *dist, int n_incl) {
   return RX_OK;
 }
 
+int rx_batch_zones(rx_batch *b, int i, double *out, int n_zones) {
+  if (!b || !out || i < 0 || i >= b->b.n_req || n_zones < 0) return RX_ERR_ARG;
+  const ReqParams &p = b->b.h_req[i];
+  if (p.model != MOD_RELXILL && p.model != MOD_RELXILLLP) return RX_ERR_ARG;
+  if (n_zones > p.n_zones) return RX_ERR_ARG;
+  if (n_zones == 0) return RX_OK;
+  std::vector<ZoneRec> z(n_zones);
+  if (hipMemcpy(z.data(), b->b.d_zone.as<ZoneRec>() + p.zone_off, sizeof(ZoneRec) * n_zones,
+                hipMemcpyDeviceToHost) != hipSuccess)
+    return RX_ERR_DEVICE;
+  for (int k = 0; k < n_zones; k++) {
+    double *o = out + 6 * k;
+    o[0] = z[k].rmean;
+    o[1] = z[k].del_emit;
+    o[2] = z[k].shift;
+    o[3] = z[k].ncf;
+    o[4] = z[k].ect_ind;
+    o[5] = z[k].ect_fac;
+  }
+  return RX_OK;
+}
+
 int rx_batch_reflfrac(rx_batch *b, int i, double *out5) {
   if (!b || !out5 || i < 0 || i >= b->b.n_req) return RX_ERR_ARG;
   LpRefl r;

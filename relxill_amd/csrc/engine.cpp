@@ -666,6 +666,7 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
   b.h_conv.clear();
   b.h_xill.clear();
   b.h_prim.clear();
+  std::vector<XillWork> xill_multi;
   for (int i = 0; i < n_req; i++) {
     if (is_relxill(req[i].model)) {
       int st = ensure_xilltable_device(err);
@@ -726,7 +727,7 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
         o_xspec += (int64_t)g.xill.n_ener * p.n_zones;
         p.zone_off = o_zone;
         o_zone += p.n_zones;
-        for (int z = 0; z < p.n_zones; z++) b.h_xill.push_back(XillWork{i, z});
+        (p.n_zones > 1 ? xill_multi : b.h_xill).push_back(XillWork{i, 0});
         b.h_prim.push_back(i);
       } else if (st == RX_OK && r.model != MOD_RELXILL_CONV) {
         if (!r.flux_in) {
@@ -763,6 +764,8 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
   b.tot_xspec = o_xspec;
   b.tot_dist = o_dist;
   b.tot_zone = o_zone;
+  b.n_xill1 = (int)b.h_xill.size();
+  b.h_xill.insert(b.h_xill.end(), xill_multi.begin(), xill_multi.end());
   b.h_ener.resize(o_ener);
   b.h_flux_in.assign(o_flux, 0.0);
   for (int i = 0; i < n_req; i++) {
@@ -925,7 +928,7 @@ int batch_run(Batch &b, int precision, std::string &err) {
   if (prof) e |= (int)hipEventRecord(b.ev[3], s);
   e |= launch_finalize(bd, s);
   if (prof) e |= (int)hipEventRecord(b.ev[4], s);
-  e |= launch_xill(bd, b.d_xill.as<XillWork>(), (int)b.h_xill.size(), s);
+  e |= launch_xill(bd, b.d_xill.as<XillWork>(), b.n_xill1, (int)b.h_xill.size() - b.n_xill1, s);
   if (prof) e |= (int)hipEventRecord(b.ev[5], s);
   e |= launch_conv(bd, b.d_conv.as<XillWork>(), (int)b.h_conv.size(), s);
   if (prof) e |= (int)hipEventRecord(b.ev[6], s);

@@ -43,7 +43,9 @@ struct Batch {
   std::vector<int> h_status;  // host-side parse status
   std::vector<double> h_ener, h_flux_in;
   std::vector<int> h_prim;
-  std::vector<XillWork> h_conv, h_xill;  // (request, zone) units of k_conv and k_xill
+  std::vector<XillWork> h_conv;  // (request, zone) units of k_conv
+  std::vector<XillWork> h_xill;  // k_xill units: one-zone requests first, then the others (all zones in one unit)
+  int n_xill1 = 0;               // number of one-zone units at the front of h_xill
   int64_t tot_ener = 0, tot_flux = 0, tot_pgrid = 0, tot_prof = 0, tot_xspec = 0, tot_dist = 0, tot_zone = 0;
   DevBuf d_req, d_ener, d_flux_in, d_out, d_pgrid, d_node, d_rad, d_prof, d_dist, d_refl, d_status, d_conv;
   DevBuf d_xill, d_xspec, d_zone, d_prim;

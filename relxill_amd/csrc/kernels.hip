@@ -32,7 +32,8 @@ namespace {
 constexpr int SETUP_THREADS = 256;
 constexpr int CONV_THREADS = 256;      // smallest convolution workgroup
 constexpr int XILL_THREADS = 256;
-constexpr int XILL_BINS = 4;  // table bins per k_xill thread
+constexpr int XILL_BINS = 4;    // table bins per k_xill thread, one-zone requests
+constexpr int XILL_BINS_Z = 2;  // the same for requests of MAX_ZONES zones (ten accumulators per bin)
 constexpr int PRIM_THREADS = 256;
 constexpr int CONV_THREADS_MAX = 1024;  // n/8 threads: 512 for 4096 bins, 1024 for 8192
 
@@ -1713,12 +1714,11 @@ __global__ __launch_bounds__(PRIM_THREADS) void k_primary(BatchDev b, const int 
 // emission angle (interp_5d_tab_incl, src/xilltable.c:812-876: 16 corners of
 // the four interpolated parameters, in the reference's order and with its
 // weight products) weighted by the angle distribution (calc_xillver_angdep,
-// src/Xillspec.cpp:557-573).  One thread per bin of the table grid; every
-// table read is coalesced across the block.  Angles with zero weight are
-// skipped (they add exactly 0).  One unit per (request, zone): the zone's Ecut
-// bracket, its angle distribution and its normalisation change
-// (src/Relxill.cpp:357-388).
-__global__ __launch_bounds__(XILL_THREADS, 4) void k_xill(BatchDev b, const XillWork *__restrict__ work, int chunks) {
+// src/Xillspec.cpp:557-573), for one-zone requests.  XILL_BINS bins per
+// thread; every table read is coalesced across the block.  Angles with zero
+// weight are skipped (they add exactly 0).  The zone's Ecut bracket, angle
+// distribution and normalisation change come from its zone record.
+__global__ __launch_bounds__(XILL_THREADS, 4) void k_xill1(BatchDev b, const XillWork *__restrict__ work, int chunks) {
   const XillWork wk = work[blockIdx.x / chunks];
   const int req = wk.req;
   if (b.status[req] != 0) return;
@@ -1788,6 +1788,154 @@ __global__ __launch_bounds__(XILL_THREADS, 4) void k_xill(BatchDev b, const Xill
 #pragma unroll
   for (int u = 0; u < XILL_BINS; u++)
     if (in[u]) dst[eb[u]] = acc[u] / zr.ncf;
+}
+
+// relxilllp: the same for the zones of a lamp-post request (src/Relxill.cpp:
+// 357-388), which differ only in the Ecut they see and in their angle
+// distributions.  One unit per (request, slice of XILL_THREADS * XILL_BINS_Z
+// table bins) for all zones, so that a table row is read once however many
+// zones use it:
+//   zone z = sum_a d_z[a] sum_c w_c row_c(a)
+//          = sum_{(e, a)} k_z(e, a) sum_{c8} w_c8 row_{c8, e}(a),
+//   k_z(e, a) = d_z[a] (1 - f_z) for e = i_z, d_z[a] f_z for e = i_z + 1, else 0,
+// with (i_z, f_z) the zone's bracket on the Ecut axis and c8 the eight
+// corners of the other three axes (their weights as the products of the
+// reference's factors).  The block first lists the pairs (e, a) that some
+// zone weights, with every zone's k_z, in LDS (in windows of XILL_THREADS
+// candidates, in a fixed order), then streams the 8 rows of each listed pair.
+// Tables without an Ecut axis use the request's bracket on the fourth axis
+// for every zone.
+template <int NZ>
+__global__ __launch_bounds__(XILL_THREADS, 3) void k_xillz(BatchDev b, const XillWork *__restrict__ work, int chunks) {
+  constexpr int NW = XILL_THREADS / 64;
+  __shared__ int s_ind[NZ];
+  __shared__ double s_fac[NZ];
+  __shared__ unsigned s_off[XILL_THREADS];
+  __shared__ double s_k[XILL_THREADS][NZ];
+  __shared__ int s_wn[NW];
+  const XillWork wk = work[blockIdx.x / chunks];
+  const int req = wk.req;
+  if (b.status[req] != 0) return;  // uniform over the block
+  const DevXillTable xt = b.xt;
+  const ReqParams &rq = b.req[req];
+  const int nz = min(rq.n_zones, NZ);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int zax = xt.ect_dim >= 0 ? xt.ect_dim : 3;
+  if (tid < nz) {
+    const ZoneRec &zr = b.zone[rq.zone_off + tid];
+    s_ind[tid] = xt.ect_dim >= 0 ? zr.ect_ind : rq.xind[3];
+    s_fac[tid] = xt.ect_dim >= 0 ? zr.ect_fac : rq.xfac[3];
+  }
+  __syncthreads();
+  int elo = s_ind[0], ehi = s_ind[0];
+  for (int z = 1; z < nz; z++) {
+    elo = min(elo, s_ind[z]);
+    ehi = max(ehi, s_ind[z]);
+  }
+  const int ni = xt.n_incl;
+  const int n_cand = (ehi - elo + 2) * ni;  // candidate pairs (e, a), e in [elo, ehi + 1]
+  // corner rows of the axes other than zax (zax index 0), in rows of ni
+  // inclinations; one step on zax moves a row by `stride` rows
+  double w8[8];
+  unsigned cb[8];
+  unsigned stride = 1;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (k > zax) stride *= (unsigned)xt.n_val[k];
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    double w = 1.0;
+    unsigned r = 0;
+    int bit = 2;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      int idx = 0;
+      if (k != zax) {
+        const int o = (c >> bit) & 1;
+        bit--;
+        w *= o ? rq.xfac[k] : (1.0 - rq.xfac[k]);
+        idx = rq.xind[k] + o;
+      }
+      r = r * (unsigned)xt.n_val[k] + (unsigned)idx;
+    }
+    w8[c] = w;
+    cb[c] = r * (unsigned)ni;
+  }
+  const double *dist = b.dist + rq.dist_off;
+  const size_t ne = (size_t)xt.n_ener;
+  const int e0 = (blockIdx.x % chunks) * (XILL_THREADS * XILL_BINS_Z) + tid;
+  int eb[XILL_BINS_Z];
+  bool in[XILL_BINS_Z];
+#pragma unroll
+  for (int u = 0; u < XILL_BINS_Z; u++) {
+    eb[u] = e0 + u * XILL_THREADS;
+    in[u] = eb[u] < xt.n_ener;
+    if (!in[u]) eb[u] = 0;  // a valid address; the result is not stored
+  }
+  double acc[NZ][XILL_BINS_Z];
+#pragma unroll
+  for (int z = 0; z < NZ; z++)
+#pragma unroll
+    for (int u = 0; u < XILL_BINS_Z; u++) acc[z][u] = 0.0;
+  for (int q0 = 0; q0 < n_cand; q0 += XILL_THREADS) {
+    // list this window's used pairs: thread tid tests candidate q0 + tid
+    const int q = q0 + tid;
+    const int e = elo + q / ni, a = q % ni;
+    auto kz = [&](int z) {  // k_z(e, a); zones beyond nz weigh nothing
+      if (z >= nz) return 0.0;
+      const double d = dist[z * ni + a];
+      const int iz = s_ind[z];
+      return e == iz ? d * (1.0 - s_fac[z]) : (e == iz + 1 ? d * s_fac[z] : 0.0);
+    };
+    bool used = false;
+    if (q < n_cand)
+      for (int z = 0; z < nz; z++) used |= kz(z) != 0.0;
+    const unsigned long long m = __ballot(used);
+    if (lane == 0) s_wn[wv] = __popcll(m);
+    __syncthreads();
+    int slot = __popcll(m & ((1ull << lane) - 1ull)), n_used = 0;
+    for (int w = 0; w < NW; w++) {
+      if (w < wv) slot += s_wn[w];
+      n_used += s_wn[w];
+    }
+    if (used) {
+      s_off[slot] = (unsigned)e * stride * (unsigned)ni + (unsigned)a;
+      for (int z = 0; z < NZ; z++) s_k[slot][z] = kz(z);
+    }
+    __syncthreads();
+    if (e0 < xt.n_ener) {
+      for (int p = 0; p < n_used; p++) {
+        const unsigned po = s_off[p];
+        double s8[XILL_BINS_Z];
+#pragma unroll
+        for (int u = 0; u < XILL_BINS_Z; u++) s8[u] = 0.0;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+          const float *src = xt.spec + (size_t)(cb[c] + po) * ne;
+#pragma unroll
+          for (int u = 0; u < XILL_BINS_Z; u++) s8[u] += w8[c] * (double)src[eb[u]];
+        }
+#pragma unroll
+        for (int z = 0; z < NZ; z++) {
+          const double k = s_k[p][z];
+#pragma unroll
+          for (int u = 0; u < XILL_BINS_Z; u++) acc[z][u] += k * s8[u];
+        }
+      }
+    }
+    __syncthreads();  // the list is rewritten by the next window
+  }
+  if (e0 >= xt.n_ener) return;
+#pragma unroll
+  for (int z = 0; z < NZ; z++) {
+    if (z < nz) {
+      const double ncf = b.zone[rq.zone_off + z].ncf;
+      double *dst = b.xspec + rq.xspec_off + (size_t)z * ne;
+#pragma unroll
+      for (int u = 0; u < XILL_BINS_Z; u++)
+        if (in[u]) dst[eb[u]] = acc[z][u] / ncf;
+    }
+  }
 }
 
 // Resident reflection set: per angle, rebin onto the convolution grid, band
@@ -1893,10 +2041,17 @@ int launch_conv(const BatchDev &b, const XillWork *conv_work, int n_conv_work, v
   return (int)hipGetLastError();
 }
 
-int launch_xill(const BatchDev &b, const XillWork *work, int n_work, void *stream) {
-  if (n_work <= 0) return 0;
-  const int chunks = (b.xt.n_ener + XILL_THREADS * XILL_BINS - 1) / (XILL_THREADS * XILL_BINS);
-  hipLaunchKernelGGL(k_xill, dim3(n_work * chunks), dim3(XILL_THREADS), 0, (hipStream_t)stream, b, work, chunks);
+int launch_xill(const BatchDev &b, const XillWork *work, int n_single, int n_multi, void *stream) {
+  if (n_single > 0) {
+    const int chunks = (b.xt.n_ener + XILL_THREADS * XILL_BINS - 1) / (XILL_THREADS * XILL_BINS);
+    hipLaunchKernelGGL(k_xill1, dim3(n_single * chunks), dim3(XILL_THREADS), 0, (hipStream_t)stream, b,
+                       work, chunks);
+  }
+  if (n_multi > 0) {
+    const int chunks = (b.xt.n_ener + XILL_THREADS * XILL_BINS_Z - 1) / (XILL_THREADS * XILL_BINS_Z);
+    hipLaunchKernelGGL(k_xillz<MAX_ZONES>, dim3(n_multi * chunks), dim3(XILL_THREADS), 0,
+                       (hipStream_t)stream, b, work + n_single, chunks);
+  }
   return (int)hipGetLastError();
 }
 

@@ -269,7 +269,8 @@ int launch_fixup(const BatchDev &b, bool cos_rows, void *stream);
 int launch_finalize(const BatchDev &b, void *stream);
 // conv_work: (request, zone) pairs; relxill requests have one per zone
 int launch_conv(const BatchDev &b, const XillWork *conv_work, int n_conv_work, void *stream);
-int launch_xill(const BatchDev &b, const XillWork *work, int n_work, void *stream);
+// work: n_single one-zone units, then n_multi units of several zones
+int launch_xill(const BatchDev &b, const XillWork *work, int n_single, int n_multi, void *stream);
 // relxill requests: the primary continuum added to the output (after launch_conv)
 int launch_primary(const BatchDev &b, const int *reqs, int n_reqs, void *stream);
 int launch_copy(const void *src, void *dst, size_t bytes, void *stream);

@@ -300,3 +300,38 @@ def test_relxilllp_batched_with_relxill(relx):
         assert st == 0
         assert rel_err(y, y0) < 1e-12
     b.close()
+
+
+def test_relxilllp_zone_records(relx):
+    """rx_batch_zones against the oracle's zones: emission angle, energy shift
+    source -> zone and normalisation ratio at 1e-12, the zone's Ecut bracketed
+    by the table nodes its index names, the factor at FP32 rounding."""
+    T, xt = relx
+    xv = rx.xilltable_view()
+    ax = np.asarray(xv["val"][xv["pid"].index(3)], dtype=np.float64)
+    rng = np.random.default_rng(12)
+    ps = []
+    for _ in range(6):
+        a = rng.uniform(-0.998, 0.998)
+        ps.append([rng.uniform(3.0, 100.0), rng.uniform(0.0, 0.5), a, rng.uniform(5.0, 85.0), -1.0, 400.0, 0.0,
+                   rng.uniform(1.2, 3.2), 3.1, 1.0, float(np.exp(rng.uniform(np.log(8.0), np.log(800.0)))), 1.0,
+                   0.0, 0.0])
+    b = rx.Batch([dict(model=rx.MODEL_RELXILLLP, energy=log_grid(500, 0.1, 100.0), params=p) for p in ps])
+    try:
+        b.run(rx.FP64)
+        b.sync()
+        for i, p in enumerate(ps):
+            zo, st = oracle.relxill_zones(oracle.MOD_RELXILLLP, T, p)
+            assert st == 0
+            zg = b.zones(i, 10)
+            np.testing.assert_allclose(zg[:, 0], 0.5 * (zo[:, 0] + zo[:, 1]), rtol=1e-12)
+            np.testing.assert_allclose(zg[:, 1:4], zo[:, [2, 3, 5]], rtol=1e-12, atol=1e-300)
+            for z in range(10):
+                k = int(zg[z, 4])
+                ect = min(max(zo[z, 4], ax[0]), ax[-1])
+                assert 0 <= k < ax.size - 1 and ax[k] <= ect * (1 + 1e-7) and ect <= ax[k + 1] * (1 + 1e-7)
+                assert abs(zg[z, 5] - (ect - ax[k]) / (ax[k + 1] - ax[k])) < 1e-6
+        with pytest.raises(rx.RxError):
+            b.zones(0, 11)
+    finally:
+        b.close()
