@@ -1668,7 +1668,6 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const Xil
 // 500-bin grid, src/PrimarySource.h:238-292), times the lamp post's flux boost,
 // rebinned onto the caller grid.  One workgroup per request, after k_conv.
 __global__ __launch_bounds__(PRIM_THREADS) void k_primary(BatchDev b, const int *__restrict__ reqs) {
-  extern __shared__ double s_p[];  // n values, then n + 1 edges
   __shared__ double s_w[PRIM_THREADS / 64];
   const int req = reqs[blockIdx.x];
   const int tid = threadIdx.x, NT = blockDim.x;
@@ -1687,13 +1686,14 @@ __global__ __launch_bounds__(PRIM_THREADS) void k_primary(BatchDev b, const int 
     }
   }
   const double nf = 1. / (block_sum4(xill_norm_part(cg, ect, gam, tid, NT), s_w) / NORM_XILLVER_TABLE);
-  double *s_e = s_p + n;
-  for (int i = tid; i <= n; i += NT) s_e[i] = cg.ener[i];
-  for (int i = tid; i < n; i += NT)
-    s_p[i] = (cutoffpl_bin(cg.ener[i], cg.ener[i + 1], cg.ln_emid[i], ect * rq.g_so, gam) * nf) * prim_scale;
-  __syncthreads();
-  const GridGuess gc = grid_guess(cg.ener, n);
-  auto ge_c = [&](int k) { return s_e[k]; };
+  // the primary on the convolution grid, evaluated where the rebin reads it
+  // (an output bin spans one or two grid bins): no staging buffer
+  const double *ce = cg.ener;
+  const double *cl = cg.ln_emid;
+  const double ect_o = ect * rq.g_so;
+  auto prim_at = [&](int i) { return (cutoffpl_bin(ce[i], ce[i + 1], cl[i], ect_o, gam) * nf) * prim_scale; };
+  const GridGuess gc = grid_guess(ce, n);
+  auto ge_c = [&](int k) { return ce[k]; };
   const double *eo = b.ener_in + rq.ener_off;
   double *out = b.out + rq.flux_off;
   const int n_o = rq.n_in;
@@ -1705,7 +1705,7 @@ __global__ __launch_bounds__(PRIM_THREADS) void k_primary(BatchDev b, const int 
       const double elo = eo[jb], ehi = eo[jb + 1];
       const int clo = count_le(ge_c, n, elo, gc);
       const int chi = count_le(ge_c, n, ehi, gc);
-      out[jb] += rebin_bin(elo, ehi, clo, chi, ge_c, n, [&](int i) { return s_p[i]; });
+      out[jb] += rebin_bin(elo, ehi, clo, chi, ge_c, n, prim_at);
     }
   }
 }
@@ -2057,9 +2057,7 @@ int launch_xill(const BatchDev &b, const XillWork *work, int n_single, int n_mul
 
 int launch_primary(const BatchDev &b, const int *reqs, int n_reqs, void *stream) {
   if (n_reqs <= 0) return 0;
-  const size_t lds = (size_t)(2 * b.cg.n + 1) * sizeof(double);
-  (void)hipFuncSetAttribute((const void *)k_primary, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(k_primary, dim3(n_reqs), dim3(PRIM_THREADS), lds, (hipStream_t)stream, b, reqs);
+  hipLaunchKernelGGL(k_primary, dim3(n_reqs), dim3(PRIM_THREADS), 0, (hipStream_t)stream, b, reqs);
   return (int)hipGetLastError();
 }
 
