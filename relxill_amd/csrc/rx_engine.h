@@ -92,7 +92,8 @@ struct ZoneRec {
   int ect_ind, pad;
 };
 
-// One k_xill unit: a relxill request and one of its zones.
+// A k_conv unit: a request and one of its zones.  k_xill units use the same
+// record with zone 0: one-zone requests, or all zones of a lamp-post request.
 struct XillWork {
   int req, zone;
 };
