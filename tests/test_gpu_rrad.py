@@ -97,3 +97,45 @@ def test_fits_rradtable(tabs, tmp_path):
             _check(T, rx.MODEL_RELLINELP, oracle.MOD_RELLINELP, p, e, tag="fits")
     finally:
         rx.rradtable_synthetic(7, 21, 50, 20)
+
+
+def test_rrad_table_limits(tabs, tmp_path):
+    """A spin above every tabulated spin takes the last one, whose innermost
+    zone starts outside Rin: refused by both; more than 64 table zones on the
+    disc is this library's documented limit (status, zero flux), where the
+    oracle still evaluates."""
+    rel, lp, _ = tables()
+    e = log_grid(500, 0.1, 10.0)
+    p = [6.4, 6.0, 0.9, 30.0, -1.0, 400.0, 0.0, 0.0, 2.0, 1.0]
+    try:
+        t = rrad_table(n_spin=2, n_rad=10, n_g=4, spins=(-0.5, 0.0), c=0.05)
+        write_fits_rradtable(tmp_path / "a.fits", t)
+        assert rx.lib.rx_rradtable_load_fits(str(tmp_path / "a.fits").encode()) == 0, rx.lib.rx_last_error()
+        T = oracle.Tables(rel, lp, rx.rradtable_view())
+        _, st_o = oracle.eval_model(oracle.MOD_RELLINELP, T, p, e)
+        y, st = rx.eval_model(rx.MODEL_RELLINELP, p, e, precision=rx.FP64, raise_on_error=False)
+        print(f"spin above the table: oracle {st_o}, library {st}", flush=True)
+        assert st_o != 0 and st != 0 and not np.any(y)
+        t = rrad_table(n_spin=3, n_rad=70, n_g=4, c=0.01)
+        write_fits_rradtable(tmp_path / "b.fits", t)
+        assert rx.lib.rx_rradtable_load_fits(str(tmp_path / "b.fits").encode()) == 0, rx.lib.rx_last_error()
+        T = oracle.Tables(rel, lp, rx.rradtable_view())
+        # Rout 400: 59 of the 70 zones lie on the disc, within the limit
+        ref, st_o = oracle.eval_model(oracle.MOD_RELLINELP, T, p, e)
+        y, st = rx.eval_model(rx.MODEL_RELLINELP, p, e, precision=rx.FP64, raise_on_error=False)
+        print(f"59 table zones: oracle {st_o}, library {st}, rel {rel_err(y, ref):.3e}", flush=True)
+        assert st_o == 0 and st == 0 and rel_err(y, ref) < TOL64
+        # a = 0.998 (the table spin) and Rout 1000: all 70
+        p3 = list(p)
+        p3[2] = 0.998
+        p3[5] = 1000.0
+        _, st_o = oracle.eval_model(oracle.MOD_RELLINELP, T, p3, e)
+        y, st = rx.eval_model(rx.MODEL_RELLINELP, p3, e, precision=rx.FP64, raise_on_error=False)
+        print(f"70 table zones: oracle {st_o}, library {st}", flush=True)
+        assert st_o == 0 and st != 0 and not np.any(y)
+        p2 = list(p)
+        p2[9] = 0.0  # without returning radiation the table is not used
+        y, st = rx.eval_model(rx.MODEL_RELLINELP, p2, e, precision=rx.FP64)
+        assert st == 0 and np.any(y)
+    finally:
+        rx.rradtable_synthetic(7, 21, 50, 20)
