@@ -531,3 +531,195 @@ def write_fits_rradtable(path, t):
     with open(path, "wb") as f:
         for blk in out:
             f.write(blk)
+
+
+# ---- closed forms for the angle distribution, the red-wing midpoint rule and
+# the lamp-post spin / height interpolation -----------------------------------
+
+def dist_table(gmin=0.3, gmax=1.4, c1=0.7, c2=0.3):
+    """Flat window [gmin, gmax] (the same at every node and radius), trff_1 =
+    c1, trff_2 = c2, and emission-angle cosines that depend on the g* node
+    only: branch 1 of node g has its cosine at the centre of angle bin g,
+    branch 2 at the centre of bin n_g + g, of 2 n_g cos-uniform bins.
+    Returns (table, n_incl)."""
+    rel = flat_table(gmin, gmax)
+    ng = rel["n_g"]
+    nn = rel["n_a"] * rel["n_mu0"] * rel["n_r"]
+    n_incl = 2 * ng
+    mu1 = 1.0 - (np.arange(ng) + 0.5) / n_incl
+    mu2 = 1.0 - (ng + np.arange(ng) + 0.5) / n_incl
+    rel["trff1"] = np.full(nn * ng, c1, dtype=np.float32)
+    rel["trff2"] = np.full(nn * ng, c2, dtype=np.float32)
+    rel["cosne1"] = np.tile(mu1, nn).astype(np.float32)
+    rel["cosne2"] = np.tile(mu2, nn).astype(np.float32)
+    return rel, n_incl
+
+
+def dist_expected(ng, gmin=0.3, gmax=1.4, c1=0.7, c2=0.3, shift=0, drop_dgstar=False, drop_g2=False):
+    """Normalised angle distribution on dist_table().  The reference adds, per
+    radius and (g* node j, branch k), r (2 pi g r)^2 trff_k d_gstar_j /
+    sqrt(g*_j (1 - g*_j)) emis weight into bin floor(n (1 - mu)) (with g =
+    g*_j (gmax - gmin) + gmin, src/Relprofile.cpp:908-938, :799-801).  With a
+    radius-independent window, transfer function and cosines the radial sum
+    r^3 emis weight is a common factor: bin j holds c1 W_j, bin n_g + j holds
+    c2 W_j, W_j = d_gstar_j g_j^2 / sqrt(g*_j (1 - g*_j)), with the g* grid
+    and its trapezoid widths of src/Relprofile.cpp:104-116 (half a step plus
+    H at both ends); gmin, gmax, c1, c2 as the float32 table stores them.
+    shift / drop_* give plausible mistakes."""
+    gmin, gmax, c1, c2 = (float(np.float32(x)) for x in (gmin, gmax, c1, c2))
+    H = 5e-3
+    gs = H + (1 - 2 * H) / (ng - 1) * np.arange(ng)
+    step = gs[1] - gs[0]
+    d = np.full(ng, step)
+    d[0] = d[-1] = 0.5 * step + H
+    if drop_dgstar:
+        d = np.ones(ng)
+    g = gs * (gmax - gmin) + gmin
+    W = d * (1.0 if drop_g2 else g ** 2) / np.sqrt(gs * (1 - gs))
+    out = np.concatenate([c1 * W, c2 * W])
+    out = np.roll(out, shift)
+    return out / out.sum()
+
+
+def midpoint_bins(e, gmin, gmax, c=1.0, rule="mid"):
+    """Photons per bin of relline on flat_table(gmin, gmax, c) with the
+    physical normalisation, for the bins whose g* range lies inside (H, 1-H)
+    of a window below 0.95 (in line-energy units): there int_romb takes the
+    midpoint rule f((lo+hi)/2) (hi - lo) (src/Relprofile.cpp:636-644), f =
+    E^3 c / ((gmax - gmin) sqrt(g*(1-g*))) summed over both branches, and
+    the normalised emissivity gives sum_r emis pi r dr / 2 = 1/4.  rule
+    "quad" integrates f instead (what Romberg everywhere would give), "lo"
+    takes the lower end: the mistakes the check must catch.  Returns
+    (indices, values).  gmin, gmax as the float32 table stores them."""
+    from scipy.integrate import quad
+    gmin, gmax = float(np.float32(gmin)), float(np.float32(gmax))
+    H = 5e-3
+    f = lambda E: E ** 3 * c / ((gmax - gmin) * np.sqrt(((E - gmin) / (gmax - gmin)) * (1 - (E - gmin) / (gmax - gmin))))
+    idx, val = [], []
+    for j in range(len(e) - 1):
+        lo, hi = e[j], e[j + 1]
+        glo, ghi = (lo - gmin) / (gmax - gmin), (hi - gmin) / (gmax - gmin)
+        if not (glo > H and ghi < 1 - H):
+            continue
+        if rule == "mid":
+            v = f(0.5 * (lo + hi)) * (hi - lo)
+        elif rule == "lo":
+            v = f(lo) * (hi - lo)
+        else:
+            v = quad(f, lo, hi, epsabs=0, epsrel=1e-13)[0]
+        idx.append(j)
+        val.append(0.25 * v / (0.5 * (lo + hi)))
+    return np.array(idx), np.array(val)
+
+
+LP_INT = (1.0, 0.4, 0.01)     # intensity factor A = A0 + A1 a + A2 h
+LP_DEL = (1.38, 0.03, 4e-4)   # emission angle D + s ln r (above 75 deg everywhere)
+LP_DEL_S = 0.04
+LP_INC = (0.2, 0.05, 1e-3)    # incidence angle Di + si ln r
+LP_INC_S = 0.02
+
+
+def affine_lp_table(unit=False):
+    """A lamp-post table whose fields are affine in spin and height at every
+    table radius: intensity A(a, h) r^-3, emission angle D(a, h) + s ln r,
+    incidence angle Di(a, h) + si ln r.  The emission angle exceeds 75 deg
+    everywhere, so rebin_emisprofile_on_radial_grid uses the log radius factor
+    (src/Rellp.cpp:168-173), under which these r-dependences are exact; the
+    radii are one log grid from 1 to 1000 rg for every spin.  Every spin has
+    its own height grid (1.5 .. 500 rg with a spin-dependent spacing), so a
+    height factor taken from the wrong spin is visible.  unit=True sets A = 1
+    (the reference the intensity ratio is taken against)."""
+    rel, lp, T = tables()
+    na, nh, nr = lp["n_a"], lp["n_h"], lp["n_r"]
+    out = dict(lp)
+    a = lp["a"].astype(np.float64)
+    t = np.arange(nh) / (nh - 1)
+    hg = np.empty((na, nh))
+    for ia in range(na):
+        p = 1.0 + 0.35 * (ia % 4)
+        hg[ia] = 1.5 * (500.0 / 1.5) ** (t ** p)
+    hg = hg.astype(np.float32)
+    rad = np.exp(np.log(1000.0) * np.arange(nr) / (nr - 1))
+    rad[-1] = 1000.0
+    rad = rad.astype(np.float32)
+    A = np.asarray(a)[:, None, None] * 0 + 1.0 if unit else \
+        LP_INT[0] + LP_INT[1] * a[:, None, None] + LP_INT[2] * hg[:, :, None].astype(np.float64)
+    r = rad.astype(np.float64)[None, None, :]
+    D = LP_DEL[0] + LP_DEL[1] * a[:, None, None] + LP_DEL[2] * hg[:, :, None].astype(np.float64)
+    Di = LP_INC[0] + LP_INC[1] * a[:, None, None] + LP_INC[2] * hg[:, :, None].astype(np.float64)
+    out["h"] = hg.ravel()
+    out["rad"] = np.tile(rad, na)
+    out["intens"] = np.broadcast_to(A * r ** -3.0, (na, nh, nr)).astype(np.float32).ravel()
+    out["del_"] = (D + LP_DEL_S * np.log(r)).astype(np.float32).ravel()
+    out["del_inc"] = (Di + LP_INC_S * np.log(r)).astype(np.float32).ravel()
+    return out
+
+
+def lp_factors(lp, a, h, mistake=None):
+    """Spin and height brackets with the interpolation factors get_ipol_factor
+    forms in float (value, grid and quotient in float, src/relutility.c:419-423;
+    binary_search_float, :135-151), the height factor per bracketing spin on
+    that spin's own grid (src/Rellp.cpp:225-241).  mistake "one_h" uses the
+    lower spin's height bracket and factor for both spins, "swap_a" exchanges
+    the spin weights."""
+    ag = lp["a"].astype(np.float32)
+    hg = lp["h"].reshape(lp["n_a"], lp["n_h"]).astype(np.float32)
+
+    def bracket(grid, v):
+        i = int(np.searchsorted(grid, v, side="right")) - 1
+        i = min(max(i, 0), len(grid) - 2)
+        f = np.float32(np.float32(v - grid[i]) / np.float32(grid[i + 1] - grid[i]))
+        return i, float(f)
+    ia, fa = bracket(ag, np.float32(a))
+    hs = [bracket(hg[ia + k], np.float32(h)) for k in (0, 1)]
+    if mistake == "one_h":
+        hs[1] = hs[0]
+    if mistake == "swap_a":
+        fa = 1.0 - fa
+    # the interpolated points: a moved by fa, h by each spin's factor
+    a_f = float(ag[ia]) + fa * (float(ag[ia + 1]) - float(ag[ia]))
+    h_f = [float(hg[ia + k][hs[k][0]]) + hs[k][1] * (float(hg[ia + k][hs[k][0] + 1]) - float(hg[ia + k][hs[k][0]]))
+           for k in (0, 1)]
+    return fa, a_f, h_f
+
+
+def lp_affine_value(coef, lp, a, h, mistake=None):
+    """c0 + c1 a + c2 h interpolated as interpol_lptable does
+    ((1 - f_a) lerp_h(spin lo) + f_a lerp_h(spin hi), src/Rellp.cpp:219-247):
+    bilinear interpolation reproduces an affine field, at the points moved by
+    the float factors."""
+    fa, a_f, h_f = lp_factors(lp, a, h, mistake)
+    return coef[0] + coef[1] * a_f + coef[2] * ((1 - fa) * h_f[0] + fa * h_f[1])
+
+
+def aberrate(delta, beta):
+    """Emission angle (from the direction to the black hole) in the rest frame
+    of a source moving away from the disc with speed beta, from the static
+    angle: cos d' = (cos d + beta) / (1 + beta cos d) (relativistic
+    aberration)."""
+    c = np.cos(delta)
+    return np.arccos((c + beta) / (1 + beta * c))
+
+
+def refl_frac_expected(lp, a, h, beta, rin, rout, nr=1000):
+    """Reflection fractions of calc_refl_frac (src/Rellp.cpp:36-89) on
+    affine_lp_table(): the emission angles at the fine radius the reference's
+    bisections land on for Rin (index nr - 2) and Rout (index 0) of the
+    descending fine grid, D + s ln r there; the largest table angle D + s
+    ln 1000 (not below pi/2); for beta > 1e-6 all three taken to the source's
+    rest frame.  f_bh = (1 - cos d_bh)/2, f_ad = (cos d_bh - cos d_ad)/2,
+    f_inf = (1 + cos d_max)/2, refl_frac = f_ad / f_inf.  Returns [refl_frac,
+    f_bh, f_ad, f_inf, f_inf_rest]."""
+    import oracle
+    D = lp_affine_value(LP_DEL, lp, a, h)
+    re = oracle.fine_radial_grid(rin, rout, nr)
+    d_bh = D + LP_DEL_S * np.log(re[nr - 2])
+    d_ad = D + LP_DEL_S * np.log(re[0])
+    d_max = max(D + LP_DEL_S * np.log(1000.0), np.pi / 2)
+    f_inf_rest = 0.5 * (1 + np.cos(d_max))
+    if beta > 1e-6:
+        d_bh, d_ad, d_max = aberrate(d_bh, beta), aberrate(d_ad, beta), aberrate(d_max, beta)
+    f_bh = 0.5 * (1 - np.cos(d_bh))
+    f_ad = 0.5 * (np.cos(d_bh) - np.cos(d_ad))
+    f_inf = 0.5 * (1 + np.cos(d_max))
+    return np.array([f_ad / f_inf, f_bh, f_ad, f_inf, f_inf_rest])
