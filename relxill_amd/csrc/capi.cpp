@@ -158,6 +158,7 @@ int rx_tables_synthetic(uint64_t seed, int n_a, int n_mu0, int n_r, int n_g) {
     return RX_ERR_NOMEM;
   }
   g.rel_loaded = true;
+  g.rel_err.clear();
   g.rel_dirty = true;
   g.config_gen++;  // batches built before hold buffers sized for the old table
   std::string e = validate_reltable(g.rel);
@@ -174,6 +175,7 @@ int rx_lptable_synthetic(uint64_t seed, int n_a, int n_h, int n_r) {
   }
   g.lp = make_synthetic_lptable(seed, n_a, n_h, n_r);
   g.lp_loaded = true;
+  g.lp_err.clear();
   g.lp_dirty = true;
   g.config_gen++;
   std::string e = validate_lptable(g.lp);
@@ -193,6 +195,7 @@ int rx_tables_load(const char *path) {
   }
   g.rel = std::move(t);
   g.rel_loaded = true;
+  g.rel_err.clear();
   g.rel_dirty = true;
   g.config_gen++;  // batches built before hold buffers sized for the old table
   return RX_OK;
@@ -210,6 +213,7 @@ int rx_lptable_load(const char *path) {
   }
   g.lp = std::move(t);
   g.lp_loaded = true;
+  g.lp_err.clear();
   g.lp_dirty = true;
   g.config_gen++;
   return RX_OK;
@@ -227,6 +231,7 @@ int rx_tables_load_fits(const char *path) {
   }
   g.rel = std::move(t);
   g.rel_loaded = true;
+  g.rel_err.clear();
   g.rel_dirty = true;
   g.config_gen++;
   return RX_OK;
@@ -244,6 +249,7 @@ int rx_lptable_load_fits(const char *path) {
   }
   g.lp = std::move(t);
   g.lp_loaded = true;
+  g.lp_err.clear();
   g.lp_dirty = true;
   g.config_gen++;
   return RX_OK;
@@ -263,6 +269,7 @@ int rx_xilltable_synthetic(uint64_t seed, int n_gam, int n_afe, int n_lxi, int n
     return RX_ERR_NOMEM;
   }
   g.xill_loaded = true;
+  g.xill_err.clear();
   g.xill_dirty = true;
   g.config_gen++;
   std::string e = validate_xilltable(g.xill);
@@ -291,6 +298,7 @@ int rx_xilltable_load_fits(const char *path) {
   }
   g.xill = std::move(t);
   g.xill_loaded = true;
+  g.xill_err.clear();
   g.xill_dirty = true;
   g.config_gen++;
   return RX_OK;
@@ -301,6 +309,10 @@ int rx_xilltable_get_view(rx_xilltable_view *v) {
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   if (!v) return RX_ERR_ARG;
   ensure_xilltable_host();
+  if (!g.xill_loaded) {
+    set_last(RX_ERR_TABLE, "xillver table: " + g.xill_err);
+    return RX_ERR_TABLE;
+  }
   const XillTableHost &t = g.xill;
   memset(v, 0, sizeof(*v));
   v->n_par = t.n_par;
@@ -325,6 +337,7 @@ int rx_rradtable_synthetic(uint64_t seed, int n_spin, int n_rad, int n_g) {
   }
   g.rrad = make_synthetic_rradtable(seed, n_spin, n_rad, n_g);
   g.rrad_loaded = true;
+  g.rrad_err.clear();
   g.rrad_dirty = true;
   g.config_gen++;
   std::string e = validate_rradtable(g.rrad);
@@ -344,6 +357,7 @@ int rx_rradtable_load_fits(const char *path) {
   }
   g.rrad = std::move(t);
   g.rrad_loaded = true;
+  g.rrad_err.clear();
   g.rrad_dirty = true;
   g.config_gen++;
   return RX_OK;
@@ -354,6 +368,10 @@ int rx_rradtable_get_view(rx_rradtable_view *v) {
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   if (!v) return RX_ERR_ARG;
   ensure_rradtable_host();
+  if (!g.rrad_loaded) {
+    set_last(RX_ERR_TABLE, "returning-radiation table: " + g.rrad_err);
+    return RX_ERR_TABLE;
+  }
   const RradTableHost &t = g.rrad;
   *v = {t.n_spin, t.n_rad, t.n_g, t.spin.data(), t.rlo.data(), t.rhi.data(), t.tf_r.data(), t.gmin.data(),
         t.gmax.data(), t.frac_g.data()};
@@ -364,6 +382,10 @@ int rx_tables_save(const char *path) {
   Global &g = G();
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   ensure_tables_host();
+  if (!g.rel_loaded) {
+    set_last(RX_ERR_TABLE, g.rel_err);
+    return RX_ERR_TABLE;
+  }
   std::string e = save_reltable(g.rel, path ? path : "");
   set_last(e.empty() ? RX_OK : RX_ERR_ARG, e);
   return e.empty() ? RX_OK : RX_ERR_ARG;
@@ -373,6 +395,10 @@ int rx_lptable_save(const char *path) {
   Global &g = G();
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   ensure_tables_host();
+  if (!g.lp_loaded) {
+    set_last(RX_ERR_TABLE, g.lp_err);
+    return RX_ERR_TABLE;
+  }
   std::string e = save_lptable(g.lp, path ? path : "");
   set_last(e.empty() ? RX_OK : RX_ERR_ARG, e);
   return e.empty() ? RX_OK : RX_ERR_ARG;
@@ -382,6 +408,10 @@ int rx_tables_view(rx_reltable_view *rel, rx_lptable_view *lp) {
   Global &g = G();
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   ensure_tables_host();
+  if ((rel && !g.rel_loaded) || (lp && !g.lp_loaded)) {
+    set_last(RX_ERR_TABLE, !g.rel_loaded ? g.rel_err : g.lp_err);
+    return RX_ERR_TABLE;
+  }
   if (rel) {
     const RelTableHost &t = g.rel;
     *rel = {t.n_a, t.n_mu0, t.n_r, t.n_g, t.a.data(), t.mu0.data(), t.r.data(), t.gmin.data(),

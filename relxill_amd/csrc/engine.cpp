@@ -286,32 +286,76 @@ void xill_corner(const XillTableHost &t, double gam, double afe, double lxi, dou
 // tables and grids
 // ---------------------------------------------------------------------------
 
+// Tables named by RELXILL_TABLE_PATH.  The reference reads its tables only
+// from that directory (get_relxill_table_path, src/relutility.c:320-328) and
+// stops with an error when it cannot open one (src/reltable.c:255-268), so a
+// set path never falls back to synthetic tables here: the first of the file
+// names that exists is loaded, and a missing or unreadable table leaves its
+// error in *err (returned by every request that needs the table).  Synthetic
+// tables are only built when the variable is unset.
+static bool file_exists(const std::string &path) {
+  FILE *f = fopen(path.c_str(), "rb");
+  if (!f) return false;
+  fclose(f);
+  return true;
+}
+
+template <class T, class LoadFits, class LoadOwn>
+static bool load_from_path(T &dst, const char *fits_name, const char *own_name, LoadFits load_fits,
+                           LoadOwn load_own, std::string &err) {
+  const std::string dir = getenv("RELXILL_TABLE_PATH");
+  const std::string fits = dir + "/" + fits_name;
+  const std::string own = own_name ? dir + "/" + own_name : std::string();
+  T t;
+  std::string e;
+  if (file_exists(fits)) {
+    e = load_fits(t, fits);
+    if (!e.empty()) e = fits + ": " + e;
+  } else if (own_name && file_exists(own)) {
+    e = load_own(t, own);
+    if (!e.empty()) e = own + ": " + e;
+  } else {
+    e = std::string("RELXILL_TABLE_PATH=") + dir + " holds no " + fits_name +
+        (own_name ? std::string(" (nor ") + own_name + ")" : std::string());
+  }
+  if (!e.empty()) {
+    err = e;
+    return false;
+  }
+  dst = std::move(t);
+  err.clear();
+  return true;
+}
+
 void ensure_tables_host() {
   Global &g = G();
+  const bool from_path = getenv("RELXILL_TABLE_PATH") != nullptr;
   if (!g.rel_loaded) {
-    bool ok = false;
-    if (const char *path = getenv("RELXILL_TABLE_PATH")) {
-      // the reference's table file and name first (src/common.h:73), then our own format
-      ok = load_reltable_fits(g.rel, std::string(path) + "/rel_table_v0.5a.fits").empty() ||
-           load_reltable(g.rel, std::string(path) + "/rel_table.rxtb").empty();
+    // the reference's table file and name first (src/common.h:73), then our own format
+    if (!from_path) {
+      g.rel = make_synthetic_reltable(1, 25, 22, 100, 20);
+      g.rel_err.clear();
+      g.rel_loaded = true;
+    } else {
+      g.rel_loaded = load_from_path(g.rel, "rel_table_v0.5a.fits", "rel_table.rxtb", load_reltable_fits,
+                                    load_reltable, g.rel_err);
     }
-    if (!ok) g.rel = make_synthetic_reltable(1, 25, 22, 100, 20);
-    g.rel_loaded = true;
-    g.rel_dirty = true;
+    if (g.rel_loaded) g.rel_dirty = true;
   }
-  if (g.rel_dirty) {
+  if (g.rel_loaded && g.rel_dirty) {
     g.rel_gmin_all = *std::min_element(g.rel.gmin.begin(), g.rel.gmin.end());
     g.rel_gmax_all = *std::max_element(g.rel.gmax.begin(), g.rel.gmax.end());
   }
   if (!g.lp_loaded) {
-    bool ok = false;
-    if (const char *path = getenv("RELXILL_TABLE_PATH")) {
-      ok = load_lptable_fits(g.lp, std::string(path) + "/rel_lp_table_v0.5b.fits").empty() ||
-           load_lptable(g.lp, std::string(path) + "/rel_lp_table.rxtb").empty();
+    if (!from_path) {
+      g.lp = make_synthetic_lptable(3, 20, 100, 100);
+      g.lp_err.clear();
+      g.lp_loaded = true;
+    } else {
+      g.lp_loaded = load_from_path(g.lp, "rel_lp_table_v0.5b.fits", "rel_lp_table.rxtb", load_lptable_fits,
+                                   load_lptable, g.lp_err);
     }
-    if (!ok) g.lp = make_synthetic_lptable(3, 20, 100, 100);
-    g.lp_loaded = true;
-    g.lp_dirty = true;
+    if (g.lp_loaded) g.lp_dirty = true;
   }
   if (g.conv_ener.empty()) g.conv_dirty = true;
   if (g.coarse_ener.empty()) {
@@ -398,6 +442,10 @@ static int upload_vec(DevBuf &d, const std::vector<T> &v) {
 int ensure_device(std::string &err) {
   Global &g = G();
   ensure_tables_host();
+  if (!g.rel_loaded) {  // every model needs the relline table
+    err = g.rel_err;
+    return RX_ERR_TABLE;
+  }
   if (!g.device_ready) {
     int count = 0;
     hipError_t e = hipGetDeviceCount(&count);
@@ -418,7 +466,8 @@ int ensure_device(std::string &err) {
     (void)hipSetDevice(g.device);
   }
   int e = 0;
-  if (g.rel_dirty || g.lp_dirty || g.conv_dirty || g.refl_dirty) {
+  const bool lp_dirty = g.lp_dirty && g.lp_loaded;
+  if (g.rel_dirty || lp_dirty || g.conv_dirty || g.refl_dirty) {
     // resident buffers are about to be overwritten or reallocated: no kernel
     // of any stream may still read them
     if (hipDeviceSynchronize() != hipSuccess) {
@@ -437,7 +486,7 @@ int ensure_device(std::string &err) {
          upload_vec(g.t_trff2, t.trff2) | upload_vec(g.t_cosne1, t.cosne1) | upload_vec(g.t_cosne2, t.cosne2);
     g.rel_dirty = false;
   }
-  if (g.lp_dirty) {
+  if (lp_dirty) {
     const LpTableHost &t = g.lp;
     if (t.n_r > MAX_LP_R) {
       err = "lamp-post table dims exceed kernel limits";
@@ -469,19 +518,25 @@ int ensure_device(std::string &err) {
 void ensure_xilltable_host() {
   Global &g = G();
   if (g.xill_loaded) return;
-  bool ok = false;
-  if (const char *path = getenv("RELXILL_TABLE_PATH")) {
+  if (!getenv("RELXILL_TABLE_PATH")) {
+    g.xill = make_synthetic_xilltable(5, 13, 5, 8, 6, 10, 2999);
+    g.xill_err.clear();
+    g.xill_loaded = true;
+  } else {
     // the reference's table name (src/common.h:163)
-    ok = load_xilltable_fits(g.xill, std::string(path) + "/xillver-a-Ec5.fits").empty();
+    g.xill_loaded = load_from_path(g.xill, "xillver-a-Ec5.fits", nullptr, load_xilltable_fits,
+                                   load_xilltable_fits, g.xill_err);
   }
-  if (!ok) g.xill = make_synthetic_xilltable(5, 13, 5, 8, 6, 10, 2999);
-  g.xill_loaded = true;
-  g.xill_dirty = true;
+  if (g.xill_loaded) g.xill_dirty = true;
 }
 
 int ensure_xilltable_device(std::string &err) {
   Global &g = G();
   ensure_xilltable_host();
+  if (!g.xill_loaded) {
+    err = "xillver table: " + g.xill_err;
+    return RX_ERR_TABLE;
+  }
   if (!g.xill_dirty) return RX_OK;
   const XillTableHost &t = g.xill;
   if (t.n_incl() > MAX_INCL) {
@@ -516,19 +571,25 @@ int ensure_xilltable_device(std::string &err) {
 void ensure_rradtable_host() {
   Global &g = G();
   if (g.rrad_loaded) return;
-  bool ok = false;
-  if (const char *path = getenv("RELXILL_TABLE_PATH")) {
+  if (!getenv("RELXILL_TABLE_PATH")) {
+    g.rrad = make_synthetic_rradtable(7, 21, 50, 20);
+    g.rrad_err.clear();
+    g.rrad_loaded = true;
+  } else {
     // the reference's table name (src/common.h:128)
-    ok = load_rradtable_fits(g.rrad, std::string(path) + "/table_returnRad_v20220301.fits").empty();
+    g.rrad_loaded = load_from_path(g.rrad, "table_returnRad_v20220301.fits", nullptr, load_rradtable_fits,
+                                   load_rradtable_fits, g.rrad_err);
   }
-  if (!ok) g.rrad = make_synthetic_rradtable(7, 21, 50, 20);
-  g.rrad_loaded = true;
-  g.rrad_dirty = true;
+  if (g.rrad_loaded) g.rrad_dirty = true;
 }
 
 int ensure_rradtable_device(std::string &err) {
   Global &g = G();
   ensure_rradtable_host();
+  if (!g.rrad_loaded) {
+    err = "returning-radiation table: " + g.rrad_err;
+    return RX_ERR_TABLE;
+  }
   if (!g.rrad_dirty) return RX_OK;
   if (hipDeviceSynchronize() != hipSuccess) {
     err = "device synchronisation failed";
@@ -672,6 +733,13 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
       int st = ensure_xilltable_device(err);
       if (st != RX_OK) return st;
       break;
+    }
+  }
+  for (int i = 0; i < n_req; i++) {
+    const int m = req[i].model;
+    if ((m == MOD_RELLINELP || m == MOD_RELCONVLP || m == MOD_RELXILLLP) && !g.lp_loaded) {
+      err = "lamp-post table: " + g.lp_err;
+      return RX_ERR_TABLE;
     }
   }
   for (int i = 0; i < n_req; i++) {

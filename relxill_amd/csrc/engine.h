@@ -85,6 +85,8 @@ struct Global {
   RelTableHost rel;
   LpTableHost lp;
   bool rel_loaded = false, lp_loaded = false;
+  // why a table named by RELXILL_TABLE_PATH could not be loaded (empty: loaded)
+  std::string rel_err, lp_err, xill_err, rrad_err;
   bool rel_dirty = true, lp_dirty = true;
   double rel_gmin_all = 0.0, rel_gmax_all = 1e30;  // g range reachable by any window
   DevBuf t_a, t_mu0, t_r, t_gmin, t_gmax, t_trff1, t_trff2, t_cosne1, t_cosne2;

@@ -128,3 +128,52 @@ def test_table_file_round_trip(tmp_path):
     for k in ("a", "mu0", "r", "gmin", "gmax", "trff1", "trff2", "cosne1", "cosne2"):
         assert np.array_equal(rel0[k], rel1[k]), k
     rx.tables_synthetic(1, 25, 22, 100, 20)  # restore the default tables
+
+
+def _fresh_process(code, table_path):
+    import subprocess
+    import sys
+    env = dict(os.environ, RELXILL_TABLE_PATH=str(table_path))
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env=env, capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    return out.stdout
+
+
+def test_table_path_without_tables_is_an_error(tmp_path):
+    """RELXILL_TABLE_PATH set but holding no tables: the reference cannot open
+    its table and fails (src/reltable.c:255-268), so the library returns
+    RX_ERR_TABLE with the reason and zero flux instead of evaluating on
+    synthetic tables; explicitly loaded tables clear the error."""
+    code = ("import numpy as np, relxill_amd as rx, relxill_amd.capi\n"
+            "e = np.geomspace(0.1, 10.0, 101)\n"
+            "f, st = rx.eval_model(rx.MODEL_RELLINE, [6.4, 3, 3, 15, 0.9, 30, -1, 400, 0, 0], e,"
+            " raise_on_error=False)\n"
+            "print(st, int(np.any(f != 0)))\n"
+            "print(rx.lib.rx_last_error().decode())\n"
+            "rel, lp = rx.capi._RelView(), rx.capi._LpView()\n"
+            "import ctypes\n"
+            "print(rx.lib.rx_tables_view(ctypes.byref(rel), None))\n"
+            "rx.tables_synthetic(1, 5, 4, 20, 8)\n"
+            "print(rx.lib.rx_tables_view(ctypes.byref(rel), None), rel.n_a)\n"
+            "print(rx.lib.rx_tables_view(None, ctypes.byref(lp)))\n"
+            "print(rx.lib.rx_last_error().decode())\n")
+    lines = _fresh_process(code, tmp_path).splitlines()
+    assert lines[0] == "10 0"
+    assert "rel_table_v0.5a.fits" in lines[1] and str(tmp_path) in lines[1]
+    assert lines[2] == "10"
+    assert lines[3] == "0 5"
+    assert lines[4] == "10" and "rel_lp_table_v0.5b.fits" in lines[5]
+
+
+def test_unreadable_table_file_is_an_error(tmp_path):
+    """A table file that exists but cannot be read is reported with the
+    loader's message (never replaced by synthetic data)."""
+    (tmp_path / "rel_table.rxtb").write_bytes(b"RXTB but truncated")
+    code = ("import ctypes, relxill_amd as rx, relxill_amd.capi\n"
+            "rel = rx.capi._RelView()\n"
+            "print(rx.lib.rx_tables_view(ctypes.byref(rel), None))\n"
+            "print(rx.lib.rx_last_error().decode())\n")
+    lines = _fresh_process(code, tmp_path).splitlines()
+    assert lines[0] == "10"
+    assert "rel_table.rxtb" in lines[1]
