@@ -7,13 +7,17 @@ parameters from the lamp-post priors (a U[-0.998,0.998], incl U[5,85] deg,
 Rin U[ISCO,50]) with a broken power law q=3, Rout=400, on the 4096-bin
 convolution grid; 25x22x100x20 relline table.  One step = one pass of the whole
 device path (setup, relline integration, renormalisation, FFT convolution,
-rebin) over the batch, FP32 with Kahan sums, inputs resident in HBM.
+rebin) over the batch, FP32 integrand with FP64 sums over the radii (which
+replace the Kahan-compensated FP32 sums: no long FP32 running sum exists),
+inputs resident in HBM.
 
 Also reported: the plain FP64 path (one request per launch), the batched
 lamp-post relline (1024 requests, 3000 bins), the relxill-style convolution of
 a resident reflection set, relxill and relxilllp (10 zones) with the xillver
 table interpolated on the device (1024 requests each), the serving loop's p50/p99
-latency under Poisson arrivals (relline/relconv/lamp-post mix 40/40/20), the
+latency under Poisson arrivals (relline/relconv/lamp-post mix 40/40/20; the
+server's own pacing loop replays a pre-generated stream, at 4k, 20k and 100k
+QPS and far above saturation), the
 maximum relative error of the FP32 path against the FP64 oracle, the dominant
 kernel's roofline figures and the FP64 CPU oracle timed on the host's cores.
 
@@ -31,6 +35,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+VALU_PEAK_FP32_TFLOPS = 157.3  # MI355X published vector FP32
+VALU_PEAK_FP64_TFLOPS = 78.6   # MI355X published vector FP64
 KERNELS = ("k_setup", "k_integrate", "k_fixup", "k_finalize", "k_conv", "k_xill", "k_primary")
 
 
@@ -209,6 +215,13 @@ def relxill_batch(rx, steps, n_req=1024, seed=9, cpu=True):
     dt64, _ = timed_steps(rx, b, rx.FP64, 2, 1)
     o64 = [b.output(i)[0] for i in range(n_req)]
     b.close()
+    # the primary continuum on a caller grid coarser than the convolution
+    # grid (200 bins over 0.1-1000 keV): each output bin sums many
+    # convolution bins in one lane
+    e_coarse = log_grid(200, 0.1, 1000.0)
+    bc = rx.Batch([dict(r, energy=e_coarse) for r in reqs])
+    _, kms_c = timed_steps(rx, bc, rx.FP32, 3, 1)
+    bc.close()
     lat = []
     for r in reqs[:8]:
         pb = rx.Batch([r])
@@ -223,6 +236,7 @@ def relxill_batch(rx, steps, n_req=1024, seed=9, cpu=True):
                plain_fp64_ms_per_spectrum=float(np.mean(lat)) * 1e3,
                max_rel_err_fp32_vs_fp64=max(rel_err(o32[i], o64[i]) for i in range(n_req)),
                kernel_ms={k: round(v, 4) for k, v in zip(KERNELS, kms32)},
+               k_primary_ms_200_bin_grid=round(float(kms_c[6]), 4),
                mean_angles_used=n_angles / n_req,
                k_xill_roofline=dict(bound="hbm", algorithmic_bytes=alg, achieved=alg / t_x / 1e9,
                                     peak=HBM_PEAK_GBS, unit="GB/s", frac=alg / t_x / 1e9 / HBM_PEAK_GBS))
@@ -277,56 +291,105 @@ def timed_steps(rx, b, prec, steps, warmup):
     return (t1 - t0) / steps, kms / steps
 
 
-def serve(rx, seconds, qps, seed=6):
-    """Poisson arrivals, mix relline/relconv/lamp-post 40/40/20, grids 300-8000 bins."""
+def serving_pool(n=2048, seed=6):
+    """cfg4 requests: relline / relconv / lamp-post relline at 40/40/20, grids
+    of 300-8000 bins (log-uniform), parameters from the lamp-post priors."""
     rng = np.random.default_rng(seed)
-    srv = rx.Server(max_inflight=4096, hbm_budget=16 << 30, n_streams=4, precision=rx.FP32)
-    tickets = []
-    t_start = time.perf_counter()
-    t_next = t_start
-    n_sub = 0
-    while True:
-        now = time.perf_counter()
-        if now - t_start > seconds:
-            break
-        if now < t_next:
-            time.sleep(min(t_next - now, 0.0005))
-            continue
-        t_next += rng.exponential(1.0 / qps)
-        n = int(np.exp(rng.uniform(np.log(300), np.log(8000))))
+    pool = []
+    for _ in range(n):
+        nb = int(np.exp(rng.uniform(np.log(300), np.log(8000))))
         a = rng.uniform(-0.998, 0.998)
         incl = rng.uniform(5.0, 85.0)
         rin = rng.uniform(kerr_rms(a), 50.0)
         u = rng.uniform()
         if u < 0.4:
-            e = log_grid(n, 0.1, 10.0)
-            t = srv.submit(1, e, [6.4, 3.0, 3.0, 15.0, a, incl, rin, 400.0, 0.0, 0.0])
+            pool.append(dict(model=1, energy=log_grid(nb, 0.1, 10.0),
+                             params=[6.4, 3.0, 3.0, 15.0, a, incl, rin, 400.0, 0.0, 0.0]))
         elif u < 0.8:
-            e = log_grid(n, 0.1, 100.0)
-            t = srv.submit(11, e, [3.0, 3.0, 15.0, a, incl, rin, 400.0, 0.0], flux_in=refl_input(e))
+            e = log_grid(nb, 0.1, 100.0)
+            pool.append(dict(model=11, energy=e, params=[3.0, 3.0, 15.0, a, incl, rin, 400.0, 0.0],
+                             flux_in=refl_input(e)))
         else:
-            e = log_grid(n, 0.1, 10.0)
-            h = rng.uniform(3.0, 100.0)
-            t = srv.submit(2, e, [6.4, h, a, incl, rin, 400.0, 0.0, 0.0, 2.0, 0.0])
-        tickets.append(t)
-        n_sub += 1
+            pool.append(dict(model=2, energy=log_grid(nb, 0.1, 10.0),
+                             params=[6.4, rng.uniform(3.0, 100.0), a, incl, rin, 400.0, 0.0, 0.0, 2.0, 0.0]))
+    return pool
+
+
+def serve(rx, pool, qps, seconds=None, n_total=None, seed=6):
+    """Poisson arrivals at qps (seconds long, or n_total requests) drawn from
+    the pre-generated pool, replayed by the server's native pacing loop (so
+    the client does not limit the offered load); p50 / p99 of submit ->
+    completion and the completed spectra per second of wall time."""
+    rng = np.random.default_rng(seed)
+    n = n_total if n_total else int(qps * seconds * 1.2) + 16
+    t = np.cumsum(rng.exponential(1.0 / qps, size=n))
+    if seconds and not n_total:
+        t = t[t <= seconds]
+    reqs = [pool[i % len(pool)] for i in range(t.size)]
+    srv = rx.Server(max_inflight=8192, hbm_budget=32 << 30, n_streams=4, precision=rx.FP32)
+    t0 = time.perf_counter()
+    tickets = srv.replay(reqs, t)
     srv.drain()
-    t_end = time.perf_counter()
-    lats, bad = [], 0
-    for t in tickets:
-        r = srv.poll(t)
-        lats.append(r[2])
+    wall = time.perf_counter() - t0
+    lats, bad = np.empty(len(tickets)), 0
+    for k, tk in enumerate(tickets):
+        r = srv.poll(tk)
+        lats[k] = r[2]
         bad += r[1] != 0
     st = srv.stats()
     srv.stop()
-    lats = np.array(lats)
-    return dict(qps_offered=qps, submitted=n_sub, failed=int(bad),
+    return dict(qps_offered=qps, submitted=len(tickets), failed=int(bad),
                 p50_ms=float(np.percentile(lats, 50)), p99_ms=float(np.percentile(lats, 99)),
-                spectra_per_s=n_sub / (t_end - t_start), batches=st["batches"],
-                max_concurrent_batches=st["max_concurrent_batches"])
+                spectra_per_s=len(tickets) / wall, mean_batch=len(tickets) / max(1, st["batches"]),
+                batches=st["batches"], max_concurrent_batches=st["max_concurrent_batches"])
+
+
+def host_cpus():
+    """What the host offers this process: logical CPUs (nproc), physical cores
+    (distinct (physical id, core id) pairs of /proc/cpuinfo), the CPU model,
+    the CPUs this process may run on (affinity) and its cgroup CPU quota."""
+    info = dict(nproc=os.cpu_count() or 1, model=None, physical_cores=None, affinity=None, cgroup_quota=None)
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    try:
+        cores, phys = set(), None
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and info["model"] is None:
+                info["model"] = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                cores.add((phys, v))
+        info["physical_cores"] = len(cores) or None
+    except OSError:
+        pass
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            info["cgroup_quota"] = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    return info
+
+
+def cpu_threads(info):
+    """Threads for the all-cores leg: the CPUs this process is granted (the
+    smaller of its affinity set and its cgroup quota), else nproc."""
+    n = info["affinity"] or info["nproc"]
+    if info["cgroup_quota"]:
+        n = min(n, max(1, int(info["cgroup_quota"])))
+    return max(1, n)
 
 
 def cpu_baseline(reqs, n_threads, gpu_out):
+    """The FP64 oracle on the host: single core on 16 requests, then the sample
+    on n_threads threads; also the reference algorithm's work counters over
+    the sample (integrand evaluations and the rest), the FLOP count of the
+    roofline."""
     import oracle
     import relxill_amd as rx
     rel, lp = rx.tables_view()
@@ -344,6 +407,31 @@ def cpu_baseline(reqs, n_threads, gpu_out):
     err = max(rel_err(gpu_out[i], outs[i]) for i in range(len(jobs)) if st[i] == 0)
     return dict(single_core=single, all_cores=allc, cpu_seconds=(t1 - t0) * n_threads,
                 max_rel_err_fp32_vs_oracle=err, n=len(jobs))
+
+
+def oracle_work(reqs, n_threads):
+    """The reference algorithm's work counters summed over every request of
+    the batch (oracle restatement, counting only)."""
+    import oracle
+    import relxill_amd as rx
+    rel, lp = rx.tables_view()
+    T = oracle.Tables(rel, lp)
+    jobs = [(r["model"], r["params"], r["energy"], r.get("flux_in")) for r in reqs]
+    _, _, stats = oracle.eval_batch_mt(T, jobs, n_threads, stats=True)
+    return stats
+
+
+def oracle_errors(reqs, outs, n_threads):
+    """max relative error per bin of outs against the FP64 oracle on the same
+    requests"""
+    import oracle
+    import relxill_amd as rx
+    rel, lp = rx.tables_view()
+    T = oracle.Tables(rel, lp)
+    jobs = [(r["model"], r["params"], r["energy"], r.get("flux_in")) for r in reqs]
+    ref, st = oracle.eval_batch_mt(T, jobs, n_threads)
+    assert all(x == 0 for x in st)
+    return max(rel_err(outs[i], ref[i]) for i in range(len(reqs)))
 
 
 def main():
@@ -441,6 +529,7 @@ def main():
     o2_64 = [b2.output(i)[0] for i in range(1024)]
     err2 = max(rel_err(o2_32[i], o2_64[i]) for i in range(1024))
     cfg2 = dict(spectra_per_s=1024 / dt2, ms_per_step=dt2 * 1e3, max_rel_err_fp32_vs_fp64=err2)
+    cfg2_reqs = make_cfg2(1024)
     log(f"cfg2 lp relline fp32: {cfg2}")
     b2.close()
 
@@ -453,13 +542,20 @@ def main():
     relxilllp = relxilllp_batch(rx, args.steps)
     log(f"relxilllp batch: {relxilllp}")
 
-    serving = serving_high = None
+    serving = serving_load = None
     if not args.no_serve:
-        qps = float(min(4000.0, 0.4 * value))
-        serving = serve(rx, args.serve_seconds, qps)
-        log(f"serving: {serving}")
-        serving_high = serve(rx, args.serve_seconds, 20000.0, seed=16)
-        log(f"serving at 20k QPS offered: {serving_high}")
+        pool = serving_pool()
+        serving = serve(rx, pool, 4000.0, seconds=args.serve_seconds)
+        log(f"serving at 4k QPS: {serving}")
+        serving_load = []
+        for q in (20000.0, 100000.0):
+            r = serve(rx, pool, q, seconds=min(args.serve_seconds, 2.0), seed=16)
+            log(f"serving at {q:.0f} QPS: {r}")
+            serving_load.append(r)
+        # saturation: far more offered than the server can take
+        r = serve(rx, pool, 2.0e6, n_total=60000, seed=26)
+        log(f"serving saturated (2M QPS offered): {r}")
+        serving_load.append(r)
 
     # roofline of the dominant kernel (relline integration): algorithmic bytes
     # per spectrum = table corners 68,800 B + grid/in/out 16 N + 8 B
@@ -487,19 +583,48 @@ def main():
                     kernel_ms_all={k: round(v, 4) for k, v in zip(KERNELS, kms32)})
 
     cpu = None
+    host = host_cpus()
+    max_err_oracle = None
     if not args.no_cpu:
-        ncpu = min(16, os.cpu_count() or 1)
+        ncpu = cpu_threads(host)
         sample = list(range(min(args.cpu_sample, len(reqs))))
         cb = cpu_baseline([reqs[i] for i in sample], ncpu, [out32[i] for i in sample])
-        log(f"cpu baseline: {cb}")
+        log(f"cpu baseline: {cb} host {host}")
+        ncores = host["physical_cores"] or host["nproc"]
         cpu = dict(value=round(cb["all_cores"], 2), unit="spectra/s", cores=ncpu, kind="port",
-                   sample=f"first {cb['n']} requests of the same batch, FP64 oracle, "
-                          f"{cb['cpu_seconds']:.1f} s of CPU work; single core "
-                          f"{cb['single_core']:.2f} spectra/s",
-                   single_core=round(cb["single_core"], 3))
+                   sample=f"first {cb['n']} requests of the same batch, FP64 oracle on {ncpu} threads "
+                          f"(the CPUs this process is granted), {cb['cpu_seconds']:.1f} s of CPU work; "
+                          f"single core {cb['single_core']:.2f} spectra/s",
+                   single_core=round(cb["single_core"], 3),
+                   host=dict(cpu_model=host["model"], nproc=host["nproc"], physical_cores=host["physical_cores"],
+                             affinity=host["affinity"], cgroup_quota=host["cgroup_quota"]),
+                   all_host_cores_extrapolated=round(cb["single_core"] * ncores, 1))
         max_err_oracle = cb["max_rel_err_fp32_vs_oracle"]
-    else:
-        max_err_oracle = None
+        # cfg2: a sample of the lamp-post batch against the oracle
+        n2 = 64
+        cfg2["max_rel_err_fp32_vs_oracle"] = oracle_errors(cfg2_reqs[:n2], o2_32[:n2], ncpu)
+        cfg2["oracle_sample"] = n2
+        log(f"cfg2 fp32 vs oracle on {n2} requests: {cfg2['max_rel_err_fp32_vs_oracle']:.3e}")
+        # algorithmic FLOPs of the integration: the reference algorithm's
+        # integrand evaluations over the whole batch (20 FLOP each: the g*
+        # map, the node lerp, E^3, the 1/sqrt(g*(1-g*)) weight and the
+        # products, with sqrt and division counted as one FLOP)
+        work = oracle_work(reqs, ncpu)
+        flops = 20.0 * work["relb_func_calls"]
+        t_int = kms32[1] * 1e-3
+        roofline_valu = dict(algorithmic_flops=flops, integrand_evaluations=work["relb_func_calls"],
+                             flop_per_evaluation=20, achieved=round(flops / t_int / 1e12, 4),
+                             peak=VALU_PEAK_FP32_TFLOPS, unit="TFLOP/s",
+                             frac=flops / t_int / 1e12 / VALU_PEAK_FP32_TFLOPS,
+                             frac_of_fp64_peak=flops / t_int / 1e12 / VALU_PEAK_FP64_TFLOPS)
+        roofline = dict(bound="valu", achieved=roofline_valu["achieved"], peak=VALU_PEAK_FP32_TFLOPS,
+                        unit="TFLOP/s", frac=roofline_valu["frac"], traffic=roofline["traffic"],
+                        kernel=roofline["kernel"], kernel_ms=roofline["kernel_ms"],
+                        valu=roofline_valu, hbm={k: roofline[k] for k in ("achieved", "peak", "unit", "frac",
+                                                                          "traffic", "traffic_source",
+                                                                          "algorithmic_bytes")},
+                        valu_busy=roofline["valu_busy"], kernel_ms_all=roofline["kernel_ms_all"])
+        log(f"roofline: {roofline}")
 
     res = {
         "metric": "spectra/sec batched FP32 vs plain FP64 on 1 GPU @<=1e-4 rel err; p99 latency",
@@ -517,7 +642,7 @@ def main():
         "config": {
             "workload": "uneven relconv batch: 4096 requests, own log grids of 300-8000 bins "
                         "(log-uniform) over 0.1-100 keV, 4096-bin convolution grid 0.00035-2000 keV, "
-                        "25x22x100x20 relline table, FP32 + Kahan",
+                        "25x22x100x20 relline table, FP32 integrand with FP64 sums over radii (in place of Kahan)",
             "n_requests": len(reqs), "mean_bins": float(nbins.mean()), "seg_bins": seg,
             "waves_per_cu": wpc,
         },
@@ -533,13 +658,15 @@ def main():
         "relxill_batch": relxill,
         "relxilllp_batch": relxilllp,
         "serving": serving,
-        "serving_20k_qps": serving_high,
+        "serving_load_curve": serving_load,
+        "serving_saturation_spectra_per_s": serving_load[-1]["spectra_per_s"] if serving_load else None,
         "p99_latency_ms": serving["p99_ms"] if serving else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     if cpu:
         res["ratio_vs_cpu_all_cores"] = round(value / cpu["value"], 1)
+        res["ratio_vs_cpu_all_host_cores_extrapolated"] = round(value / cpu["all_host_cores_extrapolated"], 1)
     print(json.dumps(res))
 
 

@@ -29,7 +29,7 @@ extern "C" {
 /* ---- status codes ---- */
 #define RX_OK 0
 #define RX_ERR_PARAM 1        /* parameters rejected (check_parameter_bounds) */
-#define RX_ERR_UNSUPPORTED 2  /* feature out of scope (relxilllp's returning radiation, extended jet) */
+#define RX_ERR_UNSUPPORTED 2  /* feature out of scope (extended jet) */
 #define RX_ERR_TABLE 10       /* table interpolation failed (grid invariants) */
 #define RX_ERR_INPUT 50       /* relconv input flux with sum <= 0 (src/LocalModel.cpp:84-86) */
 #define RX_ERR_DEVICE 100     /* HIP error / no device */
@@ -98,7 +98,10 @@ void lmodrelxill(const double *energy, int Nflux, const double *parameter, int s
  * zone; the reflection scaled by |refl_frac| over the lamp post's reflection
  * fraction (by |refl_frac| alone with switch_reflfrac_boost = 1) and the
  * primary boosted by f_inf / 0.5 g^Gamma (src/PrimarySource.cpp:66-125).
- * switch_returnrad must be 0 (RX_ERR_UNSUPPORTED otherwise). */
+ * switch_returnrad 1 (the lmodel.dat default) adds the returning radiation
+ * to the emissivity, above spin 0 corrected by each zone's xillver flux and
+ * energy-shift factors (src/Relxill.cpp:337-344, src/Xillspec.cpp:500-526);
+ * 2 and -1 use the returning part alone. */
 void lmodrelxilllp(const double *energy, int Nflux, const double *parameter, int spectrum,
                    double *flux, double *fluxError, const char *init);
 
@@ -111,6 +114,7 @@ int rx_eval_model(int model, const double *parameter, const double *energy, doub
 int rx_last_status(void);
 const char *rx_last_error(void);
 
+
 /* Processed parameters of one XSPEC parameter array (get_rel_params with
  * check_parameter_bounds, src/ModelDefinition.cpp:168-334): out12 receives
  * a, incl[rad], Index1, Index2, Rbr, Rin, Rout, lineE, z, h, gamma, beta. */
@@ -121,12 +125,15 @@ int rx_model_params(int model, const double *parameter, double *out12);
 /* ===================================================================== */
 
 /* Selects the device and uploads the tables (lazily done by the first call
- * otherwise).  Tables: in RELXILL_TABLE_PATH the reference's FITS files
- * rel_table_v0.5a.fits and rel_lp_table_v0.5b.fits (src/common.h:73,80) when
- * present and cfitsio can be loaded, else rel_table.rxtb and
- * rel_lp_table.rxtb (own binary format), else the synthetic tables of the
- * given seeds at the reference dims used by the benchmarks (25x22x100x20
- * relline, 20x100x100 lamp post). */
+ * otherwise).  Tables: with RELXILL_TABLE_PATH set, the reference's FITS
+ * files rel_table_v0.5a.fits and rel_lp_table_v0.5b.fits (src/common.h:73,80)
+ * in that directory, else rel_table.rxtb and rel_lp_table.rxtb (own binary
+ * format) there; a table that is missing or cannot be read there makes every
+ * request that needs it fail with RX_ERR_TABLE and the reason in
+ * rx_last_error(), as the reference fails when it cannot open a table
+ * (src/reltable.c:255-268).  With RELXILL_TABLE_PATH unset: seeded
+ * synthetic tables at the dims the benchmarks use (25x22x100x20 relline,
+ * 20x100x100 lamp post). */
 int rx_init(int device);
 void rx_shutdown(void);
 
@@ -279,6 +286,15 @@ int rx_batch_kernel_ms(rx_batch *b, double *ms);
  * FP64).  No counterpart in the reference (diagnostics of the batched path). */
 int rx_batch_counters(rx_batch *b, int64_t *out);
 
+/* Launch graphs (default on): a batch run whose kernel arguments and grid
+ * shapes equal the previous run's is captured into a hipGraph and replayed
+ * while they stay equal (the XSPEC entry points reuse one batch, so repeated
+ * calls on one grid replay a graph).  rx_set_graphs(0) launches every kernel
+ * directly.  rx_batch_graph_replays: runs of the batch served by a graph.
+ * No counterpart in the reference (launch mechanics of the device path). */
+int rx_set_graphs(int on);
+int64_t rx_batch_graph_replays(rx_batch *b);
+
 /* Autotuner: times candidate work shapes (segment bins x waves per CU) for
  * this batch on the device and keeps the fastest for batches of the same
  * shape class.  Returns 256 * seg_bins + waves_per_cu (>0) or -status. */
@@ -295,6 +311,12 @@ rx_server *rx_server_start(int max_inflight, int64_t hbm_budget_bytes, int n_str
                            int precision);
 /* Submits a request (copied); returns a ticket >= 0 or a negative status. */
 int64_t rx_server_submit(rx_server *s, const rx_request *req);
+/* Load generator for measuring the server itself: submits req[i] at
+ * arrival_s[i] seconds after the call (ascending), paced by a native loop so
+ * that the offered rate does not depend on the caller's language; tickets[i]
+ * receives the ticket (or a negative status).  Returns once the last request
+ * is submitted; poll or drain for the results. */
+int rx_server_replay(rx_server *s, const rx_request *req, int n, const double *arrival_s, int64_t *tickets);
 /* Non-blocking poll: 1 done (flux copied, *status set, *latency_ms set),
  * 0 pending, <0 unknown ticket. */
 int rx_server_poll(rx_server *s, int64_t ticket, double *flux, int *status, double *latency_ms);

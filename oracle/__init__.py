@@ -50,7 +50,8 @@ class RelParam(ctypes.Structure):
                 ("rbr", ctypes.c_double), ("rin", ctypes.c_double), ("rout", ctypes.c_double),
                 ("lineE", ctypes.c_double), ("z", ctypes.c_double), ("height", ctypes.c_double),
                 ("htop", ctypes.c_double), ("gamma", ctypes.c_double), ("beta", ctypes.c_double),
-                ("limb", _I), ("num_zones", _I), ("return_rad", _I), ("do_renorm", _I)]
+                ("limb", _I), ("num_zones", _I), ("return_rad", _I), ("do_renorm", _I),
+                ("rr_corr_flux", _D), ("rr_corr_gshift", _D), ("rr_corr_rgrid", _D), ("rr_corr_n", _I)]
 
 
 class Stats(ctypes.Structure):
@@ -125,6 +126,8 @@ def _load():
         "orc_xill_renorm": (None, [_F, _I, ctypes.c_double, ctypes.c_double]),
         "orc_xill_spectra": (None, [ctypes.POINTER(XillTable), ctypes.POINTER(XillParam), _D, _D]),
         "orc_xill_angdep": (None, [_D, _D, _D, _I, _I]),
+        "orc_rrad_gshift_boost": (ctypes.c_double, [ctypes.c_double] * 3),
+        "orc_xill_corr_factors": (None, [ctypes.POINTER(XillTable), ctypes.POINTER(XillParam), _D, _D]),
         "orc_spec_cutoffpl": (None, [_D, _D, _I, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
         "orc_norm_wrt_xillver": (ctypes.c_double, [_D, _D, _I, ctypes.POINTER(_I)]),
         "orc_primary_norm_factor": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
@@ -140,6 +143,8 @@ def _load():
                                 _D, _I, _D, _I, _I, ctypes.POINTER(Stats)]),
         "orc_eval_batch_mt": (_I, [ctypes.POINTER(RelTable), ctypes.POINTER(LpTable),
                                    ctypes.POINTER(Job), _I, _D, _I, _I]),
+        "orc_eval_batch_mt_stats": (_I, [ctypes.POINTER(RelTable), ctypes.POINTER(LpTable),
+                                         ctypes.POINTER(Job), _I, _D, _I, _I, ctypes.POINTER(Stats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -326,8 +331,9 @@ def relxill_conv(tables, params, conv_grid, refl_ener, refl, out_ener, nr_fine=1
     return out, dist, st
 
 
-def eval_batch_mt(tables, jobs, n_threads, conv_grid=None):
-    """jobs: list of (model, params, energy, flux_in or None); returns (fluxes, statuses)."""
+def eval_batch_mt(tables, jobs, n_threads, conv_grid=None, stats=False):
+    """jobs: list of (model, params, energy, flux_in or None); returns (fluxes,
+    statuses[, work counters summed over the jobs])."""
     arr = (Job * len(jobs))()
     keep = []
     outs = []
@@ -339,9 +345,13 @@ def eval_batch_mt(tables, jobs, n_threads, conv_grid=None):
         outs.append(fa)
         arr[i] = Job(m, _dp(pa), _dp(ea), _dp(fa), ea.size - 1, 0)
     cg = _arr(conv_grid) if conv_grid is not None else None
-    lib.orc_eval_batch_mt(ctypes.byref(tables.rel), tables.lp_ptr, arr, len(jobs),
-                          _dp(cg) if cg is not None else None,
-                          (cg.size - 1) if cg is not None else 0, n_threads)
+    so = Stats()
+    lib.orc_eval_batch_mt_stats(ctypes.byref(tables.rel), tables.lp_ptr, arr, len(jobs),
+                                _dp(cg) if cg is not None else None,
+                                (cg.size - 1) if cg is not None else 0, n_threads,
+                                ctypes.byref(so) if stats else None)
+    if stats:
+        return outs, [arr[i].status for i in range(len(jobs))], {k: getattr(so, k) for k, _ in Stats._fields_}
     return outs, [arr[i].status for i in range(len(jobs))]
 
 
@@ -394,6 +404,13 @@ def xill_spectra(xt, xp):
     flu = np.zeros(xt.n_incl * n)
     lib.orc_xill_spectra(ctypes.byref(xt.t), ctypes.byref(xp), _dp(e), _dp(flu))
     return e, flu.reshape(xt.n_incl, n)
+
+
+def xill_corr_factors(xt, xp):
+    """(corr_flux, corr_gshift) of get_xillver_fluxcorrection_factors"""
+    a, b = np.zeros(1), np.zeros(1)
+    lib.orc_xill_corr_factors(ctypes.byref(xt.t), ctypes.byref(xp), _dp(a), _dp(b))
+    return float(a[0]), float(b[0])
 
 
 def cutoffpl(ener, ect, gam, shift=1.0):

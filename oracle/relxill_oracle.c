@@ -767,8 +767,27 @@ int orc_rrad_emis(const orc_rradtable *rt, const orc_relparam *p, const double *
   for (int i = 0; i < n - 2; i++) tfr[i * n + n - 1] *= corr[1];
   /* the direct emissivity at the zone centres (inv_rebin_mean, src/relutility.c:636-663) */
   inv_rebin_mean(re, emis0, nr, rad, emis_in, n);
+  /* relxilllp's correction factors of the zone each table zone's centre
+   * falls in (rebin_corrfactors_to_rradtable_grid with rebin_to_grid,
+   * src/Relreturn_Corona.cpp:248-289; a centre beyond the last edge takes the
+   * last zone, where the reference would read one past its array) */
+  double *cf_flux = (double *)malloc(sizeof(double) * n), *cf_g = (double *)malloc(sizeof(double) * n);
+  for (int i = 0; i < n; i++) {
+    cf_flux[i] = 1.0;
+    cf_g[i] = 1.0;
+    if (p->rr_corr_n > 0) {
+      const int n0 = p->rr_corr_n;
+      int ind = orc_binary_search(p->rr_corr_rgrid, n0 + 1, rad[i]);
+      if (rad[i] < p->rr_corr_rgrid[0]) ind = 0;
+      else if (rad[i] > p->rr_corr_rgrid[n0] || ind > n0 - 1) ind = n0 - 1;
+      cf_flux[i] = p->rr_corr_flux[ind];
+      cf_g[i] = p->rr_corr_gshift[ind];
+    }
+  }
   /* calc_rrad_emis_corona (src/Relreturn_Corona.cpp:119-151) with
-   * calc_rrad_emis_zone (:87-108): sum over g of frac_g g^(Gamma-1) */
+   * calc_rrad_emis_zone (:87-108): sum over g of frac_g g^(Gamma-1), or with
+   * a correction factor of the emitting zone off 1 by more than 1e-3 of
+   * frac_g times its corrected boost over g */
   for (int i = 0; i < n; i++) {
     for (int j = 0; j < n; j++) {
       const size_t q = (size_t)k * nt * nt + (size_t)(klo + i) * nt + (klo + j);
@@ -777,15 +796,17 @@ int orc_rrad_emis(const orc_rradtable *rt, const orc_relparam *p, const double *
       for (int jj = 0; jj < ng; jj++) {
         const double g = (jj + 0.5) / ng * (gmax - gmin) + gmin;  /* get_gfac_grid, src/Relreturn_Datastruct.cpp:51-59 */
         double v = rt->frac_g[q * ng + jj];
-        if (fabs(g - 1) > 1e-3) v *= pow(g, p->gamma - 1);
+        if (fabs(cf_g[j] - 1) > 1e-3) v *= orc_rrad_gshift_boost(cf_g[j], g, p->gamma) / g;
+        else if (fabs(g - 1) > 1e-3) v *= pow(g, p->gamma - 1);
         ez += v;
       }
       term[j] = ez * tfr[i * n + j] * emis_in[j];
     }
     double sum = 0.0;
     for (int j = 0; j < n; j++) sum += term[j];
-    ret[i] = sum;
+    ret[i] = sum * cf_flux[i];
   }
+  free(cf_flux); free(cf_g);
   /* back to the fine grid (rebin_emisprofile_on_radial_grid, src/Rellp.cpp:120-175;
    * the profile's emission angles are 0, so the radius factor is linear and
    * the emissivity log-interpolated) */
@@ -1376,6 +1397,82 @@ void orc_xill_angdep(double *out, const double *flu, const double *dist, int n_i
     for (int jj = 0; jj < n_ener; jj++) out[jj] += dist[ii] * flu[(size_t)ii * n_ener + jj];
 }
 
+/* corrected_gshift_fluxboost_factor; src/Relreturn_Corona.cpp:39-83 */
+double orc_rrad_gshift_boost(double c, double g, double gamma) {
+  const double g0 = 2. / 3;
+  double corr;
+  if (c < 1) { /* parabola through f(0) = 0 */
+    const double a = (c / g0 - 1) / (g0 - 1);
+    const double b = 1 - a;
+    corr = (g >= 1) ? 1. / g * (1. / g * a + b) : g * (g * a + b);
+  } else { /* linear */
+    const double alin = (c - 1) / (g0 - 1);
+    const double blin = 1 - alin;
+    corr = (g >= 1) ? (1. / g * alin + blin) : (g * alin + blin);
+  }
+  double boost = pow(g, gamma) * corr;
+  if (g < 1 && boost > 1) boost = 1;
+  if (boost < 0) boost = 0;
+  return boost;
+}
+
+/* get_energy_flux_band / get_photon_flux_band; src/Xillspec.cpp:132-162 */
+static double energy_flux_band(const double *ener, const double *f, int n, double emin, double emax) {
+  double sum = 0.0;
+  for (int ii = 0; ii < n; ii++)
+    if (ener[ii] >= emin && ener[ii + 1] <= emax) sum += f[ii] * 0.5 * (ener[ii] + ener[ii + 1]);
+  return sum;
+}
+
+static double photon_flux_band(const double *ener, const double *f, int n, double emin, double emax) {
+  double sum = 0.0;
+  for (int ii = 0; ii < n; ii++)
+    if (ener[ii] >= emin && ener[ii + 1] <= emax) sum += f[ii];
+  return sum;
+}
+
+/* get_xillver_fluxcorrection_factors; src/Xillspec.cpp:500-526, with
+ * calc_angle_averaged_xill_spec (:109-126), get_xillver_fluxcorr (:344-362)
+ * and get_xillver_gshift_fluxcorr (:454-487) */
+void orc_xill_corr_factors(const orc_xilltable *t, const orc_xillparam *x, double *corr_flux, double *corr_gshift) {
+  const int nx = t->n_ener, n_incl = t->n_val[t->n_par - 1];
+  const float *incl = t->val[t->n_par - 1];
+  double *e = (double *)malloc(sizeof(double) * (nx + 1));
+  double *flu = (double *)malloc(sizeof(double) * (size_t)n_incl * nx);
+  double *avg = (double *)calloc(nx, sizeof(double));
+  orc_xill_spectra(t, x, e, flu);
+  /* the angle average: the weight of an inclination is
+   * norm_factor_semi_infinite_slab of the table's value (in degrees, as
+   * the xillver spectrum keeps it, src/xilltable.c:1113) times 180/pi,
+   * i.e. 0.5 cos of the degree value read as radians, over n_incl */
+  for (int ii = 0; ii < n_incl; ii++) {
+    const double incl_deg = (double)incl[ii] * 180 / M_PI;
+    const double w = 0.5 * cos(incl_deg * M_PI / 180) / n_incl;
+    for (int jj = 0; jj < nx; jj++) avg[jj] += w * flu[(size_t)ii * nx + jj];
+  }
+  /* reflected over direct energy flux in 0.1-1000 keV, the direct spectrum
+   * the normalised cut-off power law on the coarse grid */
+  double coarse[N_ENER_COARSE + 1], prim[N_ENER_COARSE];
+  orc_get_log_grid(coarse, N_ENER_COARSE + 1, EMIN_XILL_NORM, EMAX_XILL_NORM);
+  orc_spec_cutoffpl(prim, coarse, N_ENER_COARSE, x->ect, x->gam, 1.0);
+  int st = 0;
+  const double nf = 1. / orc_norm_wrt_xillver(prim, coarse, N_ENER_COARSE, &st);
+  for (int ii = 0; ii < N_ENER_COARSE; ii++) prim[ii] *= nf;
+  *corr_flux = energy_flux_band(e, avg, nx, 0.1, 1000) / energy_flux_band(coarse, prim, N_ENER_COARSE, 0.1, 1000);
+  /* photon flux of the spectrum over that of the spectrum shifted to 1.5
+   * times lower energies (rebinned, times 2/3 for the time dilation), in
+   * 0.15-500 keV, over the power law's 1.5^Gamma */
+  const double gref = 2. / 3.;
+  double *ez = (double *)malloc(sizeof(double) * (nx + 1));
+  double *fz = (double *)malloc(sizeof(double) * nx);
+  for (int ii = 0; ii <= nx; ii++) ez[ii] = e[ii] / gref;
+  orc_rebin_spectrum(ez, fz, nx, e, avg, nx);
+  for (int ii = 0; ii < nx; ii++) fz[ii] *= gref;
+  const double ratio = photon_flux_band(e, avg, nx, 0.15, 500.0) / photon_flux_band(e, fz, nx, 0.15, 500.0);
+  *corr_gshift = ratio / pow(1.5, x->gam);
+  free(e); free(flu); free(avg); free(ez); free(fz);
+}
+
 void orc_spec_cutoffpl(double *flux, const double *ener, int n, double ect, double gam, double shift) {
   const double ecut = ect * shift;
   for (int ii = 0; ii < n; ii++) {
@@ -1445,9 +1542,8 @@ int orc_relxill_model(int model_type, const orc_reltable *tab, const orc_lptable
   orc_relparam p;
   orc_xillparam x;
   int status = orc_get_relxill_params(model_type, parameter, phys_norm_env, &p, &x);
-  /* relxill models' returning radiation needs the xillver correction factors
-   * (calc_rrad_corr_factors, src/Relxill.cpp:337-341), which are not restated */
-  if (status == 0 && p.return_rad != 0) status = 2;
+  if (status == 0 && p.return_rad != 0 && !(p.return_rad == 1 || p.return_rad == -1 || p.return_rad == 2))
+    status = 34;
   if (status) {
     memset(flux, 0, sizeof(double) * n_flux);
     return status;
@@ -1485,6 +1581,7 @@ int orc_relxill_model(int model_type, const orc_reltable *tab, const orc_lptable
   orc_xillparam xs = x;
   if (p.emis_type == ORC_EMIS_LP) xs.ect /= g_so;
   orc_syspar *sp = orc_get_system_parameters(tab, lp, &p, 1000, &status);
+  double *cf_flux = (double *)malloc(sizeof(double) * nz), *cf_g = (double *)malloc(sizeof(double) * nz);
   if (sp) {
     /* 1: zones (RadialGrid, src/IonGradient.cpp:457-489; IonGradient.h:63-70)
      * with their del_emit and source -> disk energy shift (:182-193, :421-430) */
@@ -1498,6 +1595,26 @@ int orc_relxill_model(int model_type, const orc_reltable *tab, const orc_lptable
       inv_rebin_mean(sp->re, sp->del_emit, sp->nr, rmean, del_emit, nz);
       for (int iz = 0; iz < nz; iz++) shift[iz] = orc_gi_potential_lp(rmean[iz], p.a, p.height, p.beta, del_emit[iz]);
     }
+    /* 3: returning radiation above spin 0 (SPIN_MIN_RRAD_CALC_CORRFAC,
+     * src/Relxill.h:32): the xillver correction factors of every zone from
+     * its reflection spectra (calc_rrad_corr_factors, src/Relxill.cpp:164-182,
+     * :337-341), then the emissivity again with them (:344); the zones'
+     * angles and shifts do not depend on the emissivity */
+    if (p.return_rad != 0 && p.a > 0.0) {
+      for (int iz = 0; iz < nz; iz++) {
+        orc_xillparam xz = xs;
+        xz.ect = xs.ect * shift[iz];
+        orc_xill_corr_factors(xt, &xz, &cf_flux[iz], &cf_g[iz]);
+      }
+      p.rr_corr_flux = cf_flux;
+      p.rr_corr_gshift = cf_g;
+      p.rr_corr_rgrid = rgrid;
+      p.rr_corr_n = nz;
+      orc_syspar_free(sp);
+      sp = orc_get_system_parameters(tab, lp, &p, 1000, &status);
+    }
+  }
+  if (sp) {
     /* 4: the relline profile of every zone with its angle distribution */
     status = orc_calc_relline_profile(sp, conv_ener, n_conv, rgrid, nz, rel, dist, n_incl, st);
     if (status == 0) orc_renorm_relline_profile(conv_ener, n_conv, nz, rel, dist, n_incl, &p);
@@ -1551,6 +1668,7 @@ int orc_relxill_model(int model_type, const orc_reltable *tab, const orc_lptable
     memset(flux, 0, sizeof(double) * n_flux);
   }
   if (sp) orc_syspar_free(sp);
+  free(cf_flux); free(cf_g);
   free(grid); free(ener); free(xe); free(xflu); free(xill); free(rel); free(dist);
   free(xill_rb); free(conv_out); free(total); free(prim); free(rgrid); free(rmean); free(del_emit);
   free(shift);
@@ -1655,6 +1773,7 @@ typedef struct {
   int n_conv;
   int *next;
   pthread_mutex_t *mu;
+  orc_stats *total; /* optional: work counters summed over the jobs */
 } mt_ctx;
 
 static void *mt_worker(void *arg) {
@@ -1665,19 +1784,32 @@ static void *mt_worker(void *arg) {
     pthread_mutex_unlock(c->mu);
     if (i >= c->n_jobs) break;
     orc_job *j = &c->jobs[i];
+    orc_stats st;
+    memset(&st, 0, sizeof(st));
     j->status = orc_eval_model(j->model_type, c->tab, c->lp, j->parameter, j->energy, j->flux,
-                               j->n_flux, c->conv_ener, c->n_conv, 0, NULL);
+                               j->n_flux, c->conv_ener, c->n_conv, 0, c->total ? &st : NULL);
+    if (c->total) {
+      pthread_mutex_lock(c->mu);
+      c->total->relb_func_calls += st.relb_func_calls;
+      c->total->romberg_calls += st.romberg_calls;
+      c->total->romberg_levels += st.romberg_levels;
+      c->total->edge_calls += st.edge_calls;
+      c->total->midpoint_calls += st.midpoint_calls;
+      c->total->bin_integrals += st.bin_integrals;
+      pthread_mutex_unlock(c->mu);
+    }
   }
   return NULL;
 }
 
-int orc_eval_batch_mt(const orc_reltable *tab, const orc_lptable *lp, orc_job *jobs, int n_jobs,
-                      const double *conv_ener, int n_conv, int n_threads) {
+int orc_eval_batch_mt_stats(const orc_reltable *tab, const orc_lptable *lp, orc_job *jobs, int n_jobs,
+                            const double *conv_ener, int n_conv, int n_threads, orc_stats *total) {
   if (n_threads < 1) n_threads = 1;
   int next = 0;
   pthread_mutex_t mu;
   pthread_mutex_init(&mu, NULL);
-  mt_ctx ctx = {tab, lp, jobs, n_jobs, conv_ener, n_conv, &next, &mu};
+  if (total) memset(total, 0, sizeof(*total));
+  mt_ctx ctx = {tab, lp, jobs, n_jobs, conv_ener, n_conv, &next, &mu, total};
   pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * n_threads);
   for (int t = 0; t < n_threads; t++) pthread_create(&th[t], NULL, mt_worker, &ctx);
   for (int t = 0; t < n_threads; t++) pthread_join(th[t], NULL);
@@ -1686,4 +1818,9 @@ int orc_eval_batch_mt(const orc_reltable *tab, const orc_lptable *lp, orc_job *j
   int bad = 0;
   for (int i = 0; i < n_jobs; i++) bad += (jobs[i].status != 0);
   return bad;
+}
+
+int orc_eval_batch_mt(const orc_reltable *tab, const orc_lptable *lp, orc_job *jobs, int n_jobs,
+                      const double *conv_ener, int n_conv, int n_threads) {
+  return orc_eval_batch_mt_stats(tab, lp, jobs, n_jobs, conv_ener, n_conv, n_threads, NULL);
 }

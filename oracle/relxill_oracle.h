@@ -92,6 +92,12 @@ typedef struct {
   int num_zones;
   int return_rad;
   int do_renorm;
+  /* relxilllp returning radiation: the xillver flux and energy-shift
+   * correction factors of the rr_corr_n radial zones with edges
+   * rr_corr_rgrid[rr_corr_n+1] (rrad_corr_factors, src/common.h:213);
+   * rr_corr_n = 0: no correction */
+  const double *rr_corr_flux, *rr_corr_gshift, *rr_corr_rgrid;
+  int rr_corr_n;
 } orc_relparam;
 
 /* reflection fractions (src/common.h:279-285) */
@@ -237,6 +243,11 @@ void orc_xill_renorm(float *spec, int n, double lxi, double dens);
 /* get_xillver_spectra_table for a relxill model (src/Xillspec.cpp:58-88):
  * ener[n_ener+1] (elo, then the last ehi), flu[n_incl][n_ener] */
 void orc_xill_spectra(const orc_xilltable *t, const orc_xillparam *x, double *ener, double *flu);
+/* get_xillver_fluxcorrection_factors (src/Xillspec.cpp:500-526) of the
+ * xillver spectra at the given parameters: *corr_flux (reflected over
+ * incident energy flux in 0.1-1000 keV) and *corr_gshift (photon-flux boost
+ * ratio to a power law at g = 1.5) */
+void orc_xill_corr_factors(const orc_xilltable *t, const orc_xillparam *x, double *corr_flux, double *corr_gshift);
 /* calc_xillver_angdep (src/Xillspec.cpp:557-573) */
 void orc_xill_angdep(double *out, const double *flu, const double *dist, int n_incl, int n_ener);
 /* spec_cutoffpl (src/Xillspec.cpp:215-241) */
@@ -291,6 +302,13 @@ typedef struct {
 } orc_job;
 int orc_eval_batch_mt(const orc_reltable *tab, const orc_lptable *lp, orc_job *jobs, int n_jobs,
                       const double *conv_ener, int n_conv, int n_threads);
+/* corrected_gshift_fluxboost_factor (src/Relreturn_Corona.cpp:39-83): the flux
+ * boost of the xillver reflection under an energy shift g, from its boost
+ * ratio c to a power law at g = 1.5 */
+double orc_rrad_gshift_boost(double c, double g, double gamma);
+/* the same, summing the jobs' work counters into *total (when not NULL) */
+int orc_eval_batch_mt_stats(const orc_reltable *tab, const orc_lptable *lp, orc_job *jobs, int n_jobs,
+                            const double *conv_ener, int n_conv, int n_threads, orc_stats *total);
 
 #ifdef __cplusplus
 }

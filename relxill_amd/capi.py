@@ -138,10 +138,14 @@ _EXPORTS = {
     "rx_batch_kernel_ms": (ctypes.c_int, [ctypes.c_void_p, _D]),
     "rx_batch_counters": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "rx_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "rx_set_graphs": (ctypes.c_int, [ctypes.c_int]),
+    "rx_batch_graph_replays": (ctypes.c_int64, [ctypes.c_void_p]),
     "rx_server_start": (ctypes.c_void_p, [ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int]),
     "rx_server_submit": (ctypes.c_int64, [ctypes.c_void_p, ctypes.POINTER(_Request)]),
     "rx_server_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, _D,
                                       ctypes.POINTER(ctypes.c_int), _D]),
+    "rx_server_replay": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_Request), ctypes.c_int, _D,
+                                        ctypes.POINTER(ctypes.c_int64)]),
     "rx_server_drain": (ctypes.c_int, [ctypes.c_void_p]),
     "rx_server_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "rx_server_stop": (None, [ctypes.c_void_p]),
@@ -386,6 +390,9 @@ class Batch:
         _check(lib.rx_batch_kernel_ms(self.h, _dp(o)))
         return o
 
+    def graph_replays(self):
+        return int(lib.rx_batch_graph_replays(self.h))
+
     def counters(self):
         """work items, FP64 re-checks and re-check overflow of the last run"""
         o = (ctypes.c_int64 * 3)()
@@ -434,6 +441,38 @@ class Server:
             raise RxError(-t, "submit failed")
         self._nflux[t] = e.size - 1
         return t
+
+    def replay(self, requests, arrival_s):
+        """Submits requests[i] (dicts as for Batch) at arrival_s[i] seconds
+        after the call, paced by the library's native loop; returns the
+        tickets once the last one is submitted."""
+        n = len(requests)
+        arr = (_Request * n)()
+        keep = []
+        for i, r in enumerate(requests):
+            e = _arr(r["energy"])
+            p = _arr(r["params"])
+            keep += [e, p]
+            arr[i].model = r["model"]
+            arr[i].n_flux = e.size - 1
+            arr[i].energy = _dp(e)
+            arr[i].parameter = _dp(p)
+            if r.get("flux_in") is not None:
+                f = _arr(r["flux_in"])
+                keep.append(f)
+                arr[i].flux_in = _dp(f)
+            else:
+                arr[i].flux_in = None
+            arr[i].refl_set = r.get("refl_set", -1)
+        t = np.ascontiguousarray(np.asarray(arrival_s, dtype=np.float64))
+        tickets = (ctypes.c_int64 * n)()
+        _check(lib.rx_server_replay(self.h, arr, n, _dp(t), tickets))
+        out = list(tickets)
+        for i, tk in enumerate(out):
+            if tk < 0:
+                raise RxError(-tk, "submit failed")
+            self._nflux[tk] = arr[i].n_flux
+        return out
 
     def poll(self, ticket):
         """None while pending, else (flux, status, latency_ms)."""

@@ -629,6 +629,18 @@ int rx_batch_kernel_ms(rx_batch *b, double *ms) {
   return b->b.profiled ? RX_OK : RX_ERR_ARG;
 }
 
+int rx_set_graphs(int on) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  g.use_graphs = on != 0;
+  return RX_OK;
+}
+
+int64_t rx_batch_graph_replays(rx_batch *b) {
+  if (!b) return -RX_ERR_ARG;
+  return b->b.graph_replays;
+}
+
 int rx_batch_counters(rx_batch *b, int64_t *out) {
   if (!b || !out) return RX_ERR_ARG;
   for (int k = 0; k < 3; k++) out[k] = b->b.counters[k];

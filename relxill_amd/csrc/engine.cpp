@@ -166,17 +166,16 @@ int parse_request(int model, const double *par, bool phys_norm, ReqParams &p, st
   }
   if (return_rad != 0) {
     // add_returnrad_emis (src/Rellp.cpp:483-507): 1 adds the returning
-    // radiation, -1 and 2 replace the direct emissivity by it
-    if (model == MOD_RELXILLLP) {
-      err = "returning radiation of relxilllp (its xillver correction factors) is not supported";
-      return RX_ERR_UNSUPPORTED;
-    }
+    // radiation, -1 and 2 replace the direct emissivity by it; relxilllp
+    // corrects it with its zones' xillver factors above spin 0
+    // (SPIN_MIN_RRAD_CALC_CORRFAC, src/Relxill.h:32, src/Relxill.cpp:337-341)
     if (!(return_rad == 1 || return_rad == -1 || return_rad == 2)) {
       err = "switch_returnrad must be 0, 1, -1 or 2";
       return RX_ERR_PARAM;
     }
   }
   p.return_rad = return_rad;
+  p.rr_corr = (model == MOD_RELXILLLP && return_rad != 0 && p.a > 0.0) ? 1 : 0;
   p.norm = 1.0;
   if (model < 0 && p.emis_type == EMIS_BKN) {
     // norm_factor_semi_infinite_slab, src/Xillspec.cpp:528-530
@@ -552,6 +551,8 @@ int ensure_xilltable_device(std::string &err) {
   for (int i = 0; i < t.n_ener; i++) g.xill_ener[i] = t.elo[i];
   g.xill_ener[t.n_ener] = t.ehi[t.n_ener - 1];
   int e = upload_vec(g.x_spec, xilltable_renormed(t)) | upload_vec(g.x_ener, g.xill_ener);
+  // the inclination axis (returning-radiation corrections of relxilllp)
+  e |= upload_vec(g.x_incl, t.val[t.n_par - 1]);
   // the Ecut axis (the lamp post's zones bracket their own Ecut on the device)
   // and the axis ensure_ecut_within_boundarys reads
   const int k_ect = t.dim_of(XP_ECT);
@@ -683,6 +684,7 @@ void fill_dev_views(BatchDev &bd) {
     bd.xt.n_q = g.xill.n_val[q];
   }
   bd.xt.ect_vals = g.x_ect.as<float>();
+  bd.xt.incl_vals = g.x_incl.as<float>();
   bd.xt.q_vals = g.x_q.as<float>();
   bd.rr.n_spin = g.rrad.n_spin;
   bd.rr.n_rad = g.rrad.n_rad;
@@ -700,6 +702,8 @@ void fill_dev_views(BatchDev &bd) {
 // ---------------------------------------------------------------------------
 
 Batch::~Batch() {
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (graph) (void)hipGraphDestroy(graph);
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
@@ -727,6 +731,7 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
   b.h_conv.clear();
   b.h_xill.clear();
   b.h_prim.clear();
+  b.h_rrc.clear();
   std::vector<XillWork> xill_multi;
   for (int i = 0; i < n_req; i++) {
     if (is_relxill(req[i].model)) {
@@ -744,14 +749,14 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
   }
   for (int i = 0; i < n_req; i++) {
     const int m = req[i].model;
-    const int k_rr = (m == MOD_RELLINELP) ? 9 : (m == MOD_RELCONVLP) ? 8 : -1;
+    const int k_rr = (m == MOD_RELLINELP) ? 9 : (m == MOD_RELCONVLP) ? 8 : (m == MOD_RELXILLLP) ? 12 : -1;
     if (k_rr >= 0 && req[i].parameter && lround(req[i].parameter[k_rr]) != 0) {
       int st = ensure_rradtable_device(err);
       if (st != RX_OK) return st;
       break;
     }
   }
-  int64_t o_ener = 0, o_flux = 0, o_pgrid = 0, o_prof = 0, o_xspec = 0, o_dist = 0, o_zone = 0;
+  int64_t o_ener = 0, o_flux = 0, o_pgrid = 0, o_prof = 0, o_xspec = 0, o_dist = 0, o_zone = 0, o_rrc = 0;
   for (int i = 0; i < n_req; i++) {
     const rx_request &r = req[i];
     if (r.n_flux <= 0 || !r.energy || !r.parameter) {
@@ -819,6 +824,16 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
       o_pgrid += r.n_flux + 1;
     }
     if (st == RX_OK && p.return_rad != 0) st = rrad_zones(g.rrad, p, g.nr_fine);
+    p.rr_corr_off = 0;
+    if (st == RX_OK && p.rr_corr) {
+      if (g.xill.n_ener > MAX_RRC_ENER) {
+        err = "xillver table has more energy bins than the returning-radiation corrections hold";
+        return RX_ERR_ARG;
+      }
+      p.rr_corr_off = o_rrc;
+      o_rrc += 2 * (int64_t)p.n_zones;
+      for (int z = 0; z < p.n_zones; z++) b.h_rrc.push_back(XillWork{i, z});
+    }
     p.need_cosne = (p.limb != 0 || p.n_incl > 0) ? 1 : 0;
     p.dist_off = o_dist;
     o_dist += (int64_t)p.n_incl * p.n_zones;
@@ -832,6 +847,7 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
   b.tot_xspec = o_xspec;
   b.tot_dist = o_dist;
   b.tot_zone = o_zone;
+  b.tot_rrc = o_rrc;
   b.n_xill1 = (int)b.h_xill.size();
   b.h_xill.insert(b.h_xill.end(), xill_multi.begin(), xill_multi.end());
   b.h_ener.resize(o_ener);
@@ -914,6 +930,8 @@ int batch_upload(Batch &b) {
   e |= b.d_conv.ensure(sizeof(XillWork) * std::max<size_t>(1, b.h_conv.size()));
   e |= b.d_xill.ensure(sizeof(XillWork) * std::max<size_t>(1, b.h_xill.size()));
   e |= b.d_prim.ensure(sizeof(int) * std::max<size_t>(1, b.h_prim.size()));
+  e |= b.d_rrc.ensure(sizeof(XillWork) * std::max<size_t>(1, b.h_rrc.size()));
+  e |= b.d_rrcorr.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_rrc));
   e |= b.d_xspec.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_xspec));
   e |= b.d_items.ensure(sizeof(WorkItem) * (size_t)b.item_cap);
   e |= b.d_ctr.ensure(sizeof(int) * N_COUNTERS);
@@ -930,6 +948,9 @@ int batch_upload(Batch &b) {
   e |= (int)hipMemcpyAsync(b.d_status.p, b.h_status.data(), sizeof(int) * nreq, hipMemcpyHostToDevice, b.stream);
   if (!b.h_conv.empty())
     e |= (int)hipMemcpyAsync(b.d_conv.p, b.h_conv.data(), sizeof(XillWork) * b.h_conv.size(), hipMemcpyHostToDevice,
+                             b.stream);
+  if (!b.h_rrc.empty())
+    e |= (int)hipMemcpyAsync(b.d_rrc.p, b.h_rrc.data(), sizeof(XillWork) * b.h_rrc.size(), hipMemcpyHostToDevice,
                              b.stream);
   if (!b.h_prim.empty())
     e |= (int)hipMemcpyAsync(b.d_prim.p, b.h_prim.data(), sizeof(int) * b.h_prim.size(), hipMemcpyHostToDevice, b.stream);
@@ -971,7 +992,67 @@ static BatchDev dev_view(Batch &b) {
   fill_dev_views(bd);
   bd.xspec = b.d_xspec.as<double>();
   bd.zone = b.d_zone.as<ZoneRec>();
+  bd.rrcorr = b.d_rrcorr.as<double>();
   return bd;
+}
+
+// The run's kernels in stream order (events around them when profiling).
+static int launch_sequence(Batch &b, const RunKey &k, hipStream_t s) {
+  const BatchDev &bd = k.bd;
+  const bool prof = k.prof != 0;
+  int e = 0;
+  if (prof) e |= (int)hipEventRecord(b.ev[0], s);
+  e |= launch_rrad_corr(bd, static_cast<const XillWork *>(k.d_rrc), k.n_rrc, s);
+  e |= launch_setup(bd, s);
+  if (prof) e |= (int)hipEventRecord(b.ev[1], s);
+  e |= launch_integrate(bd, k.n_blocks, k.cos_rows != 0, s);
+  if (prof) e |= (int)hipEventRecord(b.ev[2], s);
+  e |= launch_fixup(bd, k.cos_rows != 0, s);
+  if (prof) e |= (int)hipEventRecord(b.ev[3], s);
+  e |= launch_finalize(bd, s);
+  if (prof) e |= (int)hipEventRecord(b.ev[4], s);
+  e |= launch_xill(bd, static_cast<const XillWork *>(k.d_xill), k.n_xill1, k.n_xill - k.n_xill1, s);
+  if (prof) e |= (int)hipEventRecord(b.ev[5], s);
+  e |= launch_conv(bd, static_cast<const XillWork *>(k.d_conv), k.n_conv, s);
+  if (prof) e |= (int)hipEventRecord(b.ev[6], s);
+  e |= launch_primary(bd, static_cast<const int *>(k.d_prim), k.n_prim, s);
+  if (prof) e |= (int)hipEventRecord(b.ev[7], s);
+  return e;
+}
+
+static void drop_graph(Batch &b) {
+  if (b.gexec) (void)hipGraphExecDestroy(b.gexec);
+  if (b.graph) (void)hipGraphDestroy(b.graph);
+  b.gexec = nullptr;
+  b.graph = nullptr;
+}
+
+// Captures the launch sequence of k into the batch's graph (0 on success).
+// The stream is back to normal whatever happens.
+static int capture_graph(Batch &b, const RunKey &k, hipStream_t s) {
+  drop_graph(b);
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    return -1;
+  }
+  const int e = launch_sequence(b, k, s);
+  hipGraph_t gr = nullptr;
+  const hipError_t ec = hipStreamEndCapture(s, &gr);
+  if (e || ec != hipSuccess || !gr) {
+    if (gr) (void)hipGraphDestroy(gr);
+    (void)hipGetLastError();
+    return -1;
+  }
+  hipGraphExec_t ex = nullptr;
+  if (hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0) != hipSuccess || !ex) {
+    (void)hipGraphDestroy(gr);
+    (void)hipGetLastError();
+    return -1;
+  }
+  b.graph = gr;
+  b.gexec = ex;
+  b.graph_key = k;
+  return 0;
 }
 
 int batch_run(Batch &b, int precision, std::string &err) {
@@ -981,28 +1062,46 @@ int batch_run(Batch &b, int precision, std::string &err) {
     return RX_ERR_ARG;
   }
   b.last_mode = (precision == RX_FP32) ? 1 : 0;
-  BatchDev bd = dev_view(b);
+  RunKey k;
+  memset(&k, 0, sizeof(k));
+  k.bd = dev_view(b);
+  k.n_blocks = g.n_cu * b.waves_per_cu;
+  k.n_xill1 = b.n_xill1;
+  k.n_xill = (int)b.h_xill.size();
+  k.n_conv = (int)b.h_conv.size();
+  k.n_prim = (int)b.h_prim.size();
+  k.d_xill = b.d_xill.p;
+  k.d_conv = b.d_conv.p;
+  k.d_prim = b.d_prim.p;
+  k.d_rrc = b.d_rrc.p;
+  k.n_rrc = (int)b.h_rrc.size();
+  k.prof = g.profile ? 1 : 0;
+  k.cos_rows = b.any_limb ? 1 : 0;
   hipStream_t s = b.stream;
-  const bool prof = g.profile;
-  const bool cos_rows = b.any_limb;
-  const int n_blocks = g.n_cu * b.waves_per_cu;
-  int e = 0;
-  if (prof) e |= (int)hipEventRecord(b.ev[0], s);
-  e |= launch_setup(bd, s);
-  if (prof) e |= (int)hipEventRecord(b.ev[1], s);
-  e |= launch_integrate(bd, n_blocks, cos_rows, s);
-  if (prof) e |= (int)hipEventRecord(b.ev[2], s);
-  e |= launch_fixup(bd, cos_rows, s);
-  if (prof) e |= (int)hipEventRecord(b.ev[3], s);
-  e |= launch_finalize(bd, s);
-  if (prof) e |= (int)hipEventRecord(b.ev[4], s);
-  e |= launch_xill(bd, b.d_xill.as<XillWork>(), b.n_xill1, (int)b.h_xill.size() - b.n_xill1, s);
-  if (prof) e |= (int)hipEventRecord(b.ev[5], s);
-  e |= launch_conv(bd, b.d_conv.as<XillWork>(), (int)b.h_conv.size(), s);
-  if (prof) e |= (int)hipEventRecord(b.ev[6], s);
-  e |= launch_primary(bd, b.d_prim.as<int>(), (int)b.h_prim.size(), s);
-  if (prof) e |= (int)hipEventRecord(b.ev[7], s);
-  b.profiled = prof;
+  b.profiled = k.prof != 0;
+  if (g.use_graphs && !b.graph_failed) {
+    if (b.gexec && memcmp(&k, &b.graph_key, sizeof(k)) == 0) {
+      if (hipGraphLaunch(b.gexec, s) == hipSuccess) {
+        b.graph_replays++;
+        return RX_OK;
+      }
+      (void)hipGetLastError();
+      drop_graph(b);
+      b.graph_failed = true;
+    } else if (b.last_valid && memcmp(&k, &b.last_key, sizeof(k)) == 0) {
+      // the second run with this configuration: capture it, then replay
+      if (capture_graph(b, k, s) == 0 && hipGraphLaunch(b.gexec, s) == hipSuccess) {
+        b.graph_replays++;
+        return RX_OK;
+      }
+      (void)hipGetLastError();
+      drop_graph(b);
+      b.graph_failed = true;  // direct launches for this batch from now on
+    }
+  }
+  const int e = launch_sequence(b, k, s);
+  b.last_key = k;
+  b.last_valid = true;
   if (e) {
     err = "kernel launch failed";
     return RX_ERR_DEVICE;

@@ -33,6 +33,18 @@ int n_model_params(int model);
 // Edges of the radial zones (RadialGrid::calculate_radial_grid, src/IonGradient.cpp:457-489).
 void rzone_grid(double rmin, double rmax, int nzones, double h, double *rgrid);
 
+// Launch configuration of one run: every kernel argument and grid shape.  A
+// run whose configuration equals the previous run's is captured into a
+// hipGraph, which is replayed for as long as the configuration stays equal
+// (the kernels read the batch's data through the same pointers).  Built with
+// its padding zeroed, so configurations compare with memcmp.
+struct RunKey {
+  BatchDev bd;
+  int n_blocks, n_xill1, n_xill, n_conv, n_prim, n_rrc;
+  const void *d_xill, *d_conv, *d_prim, *d_rrc;
+  int prof, cos_rows;
+};
+
 struct Batch {
   int n_req = 0;
   int seg_bins = 64;        // bins per work item
@@ -46,9 +58,11 @@ struct Batch {
   std::vector<XillWork> h_conv;  // (request, zone) units of k_conv
   std::vector<XillWork> h_xill;  // k_xill units: one-zone requests first, then the others (all zones in one unit)
   int n_xill1 = 0;               // number of one-zone units at the front of h_xill
+  std::vector<XillWork> h_rrc;   // (request, zone) units of the returning-radiation corrections
   int64_t tot_ener = 0, tot_flux = 0, tot_pgrid = 0, tot_prof = 0, tot_xspec = 0, tot_dist = 0, tot_zone = 0;
+  int64_t tot_rrc = 0;
   DevBuf d_req, d_ener, d_flux_in, d_out, d_pgrid, d_node, d_rad, d_prof, d_dist, d_refl, d_status, d_conv;
-  DevBuf d_xill, d_xspec, d_zone, d_prim;
+  DevBuf d_xill, d_xspec, d_zone, d_prim, d_rrc, d_rrcorr;
   DevBuf d_items, d_ctr, d_fix, d_reach, d_rnorm;
   int item_cap = 0;
   bool any_cosne = false, any_limb = false;
@@ -62,6 +76,12 @@ struct Batch {
   int64_t counters[3] = {};  // items, FP64 re-checks, re-check overflow (last run)
   int shape_class = 0;
   uint64_t config_gen = 0;  // Global::config_gen at build time
+  // graph of the launch sequence (captured on the second identical run)
+  RunKey last_key, graph_key;
+  bool last_valid = false, graph_failed = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int64_t graph_replays = 0;
   ~Batch();
 };
 
@@ -95,7 +115,7 @@ struct Global {
   XillTableHost xill;
   bool xill_loaded = false, xill_dirty = true;
   std::vector<double> xill_ener;
-  DevBuf x_spec, x_ener, x_ect, x_q;
+  DevBuf x_spec, x_ener, x_ect, x_q, x_incl;
   // returning-radiation table (loaded and uploaded on the first request with it)
   RradTableHost rrad;
   bool rrad_loaded = false, rrad_dirty = true;
@@ -117,6 +137,7 @@ struct Global {
   DevBuf d_refl_sets;
   bool refl_dirty = false;
   bool profile = false;
+  bool use_graphs = true;  // replay repeated launch sequences as hipGraphs
   // bumped by every change that invalidates built batches (tables, conv grid,
   // fine-grid size, clearing reflection sets); a batch refuses to run on a
   // newer configuration than it was built for
@@ -142,6 +163,7 @@ int rrad_zones(const RradTableHost &t, ReqParams &p, int nr_fine);
 int ensure_xilltable_device(std::string &err);
 void fill_dev_views(BatchDev &bd);
 void set_last(int status, const std::string &msg);
+
 int last_status();
 const char *last_error();
 bool phys_norm_env();

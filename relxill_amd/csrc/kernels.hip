@@ -122,6 +122,118 @@ __device__ inline double relat_abberation(double del, double beta) {
   return acos((cos(del) - beta) / (1 - beta * cos(del)));
 }
 
+// corrected_gshift_fluxboost_factor (src/Relreturn_Corona.cpp:39-83): the flux
+// boost of the xillver reflection under the energy shift g, from its boost
+// ratio c to a power law at g = 1.5 (a parabola through 0 for c < 1, a line
+// otherwise; at most 1 below g = 1, never negative)
+__device__ inline double gshift_boost(double c, double g, double gamma) {
+  const double g0 = 2. / 3;
+  double corr;
+  if (c < 1) {
+    const double a = (c / g0 - 1) / (g0 - 1);
+    const double b = 1 - a;
+    corr = (g >= 1) ? 1. / g * (1. / g * a + b) : g * (g * a + b);
+  } else {
+    const double alin = (c - 1) / (g0 - 1);
+    const double blin = 1 - alin;
+    corr = (g >= 1) ? (1. / g * alin + blin) : (g * alin + blin);
+  }
+  double boost = pow(g, gamma) * corr;
+  if (g < 1 && boost > 1) boost = 1;
+  if (boost < 0) boost = 0;
+  return boost;
+}
+
+// interpol_lptable (src/Rellp.cpp:219-247) of one request into the block's
+// LDS rows: radius (lerped in spin), intensity, emission and incidence angle
+// ((1 - f_a) lerp_h(spin lo) + f_a lerp_h(spin hi), float factors) at the
+// table's n_r radii.  Called by every thread of the block (two barriers).
+__device__ inline void lp_rows(const DevLpTable &L, const ReqParams &rq, double *jr, double *je, double *jd,
+                               double *ji, int tid, int nt) {
+  __shared__ int s_i[3];
+  __shared__ double s_f[3];
+  if (tid == 0) {
+    int lia;
+    const double lfa = lp_ifac((float)rq.a, L.a, L.n_a, lia);
+    int h0, h1;
+    const double fh0 = lp_ifac((float)rq.height, L.h + (size_t)lia * L.n_h, L.n_h, h0);
+    const double fh1 = lp_ifac((float)rq.height, L.h + (size_t)(lia + 1) * L.n_h, L.n_h, h1);
+    s_i[0] = lia;
+    s_i[1] = h0;
+    s_i[2] = h1;
+    s_f[0] = lfa;
+    s_f[1] = fh0;
+    s_f[2] = fh1;
+  }
+  __syncthreads();
+  const int ntr = L.n_r;
+  const int lia = s_i[0], h0 = s_i[1], h1 = s_i[2];
+  const double lfa = s_f[0], fh0 = s_f[1], fh1 = s_f[2];
+  for (int ii = tid; ii < ntr; ii += nt) {
+    jr[ii] = lerp1(lfa, (double)L.rad[(size_t)lia * ntr + ii], (double)L.rad[(size_t)(lia + 1) * ntr + ii]);
+    const size_t l0 = ((size_t)lia * L.n_h + h0) * ntr + ii;
+    const size_t u0 = ((size_t)lia * L.n_h + h0 + 1) * ntr + ii;
+    const size_t l1 = ((size_t)(lia + 1) * L.n_h + h1) * ntr + ii;
+    const size_t u1 = ((size_t)(lia + 1) * L.n_h + h1 + 1) * ntr + ii;
+    je[ii] = (1.0 - lfa) * lerp1(fh0, L.intens[l0], L.intens[u0]) + (lfa)*lerp1(fh1, L.intens[l1], L.intens[u1]);
+    jd[ii] = (1.0 - lfa) * lerp1(fh0, L.del[l0], L.del[u0]) + (lfa)*lerp1(fh1, L.del[l1], L.del[u1]);
+    ji[ii] = (1.0 - lfa) * lerp1(fh0, L.del_inc[l0], L.del_inc[u0]) + (lfa)*lerp1(fh1, L.del_inc[l1], L.del_inc[u1]);
+  }
+  __syncthreads();
+}
+
+// One radial zone of a relxill request (IonGradient, src/IonGradient.cpp:
+// 182-209, :421-430): its centre, for the lamp post the emission angle there
+// (inv_rebin_mean of the fine grid's angles, src/relutility.c:636-663), the
+// source -> disk energy shift and the Ecut the zone sees with its xillver
+// bracket (float factor, clamps, ensure_ecut_within_boundarys,
+// src/xilltable.c:1054-1125).  *ect receives that Ecut (lamp post).
+__device__ inline ZoneRec zone_record(const DevXillTable &xt, const ReqParams &rq, int z, bool lp, const double *jr,
+                                      const double *je, const double *jd, const double *ji, int ntr, int nr,
+                                      double *ect) {
+  const double *zg = rq.zgrid;
+  ZoneRec o;
+  o.rmean = 0.5 * (zg[z] + zg[z + 1]);
+  o.del_emit = 0.0;
+  o.shift = 1.0;
+  o.ncf = 1.0;
+  o.ect_ind = xt.ect_dim >= 0 ? rq.xind[xt.ect_dim] : 0;
+  o.ect_fac = xt.ect_dim >= 0 ? rq.xfac[xt.ect_dim] : 0.0;
+  o.pad = 0;
+  *ect = rq.xect_src;
+  if (lp) {
+    int lo = 0, hi = nr - 1;  // re(lo) > rmean >= re(hi)
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (fine_radius(rq.rin, rq.rout, mid, nr) > o.rmean) lo = mid; else hi = mid;
+    }
+    const double r0 = fine_radius(rq.rin, rq.rout, lo, nr), r1 = fine_radius(rq.rin, rq.rout, lo + 1, nr);
+    const LpFine f0 = lp_on_fine(r0, jr, je, jd, ji, ntr);
+    const LpFine f1 = lp_on_fine(r1, jr, je, jd, ji, ntr);
+    o.del_emit = lerp1((o.rmean - r1) / (r0 - r1), f1.del_emit, f0.del_emit);
+    o.shift = gi_potential_lp(o.rmean, rq.a, rq.height, rq.beta, o.del_emit);
+    *ect = rq.xect_src * o.shift;
+    if (xt.ect_dim >= 0) {
+      const float *vals = xt.ect_vals;
+      const int n = xt.n_ect;
+      float v = (float)*ect;
+      int ind = bsearch_asc(vals, n, v);
+      if (ind < 0) ind = 0;
+      else if (ind > n - 2) ind = n - 2;
+      if (v < vals[0]) v = vals[0];
+      else if (v > vals[n - 1]) v = vals[n - 1];
+      const float num = v - vals[ind];
+      const float den = vals[ind + 1] - vals[ind];
+      double fac = (float)(num / den);
+      if (*ect <= xt.q_vals[0]) fac = 0.0;
+      if (*ect >= xt.q_vals[xt.n_q - 1]) fac = 1.0;
+      o.ect_ind = ind;
+      o.ect_fac = fac;
+    }
+  }
+  return o;
+}
+
 // Where a request's profile accumulates: the caller output for line models,
 // the convolution-grid profile for convolution models.
 __device__ inline double *profile_acc(const BatchDev &b, const ReqParams &rq) {
@@ -253,6 +365,42 @@ __device__ inline double block_max(double v, double *red) {
   return r;
 }
 
+// 1/x to within an ulp or two: hardware estimate and two Newton steps
+// (cheaper than the IEEE division sequence; the rebin weights need ~1e-15).
+__device__ __forceinline__ double rcp64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// _rebin_spectrum (src/relutility.c:549-601), value of one output bin
+// [elo_, ehi_] from the counts clo = cnt(elo_), chi = cnt(ehi_) of input
+// edges at or below each end, cnt(x) = #{k : e0(k) <= x}: the reference's
+// sequential search gives imin = max(clo-1, 0), imax = max(min(chi, n0)-1, 0);
+// an overlap of zero width at the top edge gives 0.
+template <class E, class F>
+__device__ __forceinline__ double rebin_bin(double elo_, double ehi_, int clo, int chi, E e0, int n0, F f0) {
+  const int imin = clo > 0 ? clo - 1 : 0;
+  if (!((e0(0) <= ehi_) && (e0(n0) >= elo_)) || imin >= n0) return 0.0;
+  int imax = chi < n0 ? chi : n0;
+  if (imax > 0) imax--;
+  double elo = elo_, ehi = ehi_;
+  const double a0 = e0(imin), a1 = e0(imin + 1);
+  if (elo < a0) elo = a0;
+  const double b0 = e0(imax), b1 = e0(imax + 1);
+  if (ehi > b1) ehi = b1;
+  const double ra = rcp64(a1 - a0);
+  if (imax == imin) return (ehi - elo) * ra * f0(imin);
+  const double dmin = (a1 - elo) * ra;
+  const double dmax = (ehi - b0) * rcp64(b1 - b0);
+  double v = 0.0;
+  v += f0(imin) * dmin + f0(imax) * dmax;
+  for (int jj = imin + 1; jj <= imax - 1; jj++) v += f0(jj);
+  return v;
+}
+
 // block-wide deterministic sum (fixed tree order)
 __device__ inline double block_sum(double v, double *red) {
   const int tid = threadIdx.x;
@@ -290,8 +438,8 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
   __shared__ double s_jr[MAX_LP_R], s_je[MAX_LP_R], s_jd[MAX_LP_R], s_ji[MAX_LP_R];
   __shared__ double s_red[SETUP_THREADS];
   __shared__ double s_dist[MAX_ZONES * MAX_INCL];
-  __shared__ int s_st, s_ia, s_im, s_rmin, s_lia, s_lih0, s_lih1;
-  __shared__ double s_fa, s_fm, s_rms, s_lfa, s_lfh0, s_lfh1;
+  __shared__ int s_st, s_ia, s_im, s_rmin;
+  __shared__ double s_fa, s_fm, s_rms;
   constexpr int MAX_RC = MAX_NR_FINE / RAD_CHUNK;
   // energy window of each (radius chunk, zone): bins [s_ulo, s_uhi]
   __shared__ int s_ulo[MAX_RC * MAX_ZONES], s_uhi[MAX_RC * MAX_ZONES], s_pref[MAX_RC * MAX_ZONES + 1];
@@ -393,40 +541,13 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
   // lamp-post table node (interpol_lptable, src/Rellp.cpp:219-247)
   const bool lp = (rq.emis_type == EMIS_LP);
   const DevLpTable &L = b.lp;
-  if (lp && tid == 0) {
-    int lia;
-    const double lfa = lp_ifac((float)rq.a, L.a, L.n_a, lia);
-    int h0, h1;
-    const double fh0 = lp_ifac((float)rq.height, L.h + (size_t)lia * L.n_h, L.n_h, h0);
-    const double fh1 = lp_ifac((float)rq.height, L.h + (size_t)(lia + 1) * L.n_h, L.n_h, h1);
-    s_lia = lia;
-    s_lfa = lfa;
-    s_lih0 = h0;
-    s_lih1 = h1;
-    s_lfh0 = fh0;
-    s_lfh1 = fh1;
-  }
   __syncthreads();
   if (s_st) {
     if (tid == 0) b.status[req] = s_st;
     return;
   }
   const int ntr = L.n_r;
-  if (lp) {
-    const int lia = s_lia, h0 = s_lih0, h1 = s_lih1;
-    const double lfa = s_lfa, fh0 = s_lfh0, fh1 = s_lfh1;
-    for (int ii = tid; ii < ntr; ii += SETUP_THREADS) {
-      s_jr[ii] = lerp1(lfa, (double)L.rad[(size_t)lia * ntr + ii], (double)L.rad[(size_t)(lia + 1) * ntr + ii]);
-      const size_t l0 = ((size_t)lia * L.n_h + h0) * ntr + ii;
-      const size_t u0 = ((size_t)lia * L.n_h + h0 + 1) * ntr + ii;
-      const size_t l1 = ((size_t)(lia + 1) * L.n_h + h1) * ntr + ii;
-      const size_t u1 = ((size_t)(lia + 1) * L.n_h + h1 + 1) * ntr + ii;
-      s_je[ii] = (1.0 - lfa) * lerp1(fh0, L.intens[l0], L.intens[u0]) + (lfa)*lerp1(fh1, L.intens[l1], L.intens[u1]);
-      s_jd[ii] = (1.0 - lfa) * lerp1(fh0, L.del[l0], L.del[u0]) + (lfa)*lerp1(fh1, L.del[l1], L.del[u1]);
-      s_ji[ii] = (1.0 - lfa) * lerp1(fh0, L.del_inc[l0], L.del_inc[u0]) + (lfa)*lerp1(fh1, L.del_inc[l1], L.del_inc[u1]);
-    }
-    __syncthreads();
-  }
+  if (lp) lp_rows(L, rq, s_jr, s_je, s_jd, s_ji, tid, SETUP_THREADS);
 
   // fine radii: node -> fine map (:249-300), emissivity, energy window (:859-878)
   RadRec *rad = b.rad + (size_t)req * nr;
@@ -454,6 +575,7 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
   // sum_re frac_g g^(Gamma-1) tf_r emis(re) (calc_rrad_emis_corona, :119-151),
   // log-interpolated back onto the fine radii below
   __shared__ double s_rr_rad[MAX_RR], s_rr_in[MAX_RR], s_rr_ret[MAX_RR], s_rr_corr[2];
+  __shared__ double s_rr_cf[MAX_RR], s_rr_cg[MAX_RR];
   const bool rrad = lp && rq.return_rad != 0 && s_st == 0;
   const int rr_n = rq.rr_n;
   if (rrad) {
@@ -465,8 +587,23 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     for (int i = tid; i < rr_n; i += SETUP_THREADS) {
       const double lo = (i == 0) ? rin_f : rlo_t[klo + i];
       const double hi = (i == rr_n - 1) ? rout_f : rhi_t[klo + i];
-      s_rr_rad[i] = 0.5 * (lo + hi);
-      s_rr_ret[i] = 0.0;
+      const double rm = 0.5 * (lo + hi);
+      s_rr_rad[i] = rm;
+      // relxilllp above spin 0: the correction factors of the relxill zone
+      // the table zone's centre falls in (rebin_corrfactors_to_rradtable_grid,
+      // src/Relreturn_Corona.cpp:248-289; a centre beyond the last edge takes
+      // the last zone)
+      double cf = 1.0, cg = 1.0;
+      if (rq.rr_corr) {
+        const int n0 = nz;
+        int ind = bsearch_asc(zg, n0 + 1, rm);
+        if (rm < zg[0]) ind = 0;
+        else if (rm > zg[n0] || ind > n0 - 1) ind = n0 - 1;
+        cf = b.rrcorr[rq.rr_corr_off + 2 * ind];
+        cg = b.rrcorr[rq.rr_corr_off + 2 * ind + 1];
+      }
+      s_rr_cf[i] = cf;
+      s_rr_cg[i] = cg;
     }
     if (tid < 2) {  // ring-area corrections of the end zones (src/Relreturn_Table.cpp:415-435)
       const int i = (tid == 0) ? 0 : rr_n - 1;
@@ -490,22 +627,36 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
       s_rr_in[i] = lerp1((x - r1) / (r0 - r1), rad[lo + 1].coef, rad[lo].coef);
     }
     __syncthreads();
+    // calc_rrad_emis_corona (src/Relreturn_Corona.cpp:119-151): one wavefront
+    // per incident zone i, lane j its emitting zones (at most 64), summed in a
+    // fixed order; the frac_g weight g^(Gamma-1), or with a g-shift factor off
+    // 1 by more than 1e-3 of the emitting zone its corrected boost over g
+    // (calc_rrad_emis_zone, :87-108); the sum times the incident zone's flux
+    // factor
     const int ng = R.n_g;
-    for (int q = tid; q < rr_n * rr_n; q += SETUP_THREADS) {
-      const int i = q / rr_n, j = q % rr_n;  // incident zone i, emitting zone j
-      const size_t t = (size_t)k * nt * nt + (size_t)(klo + i) * nt + (klo + j);
-      const double gmin = R.gmin[t], gmax = R.gmax[t];
-      double ez = 0.0;
-      for (int jj = 0; jj < ng; jj++) {
-        const double g = (jj + 0.5) / ng * (gmax - gmin) + gmin;  // get_gfac_grid (src/Relreturn_Datastruct.cpp:51-59)
-        double v = R.frac_g[t * ng + jj];
-        if (fabs(g - 1) > 1e-3) v *= pow(g, rq.gamma - 1);
-        ez += v;
+    const int wv = tid >> 6, lane = tid & 63;
+    for (int i = wv; i < rr_n; i += SETUP_THREADS / 64) {
+      double term = 0.0;
+      const int j = lane;
+      if (j < rr_n) {
+        const size_t t = (size_t)k * nt * nt + (size_t)(klo + i) * nt + (klo + j);
+        const double gmin = R.gmin[t], gmax = R.gmax[t];
+        const double cgj = s_rr_cg[j];
+        double ez = 0.0;
+        for (int jj = 0; jj < ng; jj++) {
+          const double g = (jj + 0.5) / ng * (gmax - gmin) + gmin;  // get_gfac_grid (src/Relreturn_Datastruct.cpp:51-59)
+          double v = R.frac_g[t * ng + jj];
+          if (fabs(cgj - 1) > 1e-3) v *= gshift_boost(cgj, g, rq.gamma) / g;
+          else if (fabs(g - 1) > 1e-3) v *= pow(g, rq.gamma - 1);
+          ez += v;
+        }
+        double tf = R.tf_r[t];
+        if (j == 0 && i < rr_n - 1) tf *= s_rr_corr[0];
+        if (j == rr_n - 1 && i < rr_n - 2) tf *= s_rr_corr[1];
+        term = ez * tf * s_rr_in[j];
       }
-      double tf = R.tf_r[t];
-      if (j == 0 && i < rr_n - 1) tf *= s_rr_corr[0];
-      if (j == rr_n - 1 && i < rr_n - 2) tf *= s_rr_corr[1];
-      atomicAdd(&s_rr_ret[i], ez * tf * s_rr_in[j]);
+      const double sum = wave_sum(term);
+      if (lane == 0) s_rr_ret[i] = sum * s_rr_cf[i];
     }
     __syncthreads();
   }
@@ -662,46 +813,9 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     const DevXillTable &xt = b.xt;
     __shared__ double s_zect[MAX_ZONES];
     for (int z = tid; z < nz; z += SETUP_THREADS) {
-      ZoneRec o;
-      o.rmean = 0.5 * (zg[z] + zg[z + 1]);
-      o.del_emit = 0.0;
-      o.shift = 1.0;
-      o.ncf = 1.0;
-      o.ect_ind = xt.ect_dim >= 0 ? b.req[req].xind[xt.ect_dim] : 0;
-      o.ect_fac = xt.ect_dim >= 0 ? b.req[req].xfac[xt.ect_dim] : 0.0;
-      o.pad = 0;
-      if (lp) {
-        int lo = 0, hi = nr - 1;  // re(lo) > rmean >= re(hi)
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (fine_radius(rq.rin, rq.rout, mid, nr) > o.rmean) lo = mid; else hi = mid;
-        }
-        const double r0 = fine_radius(rq.rin, rq.rout, lo, nr), r1 = fine_radius(rq.rin, rq.rout, lo + 1, nr);
-        const LpFine f0 = lp_on_fine(r0, s_jr, s_je, s_jd, s_ji, ntr);
-        const LpFine f1 = lp_on_fine(r1, s_jr, s_je, s_jd, s_ji, ntr);
-        o.del_emit = lerp1((o.rmean - r1) / (r0 - r1), f1.del_emit, f0.del_emit);
-        o.shift = gi_potential_lp(o.rmean, rq.a, rq.height, rq.beta, o.del_emit);
-        const double ect = rq.xect_src * o.shift;
-        s_zect[z] = ect;
-        if (xt.ect_dim >= 0) {
-          const float *vals = xt.ect_vals;
-          const int n = xt.n_ect;
-          float v = (float)ect;
-          int ind = bsearch_asc(vals, n, v);
-          if (ind < 0) ind = 0;
-          else if (ind > n - 2) ind = n - 2;
-          if (v < vals[0]) v = vals[0];
-          else if (v > vals[n - 1]) v = vals[n - 1];
-          const float num = v - vals[ind];
-          const float den = vals[ind + 1] - vals[ind];
-          double fac = (float)(num / den);
-          if (ect <= xt.q_vals[0]) fac = 0.0;
-          if (ect >= xt.q_vals[xt.n_q - 1]) fac = 1.0;
-          o.ect_ind = ind;
-          o.ect_fac = fac;
-        }
-      }
-      zr[z] = o;
+      double ect;
+      zr[z] = zone_record(xt, b.req[req], z, lp, s_jr, s_je, s_jd, s_ji, ntr, nr, &ect);
+      s_zect[z] = ect;
     }
     __syncthreads();
     if (lp) {
@@ -846,6 +960,106 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     __syncthreads();
   }
   if (tid == 0) b.status[req] = s_st;
+}
+
+// relxilllp's returning radiation above spin 0: the xillver correction
+// factors of one zone (calc_rrad_corr_factors, src/Relxill.cpp:164-182, with
+// get_xillver_fluxcorrection_factors, src/Xillspec.cpp:500-526).  The zone's
+// xillver spectra for every inclination (16 table corners with the zone's own
+// Ecut bracket, as k_xill reads them) are averaged with the weights of
+// calc_angle_averaged_xill_spec (:109-126: 0.5 cos(i) / n_incl with the
+// table's inclination, in degrees, read as radians), then
+//   flux factor = energy flux of the average in 0.1-1000 keV over that of the
+//                 normalised cut-off power law on the 500-bin grid (:344-362)
+//   g factor    = photon flux of the average in 0.15-500 keV over that of the
+//                 average shifted 1.5 times lower in energy (rebinned, times
+//                 2/3), over 1.5^Gamma (:454-487).
+// One workgroup per (request, zone); the average sits in dynamic LDS; the
+// sums are fixed-order block sums.  k_setup reads the factors (it runs next).
+constexpr int RRC_THREADS = 256;
+__global__ __launch_bounds__(RRC_THREADS) void k_rrad_corr(BatchDev b, const XillWork *__restrict__ work) {
+  extern __shared__ double s_avg[];
+  __shared__ double s_jr[MAX_LP_R], s_je[MAX_LP_R], s_jd[MAX_LP_R], s_ji[MAX_LP_R];
+  __shared__ double s_red[RRC_THREADS];
+  const XillWork wk = work[blockIdx.x];
+  const int req = wk.req, z = wk.zone, tid = threadIdx.x;
+  const ReqParams &rq = b.req[req];
+  if (rq.pre_status != 0) return;  // uniform over the block
+  const DevXillTable xt = b.xt;
+  lp_rows(b.lp, rq, s_jr, s_je, s_jd, s_ji, tid, RRC_THREADS);
+  double ect;
+  const ZoneRec zr = zone_record(xt, rq, z, true, s_jr, s_je, s_jd, s_ji, b.lp.n_r, b.nr_fine, &ect);
+  int i0[4];
+  double f[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const bool e = (k == xt.ect_dim);
+    i0[k] = e ? zr.ect_ind : rq.xind[k];
+    f[k] = e ? zr.ect_fac : rq.xfac[k];
+  }
+  // the 16 corners in the reference's order (interp_5d_tab_incl, src/xilltable.c:812-876)
+  constexpr unsigned char kCorner[16] = {0x0, 0x8, 0x4, 0x2, 0xC, 0xA, 0x6, 0xE,
+                                         0x1, 0x9, 0x5, 0x3, 0xD, 0xB, 0x7, 0xF};
+  double w[16];
+  unsigned row[16];
+#pragma unroll
+  for (int c = 0; c < 16; c++) {
+    int o[4];
+    double fk[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      o[k] = (kCorner[c] >> (3 - k)) & 1;
+      fk[k] = o[k] ? f[k] : (1.0 - f[k]);
+    }
+    w[c] = fk[0] * fk[1] * fk[2] * fk[3];
+    const unsigned r = (((unsigned)(i0[0] + o[0]) * xt.n_val[1] + (i0[1] + o[1])) * xt.n_val[2] + (i0[2] + o[2])) *
+                           xt.n_val[3] + (i0[3] + o[3]);
+    row[c] = r * xt.n_incl;
+  }
+  const int ne = xt.n_ener, n_incl = xt.n_incl;
+  const double *E = xt.ener;
+  for (int e = tid; e < ne; e += RRC_THREADS) {
+    double acc = 0.0;
+    for (int a = 0; a < n_incl; a++) {
+      const double incl_deg = (double)xt.incl_vals[a] * 180 / PI;
+      const double wa = 0.5 * cos(incl_deg * PI / 180) / n_incl;
+      double sv = 0.0;
+#pragma unroll
+      for (int c = 0; c < 16; c++) sv += w[c] * (double)xt.spec[(size_t)(row[c] + a) * ne + e];
+      acc += wa * sv;
+    }
+    s_avg[e] = acc;
+  }
+  __syncthreads();
+  // band sums of the average, and of its shifted copy
+  const double gref = 2. / 3.;
+  const GridGuess gx = grid_guess(E, ne);
+  auto ge = [&](int k) { return E[k]; };
+  auto fav = [&](int k) { return s_avg[k]; };
+  double ef = 0.0, pf = 0.0, pz = 0.0;
+  for (int e = tid; e < ne; e += RRC_THREADS) {
+    const double e0 = E[e], e1 = E[e + 1];
+    if (e0 >= 0.1 && e1 <= 1000.0) ef += s_avg[e] * 0.5 * (e0 + e1);
+    if (e0 >= 0.15 && e1 <= 500.0) {
+      pf += s_avg[e];
+      const double lo = e0 / gref, hi = e1 / gref;
+      pz += rebin_bin(lo, hi, count_le(ge, ne, lo, gx), count_le(ge, ne, hi, gx), ge, ne, fav) * gref;
+    }
+  }
+  // the normalised cut-off power law of the zone (Ecut it sees, the source's
+  // Gamma) on the 500-bin grid (calc_normalized_primary_spectrum, :302-332)
+  const double nf = 1. / (block_sum(xill_norm_part(b.cg, ect, rq.xgam, tid, RRC_THREADS), s_red) / NORM_XILLVER_TABLE);
+  double pe = 0.0;
+  for (int k = tid; k < N_ENER_COARSE; k += RRC_THREADS) {
+    const double e0 = b.cg.coarse[k], e1 = b.cg.coarse[k + 1];
+    if (e0 >= 0.1 && e1 <= 1000.0) pe += cutoffpl_bin(e0, e1, b.cg.coarse_ln[k], ect, rq.xgam) * nf * 0.5 * (e0 + e1);
+  }
+  const double s_ef = block_sum(ef, s_red), s_pf = block_sum(pf, s_red), s_pz = block_sum(pz, s_red);
+  const double s_pe = block_sum(pe, s_red);
+  if (tid == 0) {
+    b.rrcorr[rq.rr_corr_off + 2 * z] = s_ef / s_pe;
+    b.rrcorr[rq.rr_corr_off + 2 * z + 1] = s_pf / s_pz / pow(1.5, rq.xgam);
+  }
 }
 
 // Clears the run's counters; launched before k_setup on the batch's stream.
@@ -1342,41 +1556,6 @@ __device__ inline void block_sum2(double a, double c, double *out, double (*s_w)
 // E_hi < 1000, a contiguous index range of the convolution grid
 __device__ inline bool in_band(const DevConvGrid &cg, int i) { return i >= cg.band_lo && i < cg.band_hi; }
 
-// 1/x to within an ulp or two: hardware estimate and two Newton steps
-// (cheaper than the IEEE division sequence; the rebin weights need ~1e-15).
-__device__ __forceinline__ double rcp64(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-x, r, 1.0);
-  return fma(r, e, r);
-}
-
-// _rebin_spectrum (src/relutility.c:549-601), value of one output bin
-// [elo_, ehi_] from the counts clo = cnt(elo_), chi = cnt(ehi_) of input
-// edges at or below each end, cnt(x) = #{k : e0(k) <= x}: the reference's
-// sequential search gives imin = max(clo-1, 0), imax = max(min(chi, n0)-1, 0);
-// an overlap of zero width at the top edge gives 0.
-template <class E, class F>
-__device__ __forceinline__ double rebin_bin(double elo_, double ehi_, int clo, int chi, E e0, int n0, F f0) {
-  const int imin = clo > 0 ? clo - 1 : 0;
-  if (!((e0(0) <= ehi_) && (e0(n0) >= elo_)) || imin >= n0) return 0.0;
-  int imax = chi < n0 ? chi : n0;
-  if (imax > 0) imax--;
-  double elo = elo_, ehi = ehi_;
-  const double a0 = e0(imin), a1 = e0(imin + 1);
-  if (elo < a0) elo = a0;
-  const double b0 = e0(imax), b1 = e0(imax + 1);
-  if (ehi > b1) ehi = b1;
-  const double ra = rcp64(a1 - a0);
-  if (imax == imin) return (ehi - elo) * ra * f0(imin);
-  const double dmin = (a1 - elo) * ra;
-  const double dmax = (ehi - b0) * rcp64(b1 - b0);
-  double v = 0.0;
-  v += f0(imin) * dmin + f0(imax) * dmax;
-  for (int jj = imin + 1; jj <= imax - 1; jj++) v += f0(jj);
-  return v;
-}
 
 
 }  // namespace
@@ -1687,7 +1866,9 @@ __global__ __launch_bounds__(PRIM_THREADS) void k_primary(BatchDev b, const int 
   }
   const double nf = 1. / (block_sum4(xill_norm_part(cg, ect, gam, tid, NT), s_w) / NORM_XILLVER_TABLE);
   // the primary on the convolution grid, evaluated where the rebin reads it
-  // (an output bin spans one or two grid bins): no staging buffer
+  // (one or two grid bins per output bin on caller grids finer than the
+  // convolution grid; every grid bin of the output bin, each with its
+  // exponential, on coarser ones): no staging buffer
   const double *ce = cg.ener;
   const double *cl = cg.ln_emid;
   const double ect_o = ect * rq.g_so;
@@ -2052,6 +2233,14 @@ int launch_xill(const BatchDev &b, const XillWork *work, int n_single, int n_mul
     hipLaunchKernelGGL(k_xillz<MAX_ZONES>, dim3(n_multi * chunks), dim3(XILL_THREADS), 0,
                        (hipStream_t)stream, b, work + n_single, chunks);
   }
+  return (int)hipGetLastError();
+}
+
+int launch_rrad_corr(const BatchDev &b, const XillWork *work, int n_work, void *stream) {
+  if (n_work <= 0) return 0;
+  const size_t lds = sizeof(double) * (size_t)b.xt.n_ener;
+  (void)hipFuncSetAttribute((const void *)k_rrad_corr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(k_rrad_corr, dim3(n_work), dim3(RRC_THREADS), lds, (hipStream_t)stream, b, work);
   return (int)hipGetLastError();
 }
 

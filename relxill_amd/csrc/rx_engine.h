@@ -31,6 +31,7 @@ constexpr int MAX_LP_R = 256;     // lamp-post table radii
 constexpr int MAX_INCL = 64;      // emission-angle bins
 constexpr int MAX_FFT = 8192;     // convolution grid size limit
 constexpr int MAX_ZONES = 10;     // radial zones of relxilllp (N_ZONES, src/common.h:134)
+constexpr int MAX_RRC_ENER = 16384;  // xillver bins held in LDS by the returning-radiation corrections
 
 enum ModelKind : int {
   MOD_RELLINE = 1,
@@ -81,6 +82,10 @@ struct ReqParams {
   // returning radiation (lamp post): 1 add, -1 or 2 replace the direct
   // emissivity; the table spin, its first zone and the number of zones used
   int return_rad, rr_k, rr_klo, rr_n;
+  // relxilllp above spin 0: the zones' xillver correction factors of the
+  // returning radiation (flux, g shift per zone) at rrcorr[rr_corr_off]
+  int rr_corr;
+  int64_t rr_corr_off;
 };
 
 // Per zone of a relxill request (IonGradient, src/IonGradient.cpp:182-209):
@@ -163,6 +168,7 @@ struct DevXillTable {
   const float *ect_vals;  // values of axis ect_dim
   const float *q_vals;    // values of axis ect_q
   int n_ect, n_q;
+  const float *incl_vals;  // the inclination axis (degrees)
 };
 
 constexpr int N_ENER_COARSE = 500;  // primary-continuum normalisation grid (src/Xillspec.h:32)
@@ -261,6 +267,7 @@ struct BatchDev {
   double *xspec;
   ZoneRec *zone;
   DevRradTable rr;
+  double *rrcorr;     // relxilllp returning radiation: 2 factors per zone
 };
 
 // ---- launchers (kernels.hip) ----
@@ -272,6 +279,9 @@ int launch_finalize(const BatchDev &b, void *stream);
 int launch_conv(const BatchDev &b, const XillWork *conv_work, int n_conv_work, void *stream);
 // work: n_single one-zone units, then n_multi units of several zones
 int launch_xill(const BatchDev &b, const XillWork *work, int n_single, int n_multi, void *stream);
+// relxilllp returning radiation above spin 0: the zones' xillver correction
+// factors, one workgroup per (request, zone) of work (before launch_setup)
+int launch_rrad_corr(const BatchDev &b, const XillWork *work, int n_work, void *stream);
 // relxill requests: the primary continuum added to the output (after launch_conv)
 int launch_primary(const BatchDev &b, const int *reqs, int n_reqs, void *stream);
 int launch_copy(const void *src, void *dst, size_t bytes, void *stream);

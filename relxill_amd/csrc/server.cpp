@@ -5,14 +5,20 @@
 // stream: whenever a slot is free and requests are pending, it admits pending
 // requests in arrival order while their device working set fits the HBM
 // budget (minus what the running batches hold) and the in-flight cap, builds
-// a batch and launches it.  Finished batches (polled with hipStreamQuery) are
-// copied back, their memory returned to the budget, and the requests marked
-// done with their latency (submit -> completion seen).  Tables and the
-// convolution grid stay resident for the server's lifetime.
+// a batch (without holding the queue lock, so submits and polls go on) and
+// launches it, followed on the same stream by the copies of its outputs,
+// statuses and counters into the slot's pinned host buffers.  While every
+// slot is busy, arrivals queue up and the next free slot takes all of them:
+// batches grow with the load.  A finished batch (hipStreamQuery) has its
+// results on the host already; its memory returns to the budget and its
+// requests are marked done with their latency (submit -> completion seen).
+// Tables and the convolution grid stay resident for the server's lifetime.
 //
-// Each request's work items depend only on the request itself (fixed segment
-// length); per-bin sums over radius chunks are added atomically in FP64, so
-// a request's result equals its solo run up to FP64 rounding of that order.
+// Each request's work items depend only on the request itself and the
+// batch's segment length; per-bin sums over radius chunks are added
+// atomically in FP64, so a request's result equals its solo run up to FP64
+// rounding of that order.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -31,8 +37,9 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-constexpr int SERVER_SEG_BINS = 64;
-constexpr int SERVER_WAVES_PER_CU = 8;
+// work-list sizing bound of the admission estimate (the smallest segment
+// length the autotuner may choose)
+constexpr int SERVER_MIN_SEG_BINS = 32;
 
 struct Pending {
   int64_t ticket;
@@ -50,11 +57,35 @@ struct Result {
   std::vector<double> flux;
 };
 
+// Pinned host memory that a batch's outputs are copied into on its stream.
+struct PinnedBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  bool ensure(size_t n) {
+    if (p && bytes >= n) return true;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess) {
+      p = nullptr;
+      return false;
+    }
+    bytes = n;
+    return true;
+  }
+  template <class T>
+  T *as() const { return reinterpret_cast<T *>(p); }
+};
+
 struct Slot {
   Batch batch;
   bool busy = false;
   std::vector<Pending> reqs;
   int64_t bytes = 0;
+  PinnedBuf h_out, h_status, h_ctr;
 };
 
 int64_t request_bytes(const Pending &p) {
@@ -67,12 +98,12 @@ int64_t request_bytes(const Pending &p) {
   const int64_t n_rc = (g.nr_fine + RAD_CHUNK - 1) / RAD_CHUNK;
   int64_t b = sizeof(ReqParams) + 8 * (2 * (n_in + 1) + 2 * n_in);
   b += 8 * (n_e + 1) + 8 * n_e * nz;
-  b += (int64_t)sizeof(WorkItem) * (n_rc + nz - 1) * ((n_e + SERVER_SEG_BINS - 1) / SERVER_SEG_BINS);
+  b += (int64_t)sizeof(WorkItem) * (n_rc + nz - 1) * ((n_e + SERVER_MIN_SEG_BINS - 1) / SERVER_MIN_SEG_BINS);
   b += (int64_t)sizeof(FixEntry) * (n_e / 8 + 1);
   b += (int64_t)sizeof(RadRec) * g.nr_fine;
   b += 8 * 2 * (int64_t)g.rel.n_r * g.rel.n_g * 2;
   b += 8 * MAX_INCL * nz + sizeof(LpRefl) + sizeof(ReqNorm) + 64;
-  if (relxill) b += (8 * (int64_t)g.xill.n_ener + (int64_t)(sizeof(XillWork) + sizeof(ZoneRec))) * nz;
+  if (relxill) b += (8 * (int64_t)g.xill.n_ener + (int64_t)(2 * sizeof(XillWork) + sizeof(ZoneRec)) + 16) * nz;
   return b;
 }
 
@@ -95,12 +126,52 @@ struct rx_server {
   int device = 0;
 
   void loop();
-  void launch_ready();
-  bool reap();
+  int launch(Slot &s, std::vector<Pending> &take, std::string &err);
+  void launch_ready(std::unique_lock<std::mutex> &lk);
+  bool reap(std::unique_lock<std::mutex> &lk);
 };
 
-void rx_server::launch_ready() {
-  // called with mu held
+// Builds, launches and queues the output copies of one batch on the slot's
+// stream (called without the queue lock: only the scheduler thread touches
+// the slots).
+int rx_server::launch(Slot &s, std::vector<Pending> &take, std::string &err) {
+  std::vector<rx_request> rq(take.size());
+  for (size_t i = 0; i < take.size(); i++) {
+    rq[i].model = take[i].model;
+    rq[i].n_flux = (int)take[i].energy.size() - 1;
+    rq[i].energy = take[i].energy.data();
+    rq[i].parameter = take[i].params.data();
+    rq[i].flux_in = take[i].flux_in.empty() ? nullptr : take[i].flux_in.data();
+    rq[i].refl_set = take[i].refl_set;
+  }
+  std::lock_guard<std::recursive_mutex> lk(G().mu);
+  int st = ensure_device(err);
+  if (st != RX_OK) return st;
+  // the autotuned work shape of the batch's shape class (else the default)
+  s.batch.seg_forced = 0;
+  s.batch.waves_forced = 0;
+  st = batch_build(s.batch, rq.data(), (int)rq.size(), err);
+  if (st != RX_OK) return st;
+  if (!s.h_out.ensure(sizeof(double) * std::max<int64_t>(1, s.batch.tot_flux)) ||
+      !s.h_status.ensure(sizeof(int) * s.batch.n_req) || !s.h_ctr.ensure(sizeof(int) * N_COUNTERS)) {
+    err = "pinned host allocation failed";
+    return RX_ERR_NOMEM;
+  }
+  st = batch_run(s.batch, precision, err);
+  if (st != RX_OK) return st;
+  hipStream_t q = s.batch.stream;
+  int e = (int)hipMemcpyAsync(s.h_out.p, s.batch.d_out.p, sizeof(double) * s.batch.tot_flux, hipMemcpyDeviceToHost, q);
+  e |= (int)hipMemcpyAsync(s.h_status.p, s.batch.d_status.p, sizeof(int) * s.batch.n_req, hipMemcpyDeviceToHost, q);
+  e |= (int)hipMemcpyAsync(s.h_ctr.p, s.batch.d_ctr.p, sizeof(int) * N_COUNTERS, hipMemcpyDeviceToHost, q);
+  if (e) {
+    err = "output copy failed";
+    return RX_ERR_DEVICE;
+  }
+  return RX_OK;
+}
+
+void rx_server::launch_ready(std::unique_lock<std::mutex> &lk) {
+  // called with mu held; released while a batch is built and launched
   for (auto &sp : slots) {
     Slot &s = *sp;
     if (s.busy || pending.empty()) continue;
@@ -115,27 +186,13 @@ void rx_server::launch_ready() {
       pending.pop_front();
     }
     if (take.empty()) break;
-    std::vector<rx_request> rq(take.size());
-    for (size_t i = 0; i < take.size(); i++) {
-      rq[i].model = take[i].model;
-      rq[i].n_flux = (int)take[i].energy.size() - 1;
-      rq[i].energy = take[i].energy.data();
-      rq[i].parameter = take[i].params.data();
-      rq[i].flux_in = take[i].flux_in.empty() ? nullptr : take[i].flux_in.data();
-      rq[i].refl_set = take[i].refl_set;
-    }
+    s.busy = true;  // reserved: its budget share is counted below
+    inflight_bytes += bytes;
+    inflight_reqs += (int)take.size();
+    lk.unlock();
     std::string err;
-    int st;
-    {
-      std::lock_guard<std::recursive_mutex> lk(G().mu);
-      st = ensure_device(err);
-      if (st == RX_OK) {
-        s.batch.seg_forced = SERVER_SEG_BINS;
-        s.batch.waves_forced = SERVER_WAVES_PER_CU;
-        st = batch_build(s.batch, rq.data(), (int)rq.size(), err);
-      }
-      if (st == RX_OK) st = batch_run(s.batch, precision, err);
-    }
+    const int st = launch(s, take, err);
+    lk.lock();
     if (st != RX_OK) {
       const auto now = Clock::now();
       for (auto &p : take) {
@@ -146,14 +203,14 @@ void rx_server::launch_ready() {
         r.latency_ms = std::chrono::duration<double, std::milli>(now - p.t_submit).count();
         completed++;
       }
+      s.busy = false;
+      inflight_bytes -= bytes;
+      inflight_reqs -= (int)take.size();
       cv.notify_all();
       continue;
     }
-    s.busy = true;
     s.reqs = std::move(take);
     s.bytes = bytes;
-    inflight_bytes += bytes;
-    inflight_reqs += (int)s.reqs.size();
     batches++;
     int64_t conc = 0;
     for (auto &o : slots) conc += o->busy ? 1 : 0;
@@ -161,30 +218,42 @@ void rx_server::launch_ready() {
   }
 }
 
-bool rx_server::reap() {
+bool rx_server::reap(std::unique_lock<std::mutex> &lk) {
   // called with mu held
   bool any = false;
   for (auto &sp : slots) {
     Slot &s = *sp;
-    if (!s.busy) continue;
+    if (!s.busy || s.reqs.empty()) continue;
     hipError_t q = hipStreamQuery(s.batch.stream);
     if (q == hipErrorNotReady) continue;
     const auto now = Clock::now();
-    std::vector<int> st(s.batch.n_req, RX_ERR_DEVICE);
-    std::vector<double> out(s.batch.tot_flux, 0.0);
-    if (q == hipSuccess) {
-      bool ok = hipMemcpy(out.data(), s.batch.d_out.p, sizeof(double) * s.batch.tot_flux, hipMemcpyDeviceToHost) == hipSuccess;
-      ok = ok && hipMemcpy(st.data(), s.batch.d_status.p, sizeof(int) * s.batch.n_req, hipMemcpyDeviceToHost) == hipSuccess;
-      if (!ok) st.assign(s.batch.n_req, RX_ERR_DEVICE);
+    const int *st = s.h_status.as<int>();
+    const double *out = s.h_out.as<double>();
+    bool ok = q == hipSuccess;
+    if (ok && precision == RX_FP32 && s.h_ctr.as<int>()[CTR_FIX_OVF] != 0) {
+      // the FP64 re-check queue overflowed: redo the batch in FP64 (rare;
+      // synchronous, outside the queue lock)
+      lk.unlock();
+      {
+        std::lock_guard<std::recursive_mutex> g(G().mu);
+        std::string err;
+        ok = batch_run(s.batch, RX_FP64, err) == RX_OK;
+        ok = ok && hipMemcpyAsync(s.h_out.p, s.batch.d_out.p, sizeof(double) * s.batch.tot_flux,
+                                  hipMemcpyDeviceToHost, s.batch.stream) == hipSuccess;
+        ok = ok && hipMemcpyAsync(s.h_status.p, s.batch.d_status.p, sizeof(int) * s.batch.n_req,
+                                  hipMemcpyDeviceToHost, s.batch.stream) == hipSuccess;
+        ok = ok && hipStreamSynchronize(s.batch.stream) == hipSuccess;
+      }
+      lk.lock();
     }
     for (size_t i = 0; i < s.reqs.size(); i++) {
       const ReqParams &p = s.batch.h_req[i];
       Result &r = results[s.reqs[i].ticket];
       r.done = true;
-      int code = public_status(st[i]);
+      const int code = ok ? public_status(st[i]) : RX_ERR_DEVICE;
       r.status = code;
-      r.flux.assign(out.begin() + p.flux_off, out.begin() + p.flux_off + p.n_in);
-      if (code != RX_OK) std::fill(r.flux.begin(), r.flux.end(), 0.0);
+      if (code == RX_OK) r.flux.assign(out + p.flux_off, out + p.flux_off + p.n_in);
+      else r.flux.assign(p.n_in, 0.0);
       r.latency_ms = std::chrono::duration<double, std::milli>(now - s.reqs[i].t_submit).count();
       completed++;
     }
@@ -202,8 +271,8 @@ void rx_server::loop() {
   (void)hipSetDevice(device);
   std::unique_lock<std::mutex> lk(mu);
   while (!stop.load()) {
-    launch_ready();
-    bool progressed = reap();
+    bool progressed = reap(lk);
+    launch_ready(lk);
     bool busy = false;
     for (auto &s : slots) busy |= s->busy;
     if (!progressed) {
@@ -219,7 +288,7 @@ void rx_server::loop() {
   // drain whatever is still on the device
   for (auto &s : slots)
     if (s->busy) (void)hipStreamSynchronize(s->batch.stream);
-  reap();
+  reap(lk);
 }
 
 extern "C" {
@@ -269,6 +338,24 @@ int64_t rx_server_submit(rx_server *s, const rx_request *req) {
   s->pending.push_back(std::move(p));
   s->cv.notify_all();
   return t;
+}
+
+int rx_server_replay(rx_server *s, const rx_request *req, int n, const double *arrival_s, int64_t *tickets) {
+  if (!s || !req || !arrival_s || !tickets || n < 0) return RX_ERR_ARG;
+  for (int i = 1; i < n; i++)
+    if (!(arrival_s[i] >= arrival_s[i - 1])) return RX_ERR_ARG;
+  const auto t0 = Clock::now();
+  for (int i = 0; i < n; i++) {
+    const auto due = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(arrival_s[i]));
+    for (;;) {
+      const auto now = Clock::now();
+      if (now >= due) break;
+      if (due - now > std::chrono::microseconds(200)) std::this_thread::sleep_for(std::chrono::microseconds(100));
+      else std::this_thread::yield();
+    }
+    tickets[i] = rx_server_submit(s, &req[i]);
+  }
+  return RX_OK;
 }
 
 int rx_server_poll(rx_server *s, int64_t ticket, double *flux, int *status, double *latency_ms) {

@@ -75,8 +75,8 @@ def test_model_params_equal_oracle(model):
 def test_parameter_bounds_of_the_reference():
     """check_parameter_bounds (src/ModelDefinition.cpp:168-210): Rin < ISCO is
     reset to the ISCO, Rout <= Rin and |a| > 0.9982 or incl outside 3-87 deg
-    are rejected; returning radiation is accepted for relline_lp / relconv_lp
-    (switch 1, -1, 2) and reported unsupported for relxilllp."""
+    are rejected; returning radiation is accepted for relline_lp, relconv_lp
+    and relxilllp (switch 1, -1, 2), other switch values rejected."""
     out, st = rx.model_params(1, [6.4, 3, 3, 15, 0.998, 30, 1.0, 400, 0, 0])
     assert st == 0 and out[5] == oracle.lib.orc_kerr_rms(0.998)
     assert rx.model_params(11, [3, 3, 15, 0.5, 30, 50.0, 40.0, 0])[1] == 1
@@ -85,7 +85,8 @@ def test_parameter_bounds_of_the_reference():
     assert rx.model_params(11, [3, 3, 15, 0.5, 88.0, -1, 400, 0])[1] == 1
     assert rx.model_params(2, [6.4, 6.0, 0.998, 30, -1, 400, 0, 0, 2, 1])[1] == 0
     assert rx.model_params(2, [6.4, 6.0, 0.998, 30, -1, 400, 0, 0, 2, 3])[1] == 1
-    assert rx.model_params(-2, [6.0, 0.0, 0.998, 30, -1, 400, 0, 2, 3.1, 1, 300, 1, 1, 0])[1] == 2
+    assert rx.model_params(-2, [6.0, 0.0, 0.998, 30, -1, 400, 0, 2, 3.1, 1, 300, 1, 1, 0])[1] == 0
+    assert rx.model_params(-2, [6.0, 0.0, 0.998, 30, -1, 400, 0, 2, 3.1, 1, 300, 1, 3, 0])[1] == 1
 
 
 def test_synthetic_table_invariants():

@@ -265,8 +265,9 @@ def test_relxilllp_random_sweep(relx):
 
 
 def test_relxilllp_entry_point_and_returning_radiation(relx):
-    """lmodrelxilllp gives the FP64 result; switch_returnrad = 1 (the lmodel.dat
-    default) is refused with RX_ERR_UNSUPPORTED and zero flux."""
+    """lmodrelxilllp gives the FP64 result, at its lmodel.dat defaults too
+    (switch_returnrad = 1, a = 0.998: returning radiation with the zones'
+    xillver correction factors)."""
     T, xt = relx
     p = [10.0, 0.1, 0.7, 50.0, -1.0, 300.0, 0.02, 2.3, 2.0, 2.0, 150.0, 2.0, 0.0, 1.0]
     e = log_grid(700, 0.2, 80.0)
@@ -274,8 +275,46 @@ def test_relxilllp_entry_point_and_returning_radiation(relx):
     assert st == 0
     y = rx.lmodrelxilllp(e, p)
     assert rel_err(y, ref) < TOL64
-    y = rx.lmodrelxilllp(e, rx.DEFAULTS[rx.MODEL_RELXILLLP])
-    assert not y.any() and rx.lib.rx_last_status() == rx.RX_ERR_UNSUPPORTED
+    d = list(rx.DEFAULTS[rx.MODEL_RELXILLLP])
+    assert d[12] == 1.0
+    ref, _, st = oracle.relxill_model(oracle.MOD_RELXILLLP, T, xt, d, e)
+    assert st == 0
+    y = rx.lmodrelxilllp(e, d)
+    print(f"lmodrelxilllp defaults: rel {rel_err(y, ref):.3e}", flush=True)
+    assert rx.lib.rx_last_status() == 0 and rel_err(y, ref) < TOL64
+
+
+@pytest.mark.parametrize("switch", [1.0, 2.0, -1.0])
+def test_relxilllp_returning_radiation(relx, switch):
+    """relxilllp with returning radiation: at spins <= 0 the plain returning
+    emissivity, above 0 with the zones' xillver correction factors
+    (src/Relxill.cpp:337-344); heights, moving sources, both signs of
+    refl_frac; one batch, FP64 and FP32, outputs and the zones' angle
+    distributions against the oracle."""
+    T, xt = relx
+    e = log_grid(1200, 0.1, 100.0)
+    ps = []
+    for a, h, beta, rf in ((-0.6, 6.0, 0.0, 1.0), (0.0, 12.0, 0.2, -1.0), (0.5, 4.0, 0.0, 2.0),
+                           (0.998, 6.0, 0.0, 1.0), (0.9, 30.0, 0.4, -2.0), (0.3, 3.5, 0.1, 0.5)):
+        ps.append([h, beta, a, 35.0, -1.0, 400.0, 0.0, 2.1, 2.7, 1.5, 120.0, rf, switch, 0.0])
+    refs = [oracle.relxill_model(oracle.MOD_RELXILLLP, T, xt, p, e) for p in ps]
+    b = rx.Batch([dict(model=rx.MODEL_RELXILLLP, energy=e, params=p) for p in ps])
+    try:
+        for prec, tol in ((rx.FP64, TOL64), (rx.FP32, TOL32)):
+            b.run(prec)
+            b.sync()
+            worst, worst_d = 0.0, 0.0
+            for i, (ref, dist_ref, st) in enumerate(refs):
+                assert st == 0, (i, st)
+                y, st_g = b.output(i)
+                assert st_g == 0, (i, rx.lib.rx_last_error())
+                worst = max(worst, rel_err(y, ref))
+                worst_d = max(worst_d, rel_err(b.dist(i, 100), dist_ref.reshape(-1)))
+            print(f"relxilllp returning radiation switch {switch} prec {prec}: rel {worst:.3e} dist {worst_d:.3e}",
+                  flush=True)
+            assert worst < tol and worst_d < tol
+    finally:
+        b.close()
 
 
 def test_relxilllp_batched_with_relxill(relx):

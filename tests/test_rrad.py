@@ -139,3 +139,132 @@ def test_library_rradtable_loader_round_trip(tmp_path):
             assert np.array_equal(v[k], np.asarray(t[k], dtype=np.float64)), k
     finally:
         rx.rradtable_synthetic()
+
+
+# ---- relxilllp: the xillver correction factors of the returning radiation ----
+
+@pytest.mark.parametrize("gam", [1.6, 2.0, 2.9])
+def test_gshift_boost_closed_forms(gam):
+    """corrected_gshift_fluxboost_factor (src/Relreturn_Corona.cpp:39-83): a
+    correction c = 1 is the power law's g^Gamma; at g = 1.5 (the shift c is
+    defined at) and at g = 2/3 both the parabola (c < 1) and the line (c >= 1)
+    give c g^Gamma; below g = 1 the boost is at most 1, and never negative."""
+    f = oracle.lib.orc_rrad_gshift_boost
+    for g in (0.4, 0.9, 1.0, 1.3, 2.5):
+        assert abs(f(1.0, g, gam) / g ** gam - 1) < 1e-14 or (g < 1 and g ** gam > 1)
+    for c in (0.35, 0.8, 1.0, 1.25, 1.9):
+        assert abs(f(c, 1.5, gam) / (c * 1.5 ** gam) - 1) < 1e-13
+        want = min(1.0, c * (2 / 3) ** gam)
+        assert abs(f(c, 2 / 3, gam) - want) < 1e-13 * want
+    assert f(3.0, 0.9, gam) == 1.0           # clamped: no boost below g = 1
+    assert f(0.1, 3.0, gam) == 0.0           # the parabola below zero: clamped
+
+
+def _power_law_xilltable(gam_t, n_ener=600):
+    """A xillver table whose every spectrum (all parameters and inclinations)
+    is the bin-integrated photon power law E^-gam_t stored times 10^logXi."""
+    from tests.helpers import xill_axes
+    ax = xill_axes()
+    e = np.geomspace(0.07, 1000.0, n_ener + 1)
+    g = (e[1:] ** (1 - gam_t) - e[:-1] ** (1 - gam_t)) / (1 - gam_t)
+    L = np.meshgrid(*ax, indexing="ij")[2]
+    spec = (10.0 ** L)[..., None] * g
+    return dict(n_par=5, pid=[0, 1, 2, 3, 7], n_val=[len(a) for a in ax],
+                val=[np.asarray(a, dtype=np.float32) for a in ax], n_ener=n_ener,
+                elo=e[:-1].astype(np.float32), ehi=e[1:].astype(np.float32),
+                spec=spec.reshape(-1, n_ener).astype(np.float32)), e, g
+
+
+@pytest.mark.parametrize("gam_t,gam,ect", [(2.0, 2.0, 300.0), (1.7, 2.4, 80.0), (2.6, 1.9, 600.0)])
+def test_correction_factors_of_a_power_law_table(gam_t, gam, ect):
+    """get_xillver_fluxcorrection_factors (src/Xillspec.cpp:500-526) on a table
+    of identical photon power laws E^-gam_t: (1) the spectrum shifted to 1.5
+    times lower energies carries 1.5^-gam_t of the photons of any band it
+    covers, so the g-shift factor is 1.5^(gam_t - Gamma) (Gamma the model's
+    index); a shift the other way, or no time-dilation factor 2/3, misses;
+    (2) the flux factor is W times the band energy flux of the spectrum over
+    that of the normalised primary, which is 1e15 / (4 pi 1e20 1.602177e-9)
+    (the normalisation's definition, src/Xillspec.cpp:179-205); W is the sum
+    of the angle weights 0.5 cos(i)/n_incl of calc_angle_averaged_xill_spec,
+    where the table's inclination (degrees) is read as radians."""
+    x, e, g = _power_law_xilltable(gam_t)
+    xt = oracle.Xill(x)
+    xp = oracle.xill_param(gam, 3.0, 1.0, ect)
+    cf, cg = oracle.xill_corr_factors(xt, xp)
+    want_g = 1.5 ** (gam_t - gam)
+    assert abs(cg / want_g - 1) < 2e-3
+    assert abs(cg / 1.5 ** (-gam_t - gam) - 1) > 0.5 and abs(cg / (want_g * 1.5) - 1) > 0.2
+    incl = np.asarray(x["val"][4], dtype=np.float64)
+    n_incl = incl.size
+    W = np.sum(0.5 * np.cos(incl) / n_incl)
+    W_deg = np.sum(0.5 * np.cos(np.radians(incl)) / n_incl)
+    ef = e.astype(np.float32).astype(np.float64)
+    em = 0.5 * (ef[1:] + ef[:-1])
+    gf = g.astype(np.float32).astype(np.float64)
+    band = (ef[:-1] >= 0.1) & (ef[1:] <= 1000.0)
+    q = 1e15 / (4 * np.pi * 1e20 * 1.602177e-9)
+    want_f = W * np.sum(gf[band] * em[band]) / q
+    assert abs(cf / want_f - 1) < 1e-3
+    assert abs(W_deg / W - 1) > 0.1  # the degree reading would be visible
+
+
+def test_returning_emissivity_with_correction_factors():
+    """The correction factors in calc_rrad_emis_corona (src/Relreturn_Corona.cpp
+    :119-151): with every returning photon at g = 1.5, a g-shift factor c turns
+    the weight 1.5^(Gamma-1) into boost(c, 1.5)/1.5 = c 1.5^(Gamma-1), and the
+    flux factor multiplies the zone's sum: equal factors in every zone scale
+    the returning emissivity by c times the flux factor at every radius;
+    c within 1e-3 of 1 leaves the plain weight."""
+    tf = [[0.03, 0.07], [0.011, 0.005]]
+    t = two_zone_table(tf, g0=1.5)
+    p, st = oracle.rel_params(oracle.MOD_RELLINELP, LP_P)
+    re = oracle.fine_radial_grid(p.rin, p.rout, 1000)
+    e0 = re ** -3.0
+    er0, st = oracle.rrad_emis(t, p, re, e0)
+    assert st == 0
+    rg = np.array([p.rin, 30.0, 200.0, p.rout])
+    for c, fl in ((0.7, 2.5), (1.4, 0.8), (1.0005, 1.7)):
+        cf = np.full(3, fl)
+        cg = np.full(3, c)
+        p.rr_corr_flux = cf.ctypes.data_as(oracle._D)
+        p.rr_corr_gshift = cg.ctypes.data_as(oracle._D)
+        p.rr_corr_rgrid = rg.ctypes.data_as(oracle._D)
+        p.rr_corr_n = 3
+        er, st = oracle.rrad_emis(t, p, re, e0)
+        assert st == 0
+        want = fl * (c if abs(c - 1) > 1e-3 else 1.0)
+        assert np.abs(er / er0 / want - 1).max() < 1e-12, c
+    p.rr_corr_n = 0
+
+
+def test_correction_factors_follow_the_zone_of_each_table_centre():
+    """rebin_corrfactors_to_rradtable_grid (src/Relreturn_Corona.cpp:248-289):
+    a table zone takes the factors of the relxill zone its centre falls in.
+    Two table zones whose centres fall in different relxill zones: the
+    returning emissivity at each centre (the log-linear profile's value
+    there) is scaled by that zone's flux factor; one zone off would swap
+    them."""
+    tf = [[0.03, 0.07], [0.011, 0.005]]
+    t = two_zone_table(tf, g0=1.0)
+    p, st = oracle.rel_params(oracle.MOD_RELLINELP, LP_P)
+    re = oracle.fine_radial_grid(p.rin, p.rout, 1000)
+    e0 = re ** -3.0
+    er0, st = oracle.rrad_emis(t, p, re, e0)
+    assert st == 0
+    rad = np.array([0.5 * (re[-1] + 20.0), 0.5 * (20.0 + re[0])])
+    rg = np.array([p.rin, 10.0, 100.0, 400.0, p.rout])  # centres (12.1, 510) in zones 1 and 3
+    cf = np.array([5.0, 2.0, 7.0, 3.0])
+    cg = np.ones(4)
+    p.rr_corr_flux = cf.ctypes.data_as(oracle._D)
+    p.rr_corr_gshift = cg.ctypes.data_as(oracle._D)
+    p.rr_corr_rgrid = rg.ctypes.data_as(oracle._D)
+    p.rr_corr_n = 4
+    er, st = oracle.rrad_emis(t, p, re, e0)
+    p.rr_corr_n = 0
+    assert st == 0
+    c0a, c1a = _line_through(re, er)
+    c0b, c1b = _line_through(re, er0)
+    ratio = np.exp((c0a - c0b) + (c1a - c1b) * rad)
+    want = np.array([cf[np.searchsorted(rg, x) - 1] for x in rad])
+    assert list(want) == [2.0, 3.0]
+    assert np.abs(ratio / want - 1).max() < 1e-9

@@ -356,6 +356,3 @@ def test_relxilllp_refl_frac_and_boost(relxlp):
     rp, _ = oracle.rel_params(oracle.MOD_RELXILLLP, RELXILLLP_P)
     sp, _ = oracle.system_parameters(T, rp)
     assert rel_err(run(-1.0, 1.0), sp["refl"][0] * r1) < 1e-13
-    p = list(RELXILLLP_P)
-    p[12] = 1.0  # returning radiation is not built
-    assert oracle.relxill_model(oracle.MOD_RELXILLLP, T, xt, p, e)[2] == 2
