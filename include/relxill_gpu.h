@@ -114,6 +114,24 @@ int rx_eval_model(int model, const double *parameter, const double *energy, doub
 int rx_last_status(void);
 const char *rx_last_error(void);
 
+/* Allocation-free variant of rx_eval_model.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc).  One request of relline, relline_lp,
+ * relconv or relconv_lp on device data the caller owns: d_energy (n_flux+1
+ * edges), d_flux_in (the input spectrum of the convolution models, n_flux;
+ * may equal d_flux_out; unused by the line models), d_flux_out (n_flux);
+ * the parameter array in host memory.  The kernels' work buffers come from a
+ * fixed device array of RX_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends; a request that needs more (grid size, table dims, fine grid)
+ * is rejected with RX_ERR_ARG.  Nothing is allocated: the library must be
+ * initialised (rx_init) with its tables resident (a changed table is
+ * uploaded by rx_init or any other entry point, not by this one).  Runs on
+ * `stream` (a hipStream_t; NULL: the null stream) and returns once it is
+ * done, with the request's status.  Every call on a GPU uses the same
+ * array: calls that overlap on one GPU race. */
+#define RX_NOALLOC_SCRATCH_BYTES (4u << 20)
+int rx_eval_model_noalloc(int model, const double *parameter, const double *d_energy, const double *d_flux_in,
+                          double *d_flux_out, int n_flux, int precision, void *stream);
 
 /* Processed parameters of one XSPEC parameter array (get_rel_params with
  * check_parameter_bounds, src/ModelDefinition.cpp:168-334): out12 receives

@@ -138,6 +138,8 @@ _EXPORTS = {
     "rx_batch_kernel_ms": (ctypes.c_int, [ctypes.c_void_p, _D]),
     "rx_batch_counters": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "rx_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "rx_eval_model_noalloc": (ctypes.c_int, [ctypes.c_int, _D, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "rx_set_graphs": (ctypes.c_int, [ctypes.c_int]),
     "rx_batch_graph_replays": (ctypes.c_int64, [ctypes.c_void_p]),
     "rx_server_start": (ctypes.c_void_p, [ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int]),

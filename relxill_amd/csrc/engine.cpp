@@ -14,14 +14,20 @@ namespace {
 constexpr double kPi = 3.14159265358979323846;
 thread_local int t_status = 0;
 thread_local std::string t_error;
+thread_local const char *t_error_static = nullptr;  // set instead of t_error where nothing may allocate
 }  // namespace
 
 void set_last(int status, const std::string &msg) {
   t_status = status;
   t_error = msg;
+  t_error_static = nullptr;
+}
+void set_last_static(int status, const char *msg) {
+  t_status = status;
+  t_error_static = msg ? msg : "";
 }
 int last_status() { return t_status; }
-const char *last_error() { return t_error.c_str(); }
+const char *last_error() { return t_error_static ? t_error_static : t_error.c_str(); }
 
 Global &G() {
   static Global g;
@@ -460,6 +466,9 @@ int ensure_device(std::string &err) {
     int n_cu = 0;
     if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, g.device) == hipSuccess && n_cu > 0)
       g.n_cu = n_cu;
+    // the device address of the allocation-free entry point's work array
+    const int st = noalloc_setup(g.device, err);
+    if (st != RX_OK) return st;
     g.device_ready = true;
   } else {
     (void)hipSetDevice(g.device);

@@ -163,7 +163,11 @@ int rrad_zones(const RradTableHost &t, ReqParams &p, int nr_fine);
 int ensure_xilltable_device(std::string &err);
 void fill_dev_views(BatchDev &bd);
 void set_last(int status, const std::string &msg);
-
+// the same with a static message (no allocation)
+void set_last_static(int status, const char *msg);
+// Looks up the device address of the allocation-free entry point's scratch
+// array on `device` (current) once (noalloc.hip).
+int noalloc_setup(int device, std::string &err);
 int last_status();
 const char *last_error();
 bool phys_norm_env();
