@@ -455,7 +455,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(requests shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+              "(requests shard across GPUs with no exchange between them)", file=sys.stderr)
         sys.exit(2)
 
     import relxill_amd as rx

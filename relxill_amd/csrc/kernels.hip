@@ -1064,7 +1064,7 @@ __global__ __launch_bounds__(RRC_THREADS) void k_rrad_corr(BatchDev b, const Xil
 
 // Clears the run's counters; launched before k_setup on the batch's stream.
 __global__ __launch_bounds__(64) void k_reset(int *ctr) {
-  if (threadIdx.x < N_COUNTERS) ctr[threadIdx.x] = 0;
+  for (int i = threadIdx.x; i < N_COUNTERS; i += 64) ctr[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1092,16 +1092,11 @@ __device__ inline void lerp_row(double ifac, const double *lo, const double *hi,
   }
 }
 
+// The lane's rows from node rows tlo (node K+1) and thi (node K), in global
+// memory or staged in LDS; clo / chi: the cosines (limb != 0).
 template <class TV>
-__device__ inline void load_radius(const BatchDev &b, int req, const RadRec &r, int limb, TV *myT, TV *myC,
-                                   RadiusView &v) {
-  const int nq = 2 * b.rt.n_g;
-  const double *node = b.node + (size_t)req * b.node_stride;
-  const size_t nq2 = (size_t)b.rt.n_r * nq;
-  const double *tlo = node + (size_t)(r.K + 1) * nq;
-  const double *thi = node + (size_t)r.K * nq;
-  const double *clo = tlo + nq2;
-  const double *chi = thi + nq2;
+__device__ inline void load_radius_from(const RadRec &r, int nq, int limb, const double *tlo, const double *thi,
+                                        const double *clo, const double *chi, TV *myT, TV *myC, RadiusView &v) {
   lerp_row<TV>(r.ifac_r, tlo, thi, nq, myT);
   if (limb) lerp_row<TV>(r.ifac_r, clo, chi, nq, myC);
   v.gmin = r.gmin;
@@ -1110,6 +1105,36 @@ __device__ inline void load_radius(const BatchDev &b, int req, const RadRec &r, 
   v.limb = limb;
   v.normL = edge_norm_node(false, v, r.ifac_r, tlo, thi, clo, chi);
   v.normU = edge_norm_node(true, v, r.ifac_r, tlo, thi, clo, chi);
+}
+
+template <class TV>
+__device__ inline void load_radius(const BatchDev &b, int req, const RadRec &r, int limb, TV *myT, TV *myC,
+                                   RadiusView &v) {
+  const int nq = 2 * b.rt.n_g;
+  const double *node = b.node + (size_t)req * b.node_stride;
+  const size_t nq2 = (size_t)b.rt.n_r * nq;
+  const double *tlo = node + (size_t)(r.K + 1) * nq;
+  const double *thi = node + (size_t)r.K * nq;
+  load_radius_from<TV>(r, nq, limb, tlo, thi, tlo + nq2, thi + nq2, myT, myC, v);
+}
+
+// The node rows kmin .. kmin + nrow - 1 of a request (contiguous, nq values
+// each) copied to LDS by the whole wavefront in one round of loads, at most
+// STAGE_MAX doubles (STAGE_MAX / 64 per lane).
+constexpr int STAGE_MAX = 640;
+__device__ inline void stage_node_rows(const double *src, int tot, double *dst, int lane) {
+  constexpr int PER = STAGE_MAX / 64;
+  double t[PER];
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int q = lane + 64 * k;
+    t[k] = (q < tot) ? src[q] : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int q = lane + 64 * k;
+    if (q < tot) dst[q] = t[k];
+  }
 }
 
 __device__ inline const double *profile_grid(const BatchDev &b, const ReqParams *rq) {
@@ -1127,7 +1152,8 @@ __host__ __device__ inline size_t integrate_rows_bytes(int ng, bool cos_rows, si
 }
 
 // Persistent wavefronts pull work items (request, chunk of 64 fine radii, bin
-// segment) from a shared counter until the list is exhausted.  Lane l owns
+// segment) from the per-XCD ranges of the work list until every range is
+// exhausted.  Lane l owns
 // fine radius 64 rc + l: its transfer-function row sits in LDS and its window,
 // weight and edge norms in registers.  For every bin of the segment each lane
 // integrates the bin at its radius (zero outside its window) and parks the
@@ -1142,8 +1168,15 @@ __host__ __device__ inline size_t integrate_rows_bytes(int ng, bool cos_rows, si
 //   (ReqNorm.cmul) as FP32, and the atomic adds the sum times ReqNorm.cinv.
 //   Bins whose Romberg stop test fell in the guard band are queued for the
 //   FP64 re-check (k_fixup).
+//   Item set-up: the node rows that the chunk's radii lerp between (nodes
+//   kmin .. kmax + 1, at most stage_rows of them) are copied to LDS by the
+//   whole wavefront in one round of coalesced loads; each lane then lerps its
+//   row and its FP64 edge norms from LDS (the same values and arithmetic as
+//   reading the rows directly, which is the fallback for wider node ranges and
+//   for emission-angle rows).  The staging area is the parked-value area,
+//   which is free between items.
 template <int MODE>
-__global__ __launch_bounds__(64, MODE == 0 ? 1 : 4) void k_integrate(BatchDev b, int cos_rows) {
+__global__ __launch_bounds__(64, MODE == 0 ? 1 : 3) void k_integrate(BatchDev b, int cos_rows, int stage_rows) {
   using TV = typename std::conditional<MODE == 0, double, float>::type;
   extern __shared__ double s_dyn[];
   const int lane = threadIdx.x;
@@ -1178,11 +1211,24 @@ __global__ __launch_bounds__(64, MODE == 0 ? 1 : 4) void k_integrate(BatchDev b,
     v.T32 = reinterpret_cast<const float *>(myT);
     v.C32 = reinterpret_cast<const float *>(myC);
   }
+  int xq = blockIdx.x % N_XQ, exhausted = 0;
   for (;;) {
-    int k = 0;
-    if (lane == 0) k = atomicAdd(&b.ctr[CTR_NEXT], 1);
-    k = __builtin_amdgcn_readfirstlane(k);
-    if (k >= n_items) break;
+    // next item of range xq; an empty range sends the wavefront to the next
+    // one, and once all N_XQ have been found empty it is done
+    int k = -1;
+    while (exhausted < N_XQ) {
+      const int lo = (int)((int64_t)n_items * xq / N_XQ), hi = (int)((int64_t)n_items * (xq + 1) / N_XQ);
+      int kk = 0;
+      if (lane == 0) kk = atomicAdd(&b.ctr[CTR_XQ + xq * XQ_STRIDE], 1);
+      kk = __builtin_amdgcn_readfirstlane(kk);
+      if (kk < hi - lo) {
+        k = lo + kk;
+        break;
+      }
+      exhausted++;
+      xq = (xq + 1) % N_XQ;
+    }
+    if (k < 0) break;
     const WorkItem it = b.items[k];
     const ReqParams *rq = b.req + it.req;
     const ReqNorm rn = b.rnorm[it.req];
@@ -1193,15 +1239,32 @@ __global__ __launch_bounds__(64, MODE == 0 ? 1 : 4) void k_integrate(BatchDev b,
     r.ielo = 1;
     r.iehi = 0;
     r.coef = 0.0;
+    r.K = 0;
     if (i < nr) {
       r = b.rad[(size_t)it.req * nr + i];
       if (r.pad != zone) {  // a radius of another zone of the chunk
         r.ielo = 1;
         r.iehi = 0;
       }
-      if (r.ielo <= r.iehi) load_radius<TV>(b, it.req, r, limb, myT, myC, v);
     }
     const bool has_win = r.ielo <= r.iehi;
+    const int kmin = wave_min_i(has_win ? r.K : INT_MAX);
+    const int kmax = wave_max_i(has_win ? r.K : -1);
+    if (!limb && kmax >= 0 && kmax - kmin + 2 <= stage_rows) {
+      const int nq = 2 * ng;
+      double *stage = reinterpret_cast<double *>(s_bins);
+      stage_node_rows(b.node + (size_t)it.req * b.node_stride + (size_t)kmin * nq, (kmax - kmin + 2) * nq, stage,
+                      lane);
+      __syncthreads();  // one wavefront: orders the staged rows before the lanes read them
+      if (has_win) {
+        const double *tlo = stage + (size_t)(r.K + 1 - kmin) * nq;
+        const double *thi = stage + (size_t)(r.K - kmin) * nq;
+        load_radius_from<TV>(r, nq, 0, tlo, thi, nullptr, nullptr, myT, myC, v);
+      }
+      __syncthreads();  // the bins overwrite the staging area
+    } else if (has_win) {
+      load_radius<TV>(b, it.req, r, limb, myT, myC, v);
+    }
     const double w64 = has_win ? r.coef * rn.fac : 0.0;   // MODE 0 weight
     const double w32 = has_win ? r.coef * rn.cmul : 0.0;  // MODE 1 scaled weight
     if constexpr (MODE == 0) s_coef[lane] = w64;
@@ -2175,24 +2238,43 @@ static size_t row_lds(const BatchDev &b, bool cos_rows, size_t elem) {
   return (size_t)(cos_rows ? 2 : 1) * 64 * (2 * b.rt.n_g + 1) * elem;
 }
 
-// lane rows, then (FP64 path) 64 weight slots, then the SUM_BINS x SUM_STRIDE parked values
-static size_t integrate_lds(const BatchDev &b, bool cos_rows, size_t elem) {
+// LDS of one k_integrate wavefront: lane rows, then (FP64 path) 64 weight
+// slots, then the SUM_BINS x SUM_STRIDE parked values, which double as the
+// node-row staging area.  *stage_rows: node rows the staging area takes (0:
+// none), as many as fit without lowering the wavefronts per CU that the LDS
+// allows (at most 12, the VGPR limit of the FP32 path), up to 8.
+static size_t integrate_lds(const BatchDev &b, bool cos_rows, size_t elem, int *stage_rows) {
   const size_t rows = integrate_rows_bytes(b.rt.n_g, cos_rows, elem);
   const size_t w = (elem == sizeof(double)) ? 64 * sizeof(double) : 0;
-  return rows + w + (size_t)SUM_BINS * SUM_STRIDE * elem;
+  const size_t park = (size_t)SUM_BINS * SUM_STRIDE * elem;
+  const size_t nq = 2 * (size_t)b.rt.n_g;
+  auto per_cu = [](size_t lds) { return std::min<size_t>(12, 163840 / ((lds + 511) & ~(size_t)511)); };
+  const size_t base = rows + w + park;
+  *stage_rows = 0;
+  if (cos_rows) return base;
+  for (int c = 8; c >= 2; c--) {
+    if ((size_t)c * nq > (size_t)STAGE_MAX) continue;
+    const size_t need = rows + w + std::max(park, (size_t)c * nq * sizeof(double));
+    if (per_cu(need) >= per_cu(base)) {
+      *stage_rows = c;
+      return need;
+    }
+  }
+  return base;
 }
 
 int launch_integrate(const BatchDev &b, int n_blocks, bool cos_rows, void *stream) {
   if (b.n_req <= 0 || n_blocks <= 0) return 0;
   hipStream_t st = (hipStream_t)stream;
+  int stage = 0;
   if (b.mode == 0) {
-    const size_t lds = integrate_lds(b, cos_rows, sizeof(double));
+    const size_t lds = integrate_lds(b, cos_rows, sizeof(double), &stage);
     (void)hipFuncSetAttribute((const void *)k_integrate<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_integrate<0>, dim3(n_blocks), dim3(64), lds, st, b, (int)cos_rows);
+    hipLaunchKernelGGL(k_integrate<0>, dim3(n_blocks), dim3(64), lds, st, b, (int)cos_rows, stage);
   } else {
-    const size_t lds = integrate_lds(b, cos_rows, sizeof(float));
+    const size_t lds = integrate_lds(b, cos_rows, sizeof(float), &stage);
     (void)hipFuncSetAttribute((const void *)k_integrate<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_integrate<1>, dim3(n_blocks), dim3(64), lds, st, b, (int)cos_rows);
+    hipLaunchKernelGGL(k_integrate<1>, dim3(n_blocks), dim3(64), lds, st, b, (int)cos_rows, stage);
   }
   return (int)hipGetLastError();
 }

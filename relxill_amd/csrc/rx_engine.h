@@ -224,14 +224,22 @@ struct FixEntry {
 // k_setup status of a request whose work items did not fit the work list.
 constexpr int ST_WORKLIST = 60;
 
+// k_integrate splits the work list into N_XQ contiguous ranges, one per XCD
+// of the MI355X (workgroups are dispatched to the XCDs round-robin), each
+// handed out by its own counter; a wavefront takes items of its own range
+// first, so the items of one request (which share node rows) are integrated
+// from one XCD's L2, and moves on to the next range when its own is empty.
+constexpr int N_XQ = 8;
+constexpr int XQ_STRIDE = 32;  // ints between the range counters (one 128-byte line each)
+
 // Device counters of one batch run (cleared by k_reset before k_setup).
 enum CounterSlot : int {
   CTR_FIX = 0,        // FixEntry entries written
   CTR_FIX_OVF = 1,    // FixEntry queue overflowed
   CTR_ITEMS = 2,      // work items written
   CTR_ITEMS_OVF = 3,  // work list overflowed
-  CTR_NEXT = 4,       // next work item to integrate
-  N_COUNTERS = 8
+  CTR_XQ = XQ_STRIDE,  // N_XQ range counters, XQ_STRIDE apart
+  N_COUNTERS = CTR_XQ + N_XQ * XQ_STRIDE
 };
 
 struct BatchDev {
