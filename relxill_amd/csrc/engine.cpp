@@ -766,6 +766,8 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
     }
   }
   int64_t o_ener = 0, o_flux = 0, o_pgrid = 0, o_prof = 0, o_xspec = 0, o_dist = 0, o_zone = 0, o_rrc = 0;
+  int64_t o_node = 0;
+  const int64_t node_stride = 2 * (int64_t)g.rel.n_r * g.rel.n_g * 2;
   for (int i = 0; i < n_req; i++) {
     const rx_request &r = req[i];
     if (r.n_flux <= 0 || !r.energy || !r.parameter) {
@@ -846,6 +848,11 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
     p.need_cosne = (p.limb != 0 || p.n_incl > 0) ? 1 : 0;
     p.dist_off = o_dist;
     o_dist += (int64_t)p.n_incl * p.n_zones;
+    p.node_off = -1;
+    if (p.n_incl > 0) {
+      p.node_off = o_node;
+      o_node += node_stride;
+    }
     p.pre_status = st;
     b.h_status[i] = st;
   }
@@ -855,6 +862,7 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
   b.tot_prof = o_prof;
   b.tot_xspec = o_xspec;
   b.tot_dist = o_dist;
+  b.tot_node = o_node;
   b.tot_zone = o_zone;
   b.tot_rrc = o_rrc;
   b.n_xill1 = (int)b.h_xill.size();
@@ -928,7 +936,7 @@ int batch_upload(Batch &b) {
   e |= b.d_flux_in.ensure(sizeof(double) * b.tot_flux);
   e |= b.d_out.ensure(sizeof(double) * b.tot_flux);
   e |= b.d_pgrid.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_pgrid));
-  e |= b.d_node.ensure(sizeof(double) * nreq * 2 * (size_t)g.rel.n_r * g.rel.n_g * 2);
+  e |= b.d_node.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_node));
   e |= b.d_rad.ensure(sizeof(RadRec) * nreq * g.nr_fine);
   e |= b.d_rnorm.ensure(sizeof(ReqNorm) * nreq);
   e |= b.d_prof.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_prof));

@@ -60,6 +60,7 @@ struct Batch {
   int n_xill1 = 0;               // number of one-zone units at the front of h_xill
   std::vector<XillWork> h_rrc;   // (request, zone) units of the returning-radiation corrections
   int64_t tot_ener = 0, tot_flux = 0, tot_pgrid = 0, tot_prof = 0, tot_xspec = 0, tot_dist = 0, tot_zone = 0;
+  int64_t tot_node = 0;  // node rows of the requests with an angle distribution
   int64_t tot_rrc = 0;
   DevBuf d_req, d_ener, d_flux_in, d_out, d_pgrid, d_node, d_rad, d_prof, d_dist, d_refl, d_status, d_conv;
   DevBuf d_xill, d_xspec, d_zone, d_prim, d_rrc, d_rrcorr;

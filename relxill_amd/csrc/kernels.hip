@@ -525,8 +525,8 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     s_tgmax[ir] = lerp2f(fa, fm, T.gmax[n00 * n_r + ir], T.gmax[n10 * n_r + ir],
                          T.gmax[n01 * n_r + ir], T.gmax[n11 * n_r + ir]);
   }
-  {
-    double *node = b.node + (size_t)req * b.node_stride;
+  if (rq.n_incl > 0) {  // node rows for the angle distribution below
+    double *node = b.node + rq.node_off;
     const size_t nq = (size_t)n_r * ng;
     const size_t s00 = n00 * nq, s10 = n10 * nq, s01 = n01 * nq, s11 = n11 * nq;
     for (size_t q = tid; q < nq; q += SETUP_THREADS) {
@@ -759,6 +759,10 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
   if (tid == 0) {
     ReqNorm n;
     n.fac = fac;
+    n.fa = fa;
+    n.fm = fm;
+    n.ia = ia;
+    n.im = im;
     const double m = cmax_all * fac;
     int k = 0;
     if (m > 0.0 && m < INFINITY) k = min(1000, max(-1000, -ilogb(m)));
@@ -840,7 +844,7 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     const int nd = rq.n_incl * nz;  // [zone][angle bin]
     for (int k = tid; k < nd; k += SETUP_THREADS) s_dist[k] = 0.0;
     __syncthreads();
-    const double *node = b.node + (size_t)req * b.node_stride;
+    const double *node = b.node + rq.node_off;
     const size_t nq2 = (size_t)n_r * ng * 2;
     const int cols = ng * 2;
     const double step = (1.0 - 2 * GFAC_H) / (ng - 1);
@@ -1092,49 +1096,104 @@ __device__ inline void lerp_row(double ifac, const double *lo, const double *hi,
   }
 }
 
-// The lane's rows from node rows tlo (node K+1) and thi (node K), in global
-// memory or staged in LDS; clo / chi: the cosines (limb != 0).
-template <class TV>
-__device__ inline void load_radius_from(const RadRec &r, int nq, int limb, const double *tlo, const double *thi,
-                                        const double *clo, const double *chi, TV *myT, TV *myC, RadiusView &v) {
-  lerp_row<TV>(r.ifac_r, tlo, thi, nq, myT);
-  if (limb) lerp_row<TV>(r.ifac_r, clo, chi, nq, myC);
+// A request's (a, mu0)-interpolated node rows (interpol_a_mu0,
+// src/Relprofile.cpp:39-80), read from the table's four corners: row ir,
+// column c (g* column c >> 1, branch c & 1) of the transfer function or of
+// the emission-angle cosines.  k_setup computes the same lerp2f for the rows
+// of the angle distribution, so every consumer sees the same bits.
+struct NodeSrc {
+  const float *a1, *a2;  // branch 0 and 1 arrays
+  size_t s00, s10, s01, s11;
+  double fa, fm;
+  int ng;
+};
+
+__device__ inline NodeSrc node_src(const DevRelTable &T, const ReqNorm &rn, bool cos) {
+  NodeSrc s;
+  s.a1 = cos ? T.cosne1 : T.trff1;
+  s.a2 = cos ? T.cosne2 : T.trff2;
+  const size_t nq = (size_t)T.n_r * T.n_g;
+  s.s00 = ((size_t)rn.ia * T.n_mu0 + rn.im) * nq;
+  s.s10 = ((size_t)(rn.ia + 1) * T.n_mu0 + rn.im) * nq;
+  s.s01 = ((size_t)rn.ia * T.n_mu0 + rn.im + 1) * nq;
+  s.s11 = ((size_t)(rn.ia + 1) * T.n_mu0 + rn.im + 1) * nq;
+  s.fa = rn.fa;
+  s.fm = rn.fm;
+  s.ng = T.n_g;
+  return s;
+}
+
+__device__ inline double node_value(const NodeSrc &s, int ir, int c) {
+  const float *arr = (c & 1) ? s.a2 : s.a1;
+  const size_t q = (size_t)ir * s.ng + (c >> 1);
+  return lerp2f(s.fa, s.fm, arr[s.s00 + q], arr[s.s10 + q], arr[s.s01 + q], arr[s.s11 + q]);
+}
+
+// Node rows kb .. kb + tot / nq - 1 (nq = 2 n_g values each, contiguous) built
+// into LDS by the whole wavefront: every lane loads the four corners of PER
+// values per round (4 PER loads in flight), then interpolates them; at most
+// STAGE_MAX values.
+constexpr int STAGE_MAX = 640;
+template <int PER>
+__device__ inline void stage_table_rows(const NodeSrc &s, int kb, int tot, int nq, double *dst, int lane) {
+  int row = kb + lane / nq, col = lane % nq;
+  for (int k0 = 0; k0 < STAGE_MAX / 64 && 64 * k0 < tot; k0 += PER) {
+    float c00[PER], c10[PER], c01[PER], c11[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      if (lane + 64 * (k0 + k) < tot) {
+        const float *arr = (col & 1) ? s.a2 : s.a1;
+        const size_t q = (size_t)row * s.ng + (col >> 1);
+        c00[k] = arr[s.s00 + q];
+        c10[k] = arr[s.s10 + q];
+        c01[k] = arr[s.s01 + q];
+        c11[k] = arr[s.s11 + q];
+      }
+      col += 64;
+      while (col >= nq) {
+        col -= nq;
+        row++;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int idx = lane + 64 * (k0 + k);
+      if (idx < tot) dst[idx] = lerp2f(s.fa, s.fm, c00[k], c10[k], c01[k], c11[k]);
+    }
+  }
+}
+
+// k_fixup: one lane's FP64 row values, straight from the table, for the g*
+// brackets that the FP64 integral of the bin [e0, e1] reads (its points lie
+// between the bin's clamped ends, whose brackets bound theirs; one bracket of
+// margin either side) and for the brackets of the two edge norms; the other
+// columns of the row are not read.
+__device__ inline void load_radius_table(const BatchDev &b, const ReqNorm &rn, const RadRec &r, int limb, double e0,
+                                         double e1, double *myT, double *myC, RadiusView &v) {
   v.gmin = r.gmin;
   v.gmax = r.gmax;
   v.del_g = 1. / (r.gmax - r.gmin);  // set_str_relbf, src/Relprofile.cpp:733
   v.limb = limb;
-  v.normL = edge_norm_node(false, v, r.ifac_r, tlo, thi, clo, chi);
-  v.normU = edge_norm_node(true, v, r.ifac_r, tlo, thi, clo, chi);
-}
-
-template <class TV>
-__device__ inline void load_radius(const BatchDev &b, int req, const RadRec &r, int limb, TV *myT, TV *myC,
-                                   RadiusView &v) {
-  const int nq = 2 * b.rt.n_g;
-  const double *node = b.node + (size_t)req * b.node_stride;
-  const size_t nq2 = (size_t)b.rt.n_r * nq;
-  const double *tlo = node + (size_t)(r.K + 1) * nq;
-  const double *thi = node + (size_t)r.K * nq;
-  load_radius_from<TV>(r, nq, limb, tlo, thi, tlo + nq2, thi + nq2, myT, myC, v);
-}
-
-// The node rows kmin .. kmin + nrow - 1 of a request (contiguous, nq values
-// each) copied to LDS by the whole wavefront in one round of loads, at most
-// STAGE_MAX doubles (STAGE_MAX / 64 per lane).
-constexpr int STAGE_MAX = 640;
-__device__ inline void stage_node_rows(const double *src, int tot, double *dst, int lane) {
-  constexpr int PER = STAGE_MAX / 64;
-  double t[PER];
-#pragma unroll
-  for (int k = 0; k < PER; k++) {
-    const int q = lane + 64 * k;
-    t[k] = (q < tot) ? src[q] : 0.0;
-  }
-#pragma unroll
-  for (int k = 0; k < PER; k++) {
-    const int q = lane + 64 * k;
-    if (q < tot) dst[q] = t[k];
-  }
+  const int ng = b.rt.n_g;
+  auto bracket = [&](double e) {
+    const double g = fmin(fmax((e - v.gmin) * v.del_g, GFAC_H), 1.0 - GFAC_H);
+    return g_bracket(g, ng, v.gs_step, v.inv_step);
+  };
+  const NodeSrc ts = node_src(b.rt, rn, false);
+  const NodeSrc cs = node_src(b.rt, rn, true);
+  auto fill = [&](int ia, int ib) {  // columns of brackets ia .. ib
+#pragma unroll 4
+    for (int c = 2 * ia; c <= 2 * ib + 3; c++) {
+      myT[c] = lerp1_(r.ifac_r, node_value(ts, r.K + 1, c), node_value(ts, r.K, c));
+      if (limb) myC[c] = lerp1_(r.ifac_r, node_value(cs, r.K + 1, c), node_value(cs, r.K, c));
+    }
+  };
+  fill(max(0, bracket(e0) - 1), min(ng - 2, bracket(e1) + 1));
+  const EdgePt pL = edge_point(false, v), pU = edge_point(true, v);
+  fill(pL.ind, pL.ind);
+  fill(pU.ind, pU.ind);
+  v.normL = edge_norm64(false, v);
+  v.normU = edge_norm64(true, v);
 }
 
 __device__ inline const double *profile_grid(const BatchDev &b, const ReqParams *rq) {
@@ -1169,14 +1228,16 @@ __host__ __device__ inline size_t integrate_rows_bytes(int ng, bool cos_rows, si
 //   Bins whose Romberg stop test fell in the guard band are queued for the
 //   FP64 re-check (k_fixup).
 //   Item set-up: the node rows that the chunk's radii lerp between (nodes
-//   kmin .. kmax + 1, at most stage_rows of them) are copied to LDS by the
-//   whole wavefront in one round of coalesced loads; each lane then lerps its
-//   row and its FP64 edge norms from LDS (the same values and arithmetic as
-//   reading the rows directly, which is the fallback for wider node ranges and
-//   for emission-angle rows).  The staging area is the parked-value area,
-//   which is free between items.
-template <int MODE>
-__global__ __launch_bounds__(64, MODE == 0 ? 1 : 3) void k_integrate(BatchDev b, int cos_rows, int stage_rows) {
+//   kmin .. kmax + 1) are built in LDS from the table's four (a, mu0)
+//   corners by the whole wavefront, stage_rows at a time, in one round of
+//   loads each; each lane then lerps its row and the FP64 values of its edge
+//   norms from LDS.  The staging area is the parked-value area, which is free
+//   between items.
+//   COS: the batch holds emission-angle rows (a request with limb
+//   darkening); COS == 0 compiles the limb factors out.
+template <int MODE, int COS>
+__global__ __launch_bounds__(64, MODE == 0 ? 1 : 3) void k_integrate(BatchDev b, int stage_rows) {
+  constexpr int cos_rows = COS;
   using TV = typename std::conditional<MODE == 0, double, float>::type;
   extern __shared__ double s_dyn[];
   const int lane = threadIdx.x;
@@ -1204,6 +1265,7 @@ __global__ __launch_bounds__(64, MODE == 0 ? 1 : 3) void k_integrate(BatchDev b,
   v.C64 = nullptr;
   v.T32 = nullptr;
   v.C32 = nullptr;
+  v.limb = 0;
   if (MODE == 0) {
     v.T64 = reinterpret_cast<const double *>(myT);
     v.C64 = reinterpret_cast<const double *>(myC);
@@ -1232,7 +1294,7 @@ __global__ __launch_bounds__(64, MODE == 0 ? 1 : 3) void k_integrate(BatchDev b,
     const WorkItem it = b.items[k];
     const ReqParams *rq = b.req + it.req;
     const ReqNorm rn = b.rnorm[it.req];
-    const int limb = rq->limb;
+    const int limb = COS ? rq->limb : 0;
     const int i = (it.rc & 0xffff) * RAD_CHUNK + lane;
     const int zone = it.rc >> 16;
     RadRec r;
@@ -1250,20 +1312,61 @@ __global__ __launch_bounds__(64, MODE == 0 ? 1 : 3) void k_integrate(BatchDev b,
     const bool has_win = r.ielo <= r.iehi;
     const int kmin = wave_min_i(has_win ? r.K : INT_MAX);
     const int kmax = wave_max_i(has_win ? r.K : -1);
-    if (!limb && kmax >= 0 && kmax - kmin + 2 <= stage_rows) {
+    if (kmax >= 0) {
+      // the lanes' rows from node rows kb .. ke + 1 built in LDS, stage_rows
+      // at a time (lanes with K in [kb, ke] are served per round), for the
+      // transfer function and then (limb) the cosines; the FP64 values at the
+      // edge brackets are kept for the edge norms
       const int nq = 2 * ng;
       double *stage = reinterpret_cast<double *>(s_bins);
-      stage_node_rows(b.node + (size_t)it.req * b.node_stride + (size_t)kmin * nq, (kmax - kmin + 2) * nq, stage,
-                      lane);
-      __syncthreads();  // one wavefront: orders the staged rows before the lanes read them
+      const NodeSrc ts = node_src(b.rt, rn, false);
+      const NodeSrc cs = node_src(b.rt, rn, true);
+      EdgePt pL{}, pU{};
       if (has_win) {
-        const double *tlo = stage + (size_t)(r.K + 1 - kmin) * nq;
-        const double *thi = stage + (size_t)(r.K - kmin) * nq;
-        load_radius_from<TV>(r, nq, 0, tlo, thi, nullptr, nullptr, myT, myC, v);
+        v.gmin = r.gmin;
+        v.gmax = r.gmax;
+        v.del_g = 1. / (r.gmax - r.gmin);  // set_str_relbf, src/Relprofile.cpp:733
+        v.limb = limb;
+        pL = edge_point(false, v);
+        pU = edge_point(true, v);
       }
-      __syncthreads();  // the bins overwrite the staging area
-    } else if (has_win) {
-      load_radius<TV>(b, it.req, r, limb, myT, myC, v);
+      for (int kb = kmin; kb <= kmax; kb += stage_rows - 1) {
+        double tL[4] = {0, 0, 0, 0}, tU[4] = {0, 0, 0, 0}, cL[4] = {0, 0, 0, 0}, cU[4] = {0, 0, 0, 0};
+        const int ke = min(kmax, kb + stage_rows - 2);
+        const bool mine = has_win && r.K >= kb && r.K <= ke;
+        const int tot = (ke - kb + 2) * nq;
+        const int klo = mine ? r.K + 1 - kb : 1, khi = mine ? r.K - kb : 0;
+        const double *lo = stage + (size_t)klo * nq;
+        const double *hi = stage + (size_t)khi * nq;
+        stage_table_rows<5>(ts, kb, tot, nq, stage, lane);
+        __syncthreads();  // one wavefront: orders the staged rows before the lanes read them
+        if (mine) {
+          lerp_row<TV>(r.ifac_r, lo, hi, nq, myT);
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            tL[k] = lerp1_(r.ifac_r, lo[pL.ind * 2 + k], hi[pL.ind * 2 + k]);
+            tU[k] = lerp1_(r.ifac_r, lo[pU.ind * 2 + k], hi[pU.ind * 2 + k]);
+          }
+        }
+        __syncthreads();  // the next rows (or the bins) overwrite the staging area
+        if (COS && limb) {
+          stage_table_rows<5>(cs, kb, tot, nq, stage, lane);
+          __syncthreads();
+          if (mine) {
+            lerp_row<TV>(r.ifac_r, lo, hi, nq, myC);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+              cL[k] = lerp1_(r.ifac_r, lo[pL.ind * 2 + k], hi[pL.ind * 2 + k]);
+              cU[k] = lerp1_(r.ifac_r, lo[pU.ind * 2 + k], hi[pU.ind * 2 + k]);
+            }
+          }
+          __syncthreads();
+        }
+        if (mine) {
+          v.normL = edge_norm_at(pL, v, tL, cL);
+          v.normU = edge_norm_at(pU, v, tU, cU);
+        }
+      }
     }
     const double w64 = has_win ? r.coef * rn.fac : 0.0;   // MODE 0 weight
     const double w32 = has_win ? r.coef * rn.cmul : 0.0;  // MODE 1 scaled weight
@@ -1356,8 +1459,8 @@ __global__ __launch_bounds__(64) void k_fixup(BatchDev b, int cos_rows) {
     const FixEntry fe = b.fix[e];
     const ReqParams *rq = b.req + fe.req;
     const RadRec r = b.rad[(size_t)fe.req * b.nr_fine + fe.rad];
-    load_radius<double>(b, fe.req, r, rq->limb, myT, myC, v);
     const double *E = profile_grid(b, rq);
+    load_radius_table(b, b.rnorm[fe.req], r, rq->limb, E[fe.bin], E[fe.bin + 1], myT, myC, v);
     const double val = integ_bin<0>(E[fe.bin], E[fe.bin + 1], v) * (r.coef * b.rnorm[fe.req].fac);
     unsafeAtomicAdd(&profile_acc(b, *rq)[(size_t)r.pad * rq->n_e + fe.bin], val - fe.v32);
   }
@@ -2240,9 +2343,9 @@ static size_t row_lds(const BatchDev &b, bool cos_rows, size_t elem) {
 
 // LDS of one k_integrate wavefront: lane rows, then (FP64 path) 64 weight
 // slots, then the SUM_BINS x SUM_STRIDE parked values, which double as the
-// node-row staging area.  *stage_rows: node rows the staging area takes (0:
-// none), as many as fit without lowering the wavefronts per CU that the LDS
-// allows (at most 12, the VGPR limit of the FP32 path), up to 8.
+// node-row staging area.  *stage_rows: node rows the staging area takes, as
+// many as fit without lowering the wavefronts per CU that the LDS allows (at
+// most 12, the VGPR limit of the FP32 path), up to 8 and at least 2.
 static size_t integrate_lds(const BatchDev &b, bool cos_rows, size_t elem, int *stage_rows) {
   const size_t rows = integrate_rows_bytes(b.rt.n_g, cos_rows, elem);
   const size_t w = (elem == sizeof(double)) ? 64 * sizeof(double) : 0;
@@ -2250,9 +2353,7 @@ static size_t integrate_lds(const BatchDev &b, bool cos_rows, size_t elem, int *
   const size_t nq = 2 * (size_t)b.rt.n_g;
   auto per_cu = [](size_t lds) { return std::min<size_t>(12, 163840 / ((lds + 511) & ~(size_t)511)); };
   const size_t base = rows + w + park;
-  *stage_rows = 0;
-  if (cos_rows) return base;
-  for (int c = 8; c >= 2; c--) {
+  for (int c = 8; c > 2; c--) {
     if ((size_t)c * nq > (size_t)STAGE_MAX) continue;
     const size_t need = rows + w + std::max(park, (size_t)c * nq * sizeof(double));
     if (per_cu(need) >= per_cu(base)) {
@@ -2260,21 +2361,25 @@ static size_t integrate_lds(const BatchDev &b, bool cos_rows, size_t elem, int *
       return need;
     }
   }
-  return base;
+  *stage_rows = 2;  // the two rows a radius lerps between (2 nq <= STAGE_MAX: n_g <= MAX_NG)
+  return rows + w + std::max(park, (size_t)2 * nq * sizeof(double));
 }
 
 int launch_integrate(const BatchDev &b, int n_blocks, bool cos_rows, void *stream) {
   if (b.n_req <= 0 || n_blocks <= 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   int stage = 0;
+  const size_t lds = integrate_lds(b, cos_rows, b.mode == 0 ? sizeof(double) : sizeof(float), &stage);
+  auto go = [&](auto kern) {
+    (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(kern, dim3(n_blocks), dim3(64), lds, st, b, stage);
+  };
   if (b.mode == 0) {
-    const size_t lds = integrate_lds(b, cos_rows, sizeof(double), &stage);
-    (void)hipFuncSetAttribute((const void *)k_integrate<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_integrate<0>, dim3(n_blocks), dim3(64), lds, st, b, (int)cos_rows, stage);
+    if (cos_rows) go(k_integrate<0, 1>);
+    else go(k_integrate<0, 0>);
   } else {
-    const size_t lds = integrate_lds(b, cos_rows, sizeof(float), &stage);
-    (void)hipFuncSetAttribute((const void *)k_integrate<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_integrate<1>, dim3(n_blocks), dim3(64), lds, st, b, (int)cos_rows, stage);
+    if (cos_rows) go(k_integrate<1, 1>);
+    else go(k_integrate<1, 0>);
   }
   return (int)hipGetLastError();
 }

@@ -1,8 +1,8 @@
 // noalloc.hip -- the allocation-free evaluation entry point.
 //
 // rx_eval_model_noalloc evaluates one line or convolution request on device
-// data that the caller owns, with every work buffer of the kernels (node
-// rows, radius records, work list, FP64 re-check queue, counters, profile)
+// data that the caller owns, with every work buffer of the kernels (radius
+// records, work list, FP64 re-check queue, counters, profile)
 // carved from one fixed device array, `scratch`, instead of memory the
 // library allocates.  The array's device address is looked up once per
 // device when the library initialises the device; host code only ever uses
@@ -129,6 +129,7 @@ int noalloc_eval(int model, const double *parameter, const double *d_energy, con
   p.pgrid_off = conv ? -1 : 0;
   p.prof_off = conv ? 0 : -1;
   p.dist_off = 0;
+  p.node_off = -1;
   p.zone_off = 0;
   p.xspec_off = 0;
   if (st == RX_OK && p.return_rad != 0) {
@@ -170,7 +171,7 @@ int noalloc_eval(int model, const double *parameter, const double *d_energy, con
   bd.flux_in = conv ? d_flux_in : nullptr;
   bd.out = d_flux_out;
   bd.pgrid = conv ? nullptr : c.take<double>((size_t)n_flux + 1);
-  bd.node = c.take<double>((size_t)bd.node_stride);
+  bd.node = nullptr;  // line and convolution requests read no node rows
   bd.rad = c.take<RadRec>((size_t)g.nr_fine);
   bd.rnorm = c.take<ReqNorm>(1);
   bd.prof = conv ? c.take<double>((size_t)g.conv_n) : nullptr;

@@ -79,6 +79,9 @@ struct ReqParams {
   double xect_src;    // Ecut in the source frame
   int64_t dist_off;
   int64_t zone_off;   // ZoneRec per zone
+  // the request's (a, mu0)-interpolated node rows in node (n_incl > 0 only:
+  // the angle distribution reads them; -1 otherwise)
+  int64_t node_off;
   // returning radiation (lamp post): 1 add, -1 or 2 replace the direct
   // emissivity; the table spin, its first zone and the number of zones used
   int return_rad, rr_k, rr_klo, rr_n;
@@ -123,6 +126,11 @@ struct RadRec {
 // cinv = 2^-k undoes it exactly.
 struct ReqNorm {
   double fac, cmul, cinv;
+  // the (a, mu0) corners of the request's table interpolation (interpol_a_mu0,
+  // src/Relprofile.cpp:39-80): spin index ia, mu0 index im and their factors;
+  // k_integrate and k_fixup build the node rows they need from these
+  double fa, fm;
+  int ia, im;
 };
 
 struct LpRefl {

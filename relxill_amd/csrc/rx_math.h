@@ -155,34 +155,30 @@ RX_HD inline double edge_norm64(bool upper, const RadiusView &v) {
   return norm * sqrt(GFAC_H);
 }
 
-// Transfer-function value of one fine radius lerped from its two table-node
-// rows (src/Relprofile.cpp:280-293): lo = node K+1, hi = node K.
-RX_HD inline double node_lerp(double ifac, const double *lo, const double *hi, int q) {
-  return lerp1_(ifac, lo[q], hi[q]);
-}
-
-// edge_norm64 with the FP64 values taken from the node rows directly (the
-// same arithmetic as a lerped row, so the same bits).  c_lo/c_hi: cosines.
-RX_HD inline double edge_norm_node(bool upper, const RadiusView &v, double ifac, const double *t_lo,
-                                   const double *t_hi, const double *c_lo, const double *c_hi) {
+// The point of int_edge's norm (src/Relprofile.cpp:609-618): the energy at
+// g* = H (lower) or 1 - H (upper), its g* and its g* bracket.
+struct EdgePt {
+  double e, egs;
+  int ind;
+};
+RX_HD inline EdgePt edge_point(bool upper, const RadiusView &v) {
 #pragma clang fp contract(off)
   const double hex = upper ? 1.0 - GFAC_H : GFAC_H;
-  const double e = (hex * (v.gmax - v.gmin) + v.gmin) * 1.0;
-  const double egs = (e - v.gmin) * v.del_g;
-  const int ind = g_bracket(egs, v.ng, v.gs_step, v.inv_step);
-  double t4[4], c4[4], tl[4], th[4];
-  for (int k = 0; k < 4; k++) {  // the 8 loads first, so that they are in flight together
-    tl[k] = t_lo[ind * 2 + k];
-    th[k] = t_hi[ind * 2 + k];
-  }
-  for (int k = 0; k < 4; k++) {
-    t4[k] = lerp1_(ifac, tl[k], th[k]);
-    c4[k] = 0.0;
-  }
-  if (v.limb)
-    for (int k = 0; k < 4; k++) c4[k] = node_lerp(ifac, c_lo, c_hi, ind * 2 + k);
+  EdgePt p;
+  p.e = (hex * (v.gmax - v.gmin) + v.gmin) * 1.0;
+  p.egs = (p.e - v.gmin) * v.del_g;
+  p.ind = g_bracket(p.egs, v.ng, v.gs_step, v.inv_step);
+  return p;
+}
+
+// edge_norm64 from the radius' FP64 values at the point's bracket: t4 =
+// {T[ind][0], T[ind][1], T[ind+1][0], T[ind+1][1]}, c4 the same for the
+// cosines (read when limb != 0).  The values are those of a lerped FP64 row,
+// so the norm has the same bits as edge_norm64's.
+RX_HD inline double edge_norm_at(const EdgePt &p, const RadiusView &v, const double *t4, const double *c4) {
+#pragma clang fp contract(off)
   double f0, f1;
-  eval64_at(e, egs, ind, v, t4, c4, f0, f1);
+  eval64_at(p.e, p.egs, p.ind, v, t4, c4, f0, f1);
   double norm = 0.0;
   norm = norm + f0;
   norm = norm + f1;
@@ -364,7 +360,7 @@ RX_HD inline void eval32_point(float s, const BinF &bf, const RadiusView &v, int
   const float thr = 1e-6f * (inv_step + 1.0f);
   ind = min(max((int)floorf(q), 0), v.ng - 2);
   inte = q - (float)ind;
-  if (fabsf(q - qi) < thr && qi >= 1.0f && qi <= (float)(v.ng - 2)) {
+  if (fabsf(q - qi) < thr) {
     // close to a g* node: decide the bracket with the FP64 coordinates
     const double eg = bf.rlo + (bf.rhi - bf.rlo) * (double)s;
     const double egs = (eg - v.gmin) * v.del_g;
@@ -473,6 +469,7 @@ RX_HD inline float romberg32_pair(const BinF &bf, const RadiusView &v, bool *nee
       add1 = (add1 + g1[0]) + g1[1];
     } else {
       // (2m+1) ds, exact: dyadic with at most 7 bits; 4 points per batch
+#pragma unroll 1
       for (int m0 = 0; m0 < nnew; m0 += 4) {
         float sk[4], g0[4], g1[4];
 #pragma unroll
