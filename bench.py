@@ -566,20 +566,23 @@ def main():
     # committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated on a copy)
     # the kernel is bound by VALU issue, not HBM: the same summary's VALU-busy
     # fraction (SQ_ACTIVE_INST_VALU over GPU-busy cycles) is reported beside it
-    traffic, traffic_src, valu_busy = None, None, None
+    # (the summary is taken at one work shape, named beside the figure: the
+    # traffic of the list's other shapes differs by the re-staged node rows)
+    traffic, traffic_src, valu_busy, traffic_shape = None, None, None, None
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic_relconv_batch4096.json")
     if os.path.exists(tpath):
         tj = json.load(open(tpath))
-        ki = tj.get("kernels", {}).get("k_integrate<1>")
-        if ki and tj.get("shape") == f"{seg},{wpc}" and len(reqs) == 4096:
+        ki = tj.get("kernels", {}).get("k_integrate<1, 0>")
+        if ki and len(reqs) == 4096:
             traffic = round(ki["read_bytes"] + ki["write_bytes"])
             traffic_src = tj.get("source")
+            traffic_shape = tj.get("shape")
             valu_busy = ki.get("valu_busy")
     roofline = dict(bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=achieved / HBM_PEAK_GBS, traffic=traffic, traffic_source=traffic_src,
-                    valu_busy=valu_busy,
+                    traffic_shape=traffic_shape, valu_busy=valu_busy,
                     algorithmic_bytes=round(alg_bytes),
-                    kernel="k_integrate<1>", kernel_ms=round(kms32[1], 4),
+                    kernel="k_integrate<1, 0>", kernel_ms=round(kms32[1], 4),
                     kernel_ms_all={k: round(v, 4) for k, v in zip(KERNELS, kms32)})
 
     cpu = None
@@ -622,7 +625,7 @@ def main():
                         kernel=roofline["kernel"], kernel_ms=roofline["kernel_ms"],
                         valu=roofline_valu, hbm={k: roofline[k] for k in ("achieved", "peak", "unit", "frac",
                                                                           "traffic", "traffic_source",
-                                                                          "algorithmic_bytes")},
+                                                                          "traffic_shape", "algorithmic_bytes")},
                         valu_busy=roofline["valu_busy"], kernel_ms_all=roofline["kernel_ms_all"])
         log(f"roofline: {roofline}")
 

@@ -1123,8 +1123,11 @@ __device__ inline NodeSrc node_src(const DevRelTable &T, const ReqNorm &rn, bool
   return s;
 }
 
+// (the branch arrays are taken into locals before the select: a select
+// between two members' addresses would put the struct in scratch memory)
 __device__ inline double node_value(const NodeSrc &s, int ir, int c) {
-  const float *arr = (c & 1) ? s.a2 : s.a1;
+  const float *a1 = s.a1, *a2 = s.a2;
+  const float *arr = (c & 1) ? a2 : a1;
   const size_t q = (size_t)ir * s.ng + (c >> 1);
   return lerp2f(s.fa, s.fm, arr[s.s00 + q], arr[s.s10 + q], arr[s.s01 + q], arr[s.s11 + q]);
 }
@@ -1137,12 +1140,13 @@ constexpr int STAGE_MAX = 640;
 template <int PER>
 __device__ inline void stage_table_rows(const NodeSrc &s, int kb, int tot, int nq, double *dst, int lane) {
   int row = kb + lane / nq, col = lane % nq;
+  const float *a1 = s.a1, *a2 = s.a2;
   for (int k0 = 0; k0 < STAGE_MAX / 64 && 64 * k0 < tot; k0 += PER) {
     float c00[PER], c10[PER], c01[PER], c11[PER];
 #pragma unroll
     for (int k = 0; k < PER; k++) {
       if (lane + 64 * (k0 + k) < tot) {
-        const float *arr = (col & 1) ? s.a2 : s.a1;
+        const float *arr = (col & 1) ? a2 : a1;
         const size_t q = (size_t)row * s.ng + (col >> 1);
         c00[k] = arr[s.s00 + q];
         c10[k] = arr[s.s10 + q];
