@@ -315,11 +315,14 @@ def serving_pool(n=2048, seed=6):
     return pool
 
 
-def serve(rx, pool, qps, seconds=None, n_total=None, seed=6):
+def serve(rx, pool, qps, seconds=None, n_total=None, seed=6, warmup_s=0.25):
     """Poisson arrivals at qps (seconds long, or n_total requests) drawn from
     the pre-generated pool, replayed by the server's native pacing loop (so
     the client does not limit the offered load); p50 / p99 of submit ->
-    completion and the completed spectra per second of wall time."""
+    completion and the completed spectra per second of wall time.  The
+    measured stream follows a warm-up stream of warmup_s seconds at the same
+    rate on the same server (its slots' first buffer allocations), which is
+    not counted."""
     rng = np.random.default_rng(seed)
     n = n_total if n_total else int(qps * seconds * 1.2) + 16
     t = np.cumsum(rng.exponential(1.0 / qps, size=n))
@@ -327,6 +330,12 @@ def serve(rx, pool, qps, seconds=None, n_total=None, seed=6):
         t = t[t <= seconds]
     reqs = [pool[i % len(pool)] for i in range(t.size)]
     srv = rx.Server(max_inflight=8192, hbm_budget=32 << 30, n_streams=4, precision=rx.FP32)
+    if warmup_s > 0:
+        tw = np.cumsum(rng.exponential(1.0 / qps, size=int(qps * warmup_s * 1.2) + 16))
+        tw = tw[tw <= warmup_s]
+        srv.replay([pool[(i * 7) % len(pool)] for i in range(tw.size)], tw)
+        srv.drain()
+    st0 = srv.stats()
     t0 = time.perf_counter()
     tickets = srv.replay(reqs, t)
     srv.drain()
@@ -338,10 +347,11 @@ def serve(rx, pool, qps, seconds=None, n_total=None, seed=6):
         bad += r[1] != 0
     st = srv.stats()
     srv.stop()
+    nb = st["batches"] - st0["batches"]
     return dict(qps_offered=qps, submitted=len(tickets), failed=int(bad),
                 p50_ms=float(np.percentile(lats, 50)), p99_ms=float(np.percentile(lats, 99)),
-                spectra_per_s=len(tickets) / wall, mean_batch=len(tickets) / max(1, st["batches"]),
-                batches=st["batches"], max_concurrent_batches=st["max_concurrent_batches"])
+                spectra_per_s=len(tickets) / wall, mean_batch=len(tickets) / max(1, nb),
+                batches=nb, max_concurrent_batches=st["max_concurrent_batches"], warmup_s=warmup_s)
 
 
 def host_cpus():

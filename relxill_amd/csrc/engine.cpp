@@ -41,6 +41,10 @@ DevBuf::~DevBuf() {
 int DevBuf::ensure(size_t n) {
   if (n == 0) n = 8;
   if (p && bytes >= n) return 0;
+  // a buffer that has to grow grows by half again at least: hipFree waits for
+  // the whole device, so a reused buffer (a server slot's) should rarely grow
+  if (p) n = std::max(n, bytes + bytes / 2);
+  n = (n + 255) & ~(size_t)255;
   if (p) (void)hipFree(p);
   p = nullptr;
   bytes = 0;
@@ -1096,7 +1100,7 @@ int batch_run(Batch &b, int precision, std::string &err) {
   k.cos_rows = b.any_limb ? 1 : 0;
   hipStream_t s = b.stream;
   b.profiled = k.prof != 0;
-  if (g.use_graphs && !b.graph_failed) {
+  if (g.use_graphs && !b.graph_failed && !b.one_shot) {
     if (b.gexec && memcmp(&k, &b.graph_key, sizeof(k)) == 0) {
       if (hipGraphLaunch(b.gexec, s) == hipSuccess) {
         b.graph_replays++;

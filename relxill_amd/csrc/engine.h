@@ -80,6 +80,9 @@ struct Batch {
   // graph of the launch sequence (captured on the second identical run)
   RunKey last_key, graph_key;
   bool last_valid = false, graph_failed = false;
+  // one-shot batches (the server's slots: every run carries new requests)
+  // never capture a launch graph, whose capture would cost more than it saves
+  bool one_shot = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int64_t graph_replays = 0;

@@ -66,6 +66,7 @@ struct PinnedBuf {
   }
   bool ensure(size_t n) {
     if (p && bytes >= n) return true;
+    if (p) n = std::max(n, bytes + bytes / 2);  // grows rarely (hipHostFree synchronises)
     if (p) (void)hipHostFree(p);
     p = nullptr;
     bytes = 0;
@@ -150,6 +151,7 @@ int rx_server::launch(Slot &s, std::vector<Pending> &take, std::string &err) {
   // the autotuned work shape of the batch's shape class (else the default)
   s.batch.seg_forced = 0;
   s.batch.waves_forced = 0;
+  s.batch.one_shot = true;
   st = batch_build(s.batch, rq.data(), (int)rq.size(), err);
   if (st != RX_OK) return st;
   if (!s.h_out.ensure(sizeof(double) * std::max<int64_t>(1, s.batch.tot_flux)) ||
