@@ -1369,11 +1369,14 @@ __global__ __launch_bounds__(64, MODE == 0 ? 1 : 3) void k_integrate(BatchDev b,
         if (mine) {
           v.normL = edge_norm_at(pL, v, tL, cL);
           v.normU = edge_norm_at(pU, v, tU, cU);
+          v.eL = (float)(2 * v.normL * (v.gmax - v.gmin));
+          v.eU = (float)(2 * v.normU * (v.gmax - v.gmin));
         }
       }
     }
     const double w64 = has_win ? r.coef * rn.fac : 0.0;   // MODE 0 weight
     const double w32 = has_win ? r.coef * rn.cmul : 0.0;  // MODE 1 scaled weight
+    const float w32f = (float)w32;
     if constexpr (MODE == 0) s_coef[lane] = w64;
     const double *E = profile_grid(b, rq);
     double *acc = profile_acc(b, *rq) + (size_t)zone * rq->n_e;
@@ -1396,7 +1399,7 @@ __global__ __launch_bounds__(64, MODE == 0 ? 1 : 3) void k_integrate(BatchDev b,
             val = integ_bin_g<0>(elo, ehi, glo, ghi, v, nullptr, &cy);
           } else {
             bool need64 = false;
-            val = (float)(integ_bin_g<1>(elo, ehi, glo, ghi, v, &need64, &cy) * w32);
+            val = integ_bin32(elo, ehi, glo, ghi, v, &need64, &cy) * w32f;
             if (need64) {
               const int slot = atomicAdd(&b.ctr[CTR_FIX], 1);
               if (slot < b.fix_cap) {

@@ -27,7 +27,8 @@
 // g* bracket always equals the FP64 one.  The Romberg tables run in FP32; a stop
 // test whose FP32 estimate falls within a guard band of the 2% threshold is
 // flagged and re-done in FP64 by the caller, so both modes stop at the level
-// the FP64 algorithm stops.  Edge integrals are FP64 in both modes.
+// the FP64 algorithm stops.  The edge integrals' bounds stay FP64 in both
+// modes (FP32 mode evaluates them in the cancellation-free quotient form).
 #pragma once
 
 #if defined(__HIPCC__)
@@ -65,6 +66,7 @@ struct RadiusView {
   double gs_step;   // (1-2H)/(ng-1)
   double inv_step;  // 1/gs_step
   float guard;      // FP32 mode: relative guard band around the Romberg threshold
+  float eL, eU;     // FP32 mode: 2 norm (gmax - gmin) of the lower / upper edge integral
 };
 
 RX_HD inline double lerp1_(double ifac, double lo, double hi) {
@@ -219,25 +221,6 @@ RX_HD inline float rx_rcpf(float x) {
 #endif
 }
 
-// int_edge for the FP32 path: the bounds and their difference stay FP64 (no
-// cancellation: sqrt(hi) - sqrt(lo) = (hi - lo) / (sqrt(hi) + sqrt(lo))), the
-// square roots and the quotient are FP32.
-RX_HD inline double int_edge_pre32(double blo, double bhi, const RadiusView &v) {
-  double lo, hi, norm;
-  if (blo <= 0.5) {
-    lo = blo;
-    hi = bhi;
-    norm = v.normL;
-  } else {
-    lo = 1.0 - bhi;
-    hi = 1.0 - blo;
-    norm = v.normU;
-  }
-  const float d = (float)(hi - lo);
-  const float sq = sqrtf((float)hi) + sqrtf((float)lo);
-  const float q = (sq > 0.0f) ? d * rx_rcpf(sq) : 0.0f;
-  return 2 * norm * (double)q * (v.gmax - v.gmin);
-}
 
 // Romberg for both branches on [a,b], FP64.  Returns t_0[niter0][0] + t_1[niter1][0].
 // R[ii] holds the current anti-diagonal t[ii][n-ii] of the reference's table
@@ -340,9 +323,10 @@ RX_HD inline BinF make_binf(double rlo, double rhi, double ga, double gb) {
 
 // Bracket, interpolation factor and E^3 / sqrt(g* (1 - g*)) at s (exact dyadic
 // in [0,1]); coordinates from the nearer end, so s = 1 gives exactly the
-// values the next bin sees at its s = 0.
-RX_HD inline void eval32_point(float s, const BinF &bf, const RadiusView &v, int &ind, float &inte,
-                               float &base) {
+// values the next bin sees at its s = 0.  *near: the FP32 g* lies within a
+// few rounding errors of a g* node, where eval32_fix decides the bracket.
+RX_HD inline void eval32_point(float s, const BinF &bf, const RadiusView &v, int &ind, float &inte, float &base,
+                               bool &near) {
   float t, u, e;
   if (s <= 0.5f) {
     t = fmaf(s, bf.dg, bf.ga);
@@ -356,32 +340,43 @@ RX_HD inline void eval32_point(float s, const BinF &bf, const RadiusView &v, int
   }
   const float inv_step = (float)v.inv_step;
   const float q = (t - (float)GFAC_H) * inv_step;
-  const float qi = rintf(q);
   const float thr = 1e-6f * (inv_step + 1.0f);
   ind = min(max((int)floorf(q), 0), v.ng - 2);
   inte = q - (float)ind;
-  if (fabsf(q - qi) < thr) {
-    // close to a g* node: decide the bracket with the FP64 coordinates
-    const double eg = bf.rlo + (bf.rhi - bf.rlo) * (double)s;
-    const double egs = (eg - v.gmin) * v.del_g;
-    ind = g_bracket(egs, v.ng, v.gs_step, v.inv_step);
-    inte = (float)((egs - gstar_node(v.gs_step, ind)) * v.inv_step);
-    t = (float)egs;
-    u = (float)(1.0 - egs);
-    e = (float)eg;
-  }
+  near = fabsf(q - rintf(q)) < thr;
   base = e * e * e * rx_rsqrtf(t * u);
 }
 
-// Both branches at K points: all brackets first, then the row reads and the
-// interpolation in one straight run (the LDS reads of the K points overlap).
+// A point close to a g* node: bracket, factor and weight from the FP64
+// coordinates, so that the bracket equals the FP64 path's.
+RX_HD inline void eval32_fix(float s, const BinF &bf, const RadiusView &v, int &ind, float &inte, float &base) {
+  const double eg = bf.rlo + (bf.rhi - bf.rlo) * (double)s;
+  const double egs = (eg - v.gmin) * v.del_g;
+  ind = g_bracket(egs, v.ng, v.gs_step, v.inv_step);
+  inte = (float)((egs - gstar_node(v.gs_step, ind)) * v.inv_step);
+  const float t = (float)egs, u = (float)(1.0 - egs), e = (float)eg;
+  base = e * e * e * rx_rsqrtf(t * u);
+}
+
+// Both branches at K points: all brackets first (one branch for the rare
+// points next to a g* node), then the row reads and the interpolation in one
+// straight run (the LDS reads of the K points overlap).
 template <int K>
 RX_HD inline void eval32_batch(const float (&s)[K], const BinF &bf, const RadiusView &v, float (&f0)[K],
                                float (&f1)[K]) {
   int ind[K];
   float inte[K], base[K];
+  bool near[K], any = false;
 #pragma unroll
-  for (int k = 0; k < K; k++) eval32_point(s[k], bf, v, ind[k], inte[k], base[k]);
+  for (int k = 0; k < K; k++) {
+    eval32_point(s[k], bf, v, ind[k], inte[k], base[k], near[k]);
+    any = any || near[k];
+  }
+  if (any) {
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      if (near[k]) eval32_fix(s[k], bf, v, ind[k], inte[k], base[k]);
+  }
 #pragma unroll
   for (int k = 0; k < K; k++) {
     const float a0 = v.T32[ind[k] * 2 + 0], a1 = v.T32[ind[k] * 2 + 1];
@@ -412,13 +407,14 @@ RX_HD inline void eval32s(float s, const BinF &bf, const RadiusView &v, float &f
 // Stop test of the reference, ob = |R - top| / R > prec (continue), written
 // without the division: ob > prec  <=>  (R > 0 and |d| > prec R) or (R == 0
 // and d != 0, ob = inf); R < 0 or NaN gives ob <= 0 or NaN (stop).  A test
-// inside the guard band lo <= ob <= hi (R > 0) sets *need64.
-RX_HD inline bool romb_continue32(float R, float top, float lo, float hi, int n, bool *need64) {
+// inside the guard band |ob - prec| <= prec guard (R > 0) sets *need64.
+RX_HD inline bool romb_continue32(float R, float top, float band, int n, bool *need64) {
   const float d = fabsf(R - top);
   if (n > ROMB_ITERMAX) return false;
   if (R > 0.0f) {
-    if (d >= lo * R && d <= hi * R) *need64 = true;
-    return d > (float)ROMB_PREC * R;
+    const float x = fmaf(-(float)ROMB_PREC, R, d);  // d - prec R, one rounding
+    if (fabsf(x) <= band * R) *need64 = true;
+    return x > 0.0f;
   }
   return R == 0.0f && d > 0.0f;
 }
@@ -449,8 +445,7 @@ RX_HD inline float romberg32_pair(const BinF &bf, const RadiusView &v, bool *nee
   R1[0] = ta1 * pas;
   float res0 = R0[0], res1 = R1[0];
   bool run0 = true, run1 = true;
-  const float lo = (float)ROMB_PREC * (1.0f - v.guard);
-  const float hi = (float)ROMB_PREC * (1.0f + v.guard);
+  const float band = (float)ROMB_PREC * v.guard;  // the guard band |ob - prec| <= prec guard
 #pragma unroll
   for (int n = 1; n <= ROMB_ITERMAX + 1; n++) {
     if (!(run0 || run1)) break;
@@ -505,11 +500,11 @@ RX_HD inline float romberg32_pair(const BinF &bf, const RadiusView &v, bool *nee
     }
     if (run0) {
       res0 = R0[n];
-      run0 = romb_continue32(R0[n], top0, lo, hi, n, need64);
+      run0 = romb_continue32(R0[n], top0, band, n, need64);
     }
     if (run1) {
       res1 = R1[n];
-      run1 = romb_continue32(R1[n], top1, lo, hi, n, need64);
+      run1 = romb_continue32(R1[n], top1, band, n, need64);
     }
   }
   return res0 + res1;
@@ -519,14 +514,16 @@ RX_HD inline float romberg32_pair(const BinF &bf, const RadiusView &v, bool *nee
 // in reduced units (multiply by emis/(gmax-gmin) for relb_func units).
 // glo_u, ghi_u: the ends' unclamped g* = (E - gmin) del_g (a caller walking
 // the bins of one radius carries the shared edge from bin to bin).
-// MODE 0: FP64; MODE 1: FP32 interior with FP64 decisions and edges.
+// MODE 0 (FP64) only; the FP32 path is integ_bin32.
 // cy (optional): the integrand pair at rlo0 when cy->ok; on return cy holds the
 // pair at rhi0 when this bin evaluated it there (a Romberg bin whose upper end
 // is the bin edge), else cy->ok is false.
-template <int MODE, class TV = typename std::conditional<MODE == 0, double, float>::type>
+template <int MODE, class TV = double>
 RX_HD inline double integ_bin_g(double rlo0, double rhi0, double glo_u, double ghi_u, const RadiusView &v,
                                 bool *need64 = nullptr, EdgePair<TV> *cy = nullptr) {
 #pragma clang fp contract(off)
+  static_assert(MODE == 0, "the FP32 bin integral is integ_bin32");
+  (void)need64;
   const bool have_a = cy && cy->ok;
   if (cy) cy->ok = false;
   // clamp to [0, 1] (the ends are finite)
@@ -545,7 +542,7 @@ RX_HD inline double integ_bin_g(double rlo0, double rhi0, double glo_u, double g
       hhi = gbhi;
       rlo = -1.0;
     }
-    flu = flu + ((MODE == 0) ? int_edge_pre(hlo, hhi, v) : int_edge_pre32(hlo, hhi, v));
+    flu = flu + int_edge_pre(hlo, hhi, v);
   }
   if (gbhi >= (1.0 - GFAC_H)) {
     double hhi = gbhi, hlo = 1.0 - GFAC_H;
@@ -555,36 +552,102 @@ RX_HD inline double integ_bin_g(double rlo0, double rhi0, double glo_u, double g
       hlo = gblo;
       rhi = -1.0;
     }
-    flu = flu + ((MODE == 0) ? int_edge_pre(hlo, hhi, v) : int_edge_pre32(hlo, hhi, v));
+    flu = flu + int_edge_pre(hlo, hhi, v);
   }
   if ((rhi >= 0) && (rlo >= 0)) {
     if (rlo >= 1.0 * RED_WING) {
       const bool use_a = have_a && !lo_edge;  // the lower end is the shared bin edge
       TV fa0 = use_a ? cy->f0 : (TV)0, fa1 = use_a ? cy->f1 : (TV)0, fb0, fb1;
-      if constexpr (MODE == 0) {
-        flu = flu + romberg64_pair(rlo, rhi, v, use_a, fa0, fa1, fb0, fb1);
-      } else {
-        // a stop decision inside the guard band is re-done in FP64 by the caller
-        const BinF bf = make_binf(rlo, rhi, ga, gb);
-        flu = flu + (double)romberg32_pair(bf, v, need64, use_a, fa0, fa1, fb0, fb1);
-      }
+      flu = flu + romberg64_pair(rlo, rhi, v, use_a, fa0, fa1, fb0, fb1);
       if (cy && !hi_edge) {
         cy->f0 = fb0;
         cy->f1 = fb1;
         cy->ok = true;
       }
     } else {
-      if (MODE == 0) {
-        const double mid = (rhi + rlo) / 2.0;
-        double f0, f1;
-        eval64_pair(mid, v, f0, f1);
-        flu = flu + (f0 * (rhi - rlo) + f1 * (rhi - rlo));
-      } else {
-        const BinF bf = make_binf(rlo, rhi, ga, gb);
-        float f0, f1;
-        eval32s(0.5f, bf, v, f0, f1);
-        flu = flu + (double)((f0 + f1) * bf.de);
+      const double mid = (rhi + rlo) / 2.0;
+      double f0, f1;
+      eval64_pair(mid, v, f0, f1);
+      flu = flu + (f0 * (rhi - rlo) + f1 * (rhi - rlo));
+    }
+  }
+  return flu;
+}
+
+// int_edge for the FP32 path: the bounds and their difference stay FP64 (no
+// cancellation: sqrt(hi) - sqrt(lo) = (hi - lo) / (sqrt(hi) + sqrt(lo))), the
+// square roots, the quotient and the per-radius factor v.eL / v.eU are FP32.
+RX_HD inline float int_edge32(double blo, double bhi, const RadiusView &v) {
+  double lo, hi;
+  float k;
+  if (blo <= 0.5) {
+    lo = blo;
+    hi = bhi;
+    k = v.eL;
+  } else {
+    lo = 1.0 - bhi;
+    hi = 1.0 - blo;
+    k = v.eU;
+  }
+  const float d = (float)(hi - lo);
+#if defined(__HIP_DEVICE_COMPILE__)
+  const float sq = __builtin_amdgcn_sqrtf((float)hi) + __builtin_amdgcn_sqrtf((float)lo);
+#else
+  const float sq = sqrtf((float)hi) + sqrtf((float)lo);
+#endif
+  return (sq > 0.0f) ? k * (d * rx_rcpf(sq)) : 0.0f;
+}
+
+// The FP32 path of integ_bin_g (MODE 1) with the value kept in FP32: every
+// decision (clamps, edge tests, red-wing switch) is the same FP64 test, the
+// edge parts come from int_edge32 and the sum of the parts is FP32.
+RX_HD inline float integ_bin32(double rlo0, double rhi0, double glo_u, double ghi_u, const RadiusView &v,
+                               bool *need64, EdgePair<float> *cy) {
+#pragma clang fp contract(off)
+  const bool have_a = cy->ok;
+  cy->ok = false;
+  const double gblo = fmin(fmax(glo_u, 0.0), 1.0);
+  const double gbhi = fmin(fmax(ghi_u, 0.0), 1.0);
+  if (gbhi == 0) return 0.0f;
+  double rlo = rlo0, rhi = rhi0;
+  double ga = glo_u, gb = ghi_u;
+  float flu = 0.0f;
+  const bool lo_edge = (gblo <= GFAC_H), hi_edge = (gbhi >= (1.0 - GFAC_H));
+  if (lo_edge) {
+    double hlo = gblo, hhi = GFAC_H;
+    rlo = (GFAC_H * (v.gmax - v.gmin) + v.gmin) * 1.0;
+    ga = (rlo - v.gmin) * v.del_g;
+    if (gbhi <= GFAC_H) {
+      hhi = gbhi;
+      rlo = -1.0;
+    }
+    flu += int_edge32(hlo, hhi, v);
+  }
+  if (hi_edge) {
+    double hhi = gbhi, hlo = 1.0 - GFAC_H;
+    rhi = ((1 - GFAC_H) * (v.gmax - v.gmin) + v.gmin) * 1.0;
+    gb = (rhi - v.gmin) * v.del_g;
+    if (gblo >= (1.0 - GFAC_H)) {
+      hlo = gblo;
+      rhi = -1.0;
+    }
+    flu += int_edge32(hlo, hhi, v);
+  }
+  if ((rhi >= 0) && (rlo >= 0)) {
+    const BinF bf = make_binf(rlo, rhi, ga, gb);
+    if (rlo >= 1.0 * RED_WING) {
+      const bool use_a = have_a && !lo_edge;  // the lower end is the shared bin edge
+      float fb0, fb1;
+      flu += romberg32_pair(bf, v, need64, use_a, use_a ? cy->f0 : 0.0f, use_a ? cy->f1 : 0.0f, fb0, fb1);
+      if (!hi_edge) {
+        cy->f0 = fb0;
+        cy->f1 = fb1;
+        cy->ok = true;
       }
+    } else {
+      float f0, f1;
+      eval32s(0.5f, bf, v, f0, f1);
+      flu += (f0 + f1) * bf.de;
     }
   }
   return flu;
