@@ -229,7 +229,7 @@ int rx_tables_view(rx_reltable_view *rel, rx_lptable_view *lp);
 int rx_set_conv_grid(int n, double emin, double emax);
 int rx_set_nr_fine(int nr_fine);
 /* FP32 path: relative guard band around the Romberg stop threshold (default
- * 2e-3); stop tests inside it are redone in FP64.  Widening it exercises the
+ * 5e-4); stop tests inside it are redone in FP64.  Widening it exercises the
  * FP64 re-check path (diagnostics). */
 int rx_set_fp32_guard(double rel);
 

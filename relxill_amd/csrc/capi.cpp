@@ -656,7 +656,7 @@ int rx_batch_autotune(rx_batch *b, int precision) {
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   std::string err;
   const int segs[] = {32, 64, 128, 256};
-  const int wpcs[] = {4, 8, 16};
+  const int wpcs[] = {4, 8, 12, 16};
   double best = 1e30;
   int best_s = 64, best_w = 8;
   for (int w : wpcs) {

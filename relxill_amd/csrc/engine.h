@@ -8,6 +8,7 @@
 
 #include "../../include/relxill_gpu.h"
 #include "rx_engine.h"
+#include "rx_math.h"
 #include "rx_tables.h"
 
 namespace rx {
@@ -146,7 +147,7 @@ struct Global {
   // fine-grid size, clearing reflection sets); a batch refuses to run on a
   // newer configuration than it was built for
   uint64_t config_gen = 1;
-  float guard = 2e-3f;
+  float guard = ROMB_GUARD;
   // autotune cache: shape class -> (segment bins, waves per CU)
   struct Tuned {
     int shape, seg_bins, waves_per_cu;

@@ -50,7 +50,10 @@ constexpr double GFAC_H = 5e-3;          // src/Relprofile.cpp:36
 constexpr double ROMB_PREC = 0.02;       // src/Relprofile.cpp:525
 constexpr int ROMB_ITERMAX = 5;          // src/Relprofile.cpp:528 (niter <= 6)
 constexpr double RED_WING = 0.95;        // src/Relprofile.cpp:636
-constexpr float ROMB_GUARD = 2e-3f;      // default relative guard band (FP32 mode)
+// default relative guard band of the FP32 stop tests: the FP32 estimate of
+// |R - top| / R carries a relative error of about 1e-6 (a few roundings of
+// R's magnitude), 5e-5 of the 2% threshold; the band is ten times that
+constexpr float ROMB_GUARD = 5e-4f;
 constexpr double PI = 3.14159265358979323846;
 
 // Per-radius view used by the bin integral.
