@@ -131,6 +131,10 @@ def _load():
         "orc_spec_cutoffpl": (None, [_D, _D, _I, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
         "orc_norm_wrt_xillver": (ctypes.c_double, [_D, _D, _I, ctypes.POINTER(_I)]),
         "orc_primary_norm_factor": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
+        "orc_mcd_shape": (ctypes.c_double, [ctypes.c_double]),
+        "orc_nthcomp_solve": (_I, [ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _D]),
+        "orc_nthcomp_photar": (None, [_D, _D, _I, _D, _I, ctypes.c_double, _D]),
+        "orc_spec_nthcomp": (None, [_D, _D, _I, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
         "orc_get_relxill_params": (_I, [_I, _D, _I, ctypes.POINTER(RelParam), ctypes.POINTER(XillParam)]),
         "orc_energy_shift_source_obs": (ctypes.c_double, [ctypes.POINTER(RelParam)]),
         "orc_relxill_zones": (_I, [_I, ctypes.POINTER(RelTable), ctypes.POINTER(LpTable), _D, _D]),
@@ -422,6 +426,30 @@ def cutoffpl(ener, ect, gam, shift=1.0):
 
 def primary_norm_factor(gam, ect):
     return lib.orc_primary_norm_factor(gam, ect)
+
+
+def nthcomp_solve(kte, gam, ktbb=0.05):
+    """the Kompaneets solution for a disc seed: (x [m_e c^2], E F_E)"""
+    x = np.zeros(901)
+    spt = np.zeros(900)
+    n = lib.orc_nthcomp_solve(ktbb / 511.0, kte / 511.0, gam, _dp(x), _dp(spt))
+    return x[:n + 1].copy(), spt[:n].copy()
+
+
+def nthcomp_photar(x, spt, ener, z=0.0):
+    e = _arr(ener)
+    xs, ss = _arr(x), _arr(spt)
+    out = np.zeros(e.size - 1)
+    lib.orc_nthcomp_photar(_dp(xs), _dp(ss), ss.size, _dp(e), e.size - 1, z, _dp(out))
+    return out
+
+
+def nthcomp(ener, kte, gam, shift=1.0):
+    """spec_nthcomp: photons per bin, E F_E(1 keV in the source) = 1"""
+    e = _arr(ener)
+    out = np.zeros(e.size - 1)
+    lib.orc_spec_nthcomp(_dp(out), _dp(e), e.size - 1, kte, gam, shift)
+    return out
 
 
 def relxill(tables, xt, params, energy, conv_grid=None, phys_norm=False):

@@ -259,6 +259,18 @@ double orc_norm_wrt_xillver(const double *flux, const double *ener, int n, int *
  * 500-bin grid (PrimarySource::calc_normalization_factor_source,
  * src/PrimarySource.h:279-292) */
 double orc_primary_norm_factor(double gam, double ect);
+
+/* ---- nthcomp primary continuum (nthcomp_oracle.c; src/donthcomp.c) ---- */
+/* diskbb photon shape per unit Tin^2 at E / Tin (f_mcdint__) */
+double orc_mcd_shape(double et);
+/* Kompaneets solve for a disc seed: x[0..jmax] (m_e c^2), spt[0..jmax-1]
+ * (E F_E); arrays of at least 901 / 900; returns jmax */
+int orc_nthcomp_solve(double tempbb, double theta, double gamma, double *x, double *spt);
+/* photons per bin of ear[0..ne] at redshift z, E F_E(1 keV, source) = 1 */
+void orc_nthcomp_photar(const double *x, const double *spt, int nth, const double *ear, int ne, double z,
+                        double *photar);
+/* spec_nthcomp (src/Xillspec.cpp:243-253) */
+void orc_spec_nthcomp(double *flux, const double *ener, int n, double kte, double gam, double ener_shift);
 /* relxill parameters: Index1 Index2 Rbr a Incl Rin Rout z gamma logxi Afe
  * Ecut refl_frac (src/modelfiles/lmodel_relxill_public.dat:46-59); relxilllp:
  * h beta a Incl Rin Rout z gamma logxi Afe Ecut refl_frac switch_returnrad
