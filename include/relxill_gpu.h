@@ -297,6 +297,17 @@ int rx_profile_enable(int on);
 int rx_copy_test(int64_t bytes, int reps, double *ms);
 /* Waits for all work of the library on its device. */
 int rx_device_synchronize(void);
+
+/* The nthcomp primary continuum (spec_nthcomp, src/Xillspec.cpp:243-253, with
+ * c_donthcomp, src/donthcomp.c:683-793): n_spec spectra on one grid of
+ * n_flux bins (energy: n_flux + 1 ascending positive edges, keV), spectrum s
+ * with photon index gam[s], electron temperature kte[s] (keV, in the frame of
+ * the source) and energy shift source -> observer ener_shift[s] (redshift
+ * 1 / ener_shift - 1), seed a 0.05 keV disc blackbody.  flux receives n_spec
+ * rows of n_flux photons per bin, normalised to E F_E = 1 at 1 keV in the
+ * source frame.  One Kompaneets solve per spectrum on the device. */
+int rx_nthcomp(const double *energy, int n_flux, int n_spec, const double *kte, const double *gam,
+               const double *ener_shift, double *flux);
 int rx_batch_kernel_ms(rx_batch *b, double *ms);
 /* Work counters of the last completed rx_batch_run (after rx_batch_sync):
  * out[0] work items integrated, out[1] FP32 bins re-checked in FP64,

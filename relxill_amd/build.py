@@ -21,7 +21,7 @@ CFLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall",
           "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable",
           "-I", os.path.join(ROOT, "include"), "-I", CSRC] + NO_PK
 
-SOURCES = ["tables.cpp", "fits.cpp", "engine.cpp", "capi.cpp", "server.cpp", "kernels.hip", "noalloc.hip"]
+SOURCES = ["tables.cpp", "fits.cpp", "engine.cpp", "capi.cpp", "server.cpp", "kernels.hip", "noalloc.hip", "nthcomp.hip"]
 
 
 def _compile(src):

@@ -135,6 +135,7 @@ _EXPORTS = {
     "rx_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "rx_device_synchronize": (ctypes.c_int, []),
     "rx_copy_test": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _D]),
+    "rx_nthcomp": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D]),
     "rx_batch_kernel_ms": (ctypes.c_int, [ctypes.c_void_p, _D]),
     "rx_batch_counters": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "rx_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -218,6 +219,21 @@ def lmodrelline(energy, params):
 
 def lmodrellinelp(energy, params):
     return _lmod(lib.lmodrellinelp, energy, params)
+
+
+def nthcomp(energy, kte, gam, ener_shift=1.0):
+    """rx_nthcomp: nthcomp spectra (photons per bin) on one grid for arrays
+    (or scalars) of kTe, Gamma and energy shift; returns (n_spec, n_bins)."""
+    e = np.ascontiguousarray(energy, dtype=np.float64)
+    kte = np.atleast_1d(np.asarray(kte, dtype=np.float64))
+    n = kte.size
+    gam = np.ascontiguousarray(np.broadcast_to(np.asarray(gam, dtype=np.float64), (n,)))
+    sh = np.ascontiguousarray(np.broadcast_to(np.asarray(ener_shift, dtype=np.float64), (n,)))
+    kte = np.ascontiguousarray(kte)
+    out = np.zeros((n, e.size - 1))
+    _check(lib.rx_nthcomp(e.ctypes.data_as(_D), e.size - 1, n, kte.ctypes.data_as(_D), gam.ctypes.data_as(_D),
+                          sh.ctypes.data_as(_D), out.ctypes.data_as(_D)))
+    return out
 
 
 def lmodrelconv(energy, params, flux):

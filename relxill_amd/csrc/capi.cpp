@@ -606,6 +606,28 @@ int rx_copy_test(int64_t bytes, int reps, double *ms) {
   return e ? RX_ERR_DEVICE : RX_OK;
 }
 
+int rx_nthcomp(const double *energy, int n_flux, int n_spec, const double *kte, const double *gam,
+               const double *ener_shift, double *flux) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!energy || !kte || !gam || !ener_shift || !flux || n_flux <= 0 || n_spec <= 0) {
+    set_last(RX_ERR_ARG, "null pointer, empty grid or no spectrum");
+    return RX_ERR_ARG;
+  }
+  std::string err;
+  int st = ensure_device(err);
+  if (st == RX_OK) {
+    try {
+      st = nthcomp_spectra(energy, n_flux, n_spec, kte, gam, ener_shift, flux, err);
+    } catch (...) {
+      st = RX_ERR_DEVICE;
+      err = "unexpected exception";
+    }
+  }
+  set_last(st, err);
+  return st;
+}
+
 int rx_device_synchronize(void) {
   Global &g = G();
   if (!g.device_ready) return RX_OK;

@@ -158,6 +158,9 @@ struct Global {
 Global &G();
 void ensure_tables_host();
 int ensure_device(std::string &err);  // device + uploads (tables, conv grid, refl sets)
+// rx_nthcomp: n_spec nthcomp spectra on one caller grid (host arrays)
+int nthcomp_spectra(const double *ener, int n_flux, int n_spec, const double *kte, const double *gam,
+                    const double *ener_shift, double *flux, std::string &err);
 void ensure_xilltable_host();
 void ensure_rradtable_host();
 int ensure_rradtable_device(std::string &err);

@@ -301,6 +301,16 @@ int launch_rrad_corr(const BatchDev &b, const XillWork *work, int n_work, void *
 // relxill requests: the primary continuum added to the output (after launch_conv)
 int launch_primary(const BatchDev &b, const int *reqs, int n_reqs, void *stream);
 int launch_copy(const void *src, void *dst, size_t bytes, void *stream);
+
+// ---- nthcomp primary continuum (nthcomp.hip) ----
+// photon energies of one Kompaneets solve (the reference's arrays of 900)
+constexpr int NTH_MAX_POINTS = 900;
+struct NthJob {
+  double kte, gam, ktbb;  // keV, photon index, seed temperature (keV)
+};
+// one solve per job: x (NTH_MAX_POINTS + 1 per job, m_e c^2), E F_E
+// (NTH_MAX_POINTS per job), the number of points
+int launch_nthcomp(const NthJob *jobs, int n_jobs, double *sol_x, double *sol_s, int *sol_n, void *stream);
 int launch_refl_prepare(const DevConvGrid &cg, const double *refl_ener, const double *refl,
                         int n_refl, int n_incl, double *spec, double *bandsum, double *fft,
                         void *stream);
