@@ -46,6 +46,11 @@ extern "C" {
 /* project id: relxill-style convolution of a resident angle-resolved
  * reflection set (rx_reflset_create) with relconv's parameters */
 #define RX_MODEL_RELXILL_CONV (-1001)
+/* project ids: the nthcomp-primary relxill models (relxillCp, relxilllpCp;
+ * the reference's MOD_TYPE_RELXILL / MOD_TYPE_RELXILLLP with
+ * PRIM_SPEC_NTHCOMP, src/ModelDefinition.cpp:38-50, :88) */
+#define RX_MODEL_RELXILLCP (-1010)
+#define RX_MODEL_RELXILLLPCP (-1011)
 
 /* ---- precision of the batched path ---- */
 #define RX_FP64 0 /* plain form: FP64 throughout */
@@ -104,6 +109,27 @@ void lmodrelxill(const double *energy, int Nflux, const double *parameter, int s
  * 2 and -1 use the returning part alone. */
 void lmodrelxilllp(const double *energy, int Nflux, const double *parameter, int spectrum,
                    double *flux, double *fluxError, const char *init);
+
+/* relxillCp: Incl a Rin Rout Rbr Index1 Index2 z gamma logxi logN Afe kTe
+ * refl_frac (lmodel_relxill_public.dat:96-110; the reference's
+ * c_lmodrelxilldensnthcomp): relxill with the nthcomp primary
+ * (spec_nthcomp, src/Xillspec.cpp:243-253; kT_bb 0.05 keV) and the nthcomp
+ * xillver table, interpolated in Gamma, A_Fe, logXi, kTe (and logN when the
+ * table has a density axis). */
+void lmodrelxilldensnthcomp(const double *energy, int Nflux, const double *parameter, int spectrum,
+                            double *flux, double *fluxError, const char *init);
+
+/* relxilllpCp: Incl a Rin Rout h beta gamma logxi logN Afe kTe refl_frac z
+ * iongrad_index iongrad_type switch_returnrad switch_reflfrac_boost
+ * (lmodel_relxill_public.dat:112-129; the reference's
+ * c_lmodrelxilllpdensnthcomp): relxilllp with the nthcomp primary.  kTe is
+ * the source's (not shifted to the observer as relxilllp's Ecut,
+ * src/Relxill.cpp:197-200); each zone sees kTe times its source -> disk
+ * shift, and its normalisation change is that of the nthcomp spectrum.
+ * iongrad_type must be 0 (constant ionisation; the gradients return
+ * RX_ERR_UNSUPPORTED). */
+void lmodrelxilllpdensnthcomp(const double *energy, int Nflux, const double *parameter, int spectrum,
+                              double *flux, double *fluxError, const char *init);
 
 /* Status-returning evaluation of one model (the body of xspec_C_wrapper_eval_model,
  * src/LocalModel.cpp:143-160, and LocalModel::eval_model, src/LocalModel.h:108-128).
@@ -197,6 +223,18 @@ typedef struct {
   const float *spec;  /* rows as stored in the file (before the 10^logXi division) */
 } rx_xilltable_view;
 int rx_xilltable_get_view(rx_xilltable_view *v);
+
+/* The nthcomp models' xillver table (kTe in the place of Ecut: the same
+ * parameter id 3; the reference's xillverCp_v3.4.fits, src/common.h:166,
+ * get_init_xillver_table, src/xilltable.c:733-740), held beside the one
+ * above.  Loaded from RELXILL_TABLE_PATH/xillverCp_v3.4.fits on the first
+ * relxillCp / relxilllpCp request when present, else a synthetic table as
+ * above with its fourth axis kTe 1-400 keV.  Five-parameter tables only: a
+ * six-parameter table (with a density axis, as the released
+ * xillverCp_v3.4.fits) is rejected with RX_ERR_TABLE. */
+int rx_xillcptable_synthetic(uint64_t seed, int n_gam, int n_afe, int n_lxi, int n_kte, int n_incl, int n_ener);
+int rx_xillcptable_load_fits(const char *path);
+int rx_xillcptable_get_view(rx_xilltable_view *v);
 
 /* Returning-radiation table of the lamp-post models (switch_returnrad = 1;
  * the reference's table_returnRad_v20220301.fits, src/common.h:126-128):

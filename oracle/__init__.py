@@ -22,6 +22,8 @@ MOD_RELCONVLP = 12
 MOD_RELXILL = -1
 MOD_RELXILLLP = -2
 MOD_RELXILL_CONV = -1001
+MOD_RELXILLCP = -1010
+MOD_RELXILLLPCP = -1011
 
 _D = ctypes.POINTER(ctypes.c_double)
 _F = ctypes.POINTER(ctypes.c_float)
@@ -75,7 +77,7 @@ class XillTable(ctypes.Structure):
 class XillParam(ctypes.Structure):
     _fields_ = [("gam", ctypes.c_double), ("afe", ctypes.c_double), ("lxi", ctypes.c_double),
                 ("ect", ctypes.c_double), ("dens", ctypes.c_double), ("incl", ctypes.c_double),
-                ("refl_frac", ctypes.c_double), ("boost", _I)]
+                ("refl_frac", ctypes.c_double), ("boost", _I), ("prim", _I)]
 
 
 class Job(ctypes.Structure):
@@ -131,6 +133,8 @@ def _load():
         "orc_spec_cutoffpl": (None, [_D, _D, _I, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
         "orc_norm_wrt_xillver": (ctypes.c_double, [_D, _D, _I, ctypes.POINTER(_I)]),
         "orc_primary_norm_factor": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
+        "orc_prim_norm_factor": (ctypes.c_double, [ctypes.POINTER(XillParam), ctypes.c_double]),
+        "orc_prim_spectrum": (None, [_D, _D, _I, ctypes.POINTER(XillParam), ctypes.c_double, ctypes.c_double]),
         "orc_mcd_shape": (ctypes.c_double, [ctypes.c_double]),
         "orc_nthcomp_solve": (_I, [ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _D]),
         "orc_nthcomp_photar": (None, [_D, _D, _I, _D, _I, ctypes.c_double, _D]),
@@ -388,8 +392,15 @@ class Xill:
         return self.arrays["n_val"][-1]
 
 
-def xill_param(gam, afe, lxi, ect, dens=15.0, incl=30.0, refl_frac=0.0):
-    return XillParam(gam, afe, lxi, ect, dens, incl, refl_frac, 0)
+def xill_param(gam, afe, lxi, ect, dens=15.0, incl=30.0, refl_frac=0.0, prim=0):
+    """xillver parameters; prim 1: the nthcomp primary (ect is kTe)"""
+    return XillParam(gam, afe, lxi, ect, dens, incl, refl_frac, 0, prim)
+
+
+def prim_norm_factor(gam, ect, prim=0):
+    """1 / calcNormWrtXillverTableSpec of the primary (cut-off power law, or
+    nthcomp with prim=1) on the 500-bin grid"""
+    return lib.orc_prim_norm_factor(ctypes.byref(xill_param(gam, 1.0, 0.0, ect, prim=prim)), ect)
 
 
 def xill_corner(xt, xp):
@@ -466,12 +477,13 @@ def relxill(tables, xt, params, energy, conv_grid=None, phys_norm=False):
 
 
 def relxill_model(model, tables, xt, params, energy, conv_grid=None, phys_norm=False):
-    """relxill (MOD_RELXILL) or relxilllp (MOD_RELXILLLP) on the caller grid;
+    """relxill (MOD_RELXILL), relxilllp (MOD_RELXILLLP) or their nthcomp
+    forms (MOD_RELXILLCP, MOD_RELXILLLPCP) on the caller grid;
     returns (flux, dist[num_zones, n_incl], status)."""
     e = _arr(energy)
     n = e.size - 1
     out = np.zeros(n)
-    nz = 10 if model == MOD_RELXILLLP else 1
+    nz = 10 if model in (MOD_RELXILLLP, MOD_RELXILLLPCP) else 1
     dist = np.zeros(nz * xt.n_incl)
     cg = _arr(conv_grid) if conv_grid is not None else None
     st = lib.orc_relxill_model(model, ctypes.byref(tables.rel), tables.lp_ptr, ctypes.byref(xt.t),
@@ -483,7 +495,7 @@ def relxill_model(model, tables, xt, params, energy, conv_grid=None, phys_norm=F
 def relxill_zones(model, tables, params):
     """per zone: r_lo, r_hi, del_emit(centre), energy shift source -> disk,
     Ecut seen by the zone, normalisation change; (array[nz, 6], status)"""
-    nz = 10 if model == MOD_RELXILLLP else 1
+    nz = 10 if model in (MOD_RELXILLLP, MOD_RELXILLLPCP) else 1
     out = np.zeros(6 * nz)
     st = lib.orc_relxill_zones(model, ctypes.byref(tables.rel), tables.lp_ptr, _dp(_arr(params)), _dp(out))
     return out.reshape(nz, 6), st

@@ -309,6 +309,8 @@ int orc_n_params(int model_type) {
     case ORC_MOD_RELXILL: return 13;
     case ORC_MOD_RELXILLLP: return 14;
     case ORC_MOD_RELXILL_CONV: return 8;  /* relxill-style conv: relconv's parameters */
+    case ORC_MOD_RELXILLCP: return 14;
+    case ORC_MOD_RELXILLLPCP: return 17;
     default: return -1;
   }
 }
@@ -1454,7 +1456,7 @@ void orc_xill_corr_factors(const orc_xilltable *t, const orc_xillparam *x, doubl
    * the normalised cut-off power law on the coarse grid */
   double coarse[N_ENER_COARSE + 1], prim[N_ENER_COARSE];
   orc_get_log_grid(coarse, N_ENER_COARSE + 1, EMIN_XILL_NORM, EMAX_XILL_NORM);
-  orc_spec_cutoffpl(prim, coarse, N_ENER_COARSE, x->ect, x->gam, 1.0);
+  orc_prim_spectrum(prim, coarse, N_ENER_COARSE, x, x->ect, 1.0);
   int st = 0;
   const double nf = 1. / orc_norm_wrt_xillver(prim, coarse, N_ENER_COARSE, &st);
   for (int ii = 0; ii < N_ENER_COARSE; ii++) prim[ii] *= nf;
@@ -1504,8 +1506,49 @@ double orc_primary_norm_factor(double gam, double ect) {
   return 1. / orc_norm_wrt_xillver(flux, ener, N_ENER_COARSE, NULL);
 }
 
+void orc_prim_spectrum(double *flux, const double *ener, int n, const orc_xillparam *x, double ect, double shift) {
+  if (x->prim) orc_spec_nthcomp(flux, ener, n, ect, x->gam, shift);
+  else orc_spec_cutoffpl(flux, ener, n, ect, x->gam, shift);
+}
+
+double orc_prim_norm_factor(const orc_xillparam *x, double ect) {
+  double ener[N_ENER_COARSE + 1], flux[N_ENER_COARSE];
+  orc_get_log_grid(ener, N_ENER_COARSE + 1, EMIN_XILL_NORM, EMAX_XILL_NORM);
+  orc_prim_spectrum(flux, ener, N_ENER_COARSE, x, ect, 1.0);
+  return 1. / orc_norm_wrt_xillver(flux, ener, N_ENER_COARSE, NULL);
+}
+
+/* relxillCp / relxilllpCp (lmodel_relxill_public.dat:96-129; convertModelType,
+ * src/ModelDefinition.cpp:38-50): the parameters in relxill's / relxilllp's
+ * order with kTe in Ecut's place; returns that model, *dens logN and
+ * *iongrad relxilllpCp's iongrad_type */
+static int cp_remap(int model_type, const double *par, double *out, double *dens, int *iongrad) {
+  /* relxillCp: Incl a Rin Rout Rbr Index1 Index2 z gamma logxi logN Afe kTe refl_frac */
+  static const int k_bkn[13] = {5, 6, 4, 1, 0, 2, 3, 7, 8, 9, 11, 12, 13};
+  /* relxilllpCp: Incl a Rin Rout h beta gamma logxi logN Afe kTe refl_frac z
+   * iongrad_index iongrad_type switch_returnrad switch_reflfrac_boost */
+  static const int k_lp[14] = {4, 5, 1, 0, 2, 3, 12, 6, 7, 9, 10, 11, 15, 16};
+  *iongrad = 0;
+  if (model_type == ORC_MOD_RELXILLCP) {
+    for (int i = 0; i < 13; i++) out[i] = par[k_bkn[i]];
+    *dens = par[10];
+    return ORC_MOD_RELXILL;
+  }
+  for (int i = 0; i < 14; i++) out[i] = par[k_lp[i]];
+  *dens = par[8];
+  *iongrad = (int)lround(par[14]);
+  return ORC_MOD_RELXILLLP;
+}
+
 int orc_get_relxill_params(int model_type, const double *par, int phys_norm_env, orc_relparam *p,
                            orc_xillparam *x) {
+  double cpar[14], dens = 15;
+  int prim = 0, iongrad = 0;
+  if (model_type == ORC_MOD_RELXILLCP || model_type == ORC_MOD_RELXILLLPCP) {
+    model_type = cp_remap(model_type, par, cpar, &dens, &iongrad);
+    par = cpar;
+    prim = 1;
+  }
   int status = orc_get_rel_params(model_type, par, phys_norm_env, p);
   /* get_xill_params (src/ModelDefinition.cpp:339-390): Ecut in keV, the
    * density at its default 15; logXi kept within 0-4.7 for the zones
@@ -1520,9 +1563,15 @@ int orc_get_relxill_params(int model_type, const double *par, int phys_norm_env,
     x->gam = par[8]; x->lxi = par[9]; x->afe = par[10]; x->ect = par[11]; x->refl_frac = par[12];
     x->incl = par[4];
   }
-  x->dens = 15;
+  /* logN: 15 for the cut-off power-law models, the parameter for the nthcomp ones
+   * (get_xill_params, src/ModelDefinition.cpp:357-368) */
+  x->dens = dens;
+  x->prim = prim;
   if (x->lxi < 0.0) x->lxi = 0.0;
   else if (x->lxi > 4.7) x->lxi = 4.7;
+  /* constant ionisation only: relxilllpCp's radial ionisation gradients
+   * (iongrad_type != 0, src/IonGradient.cpp:340-395) are not restated */
+  if (status == 0 && iongrad != 0) status = 2;
   return status;
 }
 
@@ -1576,10 +1625,11 @@ int orc_relxill_model(int model_type, const orc_reltable *tab, const orc_lptable
   if (p.z > 0)
     for (int ii = 0; ii <= n_flux; ii++) ener[ii] *= (1 + p.z);
   /* the lamp post gives Ecut in the observer frame: the source's is Ecut / g
-   * (src/Relxill.cpp:197-200, src/PrimarySource.h:204-216) */
+   * (src/Relxill.cpp:197-200, src/PrimarySource.h:204-216); kTe of the
+   * nthcomp models is the source's already */
   const double g_so = orc_energy_shift_source_obs(&p);
   orc_xillparam xs = x;
-  if (p.emis_type == ORC_EMIS_LP) xs.ect /= g_so;
+  if (p.emis_type == ORC_EMIS_LP && !x.prim) xs.ect /= g_so;
   orc_syspar *sp = orc_get_system_parameters(tab, lp, &p, 1000, &status);
   double *cf_flux = (double *)malloc(sizeof(double) * nz), *cf_g = (double *)malloc(sizeof(double) * nz);
   if (sp) {
@@ -1622,7 +1672,7 @@ int orc_relxill_model(int model_type, const orc_reltable *tab, const orc_lptable
   if (status == 0) {
     /* normalisation change source -> disk (calc_xillver_normalization_change_source_to_disk,
      * src/Xillspec.cpp:408-435) */
-    const double nf_src = orc_primary_norm_factor(xs.gam, xs.ect);
+    const double nf_src = orc_prim_norm_factor(&xs, xs.ect);
     for (int iz = 0; iz < nz; iz++) {
       /* 2 + 5: the zone's xillver spectra (Ecut seen by the zone,
        * get_xill_param_zone, src/IonGradient.cpp:196-209), angle weighting */
@@ -1630,7 +1680,7 @@ int orc_relxill_model(int model_type, const orc_reltable *tab, const orc_lptable
       xz.ect = xs.ect * shift[iz];
       orc_xill_spectra(xt, &xz, xe, xflu);
       orc_xill_angdep(xill, xflu, dist + (size_t)iz * n_incl, n_incl, nx);
-      const double ncf = orc_primary_norm_factor(xs.gam, xs.ect * shift[iz]) / nf_src;
+      const double ncf = orc_prim_norm_factor(&xs, xs.ect * shift[iz]) / nf_src;
       for (int jj = 0; jj < nx; jj++) xill[jj] /= ncf;
       /* 6: relxill_convolution_multizone (src/Relxill.cpp:432-482) */
       const double *rz = rel + (size_t)iz * n_conv;
@@ -1642,9 +1692,11 @@ int orc_relxill_model(int model_type, const orc_reltable *tab, const orc_lptable
       for (int jj = 0; jj < n_conv; jj++) total[jj] += conv_out[jj];
     }
     /* add_primary_spectrum (src/PrimarySource.cpp:66-125): the observed
-     * cut-off power law (Ecut of the source shifted by g, normalised as the
-     * xillver incident spectrum at the source, src/PrimarySource.h:238-292) */
-    orc_spec_cutoffpl(prim, conv_ener, n_conv, xs.ect, xs.gam, g_so);
+     * primary (the cut-off power law with the source's Ecut shifted by g, or
+     * nthcomp at redshift 1/g - 1), normalised as the xillver incident
+     * spectrum at the source (calc_normalized_primary_spectrum,
+     * src/Xillspec.cpp:302-332) */
+    orc_prim_spectrum(prim, conv_ener, n_conv, &xs, xs.ect, g_so);
     for (int jj = 0; jj < n_conv; jj++) prim[jj] *= nf_src;
     if (p.emis_type != ORC_EMIS_LP) {
       for (int jj = 0; jj < n_conv; jj++) total[jj] *= fabs(x.refl_frac);
@@ -1692,8 +1744,8 @@ int orc_relxill_zones(int model_type, const orc_reltable *tab, const orc_lptable
   }
   if (p.emis_type == ORC_EMIS_LP) inv_rebin_mean(sp->re, sp->del_emit, sp->nr, rmean, del, nz);
   const double g_so = orc_energy_shift_source_obs(&p);
-  const double ect_src = (p.emis_type == ORC_EMIS_LP) ? x.ect / g_so : x.ect;
-  const double nf_src = orc_primary_norm_factor(x.gam, ect_src);
+  const double ect_src = (p.emis_type == ORC_EMIS_LP && !x.prim) ? x.ect / g_so : x.ect;
+  const double nf_src = orc_prim_norm_factor(&x, ect_src);
   for (int iz = 0; iz < nz; iz++) {
     const double shift =
         (p.emis_type == ORC_EMIS_LP) ? orc_gi_potential_lp(rmean[iz], p.a, p.height, p.beta, del[iz]) : 1.0;
@@ -1703,7 +1755,7 @@ int orc_relxill_zones(int model_type, const orc_reltable *tab, const orc_lptable
     o[2] = del[iz];
     o[3] = shift;
     o[4] = ect_src * shift;
-    o[5] = orc_primary_norm_factor(x.gam, ect_src * shift) / nf_src;
+    o[5] = orc_prim_norm_factor(&x, ect_src * shift) / nf_src;
   }
   orc_syspar_free(sp);
   return 0;

@@ -37,6 +37,10 @@ extern "C" {
 /* project id: relxill-style convolution of a given angle-resolved reflection
  * set, with relconv's 8 parameters and relxill's profile normalisation */
 #define ORC_MOD_RELXILL_CONV (-1001)
+/* relxillCp / relxilllpCp: relxill / relxilllp with the nthcomp primary
+ * (parameters of lmodel_relxill_public.dat:96-129) */
+#define ORC_MOD_RELXILLCP (-1010)
+#define ORC_MOD_RELXILLLPCP (-1011)
 
 /* emissivity types (src/common.h:50-55) */
 #define ORC_EMIS_BKN 1
@@ -230,6 +234,7 @@ typedef struct {
   double gam, afe, lxi, ect, dens, incl;  /* incl in degrees */
   double refl_frac;
   int boost;  /* switch_reflfrac_boost: refl_frac relative to the lamp post's */
+  int prim;   /* primary continuum: 0 cut-off power law (ect = Ecut), 1 nthcomp (ect = kTe) */
 } orc_xillparam;
 
 /* get_xilltab_indices_for_paramvals (src/xilltable.c:297-324) */
@@ -269,6 +274,12 @@ int orc_nthcomp_solve(double tempbb, double theta, double gamma, double *x, doub
 /* photons per bin of ear[0..ne] at redshift z, E F_E(1 keV, source) = 1 */
 void orc_nthcomp_photar(const double *x, const double *spt, int nth, const double *ear, int ne, double z,
                         double *photar);
+/* calc_primary_spectrum (src/Xillspec.cpp:278-300) for the primary of x with
+ * Ecut / kTe = ect and the energy shift `shift` */
+void orc_prim_spectrum(double *flux, const double *ener, int n, const orc_xillparam *x, double ect, double shift);
+/* 1 / calcNormWrtXillverTableSpec of that primary (unshifted) on the 500-bin
+ * 0.1-1000 keV grid (src/Xillspec.cpp:179-205, :302-315) */
+double orc_prim_norm_factor(const orc_xillparam *x, double ect);
 /* spec_nthcomp (src/Xillspec.cpp:243-253) */
 void orc_spec_nthcomp(double *flux, const double *ener, int n, double kte, double gam, double ener_shift);
 /* relxill parameters: Index1 Index2 Rbr a Incl Rin Rout z gamma logxi Afe

@@ -12,8 +12,8 @@ _LIB_NAMES = {
     "FP32", "FP64", "MODEL_RELCONV", "MODEL_RELCONVLP", "MODEL_RELLINE", "MODEL_RELLINELP",
     "MODEL_RELXILL_CONV", "MODEL_RELXILL", "MODEL_RELXILLLP", "DEFAULTS", "N_PARAMS", "Batch", "RxError", "eval_model", "lib",
     "lmodrelconv", "lmodrelconvlp", "lmodrelline", "lmodrellinelp", "lmodrelxill", "lmodrelxilllp", "model_params",
-    "nthcomp",
-    "xilltable_synthetic", "xilltable_view", "rradtable_synthetic", "rradtable_view",
+    "nthcomp", "MODEL_RELXILLCP", "MODEL_RELXILLLPCP", "lmodrelxilldensnthcomp", "lmodrelxilllpdensnthcomp",
+    "xilltable_synthetic", "xillcptable_synthetic", "xilltable_view", "rradtable_synthetic", "rradtable_view",
     "reflset_create", "set_conv_grid", "tables_synthetic", "lptable_synthetic", "tables_view",
     "Server", "RX_OK", "RX_ERR_PARAM", "RX_ERR_UNSUPPORTED", "RX_ERR_TABLE", "RX_ERR_INPUT",
     "RX_ERR_DEVICE", "RX_ERR_ARG",

@@ -121,6 +121,16 @@ void lmodrelxilllp(const double *energy, int Nflux, const double *parameter, int
   rx_eval_model(RX_MODEL_RELXILLLP, parameter, energy, flux, Nflux, RX_FP64);
 }
 
+void lmodrelxilldensnthcomp(const double *energy, int Nflux, const double *parameter, int, double *flux, double *,
+                            const char *) {
+  rx_eval_model(RX_MODEL_RELXILLCP, parameter, energy, flux, Nflux, RX_FP64);
+}
+
+void lmodrelxilllpdensnthcomp(const double *energy, int Nflux, const double *parameter, int, double *flux,
+                              double *, const char *) {
+  rx_eval_model(RX_MODEL_RELXILLLPCP, parameter, energy, flux, Nflux, RX_FP64);
+}
+
 int rx_last_status(void) { return last_status(); }
 const char *rx_last_error(void) { return last_error(); }
 
@@ -255,29 +265,33 @@ int rx_lptable_load_fits(const char *path) {
   return RX_OK;
 }
 
-int rx_xilltable_synthetic(uint64_t seed, int n_gam, int n_afe, int n_lxi, int n_ect, int n_incl, int n_ener) {
+static int xill_synthetic(int prim, uint64_t seed, int n_gam, int n_afe, int n_lxi, int n_ect, int n_incl,
+                          int n_ener) {
   Global &g = G();
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   if (n_gam < 2 || n_afe < 2 || n_lxi < 2 || n_ect < 2 || n_incl < 2 || n_incl > MAX_INCL || n_ener < 2) {
     set_last(RX_ERR_ARG, "xillver table dims out of range");
     return RX_ERR_ARG;
   }
+  XillSlot &x = g.xill[prim];
   try {
-    g.xill = make_synthetic_xilltable(seed, n_gam, n_afe, n_lxi, n_ect, n_incl, n_ener);
+    // the fourth axis: Ecut 5-1000 keV, or kTe 1-400 keV for nthcomp
+    x.t = prim == PRIM_NTHCOMP ? make_synthetic_xilltable(seed, n_gam, n_afe, n_lxi, n_ect, n_incl, n_ener, 1.0, 400.0)
+                               : make_synthetic_xilltable(seed, n_gam, n_afe, n_lxi, n_ect, n_incl, n_ener);
   } catch (...) {
     set_last(RX_ERR_NOMEM, "table allocation failed");
     return RX_ERR_NOMEM;
   }
-  g.xill_loaded = true;
-  g.xill_err.clear();
-  g.xill_dirty = true;
+  x.loaded = true;
+  x.err.clear();
+  x.dirty = true;
   g.config_gen++;
-  std::string e = validate_xilltable(g.xill);
+  std::string e = validate_xilltable(x.t);
   set_last(e.empty() ? RX_OK : RX_ERR_TABLE, e);
   return e.empty() ? RX_OK : RX_ERR_TABLE;
 }
 
-int rx_xilltable_load_fits(const char *path) {
+static int xill_load_fits(int prim, const char *path) {
   Global &g = G();
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   if (!path) return RX_ERR_ARG;
@@ -296,24 +310,26 @@ int rx_xilltable_load_fits(const char *path) {
     set_last(RX_ERR_TABLE, "xillver table has more inclinations than the kernels support");
     return RX_ERR_TABLE;
   }
-  g.xill = std::move(t);
-  g.xill_loaded = true;
-  g.xill_err.clear();
-  g.xill_dirty = true;
+  XillSlot &x = g.xill[prim];
+  x.t = std::move(t);
+  x.loaded = true;
+  x.err.clear();
+  x.dirty = true;
   g.config_gen++;
   return RX_OK;
 }
 
-int rx_xilltable_get_view(rx_xilltable_view *v) {
+static int xill_get_view(int prim, rx_xilltable_view *v) {
   Global &g = G();
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   if (!v) return RX_ERR_ARG;
-  ensure_xilltable_host();
-  if (!g.xill_loaded) {
-    set_last(RX_ERR_TABLE, "xillver table: " + g.xill_err);
+  ensure_xilltable_host(prim);
+  const XillSlot &x = g.xill[prim];
+  if (!x.loaded) {
+    set_last(RX_ERR_TABLE, "xillver table: " + x.err);
     return RX_ERR_TABLE;
   }
-  const XillTableHost &t = g.xill;
+  const XillTableHost &t = x.t;
   memset(v, 0, sizeof(*v));
   v->n_par = t.n_par;
   for (int k = 0; k < t.n_par && k < 5; k++) {
@@ -327,6 +343,18 @@ int rx_xilltable_get_view(rx_xilltable_view *v) {
   v->spec = t.spec.data();
   return RX_OK;
 }
+
+int rx_xilltable_synthetic(uint64_t seed, int n_gam, int n_afe, int n_lxi, int n_ect, int n_incl, int n_ener) {
+  return xill_synthetic(PRIM_CUTOFFPL, seed, n_gam, n_afe, n_lxi, n_ect, n_incl, n_ener);
+}
+int rx_xilltable_load_fits(const char *path) { return xill_load_fits(PRIM_CUTOFFPL, path); }
+int rx_xilltable_get_view(rx_xilltable_view *v) { return xill_get_view(PRIM_CUTOFFPL, v); }
+
+int rx_xillcptable_synthetic(uint64_t seed, int n_gam, int n_afe, int n_lxi, int n_kte, int n_incl, int n_ener) {
+  return xill_synthetic(PRIM_NTHCOMP, seed, n_gam, n_afe, n_lxi, n_kte, n_incl, n_ener);
+}
+int rx_xillcptable_load_fits(const char *path) { return xill_load_fits(PRIM_NTHCOMP, path); }
+int rx_xillcptable_get_view(rx_xilltable_view *v) { return xill_get_view(PRIM_NTHCOMP, v); }
 
 int rx_rradtable_synthetic(uint64_t seed, int n_spin, int n_rad, int n_g) {
   Global &g = G();

@@ -72,14 +72,19 @@ int n_model_params(int model) {
     case MOD_RELXILL_CONV: return 8;
     case MOD_RELXILL: return 13;
     case MOD_RELXILLLP: return 14;
+    case MOD_RELXILLCP: return 14;
+    case MOD_RELXILLLPCP: return 17;
     default: return -1;
   }
 }
 
 // get_rel_params (src/ModelDefinition.cpp:278-334) with check_parameter_bounds (:168-273)
-int parse_request(int model, const double *par, bool phys_norm, ReqParams &p, std::string &err) {
+int parse_request(int model, const double *par, bool phys_norm, ReqParams &p, std::string &err, int prim) {
   memset(&p, 0, sizeof(p));
   p.model = model;
+  p.prim = prim;
+  p.nth_off = -1;
+  p.nth_sol = -1;
   p.pgrid_off = -1;
   p.prof_off = -1;
   double incl_deg = 0, htop = 0;
@@ -205,8 +210,9 @@ int parse_request(int model, const double *par, bool phys_norm, ReqParams &p, st
       if (!(p.beta < 1e-4)) p.g_so = g_inf_0 * (sqrt(1.0 - p.beta * p.beta) / (1.0 + p.beta * cos(kPi - p.incl)));
     }
     // the lamp post gives Ecut in the observer frame; the xillver tables and
-    // the normalisation use the source's (src/Relxill.cpp:197-200)
-    p.xect_src = (p.emis_type == EMIS_LP) ? p.xect / p.g_so : p.xect;
+    // the normalisation use the source's (src/Relxill.cpp:197-200).  kTe of
+    // the nthcomp models is already the source's.
+    p.xect_src = (p.emis_type == EMIS_LP && prim == PRIM_CUTOFFPL) ? p.xect / p.g_so : p.xect;
   }
   return RX_OK;
 }
@@ -527,30 +533,33 @@ int ensure_device(std::string &err) {
   return RX_OK;
 }
 
-void ensure_xilltable_host() {
-  Global &g = G();
-  if (g.xill_loaded) return;
+void ensure_xilltable_host(int prim) {
+  XillSlot &x = G().xill[prim];
+  if (x.loaded) return;
   if (!getenv("RELXILL_TABLE_PATH")) {
-    g.xill = make_synthetic_xilltable(5, 13, 5, 8, 6, 10, 2999);
-    g.xill_err.clear();
-    g.xill_loaded = true;
+    // synthetic tables: Ecut 5-1000 keV, kTe 1-400 keV on the fourth axis
+    x.t = prim == PRIM_NTHCOMP ? make_synthetic_xilltable(6, 13, 5, 8, 6, 10, 2999, 1.0, 400.0)
+                               : make_synthetic_xilltable(5, 13, 5, 8, 6, 10, 2999);
+    x.err.clear();
+    x.loaded = true;
   } else {
-    // the reference's table name (src/common.h:163)
-    g.xill_loaded = load_from_path(g.xill, "xillver-a-Ec5.fits", nullptr, load_xilltable_fits,
-                                   load_xilltable_fits, g.xill_err);
+    // the reference's table names (src/common.h:163, :166)
+    x.loaded = load_from_path(x.t, prim == PRIM_NTHCOMP ? "xillverCp_v3.4.fits" : "xillver-a-Ec5.fits", nullptr,
+                              load_xilltable_fits, load_xilltable_fits, x.err);
   }
-  if (g.xill_loaded) g.xill_dirty = true;
+  if (x.loaded) x.dirty = true;
 }
 
-int ensure_xilltable_device(std::string &err) {
-  Global &g = G();
-  ensure_xilltable_host();
-  if (!g.xill_loaded) {
-    err = "xillver table: " + g.xill_err;
+int ensure_xilltable_device(int prim, std::string &err) {
+  XillSlot &x = G().xill[prim];
+  ensure_xilltable_host(prim);
+  const char *what = prim == PRIM_NTHCOMP ? "xillver table (nthcomp): " : "xillver table: ";
+  if (!x.loaded) {
+    err = what + x.err;
     return RX_ERR_TABLE;
   }
-  if (!g.xill_dirty) return RX_OK;
-  const XillTableHost &t = g.xill;
+  if (!x.dirty) return RX_OK;
+  const XillTableHost &t = x.t;
   if (t.n_incl() > MAX_INCL) {
     err = "xillver table has more inclinations than the kernels support";
     return RX_ERR_ARG;
@@ -560,25 +569,25 @@ int ensure_xilltable_device(std::string &err) {
     return RX_ERR_DEVICE;
   }
   // interp_xill_table's energy grid (src/xilltable.c:1104-1108)
-  g.xill_ener.resize(t.n_ener + 1);
-  for (int i = 0; i < t.n_ener; i++) g.xill_ener[i] = t.elo[i];
-  g.xill_ener[t.n_ener] = t.ehi[t.n_ener - 1];
-  int e = upload_vec(g.x_spec, xilltable_renormed(t)) | upload_vec(g.x_ener, g.xill_ener);
+  x.ener.resize(t.n_ener + 1);
+  for (int i = 0; i < t.n_ener; i++) x.ener[i] = t.elo[i];
+  x.ener[t.n_ener] = t.ehi[t.n_ener - 1];
+  int e = upload_vec(x.spec, xilltable_renormed(t)) | upload_vec(x.d_ener, x.ener);
   // the inclination axis (returning-radiation corrections of relxilllp)
-  e |= upload_vec(g.x_incl, t.val[t.n_par - 1]);
-  // the Ecut axis (the lamp post's zones bracket their own Ecut on the device)
-  // and the axis ensure_ecut_within_boundarys reads
+  e |= upload_vec(x.incl, t.val[t.n_par - 1]);
+  // the Ecut / kTe axis (the lamp post's zones bracket their own value on the
+  // device) and the axis ensure_ecut_within_boundarys reads
   const int k_ect = t.dim_of(XP_ECT);
   if (k_ect >= 0) {
-    e |= upload_vec(g.x_ect, t.val[k_ect]);
+    e |= upload_vec(x.ect, t.val[k_ect]);
     const int q = t.pid[k_ect] < t.n_par ? t.pid[k_ect] : k_ect;
-    e |= upload_vec(g.x_q, t.val[q]);
+    e |= upload_vec(x.q, t.val[q]);
   }
   if (e) {
     err = "xillver table upload failed";
     return RX_ERR_DEVICE;
   }
-  g.xill_dirty = false;
+  x.dirty = false;
   return RX_OK;
 }
 
@@ -682,23 +691,29 @@ void fill_dev_views(BatchDev &bd) {
   bd.cg.coarse_ln = g.c_coarse_ln.as<double>();
   bd.cg.ln_emid = g.c_ln_emid.as<double>();
   bd.refl_sets = g.d_refl_sets.as<DevReflSet>();
-  bd.xt.n_ener = g.xill.n_ener;
-  bd.xt.n_incl = g.xill.n_incl();
-  for (int k = 0; k < 4; k++) bd.xt.n_val[k] = g.xill.n_par ? g.xill.n_val[k] : 0;
-  bd.xt.spec = g.x_spec.as<float>();
-  bd.xt.ener = g.x_ener.as<double>();
-  bd.xt.ect_dim = g.xill.n_par ? g.xill.dim_of(XP_ECT) : -1;
-  bd.xt.ect_q = -1;
-  bd.xt.n_ect = bd.xt.n_q = 0;
-  if (bd.xt.ect_dim >= 0) {
-    const int q = g.xill.pid[bd.xt.ect_dim] < g.xill.n_par ? g.xill.pid[bd.xt.ect_dim] : bd.xt.ect_dim;
-    bd.xt.ect_q = q;
-    bd.xt.n_ect = g.xill.n_val[bd.xt.ect_dim];
-    bd.xt.n_q = g.xill.n_val[q];
+  for (int pr = 0; pr < N_PRIM; pr++) {
+    const XillSlot &x = g.xill[pr];
+    const XillTableHost &t = x.t;
+    DevXillTable &xt = bd.xt[pr];
+    const bool ok = x.loaded && t.n_par > 0;
+    xt.n_ener = ok ? t.n_ener : 0;
+    xt.n_incl = ok ? t.n_incl() : 0;
+    for (int k = 0; k < 4; k++) xt.n_val[k] = ok ? t.n_val[k] : 0;
+    xt.spec = x.spec.as<float>();
+    xt.ener = x.d_ener.as<double>();
+    xt.ect_dim = ok ? t.dim_of(XP_ECT) : -1;
+    xt.ect_q = -1;
+    xt.n_ect = xt.n_q = 0;
+    if (xt.ect_dim >= 0) {
+      const int q = t.pid[xt.ect_dim] < t.n_par ? t.pid[xt.ect_dim] : xt.ect_dim;
+      xt.ect_q = q;
+      xt.n_ect = t.n_val[xt.ect_dim];
+      xt.n_q = t.n_val[q];
+    }
+    xt.ect_vals = x.ect.as<float>();
+    xt.incl_vals = x.incl.as<float>();
+    xt.q_vals = x.q.as<float>();
   }
-  bd.xt.ect_vals = g.x_ect.as<float>();
-  bd.xt.incl_vals = g.x_incl.as<float>();
-  bd.xt.q_vals = g.x_q.as<float>();
   bd.rr.n_spin = g.rrad.n_spin;
   bd.rr.n_rad = g.rrad.n_rad;
   bd.rr.n_g = g.rrad.n_g;
@@ -731,11 +746,65 @@ static bool is_conv(int model) {
   return model == MOD_RELCONV || model == MOD_RELCONVLP || model == MOD_RELXILL_CONV || is_relxill(model);
 }
 
-int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
+// The nthcomp-primary models run as relxill / relxilllp requests
+// (convertModelType maps relxillCp to MOD_TYPE_RELXILL and relxilllpCp to
+// MOD_TYPE_RELXILLLP, src/ModelDefinition.cpp:38-50; their prim_type is
+// PRIM_SPEC_NTHCOMP, :88): their parameters in those models' order
+// (src/modelfiles/lmodel_relxill_public.dat:96-129) with kTe in Ecut's place,
+// and logN.  Returns the model to run, or 0 for other models.
+struct CpRequest {
+  int model = 0;
+  double par[14] = {};
+  double dens = 15.0;
+  int status = RX_OK;  // RX_ERR_UNSUPPORTED: an ionisation gradient was asked for
+};
+
+static CpRequest remap_cp(int model, const double *par) {
+  CpRequest c;
+  if (model == MOD_RELXILLCP) {
+    // Incl a Rin Rout Rbr Index1 Index2 z gamma logxi logN Afe kTe refl_frac
+    static const int k[13] = {5, 6, 4, 1, 0, 2, 3, 7, 8, 9, 11, 12, 13};
+    for (int i = 0; i < 13; i++) c.par[i] = par[k[i]];
+    c.dens = par[10];
+    c.model = MOD_RELXILL;
+  } else if (model == MOD_RELXILLLPCP) {
+    // Incl a Rin Rout h beta gamma logxi logN Afe kTe refl_frac z iongrad_index
+    // iongrad_type switch_returnrad switch_reflfrac_boost
+    static const int k[14] = {4, 5, 1, 0, 2, 3, 12, 6, 7, 9, 10, 11, 15, 16};
+    for (int i = 0; i < 14; i++) c.par[i] = par[k[i]];
+    c.dens = par[8];
+    c.model = MOD_RELXILLLP;
+    // iongrad_type 0: constant ionisation (src/IonGradient.cpp:340-395);
+    // the radial ionisation gradients are not built
+    if (lround(par[14]) != 0) c.status = RX_ERR_UNSUPPORTED;
+  }
+  return c;
+}
+
+int batch_build(Batch &b, const rx_request *req_in, int n_req, std::string &err) {
   Global &g = G();
-  if (n_req <= 0 || !req) {
+  if (n_req <= 0 || !req_in) {
     err = "empty batch";
     return RX_ERR_ARG;
+  }
+  // nthcomp-primary requests in their relxill form
+  std::vector<rx_request> req_local;
+  std::vector<CpRequest> cp;
+  std::vector<int> prim(n_req, PRIM_CUTOFFPL);
+  const rx_request *req = req_in;
+  for (int i = 0; i < n_req; i++) {
+    const int m = req_in[i].model;
+    if ((m == MOD_RELXILLCP || m == MOD_RELXILLLPCP) && req_in[i].parameter) {
+      if (cp.empty()) {
+        req_local.assign(req_in, req_in + n_req);
+        cp.resize(n_req);
+        req = req_local.data();
+      }
+      cp[i] = remap_cp(m, req_in[i].parameter);
+      req_local[i].model = cp[i].model;
+      req_local[i].parameter = cp[i].par;
+      prim[i] = PRIM_NTHCOMP;
+    }
   }
   const bool phys = phys_norm_env();
   b.n_req = n_req;
@@ -745,12 +814,16 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
   b.h_xill.clear();
   b.h_prim.clear();
   b.h_rrc.clear();
+  b.h_nth.clear();
+  b.n_nth_sol = 0;
   std::vector<XillWork> xill_multi;
-  for (int i = 0; i < n_req; i++) {
-    if (is_relxill(req[i].model)) {
-      int st = ensure_xilltable_device(err);
-      if (st != RX_OK) return st;
-      break;
+  for (int pr = 0; pr < N_PRIM; pr++) {
+    for (int i = 0; i < n_req; i++) {
+      if (is_relxill(req[i].model) && prim[i] == pr) {
+        int st = ensure_xilltable_device(pr, err);
+        if (st != RX_OK) return st;
+        break;
+      }
     }
   }
   for (int i = 0; i < n_req; i++) {
@@ -780,11 +853,12 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
     }
     ReqParams &p = b.h_req[i];
     std::string perr;
-    int st = parse_request(r.model, r.parameter, phys, p, perr);
+    int st = parse_request(r.model, r.parameter, phys, p, perr, prim[i]);
     if (st == RX_ERR_ARG) {
       err = perr;
       return st;
     }
+    if (st == RX_OK && prim[i] == PRIM_NTHCOMP) st = cp[i].status;
     p.n_in = r.n_flux;
     p.ener_off = o_ener;
     o_ener += r.n_flux + 1;
@@ -808,15 +882,28 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
         if (lxi < 0.0) lxi = 0.0;
         else if (lxi > 4.7) lxi = 4.7;
         const double afe = r.parameter[lp ? 9 : 10];
-        xill_corner(g.xill, p.xgam, afe, lxi, p.xect_src, 15.0, incl_deg_of(r.model, r.parameter), p.xind,
-                    p.xfac);
-        p.n_incl = g.xill.n_incl();
+        const XillTableHost &xt = g.xill[p.prim].t;
+        // logN: 15 for the cut-off power-law models (get_xill_params,
+        // src/ModelDefinition.cpp:367-368), the parameter for the nthcomp ones
+        const double dens = p.prim == PRIM_NTHCOMP ? cp[i].dens : 15.0;
+        xill_corner(xt, p.xgam, afe, lxi, p.xect_src, dens, incl_deg_of(r.model, r.parameter), p.xind, p.xfac);
+        // a table without a density axis divides its rows by 10^(logN - 15)
+        // (normalizeXillverSpecLogxiDensity, src/xilltable.c:488-511)
+        p.xdens_div = (xt.dim_of(XP_DNS) < 0 && std::fabs(dens - 15) > 1e-6) ? std::pow(10, dens - 15) : 0.0;
+        p.n_incl = xt.n_incl();
         p.xspec_off = o_xspec;
-        o_xspec += (int64_t)g.xill.n_ener * p.n_zones;
+        o_xspec += (int64_t)xt.n_ener * p.n_zones;
         p.zone_off = o_zone;
         o_zone += p.n_zones;
         (p.n_zones > 1 ? xill_multi : b.h_xill).push_back(XillWork{i, 0});
         b.h_prim.push_back(i);
+        if (p.prim == PRIM_NTHCOMP) {
+          // the source's solve, then one per lamp-post zone (k_nth)
+          p.n_nth = 1 + (p.emis_type == EMIS_LP ? p.n_zones : 0);
+          p.nth_off = (int64_t)b.h_nth.size();
+          p.nth_sol = b.n_nth_sol++;
+          for (int j = 0; j < p.n_nth; j++) b.h_nth.push_back(XillWork{i, j});
+        }
       } else if (st == RX_OK && r.model != MOD_RELXILL_CONV) {
         if (!r.flux_in) {
           err = "convolution request without input spectrum";
@@ -841,7 +928,7 @@ int batch_build(Batch &b, const rx_request *req, int n_req, std::string &err) {
     if (st == RX_OK && p.return_rad != 0) st = rrad_zones(g.rrad, p, g.nr_fine);
     p.rr_corr_off = 0;
     if (st == RX_OK && p.rr_corr) {
-      if (g.xill.n_ener > MAX_RRC_ENER) {
+      if (g.xill[p.prim].t.n_ener > MAX_RRC_ENER) {
         err = "xillver table has more energy bins than the returning-radiation corrections hold";
         return RX_ERR_ARG;
       }
@@ -953,6 +1040,11 @@ int batch_upload(Batch &b) {
   e |= b.d_prim.ensure(sizeof(int) * std::max<size_t>(1, b.h_prim.size()));
   e |= b.d_rrc.ensure(sizeof(XillWork) * std::max<size_t>(1, b.h_rrc.size()));
   e |= b.d_rrcorr.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_rrc));
+  e |= b.d_nthw.ensure(sizeof(XillWork) * std::max<size_t>(1, b.h_nth.size()));
+  e |= b.d_nth.ensure(sizeof(NthRes) * std::max<size_t>(1, b.h_nth.size()));
+  e |= b.d_nthx.ensure(sizeof(double) * (NTH_MAX_POINTS + 1) * std::max(1, b.n_nth_sol));
+  e |= b.d_nths.ensure(sizeof(double) * NTH_MAX_POINTS * std::max(1, b.n_nth_sol));
+  e |= b.d_nthn.ensure(sizeof(int) * std::max(1, b.n_nth_sol));
   e |= b.d_xspec.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_xspec));
   e |= b.d_items.ensure(sizeof(WorkItem) * (size_t)b.item_cap);
   e |= b.d_ctr.ensure(sizeof(int) * N_COUNTERS);
@@ -977,6 +1069,9 @@ int batch_upload(Batch &b) {
     e |= (int)hipMemcpyAsync(b.d_prim.p, b.h_prim.data(), sizeof(int) * b.h_prim.size(), hipMemcpyHostToDevice, b.stream);
   if (!b.h_xill.empty())
     e |= (int)hipMemcpyAsync(b.d_xill.p, b.h_xill.data(), sizeof(XillWork) * b.h_xill.size(), hipMemcpyHostToDevice,
+                             b.stream);
+  if (!b.h_nth.empty())
+    e |= (int)hipMemcpyAsync(b.d_nthw.p, b.h_nth.data(), sizeof(XillWork) * b.h_nth.size(), hipMemcpyHostToDevice,
                              b.stream);
   e |= (int)hipStreamSynchronize(b.stream);
   return e ? RX_ERR_DEVICE : RX_OK;
@@ -1014,6 +1109,10 @@ static BatchDev dev_view(Batch &b) {
   bd.xspec = b.d_xspec.as<double>();
   bd.zone = b.d_zone.as<ZoneRec>();
   bd.rrcorr = b.d_rrcorr.as<double>();
+  bd.nth = b.d_nth.as<NthRes>();
+  bd.nth_x = b.d_nthx.as<double>();
+  bd.nth_s = b.d_nths.as<double>();
+  bd.nth_n = b.d_nthn.as<int>();
   return bd;
 }
 
@@ -1023,6 +1122,7 @@ static int launch_sequence(Batch &b, const RunKey &k, hipStream_t s) {
   const bool prof = k.prof != 0;
   int e = 0;
   if (prof) e |= (int)hipEventRecord(b.ev[0], s);
+  e |= launch_nth(bd, static_cast<const XillWork *>(k.d_nth), k.n_nth, s);
   e |= launch_rrad_corr(bd, static_cast<const XillWork *>(k.d_rrc), k.n_rrc, s);
   e |= launch_setup(bd, s);
   if (prof) e |= (int)hipEventRecord(b.ev[1], s);
@@ -1096,6 +1196,8 @@ int batch_run(Batch &b, int precision, std::string &err) {
   k.d_prim = b.d_prim.p;
   k.d_rrc = b.d_rrc.p;
   k.n_rrc = (int)b.h_rrc.size();
+  k.d_nth = b.d_nthw.p;
+  k.n_nth = (int)b.h_nth.size();
   k.prof = g.profile ? 1 : 0;
   k.cos_rows = b.any_limb ? 1 : 0;
   hipStream_t s = b.stream;

@@ -24,7 +24,9 @@ struct DevBuf {
 
 // Host mirror of get_rel_params + check_parameter_bounds
 // (src/ModelDefinition.cpp:153-334) for one request.  Returns RX_OK or a status.
-int parse_request(int model, const double *par, bool phys_norm, ReqParams &p, std::string &err);
+// prim: the primary continuum of the relxill models (PRIM_*)
+int parse_request(int model, const double *par, bool phys_norm, ReqParams &p, std::string &err,
+                  int prim = PRIM_CUTOFFPL);
 // relxill requests: the xillver table corner and factors of the request's
 // parameters (get_xilltab_indices_for_paramvals and interp_xill_table,
 // src/xilltable.c:297-324, :1090-1125).
@@ -41,8 +43,8 @@ void rzone_grid(double rmin, double rmax, int nzones, double h, double *rgrid);
 // its padding zeroed, so configurations compare with memcmp.
 struct RunKey {
   BatchDev bd;
-  int n_blocks, n_xill1, n_xill, n_conv, n_prim, n_rrc;
-  const void *d_xill, *d_conv, *d_prim, *d_rrc;
+  int n_blocks, n_xill1, n_xill, n_conv, n_prim, n_rrc, n_nth;
+  const void *d_xill, *d_conv, *d_prim, *d_rrc, *d_nth;
   int prof, cos_rows;
 };
 
@@ -60,11 +62,14 @@ struct Batch {
   std::vector<XillWork> h_xill;  // k_xill units: one-zone requests first, then the others (all zones in one unit)
   int n_xill1 = 0;               // number of one-zone units at the front of h_xill
   std::vector<XillWork> h_rrc;   // (request, zone) units of the returning-radiation corrections
+  std::vector<XillWork> h_nth;   // nthcomp solves: (request, 0 source / 1 + zone)
+  int n_nth_sol = 0;             // requests with the nthcomp primary (one source solution each)
   int64_t tot_ener = 0, tot_flux = 0, tot_pgrid = 0, tot_prof = 0, tot_xspec = 0, tot_dist = 0, tot_zone = 0;
   int64_t tot_node = 0;  // node rows of the requests with an angle distribution
   int64_t tot_rrc = 0;
   DevBuf d_req, d_ener, d_flux_in, d_out, d_pgrid, d_node, d_rad, d_prof, d_dist, d_refl, d_status, d_conv;
   DevBuf d_xill, d_xspec, d_zone, d_prim, d_rrc, d_rrcorr;
+  DevBuf d_nthw, d_nth, d_nthx, d_nths, d_nthn;
   DevBuf d_items, d_ctr, d_fix, d_reach, d_rnorm;
   int item_cap = 0;
   bool any_cosne = false, any_limb = false;
@@ -101,6 +106,19 @@ int batch_sync(Batch &b);
 // Public status of a request from its device status.
 int public_status(int dev_status);
 
+// One resident xillver table: the host table, why a table named by
+// RELXILL_TABLE_PATH could not be loaded (empty: loaded), its energy edges
+// (elo, then the last ehi) and the device copies (renormalised spectra,
+// edges, the Ecut / kTe axis, the axis ensure_ecut_within_boundarys reads, the
+// inclinations).
+struct XillSlot {
+  XillTableHost t;
+  bool loaded = false, dirty = true;
+  std::string err;
+  std::vector<double> ener;
+  DevBuf spec, d_ener, ect, q, incl;
+};
+
 // global state
 struct Global {
   std::recursive_mutex mu;
@@ -111,16 +129,15 @@ struct Global {
   LpTableHost lp;
   bool rel_loaded = false, lp_loaded = false;
   // why a table named by RELXILL_TABLE_PATH could not be loaded (empty: loaded)
-  std::string rel_err, lp_err, xill_err, rrad_err;
+  std::string rel_err, lp_err, rrad_err;
   bool rel_dirty = true, lp_dirty = true;
   double rel_gmin_all = 0.0, rel_gmax_all = 1e30;  // g range reachable by any window
   DevBuf t_a, t_mu0, t_r, t_gmin, t_gmax, t_trff1, t_trff2, t_cosne1, t_cosne2;
   DevBuf l_a, l_h, l_rad, l_int, l_del, l_inc;
-  // xillver table (loaded and uploaded on the first relxill request)
-  XillTableHost xill;
-  bool xill_loaded = false, xill_dirty = true;
-  std::vector<double> xill_ener;
-  DevBuf x_spec, x_ener, x_ect, x_q, x_incl;
+  // xillver tables, one per primary continuum (PRIM_*: xillver-a-Ec5.fits for
+  // the cut-off power law, xillverCp_v3.4.fits for nthcomp, src/common.h:163-166;
+  // loaded and uploaded on the first relxill request that needs one)
+  XillSlot xill[N_PRIM];
   // returning-radiation table (loaded and uploaded on the first request with it)
   RradTableHost rrad;
   bool rrad_loaded = false, rrad_dirty = true;
@@ -161,14 +178,14 @@ int ensure_device(std::string &err);  // device + uploads (tables, conv grid, re
 // rx_nthcomp: n_spec nthcomp spectra on one caller grid (host arrays)
 int nthcomp_spectra(const double *ener, int n_flux, int n_spec, const double *kte, const double *gam,
                     const double *ener_shift, double *flux, std::string &err);
-void ensure_xilltable_host();
+void ensure_xilltable_host(int prim);
 void ensure_rradtable_host();
 int ensure_rradtable_device(std::string &err);
 // the returning-radiation table zones of an LP request (select_spinIndexForTable
 // and allocate_radial_grid, src/Relreturn_Table.cpp:343-404); a status != 0 if
 // the table does not cover the disk
 int rrad_zones(const RradTableHost &t, ReqParams &p, int nr_fine);
-int ensure_xilltable_device(std::string &err);
+int ensure_xilltable_device(int prim, std::string &err);
 void fill_dev_views(BatchDev &bd);
 void set_last(int status, const std::string &msg);
 // the same with a static message (no allocation)

@@ -24,6 +24,7 @@
 
 #include "rx_engine.h"
 #include "rx_math.h"
+#include "rx_nthcomp.h"
 
 namespace rx {
 
@@ -54,6 +55,15 @@ __device__ inline double cutoffpl_bin(double e0, double e1, double ln_en, double
 
 // 1e15 / (4 pi): the xillver tables' incident flux normalisation (src/Xillspec.cpp:203)
 constexpr double NORM_XILLVER_TABLE = 1e15 / 4.0 / 3.14159265358979323846;
+
+// A xillver table value as the reference holds it after renorm_xill_spec
+// (src/xilltable.c:478-486): the resident rows are divided by 10^logXi; rows
+// of a table without a density axis are also divided by dens_div =
+// 10^(logN - 15) (rounded to float, as the reference's float array), unless
+// logN is 15 (dens_div 0).
+__device__ __forceinline__ double xval(float v, double dens_div) {
+  return dens_div != 0.0 ? (double)(float)((double)v / dens_div) : (double)v;
+}
 
 // This thread's part of calcNormWrtXillverTableSpec's sum (src/Xillspec.cpp:179-205)
 // for the cut-off power law on the 500-bin 0.1-1000 keV grid ce.
@@ -814,7 +824,7 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
   const bool relxill_model = (rq.model == MOD_RELXILL || rq.model == MOD_RELXILLLP);
   if (relxill_model && s_st == 0) {
     ZoneRec *zr = b.zone + rq.zone_off;
-    const DevXillTable &xt = b.xt;
+    const DevXillTable &xt = b.xt[rq.prim];
     __shared__ double s_zect[MAX_ZONES];
     for (int z = tid; z < nz; z += SETUP_THREADS) {
       double ect;
@@ -822,7 +832,11 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
       s_zect[z] = ect;
     }
     __syncthreads();
-    if (lp) {
+    if (lp && rq.prim == PRIM_NTHCOMP) {
+      // the nthcomp norms of the source and of every zone's kTe (k_nth)
+      const NthRes *nr = b.nth + rq.nth_off;
+      for (int z = tid; z < nz; z += SETUP_THREADS) zr[z].ncf = nr[1 + z].nf / nr[0].nf;
+    } else if (lp) {
       const double src = 1. / (block_sum(xill_norm_part(b.cg, rq.xect_src, rq.xgam, tid, SETUP_THREADS),
                                          s_red) / NORM_XILLVER_TABLE);
       for (int z = 0; z < nz; z++) {
@@ -989,7 +1003,8 @@ __global__ __launch_bounds__(RRC_THREADS) void k_rrad_corr(BatchDev b, const Xil
   const int req = wk.req, z = wk.zone, tid = threadIdx.x;
   const ReqParams &rq = b.req[req];
   if (rq.pre_status != 0) return;  // uniform over the block
-  const DevXillTable xt = b.xt;
+  const DevXillTable xt = b.xt[rq.prim];
+  const double dd = rq.xdens_div;
   lp_rows(b.lp, rq, s_jr, s_je, s_jd, s_ji, tid, RRC_THREADS);
   double ect;
   const ZoneRec zr = zone_record(xt, rq, z, true, s_jr, s_je, s_jd, s_ji, b.lp.n_r, b.nr_fine, &ect);
@@ -1029,7 +1044,7 @@ __global__ __launch_bounds__(RRC_THREADS) void k_rrad_corr(BatchDev b, const Xil
       const double wa = 0.5 * cos(incl_deg * PI / 180) / n_incl;
       double sv = 0.0;
 #pragma unroll
-      for (int c = 0; c < 16; c++) sv += w[c] * (double)xt.spec[(size_t)(row[c] + a) * ne + e];
+      for (int c = 0; c < 16; c++) sv += w[c] * xval(xt.spec[(size_t)(row[c] + a) * ne + e], dd);
       acc += wa * sv;
     }
     s_avg[e] = acc;
@@ -1050,19 +1065,89 @@ __global__ __launch_bounds__(RRC_THREADS) void k_rrad_corr(BatchDev b, const Xil
       pz += rebin_bin(lo, hi, count_le(ge, ne, lo, gx), count_le(ge, ne, hi, gx), ge, ne, fav) * gref;
     }
   }
-  // the normalised cut-off power law of the zone (Ecut it sees, the source's
-  // Gamma) on the 500-bin grid (calc_normalized_primary_spectrum, :302-332)
-  const double nf = 1. / (block_sum(xill_norm_part(b.cg, ect, rq.xgam, tid, RRC_THREADS), s_red) / NORM_XILLVER_TABLE);
-  double pe = 0.0;
-  for (int k = tid; k < N_ENER_COARSE; k += RRC_THREADS) {
-    const double e0 = b.cg.coarse[k], e1 = b.cg.coarse[k + 1];
-    if (e0 >= 0.1 && e1 <= 1000.0) pe += cutoffpl_bin(e0, e1, b.cg.coarse_ln[k], ect, rq.xgam) * nf * 0.5 * (e0 + e1);
-  }
   const double s_ef = block_sum(ef, s_red), s_pf = block_sum(pf, s_red), s_pz = block_sum(pz, s_red);
-  const double s_pe = block_sum(pe, s_red);
+  // the normalised primary of the zone (Ecut / kTe it sees, the source's
+  // Gamma) on the 500-bin grid (calc_normalized_primary_spectrum, :302-332):
+  // the cut-off power law here, nthcomp from its solve (k_nth)
+  double s_pe;
+  if (rq.prim == PRIM_NTHCOMP) {
+    s_pe = b.nth[rq.nth_off + 1 + z].pe;
+  } else {
+    const double nf =
+        1. / (block_sum(xill_norm_part(b.cg, ect, rq.xgam, tid, RRC_THREADS), s_red) / NORM_XILLVER_TABLE);
+    double pe = 0.0;
+    for (int k = tid; k < N_ENER_COARSE; k += RRC_THREADS) {
+      const double e0 = b.cg.coarse[k], e1 = b.cg.coarse[k + 1];
+      if (e0 >= 0.1 && e1 <= 1000.0)
+        pe += cutoffpl_bin(e0, e1, b.cg.coarse_ln[k], ect, rq.xgam) * nf * 0.5 * (e0 + e1);
+    }
+    s_pe = block_sum(pe, s_red);
+  }
   if (tid == 0) {
     b.rrcorr[rq.rr_corr_off + 2 * z] = s_ef / s_pe;
     b.rrcorr[rq.rr_corr_off + 2 * z + 1] = s_pf / s_pz / pow(1.5, rq.xgam);
+  }
+}
+
+// relxill models with the nthcomp primary (relxillCp, relxilllpCp): one
+// Kompaneets solve per (request, j) of work, j = 0 for the source's kTe (the
+// nthcomp models keep kTe in the source frame, src/Relxill.cpp:197-200),
+// j = 1 + z for the kTe lamp-post zone z sees (kTe times the zone's source ->
+// disk energy shift, src/IonGradient.cpp:196-209, from the zone record as
+// k_setup computes it).  Each solve writes its normalisation nf on the
+// 500-bin grid and the energy flux of the normalised spectrum in 0.1-1000 keV
+// (calc_xillver_normalization_change_source_to_disk, src/Xillspec.cpp:408-435;
+// get_xillver_fluxcorr, :344-362); the source's also keeps its solution for
+// k_primary.  First in a run: k_rrad_corr and k_setup read the results.
+constexpr int NTH_THREADS_R = 256;
+
+__global__ __launch_bounds__(NTH_THREADS_R) void k_nth(BatchDev b, const XillWork *__restrict__ work) {
+  __shared__ NthLds L;
+  __shared__ double s_red[NTH_THREADS_R];
+  __shared__ double s_kte;
+  const XillWork wk = work[blockIdx.x];
+  const int req = wk.req, j = wk.zone, tid = threadIdx.x;
+  const ReqParams &rq = b.req[req];
+  if (rq.pre_status != 0) return;  // uniform over the block
+  if (j == 0) {
+    if (tid == 0) s_kte = rq.xect_src;
+  } else {
+    // the lamp-post rows in the solve's arrays (the solve overwrites them)
+    lp_rows(b.lp, rq, L.a, L.b, L.c, L.d, tid, NTH_THREADS_R);
+    if (tid == 0) {
+      double ect;
+      (void)zone_record(b.xt[rq.prim], rq, j - 1, true, L.a, L.b, L.c, L.d, b.lp.n_r, b.nr_fine, &ect);
+      s_kte = ect;
+    }
+  }
+  __syncthreads();
+  const double kte = s_kte;
+  const int nth = nth_solve(L, kte, rq.xgam, 0.05, tid, NTH_THREADS_R);  // kT_bb 0.05 keV (src/relutility.c:625-632)
+  // the spectrum on the 500-bin grid (spec_nthcomp at z = 0), per bin in a
+  const double nfac = nthcomp_normfac(L.x, L.c, nth, 0.0);
+  const double *ce = b.cg.coarse;
+  double sp = 0.0;
+  for (int k = tid; k < N_ENER_COARSE; k += NTH_THREADS_R) {
+    const double e0 = ce[k], e1 = ce[k + 1];
+    const double f = nthcomp_bin(e0, e1, nthcomp_edge(L.x, L.c, nth, e0, 0.0), nthcomp_edge(L.x, L.c, nth, e1, 0.0),
+                                 nfac);
+    L.a[k] = f;
+    if (e0 >= 0.1 && e0 <= 1000.0) sp += f * 0.5 * (e0 + e1) * 1e20 * 1.602177e-09;
+  }
+  const double nf = 1. / (block_sum(sp, s_red) / NORM_XILLVER_TABLE);
+  double pe = 0.0;
+  for (int k = tid; k < N_ENER_COARSE; k += NTH_THREADS_R) {
+    const double e0 = ce[k], e1 = ce[k + 1];
+    if (e0 >= 0.1 && e1 <= 1000.0) pe += L.a[k] * nf * 0.5 * (e0 + e1);
+  }
+  const double s_pe = block_sum(pe, s_red);
+  if (tid == 0) b.nth[rq.nth_off + j] = NthRes{nf, s_pe};
+  if (j == 0) {
+    double *xo = b.nth_x + (size_t)rq.nth_sol * (NTH_N + 1);
+    double *so = b.nth_s + (size_t)rq.nth_sol * NTH_N;
+    for (int i = tid; i <= nth; i += NTH_THREADS_R) xo[i] = L.x[i];
+    for (int i = tid; i < NTH_N; i += NTH_THREADS_R) so[i] = L.c[i];
+    if (tid == 0) b.nth_n[rq.nth_sol] = nth;
   }
 }
 
@@ -1766,8 +1851,8 @@ __global__ __launch_bounds__(CONV_THREADS_MAX) void k_conv(BatchDev b, const Xil
   const bool relxill = (rq.model == MOD_RELXILL || rq.model == MOD_RELXILLLP);
   // input: the caller's spectrum, or for relxill each zone's angle-weighted
   // xillver spectrum on the table grid (k_xill)
-  const double *ein = relxill ? b.xt.ener : eo;
-  const int n_in = relxill ? b.xt.n_ener : n_o;
+  const double *ein = relxill ? b.xt[rq.prim].ener : eo;
+  const int n_in = relxill ? b.xt[rq.prim].n_ener : n_o;
   const DevReflSet *rs = angular ? &b.refl_sets[rq.refl_set] : nullptr;
   auto ge_in = [&](int k) { return ein[k]; };
   const BinRange rc = b.reach[req];
@@ -2037,7 +2122,18 @@ __global__ __launch_bounds__(PRIM_THREADS) void k_primary(BatchDev b, const int 
       prim_scale *= pow(dop, 2);
     }
   }
-  const double nf = 1. / (block_sum4(xill_norm_part(cg, ect, gam, tid, NT), s_w) / NORM_XILLVER_TABLE);
+  // nthcomp: the source's solve (k_nth) observed at redshift 1/g - 1
+  // (spec_nthcomp, src/Xillspec.cpp:243-253) with the norm of its spectrum
+  // at the source
+  const bool nth = rq.prim == PRIM_NTHCOMP;
+  const int sol = nth ? rq.nth_sol : 0;
+  const double *nx = b.nth_x + (size_t)sol * (NTH_N + 1);
+  const double *ns = b.nth_s + (size_t)sol * NTH_N;
+  const int nn = nth ? b.nth_n[sol] : 0;
+  const double zo = 1 / rq.g_so - 1;
+  const double nfac = nth ? nthcomp_normfac(nx, ns, nn, zo) : 0.0;
+  const double nf = nth ? b.nth[rq.nth_off].nf
+                        : 1. / (block_sum4(xill_norm_part(cg, ect, gam, tid, NT), s_w) / NORM_XILLVER_TABLE);
   // the primary on the convolution grid, evaluated where the rebin reads it
   // (one or two grid bins per output bin on caller grids finer than the
   // convolution grid; every grid bin of the output bin, each with its
@@ -2045,7 +2141,12 @@ __global__ __launch_bounds__(PRIM_THREADS) void k_primary(BatchDev b, const int 
   const double *ce = cg.ener;
   const double *cl = cg.ln_emid;
   const double ect_o = ect * rq.g_so;
-  auto prim_at = [&](int i) { return (cutoffpl_bin(ce[i], ce[i + 1], cl[i], ect_o, gam) * nf) * prim_scale; };
+  auto prim_at = [&](int i) {
+    const double v = nth ? nthcomp_bin(ce[i], ce[i + 1], nthcomp_edge(nx, ns, nn, ce[i], zo),
+                                       nthcomp_edge(nx, ns, nn, ce[i + 1], zo), nfac)
+                         : cutoffpl_bin(ce[i], ce[i + 1], cl[i], ect_o, gam);
+    return (v * nf) * prim_scale;
+  };
   const GridGuess gc = grid_guess(ce, n);
   auto ge_c = [&](int k) { return ce[k]; };
   const double *eo = b.ener_in + rq.ener_off;
@@ -2076,12 +2177,13 @@ __global__ __launch_bounds__(XILL_THREADS, 4) void k_xill1(BatchDev b, const Xil
   const XillWork wk = work[blockIdx.x / chunks];
   const int req = wk.req;
   if (b.status[req] != 0) return;
-  const DevXillTable xt = b.xt;
+  const ReqParams &rq = b.req[req];
+  const DevXillTable xt = b.xt[rq.prim];
+  const double dd = rq.xdens_div;
   // XILL_BINS bins per thread, XILL_THREADS apart: the corner weights and rows
   // are set up once for all of them, and their loads are in flight together
   const int e0 = (blockIdx.x % chunks) * (XILL_THREADS * XILL_BINS) + threadIdx.x;
   if (e0 >= xt.n_ener) return;
-  const ReqParams &rq = b.req[req];
   const ZoneRec zr = b.zone[rq.zone_off + wk.zone];
   int i0[4];
   double f[4];
@@ -2133,7 +2235,7 @@ __global__ __launch_bounds__(XILL_THREADS, 4) void k_xill1(BatchDev b, const Xil
     for (int c = 0; c < 16; c++) {
       const float *src = xt.spec + (size_t)(row[c] + a) * ne;
 #pragma unroll
-      for (int u = 0; u < XILL_BINS; u++) s[u] += w[c] * (double)src[eb[u]];
+      for (int u = 0; u < XILL_BINS; u++) s[u] += w[c] * xval(src[eb[u]], dd);
     }
 #pragma unroll
     for (int u = 0; u < XILL_BINS; u++) acc[u] += d * s[u];
@@ -2170,8 +2272,9 @@ __global__ __launch_bounds__(XILL_THREADS, 3) void k_xillz(BatchDev b, const Xil
   const XillWork wk = work[blockIdx.x / chunks];
   const int req = wk.req;
   if (b.status[req] != 0) return;  // uniform over the block
-  const DevXillTable xt = b.xt;
   const ReqParams &rq = b.req[req];
+  const DevXillTable xt = b.xt[rq.prim];
+  const double dd = rq.xdens_div;
   const int nz = min(rq.n_zones, NZ);
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int zax = xt.ect_dim >= 0 ? xt.ect_dim : 3;
@@ -2267,7 +2370,7 @@ __global__ __launch_bounds__(XILL_THREADS, 3) void k_xillz(BatchDev b, const Xil
         for (int c = 0; c < 8; c++) {
           const float *src = xt.spec + (size_t)(cb[c] + po) * ne;
 #pragma unroll
-          for (int u = 0; u < XILL_BINS_Z; u++) s8[u] += w8[c] * (double)src[eb[u]];
+          for (int u = 0; u < XILL_BINS_Z; u++) s8[u] += w8[c] * xval(src[eb[u]], dd);
         }
 #pragma unroll
         for (int z = 0; z < NZ; z++) {
@@ -2416,14 +2519,18 @@ int launch_conv(const BatchDev &b, const XillWork *conv_work, int n_conv_work, v
   return (int)hipGetLastError();
 }
 
+// the larger energy grid of the two xillver tables (units beyond a request's
+// own table return at once)
+static int xill_n_ener(const BatchDev &b) { return std::max(b.xt[0].n_ener, b.xt[1].n_ener); }
+
 int launch_xill(const BatchDev &b, const XillWork *work, int n_single, int n_multi, void *stream) {
   if (n_single > 0) {
-    const int chunks = (b.xt.n_ener + XILL_THREADS * XILL_BINS - 1) / (XILL_THREADS * XILL_BINS);
+    const int chunks = (xill_n_ener(b) + XILL_THREADS * XILL_BINS - 1) / (XILL_THREADS * XILL_BINS);
     hipLaunchKernelGGL(k_xill1, dim3(n_single * chunks), dim3(XILL_THREADS), 0, (hipStream_t)stream, b,
                        work, chunks);
   }
   if (n_multi > 0) {
-    const int chunks = (b.xt.n_ener + XILL_THREADS * XILL_BINS_Z - 1) / (XILL_THREADS * XILL_BINS_Z);
+    const int chunks = (xill_n_ener(b) + XILL_THREADS * XILL_BINS_Z - 1) / (XILL_THREADS * XILL_BINS_Z);
     hipLaunchKernelGGL(k_xillz<MAX_ZONES>, dim3(n_multi * chunks), dim3(XILL_THREADS), 0,
                        (hipStream_t)stream, b, work + n_single, chunks);
   }
@@ -2432,9 +2539,15 @@ int launch_xill(const BatchDev &b, const XillWork *work, int n_single, int n_mul
 
 int launch_rrad_corr(const BatchDev &b, const XillWork *work, int n_work, void *stream) {
   if (n_work <= 0) return 0;
-  const size_t lds = sizeof(double) * (size_t)b.xt.n_ener;
+  const size_t lds = sizeof(double) * (size_t)xill_n_ener(b);
   (void)hipFuncSetAttribute((const void *)k_rrad_corr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL(k_rrad_corr, dim3(n_work), dim3(RRC_THREADS), lds, (hipStream_t)stream, b, work);
+  return (int)hipGetLastError();
+}
+
+int launch_nth(const BatchDev &b, const XillWork *work, int n_work, void *stream) {
+  if (n_work <= 0) return 0;
+  hipLaunchKernelGGL(k_nth, dim3(n_work), dim3(NTH_THREADS_R), 0, (hipStream_t)stream, b, work);
   return (int)hipGetLastError();
 }
 

@@ -41,6 +41,10 @@ enum ModelKind : int {
   MOD_RELXILL = -1,          // relxill: broken power law, one zone, xillver table
   MOD_RELXILLLP = -2,        // relxilllp: lamp post, MAX_ZONES radial zones
   MOD_RELXILL_CONV = -1001,  // relxill-style conv of a resident reflection set
+  // the nthcomp-primary models (relxillCp, relxilllpCp); their requests run as
+  // MOD_RELXILL / MOD_RELXILLLP with ReqParams.prim = PRIM_NTHCOMP
+  MOD_RELXILLCP = -1010,
+  MOD_RELXILLLPCP = -1011,
 };
 
 enum EmisKind : int { EMIS_BKN = 1, EMIS_LP = 2 };
@@ -82,6 +86,15 @@ struct ReqParams {
   // the request's (a, mu0)-interpolated node rows in node (n_incl > 0 only:
   // the angle distribution reads them; -1 otherwise)
   int64_t node_off;
+  // primary continuum of the relxill models (PRIM_*; it also selects the
+  // xillver table, b.xt[prim]).  nthcomp: n_nth solves (the source's kTe, then
+  // the kTe every lamp-post zone sees) with their results at nth[nth_off],
+  // the source's solution in slot nth_sol.  xdens_div: the density division
+  // of the table rows (renorm_xill_spec, src/xilltable.c:478-486) for tables
+  // without a density axis, 0 when logN is 15.
+  int prim, n_nth, nth_sol, pad_prim;
+  int64_t nth_off;
+  double xdens_div;
   // returning radiation (lamp post): 1 add, -1 or 2 replace the direct
   // emissivity; the table spin, its first zone and the number of zones used
   int return_rad, rr_k, rr_klo, rr_n;
@@ -166,6 +179,18 @@ struct DevReflSet {
 
 // Resident xillver table: the spectra divided by 10^logXi at load time,
 // row ((((i0 n1 + i1) n2 + i2) n3 + i3) n_incl + ia) at spec[row * n_ener].
+// primary continua of the relxill models (PRIM_SPEC_*, src/common.h:57-59)
+enum PrimType : int { PRIM_CUTOFFPL = 0, PRIM_NTHCOMP = 1 };
+constexpr int N_PRIM = 2;
+
+// Normalisation results of one nthcomp solve: nf = 1 / calcNormWrtXillverTableSpec
+// of the spectrum on the 500-bin 0.1-1000 keV grid (src/Xillspec.cpp:179-205)
+// and pe, the energy flux of the normalised spectrum in 0.1-1000 keV
+// (get_energy_flux_band, :132-144).
+struct NthRes {
+  double nf, pe;
+};
+
 struct DevXillTable {
   int n_ener, n_incl;
   int n_val[4];         // values of the four interpolated parameters
@@ -279,11 +304,16 @@ struct BatchDev {
   DevLpTable lp;
   DevConvGrid cg;
   const DevReflSet *refl_sets;
-  DevXillTable xt;
+  DevXillTable xt[N_PRIM];  // one xillver table per primary continuum
   double *xspec;
   ZoneRec *zone;
   DevRradTable rr;
   double *rrcorr;     // relxilllp returning radiation: 2 factors per zone
+  // nthcomp primary: per solve its normalisation results, per request the
+  // source's solution (x: NTH_MAX_POINTS + 1, E F_E: NTH_MAX_POINTS, jmax)
+  NthRes *nth;
+  double *nth_x, *nth_s;
+  int *nth_n;
 };
 
 // ---- launchers (kernels.hip) ----
@@ -311,6 +341,9 @@ struct NthJob {
 // one solve per job: x (NTH_MAX_POINTS + 1 per job, m_e c^2), E F_E
 // (NTH_MAX_POINTS per job), the number of points
 int launch_nthcomp(const NthJob *jobs, int n_jobs, double *sol_x, double *sol_s, int *sol_n, void *stream);
+// relxill models with the nthcomp primary: one solve per (request, j) of
+// work (j = 0 the source, j = 1 + z zone z of a lamp post), first in a run
+int launch_nth(const BatchDev &b, const XillWork *work, int n_work, void *stream);
 int launch_refl_prepare(const DevConvGrid &cg, const double *refl_ener, const double *refl,
                         int n_refl, int n_incl, double *spec, double *bandsum, double *fft,
                         void *stream);

@@ -106,10 +106,12 @@ std::string validate_rradtable(const RradTableHost &t);
 // FRAC01, FRAC02, ... (one per spin) with rlo, rhi, tf_r, gmin, gmax, frac_g, f_ret
 std::string load_rradtable_fits(RradTableHost &t, const std::string &path);
 
-// Seeded synthetic table with the parameters Gamma, A_Fe, logXi, Ecut, Incl
-// (in that order) and n_ener log bins over 0.07-1000 keV.
+// Seeded synthetic table with the parameters Gamma, A_Fe, logXi, Ecut (or
+// kTe: the same parameter id), Incl (in that order) and n_ener log bins over
+// 0.07-1000 keV; the fourth axis is log-spaced over [e4_lo, e4_hi] (Ecut
+// 5-1000 keV; kTe 1-400 keV in the nthcomp tables).
 XillTableHost make_synthetic_xilltable(uint64_t seed, int n_gam, int n_afe, int n_lxi, int n_ect,
-                                       int n_incl, int n_ener);
+                                       int n_incl, int n_ener, double e4_lo = 5.0, double e4_hi = 1000.0);
 std::string validate_xilltable(const XillTableHost &t);
 // The spectra divided by 10^logXi of their row and, for a table with a
 // density axis, by 10^(logN - 15) (renorm_xill_spec, src/xilltable.c:478-511,

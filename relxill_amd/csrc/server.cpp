@@ -91,9 +91,10 @@ struct Slot {
 
 int64_t request_bytes(const Pending &p) {
   Global &g = G();
-  const bool relxill = (p.model == MOD_RELXILL || p.model == MOD_RELXILLLP);
+  const bool lpx = (p.model == MOD_RELXILLLP || p.model == MOD_RELXILLLPCP);
+  const bool relxill = (p.model == MOD_RELXILL || p.model == MOD_RELXILLCP || lpx);
   const bool conv = (p.model == MOD_RELCONV || p.model == MOD_RELCONVLP || p.model == MOD_RELXILL_CONV || relxill);
-  const int64_t nz = (p.model == MOD_RELXILLLP) ? MAX_ZONES : 1;
+  const int64_t nz = lpx ? MAX_ZONES : 1;
   const int64_t n_in = (int64_t)p.energy.size() - 1;
   const int64_t n_e = conv ? g.conv_n : n_in;
   const int64_t n_rc = (g.nr_fine + RAD_CHUNK - 1) / RAD_CHUNK;
@@ -104,7 +105,12 @@ int64_t request_bytes(const Pending &p) {
   b += (int64_t)sizeof(RadRec) * g.nr_fine;
   b += 8 * 2 * (int64_t)g.rel.n_r * g.rel.n_g * 2;
   b += 8 * MAX_INCL * nz + sizeof(LpRefl) + sizeof(ReqNorm) + 64;
-  if (relxill) b += (8 * (int64_t)g.xill.n_ener + (int64_t)(2 * sizeof(XillWork) + sizeof(ZoneRec)) + 16) * nz;
+  if (relxill) {
+    const int64_t ne = std::max(g.xill[0].t.n_ener, g.xill[1].t.n_ener);
+    b += (8 * ne + (int64_t)(2 * sizeof(XillWork) + sizeof(ZoneRec)) + 16) * nz;
+  }
+  if (p.model == MOD_RELXILLCP || p.model == MOD_RELXILLLPCP)  // nthcomp solves and the source's solution
+    b += (int64_t)(sizeof(XillWork) + sizeof(NthRes)) * (nz + 1) + 8 * (2 * NTH_MAX_POINTS + 1) + 4;
   return b;
 }
 

@@ -279,7 +279,7 @@ std::vector<float> axis(int n, double lo, double hi, bool log_spaced) {
 // Inclinations are the centres of n_incl equal bins in cos(i), as the angle
 // bins of the relativistic profile (src/Relprofile.cpp:799-801).
 XillTableHost make_synthetic_xilltable(uint64_t seed, int n_gam, int n_afe, int n_lxi, int n_ect,
-                                       int n_incl, int n_ener) {
+                                       int n_incl, int n_ener, double e4_lo, double e4_hi) {
   XillTableHost t;
   t.n_par = XILL_NPAR;
   const int ids[XILL_NPAR] = {XP_GAM, XP_AFE, XP_LXI, XP_ECT, XP_INC};
@@ -291,7 +291,7 @@ XillTableHost make_synthetic_xilltable(uint64_t seed, int n_gam, int n_afe, int 
   t.val[0] = axis(n_gam, 1.0, 3.4, false);
   t.val[1] = axis(n_afe, 0.5, 10.0, true);
   t.val[2] = axis(n_lxi, 0.0, 4.7, false);
-  t.val[3] = axis(n_ect, 5.0, 1000.0, true);
+  t.val[3] = axis(n_ect, e4_lo, e4_hi, true);
   t.val[4].resize(n_incl);
   for (int i = 0; i < n_incl; i++) t.val[4][i] = (float)(std::acos(1.0 - (i + 0.5) / n_incl) * 180.0 / kPi);
   t.n_ener = n_ener;
