@@ -51,6 +51,9 @@ extern "C" {
  * PRIM_SPEC_NTHCOMP, src/ModelDefinition.cpp:38-50, :88) */
 #define RX_MODEL_RELXILLCP (-1010)
 #define RX_MODEL_RELXILLLPCP (-1011)
+/* project id: the blackbody-primary relxill model (relxillNS; the reference's
+ * MOD_TYPE_RELXILL with PRIM_SPEC_BB, src/ModelDefinition.cpp:89) */
+#define RX_MODEL_RELXILLNS (-1012)
 
 /* ---- precision of the batched path ---- */
 #define RX_FP64 0 /* plain form: FP64 throughout */
@@ -130,6 +133,15 @@ void lmodrelxilldensnthcomp(const double *energy, int Nflux, const double *param
  * RX_ERR_UNSUPPORTED). */
 void lmodrelxilllpdensnthcomp(const double *energy, int Nflux, const double *parameter, int spectrum,
                               double *flux, double *fluxError, const char *init);
+
+/* relxillNS: Index1 Index2 Rbr a Incl Rin Rout z kTbb logxi Afe logN
+ * refl_frac (lmodel_relxill_public.dat:140-153; the reference's
+ * c_lmodrelxillns): relxill (broken power law, one zone) with the blackbody
+ * primary E^2 / (kT^4 (exp(E/kT) - 1)) (spec_blackbody,
+ * src/Xillspec.cpp:255-268), normalised as the xillver incident spectrum, and
+ * the blackbody xillver table interpolated in kTbb, A_Fe, logXi, logN. */
+void lmodrelxillns(const double *energy, int Nflux, const double *parameter, int spectrum,
+                   double *flux, double *fluxError, const char *init);
 
 /* Status-returning evaluation of one model (the body of xspec_C_wrapper_eval_model,
  * src/LocalModel.cpp:143-160, and LocalModel::eval_model, src/LocalModel.h:108-128).
@@ -235,6 +247,18 @@ int rx_xilltable_get_view(rx_xilltable_view *v);
 int rx_xillcptable_synthetic(uint64_t seed, int n_gam, int n_afe, int n_lxi, int n_kte, int n_incl, int n_ener);
 int rx_xillcptable_load_fits(const char *path);
 int rx_xillcptable_get_view(rx_xilltable_view *v);
+
+/* The blackbody model's xillver table (parameters kTbb, A_Fe, logXi, Dens,
+ * Incl by name; the reference's xillverNS-2.fits, src/common.h:167,
+ * get_init_xillver_table, src/xilltable.c:742-748), held beside the others.
+ * Loaded from RELXILL_TABLE_PATH/xillverNS-2.fits on the first relxillNS
+ * request when present, else a synthetic table (12 kTbb 0.5-10 keV x 5 A_Fe
+ * x 8 logXi 1-4.7 x 5 Dens 15-19 x 10 inclinations, 2999 log bins). Rows are
+ * divided by 10^logXi and 10^(Dens - 15) of their node when uploaded.
+ * Five-parameter tables only. */
+int rx_xillnstable_synthetic(uint64_t seed, int n_ktbb, int n_afe, int n_lxi, int n_dens, int n_incl, int n_ener);
+int rx_xillnstable_load_fits(const char *path);
+int rx_xillnstable_get_view(rx_xilltable_view *v);
 
 /* Returning-radiation table of the lamp-post models (switch_returnrad = 1;
  * the reference's table_returnRad_v20220301.fits, src/common.h:126-128):

@@ -24,6 +24,7 @@ MOD_RELXILLLP = -2
 MOD_RELXILL_CONV = -1001
 MOD_RELXILLCP = -1010
 MOD_RELXILLLPCP = -1011
+MOD_RELXILLNS = -1012
 
 _D = ctypes.POINTER(ctypes.c_double)
 _F = ctypes.POINTER(ctypes.c_float)
@@ -77,7 +78,7 @@ class XillTable(ctypes.Structure):
 class XillParam(ctypes.Structure):
     _fields_ = [("gam", ctypes.c_double), ("afe", ctypes.c_double), ("lxi", ctypes.c_double),
                 ("ect", ctypes.c_double), ("dens", ctypes.c_double), ("incl", ctypes.c_double),
-                ("refl_frac", ctypes.c_double), ("boost", _I), ("prim", _I)]
+                ("refl_frac", ctypes.c_double), ("boost", _I), ("prim", _I), ("ktbb", ctypes.c_double)]
 
 
 class Job(ctypes.Structure):
@@ -134,6 +135,7 @@ def _load():
         "orc_norm_wrt_xillver": (ctypes.c_double, [_D, _D, _I, ctypes.POINTER(_I)]),
         "orc_primary_norm_factor": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
         "orc_prim_norm_factor": (ctypes.c_double, [ctypes.POINTER(XillParam), ctypes.c_double]),
+        "orc_spec_blackbody": (None, [_D, _D, _I, ctypes.c_double]),
         "orc_prim_spectrum": (None, [_D, _D, _I, ctypes.POINTER(XillParam), ctypes.c_double, ctypes.c_double]),
         "orc_mcd_shape": (ctypes.c_double, [ctypes.c_double]),
         "orc_nthcomp_solve": (_I, [ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _D]),
@@ -392,15 +394,23 @@ class Xill:
         return self.arrays["n_val"][-1]
 
 
-def xill_param(gam, afe, lxi, ect, dens=15.0, incl=30.0, refl_frac=0.0, prim=0):
-    """xillver parameters; prim 1: the nthcomp primary (ect is kTe)"""
-    return XillParam(gam, afe, lxi, ect, dens, incl, refl_frac, 0, prim)
+def xill_param(gam, afe, lxi, ect, dens=15.0, incl=30.0, refl_frac=0.0, prim=0, ktbb=0.0):
+    """xillver parameters; prim 1: the nthcomp primary (ect is kTe), 2: the
+    blackbody (ktbb)"""
+    return XillParam(gam, afe, lxi, ect, dens, incl, refl_frac, 0, prim, ktbb)
 
 
-def prim_norm_factor(gam, ect, prim=0):
-    """1 / calcNormWrtXillverTableSpec of the primary (cut-off power law, or
-    nthcomp with prim=1) on the 500-bin grid"""
-    return lib.orc_prim_norm_factor(ctypes.byref(xill_param(gam, 1.0, 0.0, ect, prim=prim)), ect)
+def prim_norm_factor(gam, ect, prim=0, ktbb=0.0):
+    """1 / calcNormWrtXillverTableSpec of the primary (cut-off power law,
+    nthcomp with prim=1, the blackbody of ktbb with prim=2) on the 500-bin grid"""
+    return lib.orc_prim_norm_factor(ctypes.byref(xill_param(gam, 1.0, 0.0, ect, prim=prim, ktbb=ktbb)), ect)
+
+
+def blackbody(ener, ktbb):
+    e = _arr(ener)
+    out = np.zeros(e.size - 1)
+    lib.orc_spec_blackbody(_dp(out), _dp(e), e.size - 1, ktbb)
+    return out
 
 
 def xill_corner(xt, xp):

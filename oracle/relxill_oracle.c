@@ -311,6 +311,7 @@ int orc_n_params(int model_type) {
     case ORC_MOD_RELXILL_CONV: return 8;  /* relxill-style conv: relconv's parameters */
     case ORC_MOD_RELXILLCP: return 14;
     case ORC_MOD_RELXILLLPCP: return 17;
+    case ORC_MOD_RELXILLNS: return 13;
     default: return -1;
   }
 }
@@ -1278,7 +1279,7 @@ static const double EMAX_XILL_NORM = 1000.0;  /* src/Xillspec.h:29 */
 static const int N_ENER_COARSE = 500;         /* src/Xillspec.h:32 */
 
 /* the input value of each table parameter, as float (get_xilltab_paramvals,
- * src/xilltable.c:278-295; kTbb and Frac are 0 for relxill) */
+ * src/xilltable.c:278-295; kTbb is 0 but for the blackbody model, Frac is 0) */
 static void xill_paramvals(const orc_xillparam *x, float *v) {
   for (int k = 0; k < 8; k++) v[k] = 0.0f;
   v[ORC_XP_GAM] = (float)x->gam;
@@ -1286,6 +1287,7 @@ static void xill_paramvals(const orc_xillparam *x, float *v) {
   v[ORC_XP_LXI] = (float)x->lxi;
   v[ORC_XP_ECT] = (float)x->ect;
   v[ORC_XP_DNS] = (float)x->dens;
+  v[ORC_XP_KTB] = (float)x->ktbb;
   v[ORC_XP_INC] = (float)x->incl;
 }
 
@@ -1506,8 +1508,19 @@ double orc_primary_norm_factor(double gam, double ect) {
   return 1. / orc_norm_wrt_xillver(flux, ener, N_ENER_COARSE, NULL);
 }
 
+void orc_spec_blackbody(double *flux, const double *ener, int n, double ktbb) {
+  for (int ii = 0; ii < n; ii++) {
+    const double en = 0.5 * (ener[ii] + ener[ii + 1]);
+    flux[ii] = en * en / (pow(ktbb, 4) * (exp(en / ktbb) - 1));
+    flux[ii] *= (ener[ii + 1] - ener[ii]);
+  }
+}
+
+/* calc_primary_spectrum (src/Xillspec.cpp:278-300): the blackbody takes no
+ * energy shift */
 void orc_prim_spectrum(double *flux, const double *ener, int n, const orc_xillparam *x, double ect, double shift) {
-  if (x->prim) orc_spec_nthcomp(flux, ener, n, ect, x->gam, shift);
+  if (x->prim == 2) orc_spec_blackbody(flux, ener, n, x->ktbb);
+  else if (x->prim == 1) orc_spec_nthcomp(flux, ener, n, ect, x->gam, shift);
   else orc_spec_cutoffpl(flux, ener, n, ect, x->gam, shift);
 }
 
@@ -1542,12 +1555,24 @@ static int cp_remap(int model_type, const double *par, double *out, double *dens
 
 int orc_get_relxill_params(int model_type, const double *par, int phys_norm_env, orc_relparam *p,
                            orc_xillparam *x) {
-  double cpar[14], dens = 15;
+  double cpar[14], dens = 15, ktbb = 0;
   int prim = 0, iongrad = 0;
   if (model_type == ORC_MOD_RELXILLCP || model_type == ORC_MOD_RELXILLLPCP) {
     model_type = cp_remap(model_type, par, cpar, &dens, &iongrad);
     par = cpar;
     prim = 1;
+  } else if (model_type == ORC_MOD_RELXILLNS) {
+    /* relxillNS: Index1 Index2 Rbr a Incl Rin Rout z kTbb logxi Afe logN
+     * refl_frac; no Gamma (0) and no Ecut (300, its default) in get_xill_params
+     * (src/ModelDefinition.cpp:357-376) */
+    for (int i = 0; i < 13; i++) cpar[i] = par[i];
+    ktbb = par[8];
+    dens = par[11];
+    cpar[8] = 0.0;
+    cpar[11] = 300.0;
+    par = cpar;
+    model_type = ORC_MOD_RELXILL;
+    prim = 2;
   }
   int status = orc_get_rel_params(model_type, par, phys_norm_env, p);
   /* get_xill_params (src/ModelDefinition.cpp:339-390): Ecut in keV, the
@@ -1567,6 +1592,7 @@ int orc_get_relxill_params(int model_type, const double *par, int phys_norm_env,
    * (get_xill_params, src/ModelDefinition.cpp:357-368) */
   x->dens = dens;
   x->prim = prim;
+  x->ktbb = ktbb;
   if (x->lxi < 0.0) x->lxi = 0.0;
   else if (x->lxi > 4.7) x->lxi = 4.7;
   /* constant ionisation only: relxilllpCp's radial ionisation gradients

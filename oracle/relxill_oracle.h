@@ -41,6 +41,8 @@ extern "C" {
  * (parameters of lmodel_relxill_public.dat:96-129) */
 #define ORC_MOD_RELXILLCP (-1010)
 #define ORC_MOD_RELXILLLPCP (-1011)
+/* relxillNS: relxill with the blackbody primary (lmodel_relxill_public.dat:140-153) */
+#define ORC_MOD_RELXILLNS (-1012)
 
 /* emissivity types (src/common.h:50-55) */
 #define ORC_EMIS_BKN 1
@@ -213,6 +215,7 @@ int orc_relxill_conv(const orc_reltable *tab, const orc_relparam *p, const doubl
 #define ORC_XP_LXI 2
 #define ORC_XP_ECT 3
 #define ORC_XP_DNS 4
+#define ORC_XP_KTB 5
 #define ORC_XP_INC 7
 
 /* a five-parameter xillver table (xillTable, src/xilltable.h), the last
@@ -234,7 +237,9 @@ typedef struct {
   double gam, afe, lxi, ect, dens, incl;  /* incl in degrees */
   double refl_frac;
   int boost;  /* switch_reflfrac_boost: refl_frac relative to the lamp post's */
-  int prim;   /* primary continuum: 0 cut-off power law (ect = Ecut), 1 nthcomp (ect = kTe) */
+  int prim;   /* primary continuum: 0 cut-off power law (ect = Ecut), 1 nthcomp (ect = kTe),
+                2 blackbody (ktbb) */
+  double ktbb; /* blackbody temperature (keV) */
 } orc_xillparam;
 
 /* get_xilltab_indices_for_paramvals (src/xilltable.c:297-324) */
@@ -280,6 +285,8 @@ void orc_prim_spectrum(double *flux, const double *ener, int n, const orc_xillpa
 /* 1 / calcNormWrtXillverTableSpec of that primary (unshifted) on the 500-bin
  * 0.1-1000 keV grid (src/Xillspec.cpp:179-205, :302-315) */
 double orc_prim_norm_factor(const orc_xillparam *x, double ect);
+/* spec_blackbody (src/Xillspec.cpp:255-268) */
+void orc_spec_blackbody(double *flux, const double *ener, int n, double ktbb);
 /* spec_nthcomp (src/Xillspec.cpp:243-253) */
 void orc_spec_nthcomp(double *flux, const double *ener, int n, double kte, double gam, double ener_shift);
 /* relxill parameters: Index1 Index2 Rbr a Incl Rin Rout z gamma logxi Afe

@@ -31,13 +31,14 @@ MODEL_RELXILLLP = -2
 MODEL_RELXILL_CONV = -1001  # project id: relxill-style conv of a resident reflection set
 MODEL_RELXILLCP = -1010  # project id: relxillCp (relxill with the nthcomp primary)
 MODEL_RELXILLLPCP = -1011  # project id: relxilllpCp
+MODEL_RELXILLNS = -1012  # project id: relxillNS (relxill with the blackbody primary)
 
 FP64 = 0
 FP32 = 1
 
 N_PARAMS = {MODEL_RELLINE: 10, MODEL_RELCONV: 8, MODEL_RELLINELP: 10, MODEL_RELCONVLP: 9,
             MODEL_RELXILL_CONV: 8, MODEL_RELXILL: 13, MODEL_RELXILLLP: 14, MODEL_RELXILLCP: 14,
-            MODEL_RELXILLLPCP: 17}
+            MODEL_RELXILLLPCP: 17, MODEL_RELXILLNS: 13}
 
 # default parameter values of the reference's lmodel.dat
 # (src/modelfiles/lmodel_relxill_public.dat:1-59)
@@ -53,6 +54,8 @@ DEFAULTS = {
     MODEL_RELXILLCP: [30.0, 0.998, -1.0, 400.0, 15.0, 3.0, 3.0, 0.0, 2.0, 3.1, 15.0, 1.0, 60.0, 3.0],
     MODEL_RELXILLLPCP: [30.0, 0.998, -1.0, 400.0, 6.0, 0.0, 2.0, 3.1, 15.0, 1.0, 60.0, 1.0, 0.0, 0.0, 0.0, 1.0,
                         0.0],
+    # lmodel_relxill_public.dat:140-153
+    MODEL_RELXILLNS: [3.0, 3.0, 15.0, 0.998, 30.0, -1.0, 400.0, 0.0, 2.0, 3.1, 1.0, 15.0, 3.0],
 }
 
 
@@ -109,6 +112,10 @@ _EXPORTS = {
     "rx_xillcptable_get_view": (ctypes.c_int, [ctypes.POINTER(_XillView)]),
     "lmodrelxilldensnthcomp": (None, [_D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_char_p]),
     "lmodrelxilllpdensnthcomp": (None, [_D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_char_p]),
+    "lmodrelxillns": (None, [_D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_char_p]),
+    "rx_xillnstable_synthetic": (ctypes.c_int, [ctypes.c_uint64] + [ctypes.c_int] * 6),
+    "rx_xillnstable_load_fits": (ctypes.c_int, [ctypes.c_char_p]),
+    "rx_xillnstable_get_view": (ctypes.c_int, [ctypes.POINTER(_XillView)]),
     "rx_rradtable_synthetic": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "rx_rradtable_load_fits": (ctypes.c_int, [ctypes.c_char_p]),
     "rx_rradtable_get_view": (ctypes.c_int, [ctypes.POINTER(_RradView)]),
@@ -272,6 +279,10 @@ def lmodrelxilllpdensnthcomp(energy, params):
     return _lmod(lib.lmodrelxilllpdensnthcomp, energy, params)
 
 
+def lmodrelxillns(energy, params):
+    return _lmod(lib.lmodrelxillns, energy, params)
+
+
 def xilltable_synthetic(seed=5, n_gam=13, n_afe=5, n_lxi=8, n_ect=6, n_incl=10, n_ener=2999):
     _check(lib.rx_xilltable_synthetic(seed, n_gam, n_afe, n_lxi, n_ect, n_incl, n_ener))
 
@@ -281,12 +292,20 @@ def xillcptable_synthetic(seed=6, n_gam=13, n_afe=5, n_lxi=8, n_kte=6, n_incl=10
     _check(lib.rx_xillcptable_synthetic(seed, n_gam, n_afe, n_lxi, n_kte, n_incl, n_ener))
 
 
-def xilltable_view(nthcomp=False):
-    """Copy of the resident xillver table (the nthcomp models' table with
-    nthcomp=True): dict with n_par, pid, n_val, val (list of arrays), n_ener,
-    elo, ehi, spec (rows x n_ener, as stored in the file)."""
+def xillnstable_synthetic(seed=7, n_ktbb=12, n_afe=5, n_lxi=8, n_dens=5, n_incl=10, n_ener=2999):
+    """Synthetic table of the blackbody model (kTbb, A_Fe, logXi, Dens, Incl)."""
+    _check(lib.rx_xillnstable_synthetic(seed, n_ktbb, n_afe, n_lxi, n_dens, n_incl, n_ener))
+
+
+def xilltable_view(nthcomp=False, blackbody=False):
+    """Copy of a resident xillver table (the nthcomp models' with
+    nthcomp=True, the blackbody model's with blackbody=True): dict with n_par,
+    pid, n_val, val (list of arrays), n_ener, elo, ehi, spec (rows x n_ener, as
+    stored in the file)."""
     v = _XillView()
-    _check((lib.rx_xillcptable_get_view if nthcomp else lib.rx_xilltable_get_view)(ctypes.byref(v)))
+    get = (lib.rx_xillnstable_get_view if blackbody else
+           lib.rx_xillcptable_get_view if nthcomp else lib.rx_xilltable_get_view)
+    _check(get(ctypes.byref(v)))
     f = lambda ptr, n: np.ctypeslib.as_array(ptr, shape=(n,)).astype(np.float32).copy()
     n_val = [v.n_val[k] for k in range(v.n_par)]
     rows = int(np.prod(n_val))

@@ -131,6 +131,11 @@ void lmodrelxilllpdensnthcomp(const double *energy, int Nflux, const double *par
   rx_eval_model(RX_MODEL_RELXILLLPCP, parameter, energy, flux, Nflux, RX_FP64);
 }
 
+void lmodrelxillns(const double *energy, int Nflux, const double *parameter, int, double *flux, double *,
+                   const char *) {
+  rx_eval_model(RX_MODEL_RELXILLNS, parameter, energy, flux, Nflux, RX_FP64);
+}
+
 int rx_last_status(void) { return last_status(); }
 const char *rx_last_error(void) { return last_error(); }
 
@@ -275,8 +280,10 @@ static int xill_synthetic(int prim, uint64_t seed, int n_gam, int n_afe, int n_l
   }
   XillSlot &x = g.xill[prim];
   try {
-    // the fourth axis: Ecut 5-1000 keV, or kTe 1-400 keV for nthcomp
+    // the fourth axis: Ecut 5-1000 keV, or kTe 1-400 keV for nthcomp; the
+    // blackbody table's axes are kTbb, A_Fe, logXi, Dens, Incl
     x.t = prim == PRIM_NTHCOMP ? make_synthetic_xilltable(seed, n_gam, n_afe, n_lxi, n_ect, n_incl, n_ener, 1.0, 400.0)
+          : prim == PRIM_BBODY ? make_synthetic_xilltable_ns(seed, n_gam, n_afe, n_lxi, n_ect, n_incl, n_ener)
                                : make_synthetic_xilltable(seed, n_gam, n_afe, n_lxi, n_ect, n_incl, n_ener);
   } catch (...) {
     set_last(RX_ERR_NOMEM, "table allocation failed");
@@ -355,6 +362,12 @@ int rx_xillcptable_synthetic(uint64_t seed, int n_gam, int n_afe, int n_lxi, int
 }
 int rx_xillcptable_load_fits(const char *path) { return xill_load_fits(PRIM_NTHCOMP, path); }
 int rx_xillcptable_get_view(rx_xilltable_view *v) { return xill_get_view(PRIM_NTHCOMP, v); }
+
+int rx_xillnstable_synthetic(uint64_t seed, int n_ktbb, int n_afe, int n_lxi, int n_dens, int n_incl, int n_ener) {
+  return xill_synthetic(PRIM_BBODY, seed, n_ktbb, n_afe, n_lxi, n_dens, n_incl, n_ener);
+}
+int rx_xillnstable_load_fits(const char *path) { return xill_load_fits(PRIM_BBODY, path); }
+int rx_xillnstable_get_view(rx_xilltable_view *v) { return xill_get_view(PRIM_BBODY, v); }
 
 int rx_rradtable_synthetic(uint64_t seed, int n_spin, int n_rad, int n_g) {
   Global &g = G();

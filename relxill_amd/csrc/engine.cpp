@@ -74,6 +74,7 @@ int n_model_params(int model) {
     case MOD_RELXILLLP: return 14;
     case MOD_RELXILLCP: return 14;
     case MOD_RELXILLLPCP: return 17;
+    case MOD_RELXILLNS: return 13;
     default: return -1;
   }
 }
@@ -256,11 +257,11 @@ static int bsearch_float(const float *arr, int n, float val) {
   return klo;
 }
 
-void xill_corner(const XillTableHost &t, double gam, double afe, double lxi, double ect, double dens,
+void xill_corner(const XillTableHost &t, double gam, double afe, double lxi, double ect, double dens, double ktbb,
                  double incl_deg, int *ind, double *fac) {
   // the input value of every table parameter as float (get_xilltab_paramvals,
-  // src/xilltable.c:278-295); kTbb and Frac are 0 for relxill
-  float v[8] = {(float)gam, (float)afe, (float)lxi, (float)ect, (float)dens, 0.0f, 0.0f, (float)incl_deg};
+  // src/xilltable.c:278-295); kTbb is 0 but for the blackbody model, Frac 0
+  float v[8] = {(float)gam, (float)afe, (float)lxi, (float)ect, (float)dens, (float)ktbb, 0.0f, (float)incl_deg};
   int k_ind[XILL_NPAR];
   for (int k = 0; k < t.n_par; k++) {
     int i = bsearch_float(t.val[k].data(), t.n_val[k], v[t.pid[k]]);
@@ -539,13 +540,16 @@ void ensure_xilltable_host(int prim) {
   if (!getenv("RELXILL_TABLE_PATH")) {
     // synthetic tables: Ecut 5-1000 keV, kTe 1-400 keV on the fourth axis
     x.t = prim == PRIM_NTHCOMP ? make_synthetic_xilltable(6, 13, 5, 8, 6, 10, 2999, 1.0, 400.0)
+          : prim == PRIM_BBODY ? make_synthetic_xilltable_ns(7, 12, 5, 8, 5, 10, 2999)
                                : make_synthetic_xilltable(5, 13, 5, 8, 6, 10, 2999);
     x.err.clear();
     x.loaded = true;
   } else {
-    // the reference's table names (src/common.h:163, :166)
-    x.loaded = load_from_path(x.t, prim == PRIM_NTHCOMP ? "xillverCp_v3.4.fits" : "xillver-a-Ec5.fits", nullptr,
-                              load_xilltable_fits, load_xilltable_fits, x.err);
+    // the reference's table names (src/common.h:163, :166, :167)
+    const char *name = prim == PRIM_NTHCOMP ? "xillverCp_v3.4.fits"
+                       : prim == PRIM_BBODY ? "xillverNS-2.fits"
+                                            : "xillver-a-Ec5.fits";
+    x.loaded = load_from_path(x.t, name, nullptr, load_xilltable_fits, load_xilltable_fits, x.err);
   }
   if (x.loaded) x.dirty = true;
 }
@@ -553,7 +557,9 @@ void ensure_xilltable_host(int prim) {
 int ensure_xilltable_device(int prim, std::string &err) {
   XillSlot &x = G().xill[prim];
   ensure_xilltable_host(prim);
-  const char *what = prim == PRIM_NTHCOMP ? "xillver table (nthcomp): " : "xillver table: ";
+  const char *what = prim == PRIM_NTHCOMP ? "xillver table (nthcomp): "
+                     : prim == PRIM_BBODY ? "xillver table (blackbody): "
+                                          : "xillver table: ";
   if (!x.loaded) {
     err = what + x.err;
     return RX_ERR_TABLE;
@@ -756,6 +762,8 @@ struct CpRequest {
   int model = 0;
   double par[14] = {};
   double dens = 15.0;
+  double ktbb = 0.0;
+  int prim = PRIM_NTHCOMP;
   int status = RX_OK;  // RX_ERR_UNSUPPORTED: an ionisation gradient was asked for
 };
 
@@ -777,6 +785,18 @@ static CpRequest remap_cp(int model, const double *par) {
     // iongrad_type 0: constant ionisation (src/IonGradient.cpp:340-395);
     // the radial ionisation gradients are not built
     if (lround(par[14]) != 0) c.status = RX_ERR_UNSUPPORTED;
+  } else if (model == MOD_RELXILLNS) {
+    // relxillNS (lmodel_relxill_public.dat:140-153): Index1 Index2 Rbr a Incl
+    // Rin Rout z kTbb logxi Afe logN refl_frac -- relxill's order with kTbb in
+    // Gamma's slot and logN in Ecut's; get_xill_params leaves Gamma 0 and Ecut
+    // at its default 300 (src/ModelDefinition.cpp:357-376)
+    for (int i = 0; i < 13; i++) c.par[i] = par[i];
+    c.ktbb = par[8];
+    c.dens = par[11];
+    c.par[8] = 0.0;
+    c.par[11] = 300.0;
+    c.model = MOD_RELXILL;
+    c.prim = PRIM_BBODY;
   }
   return c;
 }
@@ -794,7 +814,7 @@ int batch_build(Batch &b, const rx_request *req_in, int n_req, std::string &err)
   const rx_request *req = req_in;
   for (int i = 0; i < n_req; i++) {
     const int m = req_in[i].model;
-    if ((m == MOD_RELXILLCP || m == MOD_RELXILLLPCP) && req_in[i].parameter) {
+    if ((m == MOD_RELXILLCP || m == MOD_RELXILLLPCP || m == MOD_RELXILLNS) && req_in[i].parameter) {
       if (cp.empty()) {
         req_local.assign(req_in, req_in + n_req);
         cp.resize(n_req);
@@ -803,7 +823,7 @@ int batch_build(Batch &b, const rx_request *req_in, int n_req, std::string &err)
       cp[i] = remap_cp(m, req_in[i].parameter);
       req_local[i].model = cp[i].model;
       req_local[i].parameter = cp[i].par;
-      prim[i] = PRIM_NTHCOMP;
+      prim[i] = cp[i].prim;
     }
   }
   const bool phys = phys_norm_env();
@@ -858,7 +878,7 @@ int batch_build(Batch &b, const rx_request *req_in, int n_req, std::string &err)
       err = perr;
       return st;
     }
-    if (st == RX_OK && prim[i] == PRIM_NTHCOMP) st = cp[i].status;
+    if (st == RX_OK && prim[i] != PRIM_CUTOFFPL) st = cp[i].status;
     p.n_in = r.n_flux;
     p.ener_off = o_ener;
     o_ener += r.n_flux + 1;
@@ -884,9 +904,11 @@ int batch_build(Batch &b, const rx_request *req_in, int n_req, std::string &err)
         const double afe = r.parameter[lp ? 9 : 10];
         const XillTableHost &xt = g.xill[p.prim].t;
         // logN: 15 for the cut-off power-law models (get_xill_params,
-        // src/ModelDefinition.cpp:367-368), the parameter for the nthcomp ones
-        const double dens = p.prim == PRIM_NTHCOMP ? cp[i].dens : 15.0;
-        xill_corner(xt, p.xgam, afe, lxi, p.xect_src, dens, incl_deg_of(r.model, r.parameter), p.xind, p.xfac);
+        // src/ModelDefinition.cpp:367-368), the parameter for the others
+        const double dens = p.prim != PRIM_CUTOFFPL ? cp[i].dens : 15.0;
+        p.xktbb = p.prim == PRIM_BBODY ? cp[i].ktbb : 0.0;
+        xill_corner(xt, p.xgam, afe, lxi, p.xect_src, dens, p.xktbb, incl_deg_of(r.model, r.parameter), p.xind,
+                    p.xfac);
         // a table without a density axis divides its rows by 10^(logN - 15)
         // (normalizeXillverSpecLogxiDensity, src/xilltable.c:488-511)
         p.xdens_div = (xt.dim_of(XP_DNS) < 0 && std::fabs(dens - 15) > 1e-6) ? std::pow(10, dens - 15) : 0.0;

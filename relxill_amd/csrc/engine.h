@@ -30,7 +30,7 @@ int parse_request(int model, const double *par, bool phys_norm, ReqParams &p, st
 // relxill requests: the xillver table corner and factors of the request's
 // parameters (get_xilltab_indices_for_paramvals and interp_xill_table,
 // src/xilltable.c:297-324, :1090-1125).
-void xill_corner(const XillTableHost &t, double gam, double afe, double lxi, double ect, double dens,
+void xill_corner(const XillTableHost &t, double gam, double afe, double lxi, double ect, double dens, double ktbb,
                  double incl_deg, int *ind, double *fac);
 int n_model_params(int model);
 // Edges of the radial zones (RadialGrid::calculate_radial_grid, src/IonGradient.cpp:457-489).
@@ -135,7 +135,8 @@ struct Global {
   DevBuf t_a, t_mu0, t_r, t_gmin, t_gmax, t_trff1, t_trff2, t_cosne1, t_cosne2;
   DevBuf l_a, l_h, l_rad, l_int, l_del, l_inc;
   // xillver tables, one per primary continuum (PRIM_*: xillver-a-Ec5.fits for
-  // the cut-off power law, xillverCp_v3.4.fits for nthcomp, src/common.h:163-166;
+  // the cut-off power law, xillverCp_v3.4.fits for nthcomp, xillverNS-2.fits
+  // for the blackbody, src/common.h:163-167;
   // loaded and uploaded on the first relxill request that needs one)
   XillSlot xill[N_PRIM];
   // returning-radiation table (loaded and uploaded on the first request with it)

@@ -53,6 +53,13 @@ __device__ inline double cutoffpl_bin(double e0, double e1, double ln_en, double
   return exp(1.0 / ecut - gam * ln_en - en / ecut) * (e1 - e0);
 }
 
+// spec_blackbody (src/Xillspec.cpp:255-268): photons of E^2 / (kT^4 (exp(E/kT) - 1))
+// dE in [e0, e1] at E = (e0 + e1) / 2
+__device__ inline double bbody_bin(double e0, double e1, double kt) {
+  const double en = 0.5 * (e0 + e1);
+  return en * en / (pow(kt, 4) * (exp(en / kt) - 1)) * (e1 - e0);
+}
+
 // 1e15 / (4 pi): the xillver tables' incident flux normalisation (src/Xillspec.cpp:203)
 constexpr double NORM_XILLVER_TABLE = 1e15 / 4.0 / 3.14159265358979323846;
 
@@ -73,6 +80,16 @@ __device__ inline double xill_norm_part(const DevConvGrid &cg, double ecut, doub
     const double e0 = cg.coarse[k], e1 = cg.coarse[k + 1];
     if (e0 >= 0.1 && e0 <= 1000.0)
       sp += cutoffpl_bin(e0, e1, cg.coarse_ln[k], ecut, gam) * 0.5 * (e0 + e1) * 1e20 * 1.602177e-09;
+  }
+  return sp;
+}
+
+// The same sum for the blackbody of kT_bb kt.
+__device__ inline double bbody_norm_part(const DevConvGrid &cg, double kt, int tid, int nt) {
+  double sp = 0.0;
+  for (int k = tid; k < N_ENER_COARSE; k += nt) {
+    const double e0 = cg.coarse[k], e1 = cg.coarse[k + 1];
+    if (e0 >= 0.1 && e0 <= 1000.0) sp += bbody_bin(e0, e1, kt) * 0.5 * (e0 + e1) * 1e20 * 1.602177e-09;
   }
   return sp;
 }
@@ -2132,8 +2149,12 @@ __global__ __launch_bounds__(PRIM_THREADS) void k_primary(BatchDev b, const int 
   const int nn = nth ? b.nth_n[sol] : 0;
   const double zo = 1 / rq.g_so - 1;
   const double nfac = nth ? nthcomp_normfac(nx, ns, nn, zo) : 0.0;
-  const double nf = nth ? b.nth[rq.nth_off].nf
-                        : 1. / (block_sum4(xill_norm_part(cg, ect, gam, tid, NT), s_w) / NORM_XILLVER_TABLE);
+  // the blackbody (relxillNS) takes no energy shift (calc_primary_spectrum,
+  // src/Xillspec.cpp:293-295)
+  const bool bb = rq.prim == PRIM_BBODY;
+  const double nf = nth  ? b.nth[rq.nth_off].nf
+                    : bb ? 1. / (block_sum4(bbody_norm_part(cg, rq.xktbb, tid, NT), s_w) / NORM_XILLVER_TABLE)
+                         : 1. / (block_sum4(xill_norm_part(cg, ect, gam, tid, NT), s_w) / NORM_XILLVER_TABLE);
   // the primary on the convolution grid, evaluated where the rebin reads it
   // (one or two grid bins per output bin on caller grids finer than the
   // convolution grid; every grid bin of the output bin, each with its
@@ -2142,9 +2163,10 @@ __global__ __launch_bounds__(PRIM_THREADS) void k_primary(BatchDev b, const int 
   const double *cl = cg.ln_emid;
   const double ect_o = ect * rq.g_so;
   auto prim_at = [&](int i) {
-    const double v = nth ? nthcomp_bin(ce[i], ce[i + 1], nthcomp_edge(nx, ns, nn, ce[i], zo),
-                                       nthcomp_edge(nx, ns, nn, ce[i + 1], zo), nfac)
-                         : cutoffpl_bin(ce[i], ce[i + 1], cl[i], ect_o, gam);
+    const double v = nth  ? nthcomp_bin(ce[i], ce[i + 1], nthcomp_edge(nx, ns, nn, ce[i], zo),
+                                        nthcomp_edge(nx, ns, nn, ce[i + 1], zo), nfac)
+                     : bb ? bbody_bin(ce[i], ce[i + 1], rq.xktbb)
+                          : cutoffpl_bin(ce[i], ce[i + 1], cl[i], ect_o, gam);
     return (v * nf) * prim_scale;
   };
   const GridGuess gc = grid_guess(ce, n);
@@ -2521,7 +2543,11 @@ int launch_conv(const BatchDev &b, const XillWork *conv_work, int n_conv_work, v
 
 // the larger energy grid of the two xillver tables (units beyond a request's
 // own table return at once)
-static int xill_n_ener(const BatchDev &b) { return std::max(b.xt[0].n_ener, b.xt[1].n_ener); }
+static int xill_n_ener(const BatchDev &b) {
+  int n = 0;
+  for (int pr = 0; pr < N_PRIM; pr++) n = std::max(n, b.xt[pr].n_ener);
+  return n;
+}
 
 int launch_xill(const BatchDev &b, const XillWork *work, int n_single, int n_multi, void *stream) {
   if (n_single > 0) {

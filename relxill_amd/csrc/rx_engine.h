@@ -45,6 +45,9 @@ enum ModelKind : int {
   // MOD_RELXILL / MOD_RELXILLLP with ReqParams.prim = PRIM_NTHCOMP
   MOD_RELXILLCP = -1010,
   MOD_RELXILLLPCP = -1011,
+  // the blackbody-primary relxill model (relxillNS), run as MOD_RELXILL with
+  // ReqParams.prim = PRIM_BBODY
+  MOD_RELXILLNS = -1012,
 };
 
 enum EmisKind : int { EMIS_BKN = 1, EMIS_LP = 2 };
@@ -95,6 +98,7 @@ struct ReqParams {
   int prim, n_nth, nth_sol, pad_prim;
   int64_t nth_off;
   double xdens_div;
+  double xktbb;  // blackbody primary: kT_bb (keV)
   // returning radiation (lamp post): 1 add, -1 or 2 replace the direct
   // emissivity; the table spin, its first zone and the number of zones used
   int return_rad, rr_k, rr_klo, rr_n;
@@ -180,8 +184,8 @@ struct DevReflSet {
 // Resident xillver table: the spectra divided by 10^logXi at load time,
 // row ((((i0 n1 + i1) n2 + i2) n3 + i3) n_incl + ia) at spec[row * n_ener].
 // primary continua of the relxill models (PRIM_SPEC_*, src/common.h:57-59)
-enum PrimType : int { PRIM_CUTOFFPL = 0, PRIM_NTHCOMP = 1 };
-constexpr int N_PRIM = 2;
+enum PrimType : int { PRIM_CUTOFFPL = 0, PRIM_NTHCOMP = 1, PRIM_BBODY = 2 };
+constexpr int N_PRIM = 3;
 
 // Normalisation results of one nthcomp solve: nf = 1 / calcNormWrtXillverTableSpec
 // of the spectrum on the 500-bin 0.1-1000 keV grid (src/Xillspec.cpp:179-205)

@@ -112,6 +112,11 @@ std::string load_rradtable_fits(RradTableHost &t, const std::string &path);
 // 5-1000 keV; kTe 1-400 keV in the nthcomp tables).
 XillTableHost make_synthetic_xilltable(uint64_t seed, int n_gam, int n_afe, int n_lxi, int n_ect,
                                        int n_incl, int n_ener, double e4_lo = 5.0, double e4_hi = 1000.0);
+// Seeded synthetic table of the blackbody models with the parameters kTbb
+// (0.5-10 keV), A_Fe, logXi (1-4.7), Dens (15-19), Incl, as xillverNS-2.fits
+// names them, and n_ener log bins over 0.07-1000 keV.
+XillTableHost make_synthetic_xilltable_ns(uint64_t seed, int n_ktbb, int n_afe, int n_lxi, int n_dens, int n_incl,
+                                          int n_ener);
 std::string validate_xilltable(const XillTableHost &t);
 // The spectra divided by 10^logXi of their row and, for a table with a
 // density axis, by 10^(logN - 15) (renorm_xill_spec, src/xilltable.c:478-511,

@@ -92,7 +92,7 @@ struct Slot {
 int64_t request_bytes(const Pending &p) {
   Global &g = G();
   const bool lpx = (p.model == MOD_RELXILLLP || p.model == MOD_RELXILLLPCP);
-  const bool relxill = (p.model == MOD_RELXILL || p.model == MOD_RELXILLCP || lpx);
+  const bool relxill = (p.model == MOD_RELXILL || p.model == MOD_RELXILLCP || p.model == MOD_RELXILLNS || lpx);
   const bool conv = (p.model == MOD_RELCONV || p.model == MOD_RELCONVLP || p.model == MOD_RELXILL_CONV || relxill);
   const int64_t nz = lpx ? MAX_ZONES : 1;
   const int64_t n_in = (int64_t)p.energy.size() - 1;
@@ -106,7 +106,8 @@ int64_t request_bytes(const Pending &p) {
   b += 8 * 2 * (int64_t)g.rel.n_r * g.rel.n_g * 2;
   b += 8 * MAX_INCL * nz + sizeof(LpRefl) + sizeof(ReqNorm) + 64;
   if (relxill) {
-    const int64_t ne = std::max(g.xill[0].t.n_ener, g.xill[1].t.n_ener);
+    int64_t ne = 0;
+    for (int pr = 0; pr < N_PRIM; pr++) ne = std::max<int64_t>(ne, g.xill[pr].t.n_ener);
     b += (8 * ne + (int64_t)(2 * sizeof(XillWork) + sizeof(ZoneRec)) + 16) * nz;
   }
   if (p.model == MOD_RELXILLCP || p.model == MOD_RELXILLLPCP)  // nthcomp solves and the source's solution

@@ -346,6 +346,80 @@ XillTableHost make_synthetic_xilltable(uint64_t seed, int n_gam, int n_afe, int 
   return t;
 }
 
+// Synthetic spectra of the blackbody models (photons per bin): the reflected
+// blackbody E^2 / (exp(E / kT) - 1) softened by a power-law wing, a smooth
+// ionisation and density dependent soft excess, an iron line growing with
+// A_Fe and the limb law of the other table; 10^logXi times the density's
+// 10^(logN - 15) as the files store them (the loader divides both out).
+XillTableHost make_synthetic_xilltable_ns(uint64_t seed, int n_ktbb, int n_afe, int n_lxi, int n_dens, int n_incl,
+                                          int n_ener) {
+  XillTableHost t;
+  t.n_par = XILL_NPAR;
+  const int ids[XILL_NPAR] = {XP_KTB, XP_AFE, XP_LXI, XP_DNS, XP_INC};
+  const int nv[XILL_NPAR] = {n_ktbb, n_afe, n_lxi, n_dens, n_incl};
+  for (int k = 0; k < XILL_NPAR; k++) {
+    t.pid[k] = ids[k];
+    t.n_val[k] = nv[k];
+  }
+  t.val[0] = axis(n_ktbb, 0.5, 10.0, true);
+  t.val[1] = axis(n_afe, 0.5, 10.0, true);
+  t.val[2] = axis(n_lxi, 1.0, 4.7, false);
+  t.val[3] = axis(n_dens, 15.0, 19.0, false);
+  t.val[4].resize(n_incl);
+  for (int i = 0; i < n_incl; i++) t.val[4][i] = (float)(std::acos(1.0 - (i + 0.5) / n_incl) * 180.0 / kPi);
+  t.n_ener = n_ener;
+  t.elo.resize(n_ener);
+  t.ehi.resize(n_ener);
+  std::vector<double> lne(n_ener), emid(n_ener), de(n_ener), line(n_ener);
+  SplitMix rng(seed);
+  Smooth s_soft(rng), s_bb(rng);
+  for (int e = 0; e < n_ener; e++) {
+    const double lo = 0.07 * std::pow(1000.0 / 0.07, (double)e / n_ener);
+    const double hi = 0.07 * std::pow(1000.0 / 0.07, (double)(e + 1) / n_ener);
+    t.elo[e] = (float)lo;
+    t.ehi[e] = (float)hi;
+    emid[e] = 0.5 * (lo + hi);
+    de[e] = hi - lo;
+    lne[e] = std::log(emid[e]);
+    const double x = (lne[e] - std::log(6.4)) / 0.03;
+    line[e] = 3.0 * std::exp(-0.5 * x * x);
+  }
+  t.spec.resize(t.n_rows() * (size_t)n_ener);
+  std::vector<double> base(n_ener);
+  size_t row = 0;
+  for (int it = 0; it < n_ktbb; it++) {
+    const double kt = t.val[0][it];
+    for (int ia = 0; ia < n_afe; ia++) {
+      const double afe = t.val[1][ia];
+      for (int ix = 0; ix < n_lxi; ix++) {
+        const double lxi = t.val[2][ix];
+        for (int id = 0; id < n_dens; id++) {
+          const double dens = t.val[3][id];
+          const double scale = std::pow(10.0, lxi) * std::pow(10.0, dens - 15.0);
+          for (int e = 0; e < n_ener; e++) {
+            const double u = lne[e] / std::log(1000.0);
+            const double y = std::min(emid[e] / kt, 700.0);
+            const double bb = emid[e] * emid[e] / std::expm1(y) + 1e-3 * std::pow(emid[e], -1.5);
+            base[e] = std::log(bb) + 0.08 * s_bb(u, kt / 10.0) +
+                      0.5 * (lxi / 4.7) * std::exp(-emid[e]) * (1.0 + 0.5 * s_soft(u, lxi / 4.7, (dens - 15) / 4));
+          }
+          for (int ii = 0; ii < n_incl; ii++, row++) {
+            const double mu = std::cos(t.val[4][ii] * kPi / 180.0);
+            const double limb = std::log(0.35 + 0.65 * mu);
+            float *dst = &t.spec[row * n_ener];
+            for (int e = 0; e < n_ener; e++) {
+              const double v = scale * de[e] * std::exp(base[e] + limb * (1.0 + 0.05 * lne[e])) *
+                               (1.0 + afe * line[e] * (0.6 + 0.4 * mu));
+              dst[e] = (float)v;
+            }
+          }
+        }
+      }
+    }
+  }
+  return t;
+}
+
 std::string validate_xilltable(const XillTableHost &t) {
   if (t.n_par != XILL_NPAR) return "xillver table must have five parameters";
   if (t.pid[XILL_NPAR - 1] != XP_INC) return "the last xillver parameter must be the inclination";
