@@ -54,6 +54,11 @@ extern "C" {
 /* project id: the blackbody-primary relxill model (relxillNS; the reference's
  * MOD_TYPE_RELXILL with PRIM_SPEC_BB, src/ModelDefinition.cpp:89) */
 #define RX_MODEL_RELXILLNS (-1012)
+/* the non-relativistic xillver models (the reference's MOD_TYPE_XILLVER,
+ * MOD_TYPE_XILLVER_NTHCOMP, MOD_TYPE_XILLVERNS, src/common.h:91-111) */
+#define RX_MODEL_XILLVER 0
+#define RX_MODEL_XILLVERCP 100
+#define RX_MODEL_XILLVERNS (-101)
 
 /* ---- precision of the batched path ---- */
 #define RX_FP64 0 /* plain form: FP64 throughout */
@@ -141,6 +146,24 @@ void lmodrelxilllpdensnthcomp(const double *energy, int Nflux, const double *par
  * src/Xillspec.cpp:255-268), normalised as the xillver incident spectrum, and
  * the blackbody xillver table interpolated in kTbb, A_Fe, logXi, logN. */
 void lmodrelxillns(const double *energy, int Nflux, const double *parameter, int spectrum,
+                   double *flux, double *fluxError, const char *init);
+
+/* The non-relativistic reflection models (LocalModel::xillver_model,
+ * src/LocalModel.cpp:104-128): the xillver table (the model's own: the three
+ * tables above) interpolated in all five parameters, the inclination too
+ * (interp_5d_tab, src/xilltable.c:878-995), times 0.5 cos(Incl)
+ * (norm_xillver_spec, src/Xillspec.cpp:536-545), rebinned onto the
+ * redshifted grid, times |refl_frac|, plus the normalised primary on that
+ * grid for refl_frac >= 0 (add_primary_component, src/Relbase.cpp:294-344).
+ *   xillver: gamma Afe Ecut logxi z Incl refl_frac (lmodel_relxill_public.dat:77-84)
+ *   xillverCp: gamma Afe kTe logxi logN z Incl refl_frac (:86-94;
+ *     the reference's c_lmodxillverdensnthcomp)
+ *   xillverNS: kTbb Afe logN logxi z Incl refl_frac (:131-138) */
+void lmodxillver(const double *energy, int Nflux, const double *parameter, int spectrum,
+                 double *flux, double *fluxError, const char *init);
+void lmodxillverdensnthcomp(const double *energy, int Nflux, const double *parameter, int spectrum,
+                            double *flux, double *fluxError, const char *init);
+void lmodxillverns(const double *energy, int Nflux, const double *parameter, int spectrum,
                    double *flux, double *fluxError, const char *init);
 
 /* Status-returning evaluation of one model (the body of xspec_C_wrapper_eval_model,

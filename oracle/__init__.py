@@ -25,6 +25,9 @@ MOD_RELXILL_CONV = -1001
 MOD_RELXILLCP = -1010
 MOD_RELXILLLPCP = -1011
 MOD_RELXILLNS = -1012
+MOD_XILLVER = 0
+MOD_XILLVERCP = 100
+MOD_XILLVERNS = -101
 
 _D = ctypes.POINTER(ctypes.c_double)
 _F = ctypes.POINTER(ctypes.c_float)
@@ -136,6 +139,8 @@ def _load():
         "orc_primary_norm_factor": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
         "orc_prim_norm_factor": (ctypes.c_double, [ctypes.POINTER(XillParam), ctypes.c_double]),
         "orc_spec_blackbody": (None, [_D, _D, _I, ctypes.c_double]),
+        "orc_xill_spectrum_incl": (None, [ctypes.POINTER(XillTable), ctypes.POINTER(XillParam), _D, _D]),
+        "orc_xillver_model": (_I, [_I, ctypes.POINTER(XillTable), _D, _D, _D, _I]),
         "orc_prim_spectrum": (None, [_D, _D, _I, ctypes.POINTER(XillParam), ctypes.c_double, ctypes.c_double]),
         "orc_mcd_shape": (ctypes.c_double, [ctypes.c_double]),
         "orc_nthcomp_solve": (_I, [ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _D]),
@@ -404,6 +409,24 @@ def prim_norm_factor(gam, ect, prim=0, ktbb=0.0):
     """1 / calcNormWrtXillverTableSpec of the primary (cut-off power law,
     nthcomp with prim=1, the blackbody of ktbb with prim=2) on the 500-bin grid"""
     return lib.orc_prim_norm_factor(ctypes.byref(xill_param(gam, 1.0, 0.0, ect, prim=prim, ktbb=ktbb)), ect)
+
+
+def xill_spectrum_incl(xt, xp):
+    """(ener[n+1], flu[n]) of the table for the xillver models (the
+    inclination interpolated)"""
+    n = xt.t.n_ener
+    e = np.zeros(n + 1)
+    flu = np.zeros(n)
+    lib.orc_xill_spectrum_incl(ctypes.byref(xt.t), ctypes.byref(xp), _dp(e), _dp(flu))
+    return e, flu
+
+
+def xillver_model(model, xt, params, energy):
+    """xillver / xillverCp / xillverNS on the caller grid; (flux, status)"""
+    e = _arr(energy)
+    out = np.zeros(e.size - 1)
+    st = lib.orc_xillver_model(model, ctypes.byref(xt.t), _dp(_arr(params)), _dp(e), _dp(out), e.size - 1)
+    return out, st
 
 
 def blackbody(ener, ktbb):

@@ -312,6 +312,9 @@ int orc_n_params(int model_type) {
     case ORC_MOD_RELXILLCP: return 14;
     case ORC_MOD_RELXILLLPCP: return 17;
     case ORC_MOD_RELXILLNS: return 13;
+    case ORC_MOD_XILLVER: return 7;
+    case ORC_MOD_XILLVERCP: return 8;
+    case ORC_MOD_XILLVERNS: return 7;
     default: return -1;
   }
 }
@@ -1357,6 +1360,13 @@ static void xill_row(const orc_xilltable *t, const orc_xillparam *x, const int *
   orc_xill_renorm(dst, t->n_ener, lxi, dens);
 }
 
+/* the corners of the four interpolated parameters in the reference's order
+ * (interp_5d_tab_incl, src/xilltable.c:812-876; interp_5d_tab, :878-995) */
+static const char corner[16][4] = {
+    {0, 0, 0, 0}, {1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {1, 1, 0, 0}, {1, 0, 1, 0},
+    {0, 1, 1, 0}, {1, 1, 1, 0}, {0, 0, 0, 1}, {1, 0, 0, 1}, {0, 1, 0, 1}, {0, 0, 1, 1},
+    {1, 1, 0, 1}, {1, 0, 1, 1}, {0, 1, 1, 1}, {1, 1, 1, 1}};
+
 void orc_xill_spectra(const orc_xilltable *t, const orc_xillparam *x, double *ener, double *flu) {
   const int n = t->n_ener, n_incl = t->n_val[t->n_par - 1];
   for (int ii = 0; ii < n; ii++) ener[ii] = t->elo[ii];
@@ -1367,10 +1377,6 @@ void orc_xill_spectra(const orc_xilltable *t, const orc_xillparam *x, double *en
   orc_xill_factors(t, x, ind, fac);
   /* interp_5d_tab_incl (src/xilltable.c:812-876), for every inclination:
    * corners in the reference's order, weights as its products */
-  static const char corner[16][4] = {
-      {0, 0, 0, 0}, {1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {1, 1, 0, 0}, {1, 0, 1, 0},
-      {0, 1, 1, 0}, {1, 1, 1, 0}, {0, 0, 0, 1}, {1, 0, 0, 1}, {0, 1, 0, 1}, {0, 0, 1, 1},
-      {1, 1, 0, 1}, {1, 0, 1, 1}, {0, 1, 1, 1}, {1, 1, 1, 1}};
   double w[16];
   for (int c = 0; c < 16; c++) {
     double f[4];
@@ -1393,6 +1399,101 @@ void orc_xill_spectra(const orc_xilltable *t, const orc_xillparam *x, double *en
     }
   }
   free(rows);
+}
+
+void orc_xill_spectrum_incl(const orc_xilltable *t, const orc_xillparam *x, double *ener, double *flu) {
+  const int n = t->n_ener, ki = t->n_par - 1;
+  for (int ii = 0; ii < n; ii++) ener[ii] = t->elo[ii];
+  ener[n] = t->ehi[n - 1];
+  int ind[5];
+  double fac[5];
+  orc_xill_indices(t, x, ind);
+  orc_xill_factors(t, x, ind, fac);
+  /* the xillver models clamp the inclination into the table as well
+   * (resetInpvalsToBoundaries, src/xilltable.c:1117-1121); float arithmetic */
+  float vi = (float)x->incl;
+  const float ilo = t->val[ki][0], ihi = t->val[ki][t->n_val[ki] - 1];
+  if (vi < ilo) vi = ilo;
+  else if (vi > ihi) vi = ihi;
+  {
+    const float num = vi - t->val[ki][ind[ki]];
+    const float den = t->val[ki][ind[ki] + 1] - t->val[ki][ind[ki]];
+    fac[ki] = (float)(num / den);
+  }
+  /* interp_5d_tab (:878-995): 32 corners, the inclination's the fifth
+   * factor of each weight product, summed in the reference's order */
+  double w[32];
+  float *rows = (float *)malloc(sizeof(float) * 32 * (size_t)n);
+  for (int b5 = 0; b5 < 2; b5++) {
+    for (int c = 0; c < 16; c++) {
+      double f[4];
+      for (int k = 0; k < 4; k++) f[k] = corner[c][k] ? fac[k] : (1.0 - fac[k]);
+      w[16 * b5 + c] = f[0] * f[1] * f[2] * f[3] * (b5 ? fac[ki] : (1 - fac[ki]));
+      int idx[5];
+      for (int k = 0; k < 4; k++) idx[k] = ind[k] + corner[c][k];
+      idx[4] = ind[ki] + b5;
+      xill_row(t, x, idx, rows + (size_t)(16 * b5 + c) * n);
+    }
+  }
+  for (int e = 0; e < n; e++) {
+    double v = 0.0;
+    for (int c = 0; c < 32; c++) v = v + w[c] * (double)rows[(size_t)c * n + e];
+    flu[e] = v;
+  }
+  free(rows);
+}
+
+/* LocalModel::xillver_model (src/LocalModel.cpp:104-128) with
+ * shift_energy_grid_redshift (src/XspecSpectrum.h:70-76), norm_xillver_spec
+ * (src/Xillspec.cpp:536-545) and add_primary_component (src/Relbase.cpp:
+ * 294-344): the table spectrum at the parameters times 0.5 cos(Incl),
+ * rebinned onto the redshifted grid, times |refl_frac|, plus the normalised
+ * primary evaluated on that grid for refl_frac >= 0 */
+int orc_xillver_model(int model_type, const orc_xilltable *xt, const double *par, const double *energy,
+                      double *flux, int n_flux) {
+  orc_xillparam x;
+  memset(&x, 0, sizeof(x));
+  double z;
+  x.dens = 15;
+  x.ect = 300;
+  switch (model_type) {
+    case ORC_MOD_XILLVER:  /* gamma Afe Ecut logxi z Incl refl_frac */
+      x.gam = par[0]; x.afe = par[1]; x.ect = par[2]; x.lxi = par[3]; z = par[4]; x.incl = par[5];
+      x.refl_frac = par[6];
+      break;
+    case ORC_MOD_XILLVERCP:  /* gamma Afe kTe logxi logN z Incl refl_frac */
+      x.gam = par[0]; x.afe = par[1]; x.ect = par[2]; x.lxi = par[3]; x.dens = par[4]; z = par[5];
+      x.incl = par[6]; x.refl_frac = par[7];
+      x.prim = 1;
+      break;
+    case ORC_MOD_XILLVERNS:  /* kTbb Afe logN logxi z Incl refl_frac */
+      x.ktbb = par[0]; x.afe = par[1]; x.dens = par[2]; x.lxi = par[3]; z = par[4]; x.incl = par[5];
+      x.refl_frac = par[6];
+      x.prim = 2;
+      break;
+    default:
+      return 51;
+  }
+  const int nx = xt->n_ener;
+  double *ener = (double *)malloc(sizeof(double) * (n_flux + 1));
+  double *xe = (double *)malloc(sizeof(double) * (nx + 1));
+  double *spec = (double *)malloc(sizeof(double) * nx);
+  double *prim = (double *)malloc(sizeof(double) * n_flux);
+  memcpy(ener, energy, sizeof(double) * (n_flux + 1));
+  if (z > 0)
+    for (int ii = 0; ii <= n_flux; ii++) ener[ii] *= (1 + z);
+  orc_xill_spectrum_incl(xt, &x, xe, spec);
+  const double slab = 0.5 * cos(x.incl * M_PI / 180);
+  for (int ii = 0; ii < nx; ii++) spec[ii] *= slab;
+  orc_rebin_spectrum(ener, flux, n_flux, xe, spec, nx);
+  const double nf = orc_prim_norm_factor(&x, x.ect);
+  orc_prim_spectrum(prim, ener, n_flux, &x, x.ect, 1.0);
+  for (int ii = 0; ii < n_flux; ii++) prim[ii] *= nf;
+  for (int ii = 0; ii < n_flux; ii++) flux[ii] *= fabs(x.refl_frac);
+  if (x.refl_frac >= 0)
+    for (int ii = 0; ii < n_flux; ii++) flux[ii] += prim[ii];
+  free(ener); free(xe); free(spec); free(prim);
+  return 0;
 }
 
 void orc_xill_angdep(double *out, const double *flu, const double *dist, int n_incl, int n_ener) {

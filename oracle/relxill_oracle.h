@@ -43,6 +43,11 @@ extern "C" {
 #define ORC_MOD_RELXILLLPCP (-1011)
 /* relxillNS: relxill with the blackbody primary (lmodel_relxill_public.dat:140-153) */
 #define ORC_MOD_RELXILLNS (-1012)
+/* the non-relativistic xillver models (MOD_TYPE_XILLVER, _NTHCOMP, NS,
+ * src/common.h:91-111; lmodel_relxill_public.dat:77-138) */
+#define ORC_MOD_XILLVER 0
+#define ORC_MOD_XILLVERCP 100
+#define ORC_MOD_XILLVERNS (-101)
 
 /* emissivity types (src/common.h:50-55) */
 #define ORC_EMIS_BKN 1
@@ -258,6 +263,13 @@ void orc_xill_spectra(const orc_xilltable *t, const orc_xillparam *x, double *en
  * incident energy flux in 0.1-1000 keV) and *corr_gshift (photon-flux boost
  * ratio to a power law at g = 1.5) */
 void orc_xill_corr_factors(const orc_xilltable *t, const orc_xillparam *x, double *corr_flux, double *corr_gshift);
+/* interp_5d_tab for the xillver models (src/xilltable.c:878-995): one
+ * spectrum, the inclination (degrees) interpolated and clamped too */
+void orc_xill_spectrum_incl(const orc_xilltable *t, const orc_xillparam *x, double *ener, double *flu);
+/* xillver, xillverCp, xillverNS on a caller grid (LocalModel::xillver_model,
+ * src/LocalModel.cpp:104-128); 0 or a status */
+int orc_xillver_model(int model_type, const orc_xilltable *xt, const double *par, const double *energy,
+                      double *flux, int n_flux);
 /* calc_xillver_angdep (src/Xillspec.cpp:557-573) */
 void orc_xill_angdep(double *out, const double *flu, const double *dist, int n_incl, int n_ener);
 /* spec_cutoffpl (src/Xillspec.cpp:215-241) */

@@ -14,6 +14,7 @@ _LIB_NAMES = {
     "lmodrelconv", "lmodrelconvlp", "lmodrelline", "lmodrellinelp", "lmodrelxill", "lmodrelxilllp", "model_params",
     "nthcomp", "MODEL_RELXILLCP", "MODEL_RELXILLLPCP", "lmodrelxilldensnthcomp", "lmodrelxilllpdensnthcomp",
     "MODEL_RELXILLNS", "lmodrelxillns", "xillnstable_synthetic",
+    "MODEL_XILLVER", "MODEL_XILLVERCP", "MODEL_XILLVERNS", "lmodxillver", "lmodxillverdensnthcomp", "lmodxillverns",
     "xilltable_synthetic", "xillcptable_synthetic", "xilltable_view", "rradtable_synthetic", "rradtable_view",
     "reflset_create", "set_conv_grid", "tables_synthetic", "lptable_synthetic", "tables_view",
     "Server", "RX_OK", "RX_ERR_PARAM", "RX_ERR_UNSUPPORTED", "RX_ERR_TABLE", "RX_ERR_INPUT",

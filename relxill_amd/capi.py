@@ -32,13 +32,17 @@ MODEL_RELXILL_CONV = -1001  # project id: relxill-style conv of a resident refle
 MODEL_RELXILLCP = -1010  # project id: relxillCp (relxill with the nthcomp primary)
 MODEL_RELXILLLPCP = -1011  # project id: relxilllpCp
 MODEL_RELXILLNS = -1012  # project id: relxillNS (relxill with the blackbody primary)
+MODEL_XILLVER = 0
+MODEL_XILLVERCP = 100
+MODEL_XILLVERNS = -101
 
 FP64 = 0
 FP32 = 1
 
 N_PARAMS = {MODEL_RELLINE: 10, MODEL_RELCONV: 8, MODEL_RELLINELP: 10, MODEL_RELCONVLP: 9,
             MODEL_RELXILL_CONV: 8, MODEL_RELXILL: 13, MODEL_RELXILLLP: 14, MODEL_RELXILLCP: 14,
-            MODEL_RELXILLLPCP: 17, MODEL_RELXILLNS: 13}
+            MODEL_RELXILLLPCP: 17, MODEL_RELXILLNS: 13, MODEL_XILLVER: 7, MODEL_XILLVERCP: 8,
+            MODEL_XILLVERNS: 7}
 
 # default parameter values of the reference's lmodel.dat
 # (src/modelfiles/lmodel_relxill_public.dat:1-59)
@@ -56,6 +60,10 @@ DEFAULTS = {
                         0.0],
     # lmodel_relxill_public.dat:140-153
     MODEL_RELXILLNS: [3.0, 3.0, 15.0, 0.998, 30.0, -1.0, 400.0, 0.0, 2.0, 3.1, 1.0, 15.0, 3.0],
+    # lmodel_relxill_public.dat:77-138
+    MODEL_XILLVER: [2.0, 1.0, 300.0, 3.1, 0.0, 30.0, -1.0],
+    MODEL_XILLVERCP: [2.0, 1.0, 60.0, 3.1, 15.0, 0.0, 30.0, -1.0],
+    MODEL_XILLVERNS: [2.0, 1.0, 15.0, 3.1, 0.0, 30.0, -1.0],
 }
 
 
@@ -113,6 +121,9 @@ _EXPORTS = {
     "lmodrelxilldensnthcomp": (None, [_D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_char_p]),
     "lmodrelxilllpdensnthcomp": (None, [_D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_char_p]),
     "lmodrelxillns": (None, [_D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_char_p]),
+    "lmodxillver": (None, [_D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_char_p]),
+    "lmodxillverdensnthcomp": (None, [_D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_char_p]),
+    "lmodxillverns": (None, [_D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_char_p]),
     "rx_xillnstable_synthetic": (ctypes.c_int, [ctypes.c_uint64] + [ctypes.c_int] * 6),
     "rx_xillnstable_load_fits": (ctypes.c_int, [ctypes.c_char_p]),
     "rx_xillnstable_get_view": (ctypes.c_int, [ctypes.POINTER(_XillView)]),
@@ -281,6 +292,18 @@ def lmodrelxilllpdensnthcomp(energy, params):
 
 def lmodrelxillns(energy, params):
     return _lmod(lib.lmodrelxillns, energy, params)
+
+
+def lmodxillver(energy, params):
+    return _lmod(lib.lmodxillver, energy, params)
+
+
+def lmodxillverdensnthcomp(energy, params):
+    return _lmod(lib.lmodxillverdensnthcomp, energy, params)
+
+
+def lmodxillverns(energy, params):
+    return _lmod(lib.lmodxillverns, energy, params)
 
 
 def xilltable_synthetic(seed=5, n_gam=13, n_afe=5, n_lxi=8, n_ect=6, n_incl=10, n_ener=2999):

@@ -136,6 +136,21 @@ void lmodrelxillns(const double *energy, int Nflux, const double *parameter, int
   rx_eval_model(RX_MODEL_RELXILLNS, parameter, energy, flux, Nflux, RX_FP64);
 }
 
+void lmodxillver(const double *energy, int Nflux, const double *parameter, int, double *flux, double *,
+                 const char *) {
+  rx_eval_model(RX_MODEL_XILLVER, parameter, energy, flux, Nflux, RX_FP64);
+}
+
+void lmodxillverdensnthcomp(const double *energy, int Nflux, const double *parameter, int, double *flux, double *,
+                            const char *) {
+  rx_eval_model(RX_MODEL_XILLVERCP, parameter, energy, flux, Nflux, RX_FP64);
+}
+
+void lmodxillverns(const double *energy, int Nflux, const double *parameter, int, double *flux, double *,
+                   const char *) {
+  rx_eval_model(RX_MODEL_XILLVERNS, parameter, energy, flux, Nflux, RX_FP64);
+}
+
 int rx_last_status(void) { return last_status(); }
 const char *rx_last_error(void) { return last_error(); }
 

@@ -75,6 +75,9 @@ int n_model_params(int model) {
     case MOD_RELXILLCP: return 14;
     case MOD_RELXILLLPCP: return 17;
     case MOD_RELXILLNS: return 13;
+    case MOD_XILLVER: return 7;
+    case MOD_XILLVERCP: return 8;
+    case MOD_XILLVERNS: return 7;
     default: return -1;
   }
 }
@@ -118,6 +121,20 @@ int parse_request(int model, const double *par, bool phys_norm, ReqParams &p, st
       p.refl_frac = par[11];
       p.xboost = (int)lround(par[13]);
       break;
+    case MOD_XILLVER:  // gamma Afe Ecut logxi z Incl refl_frac (xillverCp and xillverNS are
+                       // remapped to this order): no relativistic parameters and no
+                       // checks beyond the table's clamps (LocalModel::xillver_model,
+                       // src/LocalModel.cpp:104-128, reads only get_xill_params)
+      p.xgam = par[0];
+      p.xect = par[2];
+      p.xect_src = par[2];
+      p.z = par[4];
+      p.incl = par[5] * kPi / 180;
+      p.refl_frac = par[6];
+      p.norm = 0.5 * cos(par[5] * kPi / 180);  // norm_factor_semi_infinite_slab, src/Xillspec.cpp:528-530
+      p.n_zones = 1;
+      p.g_so = 1.0;
+      return RX_OK;
     case MOD_RELCONV:  // Index1 Index2 Rbr a Incl Rin Rout limb
     case MOD_RELXILL_CONV:
       p.emis_type = EMIS_BKN;
@@ -258,7 +275,7 @@ static int bsearch_float(const float *arr, int n, float val) {
 }
 
 void xill_corner(const XillTableHost &t, double gam, double afe, double lxi, double ect, double dens, double ktbb,
-                 double incl_deg, int *ind, double *fac) {
+                 double incl_deg, int *ind, double *fac, bool clamp_incl, int *inc_ind, double *inc_fac) {
   // the input value of every table parameter as float (get_xilltab_paramvals,
   // src/xilltable.c:278-295); kTbb is 0 but for the blackbody model, Frac 0
   float v[8] = {(float)gam, (float)afe, (float)lxi, (float)ect, (float)dens, (float)ktbb, 0.0f, (float)incl_deg};
@@ -275,7 +292,7 @@ void xill_corner(const XillTableHost &t, double gam, double afe, double lxi, dou
   for (int k = 0; k < t.n_par; k++) {
     float &x = v[t.pid[k]];
     const std::vector<float> &vals = t.val[k];
-    if (t.pid[k] != XP_INC) {
+    if (t.pid[k] != XP_INC || clamp_incl) {
       if (x < vals[0]) x = vals[0];
       else if (x > vals[t.n_val[k] - 1]) x = vals[t.n_val[k] - 1];
     }
@@ -296,6 +313,8 @@ void xill_corner(const XillTableHost &t, double gam, double afe, double lxi, dou
     ind[k] = k_ind[k];
     fac[k] = k_fac[k];
   }
+  if (inc_ind) *inc_ind = k_ind[t.n_par - 1];
+  if (inc_fac) *inc_fac = k_fac[t.n_par - 1];
 }
 
 // ---------------------------------------------------------------------------
@@ -785,6 +804,22 @@ static CpRequest remap_cp(int model, const double *par) {
     // iongrad_type 0: constant ionisation (src/IonGradient.cpp:340-395);
     // the radial ionisation gradients are not built
     if (lround(par[14]) != 0) c.status = RX_ERR_UNSUPPORTED;
+  } else if (model == MOD_XILLVERCP) {
+    // xillverCp (lmodel_relxill_public.dat:86-94): gamma Afe kTe logxi logN z
+    // Incl refl_frac -- xillver's order with logN taken out
+    static const int k[7] = {0, 1, 2, 3, 5, 6, 7};
+    for (int i = 0; i < 7; i++) c.par[i] = par[k[i]];
+    c.dens = par[4];
+    c.model = MOD_XILLVER;
+  } else if (model == MOD_XILLVERNS) {
+    // xillverNS (:131-138): kTbb Afe logN logxi z Incl refl_frac; Gamma 0,
+    // Ecut 300 as for relxillNS
+    const double v[7] = {0.0, par[1], 300.0, par[3], par[4], par[5], par[6]};
+    for (int i = 0; i < 7; i++) c.par[i] = v[i];
+    c.ktbb = par[0];
+    c.dens = par[2];
+    c.model = MOD_XILLVER;
+    c.prim = PRIM_BBODY;
   } else if (model == MOD_RELXILLNS) {
     // relxillNS (lmodel_relxill_public.dat:140-153): Index1 Index2 Rbr a Incl
     // Rin Rout z kTbb logxi Afe logN refl_frac -- relxill's order with kTbb in
@@ -814,7 +849,9 @@ int batch_build(Batch &b, const rx_request *req_in, int n_req, std::string &err)
   const rx_request *req = req_in;
   for (int i = 0; i < n_req; i++) {
     const int m = req_in[i].model;
-    if ((m == MOD_RELXILLCP || m == MOD_RELXILLLPCP || m == MOD_RELXILLNS) && req_in[i].parameter) {
+    if ((m == MOD_RELXILLCP || m == MOD_RELXILLLPCP || m == MOD_RELXILLNS || m == MOD_XILLVERCP ||
+         m == MOD_XILLVERNS) &&
+        req_in[i].parameter) {
       if (cp.empty()) {
         req_local.assign(req_in, req_in + n_req);
         cp.resize(n_req);
@@ -839,7 +876,7 @@ int batch_build(Batch &b, const rx_request *req_in, int n_req, std::string &err)
   std::vector<XillWork> xill_multi;
   for (int pr = 0; pr < N_PRIM; pr++) {
     for (int i = 0; i < n_req; i++) {
-      if (is_relxill(req[i].model) && prim[i] == pr) {
+      if ((is_relxill(req[i].model) || req[i].model == MOD_XILLVER) && prim[i] == pr) {
         int st = ensure_xilltable_device(pr, err);
         if (st != RX_OK) return st;
         break;
@@ -886,7 +923,33 @@ int batch_build(Batch &b, const rx_request *req_in, int n_req, std::string &err)
     o_flux += r.n_flux;
     p.refl_set = r.refl_set;
     if (p.n_zones < 1) p.n_zones = 1;
-    if (is_conv(r.model)) {
+    if (r.model == MOD_XILLVER) {
+      // the xillver models: the table at the parameters with the inclination
+      // interpolated as a fifth axis (k_xill over a two-bin angle
+      // distribution), rebinned with the primary (k_primary)
+      p.n_e = 0;
+      p.pgrid_off = -1;
+      p.prof_off = -1;
+      const XillTableHost &xt = g.xill[p.prim].t;
+      const double dens = p.prim != PRIM_CUTOFFPL ? cp[i].dens : 15.0;
+      p.xktbb = p.prim == PRIM_BBODY ? cp[i].ktbb : 0.0;
+      xill_corner(xt, p.xgam, r.parameter[1], r.parameter[3], p.xect_src, dens, p.xktbb, r.parameter[5], p.xind,
+                  p.xfac, true, &p.xinc_ind, &p.xinc_fac);
+      p.xdens_div = (xt.dim_of(XP_DNS) < 0 && std::fabs(dens - 15) > 1e-6) ? std::pow(10, dens - 15) : 0.0;
+      p.n_incl = xt.n_incl();
+      p.xspec_off = o_xspec;
+      o_xspec += (int64_t)xt.n_ener;
+      p.zone_off = o_zone;
+      o_zone += 1;
+      b.h_xill.push_back(XillWork{i, 0});
+      b.h_prim.push_back(i);
+      if (p.prim == PRIM_NTHCOMP) {
+        p.n_nth = 1;
+        p.nth_off = (int64_t)b.h_nth.size();
+        p.nth_sol = b.n_nth_sol++;
+        b.h_nth.push_back(XillWork{i, 0});
+      }
+    } else if (is_conv(r.model)) {
       p.n_e = g.conv_n;
       p.pgrid_off = -1;
       p.prof_off = o_prof;
@@ -958,11 +1021,12 @@ int batch_build(Batch &b, const rx_request *req_in, int n_req, std::string &err)
       o_rrc += 2 * (int64_t)p.n_zones;
       for (int z = 0; z < p.n_zones; z++) b.h_rrc.push_back(XillWork{i, z});
     }
-    p.need_cosne = (p.limb != 0 || p.n_incl > 0) ? 1 : 0;
+    const bool xill_model = r.model == MOD_XILLVER;
+    p.need_cosne = (p.limb != 0 || (p.n_incl > 0 && !xill_model)) ? 1 : 0;
     p.dist_off = o_dist;
     o_dist += (int64_t)p.n_incl * p.n_zones;
     p.node_off = -1;
-    if (p.n_incl > 0) {
+    if (p.n_incl > 0 && !xill_model) {
       p.node_off = o_node;
       o_node += node_stride;
     }
@@ -986,7 +1050,7 @@ int batch_build(Batch &b, const rx_request *req_in, int n_req, std::string &err)
     const rx_request &r = req[i];
     const ReqParams &p = b.h_req[i];
     memcpy(&b.h_ener[p.ener_off], r.energy, sizeof(double) * (r.n_flux + 1));
-    if (is_relxill(r.model) && p.z > 0) {
+    if ((is_relxill(r.model) || r.model == MOD_XILLVER) && p.z > 0) {
       // relxill evaluates on the redshifted grid (shift_energy_grid_redshift,
       // src/XspecSpectrum.h:70-76)
       for (int k = 0; k <= r.n_flux; k++) b.h_ener[p.ener_off + k] *= (1 + p.z);

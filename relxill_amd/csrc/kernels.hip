@@ -455,6 +455,30 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
   const int nr = b.nr_fine;
   const int n_r = T.n_r, ng = T.n_g;
   if (rq.pre_status != 0) return;  // status already set by the host
+  if (rq.model == MOD_XILLVER) {
+    // the xillver models (LocalModel::xillver_model, src/LocalModel.cpp:
+    // 104-128) have no relativistic part: the zone record carries the Ecut
+    // (kTe) bracket, the angle distribution the inclination's bracket, so
+    // that k_xill's sum over angles is the interpolation in the inclination
+    // (interp_5d_tab, src/xilltable.c:878-995); k_primary rebins and adds the
+    // primary
+    if (tid == 0) {
+      const DevXillTable &xt = b.xt[rq.prim];
+      ZoneRec o;
+      o.rmean = 0.0;
+      o.del_emit = 0.0;
+      o.shift = 1.0;
+      o.ncf = 1.0;
+      o.ect_ind = xt.ect_dim >= 0 ? rq.xind[xt.ect_dim] : 0;
+      o.ect_fac = xt.ect_dim >= 0 ? rq.xfac[xt.ect_dim] : 0.0;
+      o.pad = 0;
+      b.zone[rq.zone_off] = o;
+    }
+    double *d = b.dist + rq.dist_off;
+    for (int a = tid; a < rq.n_incl; a += SETUP_THREADS)
+      d[a] = (a == rq.xinc_ind) ? 1.0 - rq.xinc_fac : ((a == rq.xinc_ind + 1) ? rq.xinc_fac : 0.0);
+    return;
+  }
   if (rq.model == MOD_RELXILL || rq.model == MOD_RELXILLLP) {
     // the zones' convolutions add into the output (k_conv)
     double *out = b.out + rq.flux_off;
@@ -1583,6 +1607,7 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_finalize(BatchDev b) {
   const int req = blockIdx.x;
   const int tid = threadIdx.x;
   const ReqParams rq = b.req[req];
+  if (rq.model == MOD_XILLVER) return;  // no profile; k_primary writes the output
   const int n_e = rq.n_e;
   double *dst = profile_acc(b, rq);
   const int st = b.status[req];
@@ -2127,7 +2152,12 @@ __global__ __launch_bounds__(PRIM_THREADS) void k_primary(BatchDev b, const int 
   const int tid = threadIdx.x, NT = blockDim.x;
   const int lane = tid & 63, wave = tid >> 6, nwave = NT >> 6;
   const ReqParams &rq = b.req[req];
-  if (b.status[req] != 0 || !(rq.refl_frac >= 0)) return;
+  const bool xill = rq.model == MOD_XILLVER;
+  if (xill && b.status[req] != 0) {
+    for (int j = tid; j < rq.n_in; j += NT) b.out[rq.flux_off + j] = 0.0;
+    return;
+  }
+  if (!xill && (b.status[req] != 0 || !(rq.refl_frac >= 0))) return;
   const DevConvGrid &cg = b.cg;
   const int n = cg.n;
   const double ect = rq.xect_src, gam = rq.xgam;
@@ -2176,6 +2206,38 @@ __global__ __launch_bounds__(PRIM_THREADS) void k_primary(BatchDev b, const int 
   const int n_o = rq.n_in;
   const int nbo = (n_o + nwave - 1) / nwave;
   const int jend = min(n_o, (wave + 1) * nbo);
+  if (xill) {
+    // the xillver models: the table spectrum times 0.5 cos(Incl)
+    // (norm_xillver_spec, src/Xillspec.cpp:536-545) rebinned onto the
+    // (redshifted) caller grid, times |refl_frac|, plus the normalised primary
+    // evaluated on the caller grid itself for refl_frac >= 0
+    // (add_primary_component, src/Relbase.cpp:294-344)
+    const DevXillTable &xt = b.xt[rq.prim];
+    const int nxt = xt.n_ener;
+    const double *xs = b.xspec + rq.xspec_off;
+    const double slab = rq.norm, rf = rq.refl_frac;
+    const GridGuess gx = grid_guess(xt.ener, nxt);
+    auto ge_x = [&](int k) { return xt.ener[k]; };
+    auto refl_at = [&](int k) { return xs[k] * slab; };
+    for (int q = 0; q < nbo; q += 64) {
+      const int jb = wave * nbo + q + lane;
+      if (jb < jend) {
+        const double elo = eo[jb], ehi = eo[jb + 1];
+        const int xlo = count_le(ge_x, nxt, elo, gx);
+        const int xhi = count_le(ge_x, nxt, ehi, gx);
+        double v = rebin_bin(elo, ehi, xlo, xhi, ge_x, nxt, refl_at) * fabs(rf);
+        if (rf >= 0) {
+          const double p = nth  ? nthcomp_bin(elo, ehi, nthcomp_edge(nx, ns, nn, elo, 0.0),
+                                              nthcomp_edge(nx, ns, nn, ehi, 0.0), nfac)
+                           : bb ? bbody_bin(elo, ehi, rq.xktbb)
+                                : cutoffpl_bin(elo, ehi, log(0.5 * (elo + ehi)), ect, gam);
+          v += p * nf;
+        }
+        out[jb] = v;
+      }
+    }
+    return;
+  }
   for (int q = 0; q < nbo; q += 64) {
     const int jb = wave * nbo + q + lane;
     if (jb < jend) {

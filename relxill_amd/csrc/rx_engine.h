@@ -48,6 +48,12 @@ enum ModelKind : int {
   // the blackbody-primary relxill model (relxillNS), run as MOD_RELXILL with
   // ReqParams.prim = PRIM_BBODY
   MOD_RELXILLNS = -1012,
+  // the non-relativistic xillver models (the reference's MOD_TYPE_XILLVER,
+  // _XILLVER_NTHCOMP, _XILLVERNS, src/common.h:91-111); their requests run as
+  // MOD_XILLVER with ReqParams.prim naming the primary
+  MOD_XILLVER = 0,
+  MOD_XILLVERCP = 100,
+  MOD_XILLVERNS = -101,
 };
 
 enum EmisKind : int { EMIS_BKN = 1, EMIS_LP = 2 };
@@ -99,6 +105,10 @@ struct ReqParams {
   int64_t nth_off;
   double xdens_div;
   double xktbb;  // blackbody primary: kT_bb (keV)
+  // the xillver models: the inclination's table bracket and float factor
+  // (the fifth interpolated parameter), a two-bin angle distribution for k_xill
+  int xinc_ind, pad_xinc;
+  double xinc_fac;
   // returning radiation (lamp post): 1 add, -1 or 2 replace the direct
   // emissivity; the table spin, its first zone and the number of zones used
   int return_rad, rr_k, rr_klo, rr_n;

@@ -92,7 +92,8 @@ struct Slot {
 int64_t request_bytes(const Pending &p) {
   Global &g = G();
   const bool lpx = (p.model == MOD_RELXILLLP || p.model == MOD_RELXILLLPCP);
-  const bool relxill = (p.model == MOD_RELXILL || p.model == MOD_RELXILLCP || p.model == MOD_RELXILLNS || lpx);
+  const bool xill = (p.model == MOD_XILLVER || p.model == MOD_XILLVERCP || p.model == MOD_XILLVERNS);
+  const bool relxill = (p.model == MOD_RELXILL || p.model == MOD_RELXILLCP || p.model == MOD_RELXILLNS || lpx || xill);
   const bool conv = (p.model == MOD_RELCONV || p.model == MOD_RELCONVLP || p.model == MOD_RELXILL_CONV || relxill);
   const int64_t nz = lpx ? MAX_ZONES : 1;
   const int64_t n_in = (int64_t)p.energy.size() - 1;
@@ -110,7 +111,7 @@ int64_t request_bytes(const Pending &p) {
     for (int pr = 0; pr < N_PRIM; pr++) ne = std::max<int64_t>(ne, g.xill[pr].t.n_ener);
     b += (8 * ne + (int64_t)(2 * sizeof(XillWork) + sizeof(ZoneRec)) + 16) * nz;
   }
-  if (p.model == MOD_RELXILLCP || p.model == MOD_RELXILLLPCP)  // nthcomp solves and the source's solution
+  if (p.model == MOD_RELXILLCP || p.model == MOD_RELXILLLPCP || p.model == MOD_XILLVERCP)  // nthcomp solves
     b += (int64_t)(sizeof(XillWork) + sizeof(NthRes)) * (nz + 1) + 8 * (2 * NTH_MAX_POINTS + 1) + 4;
   return b;
 }
