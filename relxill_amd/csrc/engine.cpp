@@ -1324,7 +1324,7 @@ int batch_sync(Batch &b) {
   int ctr[N_COUNTERS] = {};
   if (hipMemcpy(ctr, b.d_ctr.p, sizeof ctr, hipMemcpyDeviceToHost) != hipSuccess) return RX_ERR_DEVICE;
   b.fix_overflow = b.last_mode == 1 && ctr[CTR_FIX_OVF] != 0;
-  b.counters[0] = std::min(ctr[CTR_ITEMS], b.item_cap);
+  b.counters[0] = std::min(ctr[CTR_ITEMS] + ctr[CTR_ITEMS_S], b.item_cap);
   b.counters[1] = (b.last_mode == 1) ? ctr[CTR_FIX] : 0;
   b.counters[2] = b.fix_overflow ? 1 : 0;
   if (b.fix_overflow) {

@@ -493,8 +493,9 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
   __shared__ double s_fa, s_fm, s_rms;
   constexpr int MAX_RC = MAX_NR_FINE / RAD_CHUNK;
   // energy window of each (radius chunk, zone): bins [s_ulo, s_uhi]
-  __shared__ int s_ulo[MAX_RC * MAX_ZONES], s_uhi[MAX_RC * MAX_ZONES], s_pref[MAX_RC * MAX_ZONES + 1];
-  __shared__ int s_base;
+  __shared__ int s_ulo[MAX_RC * MAX_ZONES], s_uhi[MAX_RC * MAX_ZONES];
+  __shared__ int s_pref[MAX_RC * MAX_ZONES + 1], s_prefs[MAX_RC * MAX_ZONES + 1];
+  __shared__ int s_base, s_bases;
   const int n_rc = (b.nr_fine + RAD_CHUNK - 1) / RAD_CHUNK;
   const int nz = rq.n_zones;
   const double *zg = b.req[req].zgrid;
@@ -982,37 +983,52 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
       for (int z = 0; z < nz; z++)
         for (int j = jlo + tid; j <= jhi; j += SETUP_THREADS) acc[(size_t)z * rq.n_e + j] = 0.0;
     // work items: the window of each (radius chunk, zone) cut into segments
-    // of seg_bins; the item's chunk and zone share its rc field
+    // of seg_bins; the item's chunk and zone share its rc field.  An item's
+    // time grows with its bins, so segments of at least S / 4 bins go to the
+    // front of the list and the shorter last segments of windows to its back:
+    // k_integrate hands out the long ones first and the wavefronts finish
+    // together on short ones.
     const int S = b.seg_bins;
+    const int thr = max(1, S / 4);
     const int n_cz = n_rc * nz;  // pairs q = c nz + z; their windows sit at c MAX_ZONES + z
     if (tid == 0) {
-      int tot = 0;
+      int tot = 0, tots = 0;
       for (int q = 0; q < n_cz; q++) {
         const int w = (q / nz) * MAX_ZONES + q % nz;
         s_pref[q] = tot;
-        if (s_uhi[w] >= s_ulo[w]) tot += (s_uhi[w] - s_ulo[w] + S) / S;
+        s_prefs[q] = tots;
+        if (s_uhi[w] >= s_ulo[w]) {
+          const int len = s_uhi[w] - s_ulo[w] + 1;
+          const int nseg = (len + S - 1) / S;
+          const int nlong = nseg - 1 + (len - (nseg - 1) * S >= thr ? 1 : 0);
+          tot += nlong;
+          tots += nseg - nlong;
+        }
       }
-      s_pref[n_cz] = tot;
       int base = atomicAdd(&b.ctr[CTR_ITEMS], tot);
-      if (base + tot > b.item_cap) {  // cannot happen: the host sizes the list for every bin
+      const int bases = atomicAdd(&b.ctr[CTR_ITEMS_S], tots);
+      // cannot happen: the host sizes the list for every bin of every request
+      if (base + tot > b.item_cap || bases + tots > b.item_cap) {
         atomicOr(&b.ctr[CTR_ITEMS_OVF], 1);
         s_st = ST_WORKLIST;
         base = -1;
       }
       s_base = base;
+      s_bases = bases;
     }
     __syncthreads();
     if (s_base >= 0) {
       for (int q = tid; q < n_cz; q += SETUP_THREADS) {
         const int c = q / nz, z = q % nz, w = c * MAX_ZONES + z;
-        int k = s_base + s_pref[q];
+        int k = s_base + s_pref[q], ks = s_bases + s_prefs[q];
         for (int j0 = s_ulo[w]; j0 <= s_uhi[w]; j0 += S) {
           WorkItem it;
           it.req = req;
           it.rc = c | (z << 16);
           it.j0 = j0;
           it.j1 = min(s_uhi[w] + 1, j0 + S);
-          b.items[k++] = it;
+          if (it.j1 - it.j0 >= thr) b.items[k++] = it;
+          else b.items[b.item_cap - 1 - ks++] = it;
         }
       }
     }
@@ -1380,7 +1396,9 @@ __global__ __launch_bounds__(64, MODE == 0 ? 1 : 3) void k_integrate(BatchDev b,
   double *s_coef = reinterpret_cast<double *>(reinterpret_cast<char *>(s_dyn) +
                                               integrate_rows_bytes(ng, cos_rows != 0, sizeof(TV)));
   TV *s_bins = (MODE == 0) ? reinterpret_cast<TV *>(s_coef + 64) : reinterpret_cast<TV *>(s_coef);
-  const int n_items = min(b.ctr[CTR_ITEMS], b.item_cap);
+  // long items at [0, n_long), short ones at the back of the list
+  const int n_long = min(b.ctr[CTR_ITEMS], b.item_cap);
+  const int n_short = min(b.ctr[CTR_ITEMS_S], b.item_cap - n_long);
   const int nr = b.nr_fine;
   // turn-round roles: SUM_LANES lanes per bin; lane (col, part) sums radii
   // SUM_BINS part .. SUM_BINS part + SUM_BINS - 1 of bin col
@@ -1405,16 +1423,22 @@ __global__ __launch_bounds__(64, MODE == 0 ? 1 : 3) void k_integrate(BatchDev b,
   }
   int xq = blockIdx.x % N_XQ, exhausted = 0;
   for (;;) {
-    // next item of range xq; an empty range sends the wavefront to the next
-    // one, and once all N_XQ have been found empty it is done
+    // next item of range xq (its share of the long items, then its share of
+    // the short ones); an empty range sends the wavefront to the next one,
+    // and once all N_XQ have been found empty it is done
     int k = -1;
     while (exhausted < N_XQ) {
-      const int lo = (int)((int64_t)n_items * xq / N_XQ), hi = (int)((int64_t)n_items * (xq + 1) / N_XQ);
+      const int lo = (int)((int64_t)n_long * xq / N_XQ), hi = (int)((int64_t)n_long * (xq + 1) / N_XQ);
+      const int los = (int)((int64_t)n_short * xq / N_XQ), his = (int)((int64_t)n_short * (xq + 1) / N_XQ);
       int kk = 0;
       if (lane == 0) kk = atomicAdd(&b.ctr[CTR_XQ + xq * XQ_STRIDE], 1);
       kk = __builtin_amdgcn_readfirstlane(kk);
       if (kk < hi - lo) {
         k = lo + kk;
+        break;
+      }
+      if (kk < (hi - lo) + (his - los)) {
+        k = b.item_cap - 1 - (los + kk - (hi - lo));
         break;
       }
       exhausted++;
@@ -2545,7 +2569,10 @@ static size_t integrate_lds(const BatchDev &b, bool cos_rows, size_t elem, int *
   const size_t w = (elem == sizeof(double)) ? 64 * sizeof(double) : 0;
   const size_t park = (size_t)SUM_BINS * SUM_STRIDE * elem;
   const size_t nq = 2 * (size_t)b.rt.n_g;
-  auto per_cu = [](size_t lds) { return std::min<size_t>(12, 163840 / ((lds + 511) & ~(size_t)511)); };
+  // wavefronts per CU that the LDS holds: gfx950 allocates a workgroup's LDS
+  // in 1280-byte granules, 128 per CU (160 KB); at most 16 (the VGPRs of the
+  // FP32 forms allow four per SIMD)
+  auto per_cu = [](size_t lds) { return std::min<size_t>(16, 128 / ((lds + 1279) / 1280)); };
   const size_t base = rows + w + park;
   for (int c = 8; c > 2; c--) {
     if ((size_t)c * nq > (size_t)STAGE_MAX) continue;

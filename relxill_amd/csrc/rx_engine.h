@@ -283,8 +283,9 @@ constexpr int XQ_STRIDE = 32;  // ints between the range counters (one 128-byte 
 enum CounterSlot : int {
   CTR_FIX = 0,        // FixEntry entries written
   CTR_FIX_OVF = 1,    // FixEntry queue overflowed
-  CTR_ITEMS = 2,      // work items written
+  CTR_ITEMS = 2,      // long work items written (from the front of the list)
   CTR_ITEMS_OVF = 3,  // work list overflowed
+  CTR_ITEMS_S = 4,    // short work items written (from the back of the list)
   CTR_XQ = XQ_STRIDE,  // N_XQ range counters, XQ_STRIDE apart
   N_COUNTERS = CTR_XQ + N_XQ * XQ_STRIDE
 };
