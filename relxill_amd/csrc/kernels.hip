@@ -463,14 +463,17 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     // (interp_5d_tab, src/xilltable.c:878-995); k_primary rebins and adds the
     // primary
     if (tid == 0) {
+      // the bracket through the request in global memory: a runtime index
+      // into the local copy rq would move that copy to scratch
+      const ReqParams &gq = b.req[req];
       const DevXillTable &xt = b.xt[rq.prim];
       ZoneRec o;
       o.rmean = 0.0;
       o.del_emit = 0.0;
       o.shift = 1.0;
       o.ncf = 1.0;
-      o.ect_ind = xt.ect_dim >= 0 ? rq.xind[xt.ect_dim] : 0;
-      o.ect_fac = xt.ect_dim >= 0 ? rq.xfac[xt.ect_dim] : 0.0;
+      o.ect_ind = xt.ect_dim >= 0 ? gq.xind[xt.ect_dim] : 0;
+      o.ect_fac = xt.ect_dim >= 0 ? gq.xfac[xt.ect_dim] : 0.0;
       o.pad = 0;
       b.zone[rq.zone_off] = o;
     }
