@@ -908,6 +908,10 @@ int batch_build(Batch &b, const rx_request *req_in, int n_req, std::string &err)
       err = "request with empty grid or missing parameters";
       return RX_ERR_ARG;
     }
+    if (r.n_flux > (int)RAD_BIN_MASK) {  // bins are 24-bit fields of the radius records
+      err = "energy grid of more than 2^24 - 1 bins";
+      return RX_ERR_ARG;
+    }
     ReqParams &p = b.h_req[i];
     std::string perr;
     int st = parse_request(r.model, r.parameter, phys, p, perr, prim[i]);
@@ -1115,6 +1119,7 @@ int batch_upload(Batch &b) {
   e |= b.d_pgrid.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_pgrid));
   e |= b.d_node.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_node));
   e |= b.d_rad.ensure(sizeof(RadRec) * nreq * g.nr_fine);
+  e |= b.d_rnode.ensure(sizeof(double) * RAD_NODE_ARRAYS * nreq * g.rel.n_r);
   e |= b.d_rnorm.ensure(sizeof(ReqNorm) * nreq);
   e |= b.d_prof.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_prof));
   e |= b.d_dist.ensure(sizeof(double) * std::max<int64_t>(1, b.tot_dist));
@@ -1178,6 +1183,7 @@ static BatchDev dev_view(Batch &b) {
   bd.pgrid = b.d_pgrid.as<double>();
   bd.node = b.d_node.as<double>();
   bd.rad = b.d_rad.as<RadRec>();
+  bd.rnode = b.d_rnode.as<double>();
   bd.rnorm = b.d_rnorm.as<ReqNorm>();
   bd.prof = b.d_prof.as<double>();
   bd.dist = b.d_dist.as<double>();

@@ -74,7 +74,7 @@ struct Batch {
   DevBuf d_req, d_ener, d_flux_in, d_out, d_pgrid, d_node, d_rad, d_prof, d_dist, d_refl, d_status, d_conv;
   DevBuf d_xill, d_xspec, d_zone, d_prim, d_rrc, d_rrcorr;
   DevBuf d_nthw, d_nth, d_nthx, d_nths, d_nthn;
-  DevBuf d_items, d_ctr, d_fix, d_reach, d_rnorm;
+  DevBuf d_items, d_ctr, d_fix, d_reach, d_rnorm, d_rnode;
   int item_cap = 0;
   bool any_cosne = false, any_limb = false;
   int fix_cap = 0;

@@ -428,6 +428,23 @@ __device__ __forceinline__ double rebin_bin(double elo_, double ehi_, int clo, i
   return v;
 }
 
+// A fine radius' lerp factor between the node radii K+1 (lo) and K (hi) and
+// its g window, lerped between the nodes' g_min and g_max (src/Relprofile.cpp:
+// 249-300).  k_setup, k_integrate and k_fixup all take them from here (k_setup
+// from its LDS copy of the node arrays, the others from the request's node
+// arrays in global memory), so that the three agree bit for bit.
+struct RadGeo {
+  double ifac_r, gmin, gmax;
+};
+__device__ inline RadGeo rad_geo(const double *tre, const double *tgmin, const double *tgmax, double re, int K) {
+#pragma clang fp contract(off)
+  RadGeo o;
+  o.ifac_r = (re - tre[K + 1]) / (tre[K] - tre[K + 1]);
+  o.gmin = lerp1(o.ifac_r, tgmin[K + 1], tgmin[K]);
+  o.gmax = lerp1(o.ifac_r, tgmax[K + 1], tgmax[K]);
+  return o;
+}
+
 // block-wide deterministic sum (fixed tree order)
 __device__ inline double block_sum(double v, double *red) {
   const int tid = threadIdx.x;
@@ -601,6 +618,14 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     if (tid == 0) b.status[req] = s_st;
     return;
   }
+  {  // the node arrays the radius records are rebuilt from (rad_geo)
+    double *gn = b.rnode + (size_t)req * RAD_NODE_ARRAYS * n_r;
+    for (int ir = tid; ir < n_r; ir += SETUP_THREADS) {
+      gn[ir] = s_tre[ir];
+      gn[n_r + ir] = s_tgmin[ir];
+      gn[2 * n_r + ir] = s_tgmax[ir];
+    }
+  }
   const int ntr = L.n_r;
   if (lp) lp_rows(L, rq, s_jr, s_je, s_jd, s_ji, tid, SETUP_THREADS);
 
@@ -732,17 +757,19 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
       if (s_tre[mid] > re) lo = mid + 1; else hi = mid;
     }
     int K = min(lo - 1, s_rmin);
-    if (K < 0) {
-      if (re - 1000.0 <= 1e-6) K = 0; else err = 13;
+    if (K < 0) {  // (re beyond the outermost node: the request fails, K stays in bounds)
+      if (!(re - 1000.0 <= 1e-6)) err = 13;
+      K = 0;
     }
     if (K >= n_r - 1) {  // cannot happen for rin >= tre[n_r-1]; keep in bounds
       K = n_r - 2;
       err = 13;
     }
-    const double ifac_r = (re - s_tre[K + 1]) / (s_tre[K] - s_tre[K + 1]);
+    const RadGeo geo = rad_geo(s_tre, s_tgmin, s_tgmax, re, K);
+    const double ifac_r = geo.ifac_r;
     if (ifac_r > 1.0 && K > 0) err = 14;
-    const double gmin = lerp1(ifac_r, s_tgmin[K + 1], s_tgmin[K]);
-    const double gmax = lerp1(ifac_r, s_tgmax[K + 1], s_tgmax[K]);
+    const double gmin = geo.gmin;
+    const double gmax = geo.gmax;
     const double tz = trapez_half(rprev, re, rnext, ii, nr);
     const double weight = tz / 2;
     double emis;
@@ -772,20 +799,12 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     }
     // zone of the radius (binary_search on the zone edges, src/Relprofile.cpp:874)
     izone = (nz > 1) ? bsearch_asc(zg, nz + 1, re) : 0;
-    RadRec r;
-    r.gmin = gmin;
-    r.gmax = gmax;
-    r.coef = emis / (gmax - gmin) * weight;  // times ReqNorm.fac
-    r.ifac_r = ifac_r;
-    r.K = K;
-    r.ielo = ielo;
-    r.iehi = iehi;
-    r.pad = izone;
-    rad[ii] = r;
+    const double coef = emis / (gmax - gmin) * weight;  // times ReqNorm.fac
+    rad[ii] = rad_pack(coef, K, izone, ielo, iehi);
     if (ielo <= iehi) {
       wlo = ielo;
       whi = iehi;
-      cmax = fmax(cmax, fabs(r.coef));
+      cmax = fmax(cmax, fabs(coef));
     }
     }  // ii < nr
     // energy window of the radius chunk (this wavefront's 64 consecutive
@@ -818,6 +837,8 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     n.fm = fm;
     n.ia = ia;
     n.im = im;
+    n.r1 = fine_radius_r1(rq.rout);
+    n.r2 = fine_radius_r1(rq.rin);
     const double m = cmax_all * fac;
     int k = 0;
     if (m > 0.0 && m < INFINITY) k = min(1000, max(-1000, -ilogb(m)));
@@ -923,15 +944,18 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     for (int i0 = 0; i0 < nr; i0 += SETUP_THREADS) {
       const int ii = i0 + tid;
       bool live = ii < nr;
-      RadRec r = {};
+      RadRec rec = rad_pack(0.0, 0, 0, 1, 0);
       if (live) {
-        r = rad[ii];
-        live = r.ielo <= r.iehi;
+        rec = rad[ii];
+        live = rad_ielo(rec) <= rad_iehi(rec);
       }
       if (__ballot(live) == 0) continue;
       const double re = live ? fine_radius(rq.rin, rq.rout, ii, nr) : 0.0;
-      const double emisw = r.coef * (r.gmax - r.gmin) * fac;  // emis * weight
-      const double *tlo = node + (size_t)(r.K + 1) * cols, *thi = node + (size_t)r.K * cols;
+      const int K = rad_K(rec), rzone = rad_zone(rec);
+      RadGeo r = {0.0, 0.0, 0.0};
+      if (live) r = rad_geo(s_tre, s_tgmin, s_tgmax, re, K);
+      const double emisw = rec.coef * (r.gmax - r.gmin) * fac;  // emis * weight
+      const double *tlo = node + (size_t)(K + 1) * cols, *thi = node + (size_t)K * cols;
       for (int q = 0; q < cols; q++) {
         const int g = q >> 1;
         const double gs = s_gs[g];
@@ -948,7 +972,7 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
           if (imu < 0 || imu >= rq.n_incl) derr = 40;
           else if (tmp != tmp) derr = 41;
           else ok = true;
-          imu += r.pad * rq.n_incl;  // the radius' zone
+          imu += rzone * rq.n_incl;  // the radius' zone
         }
         const unsigned long long m_ok = __ballot(ok);
         if (m_ok == 0) continue;
@@ -1317,8 +1341,8 @@ __device__ inline void stage_table_rows(const NodeSrc &s, int kb, int tot, int n
 // between the bin's clamped ends, whose brackets bound theirs; one bracket of
 // margin either side) and for the brackets of the two edge norms; the other
 // columns of the row are not read.
-__device__ inline void load_radius_table(const BatchDev &b, const ReqNorm &rn, const RadRec &r, int limb, double e0,
-                                         double e1, double *myT, double *myC, RadiusView &v) {
+__device__ inline void load_radius_table(const BatchDev &b, const ReqNorm &rn, const RadGeo &r, int K, int limb,
+                                         double e0, double e1, double *myT, double *myC, RadiusView &v) {
   v.gmin = r.gmin;
   v.gmax = r.gmax;
   v.del_g = 1. / (r.gmax - r.gmin);  // set_str_relbf, src/Relprofile.cpp:733
@@ -1333,8 +1357,8 @@ __device__ inline void load_radius_table(const BatchDev &b, const ReqNorm &rn, c
   auto fill = [&](int ia, int ib) {  // columns of brackets ia .. ib
 #pragma unroll 4
     for (int c = 2 * ia; c <= 2 * ib + 3; c++) {
-      myT[c] = lerp1_(r.ifac_r, node_value(ts, r.K + 1, c), node_value(ts, r.K, c));
-      if (limb) myC[c] = lerp1_(r.ifac_r, node_value(cs, r.K + 1, c), node_value(cs, r.K, c));
+      myT[c] = lerp1_(r.ifac_r, node_value(ts, K + 1, c), node_value(ts, K, c));
+      if (limb) myC[c] = lerp1_(r.ifac_r, node_value(cs, K + 1, c), node_value(cs, K, c));
     }
   };
   fill(max(0, bracket(e0) - 1), min(ng - 2, bracket(e1) + 1));
@@ -1454,21 +1478,27 @@ __global__ __launch_bounds__(64, MODE == 0 ? 1 : 3) void k_integrate(BatchDev b,
     const int limb = COS ? rq->limb : 0;
     const int i = (it.rc & 0xffff) * RAD_CHUNK + lane;
     const int zone = it.rc >> 16;
-    RadRec r;
-    r.ielo = 1;
-    r.iehi = 0;
-    r.coef = 0.0;
-    r.K = 0;
+    // the lane's radius: the stored record, and its geometry rebuilt from the
+    // request's node arrays
+    int rK = 0, ielo = 1, iehi = 0;
+    double coef = 0.0;
     if (i < nr) {
-      r = b.rad[(size_t)it.req * nr + i];
-      if (r.pad != zone) {  // a radius of another zone of the chunk
-        r.ielo = 1;
-        r.iehi = 0;
+      const RadRec pr = b.rad[(size_t)it.req * nr + i];
+      if (rad_zone(pr) == zone) {  // (else a radius of another zone of the chunk)
+        ielo = rad_ielo(pr);
+        iehi = rad_iehi(pr);
       }
+      rK = rad_K(pr);
+      coef = pr.coef;
     }
-    const bool has_win = r.ielo <= r.iehi;
-    const int kmin = wave_min_i(has_win ? r.K : INT_MAX);
-    const int kmax = wave_max_i(has_win ? r.K : -1);
+    const bool has_win = ielo <= iehi;
+    RadGeo r = {0.0, 0.0, 0.0};
+    if (has_win) {
+      const double *gn = b.rnode + (size_t)it.req * RAD_NODE_ARRAYS * b.rt.n_r;
+      r = rad_geo(gn, gn + b.rt.n_r, gn + 2 * b.rt.n_r, fine_radius_at(rn.r1, rn.r2, i, nr), rK);
+    }
+    const int kmin = wave_min_i(has_win ? rK : INT_MAX);
+    const int kmax = wave_max_i(has_win ? rK : -1);
     if (kmax >= 0) {
       // the lanes' rows from node rows kb .. ke + 1 built in LDS, stage_rows
       // at a time (lanes with K in [kb, ke] are served per round), for the
@@ -1490,9 +1520,9 @@ __global__ __launch_bounds__(64, MODE == 0 ? 1 : 3) void k_integrate(BatchDev b,
       for (int kb = kmin; kb <= kmax; kb += stage_rows - 1) {
         double tL[4] = {0, 0, 0, 0}, tU[4] = {0, 0, 0, 0}, cL[4] = {0, 0, 0, 0}, cU[4] = {0, 0, 0, 0};
         const int ke = min(kmax, kb + stage_rows - 2);
-        const bool mine = has_win && r.K >= kb && r.K <= ke;
+        const bool mine = has_win && rK >= kb && rK <= ke;
         const int tot = (ke - kb + 2) * nq;
-        const int klo = mine ? r.K + 1 - kb : 1, khi = mine ? r.K - kb : 0;
+        const int klo = mine ? rK + 1 - kb : 1, khi = mine ? rK - kb : 0;
         const double *lo = stage + (size_t)klo * nq;
         const double *hi = stage + (size_t)khi * nq;
         stage_table_rows<5>(ts, kb, tot, nq, stage, lane);
@@ -1527,8 +1557,8 @@ __global__ __launch_bounds__(64, MODE == 0 ? 1 : 3) void k_integrate(BatchDev b,
         }
       }
     }
-    const double w64 = has_win ? r.coef * rn.fac : 0.0;   // MODE 0 weight
-    const double w32 = has_win ? r.coef * rn.cmul : 0.0;  // MODE 1 scaled weight
+    const double w64 = has_win ? coef * rn.fac : 0.0;   // MODE 0 weight
+    const double w32 = has_win ? coef * rn.cmul : 0.0;  // MODE 1 scaled weight
     const float w32f = (float)w32;
     if constexpr (MODE == 0) s_coef[lane] = w64;
     const double *E = profile_grid(b, rq);
@@ -1547,7 +1577,7 @@ __global__ __launch_bounds__(64, MODE == 0 ? 1 : 3) void k_integrate(BatchDev b,
         enext = E[min(j + 2, it.j1)];  // next bin's upper edge, loaded a bin ahead
         ghi = (ehi - v.gmin) * v.del_g;
         TV val = 0;
-        if (j >= r.ielo && j <= r.iehi) {
+        if (j >= ielo && j <= iehi) {
           if constexpr (MODE == 0) {
             val = integ_bin_g<0>(elo, ehi, glo, ghi, v, nullptr, &cy);
           } else {
@@ -1618,11 +1648,15 @@ __global__ __launch_bounds__(64) void k_fixup(BatchDev b, int cos_rows) {
   for (int e = blockIdx.x * 64 + lane; e < n; e += gridDim.x * 64) {
     const FixEntry fe = b.fix[e];
     const ReqParams *rq = b.req + fe.req;
-    const RadRec r = b.rad[(size_t)fe.req * b.nr_fine + fe.rad];
+    const RadRec pr = b.rad[(size_t)fe.req * b.nr_fine + fe.rad];
+    const double *gn = b.rnode + (size_t)fe.req * RAD_NODE_ARRAYS * b.rt.n_r;
+    const ReqNorm &rn = b.rnorm[fe.req];
+    const RadGeo r = rad_geo(gn, gn + b.rt.n_r, gn + 2 * b.rt.n_r, fine_radius_at(rn.r1, rn.r2, fe.rad, b.nr_fine),
+                             rad_K(pr));
     const double *E = profile_grid(b, rq);
-    load_radius_table(b, b.rnorm[fe.req], r, rq->limb, E[fe.bin], E[fe.bin + 1], myT, myC, v);
-    const double val = integ_bin<0>(E[fe.bin], E[fe.bin + 1], v) * (r.coef * b.rnorm[fe.req].fac);
-    unsafeAtomicAdd(&profile_acc(b, *rq)[(size_t)r.pad * rq->n_e + fe.bin], val - fe.v32);
+    load_radius_table(b, b.rnorm[fe.req], r, rad_K(pr), rq->limb, E[fe.bin], E[fe.bin + 1], myT, myC, v);
+    const double val = integ_bin<0>(E[fe.bin], E[fe.bin + 1], v) * (pr.coef * b.rnorm[fe.req].fac);
+    unsafeAtomicAdd(&profile_acc(b, *rq)[(size_t)rad_zone(pr) * rq->n_e + fe.bin], val - fe.v32);
   }
 }
 

@@ -173,6 +173,7 @@ int noalloc_eval(int model, const double *parameter, const double *d_energy, con
   bd.pgrid = conv ? nullptr : c.take<double>((size_t)n_flux + 1);
   bd.node = nullptr;  // line and convolution requests read no node rows
   bd.rad = c.take<RadRec>((size_t)g.nr_fine);
+  bd.rnode = c.take<double>((size_t)RAD_NODE_ARRAYS * g.rel.n_r);
   bd.rnorm = c.take<ReqNorm>(1);
   bd.prof = conv ? c.take<double>((size_t)g.conv_n) : nullptr;
   bd.dist = c.take<double>(1);

@@ -13,7 +13,9 @@
 //                       convolution grid instead)
 //   node[n]             (a,mu0)-interpolated table node, FP64:
 //                       trff[n_r][n_g][2] followed by cosne[n_r][n_g][2]
-//   rad[n][nr_fine]     per fine radius record (RadRec)
+//   rad[n][nr_fine]     per fine radius record (RadRec, 16 bytes)
+//   rnode[n][3][n_r]    per request: the (a, mu0)-interpolated node radii,
+//                       g_min and g_max that the records' geometry is rebuilt from
 //   prof                per request profile on the conv grid (conv models)
 //   items               work list written by k_setup: (request, chunk of 64
 //                       fine radii, segment of at most seg_bins bins)
@@ -133,17 +135,34 @@ struct XillWork {
   int req, zone;
 };
 
-// One fine radius (64 bytes).  The bin integral at this radius is
+// One fine radius as k_setup stores it (16 bytes).  The bin integral at this
+// radius is
 //   flux[j] += integ_bin(E_j, E_j+1) * coef,   coef = emis*weight/(gmax-gmin)
+// The radius itself, its lerp factor between the node radii and its g window
+// are not stored: rad_geo rebuilds them from the radius index, the bracket K
+// and the request's node arrays (RAD_NODE_ARRAYS x n_r doubles per request:
+// node radii, g_min, g_max of the nodes), bit for bit as k_setup computed them.
 struct RadRec {
-  double gmin, gmax;  // energy window (line-energy units); del_g = 1 / (gmax - gmin)
-  double coef;        // emis / (gmax - gmin) * weight, times ReqNorm.fac
-  double ifac_r;      // lerp factor between the node radii K+1 (lo) and K (hi)
-  int K;       // table-node bracket: values lerp between node K+1 (lo) and K (hi)
-  int ielo;    // first bin of the window (ielo > iehi: radius does not contribute)
-  int iehi;    // last bin of the window (inclusive)
-  int pad;
+  double coef;  // emis / (gmax - gmin) * weight, times ReqNorm.fac
+  uint32_t lo;  // first bin of the window (bits 0-23) | K << 24
+  uint32_t hi;  // last bin of the window, inclusive (bits 0-23) | zone << 24
 };
+constexpr int RAD_BIN_BITS = 24;  // bins of a profile grid: fewer than 2^24
+constexpr uint32_t RAD_BIN_MASK = (1u << RAD_BIN_BITS) - 1;
+constexpr int RAD_NODE_ARRAYS = 3;
+// (ielo > iehi: the radius does not contribute)
+__host__ __device__ inline int rad_ielo(const RadRec &r) { return (int)(r.lo & RAD_BIN_MASK); }
+__host__ __device__ inline int rad_iehi(const RadRec &r) { return (int)(r.hi & RAD_BIN_MASK); }
+__host__ __device__ inline int rad_K(const RadRec &r) { return (int)(r.lo >> RAD_BIN_BITS); }
+__host__ __device__ inline int rad_zone(const RadRec &r) { return (int)(r.hi >> RAD_BIN_BITS); }
+__host__ __device__ inline RadRec rad_pack(double coef, int K, int zone, int ielo, int iehi) {
+  RadRec r;
+  r.coef = coef;
+  r.lo = (uint32_t)ielo | ((uint32_t)K << RAD_BIN_BITS);
+  r.hi = (uint32_t)iehi | ((uint32_t)zone << RAD_BIN_BITS);
+  return r;
+}
+static_assert(MAX_NR_TAB <= 256 && MAX_ZONES <= 256, "K and the zone take 8 bits of a radius record");
 
 // Per-request factors of the radius weights.  RadRec.coef holds the weight
 // without the emissivity normalisation; fac is that normalisation (1/area for
@@ -158,6 +177,8 @@ struct ReqNorm {
   // k_integrate and k_fixup build the node rows they need from these
   double fa, fm;
   int ia, im;
+  // 1/sqrt(Rout) and 1/sqrt(Rin) of the fine radial grid (fine_radius_at)
+  double r1, r2;
 };
 
 struct LpRefl {
@@ -302,6 +323,7 @@ struct BatchDev {
   double *pgrid;
   double *node;
   RadRec *rad;
+  double *rnode;  // per request: RAD_NODE_ARRAYS * rt.n_r doubles (node radii, g_min, g_max)
   ReqNorm *rnorm;
   double *prof;
   double *dist;

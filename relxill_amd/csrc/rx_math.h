@@ -698,14 +698,17 @@ RX_HD inline int bsearch_desc(const T *arr, int n, V val) {
   return klo;
 }
 
-// fine radial grid point ii (src/relutility.c:666-677)
-RX_HD inline double fine_radius(double rin, double rout, int ii, int nr) {
+// fine radial grid point ii (src/relutility.c:666-677), from r1 = 1/sqrt(Rout)
+// and r2 = 1/sqrt(Rin)
+RX_HD inline double fine_radius_at(double r1, double r2, int ii, int nr) {
 #pragma clang fp contract(off)
-  const double r1 = 1.0 / sqrt(rout);
-  const double r2 = 1.0 / sqrt(rin);
   double x = ((double)(ii)) * (r2 - r1) / (nr - 1) + r1;
   const double y = 1.0 / x;
   return y * y;
+}
+RX_HD inline double fine_radius_r1(double rout) { return 1.0 / sqrt(rout); }
+RX_HD inline double fine_radius(double rin, double rout, int ii, int nr) {
+  return fine_radius_at(fine_radius_r1(rout), fine_radius_r1(rin), ii, nr);
 }
 
 // half of 2 pi r dr of a descending grid (src/relutility.c:233-244)

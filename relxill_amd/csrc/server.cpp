@@ -103,7 +103,7 @@ int64_t request_bytes(const Pending &p) {
   b += 8 * (n_e + 1) + 8 * n_e * nz;
   b += (int64_t)sizeof(WorkItem) * (n_rc + nz - 1) * ((n_e + SERVER_MIN_SEG_BINS - 1) / SERVER_MIN_SEG_BINS);
   b += (int64_t)sizeof(FixEntry) * (n_e / 8 + 1);
-  b += (int64_t)sizeof(RadRec) * g.nr_fine;
+  b += (int64_t)sizeof(RadRec) * g.nr_fine + 8 * RAD_NODE_ARRAYS * (int64_t)g.rel.n_r;
   b += 8 * 2 * (int64_t)g.rel.n_r * g.rel.n_g * 2;
   b += 8 * MAX_INCL * nz + sizeof(LpRefl) + sizeof(ReqNorm) + 64;
   if (relxill) {
