@@ -735,32 +735,43 @@ int rx_batch_autotune(rx_batch *b, int precision) {
   std::string err;
   const int segs[] = {32, 64, 128, 256};
   const int wpcs[] = {4, 8, 12, 16};
-  double best = 1e30;
-  int best_s = 64, best_w = 8;
+  struct Cand {
+    int seg, wpc;
+    double dt;
+  };
+  std::vector<Cand> cands;
+  // the fastest of `trials` timings of `reps` runs each of one shape
+  auto time_shape = [&](int sg, int w, int trials, int reps, double &dt) {
+    batch_plan(b->b, sg, w);
+    if (batch_upload(b->b) != RX_OK) return false;
+    if (batch_run(b->b, precision, err) != RX_OK || batch_sync(b->b) != RX_OK) return false;
+    for (int trial = 0; trial < trials; trial++) {
+      auto t0 = std::chrono::steady_clock::now();
+      for (int k = 0; k < reps; k++) {
+        if (batch_run(b->b, precision, err) != RX_OK) return false;
+      }
+      if (batch_sync(b->b) != RX_OK) return false;
+      dt = std::min(dt, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / reps);
+    }
+    return true;
+  };
   for (int w : wpcs) {
     for (int sg : segs) {
-      batch_plan(b->b, sg, w);
-      if (batch_upload(b->b) != RX_OK) return -RX_ERR_DEVICE;
-      if (batch_run(b->b, precision, err) != RX_OK || batch_sync(b->b) != RX_OK) return -RX_ERR_DEVICE;
-      // the fastest of three timings of two runs each: shapes a few percent
-      // apart are told apart despite the host timer's noise
       double dt = 1e30;
-      for (int trial = 0; trial < 3; trial++) {
-        auto t0 = std::chrono::steady_clock::now();
-        const int reps = 2;
-        for (int k = 0; k < reps; k++) {
-          if (batch_run(b->b, precision, err) != RX_OK) return -RX_ERR_DEVICE;
-        }
-        if (batch_sync(b->b) != RX_OK) return -RX_ERR_DEVICE;
-        dt = std::min(dt, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / reps);
-      }
-      if (dt < best) {
-        best = dt;
-        best_s = sg;
-        best_w = w;
-      }
+      if (!time_shape(sg, w, 3, 2, dt)) return -RX_ERR_DEVICE;
+      cands.push_back({sg, w, dt});
     }
   }
+  // the three fastest timed again, in turns, with more runs: shapes within a
+  // few percent of each other are told apart despite the host timer's noise
+  std::sort(cands.begin(), cands.end(), [](const Cand &x, const Cand &y) { return x.dt < y.dt; });
+  cands.resize(std::min<size_t>(3, cands.size()));
+  for (auto &c : cands) c.dt = 1e30;
+  for (int round = 0; round < 3; round++)
+    for (auto &c : cands)
+      if (!time_shape(c.seg, c.wpc, 2, 3, c.dt)) return -RX_ERR_DEVICE;
+  std::sort(cands.begin(), cands.end(), [](const Cand &x, const Cand &y) { return x.dt < y.dt; });
+  const int best_s = cands[0].seg, best_w = cands[0].wpc;
   bool found = false;
   for (auto &t : g.tune)
     if (t.shape == b->b.shape_class) {
