@@ -237,8 +237,10 @@ int rx_lptable_save(const char *path);
 int rx_tables_load_fits(const char *path);
 int rx_lptable_load_fits(const char *path);
 
-/* xillver reflection table (five parameters, the last the inclination;
- * the reference's xillver-a-Ec5.fits, src/common.h:163).  Loaded from
+/* xillver reflection table (five or six parameters, the last the
+ * inclination; the reference's xillver-a-Ec5.fits, src/common.h:163; a sixth
+ * parameter is the first and is interpolated outside the other five, as in
+ * interp_6d_tab_incl, src/xilltable.c:997-1045).  Loaded from
  * RELXILL_TABLE_PATH/xillver-a-Ec5.fits on the first relxill request when
  * present, else a synthetic table (13 Gamma x 5 A_Fe x 8 logXi x 6 Ecut x 10
  * inclinations, 2999 log bins over 0.07-1000 keV).  The FITS loader reads
@@ -250,9 +252,9 @@ int rx_xilltable_synthetic(uint64_t seed, int n_gam, int n_afe, int n_lxi, int n
 int rx_xilltable_load_fits(const char *path);
 typedef struct {
   int n_par;
-  int pid[5];   /* parameter ids: 0 Gamma, 1 A_Fe, 2 logXi, 3 Ecut, 4 Dens, 7 Incl */
-  int n_val[5];
-  const float *val[5];
+  int pid[6];   /* parameter ids: 0 Gamma, 1 A_Fe, 2 logXi, 3 Ecut (kTe), 4 Dens, 5 kTbb, 7 Incl */
+  int n_val[6];
+  const float *val[6];
   int n_ener;
   const float *elo, *ehi;
   const float *spec;  /* rows as stored in the file (before the 10^logXi division) */
@@ -264,10 +266,14 @@ int rx_xilltable_get_view(rx_xilltable_view *v);
  * get_init_xillver_table, src/xilltable.c:733-740), held beside the one
  * above.  Loaded from RELXILL_TABLE_PATH/xillverCp_v3.4.fits on the first
  * relxillCp / relxilllpCp request when present, else a synthetic table as
- * above with its fourth axis kTe 1-400 keV.  Five-parameter tables only: a
- * six-parameter table (with a density axis, as the released
- * xillverCp_v3.4.fits) is rejected with RX_ERR_TABLE. */
+ * above with its fourth axis kTe 1-400 keV.  Five- and six-parameter tables
+ * load; kTe may not be the first of six.
+ * rx_xillcptable_synthetic_dens: a seeded six-parameter table, Gamma, A_Fe,
+ * logXi, kTe, Dens (15-19), Incl; no counterpart in the reference (a test
+ * table for the six-parameter path). */
 int rx_xillcptable_synthetic(uint64_t seed, int n_gam, int n_afe, int n_lxi, int n_kte, int n_incl, int n_ener);
+int rx_xillcptable_synthetic_dens(uint64_t seed, int n_dens, int n_gam, int n_afe, int n_lxi, int n_kte, int n_incl,
+                                  int n_ener);
 int rx_xillcptable_load_fits(const char *path);
 int rx_xillcptable_get_view(rx_xilltable_view *v);
 

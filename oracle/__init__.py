@@ -74,7 +74,7 @@ class SysPar(ctypes.Structure):
 
 
 class XillTable(ctypes.Structure):
-    _fields_ = [("n_par", _I), ("pid", _I * 5), ("n_val", _I * 5), ("val", _F * 5), ("n_ener", _I),
+    _fields_ = [("n_par", _I), ("pid", _I * 6), ("n_val", _I * 6), ("val", _F * 6), ("n_ener", _I),
                 ("elo", _F), ("ehi", _F), ("spec", _F)]
 
 
@@ -438,8 +438,8 @@ def blackbody(ener, ktbb):
 
 def xill_corner(xt, xp):
     """(indices, factors) of get_xilltab_indices_for_paramvals / interp_xill_table"""
-    ind = (_I * 5)()
-    fac = np.zeros(5)
+    ind = (_I * 6)()
+    fac = np.zeros(6)
     lib.orc_xill_indices(ctypes.byref(xt.t), ctypes.byref(xp), ind)
     lib.orc_xill_factors(ctypes.byref(xt.t), ctypes.byref(xp), ind, _dp(fac))
     return np.array(ind[:xt.t.n_par]), fac[:xt.t.n_par]

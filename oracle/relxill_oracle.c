@@ -1367,46 +1367,93 @@ static const char corner[16][4] = {
     {0, 1, 1, 0}, {1, 1, 1, 0}, {0, 0, 0, 1}, {1, 0, 0, 1}, {0, 1, 0, 1}, {0, 0, 1, 1},
     {1, 1, 0, 1}, {1, 0, 1, 1}, {0, 1, 1, 1}, {1, 1, 1, 1}};
 
+/* interp_5d_tab_incl (src/xilltable.c:812-876) over the four axes after the
+ * first `off` ones (which sit at idx0[]), for inclination ii: corners in the
+ * reference's order, weights as its products */
+static void xill_interp4(const orc_xilltable *t, const orc_xillparam *x, int off, const int *idx0, const int *ind,
+                         const double *fac, int ii, float *rows, double *out) {
+  const int n = t->n_ener;
+  double w[16];
+  for (int c = 0; c < 16; c++) {
+    double f[4];
+    for (int k = 0; k < 4; k++) f[k] = corner[c][k] ? fac[off + k] : (1.0 - fac[off + k]);
+    w[c] = f[0] * f[1] * f[2] * f[3];
+  }
+  for (int c = 0; c < 16; c++) {
+    int idx[6];
+    for (int k = 0; k < off; k++) idx[k] = idx0[k];
+    for (int k = 0; k < 4; k++) idx[off + k] = ind[off + k] + corner[c][k];
+    idx[off + 4] = ii;
+    xill_row(t, x, idx, rows + (size_t)c * n);
+  }
+  for (int e = 0; e < n; e++) {
+    double v = 0.0;
+    for (int c = 0; c < 16; c++) v = v + w[c] * (double)rows[(size_t)c * n + e];
+    out[e] = v;
+  }
+}
+
 void orc_xill_spectra(const orc_xilltable *t, const orc_xillparam *x, double *ener, double *flu) {
   const int n = t->n_ener, n_incl = t->n_val[t->n_par - 1];
   for (int ii = 0; ii < n; ii++) ener[ii] = t->elo[ii];
   ener[n] = t->ehi[n - 1];
-  int ind[5];
-  double fac[5];
+  int ind[6];
+  double fac[6];
   orc_xill_indices(t, x, ind);
   orc_xill_factors(t, x, ind, fac);
-  /* interp_5d_tab_incl (src/xilltable.c:812-876), for every inclination:
-   * corners in the reference's order, weights as its products */
-  double w[16];
-  for (int c = 0; c < 16; c++) {
-    double f[4];
-    for (int k = 0; k < 4; k++) f[k] = corner[c][k] ? fac[k] : (1.0 - fac[k]);
-    w[c] = f[0] * f[1] * f[2] * f[3];
-  }
   float *rows = (float *)malloc(sizeof(float) * 16 * (size_t)n);
-  for (int ii = 0; ii < n_incl; ii++) {
-    for (int c = 0; c < 16; c++) {
-      int idx[5];
-      for (int k = 0; k < 4; k++) idx[k] = ind[k] + corner[c][k];
-      idx[4] = ii;
-      xill_row(t, x, idx, rows + (size_t)c * n);
+  if (t->n_par == 5) {
+    for (int ii = 0; ii < n_incl; ii++) xill_interp4(t, x, 0, NULL, ind, fac, ii, rows, flu + (size_t)ii * n);
+  } else {
+    /* interp_6d_tab_incl (src/xilltable.c:997-1020): the five-parameter
+     * interpolation at both brackets of the first axis, then
+     * interp_lin_1d(fac[0], s1, s2) */
+    double *s1 = (double *)malloc(sizeof(double) * 2 * (size_t)n), *s2 = s1 + n;
+    for (int ii = 0; ii < n_incl; ii++) {
+      int i0 = ind[0];
+      xill_interp4(t, x, 1, &i0, ind, fac, ii, rows, s1);
+      i0 = ind[0] + 1;
+      xill_interp4(t, x, 1, &i0, ind, fac, ii, rows, s2);
+      double *out = flu + (size_t)ii * n;
+      for (int e = 0; e < n; e++) out[e] = fac[0] * s2[e] + (1.0 - fac[0]) * s1[e];
     }
-    double *out = flu + (size_t)ii * n;
-    for (int e = 0; e < n; e++) {
-      double v = 0.0;
-      for (int c = 0; c < 16; c++) v = v + w[c] * (double)rows[(size_t)c * n + e];
-      out[e] = v;
-    }
+    free(s1);
   }
   free(rows);
+}
+
+/* interp_5d_tab (:878-995) over the axes after the first `off` ones (at
+ * idx0[]): 32 corners, the inclination's the fifth factor of each weight
+ * product, summed in the reference's order */
+static void xill_interp5(const orc_xilltable *t, const orc_xillparam *x, int off, const int *idx0, const int *ind,
+                         const double *fac, float *rows, double *out) {
+  const int n = t->n_ener, ki = t->n_par - 1;
+  double w[32];
+  for (int b5 = 0; b5 < 2; b5++) {
+    for (int c = 0; c < 16; c++) {
+      double f[4];
+      for (int k = 0; k < 4; k++) f[k] = corner[c][k] ? fac[off + k] : (1.0 - fac[off + k]);
+      w[16 * b5 + c] = f[0] * f[1] * f[2] * f[3] * (b5 ? fac[ki] : (1 - fac[ki]));
+      int idx[6];
+      for (int k = 0; k < off; k++) idx[k] = idx0[k];
+      for (int k = 0; k < 4; k++) idx[off + k] = ind[off + k] + corner[c][k];
+      idx[ki] = ind[ki] + b5;
+      xill_row(t, x, idx, rows + (size_t)(16 * b5 + c) * n);
+    }
+  }
+  for (int e = 0; e < n; e++) {
+    double v = 0.0;
+    for (int c = 0; c < 32; c++) v = v + w[c] * (double)rows[(size_t)c * n + e];
+    out[e] = v;
+  }
 }
 
 void orc_xill_spectrum_incl(const orc_xilltable *t, const orc_xillparam *x, double *ener, double *flu) {
   const int n = t->n_ener, ki = t->n_par - 1;
   for (int ii = 0; ii < n; ii++) ener[ii] = t->elo[ii];
   ener[n] = t->ehi[n - 1];
-  int ind[5];
-  double fac[5];
+  int ind[6];
+  double fac[6];
   orc_xill_indices(t, x, ind);
   orc_xill_factors(t, x, ind, fac);
   /* the xillver models clamp the inclination into the table as well
@@ -1420,25 +1467,18 @@ void orc_xill_spectrum_incl(const orc_xilltable *t, const orc_xillparam *x, doub
     const float den = t->val[ki][ind[ki] + 1] - t->val[ki][ind[ki]];
     fac[ki] = (float)(num / den);
   }
-  /* interp_5d_tab (:878-995): 32 corners, the inclination's the fifth
-   * factor of each weight product, summed in the reference's order */
-  double w[32];
   float *rows = (float *)malloc(sizeof(float) * 32 * (size_t)n);
-  for (int b5 = 0; b5 < 2; b5++) {
-    for (int c = 0; c < 16; c++) {
-      double f[4];
-      for (int k = 0; k < 4; k++) f[k] = corner[c][k] ? fac[k] : (1.0 - fac[k]);
-      w[16 * b5 + c] = f[0] * f[1] * f[2] * f[3] * (b5 ? fac[ki] : (1 - fac[ki]));
-      int idx[5];
-      for (int k = 0; k < 4; k++) idx[k] = ind[k] + corner[c][k];
-      idx[4] = ind[ki] + b5;
-      xill_row(t, x, idx, rows + (size_t)(16 * b5 + c) * n);
-    }
-  }
-  for (int e = 0; e < n; e++) {
-    double v = 0.0;
-    for (int c = 0; c < 32; c++) v = v + w[c] * (double)rows[(size_t)c * n + e];
-    flu[e] = v;
+  if (t->n_par == 5) {
+    xill_interp5(t, x, 0, NULL, ind, fac, rows, flu);
+  } else {
+    /* interp_6d_tab (src/xilltable.c:1022-1045) */
+    double *s1 = (double *)malloc(sizeof(double) * 2 * (size_t)n), *s2 = s1 + n;
+    int i0 = ind[0];
+    xill_interp5(t, x, 1, &i0, ind, fac, rows, s1);
+    i0 = ind[0] + 1;
+    xill_interp5(t, x, 1, &i0, ind, fac, rows, s2);
+    for (int e = 0; e < n; e++) flu[e] = fac[0] * s2[e] + (1.0 - fac[0]) * s1[e];
+    free(s1);
   }
   free(rows);
 }

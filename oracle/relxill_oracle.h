@@ -223,15 +223,16 @@ int orc_relxill_conv(const orc_reltable *tab, const orc_relparam *p, const doubl
 #define ORC_XP_KTB 5
 #define ORC_XP_INC 7
 
-/* a five-parameter xillver table (xillTable, src/xilltable.h), the last
- * parameter the inclination; spec holds the rows as stored in the file
- * (row r at spec[r * n_ener], r = ((((i0 n1 + i1) n2 + i2) n3 + i3) n4 + i4),
- * src/xilltable.c:326-337), before the division by 10^logXi */
+/* a five- or six-parameter xillver table (xillTable, src/xilltable.h), the
+ * last parameter the inclination; spec holds the rows as stored in the file
+ * (row r at spec[r * n_ener], r = ((((i0 n1 + i1) n2 + i2) n3 + i3) n4 + i4)
+ * for five, one more index in front for six, src/xilltable.c:326-337), before
+ * the division by 10^logXi */
 typedef struct {
   int n_par;
-  int pid[5];
-  int n_val[5];
-  const float *val[5];
+  int pid[6];
+  int n_val[6];
+  const float *val[6];
   int n_ener;
   const float *elo, *ehi;
   const float *spec;

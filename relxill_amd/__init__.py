@@ -15,7 +15,8 @@ _LIB_NAMES = {
     "nthcomp", "MODEL_RELXILLCP", "MODEL_RELXILLLPCP", "lmodrelxilldensnthcomp", "lmodrelxilllpdensnthcomp",
     "MODEL_RELXILLNS", "lmodrelxillns", "xillnstable_synthetic",
     "MODEL_XILLVER", "MODEL_XILLVERCP", "MODEL_XILLVERNS", "lmodxillver", "lmodxillverdensnthcomp", "lmodxillverns",
-    "xilltable_synthetic", "xillcptable_synthetic", "xilltable_view", "rradtable_synthetic", "rradtable_view",
+    "xilltable_synthetic", "xillcptable_synthetic", "xillcptable_synthetic_dens", "xilltable_view",
+    "rradtable_synthetic", "rradtable_view",
     "reflset_create", "set_conv_grid", "tables_synthetic", "lptable_synthetic", "tables_view",
     "Server", "RX_OK", "RX_ERR_PARAM", "RX_ERR_UNSUPPORTED", "RX_ERR_TABLE", "RX_ERR_INPUT",
     "RX_ERR_DEVICE", "RX_ERR_ARG",

@@ -89,8 +89,8 @@ class _RelView(ctypes.Structure):
 
 
 class _XillView(ctypes.Structure):
-    _fields_ = [("n_par", ctypes.c_int), ("pid", ctypes.c_int * 5), ("n_val", ctypes.c_int * 5),
-                ("val", _F * 5), ("n_ener", ctypes.c_int), ("elo", _F), ("ehi", _F), ("spec", _F)]
+    _fields_ = [("n_par", ctypes.c_int), ("pid", ctypes.c_int * 6), ("n_val", ctypes.c_int * 6),
+                ("val", _F * 6), ("n_ener", ctypes.c_int), ("elo", _F), ("ehi", _F), ("spec", _F)]
 
 
 class _RradView(ctypes.Structure):
@@ -116,6 +116,7 @@ _EXPORTS = {
     "rx_xilltable_load_fits": (ctypes.c_int, [ctypes.c_char_p]),
     "rx_xilltable_get_view": (ctypes.c_int, [ctypes.POINTER(_XillView)]),
     "rx_xillcptable_synthetic": (ctypes.c_int, [ctypes.c_uint64] + [ctypes.c_int] * 6),
+    "rx_xillcptable_synthetic_dens": (ctypes.c_int, [ctypes.c_uint64] + [ctypes.c_int] * 7),
     "rx_xillcptable_load_fits": (ctypes.c_int, [ctypes.c_char_p]),
     "rx_xillcptable_get_view": (ctypes.c_int, [ctypes.POINTER(_XillView)]),
     "lmodrelxilldensnthcomp": (None, [_D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_char_p]),
@@ -313,6 +314,12 @@ def xilltable_synthetic(seed=5, n_gam=13, n_afe=5, n_lxi=8, n_ect=6, n_incl=10, 
 def xillcptable_synthetic(seed=6, n_gam=13, n_afe=5, n_lxi=8, n_kte=6, n_incl=10, n_ener=2999):
     """Synthetic table of the nthcomp models (fourth axis kTe, 1-400 keV)."""
     _check(lib.rx_xillcptable_synthetic(seed, n_gam, n_afe, n_lxi, n_kte, n_incl, n_ener))
+
+
+def xillcptable_synthetic_dens(seed=8, n_dens=5, n_gam=13, n_afe=5, n_lxi=8, n_kte=6, n_incl=10, n_ener=2999):
+    """Six-parameter synthetic table of the nthcomp models: Gamma, A_Fe,
+    logXi, kTe, Dens (15-19), Incl (Gamma interpolated outside the others)."""
+    _check(lib.rx_xillcptable_synthetic_dens(seed, n_dens, n_gam, n_afe, n_lxi, n_kte, n_incl, n_ener))
 
 
 def xillnstable_synthetic(seed=7, n_ktbb=12, n_afe=5, n_lxi=8, n_dens=5, n_incl=10, n_ener=2999):

@@ -354,7 +354,7 @@ static int xill_get_view(int prim, rx_xilltable_view *v) {
   const XillTableHost &t = x.t;
   memset(v, 0, sizeof(*v));
   v->n_par = t.n_par;
-  for (int k = 0; k < t.n_par && k < 5; k++) {
+  for (int k = 0; k < t.n_par && k < 6; k++) {
     v->pid[k] = t.pid[k];
     v->n_val[k] = t.n_val[k];
     v->val[k] = t.val[k].data();
@@ -374,6 +374,30 @@ int rx_xilltable_get_view(rx_xilltable_view *v) { return xill_get_view(PRIM_CUTO
 
 int rx_xillcptable_synthetic(uint64_t seed, int n_gam, int n_afe, int n_lxi, int n_kte, int n_incl, int n_ener) {
   return xill_synthetic(PRIM_NTHCOMP, seed, n_gam, n_afe, n_lxi, n_kte, n_incl, n_ener);
+}
+int rx_xillcptable_synthetic_dens(uint64_t seed, int n_dens, int n_gam, int n_afe, int n_lxi, int n_kte, int n_incl,
+                                  int n_ener) {
+  Global &g = G();
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (n_dens < 2 || n_gam < 2 || n_afe < 2 || n_lxi < 2 || n_kte < 2 || n_incl < 2 || n_incl > MAX_INCL ||
+      n_ener < 2) {
+    set_last(RX_ERR_ARG, "table dimensions out of range");
+    return RX_ERR_ARG;
+  }
+  XillSlot &x = g.xill[PRIM_NTHCOMP];
+  try {
+    x.t = make_synthetic_xilltable_dens(seed, n_dens, n_gam, n_afe, n_lxi, n_kte, n_incl, n_ener, 1.0, 400.0);
+  } catch (...) {
+    set_last(RX_ERR_NOMEM, "table allocation failed");
+    return RX_ERR_NOMEM;
+  }
+  x.loaded = true;
+  x.err.clear();
+  x.dirty = true;
+  g.config_gen++;
+  std::string e = validate_xilltable(x.t);
+  set_last(e.empty() ? RX_OK : RX_ERR_TABLE, e);
+  return e.empty() ? RX_OK : RX_ERR_TABLE;
 }
 int rx_xillcptable_load_fits(const char *path) { return xill_load_fits(PRIM_NTHCOMP, path); }
 int rx_xillcptable_get_view(rx_xilltable_view *v) { return xill_get_view(PRIM_NTHCOMP, v); }

@@ -275,7 +275,8 @@ static int bsearch_float(const float *arr, int n, float val) {
 }
 
 void xill_corner(const XillTableHost &t, double gam, double afe, double lxi, double ect, double dens, double ktbb,
-                 double incl_deg, int *ind, double *fac, bool clamp_incl, int *inc_ind, double *inc_fac) {
+                 double incl_deg, int *ind, double *fac, bool clamp_incl, int *inc_ind, double *inc_fac, int *out_ind,
+                 double *out_fac) {
   // the input value of every table parameter as float (get_xilltab_paramvals,
   // src/xilltable.c:278-295); kTbb is 0 but for the blackbody model, Frac 0
   float v[8] = {(float)gam, (float)afe, (float)lxi, (float)ect, (float)dens, (float)ktbb, 0.0f, (float)incl_deg};
@@ -309,10 +310,16 @@ void xill_corner(const XillTableHost &t, double gam, double afe, double lxi, dou
     if (ect <= vals[0]) k_fac[k_ect] = 0.0;
     if (ect >= vals[vals.size() - 1]) k_fac[k_ect] = 1.0;
   }
+  // the four inner axes (those after the sixth parameter of a six-parameter
+  // table, which is interpolated outside them: interp_6d_tab_incl,
+  // src/xilltable.c:997-1045), then that outer one and the inclination
+  const int off = t.n_par - 5;
   for (int k = 0; k < 4; k++) {
-    ind[k] = k_ind[k];
-    fac[k] = k_fac[k];
+    ind[k] = k_ind[k + off];
+    fac[k] = k_fac[k + off];
   }
+  if (out_ind) *out_ind = off ? k_ind[0] : 0;
+  if (out_fac) *out_fac = off ? k_fac[0] : 0.0;
   if (inc_ind) *inc_ind = k_ind[t.n_par - 1];
   if (inc_fac) *inc_fac = k_fac[t.n_par - 1];
 }
@@ -723,16 +730,26 @@ void fill_dev_views(BatchDev &bd) {
     const bool ok = x.loaded && t.n_par > 0;
     xt.n_ener = ok ? t.n_ener : 0;
     xt.n_incl = ok ? t.n_incl() : 0;
-    for (int k = 0; k < 4; k++) xt.n_val[k] = ok ? t.n_val[k] : 0;
+    // six parameters: the first is interpolated outside the four inner ones
+    const int off = ok ? t.n_par - 5 : 0;
+    for (int k = 0; k < 4; k++) xt.n_val[k] = ok ? t.n_val[k + off] : 0;
+    xt.n_out = (ok && off) ? t.n_val[0] : 1;
+    xt.outer_rows = 0;
+    if (ok && off) {
+      xt.outer_rows = 1;
+      for (int k = 1; k < t.n_par; k++) xt.outer_rows *= (unsigned)t.n_val[k];
+    }
     xt.spec = x.spec.as<float>();
     xt.ener = x.d_ener.as<double>();
-    xt.ect_dim = ok ? t.dim_of(XP_ECT) : -1;
+    // the Ecut axis among the inner four (never the outer one: validate_xilltable)
+    const int k_ect = ok ? t.dim_of(XP_ECT) : -1;
+    xt.ect_dim = k_ect >= 0 ? k_ect - off : -1;
     xt.ect_q = -1;
     xt.n_ect = xt.n_q = 0;
-    if (xt.ect_dim >= 0) {
-      const int q = t.pid[xt.ect_dim] < t.n_par ? t.pid[xt.ect_dim] : xt.ect_dim;
+    if (k_ect >= 0) {
+      const int q = t.pid[k_ect] < t.n_par ? t.pid[k_ect] : k_ect;  // table axes (x.ect, x.q)
       xt.ect_q = q;
-      xt.n_ect = t.n_val[xt.ect_dim];
+      xt.n_ect = t.n_val[k_ect];
       xt.n_q = t.n_val[q];
     }
     xt.ect_vals = x.ect.as<float>();
@@ -938,7 +955,7 @@ int batch_build(Batch &b, const rx_request *req_in, int n_req, std::string &err)
       const double dens = p.prim != PRIM_CUTOFFPL ? cp[i].dens : 15.0;
       p.xktbb = p.prim == PRIM_BBODY ? cp[i].ktbb : 0.0;
       xill_corner(xt, p.xgam, r.parameter[1], r.parameter[3], p.xect_src, dens, p.xktbb, r.parameter[5], p.xind,
-                  p.xfac, true, &p.xinc_ind, &p.xinc_fac);
+                  p.xfac, true, &p.xinc_ind, &p.xinc_fac, &p.xo_ind, &p.xo_fac);
       p.xdens_div = (xt.dim_of(XP_DNS) < 0 && std::fabs(dens - 15) > 1e-6) ? std::pow(10, dens - 15) : 0.0;
       p.n_incl = xt.n_incl();
       p.xspec_off = o_xspec;
@@ -975,7 +992,7 @@ int batch_build(Batch &b, const rx_request *req_in, int n_req, std::string &err)
         const double dens = p.prim != PRIM_CUTOFFPL ? cp[i].dens : 15.0;
         p.xktbb = p.prim == PRIM_BBODY ? cp[i].ktbb : 0.0;
         xill_corner(xt, p.xgam, afe, lxi, p.xect_src, dens, p.xktbb, incl_deg_of(r.model, r.parameter), p.xind,
-                    p.xfac);
+                    p.xfac, false, nullptr, nullptr, &p.xo_ind, &p.xo_fac);
         // a table without a density axis divides its rows by 10^(logN - 15)
         // (normalizeXillverSpecLogxiDensity, src/xilltable.c:488-511)
         p.xdens_div = (xt.dim_of(XP_DNS) < 0 && std::fabs(dens - 15) > 1e-6) ? std::pow(10, dens - 15) : 0.0;

@@ -30,12 +30,14 @@ int parse_request(int model, const double *par, bool phys_norm, ReqParams &p, st
 // relxill requests: the xillver table corner and factors of the request's
 // parameters (get_xilltab_indices_for_paramvals and interp_xill_table,
 // src/xilltable.c:297-324, :1090-1125).
-// The xillver models interpolate the inclination too (clamped into the
-// table like the others): clamp_incl, and its bracket and factor in
-// *inc_ind / *inc_fac.
+// ind / fac: the four inner axes.  The xillver models interpolate the
+// inclination too (clamped into the table like the others): clamp_incl, and
+// its bracket and factor in *inc_ind / *inc_fac.  A six-parameter table's
+// first axis (interpolated outside the others) goes to *out_ind / *out_fac
+// (0 and 0 for five parameters).
 void xill_corner(const XillTableHost &t, double gam, double afe, double lxi, double ect, double dens, double ktbb,
                  double incl_deg, int *ind, double *fac, bool clamp_incl = false, int *inc_ind = nullptr,
-                 double *inc_fac = nullptr);
+                 double *inc_fac = nullptr, int *out_ind = nullptr, double *out_fac = nullptr);
 int n_model_params(int model);
 // Edges of the radial zones (RadialGrid::calculate_radial_grid, src/IonGradient.cpp:457-489).
 void rzone_grid(double rmin, double rmax, int nzones, double h, double *rgrid);

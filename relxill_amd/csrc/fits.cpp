@@ -354,7 +354,8 @@ std::string load_xilltable_fits(XillTableHost &t, const std::string &path) {
   // (src/xilltable.c:182-183, :205-206, :219-220)
   ff.move_named("PARAMETERS");
   const long n_par = ff.rows();
-  if (ff.ok() && n_par != XILL_NPAR) return "xillver table " + path + ": only five-parameter tables are supported";
+  if (ff.ok() && n_par != 5 && n_par != 6)  // (new_xillTable, src/xilltable.c:78-81)
+    return "xillver table " + path + ": tables have five or six parameters";
   out.n_par = (int)n_par;
   if (ff.ok()) ff.read_int(9, 1, n_par, out.n_val);
   for (int k = 0; k < out.n_par && ff.ok(); k++) {

@@ -1120,6 +1120,7 @@ __global__ __launch_bounds__(RRC_THREADS) void k_rrad_corr(BatchDev b, const Xil
     row[c] = r * xt.n_incl;
   }
   const int ne = xt.n_ener, n_incl = xt.n_incl;
+  const int n_o = xt.n_out > 1 ? 2 : 1;
   const double *E = xt.ener;
   for (int e = tid; e < ne; e += RRC_THREADS) {
     double acc = 0.0;
@@ -1127,8 +1128,14 @@ __global__ __launch_bounds__(RRC_THREADS) void k_rrad_corr(BatchDev b, const Xil
       const double incl_deg = (double)xt.incl_vals[a] * 180 / PI;
       const double wa = 0.5 * cos(incl_deg * PI / 180) / n_incl;
       double sv = 0.0;
+      for (int o = 0; o < n_o; o++) {  // (a six-parameter table: both brackets of its first axis)
+        const double wo = n_o == 1 ? 1.0 : (o ? rq.xo_fac : 1.0 - rq.xo_fac);
+        const size_t ob = (size_t)(rq.xo_ind + o) * xt.outer_rows;
+        double so = 0.0;
 #pragma unroll
-      for (int c = 0; c < 16; c++) sv += w[c] * xval(xt.spec[(size_t)(row[c] + a) * ne + e], dd);
+        for (int c = 0; c < 16; c++) so += w[c] * xval(xt.spec[(ob + row[c] + a) * ne + e], dd);
+        sv += wo * so;
+      }
       acc += wa * sv;
     }
     s_avg[e] = acc;
@@ -2370,17 +2377,30 @@ __global__ __launch_bounds__(XILL_THREADS, 4) void k_xill1(BatchDev b, const Xil
   double acc[XILL_BINS];
 #pragma unroll
   for (int u = 0; u < XILL_BINS; u++) acc[u] = 0.0;
+  const int n_o = xt.n_out > 1 ? 2 : 1;
   for (int a = 0; a < xt.n_incl; a++) {
     const double d = dist[a];
     if (d == 0.0) continue;
     double s[XILL_BINS];
 #pragma unroll
     for (int u = 0; u < XILL_BINS; u++) s[u] = 0.0;
+    for (int o = 0; o < n_o; o++) {
+      // a six-parameter table: the five-parameter interpolation at both
+      // brackets of its first axis, lerped (interp_6d_tab_incl,
+      // src/xilltable.c:997-1020); five parameters: one pass with weight 1
+      const double wo = n_o == 1 ? 1.0 : (o ? rq.xo_fac : 1.0 - rq.xo_fac);
+      const size_t ob = (size_t)(rq.xo_ind + o) * xt.outer_rows;
+      double so[XILL_BINS];
 #pragma unroll
-    for (int c = 0; c < 16; c++) {
-      const float *src = xt.spec + (size_t)(row[c] + a) * ne;
+      for (int u = 0; u < XILL_BINS; u++) so[u] = 0.0;
 #pragma unroll
-      for (int u = 0; u < XILL_BINS; u++) s[u] += w[c] * xval(src[eb[u]], dd);
+      for (int c = 0; c < 16; c++) {
+        const float *src = xt.spec + (ob + row[c] + a) * ne;
+#pragma unroll
+        for (int u = 0; u < XILL_BINS; u++) so[u] += w[c] * xval(src[eb[u]], dd);
+      }
+#pragma unroll
+      for (int u = 0; u < XILL_BINS; u++) s[u] += wo * so[u];
     }
 #pragma unroll
     for (int u = 0; u < XILL_BINS; u++) acc[u] += d * s[u];
@@ -2435,6 +2455,7 @@ __global__ __launch_bounds__(XILL_THREADS, 3) void k_xillz(BatchDev b, const Xil
     ehi = max(ehi, s_ind[z]);
   }
   const int ni = xt.n_incl;
+  const int n_o = xt.n_out > 1 ? 2 : 1;
   const int n_cand = (ehi - elo + 2) * ni;  // candidate pairs (e, a), e in [elo, ehi + 1]
   // corner rows of the axes other than zax (zax index 0), in rows of ni
   // inclinations; one step on zax moves a row by `stride` rows
@@ -2511,11 +2532,20 @@ __global__ __launch_bounds__(XILL_THREADS, 3) void k_xillz(BatchDev b, const Xil
         double s8[XILL_BINS_Z];
 #pragma unroll
         for (int u = 0; u < XILL_BINS_Z; u++) s8[u] = 0.0;
+        for (int o = 0; o < n_o; o++) {  // (a six-parameter table: both brackets of its first axis)
+          const double wo = n_o == 1 ? 1.0 : (o ? rq.xo_fac : 1.0 - rq.xo_fac);
+          const size_t ob = (size_t)(rq.xo_ind + o) * xt.outer_rows;
+          double so[XILL_BINS_Z];
 #pragma unroll
-        for (int c = 0; c < 8; c++) {
-          const float *src = xt.spec + (size_t)(cb[c] + po) * ne;
+          for (int u = 0; u < XILL_BINS_Z; u++) so[u] = 0.0;
 #pragma unroll
-          for (int u = 0; u < XILL_BINS_Z; u++) s8[u] += w8[c] * xval(src[eb[u]], dd);
+          for (int c = 0; c < 8; c++) {
+            const float *src = xt.spec + (ob + cb[c] + po) * ne;
+#pragma unroll
+            for (int u = 0; u < XILL_BINS_Z; u++) so[u] += w8[c] * xval(src[eb[u]], dd);
+          }
+#pragma unroll
+          for (int u = 0; u < XILL_BINS_Z; u++) s8[u] += wo * so[u];
         }
 #pragma unroll
         for (int z = 0; z < NZ; z++) {

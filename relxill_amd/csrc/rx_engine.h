@@ -111,6 +111,10 @@ struct ReqParams {
   // (the fifth interpolated parameter), a two-bin angle distribution for k_xill
   int xinc_ind, pad_xinc;
   double xinc_fac;
+  // a six-parameter table: the bracket and factor of its first axis, which is
+  // interpolated outside the other five (0 and 0 for five parameters)
+  int xo_ind, pad_xo;
+  double xo_fac;
   // returning radiation (lamp post): 1 add, -1 or 2 replace the direct
   // emissivity; the table spin, its first zone and the number of zones used
   int return_rad, rr_k, rr_klo, rr_n;
@@ -228,7 +232,11 @@ struct NthRes {
 
 struct DevXillTable {
   int n_ener, n_incl;
-  int n_val[4];         // values of the four interpolated parameters
+  int n_val[4];         // values of the four (inner) interpolated parameters
+  // a six-parameter table: n_out values of its first axis, one step of which
+  // moves a row by outer_rows rows (n_out 1, outer_rows 0: five parameters)
+  int n_out;
+  unsigned outer_rows;
   const float *spec;
   const double *ener;   // n_ener + 1 edges (elo, then the last ehi)
   int ect_dim;          // table axis of Ecut (-1: none)

@@ -51,18 +51,19 @@ std::string load_lptable_fits(LpTableHost &t, const std::string &path);
 std::string validate_reltable(const RelTableHost &t);
 std::string validate_lptable(const LpTableHost &t);
 
-// xillver reflection table (the xillTable of src/xilltable.h, five-parameter
-// tables as xillver-a-Ec5.fits): n_par parameters, the last one the
-// inclination; parameter k has the id pid[k] (PARAM_* of src/common.h:141-149)
-// and n_val[k] ascending values in val[k]; the energy grid has n_ener bins
-// [elo, ehi]; spec holds row r = ((((i0 n1 + i1) n2 + i2) n3 + i3) n4 + i4) at
-// spec[r * n_ener] (row number r + 1 of the SPECTRA extension,
-// src/xilltable.c:326-337), as stored in the file (not yet divided by
-// 10^logXi).
+// xillver reflection table (the xillTable of src/xilltable.h): n_par = 5 or 6
+// parameters, the last one the inclination; parameter k has the id pid[k]
+// (PARAM_* of src/common.h:141-149) and n_val[k] ascending values in val[k];
+// the energy grid has n_ener bins [elo, ehi]; spec holds row
+// r = ((((i0 n1 + i1) n2 + i2) n3 + i3) n4 + i4) (five parameters; one more
+// index, outermost, for six) at spec[r * n_ener] (row number r + 1 of the
+// SPECTRA extension, src/xilltable.c:326-337), as stored in the file (not yet
+// divided by 10^logXi).  A sixth parameter is the first one: it is
+// interpolated outside the other five (interp_6d_tab_incl, :997-1045).
 enum XillParamId : int {
   XP_GAM = 0, XP_AFE = 1, XP_LXI = 2, XP_ECT = 3, XP_DNS = 4, XP_KTB = 5, XP_FRA = 6, XP_INC = 7
 };
-constexpr int XILL_NPAR = 5;
+constexpr int XILL_NPAR = 6;  // parameters of a table at most
 
 struct XillTableHost {
   int n_par = 0;
@@ -117,6 +118,10 @@ XillTableHost make_synthetic_xilltable(uint64_t seed, int n_gam, int n_afe, int 
 // names them, and n_ener log bins over 0.07-1000 keV.
 XillTableHost make_synthetic_xilltable_ns(uint64_t seed, int n_ktbb, int n_afe, int n_lxi, int n_dens, int n_incl,
                                           int n_ener);
+// Seeded synthetic six-parameter table: Gamma, A_Fe, logXi, Ecut (kTe), Dens
+// (15-19), Incl.
+XillTableHost make_synthetic_xilltable_dens(uint64_t seed, int n_dens, int n_gam, int n_afe, int n_lxi, int n_ect,
+                                            int n_incl, int n_ener, double e4_lo = 5.0, double e4_hi = 1000.0);
 std::string validate_xilltable(const XillTableHost &t);
 // The spectra divided by 10^logXi of their row and, for a table with a
 // density axis, by 10^(logN - 15) (renorm_xill_spec, src/xilltable.c:478-511,

@@ -281,10 +281,10 @@ std::vector<float> axis(int n, double lo, double hi, bool log_spaced) {
 XillTableHost make_synthetic_xilltable(uint64_t seed, int n_gam, int n_afe, int n_lxi, int n_ect,
                                        int n_incl, int n_ener, double e4_lo, double e4_hi) {
   XillTableHost t;
-  t.n_par = XILL_NPAR;
-  const int ids[XILL_NPAR] = {XP_GAM, XP_AFE, XP_LXI, XP_ECT, XP_INC};
-  const int nv[XILL_NPAR] = {n_gam, n_afe, n_lxi, n_ect, n_incl};
-  for (int k = 0; k < XILL_NPAR; k++) {
+  t.n_par = 5;
+  const int ids[5] = {XP_GAM, XP_AFE, XP_LXI, XP_ECT, XP_INC};
+  const int nv[5] = {n_gam, n_afe, n_lxi, n_ect, n_incl};
+  for (int k = 0; k < 5; k++) {
     t.pid[k] = ids[k];
     t.n_val[k] = nv[k];
   }
@@ -354,10 +354,10 @@ XillTableHost make_synthetic_xilltable(uint64_t seed, int n_gam, int n_afe, int 
 XillTableHost make_synthetic_xilltable_ns(uint64_t seed, int n_ktbb, int n_afe, int n_lxi, int n_dens, int n_incl,
                                           int n_ener) {
   XillTableHost t;
-  t.n_par = XILL_NPAR;
-  const int ids[XILL_NPAR] = {XP_KTB, XP_AFE, XP_LXI, XP_DNS, XP_INC};
-  const int nv[XILL_NPAR] = {n_ktbb, n_afe, n_lxi, n_dens, n_incl};
-  for (int k = 0; k < XILL_NPAR; k++) {
+  t.n_par = 5;
+  const int ids[5] = {XP_KTB, XP_AFE, XP_LXI, XP_DNS, XP_INC};
+  const int nv[5] = {n_ktbb, n_afe, n_lxi, n_dens, n_incl};
+  for (int k = 0; k < 5; k++) {
     t.pid[k] = ids[k];
     t.n_val[k] = nv[k];
   }
@@ -420,9 +420,55 @@ XillTableHost make_synthetic_xilltable_ns(uint64_t seed, int n_ktbb, int n_afe, 
   return t;
 }
 
+// A six-parameter table: the axes of make_synthetic_xilltable with Dens
+// (15-19) between the fourth (Ecut / kTe, which stays the table's axis 3,
+// the one ensure_ecut_within_boundarys reads) and the inclination; each
+// density's rows are that table's rows times 10^(logN - 15) (the scale the
+// files store) and a density-dependent soft excess, so that the density
+// changes the spectra.  Gamma is the first axis, interpolated outside the
+// other five.
+XillTableHost make_synthetic_xilltable_dens(uint64_t seed, int n_dens, int n_gam, int n_afe, int n_lxi, int n_ect,
+                                            int n_incl, int n_ener, double e4_lo, double e4_hi) {
+  const XillTableHost b = make_synthetic_xilltable(seed, n_gam, n_afe, n_lxi, n_ect, n_incl, n_ener, e4_lo, e4_hi);
+  XillTableHost t;
+  t.n_par = 6;
+  for (int k = 0; k < 4; k++) {
+    t.pid[k] = b.pid[k];
+    t.n_val[k] = b.n_val[k];
+    t.val[k] = b.val[k];
+  }
+  t.pid[4] = XP_DNS;
+  t.n_val[4] = n_dens;
+  t.val[4] = axis(n_dens, 15.0, 19.0, false);
+  t.pid[5] = b.pid[4];
+  t.n_val[5] = b.n_val[4];
+  t.val[5] = b.val[4];
+  t.n_ener = n_ener;
+  t.elo = b.elo;
+  t.ehi = b.ehi;
+  t.spec.resize(b.spec.size() * (size_t)n_dens);
+  std::vector<double> soft(n_ener);
+  for (int e = 0; e < n_ener; e++) soft[e] = std::exp(-0.5 * ((double)b.elo[e] + (double)b.ehi[e]) / 1.5);
+  const size_t n_outer = (size_t)n_gam * n_afe * n_lxi * n_ect;  // rows of b without the inclination
+  for (size_t q = 0; q < n_outer; q++)
+    for (int id = 0; id < n_dens; id++) {
+      const double x = (t.val[4][id] - 15.0) / 4.0;
+      const double scale = std::pow(10.0, t.val[4][id] - 15.0);
+      for (int ii = 0; ii < n_incl; ii++) {
+        const float *src = &b.spec[(q * n_incl + ii) * (size_t)n_ener];
+        float *dst = &t.spec[((q * n_dens + id) * n_incl + ii) * (size_t)n_ener];
+        for (int e = 0; e < n_ener; e++) dst[e] = (float)((double)src[e] * scale * (1.0 + 0.4 * x * soft[e]));
+      }
+    }
+  return t;
+}
+
 std::string validate_xilltable(const XillTableHost &t) {
-  if (t.n_par != XILL_NPAR) return "xillver table must have five parameters";
-  if (t.pid[XILL_NPAR - 1] != XP_INC) return "the last xillver parameter must be the inclination";
+  if (t.n_par != 5 && t.n_par != 6) return "xillver table must have five or six parameters";
+  if (t.pid[t.n_par - 1] != XP_INC) return "the last xillver parameter must be the inclination";
+  // a sixth parameter is interpolated outside the other five; the Ecut / kTe
+  // bracket, which the lamp post's zones move, must be an inner one
+  if (t.n_par == 6 && t.pid[0] == XP_ECT) return "Ecut / kTe as the first of six xillver parameters is not supported";
   bool seen[8] = {};
   for (int k = 0; k < t.n_par; k++) {
     if (t.pid[k] < 0 || t.pid[k] > XP_INC || seen[t.pid[k]]) return "xillver parameter ids invalid or repeated";

@@ -489,6 +489,34 @@ def affine_xill_expected(x, g, gam, afe, lxi, ect, c=AFFINE_XILL_C):
     return lin[:, None] * g[None, :]
 
 
+def affine_xilltable6(n_ener=300, c=AFFINE_XILL_C, c_dens=0.053, n_dens=4, **dims):
+    """A six-parameter table, Gamma, A_Fe, logXi, Ecut, Dens (15-19), Incl:
+    stored spectra 10^logXi 10^(logN - 15) g(E) times the affine form of
+    affine_xilltable plus c_dens logN, so that after the loader's divisions
+    the values are affine in all six parameters."""
+    a5 = xill_axes(**dims)
+    ax = a5[:4] + [np.linspace(15.0, 19.0, n_dens)] + a5[4:]
+    e = np.geomspace(0.07, 1000.0, n_ener + 1)
+    g = 1e-3 * (0.5 * (e[1:] + e[:-1])) ** -1.0 * np.diff(e)
+    G, A, L, C, D, I = np.meshgrid(*ax, indexing="ij")
+    K = np.broadcast_to(np.arange(len(ax[5])), G.shape)
+    lin = c[0] + c[1] * G + c[2] * A + c[3] * L + c[4] * C + c[5] * K + c_dens * D
+    spec = (10.0 ** L * 10.0 ** (D - 15.0) * lin)[..., None] * g
+    return dict(n_par=6, pid=[0, 1, 2, 3, 4, 7], n_val=[len(a) for a in ax],
+                val=[np.asarray(a, dtype=np.float32) for a in ax], n_ener=n_ener,
+                elo=e[:-1].astype(np.float32), ehi=e[1:].astype(np.float32),
+                spec=spec.reshape(-1, n_ener).astype(np.float32)), g
+
+
+def xill_slice(x6, i):
+    """The five-parameter table of the first-axis node i of a six-parameter
+    table view (its rows as stored)."""
+    n_val = list(x6["n_val"][1:])
+    rows = int(np.prod(n_val))
+    return dict(n_par=5, pid=list(x6["pid"][1:]), n_val=n_val, val=list(x6["val"][1:]), n_ener=x6["n_ener"],
+                elo=x6["elo"], ehi=x6["ehi"], spec=np.asarray(x6["spec"])[i * rows:(i + 1) * rows])
+
+
 def draw_relxill_params(rng):
     """relxill parameters over the lmodel.dat ranges (refl_frac of both signs)."""
     a = rng.uniform(-0.998, 0.998)
