@@ -353,12 +353,13 @@ int rx_batch_status(rx_batch *b, int i);
 /* emission-angle distribution of a relxill-style or relxill request: the
  * first n values of [zone][n_incl] (one zone except relxilllp's 10) */
 int rx_batch_dist(rx_batch *b, int i, double *dist, int n);
-/* radial zones of a relxill / relxilllp request after a run, 6 values per
+/* radial zones of a relxill / relxilllp request after a run, 8 values per
  * zone: mean radius, emission angle of the zone (relxilllp), energy shift
  * source -> zone, xillver normalisation ratio nf(Ecut_zone)/nf(Ecut_source),
- * lower Ecut table index of the zone and its interpolation factor (the
- * zones of relxill_kernel, src/relutility.c:506-544; Ecut shift,
- * src/Relxill.cpp:357-388) */
+ * lower Ecut table index of the zone and its interpolation factor, lower
+ * logXi table index and factor (the zones of relxill_kernel,
+ * src/relutility.c:506-544; Ecut shift, src/Relxill.cpp:357-388; the logXi
+ * of an ionisation gradient, src/IonGradient.cpp:340-395) */
 int rx_batch_zones(rx_batch *b, int i, double *out, int n_zones);
 /* lamp-post reflection fractions of request i: refl_frac, f_bh, f_ad, f_inf, f_inf_rest */
 int rx_batch_reflfrac(rx_batch *b, int i, double *out5);

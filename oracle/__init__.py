@@ -57,7 +57,8 @@ class RelParam(ctypes.Structure):
                 ("lineE", ctypes.c_double), ("z", ctypes.c_double), ("height", ctypes.c_double),
                 ("htop", ctypes.c_double), ("gamma", ctypes.c_double), ("beta", ctypes.c_double),
                 ("limb", _I), ("num_zones", _I), ("return_rad", _I), ("do_renorm", _I),
-                ("rr_corr_flux", _D), ("rr_corr_gshift", _D), ("rr_corr_rgrid", _D), ("rr_corr_n", _I)]
+                ("rr_corr_flux", _D), ("rr_corr_gshift", _D), ("rr_corr_rgrid", _D), ("rr_corr_n", _I),
+                ("ion_grad_type", _I), ("ion_grad_index", ctypes.c_double), ("lxi0", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
@@ -516,7 +517,7 @@ def relxill_model(model, tables, xt, params, energy, conv_grid=None, phys_norm=F
     e = _arr(energy)
     n = e.size - 1
     out = np.zeros(n)
-    nz = 10 if model in (MOD_RELXILLLP, MOD_RELXILLLPCP) else 1
+    nz = n_zones(model, params)
     dist = np.zeros(nz * xt.n_incl)
     cg = _arr(conv_grid) if conv_grid is not None else None
     st = lib.orc_relxill_model(model, ctypes.byref(tables.rel), tables.lp_ptr, ctypes.byref(xt.t),
@@ -525,13 +526,21 @@ def relxill_model(model, tables, xt, params, energy, conv_grid=None, phys_norm=F
     return out, dist.reshape(nz, xt.n_incl), st
 
 
+def n_zones(model, params):
+    """radial zones of a relxill model: 10 for the lamp post, 25 for
+    relxilllpCp with the power-law ionisation gradient (iongrad_type 1), else 1"""
+    if model == MOD_RELXILLLPCP and round(params[14]) == 1:
+        return 25
+    return 10 if model in (MOD_RELXILLLP, MOD_RELXILLLPCP) else 1
+
+
 def relxill_zones(model, tables, params):
     """per zone: r_lo, r_hi, del_emit(centre), energy shift source -> disk,
-    Ecut seen by the zone, normalisation change; (array[nz, 6], status)"""
-    nz = 10 if model in (MOD_RELXILLLP, MOD_RELXILLLPCP) else 1
-    out = np.zeros(6 * nz)
+    Ecut seen by the zone, normalisation change, logXi; (array[nz, 7], status)"""
+    nz = n_zones(model, params)
+    out = np.zeros(7 * nz)
     st = lib.orc_relxill_zones(model, ctypes.byref(tables.rel), tables.lp_ptr, _dp(_arr(params)), _dp(out))
-    return out.reshape(nz, 6), st
+    return out.reshape(nz, 7), st
 
 
 def energy_shift_source_obs(model, params):

@@ -1675,8 +1675,9 @@ double orc_prim_norm_factor(const orc_xillparam *x, double ect) {
 /* relxillCp / relxilllpCp (lmodel_relxill_public.dat:96-129; convertModelType,
  * src/ModelDefinition.cpp:38-50): the parameters in relxill's / relxilllp's
  * order with kTe in Ecut's place; returns that model, *dens logN and
- * *iongrad relxilllpCp's iongrad_type */
-static int cp_remap(int model_type, const double *par, double *out, double *dens, int *iongrad) {
+ * relxilllpCp's iongrad_type and iongrad_index */
+static int cp_remap(int model_type, const double *par, double *out, double *dens, int *iongrad,
+                    double *ion_index) {
   /* relxillCp: Incl a Rin Rout Rbr Index1 Index2 z gamma logxi logN Afe kTe refl_frac */
   static const int k_bkn[13] = {5, 6, 4, 1, 0, 2, 3, 7, 8, 9, 11, 12, 13};
   /* relxilllpCp: Incl a Rin Rout h beta gamma logxi logN Afe kTe refl_frac z
@@ -1691,15 +1692,16 @@ static int cp_remap(int model_type, const double *par, double *out, double *dens
   for (int i = 0; i < 14; i++) out[i] = par[k_lp[i]];
   *dens = par[8];
   *iongrad = (int)lround(par[14]);
+  *ion_index = par[13];
   return ORC_MOD_RELXILLLP;
 }
 
 int orc_get_relxill_params(int model_type, const double *par, int phys_norm_env, orc_relparam *p,
                            orc_xillparam *x) {
-  double cpar[14], dens = 15, ktbb = 0;
+  double cpar[14], dens = 15, ktbb = 0, ion_index = 0;
   int prim = 0, iongrad = 0;
   if (model_type == ORC_MOD_RELXILLCP || model_type == ORC_MOD_RELXILLLPCP) {
-    model_type = cp_remap(model_type, par, cpar, &dens, &iongrad);
+    model_type = cp_remap(model_type, par, cpar, &dens, &iongrad, &ion_index);
     par = cpar;
     prim = 1;
   } else if (model_type == ORC_MOD_RELXILLNS) {
@@ -1734,11 +1736,22 @@ int orc_get_relxill_params(int model_type, const double *par, int phys_norm_env,
   x->dens = dens;
   x->prim = prim;
   x->ktbb = ktbb;
+  p->ion_grad_type = 0;
+  p->ion_grad_index = 0.0;
+  p->lxi0 = x->lxi;
   if (x->lxi < 0.0) x->lxi = 0.0;
   else if (x->lxi > 4.7) x->lxi = 4.7;
-  /* constant ionisation only: relxilllpCp's radial ionisation gradients
-   * (iongrad_type != 0, src/IonGradient.cpp:340-395) are not restated */
-  if (status == 0 && iongrad != 0) status = 2;
+  /* relxilllpCp's ionisation gradient (IonGradient::calculate_gradient,
+   * src/IonGradient.cpp:340-395): the power law (1) on N_ZONES_IONGRAD zones
+   * (get_num_zones, src/relutility.c:517-527); the alpha-disc gradient (2)
+   * is not restated */
+  if (status == 0 && iongrad == 1) {
+    p->ion_grad_type = 1;
+    p->ion_grad_index = ion_index;
+    p->num_zones = 25;
+  } else if (status == 0 && iongrad != 0) {
+    status = 2;
+  }
   return status;
 }
 
@@ -1749,6 +1762,20 @@ double orc_energy_shift_source_obs(const orc_relparam *p) {
   const double g_inf_0 = sqrt(1.0 - (2 * p->height / (p->height * p->height + p->a * p->a)));
   if (p->beta < 1e-4) return g_inf_0;
   return g_inf_0 * doppler_factor(M_PI - p->incl, p->beta);
+}
+
+void orc_zone_lxi(const orc_relparam *p, double lxi, const double *rmean, int nz, double *out) {
+  for (int iz = 0; iz < nz; iz++) out[iz] = lxi;
+  if (p->ion_grad_type != 1) return;
+  /* calc_ion_grad_pl (src/IonGradient.cpp:174-182), then
+   * adjust_parameter_to_be_within_xillver_bounds (:36-44, :388) */
+  for (int iz = 0; iz < nz; iz++) {
+    double v = exp(p->lxi0) * pow(rmean[iz] / rmean[0], -1.0 * p->ion_grad_index);
+    v = log(v);
+    if (v < 0.0) v = 0.0;
+    else if (v > 4.7) v = 4.7;
+    out[iz] = v;
+  }
 }
 
 int orc_relxill_model(int model_type, const orc_reltable *tab, const orc_lptable *lp,
@@ -1787,6 +1814,7 @@ int orc_relxill_model(int model_type, const orc_reltable *tab, const orc_lptable
   double *rmean = (double *)malloc(sizeof(double) * nz);
   double *del_emit = (double *)malloc(sizeof(double) * nz);
   double *shift = (double *)malloc(sizeof(double) * nz);
+  double *zlxi = (double *)malloc(sizeof(double) * nz);
   /* shift_energy_grid_redshift (src/XspecSpectrum.h:70-76) */
   memcpy(ener, energy, sizeof(double) * (n_flux + 1));
   if (p.z > 0)
@@ -1808,6 +1836,7 @@ int orc_relxill_model(int model_type, const orc_reltable *tab, const orc_lptable
       shift[iz] = 1.0;
       del_emit[iz] = 0.0;
     }
+    orc_zone_lxi(&p, x.lxi, rmean, nz, zlxi);
     if (p.emis_type == ORC_EMIS_LP) {
       inv_rebin_mean(sp->re, sp->del_emit, sp->nr, rmean, del_emit, nz);
       for (int iz = 0; iz < nz; iz++) shift[iz] = orc_gi_potential_lp(rmean[iz], p.a, p.height, p.beta, del_emit[iz]);
@@ -1821,6 +1850,7 @@ int orc_relxill_model(int model_type, const orc_reltable *tab, const orc_lptable
       for (int iz = 0; iz < nz; iz++) {
         orc_xillparam xz = xs;
         xz.ect = xs.ect * shift[iz];
+        xz.lxi = zlxi[iz];
         orc_xill_corr_factors(xt, &xz, &cf_flux[iz], &cf_g[iz]);
       }
       p.rr_corr_flux = cf_flux;
@@ -1845,6 +1875,7 @@ int orc_relxill_model(int model_type, const orc_reltable *tab, const orc_lptable
        * get_xill_param_zone, src/IonGradient.cpp:196-209), angle weighting */
       orc_xillparam xz = xs;
       xz.ect = xs.ect * shift[iz];
+      xz.lxi = zlxi[iz];
       orc_xill_spectra(xt, &xz, xe, xflu);
       orc_xill_angdep(xill, xflu, dist + (size_t)iz * n_incl, n_incl, nx);
       const double ncf = orc_prim_norm_factor(&xs, xs.ect * shift[iz]) / nf_src;
@@ -1891,6 +1922,7 @@ int orc_relxill_model(int model_type, const orc_reltable *tab, const orc_lptable
   free(grid); free(ener); free(xe); free(xflu); free(xill); free(rel); free(dist);
   free(xill_rb); free(conv_out); free(total); free(prim); free(rgrid); free(rmean); free(del_emit);
   free(shift);
+  free(zlxi);
   return status;
 }
 
@@ -1913,16 +1945,19 @@ int orc_relxill_zones(int model_type, const orc_reltable *tab, const orc_lptable
   const double g_so = orc_energy_shift_source_obs(&p);
   const double ect_src = (p.emis_type == ORC_EMIS_LP && !x.prim) ? x.ect / g_so : x.ect;
   const double nf_src = orc_prim_norm_factor(&x, ect_src);
+  double zlxi[64];
+  orc_zone_lxi(&p, x.lxi, rmean, nz, zlxi);
   for (int iz = 0; iz < nz; iz++) {
     const double shift =
         (p.emis_type == ORC_EMIS_LP) ? orc_gi_potential_lp(rmean[iz], p.a, p.height, p.beta, del[iz]) : 1.0;
-    double *o = out + 6 * iz;
+    double *o = out + 7 * iz;
     o[0] = rgrid[iz];
     o[1] = rgrid[iz + 1];
     o[2] = del[iz];
     o[3] = shift;
     o[4] = ect_src * shift;
     o[5] = orc_prim_norm_factor(&x, ect_src * shift) / nf_src;
+    o[6] = zlxi[iz];
   }
   orc_syspar_free(sp);
   return 0;

@@ -109,6 +109,10 @@ typedef struct {
    * rr_corr_n = 0: no correction */
   const double *rr_corr_flux, *rr_corr_gshift, *rr_corr_rgrid;
   int rr_corr_n;
+  /* relxilllpCp's ionisation gradient (src/IonGradient.cpp:340-395): type
+   * (0 constant, 1 power law), index and the input logXi it starts from */
+  int ion_grad_type;
+  double ion_grad_index, lxi0;
 } orc_relparam;
 
 /* reflection fractions (src/common.h:279-285) */
@@ -310,6 +314,11 @@ int orc_get_relxill_params(int model_type, const double *parameter, int phys_nor
                            orc_xillparam *x);
 /* energy shift primary source -> observer (src/Relphysics.cpp:239-254) */
 double orc_energy_shift_source_obs(const orc_relparam *p);
+/* the zones' logXi (IonGradient::calculate_gradient, src/IonGradient.cpp:
+ * 340-395): lxi (the request's, kept within 0-4.7) in every zone, or for the
+ * power-law gradient log(exp(lxi0) (rmean / rmean[0])^-index) kept within
+ * 0-4.7 (calc_ion_grad_pl, :174-182; :36-44, :388) */
+void orc_zone_lxi(const orc_relparam *p, double lxi, const double *rmean, int nz, double *out);
 /* relxill_kernel (src/Relxill.cpp:289-414 with src/PrimarySource.cpp:66-125)
  * for relxill (broken power law, one zone) and relxilllp (lamp post, 10
  * zones with the source -> disk energy shift of Ecut and the normalisation
@@ -320,7 +329,8 @@ int orc_relxill_model(int model_type, const orc_reltable *tab, const orc_lptable
                       int n_flux, const double *conv_ener, int n_conv, int phys_norm_env, double *dist_out,
                       orc_stats *st);
 /* the zones of a relxill model: per zone {r_lo, r_hi, del_emit at the centre,
- * source -> disk energy shift, Ecut seen by the zone, normalisation change} */
+ * source -> disk energy shift, Ecut seen by the zone, normalisation change,
+ * logXi of the zone} */
 int orc_relxill_zones(int model_type, const orc_reltable *tab, const orc_lptable *lp, const double *parameter,
                       double *out);
 int orc_relxill(const orc_reltable *tab, const orc_xilltable *xt, const double *parameter,

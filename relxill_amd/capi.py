@@ -455,10 +455,11 @@ class Batch:
         return d
 
     def zones(self, i, n_zones):
-        """(n_zones, 6): rmean, del_emit, shift, ncf, Ecut index, Ecut factor"""
-        o = np.zeros(6 * n_zones)
+        """(n_zones, 8): rmean, del_emit, shift, ncf, Ecut index, Ecut factor,
+        logXi index, logXi factor"""
+        o = np.zeros(8 * n_zones)
         _check(lib.rx_batch_zones(self.h, i, _dp(o), n_zones))
-        return o.reshape(n_zones, 6)
+        return o.reshape(n_zones, 8)
 
     def reflfrac(self, i):
         o = np.zeros(5)

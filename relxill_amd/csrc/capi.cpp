@@ -619,13 +619,15 @@ int rx_batch_zones(rx_batch *b, int i, double *out, int n_zones) {
                 hipMemcpyDeviceToHost) != hipSuccess)
     return RX_ERR_DEVICE;
   for (int k = 0; k < n_zones; k++) {
-    double *o = out + 6 * k;
+    double *o = out + 8 * k;
     o[0] = z[k].rmean;
     o[1] = z[k].del_emit;
     o[2] = z[k].shift;
     o[3] = z[k].ncf;
     o[4] = z[k].ect_ind;
     o[5] = z[k].ect_fac;
+    o[6] = z[k].lxi_ind;
+    o[7] = z[k].lxi_fac;
   }
   return RX_OK;
 }

@@ -219,7 +219,7 @@ int parse_request(int model, const double *par, bool phys_norm, ReqParams &p, st
   if (model == MOD_RELXILL || model == MOD_RELXILLLP) {
     // zones (get_num_zones, src/relutility.c:506-544: N_ZONES for the lamp
     // post, one otherwise) and their edges
-    p.n_zones = (model == MOD_RELXILLLP) ? MAX_ZONES : 1;
+    p.n_zones = (model == MOD_RELXILLLP) ? LP_ZONES : 1;
     rzone_grid(p.rin, p.rout, p.n_zones, p.height, p.zgrid);
     if (p.emis_type == EMIS_LP) {
       // energy_shift_source_obs (src/Relphysics.cpp:239-254, :134-136, :217-230)
@@ -607,6 +607,9 @@ int ensure_xilltable_device(int prim, std::string &err) {
   int e = upload_vec(x.spec, xilltable_renormed(t)) | upload_vec(x.d_ener, x.ener);
   // the inclination axis (returning-radiation corrections of relxilllp)
   e |= upload_vec(x.incl, t.val[t.n_par - 1]);
+  // the logXi axis (the zones of an ionisation gradient bracket their own)
+  const int k_lxi = t.dim_of(XP_LXI);
+  if (k_lxi >= 0) e |= upload_vec(x.lxi, t.val[k_lxi]);
   // the Ecut / kTe axis (the lamp post's zones bracket their own value on the
   // device) and the axis ensure_ecut_within_boundarys reads
   const int k_ect = t.dim_of(XP_ECT);
@@ -755,6 +758,10 @@ void fill_dev_views(BatchDev &bd) {
     xt.ect_vals = x.ect.as<float>();
     xt.incl_vals = x.incl.as<float>();
     xt.q_vals = x.q.as<float>();
+    const int k_lxi = ok ? t.dim_of(XP_LXI) : -1;
+    xt.lxi_dim = (k_lxi >= off) ? k_lxi - off : -1;  // (a first axis of six is bracketed on the host)
+    xt.n_lxi = k_lxi >= 0 ? t.n_val[k_lxi] : 0;
+    xt.lxi_vals = x.lxi.as<float>();
   }
   bd.rr.n_spin = g.rrad.n_spin;
   bd.rr.n_rad = g.rrad.n_rad;
@@ -800,7 +807,9 @@ struct CpRequest {
   double dens = 15.0;
   double ktbb = 0.0;
   int prim = PRIM_NTHCOMP;
-  int status = RX_OK;  // RX_ERR_UNSUPPORTED: an ionisation gradient was asked for
+  int status = RX_OK;  // RX_ERR_UNSUPPORTED: an ionisation gradient that is not built was asked for
+  int ion_grad = 0;     // relxilllpCp's iongrad_type (1: the power law)
+  double ion_index = 0.0;
 };
 
 static CpRequest remap_cp(int model, const double *par) {
@@ -818,9 +827,16 @@ static CpRequest remap_cp(int model, const double *par) {
     for (int i = 0; i < 14; i++) c.par[i] = par[k[i]];
     c.dens = par[8];
     c.model = MOD_RELXILLLP;
-    // iongrad_type 0: constant ionisation (src/IonGradient.cpp:340-395);
-    // the radial ionisation gradients are not built
-    if (lround(par[14]) != 0) c.status = RX_ERR_UNSUPPORTED;
+    // iongrad_type (src/IonGradient.cpp:340-395): 0 constant ionisation, 1 the
+    // power law in radius (calc_ion_grad_pl, :174-182) with iongrad_index;
+    // the alpha-disc gradient (2) is not built
+    const long ig = lround(par[14]);
+    if (ig == 1) {
+      c.ion_grad = 1;
+      c.ion_index = par[13];
+    } else if (ig != 0) {
+      c.status = RX_ERR_UNSUPPORTED;
+    }
   } else if (model == MOD_XILLVERCP) {
     // xillverCp (lmodel_relxill_public.dat:86-94): gamma Afe kTe logxi logN z
     // Incl refl_frac -- xillver's order with logN taken out
@@ -937,6 +953,16 @@ int batch_build(Batch &b, const rx_request *req_in, int n_req, std::string &err)
       return st;
     }
     if (st == RX_OK && prim[i] != PRIM_CUTOFFPL) st = cp[i].status;
+    if (st == RX_OK && prim[i] != PRIM_CUTOFFPL && cp[i].ion_grad == 1) {
+      // an ionisation gradient: N_ZONES_IONGRAD zones (get_num_zones,
+      // src/relutility.c:517-527) whose logXi the device sets from the input
+      // one (zone_record)
+      p.n_zones = MAX_ZONES;
+      rzone_grid(p.rin, p.rout, p.n_zones, p.height, p.zgrid);
+      p.ion_grad = 1;
+      p.ion_index = cp[i].ion_index;
+      p.xlxi0 = cp[i].par[8];  // relxilllp's logxi slot, before the 0-4.7 clamp
+    }
     p.n_in = r.n_flux;
     p.ener_off = o_ener;
     o_ener += r.n_flux + 1;
@@ -1001,7 +1027,12 @@ int batch_build(Batch &b, const rx_request *req_in, int n_req, std::string &err)
         o_xspec += (int64_t)xt.n_ener * p.n_zones;
         p.zone_off = o_zone;
         o_zone += p.n_zones;
-        (p.n_zones > 1 ? xill_multi : b.h_xill).push_back(XillWork{i, 0});
+        if (p.ion_grad) {
+          // zones that differ in logXi as well as in Ecut: one k_xill1 unit each
+          for (int z = 0; z < p.n_zones; z++) b.h_xill.push_back(XillWork{i, z});
+        } else {
+          (p.n_zones > 1 ? xill_multi : b.h_xill).push_back(XillWork{i, 0});
+        }
         b.h_prim.push_back(i);
         if (p.prim == PRIM_NTHCOMP) {
           // the source's solve, then one per lamp-post zone (k_nth)
@@ -1065,6 +1096,8 @@ int batch_build(Batch &b, const rx_request *req_in, int n_req, std::string &err)
   b.tot_rrc = o_rrc;
   b.n_xill1 = (int)b.h_xill.size();
   b.h_xill.insert(b.h_xill.end(), xill_multi.begin(), xill_multi.end());
+  b.max_zones = 1;
+  for (int i = 0; i < n_req; i++) b.max_zones = std::max(b.max_zones, b.h_req[i].n_zones);
   b.h_ener.resize(o_ener);
   b.h_flux_in.assign(o_flux, 0.0);
   for (int i = 0; i < n_req; i++) {
@@ -1193,6 +1226,7 @@ static BatchDev dev_view(Batch &b) {
   bd.nr_fine = g.nr_fine;
   bd.seg_bins = b.seg_bins;
   bd.node_stride = 2 * g.rel.n_r * g.rel.n_g * 2;
+  bd.max_zones = b.max_zones;
   bd.req = b.d_req.as<ReqParams>();
   bd.ener_in = b.d_ener.as<double>();
   bd.flux_in = b.d_flux_in.as<double>();

@@ -56,6 +56,7 @@ struct RunKey {
 
 struct Batch {
   int n_req = 0;
+  int max_zones = 1;          // most zones of a request
   int seg_bins = 64;        // bins per work item
   int seg_forced = 0;       // 0: automatic / autotuned
   int waves_per_cu = 8;     // persistent integration wavefronts per CU
@@ -122,7 +123,7 @@ struct XillSlot {
   bool loaded = false, dirty = true;
   std::string err;
   std::vector<double> ener;
-  DevBuf spec, d_ener, ect, q, incl;
+  DevBuf spec, d_ener, ect, q, incl, lxi;
 };
 
 // global state

@@ -34,7 +34,7 @@ constexpr int SETUP_THREADS = 256;
 constexpr int CONV_THREADS = 256;      // smallest convolution workgroup
 constexpr int XILL_THREADS = 256;
 constexpr int XILL_BINS = 4;    // table bins per k_xill thread, one-zone requests
-constexpr int XILL_BINS_Z = 2;  // the same for requests of MAX_ZONES zones (ten accumulators per bin)
+constexpr int XILL_BINS_Z = 2;  // the same for requests of LP_ZONES zones (ten accumulators per bin)
 constexpr int PRIM_THREADS = 256;
 constexpr int CONV_THREADS_MAX = 1024;  // n/8 threads: 512 for 4096 bins, 1024 for 8192
 
@@ -226,8 +226,33 @@ __device__ inline ZoneRec zone_record(const DevXillTable &xt, const ReqParams &r
   o.ncf = 1.0;
   o.ect_ind = xt.ect_dim >= 0 ? rq.xind[xt.ect_dim] : 0;
   o.ect_fac = xt.ect_dim >= 0 ? rq.xfac[xt.ect_dim] : 0.0;
-  o.pad = 0;
+  o.lxi_ind = xt.lxi_dim >= 0 ? rq.xind[xt.lxi_dim] : 0;
+  o.lxi_fac = xt.lxi_dim >= 0 ? rq.xfac[xt.lxi_dim] : 0.0;
   *ect = rq.xect_src;
+  if (rq.ion_grad == 1 && xt.lxi_dim >= 0) {
+    // the power-law gradient (calc_ion_grad_pl, src/IonGradient.cpp:174-182)
+    // from the input logXi in the zones' mean radius, kept within 0-4.7
+    // (adjust_parameter_to_be_within_xillver_bounds, :36-44, :388), and its
+    // bracket on the table's logXi axis as the request's parameters get
+    // theirs (float value, clamped into the table, float factor)
+    const double rm0 = 0.5 * (zg[0] + zg[1]);
+    double lxi = exp(rq.xlxi0) * pow(o.rmean / rm0, -1.0 * rq.ion_index);
+    lxi = log(lxi);
+    if (lxi < 0.0) lxi = 0.0;
+    else if (lxi > 4.7) lxi = 4.7;
+    const float *vals = xt.lxi_vals;
+    const int n = xt.n_lxi;
+    float v = (float)lxi;
+    int ind = bsearch_asc(vals, n, v);
+    if (ind < 0) ind = 0;
+    else if (ind > n - 2) ind = n - 2;
+    if (v < vals[0]) v = vals[0];
+    else if (v > vals[n - 1]) v = vals[n - 1];
+    const float num = v - vals[ind];
+    const float den = vals[ind + 1] - vals[ind];
+    o.lxi_ind = ind;
+    o.lxi_fac = (float)(num / den);
+  }
   if (lp) {
     int lo = 0, hi = nr - 1;  // re(lo) > rmean >= re(hi)
     while (hi - lo > 1) {
@@ -464,6 +489,9 @@ __device__ inline double block_sum(double v, double *red) {
 // ---------------------------------------------------------------------------
 // setup
 // ---------------------------------------------------------------------------
+// NZ: the most zones of a request in the batch (LP_ZONES, or MAX_ZONES with
+// an ionisation gradient), which sizes the zone arrays in LDS
+template <int NZ>
 __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
   const int req = blockIdx.x;
   const int tid = threadIdx.x;
@@ -491,7 +519,8 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
       o.ncf = 1.0;
       o.ect_ind = xt.ect_dim >= 0 ? gq.xind[xt.ect_dim] : 0;
       o.ect_fac = xt.ect_dim >= 0 ? gq.xfac[xt.ect_dim] : 0.0;
-      o.pad = 0;
+      o.lxi_ind = xt.lxi_dim >= 0 ? gq.xind[xt.lxi_dim] : 0;
+      o.lxi_fac = xt.lxi_dim >= 0 ? gq.xfac[xt.lxi_dim] : 0.0;
       b.zone[rq.zone_off] = o;
     }
     double *d = b.dist + rq.dist_off;
@@ -508,18 +537,18 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
   __shared__ double s_tre[MAX_NR_TAB], s_tgmin[MAX_NR_TAB], s_tgmax[MAX_NR_TAB];
   __shared__ double s_jr[MAX_LP_R], s_je[MAX_LP_R], s_jd[MAX_LP_R], s_ji[MAX_LP_R];
   __shared__ double s_red[SETUP_THREADS];
-  __shared__ double s_dist[MAX_ZONES * MAX_INCL];
+  __shared__ double s_dist[NZ * MAX_INCL];
   __shared__ int s_st, s_ia, s_im, s_rmin;
   __shared__ double s_fa, s_fm, s_rms;
   constexpr int MAX_RC = MAX_NR_FINE / RAD_CHUNK;
   // energy window of each (radius chunk, zone): bins [s_ulo, s_uhi]
-  __shared__ int s_ulo[MAX_RC * MAX_ZONES], s_uhi[MAX_RC * MAX_ZONES];
-  __shared__ int s_pref[MAX_RC * MAX_ZONES + 1], s_prefs[MAX_RC * MAX_ZONES + 1];
+  __shared__ int s_ulo[MAX_RC * NZ], s_uhi[MAX_RC * NZ];
+  __shared__ int s_pref[MAX_RC * NZ + 1], s_prefs[MAX_RC * NZ + 1];
   __shared__ int s_base, s_bases;
   const int n_rc = (b.nr_fine + RAD_CHUNK - 1) / RAD_CHUNK;
   const int nz = rq.n_zones;
   const double *zg = b.req[req].zgrid;
-  for (int c = tid; c < n_rc * MAX_ZONES; c += SETUP_THREADS) {
+  for (int c = tid; c < n_rc * NZ; c += SETUP_THREADS) {
     s_ulo[c] = 0x7fffffff;
     s_uhi[c] = -1;
   }
@@ -815,8 +844,8 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
       const int zl = wave_min_i(izone == z ? wlo : 0x7fffffff);
       const int zh = wave_max_i(izone == z ? whi : -1);
       if ((tid & 63) == 0) {
-        s_ulo[(ii / RAD_CHUNK) * MAX_ZONES + z] = zl;
-        s_uhi[(ii / RAD_CHUNK) * MAX_ZONES + z] = zh;
+        s_ulo[(ii / RAD_CHUNK) * NZ + z] = zl;
+        s_uhi[(ii / RAD_CHUNK) * NZ + z] = zh;
       }
     }
   }
@@ -891,7 +920,7 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
   if (relxill_model && s_st == 0) {
     ZoneRec *zr = b.zone + rq.zone_off;
     const DevXillTable &xt = b.xt[rq.prim];
-    __shared__ double s_zect[MAX_ZONES];
+    __shared__ double s_zect[NZ];
     for (int z = tid; z < nz; z += SETUP_THREADS) {
       double ect;
       zr[z] = zone_record(xt, b.req[req], z, lp, s_jr, s_je, s_jd, s_ji, ntr, nr, &ect);
@@ -997,8 +1026,8 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     int jlo = 0x7fffffff, jhi = -1;
     for (int c = 0; c < n_rc; c++) {
       for (int z = 0; z < nz; z++) {
-        jlo = min(jlo, s_ulo[c * MAX_ZONES + z]);
-        jhi = max(jhi, s_uhi[c * MAX_ZONES + z]);
+        jlo = min(jlo, s_ulo[c * NZ + z]);
+        jhi = max(jhi, s_uhi[c * NZ + z]);
       }
     }
     if (tid == 0) b.reach[req] = BinRange{jlo, jhi};
@@ -1017,11 +1046,11 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     // together on short ones.
     const int S = b.seg_bins;
     const int thr = max(1, S / 4);
-    const int n_cz = n_rc * nz;  // pairs q = c nz + z; their windows sit at c MAX_ZONES + z
+    const int n_cz = n_rc * nz;  // pairs q = c nz + z; their windows sit at c NZ + z
     if (tid == 0) {
       int tot = 0, tots = 0;
       for (int q = 0; q < n_cz; q++) {
-        const int w = (q / nz) * MAX_ZONES + q % nz;
+        const int w = (q / nz) * NZ + q % nz;
         s_pref[q] = tot;
         s_prefs[q] = tots;
         if (s_uhi[w] >= s_ulo[w]) {
@@ -1046,7 +1075,7 @@ __global__ __launch_bounds__(SETUP_THREADS, 3) void k_setup(BatchDev b) {
     __syncthreads();
     if (s_base >= 0) {
       for (int q = tid; q < n_cz; q += SETUP_THREADS) {
-        const int c = q / nz, z = q % nz, w = c * MAX_ZONES + z;
+        const int c = q / nz, z = q % nz, w = c * NZ + z;
         int k = s_base + s_pref[q], ks = s_bases + s_prefs[q];
         for (int j0 = s_ulo[w]; j0 <= s_uhi[w]; j0 += S) {
           WorkItem it;
@@ -1095,10 +1124,10 @@ __global__ __launch_bounds__(RRC_THREADS) void k_rrad_corr(BatchDev b, const Xil
   int i0[4];
   double f[4];
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
-    const bool e = (k == xt.ect_dim);
-    i0[k] = e ? zr.ect_ind : rq.xind[k];
-    f[k] = e ? zr.ect_fac : rq.xfac[k];
+  for (int k = 0; k < 4; k++) {  // the zone's own Ecut and logXi brackets
+    const bool e = (k == xt.ect_dim), l = (k == xt.lxi_dim);
+    i0[k] = e ? zr.ect_ind : (l ? zr.lxi_ind : rq.xind[k]);
+    f[k] = e ? zr.ect_fac : (l ? zr.lxi_fac : rq.xfac[k]);
   }
   // the 16 corners in the reference's order (interp_5d_tab_incl, src/xilltable.c:812-876)
   constexpr unsigned char kCorner[16] = {0x0, 0x8, 0x4, 0x2, 0xC, 0xA, 0x6, 0xE,
@@ -2340,10 +2369,10 @@ __global__ __launch_bounds__(XILL_THREADS, 4) void k_xill1(BatchDev b, const Xil
   int i0[4];
   double f[4];
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
-    const bool ect = (k == xt.ect_dim);
-    i0[k] = ect ? zr.ect_ind : rq.xind[k];
-    f[k] = ect ? zr.ect_fac : rq.xfac[k];
+  for (int k = 0; k < 4; k++) {  // the zone's own Ecut and logXi brackets
+    const bool ect = (k == xt.ect_dim), lxi = (k == xt.lxi_dim);
+    i0[k] = ect ? zr.ect_ind : (lxi ? zr.lxi_ind : rq.xind[k]);
+    f[k] = ect ? zr.ect_fac : (lxi ? zr.lxi_fac : rq.xfac[k]);
   }
   // corner c has the offsets of bits 3, 2, 1, 0 of kCorner[c] on axes 0..3
   constexpr unsigned char kCorner[16] = {0x0, 0x8, 0x4, 0x2, 0xC, 0xA, 0x6, 0xE,
@@ -2618,7 +2647,10 @@ __global__ __launch_bounds__(256) void k_copy(const double4 *__restrict__ src, d
 int launch_setup(const BatchDev &b, void *stream) {
   if (b.n_req <= 0) return 0;
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, (hipStream_t)stream, b.ctr);
-  hipLaunchKernelGGL(k_setup, dim3(b.n_req), dim3(SETUP_THREADS), 0, (hipStream_t)stream, b);
+  if (b.max_zones > LP_ZONES)
+    hipLaunchKernelGGL(k_setup<MAX_ZONES>, dim3(b.n_req), dim3(SETUP_THREADS), 0, (hipStream_t)stream, b);
+  else
+    hipLaunchKernelGGL(k_setup<LP_ZONES>, dim3(b.n_req), dim3(SETUP_THREADS), 0, (hipStream_t)stream, b);
   return (int)hipGetLastError();
 }
 
@@ -2713,7 +2745,7 @@ int launch_xill(const BatchDev &b, const XillWork *work, int n_single, int n_mul
   }
   if (n_multi > 0) {
     const int chunks = (xill_n_ener(b) + XILL_THREADS * XILL_BINS_Z - 1) / (XILL_THREADS * XILL_BINS_Z);
-    hipLaunchKernelGGL(k_xillz<MAX_ZONES>, dim3(n_multi * chunks), dim3(XILL_THREADS), 0,
+    hipLaunchKernelGGL(k_xillz<LP_ZONES>, dim3(n_multi * chunks), dim3(XILL_THREADS), 0,
                        (hipStream_t)stream, b, work + n_single, chunks);
   }
   return (int)hipGetLastError();

@@ -32,7 +32,8 @@ constexpr int MAX_NG = 64;        // g* points (trff values per radius: 2*ng)
 constexpr int MAX_LP_R = 256;     // lamp-post table radii
 constexpr int MAX_INCL = 64;      // emission-angle bins
 constexpr int MAX_FFT = 8192;     // convolution grid size limit
-constexpr int MAX_ZONES = 10;     // radial zones of relxilllp (N_ZONES, src/common.h:134)
+constexpr int LP_ZONES = 10;      // radial zones of relxilllp (N_ZONES, src/common.h:134)
+constexpr int MAX_ZONES = 25;     // with an ionisation gradient (N_ZONES_IONGRAD, src/common.h:135)
 constexpr int MAX_RRC_ENER = 16384;  // xillver bins held in LDS by the returning-radiation corrections
 
 enum ModelKind : int {
@@ -41,7 +42,7 @@ enum ModelKind : int {
   MOD_RELCONV = 11,
   MOD_RELCONVLP = 12,
   MOD_RELXILL = -1,          // relxill: broken power law, one zone, xillver table
-  MOD_RELXILLLP = -2,        // relxilllp: lamp post, MAX_ZONES radial zones
+  MOD_RELXILLLP = -2,        // relxilllp: lamp post, LP_ZONES radial zones (MAX_ZONES with a gradient)
   MOD_RELXILL_CONV = -1001,  // relxill-style conv of a resident reflection set
   // the nthcomp-primary models (relxillCp, relxilllpCp); their requests run as
   // MOD_RELXILL / MOD_RELXILLLP with ReqParams.prim = PRIM_NTHCOMP
@@ -115,6 +116,11 @@ struct ReqParams {
   // interpolated outside the other five (0 and 0 for five parameters)
   int xo_ind, pad_xo;
   double xo_fac;
+  // relxilllpCp's ionisation gradient (iongrad_type, src/IonGradient.cpp:
+  // 340-395): 0 constant, 1 the power law of calc_ion_grad_pl (:174-182) in
+  // the zones' radius with index ion_index from the input logXi xlxi0
+  int ion_grad, pad_ion;
+  double ion_index, xlxi0;
   // returning radiation (lamp post): 1 add, -1 or 2 replace the direct
   // emissivity; the table spin, its first zone and the number of zones used
   int return_rad, rr_k, rr_klo, rr_n;
@@ -129,8 +135,8 @@ struct ReqParams {
 // its xillver axis bracket and factor, and the normalisation change of the
 // incident spectrum (src/Xillspec.cpp:408-435).
 struct ZoneRec {
-  double rmean, del_emit, shift, ncf, ect_fac;
-  int ect_ind, pad;
+  double rmean, del_emit, shift, ncf, ect_fac, lxi_fac;
+  int ect_ind, lxi_ind;  // the zone's brackets on the Ecut and logXi axes (with ect_fac, lxi_fac)
 };
 
 // A k_conv unit: a request and one of its zones.  k_xill units use the same
@@ -245,6 +251,9 @@ struct DevXillTable {
   const float *q_vals;    // values of axis ect_q
   int n_ect, n_q;
   const float *incl_vals;  // the inclination axis (degrees)
+  int lxi_dim;             // inner axis of logXi (-1: none)
+  int n_lxi;
+  const float *lxi_vals;   // its values (the zones of an ionisation gradient bracket their own)
 };
 
 constexpr int N_ENER_COARSE = 500;  // primary-continuum normalisation grid (src/Xillspec.h:32)
@@ -324,6 +333,7 @@ struct BatchDev {
   int nr_fine;
   int seg_bins;     // bins per work item (at most)
   int node_stride;  // doubles per request in node (2 * n_r * n_g * 2)
+  int max_zones;    // most zones of a request (k_setup's zone arrays: LP_ZONES or MAX_ZONES)
   ReqParams *req;
   const double *ener_in;
   const double *flux_in;

@@ -186,7 +186,7 @@ def test_entry_points(cp):
     y = rx.lmodrelxilldensnthcomp(e, bad)
     assert not y.any() and rx.lib.rx_last_status() == rx.RX_ERR_PARAM
     grad = list(q)
-    grad[14] = 1.0  # an ionisation gradient
+    grad[14] = 2.0  # the alpha-disc ionisation gradient (not built)
     y = rx.lmodrelxilllpdensnthcomp(e, grad)
     assert not y.any() and rx.lib.rx_last_status() == rx.RX_ERR_UNSUPPORTED
 

@@ -165,11 +165,11 @@ def test_density_division_without_density_axis(cp):
     assert rel_err(y16 * 10 ** 1.3, y15) < 1e-6
 
 
-def test_ionisation_gradient_is_unsupported(cp):
-    """iongrad_type != 0 asks for a radial ionisation gradient, which is not
-    built: status 2 (RX_ERR_UNSUPPORTED) and zero flux."""
+def test_alpha_ionisation_gradient_is_unsupported(cp):
+    """iongrad_type 2 asks for the alpha-disc ionisation gradient, which is
+    not built: status 2 (RX_ERR_UNSUPPORTED) and zero flux."""
     T, xt = cp
     p = list(LPCP_P)
-    p[14] = 1.0
+    p[14] = 2.0
     y, _, st = oracle.relxill_model(oracle.MOD_RELXILLLPCP, T, xt, p, log_grid(50, 0.1, 100.0))
     assert st == 2 and not y.any()
