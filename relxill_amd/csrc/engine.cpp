@@ -1329,7 +1329,10 @@ int batch_run(Batch &b, int precision, std::string &err) {
   RunKey k;
   memset(&k, 0, sizeof(k));
   k.bd = dev_view(b);
-  k.n_blocks = g.n_cu * b.waves_per_cu;
+  // persistent wavefronts: CUs x waves per CU, but no more than the list can
+  // hold items (a small batch -- the server's -- would start thousands that
+  // find the list empty)
+  k.n_blocks = (int)std::min<int64_t>((int64_t)g.n_cu * b.waves_per_cu, std::max(1, b.item_cap));
   k.n_xill1 = b.n_xill1;
   k.n_xill = (int)b.h_xill.size();
   k.n_conv = (int)b.h_conv.size();

@@ -13,6 +13,7 @@
 // that overlap on one GPU race.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 
 #include "engine.h"
@@ -199,7 +200,7 @@ int noalloc_eval(int model, const double *parameter, const double *d_energy, con
   e |= (int)hipMemcpyAsync(bd.status, &p.pre_status, sizeof(int), hipMemcpyHostToDevice, s);
   e |= (int)hipMemcpyAsync(d_work, &w0, sizeof(w0), hipMemcpyHostToDevice, s);
   const bool cos_rows = p.limb != 0;
-  const int n_blocks = g.n_cu * wpc;
+  const int n_blocks = std::min(g.n_cu * wpc, std::max(1, item_cap));  // no more wavefronts than items
   int out[2] = {0, 0};  // status, FP64 re-check queue overflow
   for (int pass = 0; pass < 2 && e == 0; pass++) {
     bd.mode = (pass == 0 && precision == RX_FP32) ? 1 : 0;
