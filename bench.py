@@ -131,11 +131,10 @@ def relxill_style(rx, steps):
                 max_rel_err_fp32_vs_fp64=max(rel_err(o32[i], o64[i]) for i in range(len(reqs))))
 
 
-def relxilllp_batch(rx, steps, n_req=1024, seed=10):
-    """relxilllp (lamp post, 10 zones) on the same table and grids: h U[3,100],
-    a, incl, Rin over their ranges, the xillver parameters drawn as for relxill,
-    refl_frac 1, no returning radiation; FP32 and FP64 batches, plain FP64, the
-    FP32-vs-FP64 error and the per-kernel times."""
+def make_relxilllp_reqs(rx, n_req=1024, seed=10):
+    """relxilllp requests: h U[3,100], a, incl, Rin over their ranges, the
+    xillver parameters drawn as for relxill, refl_frac 1, no returning
+    radiation, output on 3000 bins over 0.1-100 keV"""
     rng = np.random.default_rng(seed)
     e_out = log_grid(3000, 0.1, 100.0)
     reqs = []
@@ -146,6 +145,14 @@ def relxilllp_batch(rx, steps, n_req=1024, seed=10):
                                  rng.uniform(kerr_rms(a), 50.0), 400.0, 0.0, rng.uniform(1.0, 3.4),
                                  rng.uniform(0.0, 4.7), rng.uniform(0.5, 10.0),
                                  float(np.exp(rng.uniform(np.log(5.0), np.log(1000.0)))), 1.0, 0.0, 0.0]))
+    return reqs
+
+
+def relxilllp_batch(rx, steps, n_req=1024, seed=10):
+    """relxilllp (lamp post, 10 zones) on the same table and grids
+    (make_relxilllp_reqs); FP32 and FP64 batches, plain FP64, the
+    FP32-vs-FP64 error and the per-kernel times."""
+    reqs = make_relxilllp_reqs(rx, n_req, seed)
     b = rx.Batch(reqs)
     b.autotune(rx.FP32)
     dt32, kms32 = timed_steps(rx, b, rx.FP32, max(3, steps // 2), 1)
@@ -187,13 +194,9 @@ def relxilllp_batch(rx, steps, n_req=1024, seed=10):
                 kernel_ms={k: round(v, 4) for k, v in zip(KERNELS, kms32)})
 
 
-def relxill_batch(rx, steps, n_req=1024, seed=9, cpu=True):
-    """relxill (broken power law, one zone) on the default synthetic xillver
-    table (13 Gamma x 5 A_Fe x 8 logXi x 6 Ecut x 10 inclinations x 2999 bins):
-    a batch of parameter sets drawn over the lmodel.dat ranges, output on 3000
-    bins over 0.1-100 keV.  FP32 and FP64 batches, plain FP64 (one request per
-    launch), the k_xill roofline and the oracle on one host core."""
-    rx.xilltable_synthetic()
+def make_relxill_reqs(rx, n_req=1024, seed=9):
+    """relxill requests: a, incl, Rin and the four xillver parameters drawn
+    over their ranges, output on 3000 bins over 0.1-100 keV"""
     rng = np.random.default_rng(seed)
     e_out = log_grid(3000, 0.1, 100.0)
     reqs = []
@@ -205,6 +208,17 @@ def relxill_batch(rx, steps, n_req=1024, seed=9, cpu=True):
                          params=[3.0, 3.0, 15.0, a, incl, rin, 400.0, 0.0, rng.uniform(1.0, 3.4),
                                  rng.uniform(0.0, 4.7), rng.uniform(0.5, 10.0),
                                  float(np.exp(rng.uniform(np.log(5.0), np.log(1000.0)))), 3.0]))
+    return reqs
+
+
+def relxill_batch(rx, steps, n_req=1024, seed=9, cpu=True):
+    """relxill (broken power law, one zone) on the default synthetic xillver
+    table (13 Gamma x 5 A_Fe x 8 logXi x 6 Ecut x 10 inclinations x 2999 bins):
+    a batch of parameter sets drawn over the lmodel.dat ranges, output on 3000
+    bins over 0.1-100 keV.  FP32 and FP64 batches, plain FP64 (one request per
+    launch), the k_xill roofline and the oracle on one host core."""
+    rx.xilltable_synthetic()
+    reqs = make_relxill_reqs(rx, n_req, seed)
     b = rx.Batch(reqs)
     b.autotune(rx.FP32)
     dt32, kms32 = timed_steps(rx, b, rx.FP32, max(3, steps // 2), 1)
@@ -460,6 +474,9 @@ def main():
                     help="only the headline batch (FP32), for profiler runs")
     ap.add_argument("--relxill-only", action="store_true",
                     help="only the relxill batch (for profiler runs)")
+    ap.add_argument("--profile-batch", choices=["relxill", "relxilllp"], default="",
+                    help="one 1024-request relxill or relxilllp batch at the --shape work shape, FP32, "
+                         "--warmup untimed and --steps timed runs, nothing else (for profiler runs)")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="run a plain copy kernel of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -481,6 +498,17 @@ def main():
         assert rx.lib.rx_copy_test(nbytes, 5, ms.ctypes.data_as(rx.capi._D)) == 0
         print(json.dumps({"copy_bytes": nbytes, "copy_ms": float(ms[0]),
                           "copy_GBps": 2 * nbytes / ms[0] / 1e6}))
+        return
+
+    if args.profile_batch:
+        rx.xilltable_synthetic()
+        make = make_relxill_reqs if args.profile_batch == "relxill" else make_relxilllp_reqs
+        b = rx.Batch(make(rx))
+        seg, wpc = (int(x) for x in (args.shape or "128,12").split(","))
+        b.set_shape(seg, wpc)
+        dt, kms = timed_steps(rx, b, rx.FP32, args.steps, args.warmup)
+        print(json.dumps({"batch": args.profile_batch, "ms_per_step": dt * 1e3, "kernel_ms": kms.tolist()}))
+        b.close()
         return
 
     if args.relxill_only:
