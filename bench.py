@@ -153,6 +153,7 @@ def relxilllp_batch(rx, steps, n_req=1024, seed=10):
     (make_relxilllp_reqs); FP32 and FP64 batches, plain FP64, the
     FP32-vs-FP64 error and the per-kernel times."""
     reqs = make_relxilllp_reqs(rx, n_req, seed)
+    e_out = reqs[0]["energy"]
     b = rx.Batch(reqs)
     b.autotune(rx.FP32)
     dt32, kms32 = timed_steps(rx, b, rx.FP32, max(3, steps // 2), 1)
@@ -219,6 +220,7 @@ def relxill_batch(rx, steps, n_req=1024, seed=9, cpu=True):
     launch), the k_xill roofline and the oracle on one host core."""
     rx.xilltable_synthetic()
     reqs = make_relxill_reqs(rx, n_req, seed)
+    e_out = reqs[0]["energy"]
     b = rx.Batch(reqs)
     b.autotune(rx.FP32)
     dt32, kms32 = timed_steps(rx, b, rx.FP32, max(3, steps // 2), 1)
