@@ -27,6 +27,7 @@
 #include <memory>
 #include <thread>
 #include <unordered_map>
+#include <cmath>
 #include <vector>
 
 #include "engine.h"
@@ -95,7 +96,10 @@ int64_t request_bytes(const Pending &p) {
   const bool xill = (p.model == MOD_XILLVER || p.model == MOD_XILLVERCP || p.model == MOD_XILLVERNS);
   const bool relxill = (p.model == MOD_RELXILL || p.model == MOD_RELXILLCP || p.model == MOD_RELXILLNS || lpx || xill);
   const bool conv = (p.model == MOD_RELCONV || p.model == MOD_RELCONVLP || p.model == MOD_RELXILL_CONV || relxill);
-  const int64_t nz = lpx ? MAX_ZONES : 1;
+  // zones: LP_ZONES for the lamp post, MAX_ZONES for relxilllpCp with the
+  // power-law ionisation gradient (iongrad_type, parameter 15)
+  const bool grad = p.model == MOD_RELXILLLPCP && p.params.size() > 14 && std::lround(p.params[14]) == 1;
+  const int64_t nz = grad ? MAX_ZONES : (lpx ? LP_ZONES : 1);
   const int64_t n_in = (int64_t)p.energy.size() - 1;
   const int64_t n_e = conv ? g.conv_n : n_in;
   const int64_t n_rc = (g.nr_fine + RAD_CHUNK - 1) / RAD_CHUNK;
